@@ -1,0 +1,183 @@
+// Harness around the compiled, unmodified reference (oracle/_ref/libsdl_ref.so).
+//
+// It is test infrastructure: it generates the same seeded events as the rest of
+// this repository (sdl_amd/csrc/gen), feeds them hit by hit through the
+// reference's own API (SDL::Event::addHitToModule, Event.cc:120-139) after
+// loading the three geometry singletons from the generated text files
+// (TiltedGeometry.cc:18-41, EndcapGeometry.cc:18-61, ModuleConnectionMap.cc:18-47),
+// runs the module-map chain
+//     createMiniDoublets, createSegmentsWithModuleMap, createTriplets,
+//     createTrackletsWithModuleMap, createTrackCandidatesFromTriplets,
+//     createTrackCandidatesFromTracklets                       (Event.h:240-300)
+// and writes every object as a tuple of hit indices, plus the per-stage counters.
+//
+// Usage:
+//   sdl_ref_driver dump  <workload> <geomdir> <event0> <nevents> <outfile>
+//   sdl_ref_driver time  <workload> <geomdir> <event0> <nevents> <threads> [stages]
+// `stages` limits the chain to its first k stages (1..6, default 6).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <sys/stat.h>
+#include <thread>
+#include <vector>
+#include <atomic>
+
+#include "Event.h"
+#include "EndcapGeometry.h"
+#include "ModuleConnectionMap.h"
+#include "TiltedGeometry.h"
+#include "Tracklet.h"
+#include "TrackCandidate.h"
+#include "sdl_gen.h"
+
+namespace {
+
+bool fileExists(const std::string& p) {
+    struct stat st;
+    return stat(p.c_str(), &st) == 0;
+}
+
+int geomKindOf(int workload) { return (workload == 1 || workload == 4) ? 1 : 2; }
+
+void loadGeometry(int workload, const std::string& dir, sdlgen::Geometry& g) {
+    int kind = geomKindOf(workload);
+    sdlgen::buildGeometry(kind, g, kind == 1 ? 50000 : 200000);
+    std::string t = dir + "/tilted_" + std::to_string(kind) + ".txt";
+    std::string e = dir + "/endcap_" + std::to_string(kind) + ".txt";
+    std::string m = dir + "/map_" + std::to_string(kind) + ".txt";
+    sdlgen::writeGeometryFiles(g, t, e, m);
+    SDL::tiltedGeometry.load(t);
+    SDL::endcapGeometry.load(e);
+    SDL::moduleConnectionMap.load(m);
+    // Touch every key once so that later lookups (std::map::operator[]) never
+    // insert; this makes the singletons safe to read from several threads.
+    for (const sdlgen::Sensor& s : g.sensors) {
+        SDL::tiltedGeometry.getDrDz(s.detId);
+        SDL::tiltedGeometry.getSlope(s.detId);
+        SDL::endcapGeometry.getSlopeLower(s.detId);
+        SDL::endcapGeometry.getCentroidPhi(s.detId);
+        SDL::moduleConnectionMap.getConnectedModuleDetIds(s.detId);
+    }
+}
+
+void runChain(SDL::Event& ev, int stages) {
+    if (stages >= 1) ev.createMiniDoublets();
+    if (stages >= 2) ev.createSegmentsWithModuleMap();
+    if (stages >= 3) ev.createTriplets();
+    if (stages >= 4) ev.createTrackletsWithModuleMap();
+    if (stages >= 5) ev.createTrackCandidatesFromTriplets();
+    if (stages >= 6) ev.createTrackCandidatesFromTracklets();
+}
+
+void fill(SDL::Event& sev, const sdlgen::Event& ev) {
+    for (size_t i = 0; i < ev.x.size(); ++i)
+        sev.addHitToModule(SDL::Hit(ev.x[i], ev.y[i], ev.z[i], ev.idx[i]), ev.detId[i]);
+}
+
+void putMD(FILE* f, const SDL::MiniDoublet* md) {
+    std::fprintf(f, " %d %d", md->lowerHitPtr()->idx(), md->upperHitPtr()->idx());
+}
+void putSG(FILE* f, const SDL::Segment* sg) {
+    putMD(f, sg->innerMiniDoubletPtr());
+    putMD(f, sg->outerMiniDoubletPtr());
+}
+
+void dumpEvent(FILE* f, SDL::Event& sev, long ievent) {
+    std::fprintf(f, "E %ld\n", ievent);
+    // Objects are reached through the per-module pointer lists, which hold every
+    // MD, SG, TP and TL (Event.cc:141-193); TCs through the layers (Event.cc:232-239).
+    for (SDL::Module* m : sev.getModulePtrs()) {
+        for (SDL::MiniDoublet* md : m->getMiniDoubletPtrs()) { std::fprintf(f, "MD"); putMD(f, md); std::fprintf(f, "\n"); }
+        for (SDL::Segment* sg : m->getSegmentPtrs()) { std::fprintf(f, "SG"); putSG(f, sg); std::fprintf(f, "\n"); }
+        for (SDL::Triplet* tp : m->getTripletPtrs()) {
+            std::fprintf(f, "TP"); putSG(f, tp->innerSegmentPtr()); putSG(f, tp->outerSegmentPtr()); std::fprintf(f, "\n");
+        }
+        for (SDL::Tracklet* tl : m->getTrackletPtrs()) {
+            std::fprintf(f, "TL"); putSG(f, tl->innerSegmentPtr()); putSG(f, tl->outerSegmentPtr()); std::fprintf(f, "\n");
+        }
+    }
+    for (SDL::Layer* L : sev.getLayerPtrs()) {
+        for (SDL::TrackCandidate* tc : L->getTrackCandidatePtrs()) {
+            // A TC made from triplets stores two Triplets, one made from tracklets two
+            // Tracklets (TrackCandidate.cc:38-88 throw on the wrong accessor).
+            SDL::TrackletBase* a = tc->innerTrackletBasePtr();
+            SDL::TrackletBase* b = tc->outerTrackletBasePtr();
+            bool fromTP = dynamic_cast<SDL::Triplet*>(a) != nullptr;
+            std::fprintf(f, fromTP ? "TCP" : "TCL");
+            putSG(f, a->innerSegmentPtr()); putSG(f, a->outerSegmentPtr());
+            putSG(f, b->innerSegmentPtr()); putSG(f, b->outerSegmentPtr());
+            std::fprintf(f, "\n");
+        }
+    }
+    std::fprintf(f, "N %u %u %u %u %u\n", sev.getNumberOfMiniDoubletCandidates(), sev.getNumberOfSegmentCandidates(),
+                 sev.getNumberOfTripletCandidates(), sev.getNumberOfTrackletCandidates(),
+                 sev.getNumberOfTrackCandidateCandidates());
+    std::fprintf(f, "C %u %u %u %u %u\n", sev.getNumberOfMiniDoublets(), sev.getNumberOfSegments(),
+                 sev.getNumberOfTriplets(), sev.getNumberOfTracklets(), sev.getNumberOfTrackCandidates());
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 7) {
+        std::fprintf(stderr, "usage: %s dump|time workload geomdir event0 nevents out|threads [stages]\n", argv[0]);
+        return 2;
+    }
+    std::string mode = argv[1];
+    int workload = std::atoi(argv[2]);
+    std::string geomdir = argv[3];
+    long e0 = std::atol(argv[4]);
+    long ne = std::atol(argv[5]);
+    sdlgen::Geometry g;
+    loadGeometry(workload, geomdir, g);
+    sdlgen::EventConfig cfg = sdlgen::presetConfig(workload);
+
+    if (mode == "dump") {
+        FILE* f = std::fopen(argv[6], "w");
+        if (!f) return 3;
+        for (long ie = e0; ie < e0 + ne; ++ie) {
+            sdlgen::Event ev;
+            sdlgen::generateEvent(g, cfg, ie, ev);
+            SDL::Event sev;
+            fill(sev, ev);
+            runChain(sev, 6);
+            dumpEvent(f, sev, ie);
+        }
+        std::fclose(f);
+        return 0;
+    }
+    if (mode == "time") {
+        int threads = std::atoi(argv[6]);
+        int stages = argc > 7 ? std::atoi(argv[7]) : 6;
+        std::vector<sdlgen::Event> evs(ne);
+        for (long i = 0; i < ne; ++i) sdlgen::generateEvent(g, cfg, e0 + i, evs[i]);
+        std::atomic<long> next(0);
+        std::vector<double> perEvent(ne, 0.0);
+        auto t0 = std::chrono::steady_clock::now();
+        auto work = [&]() {
+            for (;;) {
+                long i = next.fetch_add(1);
+                if (i >= ne) break;
+                auto a = std::chrono::steady_clock::now();
+                SDL::Event sev;
+                fill(sev, evs[i]);
+                runChain(sev, stages);
+                auto b = std::chrono::steady_clock::now();
+                perEvent[i] = std::chrono::duration<double>(b - a).count();
+            }
+        };
+        std::vector<std::thread> th;
+        for (int t = 0; t < threads; ++t) th.emplace_back(work);
+        for (auto& t : th) t.join();
+        double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        double sum = 0;
+        for (double x : perEvent) sum += x;
+        std::printf("{\"events\": %ld, \"threads\": %d, \"wall_s\": %.6f, \"events_per_s\": %.6f, \"mean_event_s\": %.6f}\n",
+                    ne, threads, wall, ne / wall, sum / ne);
+        return 0;
+    }
+    return 2;
+}
