@@ -1,0 +1,136 @@
+"""ctypes wrapper of the CPU ORACLE (oracle/_build/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+REF_DRIVER = os.path.join(_HERE, "_ref", "sdl_ref_driver")
+
+GLIBC, CR = 0, 1
+STAGES = ("MD", "SG", "TP", "TL", "TCP", "TCL")
+WIDTH = {"MD": 2, "SG": 4, "TP": 8, "TL": 8, "TCP": 16, "TCL": 16}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_geom_load.restype = ctypes.c_void_p
+        L.oracle_geom_load.argtypes = [ctypes.c_char_p] * 3
+        L.oracle_geom_free.argtypes = [ctypes.c_void_p]
+        L.oracle_run.restype = ctypes.c_void_p
+        L.oracle_run.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int, ctypes.c_int]
+        L.oracle_result_count.restype = ctypes.c_long
+        L.oracle_result_count.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_result_copy.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.oracle_result_counters.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_result_free.argtypes = [ctypes.c_void_p]
+        L.oracle_explain.restype = ctypes.c_int
+        L.oracle_explain.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 3
+        L.oracle_time.restype = ctypes.c_double
+        L.oracle_time.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_int, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+_geoms = {}
+
+
+def geometry(paths):
+    key = tuple(paths)
+    if key not in _geoms:
+        h = lib().oracle_geom_load(*[p.encode() for p in paths])
+        if not h:
+            raise RuntimeError(f"oracle could not load geometry {paths}")
+        _geoms[key] = h
+    return _geoms[key]
+
+
+def _ptrs(ev):
+    return [ev.x.ctypes.data, ev.y.ctypes.data, ev.z.ctypes.data, ev.detId.ctypes.data, ev.idx.ctypes.data]
+
+
+def run(paths, ev, libm=GLIBC, stages=6):
+    """Returns ({stage: int32 array [n, width]}, counters dict)."""
+    L = lib()
+    g = geometry(paths)
+    r = L.oracle_run(g, ev.nhits, *_ptrs(ev), libm, stages)
+    try:
+        out = {}
+        for i, s in enumerate(STAGES):
+            n = L.oracle_result_count(r, i)
+            a = np.empty((n, WIDTH[s]), np.int32)
+            if n:
+                L.oracle_result_copy(r, i, a.ctypes.data)
+            out[s] = a
+        c = np.zeros(10, np.uint64)
+        L.oracle_result_counters(r, c.ctypes.data)
+        counters = dict(zip(("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC"),
+                            [int(v) for v in c]))
+        return out, counters
+    finally:
+        L.oracle_result_free(r)
+
+
+def explain(paths, ev, kind, a, b=None):
+    """Re-evaluates one MD (kind 0, a = (lo, up)), SG (kind 1, a = 4 idx) or TL
+    (kind 2, a, b = 8 idx) candidate under both libm policies."""
+    L = lib()
+    g = geometry(paths)
+    aa = np.asarray(a, np.int32)
+    bb = np.asarray(b if b is not None else [0] * 4, np.int32)
+    out = np.zeros(9, np.float64)
+    L.oracle_explain(g, ev.nhits, *_ptrs(ev), kind, aa.ctypes.data, bb.ctypes.data, out.ctypes.data)
+    keys = ("cut_glibc", "cut_cr", "var", "lo", "hi", "var_cr", "lo_cr", "hi_cr", "ulps")
+    return dict(zip(keys, out.tolist()))
+
+
+def time_events(paths, events, libm=GLIBC, threads=1):
+    """Wall seconds to run the whole chain on `events` with `threads` threads."""
+    L = lib()
+    g = geometry(paths)
+    off = np.zeros(len(events) + 1, np.int64)
+    off[1:] = np.cumsum([e.nhits for e in events])
+    cat = [np.ascontiguousarray(np.concatenate([getattr(e, f) for e in events])) for f in ("x", "y", "z", "detId", "idx")]
+    return L.oracle_time(g, len(events), off.ctypes.data, *[c.ctypes.data for c in cat], libm, threads)
+
+
+def parse_ref_dump(path):
+    """Parses a reference-driver dump into {event: ({stage: array}, counters)}."""
+    events = {}
+    cur = None
+    rows = None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            tag = parts[0]
+            if tag == "E":
+                cur = int(parts[1])
+                rows = {s: [] for s in STAGES}
+                events[cur] = [rows, {}]
+            elif tag in WIDTH:
+                rows[tag].append([int(v) for v in parts[1:]])
+            elif tag == "N":
+                events[cur][1].update(dict(zip(("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand"), map(int, parts[1:]))))
+            elif tag == "C":
+                events[cur][1].update(dict(zip(("nMD", "nSG", "nTP", "nTL", "nTC"), map(int, parts[1:]))))
+    out = {}
+    for e, (rows, cnt) in events.items():
+        out[e] = ({s: np.asarray(rows[s], np.int32).reshape(-1, WIDTH[s]) for s in STAGES}, cnt)
+    return out
+
+
+def as_multiset(a):
+    """Sorted rows: multiset comparison key for tuple arrays."""
+    if len(a) == 0:
+        return a
+    return a[np.lexsort(a.T[::-1])]

@@ -1,0 +1,1218 @@
+// CPU ORACLE for the SDL segment-linking chain -- test infrastructure only.
+// See sdl_oracle.h for its role and the two libm policies.  Every function
+// below cites the reference file:line whose behaviour it restates.
+#include "sdl_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace oracle {
+
+constexpr double kPI = 3.14159265358979323846;  // M_PI
+constexpr float kSlopeInfF = 123456789.0f;       // SDL_INF compared as float (MiniDoublet.cc:3,1130)
+
+// float(2.99792458e-3 * 3.8) and its exact half (MiniDoublet.cc:746-747)
+const float kRinv1GeVf = float(2.99792458e-3 * 3.8);
+const float k2Rinv1GeVf = float(double(kRinv1GeVf) / 2.);
+constexpr float kPtCut = 1.0f;        // PTCUT (Constants.h:4)
+constexpr float kSinAlphaMax = 0.95f;
+constexpr float kDeltaZLum = 15.f;
+constexpr float kPixelPSZpitch = 0.15f;
+constexpr float kStrip2SZpitch = 5.0f;
+
+// ------------------------------------------------------------------ libm policy
+struct Libm {
+    bool cr;
+    float asinF(float v) const { return cr ? float(std::asin(double(v))) : std::asin(v); }
+    float atanF(float v) const { return cr ? float(std::atan(double(v))) : std::atan(v); }
+    float sinF(float v) const { return cr ? float(std::sin(double(v))) : std::sin(v); }
+    float cosF(float v) const { return cr ? float(std::cos(double(v))) : std::cos(v); }
+    float tanF(float v) const { return cr ? float(std::tan(double(v))) : std::tan(v); }
+};
+
+// std::min / std::max semantics (NaN handling follows the argument order).
+inline float fmin2(float a, float b) { return (b < a) ? b : a; }
+inline float fmax2(float a, float b) { return (a < b) ? b : a; }
+// pow(x, 2) on a float argument: exact in double.
+inline double sq(double v) { return v * v; }
+
+// Unqualified abs(float) binds to int abs(int) (A-1): truncation toward zero,
+// with the x86 cvttss2si result INT_MIN for NaN / out-of-range values.
+inline int absTrunc(float v) {
+    int i = (v > -2147483648.f && v < 2147483648.f) ? int(v) : INT_MIN;
+    return i == INT_MIN ? INT_MIN : (i < 0 ? -i : i);
+}
+
+// MathUtil.cc:3-19
+inline float phiMpiPi(float x) {
+    if (std::isnan(x) || std::isinf(x)) return x;
+    while (double(x) >= kPI) x = float(double(x) - 2. * kPI);
+    while (double(x) < -kPI) x = float(double(x) + 2. * kPI);
+    return x;
+}
+// MathUtil.cc:21-27
+inline float atan2Sdl(float y, float x) {
+    if (x != 0) return float(std::atan2(double(y), double(x)));
+    if (y == 0) return 0.f;
+    if (y > 0) return float(kPI / 2);
+    return float(-kPI / 2);
+}
+
+// A point with the derived quantities of Hit::setDerivedQuantities (Hit.cc:68-83).
+struct P3 {
+    float x, y, z;
+    float rt() const { return float(std::sqrt(double(x * x + y * y))); }
+    float r3() const { return float(std::sqrt(double(x * x + y * y + z * z))); }
+    float phi() const { return phiMpiPi(float(kPI + double(atan2Sdl(-y, -x)))); }
+};
+inline P3 sub(const P3& a, const P3& b) { return P3{a.x - b.x, a.y - b.y, a.z - b.z}; }  // Hit.cc:195-198
+// Hit() then += a, += b, /= 2 (Tracklet.cc:1131-1134)
+inline P3 midpoint(const P3& a, const P3& b) {
+    P3 m{0.f + a.x, 0.f + a.y, 0.f + a.z};
+    m.x = (m.x + b.x) / 2.0f; m.y = (m.y + b.y) / 2.0f; m.z = (m.z + b.z) / 2.0f;
+    return m;
+}
+// Hit::deltaPhi (Hit.cc:149-152): phi of the argument minus phi of this
+inline float deltaPhi(float phiA, const P3& b) { return phiMpiPi(b.phi() - phiA); }
+
+struct HitD {
+    P3 p;
+    float rt, r3, phi;
+    int idx;       // user hit index
+    int slot;      // module slot in the event
+    int hiEdge = -1, loEdge = -1;  // indices into Event::edges
+};
+
+// ------------------------------------------------------------------ modules
+enum { SUB_ENDCAP = 4, SUB_BARREL = 5 };
+enum { SIDE_NEGZ = 1, SIDE_POSZ = 2, SIDE_CENTER = 3 };
+enum { TYPE_PS = 0, TYPE_2S = 1 };
+enum { LT_PIXEL = 0, LT_STRIP = 1 };
+
+struct ModInfo {
+    uint32_t detId, partner;
+    int subdet, side, layer, rod, ring, module;
+    bool isLower, isInverted;
+    int type, layerType;
+};
+
+// Module.cc:159-358, restated.
+ModInfo decode(uint32_t d) {
+    ModInfo m{};
+    m.detId = d;
+    m.subdet = (d >> 25) & 7;
+    if (m.subdet == SUB_ENDCAP) { m.side = (d >> 23) & 3; m.layer = (d >> 18) & 7; m.ring = (d >> 12) & 15; m.rod = 0; }
+    else if (m.subdet == SUB_BARREL) { m.side = (d >> 18) & 3; m.layer = (d >> 20) & 7; m.rod = (d >> 10) & 127; m.ring = 0; }
+    else { m.side = 0; m.layer = 0; m.rod = 0; m.ring = 0; }
+    m.module = (d >> 2) & 127;
+    bool inv = false;
+    if (m.subdet == SUB_ENDCAP) {
+        if (m.side == SIDE_NEGZ) inv = m.module % 2 == 1;
+        else if (m.side == SIDE_POSZ) inv = m.module % 2 == 0;
+    } else if (m.subdet == SUB_BARREL) {
+        if (m.side == SIDE_CENTER) {
+            if (m.layer <= 3) inv = m.module % 2 == 1;
+            else if (m.layer >= 4) inv = m.module % 2 == 0;
+        } else if (m.side == SIDE_NEGZ || m.side == SIDE_POSZ) {
+            if (m.layer <= 2) inv = m.module % 2 == 1;
+            else if (m.layer == 3) inv = m.module % 2 == 0;
+        }
+    }
+    m.isInverted = inv;
+    m.isLower = inv ? !(d & 1) : (d & 1);
+    if (m.isLower) m.partner = inv ? d - 1 : d + 1;
+    else m.partner = inv ? d + 1 : d - 1;
+    if (m.subdet == SUB_BARREL) m.type = m.layer <= 3 ? TYPE_PS : TYPE_2S;
+    else if (m.layer <= 2) m.type = m.ring <= 10 ? TYPE_PS : TYPE_2S;
+    else m.type = m.ring <= 7 ? TYPE_PS : TYPE_2S;
+    if (m.type == TYPE_2S) m.layerType = LT_STRIP;
+    else if (inv) m.layerType = m.isLower ? LT_STRIP : LT_PIXEL;
+    else m.layerType = m.isLower ? LT_PIXEL : LT_STRIP;
+    return m;
+}
+
+// MiniDoublet.cc:1242-1257
+bool isTighterTilted(const ModInfo& m) {
+    if (m.subdet != SUB_BARREL || m.side == SIDE_CENTER) return false;
+    if (m.layer == 3) return true;
+    if (m.side == SIDE_NEGZ && m.layer == 2 && m.rod > 5) return true;
+    if (m.side == SIDE_POSZ && m.layer == 2 && m.rod < 8) return true;
+    if (m.side == SIDE_NEGZ && m.layer == 1 && m.rod > 9) return true;
+    if (m.side == SIDE_POSZ && m.layer == 1 && m.rod < 4) return true;
+    return false;
+}
+// MiniDoublet.cc:1260-1332
+float moduleGapSize(const ModInfo& m) {
+    static const float flat[6] = {0.26f, 0.16f, 0.16f, 0.18f, 0.18f, 0.18f};
+    int iL = m.layer - 1;
+    if (m.subdet == SUB_BARREL && m.side == SIDE_CENTER) return flat[iL];
+    if (isTighterTilted(m)) return 0.26f;
+    if (m.subdet == SUB_ENDCAP) {
+        int j = m.ring - 1;
+        if (iL == 0 || iL == 1) return j < 10 ? 0.4f : 0.18f;
+        if (iL == 2 || iL == 3) return j < 8 ? 0.4f : 0.18f;
+        return j < 9 ? 0.4f : 0.18f;
+    }
+    return 0.4f;
+}
+
+// ------------------------------------------------------------------ geometry
+struct Geometry {
+    std::unordered_map<uint32_t, float> drdz, slope, ecSlopeLower, ecCentroidPhi;
+    std::unordered_map<uint32_t, std::vector<uint32_t>> conn;
+    // A missing key reads as 0 / empty (std::map::operator[] in TiltedGeometry.cc:45,50,
+    // EndcapGeometry.cc:65-100, ModuleConnectionMap.cc:105).
+    static float get(const std::unordered_map<uint32_t, float>& m, uint32_t k) {
+        auto it = m.find(k);
+        return it == m.end() ? 0.f : it->second;
+    }
+    const std::vector<uint32_t>& connected(uint32_t k) const {
+        static const std::vector<uint32_t> none;
+        auto it = conn.find(k);
+        return it == conn.end() ? none : it->second;
+    }
+};
+
+bool loadGeometry(Geometry& g, const char* tilted, const char* endcap, const char* map) {
+    std::ifstream ft(tilted), fe(endcap), fm(map);
+    if (!ft || !fe || !fm) return false;
+    std::string line;
+    while (std::getline(ft, line)) {  // TiltedGeometry.cc:26-41
+        std::stringstream ss(line);
+        unsigned int d; float a, b;
+        ss >> d >> a >> b;
+        g.drdz[d] = a; g.slope[d] = b;
+    }
+    while (std::getline(fe, line)) {  // EndcapGeometry.cc:33-60 (column 7 is stored as phi)
+        std::stringstream ss(line);
+        unsigned int d; float avgr2, yl, sl, yh, sh, cr, cp, cz;
+        ss >> d >> avgr2 >> yl >> sl >> yh >> sh >> cr >> cp >> cz;
+        g.ecSlopeLower[d] = sl; g.ecCentroidPhi[d] = cr;
+    }
+    while (std::getline(fm, line)) {  // ModuleConnectionMap.cc:26-46
+        std::stringstream ss(line);
+        unsigned int d; int n;
+        ss >> d >> n;
+        std::vector<uint32_t> v;
+        for (int i = 0; i < n; ++i) { unsigned int c; ss >> c; v.push_back(c); }
+        g.conn[d] = v;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ tracing
+struct Trace {
+    int n = 0;
+    int failed = -1;
+    double v[32], lo[32], hi[32];
+    int id[32];
+    void add(int cid, double var, double l, double h, bool pass) {
+        if (n < 32) { id[n] = cid; v[n] = var; lo[n] = l; hi[n] = h; ++n; }
+        if (!pass && failed < 0) failed = cid;
+    }
+};
+const double kNone = std::nan("");
+#define CUT(cid, cond, var, l, h)                        \
+    do {                                                 \
+        bool _pass = (cond);                             \
+        if (tr) tr->add(cid, var, l, h, _pass);          \
+        if (!_pass) return false;                        \
+    } while (0)
+
+// ------------------------------------------------------------------ objects
+struct MDRec {
+    int lo, up, anchor;  // hit indices in the event
+    int slot;            // lower module slot
+    float dz, shiftedDz, dphi, dphichange;
+};
+struct SGRec {
+    int mdIn, mdOut;
+    float dphi, dphimin, dphimax, dphichange, dphichangemin, dphichangemax;
+};
+struct PairRec { int in, out; };
+
+struct Slot {
+    ModInfo info;
+    float drdzStrip = 0, slopeStrip = 0, ecSlopeStrip = 0, centroidPhi = 0, gap = 0;
+    std::vector<int> hits, mds, sgs, tps, tls;
+};
+
+struct Event {
+    const Geometry* g;
+    Libm lm;
+    std::vector<HitD> hits, edges;
+    std::vector<Slot> slots;
+    std::unordered_map<uint32_t, int> slotOf;
+    std::vector<int> lowerOrder;
+    std::vector<MDRec> mds;
+    std::vector<SGRec> sgs;
+    std::vector<PairRec> tps, tls;
+    std::vector<std::pair<int, PairRec>> tcs;  // (0 = from triplets, 1 = from tracklets)
+    unsigned long long nCand[5] = {0, 0, 0, 0, 0};
+
+    // Event::getModule (Event.cc:54-78)
+    int getSlot(uint32_t d) {
+        auto it = slotOf.find(d);
+        if (it != slotOf.end()) return it->second;
+        int s = int(slots.size());
+        Slot sl;
+        sl.info = decode(d);
+        uint32_t stripId = sl.info.layerType == LT_PIXEL ? sl.info.partner : d;
+        sl.drdzStrip = Geometry::get(g->drdz, stripId);
+        sl.slopeStrip = Geometry::get(g->slope, stripId);
+        sl.ecSlopeStrip = Geometry::get(g->ecSlopeLower, stripId);
+        sl.centroidPhi = Geometry::get(g->ecCentroidPhi, d);
+        sl.gap = moduleGapSize(sl.info);
+        slots.push_back(sl);
+        slotOf[d] = s;
+        if (slots[s].info.isLower) lowerOrder.push_back(s);
+        return s;
+    }
+    int findSlot(uint32_t d) const {
+        auto it = slotOf.find(d);
+        return it == slotOf.end() ? -1 : it->second;
+    }
+
+    // GeometryUtil.cc:3-23: hit + sign * 2.5 * (cos phi_c, sin phi_c, 0)
+    HitD edgeHit(const HitD& h, bool up) const {
+        float phic = slots[h.slot].centroidPhi;
+        float sign = up ? 1.f : -1.f;
+        P3 e{float(double(sign) * 2.5 * std::cos(double(phic))), float(double(sign) * 2.5 * std::sin(double(phic))), 0.f};
+        e.x += h.p.x; e.y += h.p.y; e.z += h.p.z;
+        HitD r;
+        r.p = e; r.rt = e.rt(); r.r3 = e.r3(); r.phi = e.phi(); r.idx = -1; r.slot = h.slot;
+        return r;
+    }
+
+    // Event::addHitToModule (Event.cc:120-139)
+    void addHit(float x, float y, float z, int idx, uint32_t d) {
+        HitD h;
+        h.p = P3{x, y, z};
+        h.rt = h.p.rt(); h.r3 = h.p.r3(); h.phi = h.p.phi();
+        h.idx = idx;
+        h.slot = getSlot(d);
+        int hi = int(hits.size());
+        hits.push_back(h);
+        slots[h.slot].hits.push_back(hi);
+        const ModInfo& m = slots[h.slot].info;
+        if (m.subdet == SUB_ENDCAP && m.type == TYPE_2S) {
+            edges.push_back(edgeHit(hits[hi], true));
+            hits[hi].hiEdge = int(edges.size()) - 1;
+            edges.push_back(edgeHit(hits[hi], false));
+            hits[hi].loEdge = int(edges.size()) - 1;
+        }
+    }
+
+    // ---------------------------------------------------------------- MD
+    // MiniDoublet::dPhiThreshold (MiniDoublet.cc:741-810)
+    float dPhiThreshold(const HitD& h, const Slot& s, float dPhi = 0, float dz = 1) const {
+        static const float mulsB[6] = {0.0052f, 0.0038f, 0.0034f, 0.0034f, 0.0032f, 0.0034f};
+        static const float mulsE[5] = {0.006f, 0.006f, 0.006f, 0.006f, 0.006f};
+        static const float rB[6] = {21.8f, 34.6f, 49.6f, 67.4f, 87.6f, 106.8f};
+        static const float rE[5] = {131.4f, 156.2f, 185.6f, 220.3f, 261.5f};
+        const ModInfo& m = s.info;
+        const float ptCut = 1;
+        float rt = h.rt;
+        unsigned iL = m.layer - 1;
+        const float miniSlope = lm.asinF(fmin2(rt * k2Rinv1GeVf / ptCut, kSinAlphaMax));
+        const float rLay = m.subdet == SUB_BARREL ? rB[iL] : rE[iL];
+        const float miniPVoff = float(0.1 / double(rLay));
+        const float miniMuls = m.subdet == SUB_BARREL ? mulsB[iL] * 3.f / ptCut : mulsE[iL] * 3.f / ptCut;
+        const bool isTilted = m.subdet == SUB_BARREL && m.side != SIDE_CENTER;
+        const float drdz = s.drdzStrip;
+        const float miniTilt =
+            isTilted ? float(double(0.5f * kPixelPSZpitch * drdz) / std::sqrt(double(1.f + drdz * drdz)) / double(s.gap))
+                     : 0.f;
+        const float miniLum = float(absTrunc(dPhi * kDeltaZLum / dz));
+        if (m.subdet == SUB_BARREL && m.side == SIDE_CENTER)
+            return float(double(miniSlope) + std::sqrt(sq(miniMuls) + sq(miniPVoff)));
+        if (m.subdet == SUB_BARREL)
+            return float(double(miniSlope) + std::sqrt(sq(miniMuls) + sq(miniPVoff) + sq(miniTilt * miniSlope)));
+        return float(double(miniSlope) + std::sqrt(sq(miniMuls) + sq(miniPVoff) + sq(miniLum)));
+    }
+
+    // MiniDoublet::shiftStripHits (MiniDoublet.cc:989-1209)
+    void shiftStripHits(const HitD& lower, const HitD& upper, const Slot& s, float& xn, float& yn, float& zn) const {
+        const ModInfo& m = s.info;
+        const HitD *pix, *str;
+        if (m.type == TYPE_PS) {
+            if (m.layerType == LT_PIXEL) { pix = &lower; str = &upper; }
+            else { pix = &upper; str = &lower; }
+        } else { pix = &lower; str = &upper; }
+        bool isEndcap = m.subdet == SUB_ENDCAP;
+        float angleA = float(std::fabs(double(lm.atanF(pix->rt / pix->p.z))));
+        float angleB = isEndcap ? float(kPI / 2.) : lm.atanF(s.drdzStrip);
+        float sep = s.gap;
+        if (m.type == TYPE_PS && m.layerType != LT_PIXEL) sep *= -1;
+        float drprime = (sep / lm.sinF(angleA + angleB)) * lm.sinF(angleA);
+        float slope = isEndcap ? s.ecSlopeStrip : s.slopeStrip;
+        float absArctanSlope = slope != kSlopeInfF ? float(std::fabs(double(lm.atanF(slope)))) : float(kPI / 2);
+        float xp = pix->p.x, yp = pix->p.y;
+        float angleM;
+        if (xp > 0 && yp > 0) angleM = absArctanSlope;
+        else if (xp > 0 && yp < 0) angleM = float(kPI - double(absArctanSlope));
+        else if (xp < 0 && yp < 0) angleM = float(kPI + double(absArctanSlope));
+        else angleM = float(2 * kPI - double(absArctanSlope));
+        float dx = drprime * lm.sinF(angleM), dy = drprime * lm.cosF(angleM);
+        float xa = xp + dx, ya = yp + dy;
+        float xo = str->p.x, yo = str->p.y;
+        if (slope == kSlopeInfF) { xn = xa; yn = yo; }
+        else if (slope == 0) { xn = xo; yn = ya; }
+        else {
+            xn = (slope * xa + (1.f / slope) * xo - ya + yo) / (slope + (1.f / slope));
+            yn = (xn - xa) * slope + ya;
+        }
+        float absdzprime = float(std::fabs(double(sep / lm.sinF(angleA + angleB) * lm.cosF(angleA))));
+        float abszn = m.layerType == LT_PIXEL ? float(std::fabs(double(pix->p.z)) + double(absdzprime))
+                                              : float(std::fabs(double(pix->p.z)) - double(absdzprime));
+        zn = abszn * ((pix->p.z > 0) ? 1 : -1);
+    }
+
+    static float dPhiChange(const P3& a, float phiA, const P3& b) { return deltaPhi(phiA, sub(b, a)); }
+
+    // MiniDoublet::runMiniDoubletDefaultAlgoBarrel (MiniDoublet.cc:243-447)
+    bool mdBarrel(const HitD& lo, const HitD& up, const Slot& s, MDRec& md, Trace* tr) const {
+        const ModInfo& m = s.info;
+        md.dz = lo.p.z - up.p.z;
+        const float dz = md.dz;
+        const float dzCut = m.type == TYPE_PS ? 2.f : 10.f;
+        const float sign = float(((dz > 0) - (dz < 0)) * ((lo.p.z > 0) - (lo.p.z < 0)));
+        const float invertedcrossercut = float(absTrunc(dz) > 2) * sign;
+        CUT(0, std::abs(dz) < dzCut && invertedcrossercut <= 0, std::abs(dz), kNone, dzCut);
+        float miniCut = m.layerType == LT_PIXEL ? dPhiThreshold(lo, s) : dPhiThreshold(up, s);
+        bool tilted = m.side != SIDE_CENTER;
+        P3 shifted{0, 0, 0};
+        float shiftedPhi = 0, shiftedRt = 0;
+        if (tilted) {
+            float xn, yn, zn;
+            shiftStripHits(lo, up, s, xn, yn, zn);
+            shifted = P3{xn, yn, m.layerType == LT_PIXEL ? up.p.z : lo.p.z};
+            shiftedPhi = shifted.phi();
+            shiftedRt = shifted.rt();
+            if (m.layerType == LT_PIXEL) md.dphi = deltaPhi(lo.phi, shifted);
+            else md.dphi = phiMpiPi(up.phi - shiftedPhi);
+        } else {
+            md.dphi = deltaPhi(lo.phi, up.p);
+        }
+        CUT(1, std::abs(md.dphi) < miniCut, std::abs(md.dphi), kNone, miniCut);
+        if (tilted) {
+            if (m.layerType == LT_PIXEL)
+                md.dphichange = lo.rt < shiftedRt ? dPhiChange(lo.p, lo.phi, shifted) : dPhiChange(shifted, shiftedPhi, lo.p);
+            else
+                md.dphichange = shiftedRt < up.rt ? dPhiChange(shifted, shiftedPhi, up.p) : dPhiChange(up.p, up.phi, shifted);
+        } else {
+            md.dphichange = dPhiChange(lo.p, lo.phi, up.p);
+        }
+        CUT(2, std::abs(md.dphichange) < miniCut, std::abs(md.dphichange), kNone, miniCut);
+        return true;
+    }
+
+    // MiniDoublet::runMiniDoubletDefaultAlgoEndcap (MiniDoublet.cc:449-684)
+    bool mdEndcap(const HitD& lo, const HitD& up, const Slot& s, MDRec& md, Trace* tr) const {
+        const ModInfo& m = s.info;
+        md.dz = lo.p.z - up.p.z;
+        float dz = md.dz;
+        const float dzCut = 10.f;  // side() is never Endcap (A-4)
+        CUT(10, std::abs(dz) < dzCut, std::abs(dz), kNone, dzCut);
+        const float drtCut = m.type == TYPE_PS ? 2.f : 10.f;
+        float drt = std::abs(lo.rt - up.rt);
+        CUT(11, drt < drtCut, drt, kNone, drtCut);
+        float xn, yn, zn;
+        shiftStripHits(lo, up, s, xn, yn, zn);
+        if (m.type == TYPE_PS && m.layerType != LT_PIXEL) {
+            P3 loS{xn, yn, lo.p.z};
+            md.dphi = phiMpiPi(up.phi - loS.phi());
+        } else {
+            P3 upS{xn, yn, up.p.z};
+            md.dphi = deltaPhi(lo.phi, upS);
+        }
+        if (m.type == TYPE_PS) {
+            md.shiftedDz = m.layerType == LT_PIXEL ? lo.p.z - zn : up.p.z - zn;
+            dz = md.shiftedDz;
+        }
+        float miniCut = m.layerType == LT_PIXEL ? dPhiThreshold(lo, s, md.dphi, dz) : dPhiThreshold(up, s, md.dphi, dz);
+        CUT(12, std::abs(md.dphi) < miniCut, std::abs(md.dphi), kNone, miniCut);
+        float dzFrac = float(double(std::abs(dz)) / std::fabs(double(lo.p.z)));
+        md.dphichange = md.dphi / dzFrac * (1.f + dzFrac);
+        CUT(13, std::abs(md.dphichange) < miniCut, std::abs(md.dphichange), kNone, miniCut);
+        return true;
+    }
+
+    // MiniDoublet ctor + runMiniDoubletDefaultAlgo (MiniDoublet.cc:28-81, 228-241)
+    bool mdAlgo(int loIdx, int upIdx, MDRec& md, Trace* tr) const {
+        const HitD& lo = hits[loIdx];
+        const HitD& up = hits[upIdx];
+        const Slot& s = slots[lo.slot];
+        md.lo = loIdx; md.up = upIdx; md.slot = lo.slot;
+        md.dz = md.shiftedDz = md.dphi = md.dphichange = 0;
+        md.anchor = (s.info.type == TYPE_PS && s.info.layerType != LT_PIXEL) ? upIdx : loIdx;
+        if (s.info.subdet == SUB_BARREL) return mdBarrel(lo, up, s, md, tr);
+        return mdEndcap(lo, up, s, md, tr);
+    }
+
+    // ---------------------------------------------------------------- SG
+    // Segment::dAlphaThreshold (Segment.cc:886-994)
+    void dAlphaThreshold(const MDRec& inMD, const MDRec& outMD, float th[3]) const {
+        static const float mulsB[6] = {0.0052f, 0.0038f, 0.0034f, 0.0034f, 0.0032f, 0.0034f};
+        static const float mulsE[5] = {0.006f, 0.006f, 0.006f, 0.006f, 0.006f};
+        const Slot& si = slots[inMD.slot];
+        const Slot& so = slots[outMD.slot];
+        const ModInfo& mi = si.info;
+        const ModInfo& mo = so.info;
+        float sdMuls = mi.subdet == SUB_BARREL ? mulsB[mi.layer - 1] * 3.f / kPtCut : mulsE[mi.layer - 1] * 3.f / kPtCut;
+        const HitD& ai = hits[inMD.anchor];
+        const HitD& ao = hits[outMD.anchor];
+        float segmentDr = float(std::sqrt(sq(ao.p.y - ai.p.y) + sq(ao.p.x - ai.p.x)));
+        const float dAlpha_Bfield = lm.asinF(fmin2(segmentDr * k2Rinv1GeVf / kPtCut, kSinAlphaMax));
+        const bool inTilted = mi.subdet == SUB_BARREL && mi.side != SIDE_CENTER;
+        const bool outTilted = mo.subdet == SUB_BARREL && mo.side != SIDE_CENTER;
+        const float di = si.drdzStrip, dO = so.drdzStrip;
+        const float inTilt = inTilted ? float(double(0.5f * kPixelPSZpitch * di) / std::sqrt(double(1.f + di * di)) / double(si.gap)) : 0.f;
+        const float outTilt = outTilted ? float(double(0.5f * kPixelPSZpitch * dO) / std::sqrt(double(1.f + dO * dO)) / double(so.gap)) : 0.f;
+        float miniDelta = si.gap;
+        float lumIn, lumOut;
+        if (mi.subdet == SUB_BARREL) lumIn = inTilt * dAlpha_Bfield;
+        else lumIn = float(double(inMD.dphi) * 15.0 / double(mi.type == TYPE_PS ? inMD.shiftedDz : inMD.dz));
+        if (mo.subdet == SUB_BARREL) lumOut = outTilt * dAlpha_Bfield;
+        else lumOut = float(double(outMD.dphi) * 15.0 / double(mo.type == TYPE_PS ? outMD.shiftedDz : outMD.dz));
+        float resIn = 0.02f / miniDelta * (mi.subdet == SUB_BARREL ? 1.0f : std::abs(ai.p.z / ai.rt));
+        float resOut = 0.02f / miniDelta * (mo.subdet == SUB_BARREL ? 1.0f : std::abs(ao.p.z / ao.rt));
+        float res = resIn + resOut;
+        double base = sq(res) + sq(sdMuls);
+        if (mi.subdet == SUB_BARREL && mi.side == SIDE_CENTER) th[0] = float(double(dAlpha_Bfield) + std::sqrt(base));
+        else th[0] = float(double(dAlpha_Bfield) + std::sqrt(base + sq(lumIn)));
+        if (mo.subdet == SUB_BARREL && mo.side == SIDE_CENTER) th[1] = float(double(dAlpha_Bfield) + std::sqrt(base));
+        else th[1] = float(double(dAlpha_Bfield) + std::sqrt(base + sq(lumOut)));
+        th[2] = float(double(dAlpha_Bfield) + std::sqrt(base));
+    }
+
+    bool sgAlphaCuts(const MDRec& inMD, const MDRec& outMD, float sgAlpha, Trace* tr) const {
+        float th[3];
+        dAlphaThreshold(inMD, outMD, th);
+        float a = inMD.dphichange - sgAlpha;
+        CUT(24, std::abs(a) < th[0], std::abs(a), kNone, th[0]);
+        float b = outMD.dphichange - sgAlpha;
+        CUT(25, std::abs(b) < th[1], std::abs(b), kNone, th[1]);
+        float c = outMD.dphichange - inMD.dphichange;
+        CUT(26, std::abs(c) < th[2], std::abs(c), kNone, th[2]);
+        return true;
+    }
+
+    // Segment::runSegmentDefaultAlgoBarrel (Segment.cc:364-599)
+    bool sgBarrel(const MDRec& inMD, const MDRec& outMD, SGRec& sg, Trace* tr) const {
+        static const float mulsB[6] = {0.0052f, 0.0038f, 0.0034f, 0.0034f, 0.0032f, 0.0034f};
+        const ModInfo& mi = slots[inMD.slot].info;
+        const float sdMuls = mulsB[mi.layer - 1] * 3.f / kPtCut;
+        const HitD& ai = hits[inMD.anchor];
+        const HitD& ao = hits[outMD.anchor];
+        float rtIn = ai.rt, rtOut = ao.rt, zIn = ai.p.z, zOut = ao.p.z;
+        const float sdSlope = lm.asinF(fmin2(rtOut * k2Rinv1GeVf / kPtCut, kSinAlphaMax));
+        const float sdPVoff = 0.1f / rtOut;
+        const float dzDrtScale = lm.tanF(sdSlope) / sdSlope;
+        const float zGeom = mi.layer <= 2 ? 2.f * kPixelPSZpitch : 2.f * kStrip2SZpitch;
+        float zLo = zIn + (zIn - kDeltaZLum) * (rtOut / rtIn - 1.f) * (zIn > 0.f ? 1.f : dzDrtScale) - zGeom;
+        float zHi = zIn + (zIn + kDeltaZLum) * (rtOut / rtIn - 1.f) * (zIn < 0.f ? 1.f : dzDrtScale) + zGeom;
+        CUT(20, zOut >= zLo && zOut <= zHi, zOut, zLo, zHi);
+        const float sdCut = float(double(sdSlope) + std::sqrt(double(sdMuls * sdMuls + sdPVoff * sdPVoff)));
+        const float dPhi = deltaPhi(ai.phi, ao.p);
+        CUT(21, std::abs(dPhi) <= sdCut, std::abs(dPhi), kNone, sdCut);
+        sg.dphichange = dPhiChange(ai.p, ai.phi, ao.p);
+        CUT(22, std::abs(sg.dphichange) <= sdCut, std::abs(sg.dphichange), kNone, sdCut);
+        return sgAlphaCuts(inMD, outMD, sg.dphichange, tr);
+    }
+
+    // Segment::runSegmentDefaultAlgoEndcap (Segment.cc:601-859)
+    bool sgEndcap(const MDRec& inMD, const MDRec& outMD, SGRec& sg, Trace* tr) const {
+        const Slot& si = slots[inMD.slot];
+        const Slot& so = slots[outMD.slot];
+        const ModInfo& mi = si.info;
+        const ModInfo& mo = so.info;
+        bool found = false;
+        for (uint32_t d : g->connected(mi.detId))
+            if (d == mo.detId) { found = true; break; }
+        bool l2r123 = (mi.ring == 1 || mi.ring == 2 || mi.ring == 3) && mi.layer == 2;
+        CUT(30, found || l2r123, kNone, kNone, kNone);
+        const HitD& ai = hits[inMD.anchor];
+        const HitD& ao = hits[outMD.anchor];
+        bool outEC2S = mo.type == TYPE_2S && mo.subdet == SUB_ENDCAP;
+        const HitD& aoHi = outEC2S ? edges[ao.hiEdge] : ao;
+        const HitD& aoLo = outEC2S ? edges[ao.loEdge] : ao;
+        float rtIn = ai.rt, rtOut = ao.rt, zIn = ai.p.z, zOut = ao.p.z;
+        const float sdSlope = lm.asinF(fmin2(rtOut * k2Rinv1GeVf / kPtCut, kSinAlphaMax));
+        const float rtGeom = (rtIn < 60.f && rtOut < 60.f) ? 2.f * kPixelPSZpitch
+                             : ((rtIn < 60.f || rtOut < 60.f) ? (kPixelPSZpitch + kStrip2SZpitch) : 2.f * kStrip2SZpitch);
+        CUT(31, !(zIn * zOut < 0), zIn * zOut, 0, kNone);
+        const float dz = zOut - zIn;
+        const float dLum = std::copysign(kDeltaZLum, zIn);
+        const float drtDzScale = sdSlope / lm.tanF(sdSlope);
+        float rtLo = fmax2(rtIn * (1.f + dz / (zIn + dLum) * drtDzScale) - rtGeom, rtIn - 0.5f * rtGeom);
+        float rtHi = rtIn * (zOut - dLum) / (zIn - dLum) + rtGeom;
+        CUT(32, rtOut >= rtLo && rtOut <= rtHi, rtOut, rtLo, rtHi);
+        const float dPhiPos = deltaPhi(ai.phi, ao.p);
+        const float sdCut = sdSlope;
+        const float dHigh = deltaPhi(ai.phi, aoHi.p);
+        const float dLow = deltaPhi(ai.phi, aoLo.p);
+        bool hiBigger = absTrunc(dHigh) > absTrunc(dLow);  // A-3
+        sg.dphi = dPhiPos;
+        sg.dphimax = hiBigger ? dHigh : dLow;
+        sg.dphimin = hiBigger ? dLow : dHigh;
+        CUT(33, std::abs(dPhiPos) <= sdCut, std::abs(dPhiPos), kNone, sdCut);
+        const float dzFrac = dz / zIn;
+        sg.dphichange = dPhiPos / dzFrac * (1.f + dzFrac);
+        sg.dphichangemin = sg.dphimin / dzFrac * (1.f + dzFrac);
+        sg.dphichangemax = sg.dphimax / dzFrac * (1.f + dzFrac);
+        CUT(34, std::abs(sg.dphichange) <= sdCut, std::abs(sg.dphichange), kNone, sdCut);
+        return sgAlphaCuts(inMD, outMD, sg.dphichange, tr);
+    }
+
+    // Segment::runSegmentDefaultAlgo dispatch (Segment.cc:337-362)
+    bool sgAlgo(int mdIn, int mdOut, SGRec& sg, Trace* tr) const {
+        sg.mdIn = mdIn; sg.mdOut = mdOut;
+        sg.dphi = sg.dphimin = sg.dphimax = sg.dphichange = sg.dphichangemin = sg.dphichangemax = 0;
+        const MDRec& a = mds[mdIn];
+        const MDRec& b = mds[mdOut];
+        bool inB = slots[a.slot].info.subdet == SUB_BARREL;
+        bool outB = slots[b.slot].info.subdet == SUB_BARREL;
+        if (inB) return outB ? sgBarrel(a, b, sg, tr) : sgEndcap(a, b, sg, tr);
+        return outB ? sgBarrel(a, b, sg, tr) : sgEndcap(a, b, sg, tr);
+    }
+
+    // ---------------------------------------------------------------- TL
+    // Tracklet::runDeltaBetaIterations (Tracklet.cc:3181-3324): the state the cuts use
+    void deltaBetaIterations(float& betaIn, float& betaOut, float& betaAv, float& ptBeta, float sdInDr, float sdOutDr,
+                             float dr, int lIn, float ptBetaMax) const {
+        auto corr = [&](float len, float pt, float signOf) {
+            return double(std::copysign(double(lm.asinF(fmin2(len * k2Rinv1GeVf / std::abs(pt), kSinAlphaMax))), double(signOf)));
+        };
+        auto ptOf = [&](float beta) { return float(double(dr * k2Rinv1GeVf) / std::sin(double(beta))); };
+        if (betaIn * betaOut > 0.f &&
+            (std::abs(ptBeta) < 4.f * ptBetaMax || (lIn >= 11 && std::abs(ptBeta) < 8.f * ptBetaMax))) {
+            const float inUpd = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
+            const float outUpd = float(double(betaOut) + corr(sdOutDr, ptBeta, betaOut));
+            betaAv = 0.5f * (inUpd + outUpd);
+            ptBeta = ptOf(betaAv);
+            betaIn = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
+            betaOut = float(double(betaOut) + corr(sdOutDr, ptBeta, betaOut));
+            betaAv = 0.5f * (betaIn + betaOut);
+            ptBeta = ptOf(betaAv);
+        } else if (lIn < 11 && double(std::abs(betaOut)) < 0.2 * double(std::abs(betaIn)) &&
+                   std::abs(ptBeta) < 12.f * ptBetaMax) {
+            const float ptBetaIn = ptOf(betaIn);
+            const float inUpd = float(double(betaIn) + corr(sdInDr, ptBetaIn, betaIn));
+            const float outUpd = float(double(betaOut) + corr(sdOutDr, ptBetaIn, betaIn));
+            betaAv = (std::abs(betaOut) > 0.2f * std::abs(betaIn)) ? (0.5f * (inUpd + outUpd)) : inUpd;
+            ptBeta = ptOf(betaAv);
+            betaIn = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
+            betaOut = float(double(betaOut) + corr(sdOutDr, ptBeta, betaIn));
+            betaAv = 0.5f * (betaIn + betaOut);
+            ptBeta = ptOf(betaAv);
+        }
+    }
+
+    // Shared tail of the three variants: rescale of the RH ranges and the dBeta cut
+    // (Tracklet.cc:1344-1501, 1875-1990, 2377-2520).
+    static void rescaleRH(float beta, float& mn, float& mx) {
+        const float sf = (std::abs(mn + mx) > 0) ? float(2.f * beta / std::abs(mn + mx)) : float(0.);
+        mn *= sf; mx *= sf;
+    }
+
+    // runTrackletDefaultAlgoBarrelBarrelBarrelBarrel_v2 (Tracklet.cc:902-1538)
+    bool tlBBBB(const SGRec& sIn, const SGRec& sOut, Trace* tr) const {
+        const MDRec& mInLo = mds[sIn.mdIn];
+        const MDRec& mInUp = mds[sIn.mdOut];
+        const MDRec& mOutLo = mds[sOut.mdIn];
+        const MDRec& mOutUp = mds[sOut.mdOut];
+        const HitD& hInLo = hits[mInLo.anchor];
+        const HitD& hInUp = hits[mInUp.anchor];
+        const HitD& hOutLo = hits[mOutLo.anchor];
+        const HitD& hOutUp = hits[mOutUp.anchor];
+        const bool isPS_InLo = slots[mInLo.slot].info.type == TYPE_PS;
+        const bool isPS_OutLo = slots[mOutLo.slot].info.type == TYPE_PS;
+        const float rt_InLo = hInLo.rt, rt_OutLo = hOutLo.rt, z_InLo = hInLo.p.z, z_OutLo = hOutLo.p.z;
+        const float alpha1GeV_OutLo = lm.asinF(fmin2(rt_OutLo * k2Rinv1GeVf / kPtCut, kSinAlphaMax));
+        const float rtRatio = rt_OutLo / rt_InLo;
+        const float dzDrtScale = float(std::tan(double(alpha1GeV_OutLo)) / double(alpha1GeV_OutLo));
+        const float zpitch_InLo = isPS_InLo ? kPixelPSZpitch : kStrip2SZpitch;
+        const float zpitch_OutLo = isPS_OutLo ? kPixelPSZpitch : kStrip2SZpitch;
+        const float zHi = z_InLo + (z_InLo + kDeltaZLum) * (rtRatio - 1.f) * (z_InLo < 0.f ? 1.f : dzDrtScale) + (zpitch_InLo + zpitch_OutLo);
+        const float zLo = z_InLo + (z_InLo - kDeltaZLum) * (rtRatio - 1.f) * (z_InLo > 0.f ? 1.f : dzDrtScale) - (zpitch_InLo + zpitch_OutLo);
+        CUT(40, z_OutLo >= zLo && z_OutLo <= zHi, z_OutLo, zLo, zHi);
+        const float drt_OutLo_InLo = rt_OutLo - rt_InLo;
+        const float r3_InLo = float(std::sqrt(double(z_InLo * z_InLo + rt_InLo * rt_InLo)));
+        const float drt_InSeg = hInUp.rt - hInLo.rt;
+        const float dz_InSeg = hInUp.p.z - hInLo.p.z;
+        const float dr3_InSeg = hInUp.r3 - hInLo.r3;
+        const float coshEta = dr3_InSeg / drt_InSeg;
+        float dzErr = (zpitch_InLo + zpitch_OutLo) * (zpitch_InLo + zpitch_OutLo) * 2.f;
+        const float sdlThetaMulsF = float(double(0.015f) * std::sqrt(double(0.1f) + 0.2 * double(rt_OutLo - rt_InLo) / double(50.f)) *
+                                          std::sqrt(double(r3_InLo / rt_InLo)));
+        const float sdlMuls = sdlThetaMulsF * 3.f / kPtCut * 4.f;
+        dzErr += sdlMuls * sdlMuls * drt_OutLo_InLo * drt_OutLo_InLo / 3.f * coshEta * coshEta;
+        dzErr = float(std::sqrt(double(dzErr)));
+        const float dzMean = dz_InSeg / drt_InSeg * drt_OutLo_InLo;
+        const float zWindow = dzErr / drt_InSeg * drt_OutLo_InLo + (zpitch_InLo + zpitch_OutLo);
+        const float zLoP = z_InLo + dzMean * (z_InLo > 0.f ? 1.f : dzDrtScale) - zWindow;
+        const float zHiP = z_InLo + dzMean * (z_InLo < 0.f ? 1.f : dzDrtScale) + zWindow;
+        CUT(41, z_OutLo >= zLoP && z_OutLo <= zHiP, z_OutLo, zLoP, zHiP);
+        const float sdlPVoff = 0.1f / rt_OutLo;
+        const float sdlCut = float(double(alpha1GeV_OutLo) + std::sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
+        const float deltaPhiPos = deltaPhi(hInUp.phi, hOutUp.p);
+        CUT(42, std::abs(deltaPhiPos) <= sdlCut, std::abs(deltaPhiPos), kNone, sdlCut);
+        const P3 mid = midpoint(hInLo.p, hOutLo.p);
+        const float dPhi = deltaPhi(mid.phi(), sub(hOutLo.p, hInLo.p));
+        CUT(43, std::abs(dPhi) <= sdlCut, std::abs(dPhi), kNone, sdlCut);
+        const float alpha_InLo = sIn.dphichange;
+        const ModInfo& mOU = slots[mOutUp.slot].info;
+        const bool isEC_lastLayer = mOU.subdet == SUB_ENDCAP && mOU.type == TYPE_2S;
+        // GeometryUtil::stripHigh/LowEdgeHit recomputed on the anchor (Tracklet.cc:1171-1172)
+        const HitD hiE = isEC_lastLayer ? edgeHit(hOutUp, true) : hOutUp;
+        const HitD loE = isEC_lastLayer ? edgeHit(hOutUp, false) : hOutUp;
+        const float alpha_OutUp = deltaPhi(hOutUp.phi, sub(hOutUp.p, hOutLo.p));
+        const float alpha_OutUp_hi = deltaPhi(hiE.phi, sub(hiE.p, hOutLo.p));
+        const float alpha_OutUp_lo = deltaPhi(loE.phi, sub(loE.p, hOutLo.p));
+        const P3 axis = sub(hOutUp.p, hInLo.p);
+        const P3 axisHi = sub(hiE.p, hInLo.p);
+        const P3 axisLo = sub(loE.p, hInLo.p);
+        float betaIn = alpha_InLo - deltaPhi(hInLo.phi, axis);
+        float betaInRHmin = betaIn, betaInRHmax = betaIn;
+        float betaOut = -alpha_OutUp + deltaPhi(hOutUp.phi, axis);
+        float betaOutRHmin = -alpha_OutUp_hi + deltaPhi(hiE.phi, axisHi);
+        float betaOutRHmax = -alpha_OutUp_lo + deltaPhi(loE.phi, axisLo);
+        const float drt_tl_axis = axis.rt();
+        const float rt_InSeg = sub(hInUp.p, hInLo.p).rt();
+        const float betaIn_cut = lm.asinF(fmin2((-rt_InSeg * 1.f + drt_tl_axis) * k2Rinv1GeVf / kPtCut, kSinAlphaMax)) + (0.02f / drt_InSeg);
+        CUT(44, std::abs(betaInRHmin) < betaIn_cut, std::abs(betaInRHmin), kNone, betaIn_cut);
+        float betaAv = 0.5f * (betaIn + betaOut);
+        float pt_beta = float(double(drt_tl_axis * k2Rinv1GeVf) / std::sin(double(betaAv)));
+        const float pt_betaMax = 7.0f;
+        const int lIn = 5, lOut = isEC_lastLayer ? 11 : 5;
+        const float sdOut_dr = sub(hOutUp.p, hOutLo.p).rt();
+        const float sdOut_d = hOutUp.rt - hOutLo.rt;
+        deltaBetaIterations(betaIn, betaOut, betaAv, pt_beta, rt_InSeg, sdOut_dr, drt_tl_axis, lIn, pt_betaMax);
+        rescaleRH(betaIn, betaInRHmin, betaInRHmax);
+        rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
+        const float dBetaMuls = sdlThetaMulsF * 4.f / fmin2(std::abs(pt_beta), pt_betaMax);
+        const float alpha_OutLo = sOut.dphichange;
+        const float alphaInAbsReg = fmax2(std::abs(alpha_InLo), lm.asinF(fmin2(hInLo.rt * k2Rinv1GeVf / 3.0f, kSinAlphaMax)));
+        const float alphaOutAbsReg = fmax2(std::abs(alpha_OutLo), lm.asinF(fmin2(hOutLo.rt * k2Rinv1GeVf / 3.0f, kSinAlphaMax)));
+        const float dBetaInLum = lIn < 11 ? 0.0f : std::abs(alphaInAbsReg * kDeltaZLum / hInLo.p.z);
+        const float dBetaOutLum = lOut < 11 ? 0.0f : std::abs(alphaOutAbsReg * kDeltaZLum / hOutLo.p.z);
+        const float dBetaLum2 = (dBetaInLum + dBetaOutLum) * (dBetaInLum + dBetaOutLum);
+        const float sinDPhi = lm.sinF(dPhi);
+        const float dBetaROut2 = float(sq(double((hiE.rt - loE.rt) * sinDPhi / drt_tl_axis)));
+        const float betaOut_cut = float(double(lm.asinF(fmin2(drt_tl_axis * k2Rinv1GeVf / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
+                                        std::sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
+        CUT(45, std::abs(betaOut) < betaOut_cut, std::abs(betaOut), kNone, betaOut_cut);
+        const float dBetaRes = 0.02f / fmin2(sdOut_d, drt_InSeg);
+        const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
+                                      0.25 * sq(double(std::abs(betaInRHmin - betaInRHmax) + std::abs(betaOutRHmin - betaOutRHmax))));
+        const float dBeta = betaIn - betaOut;
+        CUT(46, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
+        return true;
+    }
+
+    // runTrackletDefaultAlgoBarrelBarrelEndcapEndcap (Tracklet.cc:1540-2016)
+    bool tlBBEE(const SGRec& sIn, const SGRec& sOut, Trace* tr) const {
+        const MDRec& mInLo = mds[sIn.mdIn];
+        const MDRec& mInUp = mds[sIn.mdOut];
+        const MDRec& mOutLo = mds[sOut.mdIn];
+        const MDRec& mOutUp = mds[sOut.mdOut];
+        const HitD& hInLo = hits[mInLo.anchor];
+        const HitD& hInUp = hits[mInUp.anchor];
+        const HitD& hOutLo = hits[mOutLo.anchor];
+        const HitD& hOutUp = hits[mOutUp.anchor];
+        const bool psInLo = slots[mInLo.slot].info.type == TYPE_PS;
+        const bool psOutLo = slots[mOutLo.slot].info.type == TYPE_PS;
+        const float zGeom = (psInLo ? kPixelPSZpitch : kStrip2SZpitch) + (psOutLo ? kPixelPSZpitch : kStrip2SZpitch);
+        const float rtIn = hInLo.rt, rtOut = hOutLo.rt, zIn = hInLo.p.z, zOut = hOutLo.p.z;
+        const float sdlSlope = lm.asinF(fmin2(rtOut * k2Rinv1GeVf / kPtCut, kSinAlphaMax));
+        const float dzDrtScale = float(std::tan(double(sdlSlope)) / double(sdlSlope));
+        CUT(50, zIn * zOut > 0, zIn * zOut, 0, kNone);
+        const float dLum = std::copysign(kDeltaZLum, zIn);
+        const float rtGeom1 = psOutLo ? kPixelPSZpitch : kStrip2SZpitch;
+        const float zGeom1 = std::copysign(zGeom, zIn);
+        const float rtLo = rtIn * (1.f + (zOut - zIn - zGeom1) / (zIn + zGeom1 + dLum) / dzDrtScale) - rtGeom1;
+        CUT(51, rtOut >= rtLo, rtOut, rtLo, kNone);
+        float zInForHi = zIn - zGeom1 - dLum;
+        if (zInForHi * zIn < 0) zInForHi = std::copysign(0.1f, zIn);
+        const float rtHi = rtIn * (1.f + (zOut - zIn + zGeom1) / zInForHi) + rtGeom1;
+        CUT(52, rtOut >= rtLo && rtOut <= rtHi, rtOut, rtLo, rtHi);
+        const float rIn = float(std::sqrt(double(zIn * zIn + rtIn * rtIn)));
+        const float drtSDIn = hInUp.rt - hInLo.rt;
+        const float dzSDIn = hInUp.p.z - hInLo.p.z;
+        const float dr3SDIn = hInUp.r3 - hInLo.r3;
+        const float coshEta = dr3SDIn / drtSDIn;
+        const float dzOutInAbs = std::abs(zOut - zIn);
+        const float multDzDr = dzOutInAbs * coshEta / (coshEta * coshEta - 1.f);
+        const float kZ = (zOut - zIn) / dzSDIn;
+        const float sdlThetaMulsF = float(double(0.015f) * std::sqrt(double(0.1f) + 0.2 * double(rtOut - rtIn) / double(50.f)) *
+                                          std::sqrt(double(rIn / rtIn)));
+        const float sdlMuls = sdlThetaMulsF * 3.f / kPtCut * 4.f;
+        CUT(53, kZ >= 0 && rtOut >= rtLo && rtOut <= rtHi, kZ, 0, kNone);
+        const float sdlPVoff = 0.1f / rtOut;
+        const float sdlCut = float(double(sdlSlope) + std::sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
+        const float deltaPhiPos = deltaPhi(hInUp.phi, hOutUp.p);
+        CUT(54, std::abs(deltaPhiPos) <= sdlCut, std::abs(deltaPhiPos), kNone, sdlCut);
+        const P3 mid = midpoint(hInLo.p, hOutLo.p);
+        const float dPhi = deltaPhi(mid.phi(), sub(hOutLo.p, hInLo.p));
+        CUT(55, std::abs(dPhi) <= sdlCut, std::abs(dPhi), kNone, sdlCut);
+        const float sdIn_alpha = sIn.dphichange;
+        const float sdOut_alpha = sIn.dphichange;  // A-8
+        const float sdOut_alphaOut = deltaPhi(hOutUp.phi, sub(hOutUp.p, hOutLo.p));
+        const float sdOut_alphaOut_min = phiMpiPi(sOut.dphichangemin - sOut.dphimin);
+        const float sdOut_alphaOut_max = phiMpiPi(sOut.dphichangemax - sOut.dphimax);
+        const P3 dr3 = sub(hOutUp.p, hInLo.p);
+        float betaIn = sdIn_alpha - deltaPhi(hInLo.phi, dr3);
+        float betaOut = -sdOut_alphaOut + deltaPhi(hOutUp.phi, dr3);
+        float betaInRHmin = betaIn, betaInRHmax = betaIn;  // the 2S test never holds (A-9)
+        float betaOutRHmin = betaOut - sdOut_alphaOut_min + sdOut_alphaOut;
+        float betaOutRHmax = betaOut - sdOut_alphaOut_max + sdOut_alphaOut;
+        if (std::abs(betaOutRHmin) > std::abs(betaOutRHmax)) std::swap(betaOutRHmax, betaOutRHmin);
+        if (std::abs(betaInRHmin) > std::abs(betaInRHmax)) std::swap(betaInRHmax, betaInRHmin);
+        const float sdIn_dr = sub(hInUp.p, hInLo.p).rt();
+        const float sdIn_d = hInUp.rt - hInLo.rt;
+        const float dr = dr3.rt();
+        const float betaIn_cut = lm.asinF(fmin2((-sdIn_dr * 1.f + dr) * k2Rinv1GeVf / kPtCut, kSinAlphaMax)) + (0.02f / sdIn_d);
+        CUT(56, std::abs(betaInRHmin) < betaIn_cut, std::abs(betaInRHmin), kNone, betaIn_cut);
+        float betaAv = 0.5f * (betaIn + betaOut);
+        float pt_beta = float(double(dr * k2Rinv1GeVf) / std::sin(double(betaAv)));
+        const float pt_betaMax = 7.0f;
+        const int lIn = 5, lOut = 11;
+        const float sdOut_dr = sub(hOutUp.p, hOutLo.p).rt();
+        const float sdOut_d = hOutUp.rt - hOutLo.rt;
+        deltaBetaIterations(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn, pt_betaMax);
+        rescaleRH(betaIn, betaInRHmin, betaInRHmax);
+        rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
+        const float dBetaMuls = sdlThetaMulsF * 4.f / fmin2(std::abs(pt_beta), pt_betaMax);
+        const float alphaInAbsReg = fmax2(std::abs(sdIn_alpha), lm.asinF(fmin2(hInLo.rt * k2Rinv1GeVf / 3.0f, kSinAlphaMax)));
+        const float alphaOutAbsReg = fmax2(std::abs(sdOut_alpha), lm.asinF(fmin2(hOutLo.rt * k2Rinv1GeVf / 3.0f, kSinAlphaMax)));
+        const float dBetaInLum = lIn < 11 ? 0.0f : std::abs(alphaInAbsReg * kDeltaZLum / hInLo.p.z);
+        const float dBetaOutLum = lOut < 11 ? 0.0f : std::abs(alphaOutAbsReg * kDeltaZLum / hOutLo.p.z);
+        const float dBetaLum2 = (dBetaInLum + dBetaOutLum) * (dBetaInLum + dBetaOutLum);
+        const float sinDPhi = lm.sinF(dPhi);
+        const bool out2S = slots[mOutUp.slot].info.type == TYPE_2S;
+        const HitD& hiE = out2S ? edges[hOutUp.hiEdge] : hOutUp;
+        const HitD& loE = out2S ? edges[hOutUp.loEdge] : hOutUp;
+        const float dBetaROut2 = float(sq(double((hiE.rt - loE.rt) * sinDPhi / dr)));
+        const float betaOut_cut = float(double(lm.asinF(fmin2(dr * k2Rinv1GeVf / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
+                                        std::sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
+        CUT(57, std::abs(betaOut) < betaOut_cut, std::abs(betaOut), kNone, betaOut_cut);
+        const float dBetaRes = 0.02f / fmin2(sdOut_d, sdIn_d);
+        const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
+                                      0.25 * sq(double(std::abs(betaInRHmin - betaInRHmax) + std::abs(betaOutRHmin - betaOutRHmax))));
+        const float dBeta = betaIn - betaOut;
+        CUT(58, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
+        return true;
+    }
+
+    // runTrackletDefaultAlgoEndcapEndcapEndcapEndcap (Tracklet.cc:2018-2665)
+    bool tlEEEE(const SGRec& sIn, const SGRec& sOut, Trace* tr) const {
+        const MDRec& mInLo = mds[sIn.mdIn];
+        const MDRec& mInUp = mds[sIn.mdOut];
+        const MDRec& mOutLo = mds[sOut.mdIn];
+        const MDRec& mOutUp = mds[sOut.mdOut];
+        const HitD& hInLo = hits[mInLo.anchor];
+        const HitD& hInUp = hits[mInUp.anchor];
+        const HitD& hOutLo = hits[mOutLo.anchor];
+        const HitD& hOutUp = hits[mOutUp.anchor];
+        const bool psInLo = slots[mInLo.slot].info.type == TYPE_PS;
+        const bool psOutLo = slots[mOutLo.slot].info.type == TYPE_PS;
+        const bool psInUp = slots[mInUp.slot].info.type == TYPE_PS;
+        const float rtIn = hInLo.rt, rtOut = hOutLo.rt, zIn = hInLo.p.z, zOut = hOutLo.p.z;
+        const float sdlSlope = lm.asinF(fmin2(rtOut * k2Rinv1GeVf / kPtCut, kSinAlphaMax));
+        const float dzDrtScale = float(std::tan(double(sdlSlope)) / double(sdlSlope));
+        CUT(60, zIn * zOut > 0, zIn * zOut, 0, kNone);
+        const float dLum = std::copysign(kDeltaZLum, zIn);
+        const float rtGeom = (psInLo && psOutLo) ? 2.f * kPixelPSZpitch
+                             : (psInLo || psOutLo) ? (kPixelPSZpitch + kStrip2SZpitch) : 2.f * kStrip2SZpitch;
+        const float dz = zOut - zIn;
+        const float rtLo = rtIn * (1.f + dz / (zIn + dLum) / dzDrtScale) - rtGeom;
+        CUT(61, rtOut >= rtLo, rtOut, rtLo, kNone);
+        const float rtHi = rtIn * (1.f + dz / (zIn - dLum)) + rtGeom;
+        CUT(62, rtOut >= rtLo && rtOut <= rtHi, rtOut, rtLo, rtHi);
+        const float drtSDIn = hInUp.rt - hInLo.rt;
+        const float dzSDIn = hInUp.p.z - hInLo.p.z;
+        const float dr3SDIn = hInUp.r3 - hInLo.r3;
+        const float coshEta = dr3SDIn / drtSDIn;
+        const float dzOutInAbs = std::abs(zOut - zIn);
+        const float multDzDr = dzOutInAbs * coshEta / (coshEta * coshEta - 1.f);
+        const float kZ = (zOut - zIn) / dzSDIn;
+        const float sdlThetaMulsF = float(double(0.015f) * std::sqrt(double(0.1f) + 0.2 * double(rtOut - rtIn) / double(50.f)));
+        const float sdlMuls = sdlThetaMulsF * 3.f / kPtCut * 4.f;
+        float drtErr = kPixelPSZpitch * kPixelPSZpitch * 2.f / dzSDIn / dzSDIn * dzOutInAbs * dzOutInAbs;
+        drtErr += sdlMuls * sdlMuls * multDzDr * multDzDr / 3.f * coshEta * coshEta;
+        drtErr = float(std::sqrt(double(drtErr)));
+        const float drtMean = drtSDIn * dzOutInAbs / std::abs(dzSDIn);
+        const float rtWindow = drtErr + rtGeom;
+        const float rtLoP = rtIn + drtMean / dzDrtScale - rtWindow;
+        const float rtHiP = rtIn + drtMean + rtWindow;
+        if (psInLo && psInUp) CUT(63, kZ >= 0 && rtOut >= rtLoP && rtOut <= rtHiP, rtOut, rtLoP, rtHiP);
+        const float sdlPVoff = 0.1f / rtOut;
+        const float sdlCut = float(double(sdlSlope) + std::sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
+        const float deltaPhiPos = deltaPhi(hInUp.phi, hOutUp.p);
+        CUT(64, std::abs(deltaPhiPos) <= sdlCut, std::abs(deltaPhiPos), kNone, sdlCut);
+        const P3 mid = midpoint(hInLo.p, hOutLo.p);
+        const float dPhi = deltaPhi(mid.phi(), sub(hOutLo.p, hInLo.p));
+        CUT(65, std::abs(dPhi) <= sdlCut, std::abs(dPhi), kNone, sdlCut);
+        const float sdIn_alpha = sIn.dphichange;
+        const float sdOut_alpha = sIn.dphichange;  // A-8
+        const float sdOut_dPhiPos = deltaPhi(hOutLo.phi, hOutUp.p);
+        const float sdOut_alphaOutRHmin = phiMpiPi(sOut.dphichangemin - sdOut_dPhiPos);
+        const float sdOut_alphaOutRHmax = phiMpiPi(sOut.dphichangemax - sdOut_dPhiPos);
+        const float sdOut_alphaOut = phiMpiPi(sOut.dphichange - sdOut_dPhiPos);
+        const P3 dr3 = sub(hOutUp.p, hInLo.p);
+        float betaIn = sdIn_alpha - deltaPhi(hInLo.phi, dr3);
+        float betaOut = -sdOut_alphaOut + deltaPhi(hOutUp.phi, dr3);
+        float betaInRHmin = betaIn + sIn.dphichangemin - sdIn_alpha;
+        float betaInRHmax = betaIn + sIn.dphichangemax - sdIn_alpha;
+        if (std::abs(betaInRHmin) > std::abs(betaInRHmax)) std::swap(betaInRHmax, betaInRHmin);
+        float betaOutRHmin = betaOut - sdOut_alphaOutRHmin + sdOut_alphaOut;
+        float betaOutRHmax = betaOut - sdOut_alphaOutRHmax + sdOut_alphaOut;
+        if (std::abs(betaOutRHmin) > std::abs(betaOutRHmax)) std::swap(betaOutRHmax, betaOutRHmin);
+        const float sdIn_dr = sub(hInUp.p, hInLo.p).rt();
+        const float sdIn_d = hInUp.rt - hInLo.rt;
+        const float dr = dr3.rt();
+        // the betaIn cut is computed but never rejects (A-11)
+        float betaAv = 0.5f * (betaIn + betaOut);
+        float pt_beta = float(double(dr * k2Rinv1GeVf) / std::sin(double(betaAv)));
+        const float pt_betaMax = 7.0f;
+        const int lIn = 11, lOut = 13;
+        const float sdOut_dr = sub(hOutUp.p, hOutLo.p).rt();
+        const float sdOut_d = hOutUp.rt - hOutLo.rt;
+        deltaBetaIterations(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn, pt_betaMax);
+        rescaleRH(betaIn, betaInRHmin, betaInRHmax);
+        rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
+        const float dBetaMuls = sdlThetaMulsF * 4.f / fmin2(std::abs(pt_beta), pt_betaMax);
+        const float alphaInAbsReg = fmax2(std::abs(sdIn_alpha), lm.asinF(fmin2(hInLo.rt * k2Rinv1GeVf / 3.0f, kSinAlphaMax)));
+        const float alphaOutAbsReg = fmax2(std::abs(sdOut_alpha), lm.asinF(fmin2(hOutLo.rt * k2Rinv1GeVf / 3.0f, kSinAlphaMax)));
+        const float dBetaInLum = lIn < 11 ? 0.0f : std::abs(alphaInAbsReg * kDeltaZLum / hInLo.p.z);
+        const float dBetaOutLum = lOut < 11 ? 0.0f : std::abs(alphaOutAbsReg * kDeltaZLum / hOutLo.p.z);
+        const float dBetaLum2 = (dBetaInLum + dBetaOutLum) * (dBetaInLum + dBetaOutLum);
+        const float betaOut_cut = float(double(lm.asinF(fmin2(dr * k2Rinv1GeVf / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
+                                        std::sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
+        CUT(67, std::abs(betaOut) < betaOut_cut, std::abs(betaOut), kNone, betaOut_cut);
+        const float dBetaRes = 0.02f / fmin2(sdOut_d, sdIn_d);
+        const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + 0.f) +
+                                      0.25 * sq(double(std::abs(betaInRHmin - betaInRHmax) + std::abs(betaOutRHmin - betaOutRHmax))));
+        const float dBeta = betaIn - betaOut;
+        CUT(68, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
+        return true;
+    }
+
+    // Tracklet::runTrackletDefaultAlgo dispatch (Tracklet.cc:144-296)
+    bool tlAlgo(const SGRec& sIn, const SGRec& sOut, Trace* tr) const {
+        bool b0 = slots[mds[sIn.mdIn].slot].info.subdet == SUB_BARREL;
+        bool b1 = slots[mds[sIn.mdOut].slot].info.subdet == SUB_BARREL;
+        bool b2 = slots[mds[sOut.mdIn].slot].info.subdet == SUB_BARREL;
+        bool b3 = slots[mds[sOut.mdOut].slot].info.subdet == SUB_BARREL;
+        if (b0 && b1 && b2) return tlBBBB(sIn, sOut, tr);          // BBBB, BBBE
+        if (b0 && b1 && !b2 && !b3) return tlBBEE(sIn, sOut, tr);  // BBEE
+        if (b0 && !b1 && !b2 && !b3) return tlBBEE(sIn, sOut, tr); // BEEE
+        if (!b0 && !b1 && !b2 && !b3) return tlEEEE(sIn, sOut, tr);
+        if (tr) tr->add(99, kNone, kNone, kNone, false);
+        return false;                                              // other mixes fail (A-14)
+    }
+
+    // MiniDoublet::isIdxMatched (MiniDoublet.cc:686-693)
+    bool mdMatch(int a, int b) const {
+        return hits[mds[a].lo].idx == hits[mds[b].lo].idx && hits[mds[a].up].idx == hits[mds[b].up].idx;
+    }
+    // Segment::isIdxMatched (Segment.cc:868-875)
+    bool sgMatch(int a, int b) const { return mdMatch(sgs[a].mdIn, sgs[b].mdIn) && mdMatch(sgs[a].mdOut, sgs[b].mdOut); }
+
+    int lowerSlotOfMD(int md) const { return mds[md].slot; }
+
+    // ---------------------------------------------------------------- stages
+    // Event::createMiniDoublets (Event.cc:245-305)
+    void createMiniDoublets() {
+        std::vector<int> order = lowerOrder;
+        for (int ls : order) {
+            int us = getSlot(slots[ls].info.partner);
+            const std::vector<int> loHits = slots[ls].hits;
+            const std::vector<int> upHits = slots[us].hits;
+            for (int lo : loHits)
+                for (int up : upHits) {
+                    nCand[0]++;
+                    MDRec md;
+                    if (mdAlgo(lo, up, md, nullptr)) {
+                        slots[ls].mds.push_back(int(mds.size()));
+                        mds.push_back(md);
+                    }
+                }
+        }
+    }
+    // Event::createSegmentsFromInnerLowerModule (Event.cc:457-522)
+    void createSegments() {
+        for (int ls : lowerOrder) {
+            const std::vector<uint32_t>& conn = g->connected(slots[ls].info.detId);
+            for (int mi : slots[ls].mds)
+                for (uint32_t d : conn) {
+                    int os = findSlot(d);
+                    if (os < 0) continue;
+                    for (int mo : slots[os].mds) {
+                        nCand[1]++;
+                        SGRec sg;
+                        if (sgAlgo(mi, mo, sg, nullptr)) {
+                            slots[ls].sgs.push_back(int(sgs.size()));
+                            sgs.push_back(sg);
+                        }
+                    }
+                }
+        }
+    }
+    // Event::createTripletsFromInnerLowerModule (Event.cc:539-597) + Triplet.cc:50-122
+    void createTriplets() {
+        for (int ls : lowerOrder) {
+            const std::vector<uint32_t>& conn = g->connected(slots[ls].info.detId);
+            for (int si : slots[ls].sgs)
+                for (uint32_t d : conn) {
+                    int os = findSlot(d);
+                    if (os < 0) continue;
+                    for (int so : slots[os].sgs) {
+                        nCand[2]++;
+                        if (mdMatch(sgs[si].mdOut, sgs[so].mdIn)) {
+                            slots[ls].tps.push_back(int(tps.size()));
+                            tps.push_back({si, so});
+                        }
+                    }
+                }
+        }
+    }
+    // Event::createTrackletsFromInnerLowerModule (Event.cc:627-693)
+    void createTracklets() {
+        for (int ls : lowerOrder) {
+            for (int si : slots[ls].sgs) {
+                uint32_t bId = slots[mds[sgs[si].mdOut].slot].info.detId;
+                for (uint32_t d : g->connected(bId)) {
+                    int os = findSlot(d);
+                    if (os < 0) continue;
+                    for (int so : slots[os].sgs) {
+                        nCand[3]++;
+                        if (tlAlgo(sgs[si], sgs[so], nullptr)) {
+                            slots[ls].tls.push_back(int(tls.size()));
+                            tls.push_back({si, so});
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // Event::createTrackCandidatesFromInnerModulesFromTriplets (Event.cc:889-982)
+    void createTCFromTriplets() {
+        for (int ls : lowerOrder) {
+            for (int ti : slots[ls].tps) {
+                int mdC = sgs[tps[ti].out].mdOut;
+                uint32_t cId = slots[mds[mdC].slot].info.detId;
+                for (uint32_t d : g->connected(cId)) {
+                    int os = findSlot(d);
+                    if (os < 0) continue;
+                    for (int to : slots[os].tps) {
+                        nCand[4]++;
+                        SGRec bridge;
+                        if (!sgAlgo(mdC, sgs[tps[to].in].mdIn, bridge, nullptr)) continue;
+                        if (!tlAlgo(bridge, sgs[tps[to].out], nullptr)) continue;
+                        tcs.push_back({0, {ti, to}});
+                    }
+                }
+            }
+        }
+    }
+    // Event::createTrackCandidatesFromInnerModulesFromTracklets (Event.cc:999-1062)
+    // + TrackCandidate::runTrackCandidateDefaultAlgo (TrackCandidate.cc:143-226, A-13)
+    void createTCFromTracklets() {
+        for (int ls : lowerOrder) {
+            for (int li : slots[ls].tls) {
+                uint32_t bId = slots[mds[sgs[tls[li].in].mdOut].slot].info.detId;
+                for (uint32_t d : g->connected(bId)) {
+                    int os = findSlot(d);
+                    if (os < 0) continue;
+                    for (int lo : slots[os].tls) {
+                        nCand[4]++;
+                        if (sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo}});
+                    }
+                }
+            }
+        }
+    }
+
+    void run(int stages) {
+        if (stages >= 1) createMiniDoublets();
+        if (stages >= 2) createSegments();
+        if (stages >= 3) createTriplets();
+        if (stages >= 4) createTracklets();
+        if (stages >= 5) createTCFromTriplets();
+        if (stages >= 6) createTCFromTracklets();
+    }
+
+    // ------------------------------------------------------------- exporting
+    void putMD(std::vector<int>& o, int m) const { o.push_back(hits[mds[m].lo].idx); o.push_back(hits[mds[m].up].idx); }
+    void putSG(std::vector<int>& o, int s) const { putMD(o, sgs[s].mdIn); putMD(o, sgs[s].mdOut); }
+    std::vector<int> tuples(int stage) const {
+        std::vector<int> o;
+        if (stage == ORACLE_MD) for (size_t i = 0; i < mds.size(); ++i) putMD(o, int(i));
+        if (stage == ORACLE_SG) for (size_t i = 0; i < sgs.size(); ++i) putSG(o, int(i));
+        if (stage == ORACLE_TP) for (auto& t : tps) { putSG(o, t.in); putSG(o, t.out); }
+        if (stage == ORACLE_TL) for (auto& t : tls) { putSG(o, t.in); putSG(o, t.out); }
+        for (auto& tc : tcs) {
+            if (stage == ORACLE_TCP && tc.first == 0) {
+                putSG(o, tps[tc.second.in].in); putSG(o, tps[tc.second.in].out);
+                putSG(o, tps[tc.second.out].in); putSG(o, tps[tc.second.out].out);
+            }
+            if (stage == ORACLE_TCL && tc.first == 1) {
+                putSG(o, tls[tc.second.in].in); putSG(o, tls[tc.second.in].out);
+                putSG(o, tls[tc.second.out].in); putSG(o, tls[tc.second.out].out);
+            }
+        }
+        return o;
+    }
+};
+
+struct Result {
+    std::vector<int> t[6];
+    unsigned long long counters[10];
+};
+
+void fillEvent(Event& ev, const Geometry* g, int libm, int n, const float* x, const float* y, const float* z,
+               const unsigned* detId, const int* idx) {
+    ev.g = g;
+    ev.lm.cr = libm == ORACLE_LIBM_CR;
+    ev.hits.reserve(n);
+    for (int i = 0; i < n; ++i) ev.addHit(x[i], y[i], z[i], idx[i], detId[i]);
+}
+
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+void* oracle_geom_load(const char* tiltedPath, const char* endcapPath, const char* mapPath) {
+    auto* g = new Geometry();
+    if (!loadGeometry(*g, tiltedPath, endcapPath, mapPath)) { delete g; return nullptr; }
+    return g;
+}
+
+void oracle_geom_free(void* geom) { delete static_cast<Geometry*>(geom); }
+
+void* oracle_run(void* geom, int nhits, const float* x, const float* y, const float* z, const unsigned* detId,
+                 const int* idx, int libm, int stages) {
+    Event ev;
+    fillEvent(ev, static_cast<Geometry*>(geom), libm, nhits, x, y, z, detId, idx);
+    ev.run(stages);
+    auto* r = new Result();
+    for (int s = 0; s < 6; ++s) r->t[s] = ev.tuples(s);
+    for (int s = 0; s < 5; ++s) r->counters[s] = ev.nCand[s];
+    r->counters[5] = ev.mds.size();
+    r->counters[6] = ev.sgs.size();
+    r->counters[7] = ev.tps.size();
+    r->counters[8] = ev.tls.size();
+    r->counters[9] = ev.tcs.size();
+    return r;
+}
+
+static const int kWidth[6] = {2, 4, 8, 8, 16, 16};
+
+long oracle_result_count(void* res, int stage) {
+    return long(static_cast<Result*>(res)->t[stage].size() / kWidth[stage]);
+}
+void oracle_result_copy(void* res, int stage, int* out) {
+    const std::vector<int>& v = static_cast<Result*>(res)->t[stage];
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(int));
+}
+void oracle_result_counters(void* res, unsigned long long* out10) {
+    std::memcpy(out10, static_cast<Result*>(res)->counters, sizeof(unsigned long long) * 10);
+}
+void oracle_result_free(void* res) { delete static_cast<Result*>(res); }
+
+int oracle_explain(void* geom, int nhits, const float* x, const float* y, const float* z, const unsigned* detId,
+                   const int* idx, int kind, const int* a, const int* b, double* out) {
+    Geometry* g = static_cast<Geometry*>(geom);
+    Trace trs[2];
+    int decided[2] = {-1, -1};
+    for (int mode = 0; mode < 2; ++mode) {
+        Event ev;
+        fillEvent(ev, g, mode, nhits, x, y, z, detId, idx);
+        std::unordered_map<int, int> pos;  // user idx -> hit position
+        for (int i = 0; i < nhits; ++i) pos[idx[i]] = i;
+        auto makeMD = [&](int lo, int up) -> int {
+            MDRec md;
+            ev.mdAlgo(pos.at(lo), pos.at(up), md, nullptr);
+            ev.mds.push_back(md);
+            return int(ev.mds.size()) - 1;
+        };
+        bool ok;
+        if (kind == 0) {
+            MDRec md;
+            ok = ev.mdAlgo(pos.at(a[0]), pos.at(a[1]), md, &trs[mode]);
+        } else if (kind == 1) {
+            int m0 = makeMD(a[0], a[1]), m1 = makeMD(a[2], a[3]);
+            SGRec sg;
+            ok = ev.sgAlgo(m0, m1, sg, &trs[mode]);
+        } else {
+            int m0 = makeMD(a[0], a[1]), m1 = makeMD(a[2], a[3]), m2 = makeMD(b[0], b[1]), m3 = makeMD(b[2], b[3]);
+            SGRec s0, s1;
+            ev.sgAlgo(m0, m1, s0, nullptr);
+            ev.sgAlgo(m2, m3, s1, nullptr);
+            ok = ev.tlAlgo(s0, s1, &trs[mode]);
+        }
+        decided[mode] = ok ? -1 : trs[mode].failed;
+    }
+    // the deciding cut: the one that failed under one policy
+    int cut = decided[0] >= 0 ? decided[0] : decided[1];
+    out[0] = decided[0];
+    out[1] = decided[1];
+    for (int k = 2; k < 9; ++k) out[k] = kNone;
+    if (cut < 0) return 1;
+    for (int mode = 0; mode < 2; ++mode)
+        for (int i = 0; i < trs[mode].n; ++i)
+            if (trs[mode].id[i] == cut) {
+                out[2 + 3 * mode] = trs[mode].v[i];
+                out[3 + 3 * mode] = trs[mode].lo[i];
+                out[4 + 3 * mode] = trs[mode].hi[i];
+            }
+    // distance (in float ulps at the threshold) between variable and nearest bound, GLIBC policy
+    double v = out[2], best = 1e300;
+    for (int k = 3; k <= 4; ++k) {
+        double t = out[k];
+        if (std::isnan(t) || std::isnan(v)) continue;
+        float tf = float(t);
+        float ulp = std::nextafter(std::abs(tf), INFINITY) - std::abs(tf);
+        best = std::min(best, std::abs(v - t) / double(ulp));
+    }
+    out[8] = best;
+    return 0;
+}
+
+double oracle_time(void* geom, int nevents, const long* offsets, const float* x, const float* y, const float* z,
+                   const unsigned* detId, const int* idx, int libm, int threads) {
+    Geometry* g = static_cast<Geometry*>(geom);
+    std::atomic<int> next(0);
+    auto t0 = std::chrono::steady_clock::now();
+    auto work = [&]() {
+        for (;;) {
+            int e = next.fetch_add(1);
+            if (e >= nevents) break;
+            long o = offsets[e], n = offsets[e + 1] - offsets[e];
+            Event ev;
+            fillEvent(ev, g, libm, int(n), x + o, y + o, z + o, detId + o, idx + o);
+            ev.run(6);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < std::max(1, threads); ++t) th.emplace_back(work);
+    for (auto& t : th) t.join();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// CPU ORACLE for the SDL segment-linking chain -- test infrastructure only.
+//
+// A from-scratch C++ restatement of the reference algorithm (bsathian/SDL),
+// used by tests/ to check the HIP path, and by bench.py's cpu_baseline leg.
+// Nothing in the product (sdl_amd/) may include, link or call it.
+//
+// Arithmetic follows the compiled reference exactly: float where the reference
+// computes in float, double where its unqualified calls and double literals
+// promote (sqrt, sin, tan, pow, atan2, 0.2*, 1./, M_PI, ...), and the
+// integer-truncating abs(float) of libstdc++ where the reference calls the
+// unqualified abs (MiniDoublet.cc:273,787, Segment.cc:743-744).
+//
+// Two libm policies:
+//   LIBM_GLIBC : std::asin/atan/sin/cos/tan on floats call glibc's float
+//                functions, as the compiled reference does;
+//   LIBM_CR    : those float functions are evaluated in double and rounded
+//                once to float (the correctly rounded value in practice),
+//                which is what the HIP kernels compute.
+// The GLIBC policy is pinned against the compiled reference; the CR policy is
+// the bit-exact counterpart of the device path.  Differences between the two
+// are libm rounding effects and are listed with their deciding cut margins.
+#pragma once
+
+#include <cstdint>
+
+extern "C" {
+
+enum { ORACLE_LIBM_GLIBC = 0, ORACLE_LIBM_CR = 1 };
+
+// Stage indices used by the result accessors.
+enum { ORACLE_MD = 0, ORACLE_SG = 1, ORACLE_TP = 2, ORACLE_TL = 3, ORACLE_TCP = 4, ORACLE_TCL = 5 };
+
+// Geometry in the reference's three text formats (TiltedGeometry.cc:36,
+// EndcapGeometry.cc:48, ModuleConnectionMap.cc:36-44).  Returns null on error.
+void* oracle_geom_load(const char* tiltedPath, const char* endcapPath, const char* mapPath);
+void oracle_geom_free(void* geom);
+
+// Runs the first `stages` (1..6) module-map stages on one event.
+void* oracle_run(void* geom, int nhits, const float* x, const float* y, const float* z, const unsigned* detId,
+                 const int* idx, int libm, int stages);
+// Tuple width: MD 2, SG 4, TP 8, TL 8, TCP 16, TCL 16 hit indices.
+long oracle_result_count(void* res, int stage);
+void oracle_result_copy(void* res, int stage, int* out);
+// out[0..4] candidates (MD, SG, TP, TL, TC), out[5..9] objects (MD, SG, TP, TL, TC)
+void oracle_result_counters(void* res, unsigned long long* out10);
+void oracle_result_free(void* res);
+
+// Re-evaluates one candidate under both libm policies and reports the cut where
+// the decisions differ.  kind: 0 MD (a,b = lower/upper hit positions in the
+// event arrays), 1 SG (a,b = MD tuples (4 ints) looked up in the GLIBC result),
+// 2 TL (a,b = SG tuples (8 ints)).  out: [cutGlibc, cutCR, var, lo, hi,
+// var_cr, lo_cr, hi_cr, ulpDistance].  Returns 0 on success.
+int oracle_explain(void* geom, int nhits, const float* x, const float* y, const float* z, const unsigned* detId,
+                   const int* idx, int kind, const int* a, const int* b, double* out);
+
+// Times `nevents` generated-elsewhere events run by `threads` threads (one event
+// per thread at a time); events are passed as concatenated hit arrays with
+// offsets[nevents+1].  Returns wall seconds.
+double oracle_time(void* geom, int nevents, const long* offsets, const float* x, const float* y, const float* z,
+                   const unsigned* detId, const int* idx, int libm, int threads);
+}
