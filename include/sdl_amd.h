@@ -1,0 +1,122 @@
+/* C-ABI of the MI355X SDL segment-linking library (libsdl_amd.so).
+ *
+ * The reference (bsathian/SDL) exposes a C++ class, SDL::Event (Event.h:24-411),
+ * plus three process-global geometry singletons loaded from text files
+ * (TiltedGeometry.h:33, EndcapGeometry.h:45, ModuleConnectionMap.h:32).  This
+ * header is the drop-in boundary that replaces them: plain pointers and sizes,
+ * int status codes (0 = ok, < 0 = error, message in sdl_last_error()).  The C++
+ * shim in sdl_amd/include/SDL/ re-exposes the reference's class API on top of it.
+ *
+ * One handle (sdl_event) holds a batch of one or more events and owns one HIP
+ * stream; handles are independent.  All device work of a handle is ordered on
+ * its stream; the geometry is immutable after loading and may be shared by
+ * any number of handles.
+ */
+#ifndef SDL_AMD_H
+#define SDL_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdl_geom sdl_geom;
+typedef struct sdl_event sdl_event;
+
+enum {
+    SDL_OK = 0,
+    SDL_ERR_ARG = -1,
+    SDL_ERR_IO = -2,
+    SDL_ERR_HIP = -3,
+    SDL_ERR_UNKNOWN_DETID = -4,
+    SDL_ERR_DUPLICATE_CONNECTION = -5,
+    SDL_ERR_CAPACITY = -6,
+    SDL_ERR_STATE = -7
+};
+
+/* Stage identifiers of sdl_event_run / export (Event.h:240-300). */
+enum {
+    SDL_STAGE_MD = 0,   /* createMiniDoublets                   Event.cc:245-305 */
+    SDL_STAGE_SG = 1,   /* createSegmentsWithModuleMap          Event.cc:445-522 */
+    SDL_STAGE_TP = 2,   /* createTriplets                       Event.cc:524-597 */
+    SDL_STAGE_TL = 3,   /* createTrackletsWithModuleMap         Event.cc:611-693 */
+    SDL_STAGE_TCP = 4,  /* createTrackCandidatesFromTriplets    Event.cc:874-982 */
+    SDL_STAGE_TCL = 5   /* createTrackCandidatesFromTracklets   Event.cc:984-1062 */
+};
+
+/* Last error message of the calling thread. */
+const char* sdl_last_error(void);
+
+/* Loads the three geometry files in the reference's formats:
+ *   tilted  "detid drdz slope"                 (TiltedGeometry.cc:18-41)
+ *   endcap  "detid avgr2 yl sl yh sh c7 c8 cz" (EndcapGeometry.cc:18-61; column 7 is read as the centroid phi)
+ *   map     "detid n d1 ... dn"                (ModuleConnectionMap.cc:18-47)
+ * and uploads an immutable device copy (module table + CSR connection map) to
+ * the current HIP device.  A connection list holding the same detId twice is
+ * rejected with SDL_ERR_DUPLICATE_CONNECTION. */
+int sdl_geom_load(const char* tiltedPath, const char* endcapPath, const char* mapPath, sdl_geom** out);
+void sdl_geom_free(sdl_geom* g);
+/* Number of sensor modules known to the geometry. */
+int sdl_geom_num_modules(const sdl_geom* g);
+
+/* Creates an event handle (batch of events) on the geometry's device. */
+int sdl_event_create(sdl_geom* g, sdl_event** out);
+void sdl_event_destroy(sdl_event* ev);
+
+/* Clears all hits and objects (keeps device allocations for reuse). */
+int sdl_event_reset(sdl_event* ev);
+
+/* Appends hits of the current (last) event from HOST arrays: the buffered form of
+ * Event::addHitToModule(Hit(x, y, z, idx), detId) (Event.cc:120-139).  idx must be
+ * unique within an event (objects are matched by hit index, Hit.cc:175-182). */
+int sdl_event_add_hits(sdl_event* ev, int n, const float* x, const float* y, const float* z, const int* idx,
+                       const unsigned int* detId);
+/* Starts a new event in the batch (the next add_hits call fills it). */
+int sdl_event_next(sdl_event* ev);
+
+/* Sets the whole batch from DEVICE-resident arrays: hits of event e occupy
+ * [offsets[e], offsets[e+1]) (offsets is a HOST array of nevents+1 entries).
+ * Every detId must be known to the geometry (else SDL_ERR_UNKNOWN_DETID). */
+int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offsets, const float* dx, const float* dy,
+                              const float* dz, const int* didx, const unsigned int* ddetId);
+
+/* Runs one stage on every event of the batch (stages must run in order; a
+ * stage whose inputs are missing returns SDL_ERR_STATE).  algo: 0 = Default,
+ * the only algorithm with device kernels. */
+int sdl_event_run(sdl_event* ev, int stage, int algo);
+/* Runs MD, SG, TP, TL, TCP and TCL back to back. */
+int sdl_event_run_chain(sdl_event* ev);
+/* Waits for the handle's stream. */
+int sdl_event_sync(sdl_event* ev);
+
+/* Counters of event `e` (-1 = whole batch): out[0..4] candidates considered for
+ * MD, SG, TP, TL, TC; out[5..9] objects formed (Event.cc:1064-1167). */
+int sdl_event_counts(sdl_event* ev, int e, unsigned long long out[10]);
+/* Per-layer counters of event e (Event.cc:1169-1344): out[11*k + j] for
+ * k = 0..9 (as above) and j = 0..5 barrel layers 1-6, 6..10 endcap layers 1-5. */
+int sdl_event_layer_counts(sdl_event* ev, int e, unsigned long long out[110]);
+
+/* Exports the objects of one stage of event e as hit-index tuples
+ * (MD 2, SG 4, TP 8, TL 8, TCP 16, TCL 16 ints per object) into out (cap ints).
+ * *n receives the number of objects; returns SDL_ERR_CAPACITY if cap is short. */
+int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n);
+
+/* Number of events in the batch. */
+int sdl_event_num_events(const sdl_event* ev);
+
+/* The HIP stream of the handle (hipStream_t), for callers that order their own work after it. */
+void* sdl_event_stream(sdl_event* ev);
+
+/* Diagnostics with no counterpart in the reference: when timing is on, the
+ * hit-preparation, MD, SG and TL kernels of each run are bracketed with HIP
+ * events on the handle's stream; sdl_event_kernel_ms writes their durations
+ * (ms, in that order, n <= 4) for the last run. */
+int sdl_event_set_timing(sdl_event* ev, int on);
+int sdl_event_kernel_ms(sdl_event* ev, double* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

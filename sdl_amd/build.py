@@ -1,0 +1,73 @@
+"""Builds the native libraries in-tree.
+
+* ``sdl_amd/_lib/libsdl_amd.so`` -- the HIP kernels and the C-ABI (gfx950).
+* ``sdl_amd/_lib/libsdl_gen.so`` -- the synthetic detector / event generator (host C++).
+* ``sdl_amd/_lib/libsdl_shim.so`` and ``sdl_amd/_lib/sdl_shim_demo`` -- the C++
+  ``SDL::Event`` shim over the C-ABI and a small driver that uses it.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "_lib")
+CSRC = os.path.join(HERE, "csrc")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    # bit-exact parity with the reference's float/double arithmetic: no FMA contraction
+    "-ffp-contract=off",
+    # no packed FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return r
+
+
+def build_gen():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libsdl_gen.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", os.path.join(CSRC, "gen", "sdl_gen.cc"), "-o", out])
+    return out
+
+
+def build_hip():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libsdl_amd.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(CSRC, "sdl_amd.hip"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_shim():
+    os.makedirs(LIB, exist_ok=True)
+    inc = [f"-I{os.path.join(HERE, 'include')}", f"-I{os.path.join(ROOT, 'include')}"]
+    so = os.path.join(LIB, "libsdl_shim.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", *inc, os.path.join(CSRC, "shim", "sdl_shim.cc"),
+          f"-L{LIB}", "-lsdl_amd", "-Wl,-rpath,$ORIGIN", "-o", so])
+    demo = os.path.join(LIB, "sdl_shim_demo")
+    _run(["g++", "-O2", "-std=c++17", *inc, f"-I{os.path.join(CSRC, 'gen')}", os.path.join(CSRC, "shim", "shim_demo.cc"),
+          os.path.join(CSRC, "gen", "sdl_gen.cc"), f"-L{LIB}", "-lsdl_shim", "-lsdl_amd", "-Wl,-rpath,$ORIGIN", "-o", demo])
+    return so
+
+
+def build_all(with_shim=True):
+    build_gen()
+    build_hip()
+    if with_shim and os.path.exists(os.path.join(CSRC, "shim", "sdl_shim.cc")):
+        build_shim()
+
+
+if __name__ == "__main__":
+    build_all()

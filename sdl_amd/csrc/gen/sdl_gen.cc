@@ -485,6 +485,15 @@ EventConfig presetConfig(int workload) {
             c.ntracks_min = 2000; c.ntracks_max = 10000; c.n_hard = 20; c.pt_mode = 1; c.eta_max = 2.4; c.ms = true;
             c.frac_lowpt = 0.15; c.frac_displaced = 0.05; c.noise_rate = 0.10;
             break;
+        // small slices with the same geometry and distributions, for parity tests
+        // at sizes the CPU reference finishes in seconds
+        case 12:
+            c.ntracks_min = c.ntracks_max = 300; c.n_hard = 2; c.pt_mode = 1; c.eta_max = 2.4; c.ms = true;
+            break;
+        case 15:
+            c.ntracks_min = c.ntracks_max = 300; c.n_hard = 2; c.pt_mode = 1; c.eta_max = 2.4; c.ms = true;
+            c.frac_lowpt = 0.15; c.frac_displaced = 0.05; c.noise_rate = 0.10;
+            break;
         default:
             break;
     }

@@ -104,7 +104,8 @@ void buildGeometry(int kind, Geometry& g, int mapTracks = 200000);
 void writeGeometryFiles(const Geometry& g, const std::string& tiltedPath,
                         const std::string& endcapPath, const std::string& mapPath);
 
-// Returns a preset for workload 1..5 (see the workload table in the docs).
+// Returns a preset for workload 1..5 (the five benchmark configurations) or
+// 12 / 15 (300-track slices of workloads 2 / 5 used by the parity tests).
 EventConfig presetConfig(int workload);
 
 // Generates event number `ievent` of the given configuration.

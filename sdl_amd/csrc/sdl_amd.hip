@@ -1,0 +1,1657 @@
+// MI355X (gfx950) implementation of the SDL module-map chain behind the C-ABI
+// of include/sdl_amd.h.
+//
+// Data layout (all flat SoA in HBM, one "slot" per (event, sensor module)):
+//   hits        x y z idx (input) + rt r3 phi slot, and the two strip-edge
+//               points of endcap-2S hits (Event.cc:132-138)
+//   slots       hit range (counting sort by slot), MD/SG/TP/TL ranges
+//   objects     dense arrays ordered by their inner object: MDs by lower slot,
+//               SGs by inner MD, TPs and TLs by inner SG, TCs by tracklet.
+// Each combinatorial stage runs one wavefront per work item (lower slot for
+// MD, inner MD for SG, inner SG for TL); the 64 lanes evaluate candidates in
+// the reference's loop order and survivors are compacted with a wave ballot
+// into a slab sized by the candidate count, then repacked densely after an
+// exclusive scan.  TP, TC-from-triplets and TC-from-tracklets are integer joins
+// over those dense arrays (see the comments at each kernel).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/sdl_amd.h"
+#include "sdl_cuts.h"
+
+using namespace sdl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t _e = (call);                                                               \
+        if (_e != hipSuccess) return fail(SDL_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n) {
+        if (n <= cap && p) return 0;
+        size_t want = std::max<size_t>(n, 1);
+        want = std::max(want, cap + cap / 2);
+        // queued work may still read the old buffer
+        if (p) (void)hipDeviceSynchronize();
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
+        cap = want;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// ------------------------------------------------------------------ scans
+// Exclusive scan of n counts into n+1 offsets (out[n] = total), three phases.
+constexpr int kScanItems = 8;
+
+template <class TIn>
+__global__ void k_scan_local(const TIn* in, long long* out, long long* blockSums, long long n) {
+    __shared__ long long sh[kBlock];
+    const long long base = (long long)blockIdx.x * kBlock * kScanItems + threadIdx.x * kScanItems;
+    long long v[kScanItems];
+    long long s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        long long i = base + k;
+        v[k] = i < n ? (long long)in[i] : 0;
+        s += v[k];
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        long long t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    long long run = sh[threadIdx.x] - s;
+    for (int k = 0; k < kScanItems; ++k) {
+        long long i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kBlock - 1) blockSums[blockIdx.x] = sh[kBlock - 1];
+}
+
+__global__ void k_scan_add(long long* out, const long long* blockOff, long long n) {
+    const long long base = (long long)blockIdx.x * kBlock * kScanItems;
+    const long long add = blockOff[blockIdx.x];
+    for (int k = threadIdx.x; k < kBlock * kScanItems; k += kBlock) {
+        long long i = base + k;
+        if (i < n) out[i] += add;
+    }
+}
+
+__global__ void k_set_total(long long* out, long long n, const long long* blockOff, long long nb) {
+    out[n] = blockOff[nb];
+}
+
+struct ScanScratch {
+    DBuf<long long> sums[4], offs[4];
+};
+
+// out must hold n+1 entries; out[n] receives the total.
+template <class TIn>
+int scanExclusive(const TIn* in, long long* out, long long n, ScanScratch& sc, hipStream_t st, int level = 0) {
+    if (n <= 0) {
+        HIP_OK(hipMemsetAsync(out, 0, sizeof(long long), st));
+        return 0;
+    }
+    if (level >= 4) return fail(SDL_ERR_CAPACITY, "scan too deep");
+    const long long per = (long long)kBlock * kScanItems;
+    const long long nb = (n + per - 1) / per;
+    if (int r = sc.sums[level].ensure(nb + 1)) return r;
+    if (int r = sc.offs[level].ensure(nb + 1)) return r;
+    long long* sums = sc.sums[level].p;
+    long long* offs = sc.offs[level].p;
+    hipLaunchKernelGGL(k_scan_local<TIn>, dim3(unsigned(nb)), dim3(kBlock), 0, st, in, out, sums, n);
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, st, out, n, sums, 0);  // total = sums[0]
+        return 0;
+    }
+    if (int r = scanExclusive<long long>(sums, offs, nb, sc, st, level + 1)) return r;
+    hipLaunchKernelGGL(k_scan_add, dim3(unsigned(nb)), dim3(kBlock), 0, st, out, offs, n);
+    hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, st, out, n, offs, nb);
+    return 0;
+}
+
+// ------------------------------------------------------------------ hits
+__global__ void k_hit_prep(int nh, const float* x, const float* y, const float* z, const unsigned* detId,
+                           const long long* evOff, int E, const uint32_t* sortedDet, int NM, const ModDev* mods,
+                           float* rt, float* r3, float* phi, int* slot, float4* eHi, float4* eLo, int* slotCount,
+                           int* unknown) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nh) return;
+    // event of the hit
+    int lo = 0, hi = E;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (evOff[mid] <= i) lo = mid; else hi = mid;
+    }
+    const int e = lo;
+    // module of the hit: binary search of the sorted detIds
+    const uint32_t d = detId[i];
+    int a = 0, b = NM;
+    while (a < b) {
+        int mid = (a + b) >> 1;
+        if (sortedDet[mid] < d) a = mid + 1; else b = mid;
+    }
+    if (a >= NM || sortedDet[a] != d) {
+        atomicAdd(unknown, 1);
+        slot[i] = -1;
+        return;
+    }
+    const float xx = x[i], yy = y[i], zz = z[i];
+    rt[i] = rtOf(xx, yy);
+    r3[i] = r3Of(xx, yy, zz);
+    phi[i] = phiOf(xx, yy);
+    const int s = e * NM + a;
+    slot[i] = s;
+    const ModDev m = mods[a];
+    if (m.subdet == SUB_ENDCAP && m.type == TYPE_2S) {
+        HitView h{xx, yy, zz, 0, 0, 0};
+        HitView hh = edgeOf(h, m.centroidPhi, true);
+        HitView hl = edgeOf(h, m.centroidPhi, false);
+        eHi[i] = make_float4(hh.x, hh.y, hh.rt, hh.phi);
+        eLo[i] = make_float4(hl.x, hl.y, hl.rt, hl.phi);
+    }
+    atomicAdd(&slotCount[s], 1);
+}
+
+__global__ void k_hit_scatter(int nh, const int* slot, const long long* slotStart, int* slotFill, int* sortedHit) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nh) return;
+    const int s = slot[i];
+    if (s < 0) return;
+    const int pos = atomicAdd(&slotFill[s], 1);
+    sortedHit[slotStart[s] + pos] = i;
+}
+
+// Restores insertion order inside each slot (Module::addHit appends, Module.cc:128-137).
+__global__ void k_sort_segments(int nseg, const long long* start, int* vals) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const long long a = start[s], b = start[s + 1];
+    for (long long i = a + 1; i < b; ++i) {
+        int v = vals[i];
+        long long j = i - 1;
+        while (j >= a && vals[j] > v) {
+            vals[j + 1] = vals[j];
+            --j;
+        }
+        vals[j + 1] = v;
+    }
+}
+
+__device__ __forceinline__ HitView loadHit(int h, const float* x, const float* y, const float* z, const float* rt,
+                                           const float* r3, const float* phi) {
+    return HitView{x[h], y[h], z[h], rt[h], r3[h], phi[h]};
+}
+
+// ------------------------------------------------------------------ MD
+// Candidate count per lower slot: lower hits x upper hits (Event.cc:257-305).
+__global__ void k_md_cap(int NS, int NM, const ModDev* mods, const long long* slotHitStart, int* mdCap) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= NS) return;
+    const int m = s % NM;
+    const ModDev md = mods[m];
+    int cap = 0;
+    if (md.isLower && md.partner >= 0) {
+        const int ps = s - m + md.partner;
+        const long long nLo = slotHitStart[s + 1] - slotHitStart[s];
+        const long long nUp = slotHitStart[ps + 1] - slotHitStart[ps];
+        cap = int(nLo * nUp);
+    }
+    mdCap[s] = cap;
+}
+
+struct HitsDev {
+    const float *x, *y, *z, *rt, *r3, *phi;
+    const int* idx;
+    const float4 *eHi, *eLo;
+};
+
+__global__ void __launch_bounds__(kBlock) k_md(int NS, int NM, const ModDev* mods, HitsDev H, const long long* slotHitStart,
+                                               const int* sortedHit, const int* mdCap, const long long* mdCapOff,
+                                               int2* slabHits, float4* slabF, int* mdCount) {
+    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= NS) return;
+    const int cap = mdCap[item];
+    if (cap == 0) {
+        if (lane == 0) mdCount[item] = 0;
+        return;
+    }
+    const int m = item % NM;
+    const ModDev md = mods[m];
+    const int ps = item - m + md.partner;
+    const long long loStart = slotHitStart[item];
+    const long long upStart = slotHitStart[ps];
+    const int nUp = int(slotHitStart[ps + 1] - upStart);
+    const long long off = mdCapOff[item];
+    int base = 0;
+    for (int k0 = 0; k0 < cap; k0 += kWave) {
+        const int k = k0 + lane;
+        bool pass = false;
+        MDOut o;
+        int lo = 0, up = 0;
+        if (k < cap) {
+            const int i = k / nUp, j = k - (k / nUp) * nUp;
+            lo = sortedHit[loStart + i];
+            up = sortedHit[upStart + j];
+            const HitView hl = loadHit(lo, H.x, H.y, H.z, H.rt, H.r3, H.phi);
+            const HitView hu = loadHit(up, H.x, H.y, H.z, H.rt, H.r3, H.phi);
+            pass = mdAlgo(hl, hu, md, o);
+        }
+        const unsigned long long mask = __ballot(pass);
+        if (pass) {
+            const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+            slabHits[off + pos] = make_int2(lo, up);
+            slabF[off + pos] = make_float4(o.dz, o.shiftedDz, o.dphi, o.dphichange);
+        }
+        base += __popcll(mask);
+    }
+    if (lane == 0) mdCount[item] = base;
+}
+
+struct MDDev {
+    int *lo, *up, *anchor, *slot;
+    float4* a;   // anchor x, y, z, rt
+    float4* b;   // anchor r3, phi; dz, shiftedDz
+    float2* c;   // dphi, dphichange
+};
+
+// Dense repack; the anchor is the pixel hit of a PS module, else the lower hit
+// (MiniDoublet.cc:60-81).
+__global__ void k_md_repack(int NS, int NM, const ModDev* mods, HitsDev H, const long long* mdCapOff,
+                            const int* mdCount, const long long* mdStart, const int2* slabHits, const float4* slabF,
+                            MDDev D) {
+    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= NS) return;
+    const int n = mdCount[item];
+    if (n == 0) return;
+    const ModDev md = mods[item % NM];
+    const bool upAnchor = md.type == TYPE_PS && md.layerType != LT_PIXEL;
+    const long long src = mdCapOff[item], dst = mdStart[item];
+    for (int k = lane; k < n; k += kWave) {
+        const int2 h = slabHits[src + k];
+        const float4 f = slabF[src + k];
+        const int an = upAnchor ? h.y : h.x;
+        const long long o = dst + k;
+        D.lo[o] = h.x;
+        D.up[o] = h.y;
+        D.anchor[o] = an;
+        D.slot[o] = item;
+        D.a[o] = make_float4(H.x[an], H.y[an], H.z[an], H.rt[an]);
+        D.b[o] = make_float4(H.r3[an], H.phi[an], f.x, f.y);
+        D.c[o] = make_float2(f.z, f.w);
+    }
+}
+
+__device__ __forceinline__ MDView loadMD(int i, const MDDev& D) {
+    const float4 a = D.a[i], b = D.b[i];
+    const float2 c = D.c[i];
+    MDView v;
+    v.a = HitView{a.x, a.y, a.z, a.w, b.x, b.y};
+    v.dz = b.z;
+    v.shiftedDz = b.w;
+    v.dphi = c.x;
+    v.dphichange = c.y;
+    v.mod = 0;
+    return v;
+}
+
+// ------------------------------------------------------------------ SG
+// SG candidates of one inner MD: MDs of every connected lower module (Event.cc:457-522).
+__global__ void k_sg_cap(int nMD, int NM, const int* mdSlot, const int* connStart, const int* connList,
+                         const int* mdCount, int* sgCap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nMD) return;
+    const int s = mdSlot[i];
+    const int m = s % NM, eb = s - m;
+    int c = 0;
+    for (int k = connStart[m]; k < connStart[m + 1]; ++k) c += mdCount[eb + connList[k]];
+    sgCap[i] = c;
+}
+
+struct SGF {
+    float v[6];  // dphi, dphimin, dphimax, dphichange, dphichangemin, dphichangemax
+};
+
+// Flattened walk over the connected modules of one work item: returns the
+// connection entry holding candidate k and the index inside it.  pre holds the
+// wave-wide inclusive prefix of per-entry counts (entry j in lane j).  Must be
+// called by all 64 lanes (the shuffles read every lane).
+__device__ __forceinline__ int findEntry(int k, int pre, int nEntries, int& inner) {
+    int e = 0;
+    int prev = 0;
+    for (int j = 0; j < nEntries; ++j) {
+        const int pj = __shfl(pre, j);
+        if (k >= pj) { e = j + 1; prev = pj; }
+    }
+    inner = k - prev;
+    return e;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sg(int nMD, int NM, const ModDev* mods, HitsDev H, MDDev D,
+                                               const int* connStart, const int* connList, const long long* mdStart,
+                                               const int* mdCount, const int* sgCap, const long long* sgCapOff,
+                                               int* slabOut, SGF* slabF, int* sgCount) {
+    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= nMD) return;
+    const int cap = sgCap[item];
+    if (cap == 0) {
+        if (lane == 0) sgCount[item] = 0;
+        return;
+    }
+    const int sIn = D.slot[item];
+    const int mIn = sIn % NM, eb = sIn - mIn;
+    const ModDev mi = mods[mIn];
+    const MDView in = loadMD(item, D);
+    const int c0 = connStart[mIn], nc = connStart[mIn + 1] - c0;
+    const long long off = sgCapOff[item];
+    int base = 0;
+    // connection entries are processed in chunks of 64 (one per lane)
+    for (int cb = 0; cb < nc; cb += kWave) {
+        const int ne = min(kWave, nc - cb);
+        int cnt = 0;
+        if (lane < ne) cnt = mdCount[eb + connList[c0 + cb + lane]];
+        // inclusive prefix over lanes
+        int pre = cnt;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int t = __shfl_up(pre, o);
+            if (lane >= o) pre += t;
+        }
+        const int total = __shfl(pre, kWave - 1);
+        for (int k0 = 0; k0 < total; k0 += kWave) {
+            const int k = k0 + lane;
+            bool pass = false;
+            SGOut o;
+            int mdOut = 0;
+            int inner;
+            const int e = findEntry(k, pre, ne, inner);
+            if (k < total) {
+                const int modOut = connList[c0 + cb + e];
+                mdOut = int(mdStart[eb + modOut]) + inner;
+                const ModDev mO = mods[modOut];
+                const MDView out = loadMD(mdOut, D);
+                HitView hiE = out.a, loE = out.a;
+                if (mO.subdet == SUB_ENDCAP && mO.type == TYPE_2S) {
+                    const int an = D.anchor[mdOut];
+                    const float4 a = H.eHi[an], b = H.eLo[an];
+                    hiE.phi = a.w;
+                    loE.phi = b.w;
+                }
+                pass = sgAlgo(in, out, mi, mO, true, hiE, loE, o);
+            }
+            const unsigned long long mask = __ballot(pass);
+            if (pass) {
+                const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+                slabOut[off + pos] = mdOut;
+                SGF f;
+                f.v[0] = o.dphi; f.v[1] = o.dphimin; f.v[2] = o.dphimax;
+                f.v[3] = o.dphichange; f.v[4] = o.dphichangemin; f.v[5] = o.dphichangemax;
+                slabF[off + pos] = f;
+            }
+            base += __popcll(mask);
+        }
+    }
+    if (lane == 0) sgCount[item] = base;
+}
+
+struct SGDev {
+    int *in, *out;
+    SGF* f;
+};
+
+__global__ void k_sg_repack(int nMD, const long long* sgCapOff, const int* sgCount, const long long* sgStart,
+                            const int* slabOut, const SGF* slabF, SGDev S) {
+    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= nMD) return;
+    const int n = sgCount[item];
+    const long long src = sgCapOff[item], dst = sgStart[item];
+    for (int k = lane; k < n; k += kWave) {
+        S.in[dst + k] = item;
+        S.out[dst + k] = slabOut[src + k];
+        S.f[dst + k] = slabF[src + k];
+    }
+}
+
+// Per-slot ranges of objects ordered by an inner MD: first = start[mdStart[slot]].
+__global__ void k_slot_range(int NS, const long long* mdStart, const long long* objStartByMd, long long* slotObjStart) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > NS) return;
+    slotObjStart[s] = objStartByMd[mdStart[s]];
+}
+
+// SGs ending at each MD (reverse index, used by the joins)
+__global__ void k_count_by(int n, const int* key, int* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    atomicAdd(&count[key[i]], 1);
+}
+__global__ void k_scatter_by(int n, const int* key, const long long* start, int* fill, int* list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = key[i];
+    const int pos = atomicAdd(&fill[k], 1);
+    list[start[k] + pos] = i;
+}
+
+// ------------------------------------------------------------------ TP
+// Triplet = two SGs sharing the middle MD (Triplet.cc:50-122, Segment.cc:861-866).
+// The inner SG s ends at MD m; the outer SGs stored at m's module with inner MD m
+// are exactly the dense range of SGs whose inner MD is m.
+__global__ void k_tp_count(int nSG, const int* sgOut, const long long* sgStart, int* tpCount) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSG) return;
+    const int m = sgOut[s];
+    tpCount[s] = int(sgStart[m + 1] - sgStart[m]);
+}
+__global__ void k_tp_write(int nSG, const int* sgOut, const long long* sgStart, const long long* tpStart, int* tpIn,
+                           int* tpOutArr) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSG) return;
+    const int m = sgOut[s];
+    const long long a = sgStart[m], n = sgStart[m + 1] - a, d = tpStart[s];
+    for (long long k = 0; k < n; ++k) {
+        tpIn[d + k] = s;
+        tpOutArr[d + k] = int(a + k);
+    }
+}
+
+// ------------------------------------------------------------------ TL
+// TL candidates of one inner SG: SGs stored at every module connected to the
+// inner SG's outer-MD module (Event.cc:627-693).
+__global__ void k_tl_cap(int nSG, int NM, const int* sgOut, const int* mdSlot, const int* connStart,
+                         const int* connList, const long long* slotSgStart, long long* tlCap) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSG) return;
+    const int b = mdSlot[sgOut[s]];
+    const int m = b % NM, eb = b - m;
+    long long c = 0;
+    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
+        const int t = eb + connList[k];
+        c += slotSgStart[t + 1] - slotSgStart[t];
+    }
+    tlCap[s] = c;
+}
+
+__device__ __forceinline__ SGView loadSG(int s, const SGDev& S, const MDDev& D, int NM) {
+    SGView v;
+    const int a = S.in[s], b = S.out[s];
+    v.in = loadMD(a, D);
+    v.out = loadMD(b, D);
+    v.in.mod = D.slot[a] % NM;
+    v.out.mod = D.slot[b] % NM;
+    const SGF f = S.f[s];
+    v.dphi = f.v[0]; v.dphimin = f.v[1]; v.dphimax = f.v[2];
+    v.dphichange = f.v[3]; v.dphichangemin = f.v[4]; v.dphichangemax = f.v[5];
+    return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_tl(int nSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
+                                               const int* connStart, const int* connList, const long long* slotSgStart,
+                                               const long long* tlCap, const long long* tlCapOff, int* slabOut,
+                                               int* tlCount) {
+    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= nSG) return;
+    const long long cap = tlCap[item];
+    if (cap == 0) {
+        if (lane == 0) tlCount[item] = 0;
+        return;
+    }
+    const SGView sIn = loadSG(item, S, D, NM);
+    const int b = D.slot[S.out[item]];
+    const int m = b % NM, eb = b - m;
+    const int c0 = connStart[m], nc = connStart[m + 1] - c0;
+    const long long off = tlCapOff[item];
+    int base = 0;
+    for (int cb = 0; cb < nc; cb += kWave) {
+        const int ne = min(kWave, nc - cb);
+        int cnt = 0;
+        if (lane < ne) {
+            const int t = eb + connList[c0 + cb + lane];
+            cnt = int(slotSgStart[t + 1] - slotSgStart[t]);
+        }
+        int pre = cnt;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int t = __shfl_up(pre, o);
+            if (lane >= o) pre += t;
+        }
+        const int total = __shfl(pre, kWave - 1);
+        for (int k0 = 0; k0 < total; k0 += kWave) {
+            const int k = k0 + lane;
+            bool pass = false;
+            int so = 0;
+            int inner;
+            const int e = findEntry(k, pre, ne, inner);
+            if (k < total) {
+                const int t = eb + connList[c0 + cb + e];
+                so = int(slotSgStart[t]) + inner;
+                const SGView sOut = loadSG(so, S, D, NM);
+                HitView hiE = sOut.out.a, loE = sOut.out.a;
+                const ModDev mOU = mods[sOut.out.mod];
+                if (mOU.subdet == SUB_ENDCAP && mOU.type == TYPE_2S) {
+                    const int an = D.anchor[S.out[so]];
+                    const float4 a = H.eHi[an], c = H.eLo[an];
+                    hiE.rt = a.z;
+                    loE.rt = c.z;
+                }
+                pass = tlAlgo(sIn, sOut, mods, hiE, loE);
+            }
+            const unsigned long long mask = __ballot(pass);
+            if (pass) {
+                const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+                slabOut[off + pos] = so;
+            }
+            base += __popcll(mask);
+        }
+    }
+    if (lane == 0) tlCount[item] = base;
+}
+
+__global__ void k_tl_repack(int nSG, const long long* tlCapOff, const int* tlCount, const long long* tlStart,
+                            const int* slabOut, int* tlIn, int* tlOutArr) {
+    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= nSG) return;
+    const int n = tlCount[item];
+    const long long src = tlCapOff[item], dst = tlStart[item];
+    for (int k = lane; k < n; k += kWave) {
+        tlIn[dst + k] = item;
+        tlOutArr[dst + k] = slabOut[src + k];
+    }
+}
+
+// ------------------------------------------------------------------ TC
+// TC from tracklets (Event.cc:999-1062, TrackCandidate.cc:143-226): the dR cut
+// never rejects, so the pair passes iff the outer SG of the inner TL is the
+// inner SG of the outer TL; those TLs are the dense range of TLs of that SG.
+__global__ void k_tcl_count(int nTL, const int* tlOutArr, const long long* tlStart, int* cnt) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nTL) return;
+    const int s = tlOutArr[l];
+    cnt[l] = int(tlStart[s + 1] - tlStart[s]);
+}
+__global__ void k_tcl_write(int nTL, const int* tlOutArr, const long long* tlStart, const long long* start, int2* tc) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nTL) return;
+    const int s = tlOutArr[l];
+    const long long a = tlStart[s], n = tlStart[s + 1] - a, d = start[l];
+    for (long long k = 0; k < n; ++k) tc[d + k] = make_int2(l, int(a + k));
+}
+
+// TC from triplets (Event.cc:889-982).  The candidate (inner TP t ending at MD C,
+// outer TP t' = (s3, s4) starting at MD D) needs the bridge SG(C, D) to pass the
+// SG algorithm and the tracklet (bridge, s4) to pass the TL algorithm.  Both are
+// pure functions of their inputs that the SG and TL stages already evaluated on
+// exactly these candidates (D's module is connected to C's because s3..s4 and the
+// bridge are module-map objects), so: for every tracklet (b, s4) with b = C->D,
+// the TCs are {TPs ending at C} x {TP(s3, s4)} when the SG s3 = D->s4.inner exists.
+__global__ void k_tcp_count(int nTL, const int* tlIn, const int* tlOutArr, const int* sgIn, const int* sgOut,
+                            const long long* sgStart, const int* tpEndCount, int* s3Of, int* cnt) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nTL) return;
+    const int b = tlIn[l], s4 = tlOutArr[l];
+    const int mdC = sgIn[b], mdD = sgOut[b], mdE = sgIn[s4];
+    int s3 = -1;
+    for (long long k = sgStart[mdD]; k < sgStart[mdD + 1]; ++k)
+        if (sgOut[k] == mdE) { s3 = int(k); break; }
+    s3Of[l] = s3;
+    cnt[l] = s3 >= 0 ? tpEndCount[mdC] : 0;
+}
+// # TPs ending at MD m = sum over SGs s2 ending at m of # SGs ending at s2's inner MD
+__global__ void k_tp_end_count(int nMD, const long long* sgEndStart, const int* sgEndList, const int* sgIn,
+                               int* tpEndCount) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nMD) return;
+    int c = 0;
+    for (long long k = sgEndStart[m]; k < sgEndStart[m + 1]; ++k) {
+        const int s2 = sgEndList[k];
+        const int mx = sgIn[s2];
+        c += int(sgEndStart[mx + 1] - sgEndStart[mx]);
+    }
+    tpEndCount[m] = c;
+}
+__global__ void k_tcp_write(int nTL, const int* tlIn, const int* tlOutArr, const int* sgIn, const long long* sgStart,
+                            const long long* sgEndStart, const int* sgEndList, const long long* tpStart,
+                            const int* s3Of, const long long* start, int2* tc) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nTL) return;
+    const int s3 = s3Of[l];
+    if (s3 < 0) return;
+    const int b = tlIn[l], s4 = tlOutArr[l];
+    const int mdC = sgIn[b];
+    const long long tOuter = tpStart[s3] + (s4 - sgStart[sgIn[s4]]);
+    long long d = start[l];
+    for (long long k = sgEndStart[mdC]; k < sgEndStart[mdC + 1]; ++k) {
+        const int s2 = sgEndList[k];
+        const int mx = sgIn[s2];
+        for (long long q = sgEndStart[mx]; q < sgEndStart[mx + 1]; ++q) {
+            const int s1 = sgEndList[q];
+            const long long tInner = tpStart[s1] + (s2 - sgStart[mx]);
+            tc[d++] = make_int2(int(tInner), int(tOuter));
+        }
+    }
+}
+
+// ------------------------------------------------------------------ counters
+// Layer bin of a module: barrel layers 0..5, endcap layers 6..10.
+__device__ __forceinline__ int layerBin(const ModDev& m) {
+    return m.subdet == SUB_BARREL ? m.layer - 1 : 6 + m.layer - 1;
+}
+
+constexpr int kNCounters = 10;
+constexpr int kNBins = 11;
+
+// counters[(e * 10 + which) * 11 + bin] += value of each item (1 when vals is
+// null); item -> slot via slotOf, or the item is a slot itself when slotOf is
+// null.  Bins are pre-aggregated in LDS, one global atomic per touched bin.
+constexpr int kMaxLdsBins = 4096;
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_accumulate(long long n, const T* vals, const int* slotOf, int NM, int E,
+                                                       const ModDev* mods, int which, unsigned long long* counters) {
+    __shared__ unsigned long long bins[kMaxLdsBins];
+    const int nb = E * kNBins;
+    const bool lds = nb <= kMaxLdsBins;
+    if (lds)
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0;
+    __syncthreads();
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const long long v = vals ? (long long)vals[i] : 1;
+        if (v == 0) continue;
+        const int s = slotOf ? slotOf[i] : int(i);
+        const int e = s / NM;
+        const int bin = layerBin(mods[s % NM]);
+        if (lds) atomicAdd(&bins[e * kNBins + bin], (unsigned long long)v);
+        else atomicAdd(&counters[(e * kNCounters + which) * kNBins + bin], (unsigned long long)v);
+    }
+    __syncthreads();
+    if (lds)
+        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+            if (bins[b]) atomicAdd(&counters[((b / kNBins) * kNCounters + which) * kNBins + (b % kNBins)], bins[b]);
+}
+
+// slot of the inner TP's first MD for every TC made from triplets
+__global__ void k_tcp_slot(long long n, const int2* tcp, const int* tpIn, const int* sgIn, const int* mdSlot, int* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = mdSlot[sgIn[tpIn[tcp[i].x]]];
+}
+
+// TP candidates of an inner SG at module A: SGs stored at every module connected to A (Event.cc:539-597)
+__global__ void k_tp_cand(int nSG, int NM, const int* sgIn, const int* mdSlot, const int* connStart,
+                          const int* connList, const long long* slotSgStart, long long* cand) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSG) return;
+    const int a = mdSlot[sgIn[s]];
+    const int m = a % NM, eb = a - m;
+    long long c = 0;
+    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
+        const int t = eb + connList[k];
+        c += slotSgStart[t + 1] - slotSgStart[t];
+    }
+    cand[s] = c;
+}
+// TC candidates: TC(TP) per inner TP (TPs at modules connected to its outermost MD
+// module, Event.cc:889-982) and TC(TL) per inner TL (TLs at modules connected to its
+// second MD module, Event.cc:999-1062).
+__global__ void k_tc_cand_tp(long long nTP, int NM, const int* tpIn, const int* tpOutArr, const int* sgIn,
+                             const int* sgOut, const int* mdSlot, const int* connStart, const int* connList,
+                             const long long* slotTpStart, long long* cand, int* slotOfItem) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nTP) return;
+    const int c = mdSlot[sgOut[tpOutArr[t]]];
+    const int m = c % NM, eb = c - m;
+    long long n = 0;
+    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
+        const int d = eb + connList[k];
+        n += slotTpStart[d + 1] - slotTpStart[d];
+    }
+    cand[t] = n;
+    slotOfItem[t] = mdSlot[sgIn[tpIn[t]]];
+}
+__global__ void k_tc_cand_tl(long long nTL, int NM, const int* tlIn, const int* sgIn, const int* sgOut,
+                             const int* mdSlot, const int* connStart, const int* connList,
+                             const long long* slotTlStart, long long* cand, int* slotOfItem) {
+    const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nTL) return;
+    const int b = mdSlot[sgOut[tlIn[l]]];
+    const int m = b % NM, eb = b - m;
+    long long n = 0;
+    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
+        const int d = eb + connList[k];
+        n += slotTlStart[d + 1] - slotTlStart[d];
+    }
+    cand[l] = n;
+    slotOfItem[l] = mdSlot[sgIn[tlIn[l]]];
+}
+__global__ void k_gather_slot(long long n, const int* idxA, const int* idxB, const int* mdSlot, int* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = mdSlot[idxB ? idxB[idxA[i]] : idxA[i]];
+}
+
+// Per-slot object starts for objects ordered by inner SG: slotStart[s] = start[slotSgStart[s]]
+__global__ void k_slot_range_sg(int NS, const long long* slotSgStart, const long long* objStartBySg,
+                                long long* slotObjStart) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > NS) return;
+    slotObjStart[s] = objStartBySg[slotSgStart[s]];
+}
+
+// ------------------------------------------------------------------ export
+__global__ void k_export_md(long long a, long long n, const int* lo, const int* up, const int* idx, int* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[2 * i] = idx[lo[a + i]];
+    out[2 * i + 1] = idx[up[a + i]];
+}
+__device__ __forceinline__ void putSG(int s, const int* sgIn, const int* sgOut, const int* lo, const int* up,
+                                      const int* idx, int* o) {
+    const int a = sgIn[s], b = sgOut[s];
+    o[0] = idx[lo[a]]; o[1] = idx[up[a]]; o[2] = idx[lo[b]]; o[3] = idx[up[b]];
+}
+// kind: 1 SG, 2 TP/TL pair of SGs, 3 TC of TP pairs, 4 TC of TL pairs
+__global__ void k_export(long long a, long long n, int kind, const int* pA, const int* pB, const int2* pC,
+                         const int* tpIn, const int* tpOutArr, const int* tlIn, const int* tlOutArr, const int* sgIn,
+                         const int* sgOut, const int* lo, const int* up, const int* idx, int* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long o = a + i;
+    if (kind == 1) {
+        putSG(int(o), sgIn, sgOut, lo, up, idx, out + 4 * i);
+    } else if (kind == 2) {
+        putSG(pA[o], sgIn, sgOut, lo, up, idx, out + 8 * i);
+        putSG(pB[o], sgIn, sgOut, lo, up, idx, out + 8 * i + 4);
+    } else {
+        const int2 c = pC[o];
+        const int* in = kind == 3 ? tpIn : tlIn;
+        const int* ou = kind == 3 ? tpOutArr : tlOutArr;
+        putSG(in[c.x], sgIn, sgOut, lo, up, idx, out + 16 * i);
+        putSG(ou[c.x], sgIn, sgOut, lo, up, idx, out + 16 * i + 4);
+        putSG(in[c.y], sgIn, sgOut, lo, up, idx, out + 16 * i + 8);
+        putSG(ou[c.y], sgIn, sgOut, lo, up, idx, out + 16 * i + 12);
+    }
+}
+
+// ------------------------------------------------------------------ host geometry
+ModDev decodeModule(uint32_t d) {
+    // Module.cc:159-358
+    ModDev m{};
+    m.detId = d;
+    m.partner = -1;
+    m.subdet = (d >> 25) & 7;
+    if (m.subdet == SUB_ENDCAP) { m.side = (d >> 23) & 3; m.layer = (d >> 18) & 7; m.ring = (d >> 12) & 15; m.rod = 0; }
+    else if (m.subdet == SUB_BARREL) { m.side = (d >> 18) & 3; m.layer = (d >> 20) & 7; m.rod = (d >> 10) & 127; m.ring = 0; }
+    const int mod = (d >> 2) & 127;
+    bool inv = false;
+    if (m.subdet == SUB_ENDCAP) {
+        if (m.side == SIDE_NEGZ) inv = mod % 2 == 1;
+        else if (m.side == SIDE_POSZ) inv = mod % 2 == 0;
+    } else if (m.subdet == SUB_BARREL) {
+        if (m.side == SIDE_CENTER) inv = m.layer <= 3 ? mod % 2 == 1 : (m.layer >= 4 ? mod % 2 == 0 : false);
+        else if (m.side == SIDE_NEGZ || m.side == SIDE_POSZ) inv = m.layer <= 2 ? mod % 2 == 1 : (m.layer == 3 ? mod % 2 == 0 : false);
+    }
+    m.isLower = inv ? !(d & 1) : (d & 1);
+    if (m.subdet == SUB_BARREL) m.type = m.layer <= 3 ? TYPE_PS : TYPE_2S;
+    else if (m.layer <= 2) m.type = m.ring <= 10 ? TYPE_PS : TYPE_2S;
+    else m.type = m.ring <= 7 ? TYPE_PS : TYPE_2S;
+    if (m.type == TYPE_2S) m.layerType = LT_STRIP;
+    else if (inv) m.layerType = m.isLower ? LT_STRIP : LT_PIXEL;
+    else m.layerType = m.isLower ? LT_PIXEL : LT_STRIP;
+    // MiniDoublet::isTighterTiltedModules / moduleGapSize (MiniDoublet.cc:1242-1332)
+    bool tighter = false;
+    if (m.subdet == SUB_BARREL && m.side != SIDE_CENTER) {
+        tighter = m.layer == 3 || (m.side == SIDE_NEGZ && m.layer == 2 && m.rod > 5) ||
+                  (m.side == SIDE_POSZ && m.layer == 2 && m.rod < 8) ||
+                  (m.side == SIDE_NEGZ && m.layer == 1 && m.rod > 9) || (m.side == SIDE_POSZ && m.layer == 1 && m.rod < 4);
+    }
+    const float flat[6] = {0.26f, 0.16f, 0.16f, 0.18f, 0.18f, 0.18f};
+    const int iL = m.layer - 1;
+    if (m.subdet == SUB_BARREL && m.side == SIDE_CENTER) m.gap = (iL >= 0 && iL < 6) ? flat[iL] : 0.f;
+    else if (tighter) m.gap = 0.26f;
+    else if (m.subdet == SUB_ENDCAP) {
+        const int j = m.ring - 1;
+        if (iL == 0 || iL == 1) m.gap = j < 10 ? 0.4f : 0.18f;
+        else if (iL == 2 || iL == 3) m.gap = j < 8 ? 0.4f : 0.18f;
+        else m.gap = j < 9 ? 0.4f : 0.18f;
+    } else m.gap = 0.4f;
+    return m;
+}
+
+uint32_t partnerOf(uint32_t d) {
+    const ModDev m = decodeModule(d);
+    const int mod = (d >> 2) & 127;
+    bool inv = false;
+    if (m.subdet == SUB_ENDCAP) inv = m.side == SIDE_NEGZ ? mod % 2 == 1 : (m.side == SIDE_POSZ ? mod % 2 == 0 : false);
+    else if (m.subdet == SUB_BARREL) {
+        if (m.side == SIDE_CENTER) inv = m.layer <= 3 ? mod % 2 == 1 : mod % 2 == 0;
+        else inv = m.layer <= 2 ? mod % 2 == 1 : (m.layer == 3 ? mod % 2 == 0 : false);
+    }
+    if (m.isLower) return inv ? d - 1 : d + 1;
+    return inv ? d + 1 : d - 1;
+}
+
+}  // namespace
+
+struct sdl_geom {
+    int device = 0;
+    int NM = 0;
+    std::vector<uint32_t> detIds;
+    std::vector<ModDev> mods;
+    std::vector<int> connStart, connList;
+    uint32_t* dDet = nullptr;
+    ModDev* dMods = nullptr;
+    int *dConnStart = nullptr, *dConnList = nullptr;
+};
+
+struct sdl_event {
+    sdl_geom* g = nullptr;
+    hipStream_t st = nullptr;
+    // host staging (add_hits)
+    std::vector<float> hx, hy, hz;
+    std::vector<int> hidx;
+    std::vector<unsigned> hdet;
+    std::vector<long long> hoff{0};
+    bool hostStaged = true;
+    // batch description
+    int E = 0;
+    long long NH = 0;
+    std::vector<long long> evOff;
+    const float *px = nullptr, *py = nullptr, *pz = nullptr;
+    const int* pidx = nullptr;
+    const unsigned* pdet = nullptr;
+    int stage = -1;   // last completed stage
+    bool hitsReady = false;
+    long long nMD = 0, nSG = 0, nTP = 0, nTL = 0, nTCP = 0, nTCL = 0;
+    // device buffers
+    DBuf<float> bx, by, bz;
+    DBuf<int> bidx;
+    DBuf<unsigned> bdet;
+    DBuf<long long> dEvOff;
+    DBuf<float> rt, r3, phi;
+    DBuf<int> hslot;
+    DBuf<float4> eHi, eLo;
+    DBuf<int> slotHitCount, slotFill, sortedHit, unknown;
+    DBuf<long long> slotHitStart;
+    DBuf<int> mdCap, mdCount;
+    DBuf<long long> mdCapOff, mdStart;
+    DBuf<int2> mdSlabHits;
+    DBuf<float4> mdSlabF;
+    DBuf<int> mdLo, mdUp, mdAnchor, mdSlot;
+    DBuf<float4> mdA, mdB;
+    DBuf<float2> mdC;
+    DBuf<int> sgCap, sgCount, sgSlabOut, sgIn, sgOut;
+    DBuf<long long> sgCapOff, sgStart, slotSgStart;
+    DBuf<SGF> sgSlabF, sgF;
+    DBuf<int> sgEndCount, sgEndFill, sgEndList;
+    DBuf<long long> sgEndStart;
+    DBuf<int> tpCount, tpIn, tpOut, tpEndCount;
+    DBuf<long long> tpStart, slotTpStart;
+    DBuf<long long> tlCap, tlCapOff, tlStart, slotTlStart;
+    DBuf<int> tlCount, tlSlabOut, tlIn, tlOut;
+    DBuf<int> tcCount, s3Of;
+    DBuf<long long> tcpStart, tclStart;
+    DBuf<int2> tcp, tcl;
+    DBuf<long long> cand;
+    DBuf<int> itemSlot;
+    DBuf<unsigned long long> counters;
+    DBuf<long long> tmp64;
+    DBuf<int> exportBuf;
+    ScanScratch scan;
+    std::vector<unsigned long long> hostCounters;
+    // optional per-kernel timing of the combinatorial kernels (prep, MD, SG, TL)
+    bool timing = false;
+    hipEvent_t tev[8][2] = {};
+    double kernelMs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+enum { T_PREP = 0, T_MD = 1, T_SG = 2, T_TL = 3, T_NTIMERS = 4 };
+
+void timerBegin(sdl_event* ev, int k) {
+    if (!ev->timing) return;
+    if (!ev->tev[k][0]) {
+        (void)hipEventCreate(&ev->tev[k][0]);
+        (void)hipEventCreate(&ev->tev[k][1]);
+    }
+    (void)hipEventRecord(ev->tev[k][0], ev->st);
+}
+void timerEnd(sdl_event* ev, int k) {
+    if (!ev->timing) return;
+    (void)hipEventRecord(ev->tev[k][1], ev->st);
+}
+void timerCollect(sdl_event* ev, int k) {
+    if (!ev->timing || !ev->tev[k][0]) return;
+    float ms = 0.f;
+    if (hipEventSynchronize(ev->tev[k][1]) == hipSuccess && hipEventElapsedTime(&ms, ev->tev[k][0], ev->tev[k][1]) == hipSuccess)
+        ev->kernelMs[k] = ms;
+}
+
+long long readLL(const long long* d, hipStream_t st) {
+    long long v = 0;
+    if (hipMemcpyAsync(&v, d, sizeof(long long), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+    if (hipStreamSynchronize(st) != hipSuccess) return -1;
+    return v;
+}
+
+unsigned gridFor(long long n, int per) { return unsigned((n + per - 1) / per); }
+
+int loadGeomFiles(const char* tilted, const char* endcap, const char* map, sdl_geom& G) {
+    std::unordered_map<uint32_t, float> drdz, slope, ecSlope, ecPhi;
+    std::vector<std::pair<uint32_t, std::vector<uint32_t>>> conn;
+    std::ifstream ft(tilted), fe(endcap), fm(map);
+    if (!ft) return fail(SDL_ERR_IO, std::string("cannot open ") + tilted);
+    if (!fe) return fail(SDL_ERR_IO, std::string("cannot open ") + endcap);
+    if (!fm) return fail(SDL_ERR_IO, std::string("cannot open ") + map);
+    std::string line;
+    while (std::getline(ft, line)) {
+        std::stringstream ss(line);
+        unsigned d; float a, b;
+        if (ss >> d >> a >> b) { drdz[d] = a; slope[d] = b; }
+    }
+    while (std::getline(fe, line)) {
+        std::stringstream ss(line);
+        unsigned d; float avgr2, yl, sl, yh, sh, c7, c8, cz;
+        if (ss >> d >> avgr2 >> yl >> sl >> yh >> sh >> c7 >> c8 >> cz) { ecSlope[d] = sl; ecPhi[d] = c7; }
+    }
+    std::unordered_set<uint32_t> ids;
+    while (std::getline(fm, line)) {
+        std::stringstream ss(line);
+        unsigned d; int n;
+        if (!(ss >> d >> n)) continue;
+        std::vector<uint32_t> v;
+        std::unordered_set<uint32_t> seen;
+        for (int i = 0; i < n; ++i) {
+            unsigned c;
+            if (!(ss >> c)) break;
+            if (!seen.insert(c).second)
+                return fail(SDL_ERR_DUPLICATE_CONNECTION, "connection list of " + std::to_string(d) + " repeats " + std::to_string(c));
+            v.push_back(c);
+            ids.insert(c);
+        }
+        ids.insert(d);
+        conn.push_back({d, v});
+    }
+    for (auto& kv : drdz) ids.insert(kv.first);
+    for (auto& kv : ecSlope) ids.insert(kv.first);
+    std::vector<uint32_t> base(ids.begin(), ids.end());
+    for (uint32_t d : base) ids.insert(partnerOf(d));
+    G.detIds.assign(ids.begin(), ids.end());
+    std::sort(G.detIds.begin(), G.detIds.end());
+    G.NM = int(G.detIds.size());
+    std::unordered_map<uint32_t, int> index;
+    for (int i = 0; i < G.NM; ++i) index[G.detIds[i]] = i;
+    auto get = [](const std::unordered_map<uint32_t, float>& m, uint32_t k) {
+        auto it = m.find(k);
+        return it == m.end() ? 0.f : it->second;  // a missing key reads 0 (TiltedGeometry.cc:45,50)
+    };
+    G.mods.resize(G.NM);
+    for (int i = 0; i < G.NM; ++i) {
+        ModDev m = decodeModule(G.detIds[i]);
+        const uint32_t p = partnerOf(G.detIds[i]);
+        m.partner = index.count(p) ? index[p] : -1;
+        const uint32_t strip = m.layerType == LT_PIXEL ? p : G.detIds[i];
+        m.drdzStrip = get(drdz, strip);
+        m.slopeStrip = m.subdet == SUB_ENDCAP ? get(ecSlope, strip) : get(slope, strip);
+        m.centroidPhi = get(ecPhi, G.detIds[i]);
+        G.mods[i] = m;
+    }
+    std::vector<std::vector<int>> adj(G.NM);
+    for (auto& kv : conn) {
+        std::vector<int>& a = adj[index[kv.first]];
+        a.clear();  // a later line for the same detId replaces the earlier one (ModuleConnectionMap.cc:45)
+        for (uint32_t c : kv.second) a.push_back(index[c]);
+    }
+    G.connStart.assign(G.NM + 1, 0);
+    for (int i = 0; i < G.NM; ++i) G.connStart[i + 1] = G.connStart[i] + int(adj[i].size());
+    G.connList.clear();
+    for (int i = 0; i < G.NM; ++i) G.connList.insert(G.connList.end(), adj[i].begin(), adj[i].end());
+    return 0;
+}
+
+int ensureHits(sdl_event* ev) {
+    const long long NH = ev->NH;
+    const int NS = ev->E * ev->g->NM;
+    if (int r = ev->rt.ensure(NH)) return r;
+    if (int r = ev->r3.ensure(NH)) return r;
+    if (int r = ev->phi.ensure(NH)) return r;
+    if (int r = ev->hslot.ensure(NH)) return r;
+    if (int r = ev->eHi.ensure(NH)) return r;
+    if (int r = ev->eLo.ensure(NH)) return r;
+    if (int r = ev->sortedHit.ensure(NH)) return r;
+    if (int r = ev->slotHitCount.ensure(NS)) return r;
+    if (int r = ev->slotFill.ensure(NS)) return r;
+    if (int r = ev->slotHitStart.ensure(NS + 1)) return r;
+    if (int r = ev->unknown.ensure(1)) return r;
+    if (int r = ev->counters.ensure(size_t(ev->E) * kNCounters * kNBins)) return r;
+    return 0;
+}
+
+HitsDev hitsView(sdl_event* ev) {
+    return HitsDev{ev->px, ev->py, ev->pz, ev->rt.p, ev->r3.p, ev->phi.p, ev->pidx, ev->eHi.p, ev->eLo.p};
+}
+MDDev mdView(sdl_event* ev) {
+    return MDDev{ev->mdLo.p, ev->mdUp.p, ev->mdAnchor.p, ev->mdSlot.p, ev->mdA.p, ev->mdB.p, ev->mdC.p};
+}
+SGDev sgView(sdl_event* ev) { return SGDev{ev->sgIn.p, ev->sgOut.p, ev->sgF.p}; }
+
+template <class T>
+int accumulate(sdl_event* ev, long long n, const T* vals, const int* slotOf, int which) {
+    if (n <= 0) return 0;
+    const unsigned grid = std::min<unsigned>(gridFor(n, kBlock), 1024u);
+    hipLaunchKernelGGL(k_accumulate<T>, dim3(grid), dim3(kBlock), 0, ev->st, n, vals, slotOf, ev->g->NM, ev->E,
+                       ev->g->dMods, which, ev->counters.p);
+    return 0;
+}
+
+// Hit ingestion: module lookup, derived quantities, counting sort by slot.
+int runPrep(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const int NS = ev->E * G->NM;
+    if (int r = ensureHits(ev)) return r;
+    HIP_OK(hipMemsetAsync(ev->slotHitCount.p, 0, sizeof(int) * NS, st));
+    HIP_OK(hipMemsetAsync(ev->slotFill.p, 0, sizeof(int) * NS, st));
+    HIP_OK(hipMemsetAsync(ev->unknown.p, 0, sizeof(int), st));
+    HIP_OK(hipMemsetAsync(ev->counters.p, 0, sizeof(unsigned long long) * ev->E * kNCounters * kNBins, st));
+    if (int r = ev->dEvOff.ensure(ev->E + 1)) return r;
+    HIP_OK(hipMemcpyAsync(ev->dEvOff.p, ev->evOff.data(), sizeof(long long) * (ev->E + 1), hipMemcpyHostToDevice, st));
+    const int nh = int(ev->NH);
+    timerBegin(ev, T_PREP);
+    if (nh > 0)
+        hipLaunchKernelGGL(k_hit_prep, dim3(gridFor(nh, kBlock)), dim3(kBlock), 0, st, nh, ev->px, ev->py, ev->pz,
+                           ev->pdet, ev->dEvOff.p, ev->E, G->dDet, G->NM, G->dMods, ev->rt.p, ev->r3.p, ev->phi.p,
+                           ev->hslot.p, ev->eHi.p, ev->eLo.p, ev->slotHitCount.p, ev->unknown.p);
+    timerEnd(ev, T_PREP);
+    if (int r = scanExclusive<int>(ev->slotHitCount.p, ev->slotHitStart.p, NS, ev->scan, st)) return r;
+    if (nh > 0) {
+        hipLaunchKernelGGL(k_hit_scatter, dim3(gridFor(nh, kBlock)), dim3(kBlock), 0, st, nh, ev->hslot.p,
+                           ev->slotHitStart.p, ev->slotFill.p, ev->sortedHit.p);
+        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, NS, ev->slotHitStart.p,
+                           ev->sortedHit.p);
+    }
+    int unknown = 0;
+    HIP_OK(hipMemcpyAsync(&unknown, ev->unknown.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (unknown) return fail(SDL_ERR_UNKNOWN_DETID, std::to_string(unknown) + " hits have a detId unknown to the geometry");
+    ev->hitsReady = true;
+    return 0;
+}
+
+int runMD(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const int NS = ev->E * G->NM;
+    if (int r = ev->mdCap.ensure(NS)) return r;
+    if (int r = ev->mdCount.ensure(NS)) return r;
+    if (int r = ev->mdCapOff.ensure(NS + 1)) return r;
+    if (int r = ev->mdStart.ensure(NS + 1)) return r;
+    hipLaunchKernelGGL(k_md_cap, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods,
+                       ev->slotHitStart.p, ev->mdCap.p);
+    if (int r = scanExclusive<int>(ev->mdCap.p, ev->mdCapOff.p, NS, ev->scan, st)) return r;
+    const long long capTotal = readLL(ev->mdCapOff.p + NS, st);
+    if (capTotal < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->mdSlabHits.ensure(capTotal)) return r;
+    if (int r = ev->mdSlabF.ensure(capTotal)) return r;
+    HitsDev H = hitsView(ev);
+    timerBegin(ev, T_MD);
+    hipLaunchKernelGGL(k_md, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods, H,
+                       ev->slotHitStart.p, ev->sortedHit.p, ev->mdCap.p, ev->mdCapOff.p, ev->mdSlabHits.p,
+                       ev->mdSlabF.p, ev->mdCount.p);
+    timerEnd(ev, T_MD);
+    if (int r = scanExclusive<int>(ev->mdCount.p, ev->mdStart.p, NS, ev->scan, st)) return r;
+    ev->nMD = readLL(ev->mdStart.p + NS, st);
+    if (ev->nMD < 0) return fail(SDL_ERR_HIP, "readback failed");
+    const long long n = std::max(ev->nMD, 1LL);
+    if (int r = ev->mdLo.ensure(n)) return r;
+    if (int r = ev->mdUp.ensure(n)) return r;
+    if (int r = ev->mdAnchor.ensure(n)) return r;
+    if (int r = ev->mdSlot.ensure(n)) return r;
+    if (int r = ev->mdA.ensure(n)) return r;
+    if (int r = ev->mdB.ensure(n)) return r;
+    if (int r = ev->mdC.ensure(n)) return r;
+    hipLaunchKernelGGL(k_md_repack, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods, H,
+                       ev->mdCapOff.p, ev->mdCount.p, ev->mdStart.p, ev->mdSlabHits.p, ev->mdSlabF.p, mdView(ev));
+    accumulate<int>(ev, NS, ev->mdCap.p, nullptr, 0);
+    accumulate<int>(ev, NS, ev->mdCount.p, nullptr, 5);
+    return 0;
+}
+
+int runSG(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const int NS = ev->E * G->NM;
+    const long long nMD = ev->nMD;
+    const long long nm = std::max(nMD, 1LL);
+    if (int r = ev->sgCap.ensure(nm)) return r;
+    if (int r = ev->sgCount.ensure(nm)) return r;
+    if (int r = ev->sgCapOff.ensure(nm + 1)) return r;
+    if (int r = ev->sgStart.ensure(nm + 1)) return r;
+    if (int r = ev->slotSgStart.ensure(NS + 1)) return r;
+    if (nMD > 0)
+        hipLaunchKernelGGL(k_sg_cap, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, int(nMD), G->NM, ev->mdSlot.p,
+                           G->dConnStart, G->dConnList, ev->mdCount.p, ev->sgCap.p);
+    if (int r = scanExclusive<int>(ev->sgCap.p, ev->sgCapOff.p, nMD, ev->scan, st)) return r;
+    const long long capTotal = readLL(ev->sgCapOff.p + nMD, st);
+    if (capTotal < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->sgSlabOut.ensure(capTotal)) return r;
+    if (int r = ev->sgSlabF.ensure(capTotal)) return r;
+    timerBegin(ev, T_SG);
+    if (nMD > 0)
+        hipLaunchKernelGGL(k_sg, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, int(nMD), G->NM, G->dMods,
+                           hitsView(ev), mdView(ev), G->dConnStart, G->dConnList, ev->mdStart.p, ev->mdCount.p,
+                           ev->sgCap.p, ev->sgCapOff.p, ev->sgSlabOut.p, ev->sgSlabF.p, ev->sgCount.p);
+    timerEnd(ev, T_SG);
+    if (int r = scanExclusive<int>(ev->sgCount.p, ev->sgStart.p, nMD, ev->scan, st)) return r;
+    ev->nSG = readLL(ev->sgStart.p + nMD, st);
+    if (ev->nSG < 0) return fail(SDL_ERR_HIP, "readback failed");
+    const long long ns = std::max(ev->nSG, 1LL);
+    if (int r = ev->sgIn.ensure(ns)) return r;
+    if (int r = ev->sgOut.ensure(ns)) return r;
+    if (int r = ev->sgF.ensure(ns)) return r;
+    if (nMD > 0)
+        hipLaunchKernelGGL(k_sg_repack, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, int(nMD),
+                           ev->sgCapOff.p, ev->sgCount.p, ev->sgStart.p, ev->sgSlabOut.p, ev->sgSlabF.p, sgView(ev));
+    hipLaunchKernelGGL(k_slot_range, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, NS, ev->mdStart.p,
+                       ev->sgStart.p, ev->slotSgStart.p);
+    // reverse index: SGs by outer MD
+    if (int r = ev->sgEndCount.ensure(nm)) return r;
+    if (int r = ev->sgEndFill.ensure(nm)) return r;
+    if (int r = ev->sgEndStart.ensure(nm + 1)) return r;
+    if (int r = ev->sgEndList.ensure(ns)) return r;
+    HIP_OK(hipMemsetAsync(ev->sgEndCount.p, 0, sizeof(int) * nm, st));
+    HIP_OK(hipMemsetAsync(ev->sgEndFill.p, 0, sizeof(int) * nm, st));
+    if (ev->nSG > 0)
+        hipLaunchKernelGGL(k_count_by, dim3(gridFor(ev->nSG, kBlock)), dim3(kBlock), 0, st, int(ev->nSG), ev->sgOut.p,
+                           ev->sgEndCount.p);
+    if (int r = scanExclusive<int>(ev->sgEndCount.p, ev->sgEndStart.p, nMD, ev->scan, st)) return r;
+    if (ev->nSG > 0) {
+        hipLaunchKernelGGL(k_scatter_by, dim3(gridFor(ev->nSG, kBlock)), dim3(kBlock), 0, st, int(ev->nSG),
+                           ev->sgOut.p, ev->sgEndStart.p, ev->sgEndFill.p, ev->sgEndList.p);
+        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, int(nMD),
+                           ev->sgEndStart.p, ev->sgEndList.p);
+    }
+    accumulate<int>(ev, nMD, ev->sgCap.p, ev->mdSlot.p, 1);
+    accumulate<int>(ev, nMD, ev->sgCount.p, ev->mdSlot.p, 6);
+    return 0;
+}
+
+int runTP(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const long long nSG = ev->nSG;
+    const long long ns = std::max(nSG, 1LL);
+    const int NS = ev->E * G->NM;
+    if (int r = ev->tpCount.ensure(ns)) return r;
+    if (int r = ev->tpStart.ensure(ns + 1)) return r;
+    if (int r = ev->slotTpStart.ensure(NS + 1)) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tp_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), ev->sgOut.p,
+                           ev->sgStart.p, ev->tpCount.p);
+    if (int r = scanExclusive<int>(ev->tpCount.p, ev->tpStart.p, nSG, ev->scan, st)) return r;
+    ev->nTP = readLL(ev->tpStart.p + nSG, st);
+    if (ev->nTP < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->tpIn.ensure(std::max(ev->nTP, 1LL))) return r;
+    if (int r = ev->tpOut.ensure(std::max(ev->nTP, 1LL))) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tp_write, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), ev->sgOut.p,
+                           ev->sgStart.p, ev->tpStart.p, ev->tpIn.p, ev->tpOut.p);
+    hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, NS, ev->slotSgStart.p,
+                       ev->tpStart.p, ev->slotTpStart.p);
+    // counters: candidates per inner SG, objects per inner SG
+    if (int r = ev->cand.ensure(ns)) return r;
+    if (nSG > 0) {
+        hipLaunchKernelGGL(k_tp_cand, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), G->NM, ev->sgIn.p,
+                           ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->cand.p);
+        if (int r = ev->itemSlot.ensure(ns)) return r;
+        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, nSG, ev->sgIn.p,
+                           (const int*)nullptr, ev->mdSlot.p, ev->itemSlot.p);
+        accumulate<long long>(ev, nSG, ev->cand.p, ev->itemSlot.p, 2);
+        accumulate<int>(ev, nSG, ev->tpCount.p, ev->itemSlot.p, 7);
+    }
+    return 0;
+}
+
+int runTL(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const long long nSG = ev->nSG;
+    const long long ns = std::max(nSG, 1LL);
+    const int NS = ev->E * G->NM;
+    if (int r = ev->tlCap.ensure(ns)) return r;
+    if (int r = ev->tlCapOff.ensure(ns + 1)) return r;
+    if (int r = ev->tlCount.ensure(ns)) return r;
+    if (int r = ev->tlStart.ensure(ns + 1)) return r;
+    if (int r = ev->slotTlStart.ensure(NS + 1)) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), G->NM, ev->sgOut.p,
+                           ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->tlCap.p);
+    if (int r = scanExclusive<long long>(ev->tlCap.p, ev->tlCapOff.p, nSG, ev->scan, st)) return r;
+    const long long capTotal = readLL(ev->tlCapOff.p + nSG, st);
+    if (capTotal < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->tlSlabOut.ensure(capTotal)) return r;
+    timerBegin(ev, T_TL);
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tl, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, int(nSG), G->NM, G->dMods,
+                           hitsView(ev), mdView(ev), sgView(ev), G->dConnStart, G->dConnList, ev->slotSgStart.p,
+                           ev->tlCap.p, ev->tlCapOff.p, ev->tlSlabOut.p, ev->tlCount.p);
+    timerEnd(ev, T_TL);
+    if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, nSG, ev->scan, st)) return r;
+    ev->nTL = readLL(ev->tlStart.p + nSG, st);
+    if (ev->nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->tlIn.ensure(std::max(ev->nTL, 1LL))) return r;
+    if (int r = ev->tlOut.ensure(std::max(ev->nTL, 1LL))) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tl_repack, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, int(nSG),
+                           ev->tlCapOff.p, ev->tlCount.p, ev->tlStart.p, ev->tlSlabOut.p, ev->tlIn.p, ev->tlOut.p);
+    hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, NS, ev->slotSgStart.p,
+                       ev->tlStart.p, ev->slotTlStart.p);
+    if (nSG > 0) {
+        if (int r = ev->itemSlot.ensure(ns)) return r;
+        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, nSG, ev->sgIn.p,
+                           (const int*)nullptr, ev->mdSlot.p, ev->itemSlot.p);
+        accumulate<long long>(ev, nSG, ev->tlCap.p, ev->itemSlot.p, 3);
+        accumulate<int>(ev, nSG, ev->tlCount.p, ev->itemSlot.p, 8);
+    }
+    return 0;
+}
+
+int runTCP(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const long long nTL = ev->nTL, nMD = ev->nMD;
+    const long long nl = std::max(nTL, 1LL);
+    if (int r = ev->tpEndCount.ensure(std::max(nMD, 1LL))) return r;
+    if (int r = ev->tcCount.ensure(nl)) return r;
+    if (int r = ev->s3Of.ensure(nl)) return r;
+    if (int r = ev->tcpStart.ensure(nl + 1)) return r;
+    if (nMD > 0)
+        hipLaunchKernelGGL(k_tp_end_count, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, int(nMD),
+                           ev->sgEndStart.p, ev->sgEndList.p, ev->sgIn.p, ev->tpEndCount.p);
+    if (nTL > 0)
+        hipLaunchKernelGGL(k_tcp_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlIn.p,
+                           ev->tlOut.p, ev->sgIn.p, ev->sgOut.p, ev->sgStart.p, ev->tpEndCount.p, ev->s3Of.p,
+                           ev->tcCount.p);
+    if (int r = scanExclusive<int>(ev->tcCount.p, ev->tcpStart.p, nTL, ev->scan, st)) return r;
+    ev->nTCP = readLL(ev->tcpStart.p + nTL, st);
+    if (ev->nTCP < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->tcp.ensure(std::max(ev->nTCP, 1LL))) return r;
+    if (nTL > 0)
+        hipLaunchKernelGGL(k_tcp_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlIn.p,
+                           ev->tlOut.p, ev->sgIn.p, ev->sgStart.p, ev->sgEndStart.p, ev->sgEndList.p, ev->tpStart.p,
+                           ev->s3Of.p, ev->tcpStart.p, ev->tcp.p);
+    // counters: a TC made from triplets counts at its inner TP's module (Event.cc:961)
+    if (ev->nTCP > 0) {
+        if (int r = ev->itemSlot.ensure(ev->nTCP)) return r;
+        hipLaunchKernelGGL(k_tcp_slot, dim3(gridFor(ev->nTCP, kBlock)), dim3(kBlock), 0, st, ev->nTCP, ev->tcp.p,
+                           ev->tpIn.p, ev->sgIn.p, ev->mdSlot.p, ev->itemSlot.p);
+        accumulate<int>(ev, ev->nTCP, (const int*)nullptr, ev->itemSlot.p, 9);
+    }
+    const long long nTP = ev->nTP;
+    if (nTP > 0) {
+        if (int r = ev->cand.ensure(nTP)) return r;
+        if (int r = ev->itemSlot.ensure(nTP)) return r;
+        hipLaunchKernelGGL(k_tc_cand_tp, dim3(gridFor(nTP, kBlock)), dim3(kBlock), 0, st, nTP, G->NM, ev->tpIn.p,
+                           ev->tpOut.p, ev->sgIn.p, ev->sgOut.p, ev->mdSlot.p, G->dConnStart, G->dConnList,
+                           ev->slotTpStart.p, ev->cand.p, ev->itemSlot.p);
+        accumulate<long long>(ev, nTP, ev->cand.p, ev->itemSlot.p, 4);
+    }
+    return 0;
+}
+
+int runTCL(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const long long nTL = ev->nTL;
+    const long long nl = std::max(nTL, 1LL);
+    if (int r = ev->tcCount.ensure(nl)) return r;
+    if (int r = ev->tclStart.ensure(nl + 1)) return r;
+    if (nTL > 0)
+        hipLaunchKernelGGL(k_tcl_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlOut.p,
+                           ev->tlStart.p, ev->tcCount.p);
+    if (int r = scanExclusive<int>(ev->tcCount.p, ev->tclStart.p, nTL, ev->scan, st)) return r;
+    ev->nTCL = readLL(ev->tclStart.p + nTL, st);
+    if (ev->nTCL < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->tcl.ensure(std::max(ev->nTCL, 1LL))) return r;
+    if (nTL > 0) {
+        hipLaunchKernelGGL(k_tcl_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlOut.p,
+                           ev->tlStart.p, ev->tclStart.p, ev->tcl.p);
+        if (int r = ev->itemSlot.ensure(nl)) return r;
+        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, nTL, ev->tlIn.p,
+                           ev->sgIn.p, ev->mdSlot.p, ev->itemSlot.p);
+        accumulate<int>(ev, nTL, ev->tcCount.p, ev->itemSlot.p, 9);
+        if (int r = ev->cand.ensure(nl)) return r;
+        hipLaunchKernelGGL(k_tc_cand_tl, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, nTL, G->NM, ev->tlIn.p,
+                           ev->sgIn.p, ev->sgOut.p, ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotTlStart.p,
+                           ev->cand.p, ev->itemSlot.p);
+        accumulate<long long>(ev, nTL, ev->cand.p, ev->itemSlot.p, 4);
+    }
+    return 0;
+}
+
+int stageRange(sdl_event* ev, int e, int stage, long long& a, long long& n) {
+    // objects of event e are contiguous: compute first/last via the slot ranges
+    const int NM = ev->g->NM;
+    const long long s0 = (long long)e * NM, s1 = (long long)(e + 1) * NM;
+    auto rd = [&](const long long* p, long long i) { return readLL(p + i, ev->st); };
+    long long lo = 0, hi = 0;
+    switch (stage) {
+        case SDL_STAGE_MD: lo = rd(ev->mdStart.p, s0); hi = rd(ev->mdStart.p, s1); break;
+        case SDL_STAGE_SG: lo = rd(ev->slotSgStart.p, s0); hi = rd(ev->slotSgStart.p, s1); break;
+        case SDL_STAGE_TP: lo = rd(ev->slotTpStart.p, s0); hi = rd(ev->slotTpStart.p, s1); break;
+        case SDL_STAGE_TL: lo = rd(ev->slotTlStart.p, s0); hi = rd(ev->slotTlStart.p, s1); break;
+        case SDL_STAGE_TCP:
+        case SDL_STAGE_TCL: {
+            const long long l0 = rd(ev->slotTlStart.p, s0), l1 = rd(ev->slotTlStart.p, s1);
+            const long long* st = stage == SDL_STAGE_TCP ? ev->tcpStart.p : ev->tclStart.p;
+            lo = rd(st, l0);
+            hi = rd(st, l1);
+            break;
+        }
+        default: return fail(SDL_ERR_ARG, "bad stage");
+    }
+    if (lo < 0 || hi < 0) return fail(SDL_ERR_HIP, "readback failed");
+    a = lo;
+    n = hi - lo;
+    return 0;
+}
+
+int beginBatch(sdl_event* ev) {
+    if (!ev->hostStaged) return 0;
+    // upload host-staged hits
+    ev->E = int(ev->hoff.size()) - 1;
+    ev->evOff = ev->hoff;
+    ev->NH = ev->evOff.back();
+    const long long n = std::max(ev->NH, 1LL);
+    if (int r = ev->bx.ensure(n)) return r;
+    if (int r = ev->by.ensure(n)) return r;
+    if (int r = ev->bz.ensure(n)) return r;
+    if (int r = ev->bidx.ensure(n)) return r;
+    if (int r = ev->bdet.ensure(n)) return r;
+    if (ev->NH > 0) {
+        HIP_OK(hipMemcpyAsync(ev->bx.p, ev->hx.data(), sizeof(float) * ev->NH, hipMemcpyHostToDevice, ev->st));
+        HIP_OK(hipMemcpyAsync(ev->by.p, ev->hy.data(), sizeof(float) * ev->NH, hipMemcpyHostToDevice, ev->st));
+        HIP_OK(hipMemcpyAsync(ev->bz.p, ev->hz.data(), sizeof(float) * ev->NH, hipMemcpyHostToDevice, ev->st));
+        HIP_OK(hipMemcpyAsync(ev->bidx.p, ev->hidx.data(), sizeof(int) * ev->NH, hipMemcpyHostToDevice, ev->st));
+        HIP_OK(hipMemcpyAsync(ev->bdet.p, ev->hdet.data(), sizeof(unsigned) * ev->NH, hipMemcpyHostToDevice, ev->st));
+    }
+    ev->px = ev->bx.p; ev->py = ev->by.p; ev->pz = ev->bz.p; ev->pidx = ev->bidx.p; ev->pdet = ev->bdet.p;
+    return 0;
+}
+
+}  // namespace
+
+// ====================================================================== C-ABI
+extern "C" {
+
+const char* sdl_last_error(void) { return g_err.c_str(); }
+
+int sdl_geom_load(const char* tiltedPath, const char* endcapPath, const char* mapPath, sdl_geom** out) {
+    if (!tiltedPath || !endcapPath || !mapPath || !out) return fail(SDL_ERR_ARG, "null argument");
+    auto* G = new sdl_geom();
+    if (int r = loadGeomFiles(tiltedPath, endcapPath, mapPath, *G)) { delete G; return r; }
+    (void)hipGetDevice(&G->device);
+    auto up = [&](void** d, const void* h, size_t bytes) -> int {
+        if (hipMalloc(d, std::max<size_t>(bytes, 4)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
+        if (bytes && hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(SDL_ERR_HIP, "hipMemcpy failed");
+        return 0;
+    };
+    int r = up((void**)&G->dDet, G->detIds.data(), sizeof(uint32_t) * G->detIds.size());
+    if (!r) r = up((void**)&G->dMods, G->mods.data(), sizeof(ModDev) * G->mods.size());
+    if (!r) r = up((void**)&G->dConnStart, G->connStart.data(), sizeof(int) * G->connStart.size());
+    if (!r) r = up((void**)&G->dConnList, G->connList.data(), sizeof(int) * G->connList.size());
+    if (r) { sdl_geom_free(G); return r; }
+    *out = G;
+    return 0;
+}
+
+void sdl_geom_free(sdl_geom* g) {
+    if (!g) return;
+    if (g->dDet) (void)hipFree(g->dDet);
+    if (g->dMods) (void)hipFree(g->dMods);
+    if (g->dConnStart) (void)hipFree(g->dConnStart);
+    if (g->dConnList) (void)hipFree(g->dConnList);
+    delete g;
+}
+
+int sdl_geom_num_modules(const sdl_geom* g) { return g ? g->NM : 0; }
+
+int sdl_event_create(sdl_geom* g, sdl_event** out) {
+    if (!g || !out) return fail(SDL_ERR_ARG, "null argument");
+    auto* ev = new sdl_event();
+    ev->g = g;
+    if (hipSetDevice(g->device) != hipSuccess || hipStreamCreateWithFlags(&ev->st, hipStreamNonBlocking) != hipSuccess) {
+        delete ev;
+        return fail(SDL_ERR_HIP, "stream creation failed");
+    }
+    *out = ev;
+    return 0;
+}
+
+void sdl_event_destroy(sdl_event* ev) {
+    if (!ev) return;
+    (void)hipStreamSynchronize(ev->st);
+    // DBuf members free on release
+    auto rel = [](auto& b) { b.release(); };
+    rel(ev->bx); rel(ev->by); rel(ev->bz); rel(ev->bidx); rel(ev->bdet); rel(ev->dEvOff);
+    rel(ev->rt); rel(ev->r3); rel(ev->phi); rel(ev->hslot); rel(ev->eHi); rel(ev->eLo);
+    rel(ev->slotHitCount); rel(ev->slotFill); rel(ev->sortedHit); rel(ev->unknown); rel(ev->slotHitStart);
+    rel(ev->mdCap); rel(ev->mdCount); rel(ev->mdCapOff); rel(ev->mdStart); rel(ev->mdSlabHits); rel(ev->mdSlabF);
+    rel(ev->mdLo); rel(ev->mdUp); rel(ev->mdAnchor); rel(ev->mdSlot); rel(ev->mdA); rel(ev->mdB); rel(ev->mdC);
+    rel(ev->sgCap); rel(ev->sgCount); rel(ev->sgSlabOut); rel(ev->sgIn); rel(ev->sgOut); rel(ev->sgCapOff);
+    rel(ev->sgStart); rel(ev->slotSgStart); rel(ev->sgSlabF); rel(ev->sgF); rel(ev->sgEndCount); rel(ev->sgEndFill);
+    rel(ev->sgEndList); rel(ev->sgEndStart); rel(ev->tpCount); rel(ev->tpIn); rel(ev->tpOut); rel(ev->tpEndCount);
+    rel(ev->tpStart); rel(ev->slotTpStart); rel(ev->tlCap); rel(ev->tlCapOff); rel(ev->tlStart); rel(ev->slotTlStart);
+    rel(ev->tlCount); rel(ev->tlSlabOut); rel(ev->tlIn); rel(ev->tlOut); rel(ev->tcCount); rel(ev->s3Of);
+    rel(ev->tcpStart); rel(ev->tclStart); rel(ev->tcp); rel(ev->tcl); rel(ev->cand); rel(ev->itemSlot);
+    rel(ev->counters); rel(ev->tmp64); rel(ev->exportBuf);
+    for (auto& t : ev->tev)
+        for (auto& x : t)
+            if (x) (void)hipEventDestroy(x);
+    for (auto& l : ev->scan.sums) l.release();
+    for (auto& l : ev->scan.offs) l.release();
+    (void)hipStreamDestroy(ev->st);
+    delete ev;
+}
+
+int sdl_event_reset(sdl_event* ev) {
+    if (!ev) return fail(SDL_ERR_ARG, "null event");
+    ev->hx.clear(); ev->hy.clear(); ev->hz.clear(); ev->hidx.clear(); ev->hdet.clear();
+    ev->hoff.assign(1, 0);
+    ev->hostStaged = true;
+    ev->E = 0; ev->NH = 0;
+    ev->stage = -1;
+    ev->hitsReady = false;
+    ev->nMD = ev->nSG = ev->nTP = ev->nTL = ev->nTCP = ev->nTCL = 0;
+    return 0;
+}
+
+int sdl_event_add_hits(sdl_event* ev, int n, const float* x, const float* y, const float* z, const int* idx,
+                       const unsigned int* detId) {
+    if (!ev || n < 0 || (n > 0 && (!x || !y || !z || !idx || !detId))) return fail(SDL_ERR_ARG, "bad arguments");
+    if (!ev->hostStaged) sdl_event_reset(ev);
+    ev->hx.insert(ev->hx.end(), x, x + n);
+    ev->hy.insert(ev->hy.end(), y, y + n);
+    ev->hz.insert(ev->hz.end(), z, z + n);
+    ev->hidx.insert(ev->hidx.end(), idx, idx + n);
+    ev->hdet.insert(ev->hdet.end(), detId, detId + n);
+    if (ev->hoff.size() == 1) ev->hoff.push_back(0);
+    ev->hoff.back() += n;
+    ev->stage = -1;
+    ev->hitsReady = false;
+    return 0;
+}
+
+int sdl_event_next(sdl_event* ev) {
+    if (!ev) return fail(SDL_ERR_ARG, "null event");
+    if (ev->hoff.size() == 1) ev->hoff.push_back(0);
+    ev->hoff.push_back(ev->hoff.back());
+    return 0;
+}
+
+int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offsets, const float* dx, const float* dy,
+                              const float* dz, const int* didx, const unsigned int* ddetId) {
+    if (!ev || nevents <= 0 || !offsets) return fail(SDL_ERR_ARG, "bad arguments");
+    ev->hostStaged = false;
+    ev->E = nevents;
+    ev->evOff.assign(offsets, offsets + nevents + 1);
+    ev->NH = ev->evOff.back();
+    ev->px = dx; ev->py = dy; ev->pz = dz; ev->pidx = didx; ev->pdet = ddetId;
+    ev->stage = -1;
+    ev->hitsReady = false;
+    return 0;
+}
+
+int sdl_event_run(sdl_event* ev, int stage, int algo) {
+    if (!ev) return fail(SDL_ERR_ARG, "null event");
+    if (algo != 0) return fail(SDL_ERR_ARG, "only the default algorithm has device kernels");
+    if (stage != ev->stage + 1) return fail(SDL_ERR_STATE, "stages must run in order");
+    if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
+    int r = 0;
+    if (!ev->hitsReady) {
+        if (ev->hostStaged && ev->hoff.size() == 1) ev->hoff.push_back(0);
+        if ((r = beginBatch(ev))) return r;
+        if (ev->E <= 0) return fail(SDL_ERR_STATE, "no event");
+        if ((r = runPrep(ev))) return r;
+    }
+    switch (stage) {
+        case SDL_STAGE_MD: r = runMD(ev); break;
+        case SDL_STAGE_SG: r = runSG(ev); break;
+        case SDL_STAGE_TP: r = runTP(ev); break;
+        case SDL_STAGE_TL: r = runTL(ev); break;
+        case SDL_STAGE_TCP: r = runTCP(ev); break;
+        case SDL_STAGE_TCL: r = runTCL(ev); break;
+        default: return fail(SDL_ERR_ARG, "bad stage");
+    }
+    if (r) return r;
+    if (hipGetLastError() != hipSuccess) return fail(SDL_ERR_HIP, "kernel launch failed");
+    ev->stage = stage;
+    return 0;
+}
+
+int sdl_event_run_chain(sdl_event* ev) {
+    if (!ev) return fail(SDL_ERR_ARG, "null event");
+    ev->stage = -1;
+    ev->hitsReady = false;
+    for (int s = 0; s <= SDL_STAGE_TCL; ++s)
+        if (int r = sdl_event_run(ev, s, 0)) return r;
+    return 0;
+}
+
+int sdl_event_sync(sdl_event* ev) {
+    if (!ev) return fail(SDL_ERR_ARG, "null event");
+    HIP_OK(hipStreamSynchronize(ev->st));
+    return 0;
+}
+
+int sdl_event_layer_counts(sdl_event* ev, int e, unsigned long long out[110]) {
+    if (!ev || !out || e < 0 || e >= ev->E) return fail(SDL_ERR_ARG, "bad arguments");
+    HIP_OK(hipMemcpyAsync(out, ev->counters.p + size_t(e) * kNCounters * kNBins,
+                          sizeof(unsigned long long) * kNCounters * kNBins, hipMemcpyDeviceToHost, ev->st));
+    HIP_OK(hipStreamSynchronize(ev->st));
+    return 0;
+}
+
+int sdl_event_counts(sdl_event* ev, int e, unsigned long long out[10]) {
+    if (!ev || !out || e < -1 || e >= ev->E) return fail(SDL_ERR_ARG, "bad arguments");
+    std::vector<unsigned long long> all(size_t(ev->E) * kNCounters * kNBins);
+    HIP_OK(hipMemcpyAsync(all.data(), ev->counters.p, sizeof(unsigned long long) * all.size(), hipMemcpyDeviceToHost, ev->st));
+    HIP_OK(hipStreamSynchronize(ev->st));
+    for (int k = 0; k < kNCounters; ++k) out[k] = 0;
+    for (int ee = 0; ee < ev->E; ++ee) {
+        if (e >= 0 && ee != e) continue;
+        for (int k = 0; k < kNCounters; ++k)
+            for (int b = 0; b < kNBins; ++b) out[k] += all[(size_t(ee) * kNCounters + k) * kNBins + b];
+    }
+    return 0;
+}
+
+int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n) {
+    if (!ev || !n || e < 0 || e >= ev->E || stage < 0 || stage > SDL_STAGE_TCL) return fail(SDL_ERR_ARG, "bad arguments");
+    if (stage > ev->stage) return fail(SDL_ERR_STATE, "stage has not run");
+    long long a = 0, cnt = 0;
+    if (int r = stageRange(ev, e, stage, a, cnt)) return r;
+    *n = cnt;
+    const int width[6] = {2, 4, 8, 8, 16, 16};
+    if (!out) return 0;
+    if (cap < cnt * width[stage]) return fail(SDL_ERR_CAPACITY, "export buffer too small");
+    if (cnt == 0) return 0;
+    if (int r = ev->exportBuf.ensure(cnt * width[stage])) return r;
+    const unsigned grid = gridFor(cnt, kBlock);
+    if (stage == SDL_STAGE_MD)
+        hipLaunchKernelGGL(k_export_md, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, ev->mdLo.p, ev->mdUp.p, ev->pidx,
+                           ev->exportBuf.p);
+    else {
+        const int kind = stage == SDL_STAGE_SG ? 1 : (stage == SDL_STAGE_TP || stage == SDL_STAGE_TL) ? 2
+                                                                                                     : (stage == SDL_STAGE_TCP ? 3 : 4);
+        const int* pA = stage == SDL_STAGE_TP ? ev->tpIn.p : ev->tlIn.p;
+        const int* pB = stage == SDL_STAGE_TP ? ev->tpOut.p : ev->tlOut.p;
+        const int2* pC = stage == SDL_STAGE_TCP ? ev->tcp.p : ev->tcl.p;
+        hipLaunchKernelGGL(k_export, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, kind, pA, pB, pC, ev->tpIn.p,
+                           ev->tpOut.p, ev->tlIn.p, ev->tlOut.p, ev->sgIn.p, ev->sgOut.p, ev->mdLo.p, ev->mdUp.p,
+                           ev->pidx, ev->exportBuf.p);
+    }
+    HIP_OK(hipMemcpyAsync(out, ev->exportBuf.p, sizeof(int) * cnt * width[stage], hipMemcpyDeviceToHost, ev->st));
+    HIP_OK(hipStreamSynchronize(ev->st));
+    return 0;
+}
+
+int sdl_event_num_events(const sdl_event* ev) { return ev ? ev->E : 0; }
+
+int sdl_event_set_timing(sdl_event* ev, int on) {
+    if (!ev) return fail(SDL_ERR_ARG, "null event");
+    ev->timing = on != 0;
+    return 0;
+}
+
+int sdl_event_kernel_ms(sdl_event* ev, double* out, int n) {
+    if (!ev || !out || n < 0) return fail(SDL_ERR_ARG, "bad arguments");
+    for (int k = 0; k < T_NTIMERS; ++k) timerCollect(ev, k);
+    for (int k = 0; k < n && k < T_NTIMERS; ++k) out[k] = ev->kernelMs[k];
+    return 0;
+}
+
+void* sdl_event_stream(sdl_event* ev) { return ev ? (void*)ev->st : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,636 @@
+// Device-side cut algebra of the SDL chain for gfx950.
+//
+// Every function evaluates the reference's selection for ONE candidate in a
+// single lane, in FP32 where the reference computes in float and in FP64 where
+// its unqualified calls and double literals promote (A-16 of the reference's
+// arithmetic: sqrt/sin/tan/pow/atan2 and 0.2*, 1./, M_PI terms).  The float
+// transcendentals that the reference takes from libm in float (asinf, atanf,
+// sinf, cosf, tanf) are evaluated in FP64 and rounded once, i.e. correctly
+// rounded in practice.  The unqualified abs(float) of the reference binds to
+// int abs(int) (MiniDoublet.cc:273,787, Segment.cc:743-744) and is emulated
+// with the x86 truncation rule.  Compiled with -ffp-contract=off: no FMAs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdint>
+
+namespace sdl {
+
+constexpr double kPI = 3.14159265358979323846;
+constexpr float kSlopeInfF = 123456789.0f;   // SDL_INF as compared in float (MiniDoublet.cc:3)
+constexpr float kPtCut = 1.0f;               // PTCUT (Constants.h:4)
+constexpr float kSinAlphaMax = 0.95f;
+constexpr float kDeltaZLum = 15.f;
+constexpr float kPixelPSZpitch = 0.15f;
+constexpr float kStrip2SZpitch = 5.0f;
+
+enum : uint8_t { SUB_ENDCAP = 4, SUB_BARREL = 5 };
+enum : uint8_t { SIDE_NEGZ = 1, SIDE_POSZ = 2, SIDE_CENTER = 3 };
+enum : uint8_t { TYPE_PS = 0, TYPE_2S = 1 };
+enum : uint8_t { LT_PIXEL = 0, LT_STRIP = 1 };
+
+// One sensor module of the geometry (32 bytes).  The strip-sensor lookups of
+// the reference (drdz, slope) are resolved once at load time: for a lower
+// module whose sensor is the pixel one they come from its partner
+// (MiniDoublet.cc:779-781, 1032-1048; Segment.cc:926-929).
+struct ModDev {
+    uint32_t detId;
+    int32_t partner;                 // module index of the partner sensor
+    uint8_t subdet, side, layer, ring;
+    uint8_t rod, type, layerType, isLower;
+    float drdzStrip;                 // tiltedGeometry.getDrDz(strip sensor)
+    float slopeStrip;                // barrel: tiltedGeometry.getSlope; endcap: endcapGeometry.getSlopeLower
+    float centroidPhi;               // endcapGeometry.getCentroidPhi(own detId)
+    float gap;                       // MiniDoublet::moduleGapSize
+};
+
+// k2Rinv1GeVf = float(float(2.99792458e-3 * 3.8) / 2.) (MiniDoublet.cc:746-747)
+__device__ __forceinline__ float k2Rinv() {
+    return float(double(float(2.99792458e-3 * 3.8)) / 2.);
+}
+
+__device__ __forceinline__ float asinF(float v) { return float(asin(double(v))); }
+__device__ __forceinline__ float atanF(float v) { return float(atan(double(v))); }
+__device__ __forceinline__ float sinF(float v) { return float(sin(double(v))); }
+__device__ __forceinline__ float cosF(float v) { return float(cos(double(v))); }
+__device__ __forceinline__ float tanF(float v) { return float(tan(double(v))); }
+__device__ __forceinline__ float sqrtF(float v) { return float(sqrt(double(v))); }
+__device__ __forceinline__ float fabsF(float v) { return v < 0.f ? -v : (v == 0.f ? 0.f : v); }
+// std::min / std::max argument-order semantics (NaN follows the first argument)
+__device__ __forceinline__ float fmin2(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fmax2(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double sq(double v) { return v * v; }
+
+__device__ __forceinline__ int absTrunc(float v) {
+    int i = (v > -2147483648.f && v < 2147483648.f) ? int(v) : INT_MIN;
+    return i == INT_MIN ? INT_MIN : (i < 0 ? -i : i);
+}
+
+// MathUtil.cc:3-19
+__device__ __forceinline__ float phiMpiPi(float x) {
+    if (isnan(x) || isinf(x)) return x;
+    while (double(x) >= kPI) x = float(double(x) - 2. * kPI);
+    while (double(x) < -kPI) x = float(double(x) + 2. * kPI);
+    return x;
+}
+// MathUtil.cc:21-27
+__device__ __forceinline__ float atan2Sdl(float y, float x) {
+    if (x != 0) return float(atan2(double(y), double(x)));
+    if (y == 0) return 0.f;
+    if (y > 0) return float(kPI / 2);
+    return float(-kPI / 2);
+}
+// derived quantities of a point (Hit.cc:68-83)
+__device__ __forceinline__ float rtOf(float x, float y) { return sqrtF(x * x + y * y); }
+__device__ __forceinline__ float r3Of(float x, float y, float z) { return sqrtF(x * x + y * y + z * z); }
+__device__ __forceinline__ float phiOf(float x, float y) { return phiMpiPi(float(kPI + double(atan2Sdl(-y, -x)))); }
+// Hit::deltaPhi (Hit.cc:149-152)
+__device__ __forceinline__ float dPhi(float phiA, float phiB) { return phiMpiPi(phiB - phiA); }
+// Hit::deltaPhiChange (Hit.cc:154-173): deltaPhi(a, b - a)
+__device__ __forceinline__ float dPhiChange(float ax, float ay, float phiA, float bx, float by) {
+    return phiMpiPi(phiOf(bx - ax, by - ay) - phiA);
+}
+
+struct HitView {
+    float x, y, z, rt, r3, phi;
+};
+
+// ------------------------------------------------------------------ MD
+// MiniDoublet::moduleGapSize is precomputed per module (ModDev::gap).
+
+// MiniDoublet::dPhiThreshold (MiniDoublet.cc:741-810)
+__device__ __forceinline__ float dPhiThreshold(float rt, const ModDev& m, float dPhiArg, float dzArg) {
+    const float mulsB[6] = {0.0052f, 0.0038f, 0.0034f, 0.0034f, 0.0032f, 0.0034f};
+    const float rB[6] = {21.8f, 34.6f, 49.6f, 67.4f, 87.6f, 106.8f};
+    const float rE[5] = {131.4f, 156.2f, 185.6f, 220.3f, 261.5f};
+    const int iL = m.layer - 1;
+    const bool barrel = m.subdet == SUB_BARREL;
+    const float miniSlope = asinF(fmin2(rt * k2Rinv() / 1.f, kSinAlphaMax));
+    const float rLay = barrel ? rB[iL] : rE[iL];
+    const float miniPVoff = float(0.1 / double(rLay));
+    const float miniMuls = barrel ? mulsB[iL] * 3.f / 1.f : 0.006f * 3.f / 1.f;
+    if (barrel && m.side == SIDE_CENTER) return float(double(miniSlope) + sqrt(sq(miniMuls) + sq(miniPVoff)));
+    if (barrel) {
+        const float d = m.drdzStrip;
+        const float miniTilt = float(double(0.5f * kPixelPSZpitch * d) / sqrt(double(1.f + d * d)) / double(m.gap));
+        return float(double(miniSlope) + sqrt(sq(miniMuls) + sq(miniPVoff) + sq(miniTilt * miniSlope)));
+    }
+    const float miniLum = float(absTrunc(dPhiArg * kDeltaZLum / dzArg));
+    return float(double(miniSlope) + sqrt(sq(miniMuls) + sq(miniPVoff) + sq(miniLum)));
+}
+
+// MiniDoublet::shiftStripHits (MiniDoublet.cc:989-1209); pix / str are the
+// pixel-side and strip-side hits of the pair.
+__device__ __forceinline__ void shiftStripHits(const HitView& pix, const HitView& str, const ModDev& m, float& xn,
+                                               float& yn, float& zn) {
+    const bool isEndcap = m.subdet == SUB_ENDCAP;
+    const float angleA = fabsF(atanF(pix.rt / pix.z));
+    const float angleB = isEndcap ? float(kPI / 2.) : atanF(m.drdzStrip);
+    float sep = m.gap;
+    if (m.type == TYPE_PS && m.layerType != LT_PIXEL) sep *= -1;
+    const float sAB = sinF(angleA + angleB);
+    const float drprime = (sep / sAB) * sinF(angleA);
+    const float slope = m.slopeStrip;
+    const float absArctanSlope = slope != kSlopeInfF ? fabsF(atanF(slope)) : float(kPI / 2);
+    const float xp = pix.x, yp = pix.y;
+    float angleM;
+    if (xp > 0 && yp > 0) angleM = absArctanSlope;
+    else if (xp > 0 && yp < 0) angleM = float(kPI - double(absArctanSlope));
+    else if (xp < 0 && yp < 0) angleM = float(kPI + double(absArctanSlope));
+    else angleM = float(2 * kPI - double(absArctanSlope));
+    const float xa = xp + drprime * sinF(angleM);
+    const float ya = yp + drprime * cosF(angleM);
+    const float xo = str.x, yo = str.y;
+    if (slope == kSlopeInfF) { xn = xa; yn = yo; }
+    else if (slope == 0) { xn = xo; yn = ya; }
+    else {
+        const float inv = 1.f / slope;
+        xn = (slope * xa + inv * xo - ya + yo) / (slope + inv);
+        yn = (xn - xa) * slope + ya;
+    }
+    const float absdzprime = fabsF(sep / sAB * cosF(angleA));
+    const double az = double(fabsF(pix.z));
+    const float abszn = m.layerType == LT_PIXEL ? float(az + double(absdzprime)) : float(az - double(absdzprime));
+    zn = abszn * ((pix.z > 0) ? 1 : -1);
+}
+
+struct MDOut {
+    float dz, shiftedDz, dphi, dphichange;
+};
+
+// runMiniDoubletDefaultAlgo{Barrel,Endcap} (MiniDoublet.cc:243-684)
+__device__ __forceinline__ bool mdAlgo(const HitView& lo, const HitView& up, const ModDev& m, MDOut& o) {
+    o.dz = lo.z - up.z;
+    o.shiftedDz = 0.f;
+    o.dphi = 0.f;
+    o.dphichange = 0.f;
+    const bool pixLower = m.layerType == LT_PIXEL;
+    if (m.subdet == SUB_BARREL) {
+        const float dz = o.dz;
+        const float dzCut = m.type == TYPE_PS ? 2.f : 10.f;
+        const float sign = float(((dz > 0) - (dz < 0)) * ((lo.z > 0) - (lo.z < 0)));
+        const float invertedcrossercut = float(absTrunc(dz) > 2) * sign;
+        if (!(fabsF(dz) < dzCut && invertedcrossercut <= 0)) return false;
+        const float miniCut = pixLower ? dPhiThreshold(lo.rt, m, 0.f, 1.f) : dPhiThreshold(up.rt, m, 0.f, 1.f);
+        if (m.side != SIDE_CENTER) {
+            float xn, yn, zn;
+            if (m.type == TYPE_PS) {
+                if (pixLower) shiftStripHits(lo, up, m, xn, yn, zn);
+                else shiftStripHits(up, lo, m, xn, yn, zn);
+            } else {
+                shiftStripHits(lo, up, m, xn, yn, zn);
+            }
+            const float sPhi = phiOf(xn, yn);
+            const float sRt = rtOf(xn, yn);
+            if (pixLower) o.dphi = dPhi(lo.phi, sPhi);
+            else o.dphi = dPhi(sPhi, up.phi);
+            if (!(fabsF(o.dphi) < miniCut)) return false;
+            if (pixLower)
+                o.dphichange = lo.rt < sRt ? dPhiChange(lo.x, lo.y, lo.phi, xn, yn) : dPhiChange(xn, yn, sPhi, lo.x, lo.y);
+            else
+                o.dphichange = sRt < up.rt ? dPhiChange(xn, yn, sPhi, up.x, up.y) : dPhiChange(up.x, up.y, up.phi, xn, yn);
+        } else {
+            o.dphi = dPhi(lo.phi, up.phi);
+            if (!(fabsF(o.dphi) < miniCut)) return false;
+            o.dphichange = dPhiChange(lo.x, lo.y, lo.phi, up.x, up.y);
+        }
+        return fabsF(o.dphichange) < miniCut;
+    }
+    // endcap
+    float dz = o.dz;
+    if (!(fabsF(dz) < 10.f)) return false;  // side() is never Endcap (A-4)
+    const float drtCut = m.type == TYPE_PS ? 2.f : 10.f;
+    if (!(fabsF(lo.rt - up.rt) < drtCut)) return false;
+    float xn, yn, zn;
+    if (m.type == TYPE_PS && !pixLower) shiftStripHits(up, lo, m, xn, yn, zn);
+    else shiftStripHits(lo, up, m, xn, yn, zn);
+    const float sPhi = phiOf(xn, yn);
+    if (m.type == TYPE_PS && !pixLower) o.dphi = dPhi(sPhi, up.phi);
+    else o.dphi = dPhi(lo.phi, sPhi);
+    if (m.type == TYPE_PS) {
+        o.shiftedDz = pixLower ? lo.z - zn : up.z - zn;
+        dz = o.shiftedDz;
+    }
+    const float miniCut = pixLower ? dPhiThreshold(lo.rt, m, o.dphi, dz) : dPhiThreshold(up.rt, m, o.dphi, dz);
+    if (!(fabsF(o.dphi) < miniCut)) return false;
+    const float dzFrac = float(double(fabsF(dz)) / double(fabsF(lo.z)));
+    o.dphichange = o.dphi / dzFrac * (1.f + dzFrac);
+    return fabsF(o.dphichange) < miniCut;
+}
+
+// ------------------------------------------------------------------ SG
+// What the segment and tracklet algebra reads of a mini-doublet.
+struct MDView {
+    HitView a;          // anchor hit
+    float dz, shiftedDz, dphi, dphichange;
+    int mod;            // lower module index
+};
+
+struct SGOut {
+    float dphi, dphimin, dphimax, dphichange, dphichangemin, dphichangemax;
+};
+
+// Segment::dAlphaThreshold (Segment.cc:886-994) and the three alpha cuts
+// (Segment.cc:538-592 / 789-853).
+__device__ __forceinline__ bool sgAlphaCuts(const MDView& in, const MDView& out, const ModDev& mi, const ModDev& mo,
+                                            float sgAlpha) {
+    const float mulsB[6] = {0.0052f, 0.0038f, 0.0034f, 0.0034f, 0.0032f, 0.0034f};
+    const bool inB = mi.subdet == SUB_BARREL, outB = mo.subdet == SUB_BARREL;
+    const float sdMuls = inB ? mulsB[mi.layer - 1] * 3.f / kPtCut : 0.006f * 3.f / kPtCut;
+    const float segmentDr = float(sqrt(sq(out.a.y - in.a.y) + sq(out.a.x - in.a.x)));
+    const float dAlphaB = asinF(fmin2(segmentDr * k2Rinv() / kPtCut, kSinAlphaMax));
+    float lumIn, lumOut;
+    if (inB) {
+        float t = 0.f;
+        if (mi.side != SIDE_CENTER) {
+            const float d = mi.drdzStrip;
+            t = float(double(0.5f * kPixelPSZpitch * d) / sqrt(double(1.f + d * d)) / double(mi.gap));
+        }
+        lumIn = t * dAlphaB;
+    } else {
+        lumIn = float(double(in.dphi) * 15.0 / double(mi.type == TYPE_PS ? in.shiftedDz : in.dz));
+    }
+    if (outB) {
+        float t = 0.f;
+        if (mo.side != SIDE_CENTER) {
+            const float d = mo.drdzStrip;
+            t = float(double(0.5f * kPixelPSZpitch * d) / sqrt(double(1.f + d * d)) / double(mo.gap));
+        }
+        lumOut = t * dAlphaB;
+    } else {
+        lumOut = float(double(out.dphi) * 15.0 / double(mo.type == TYPE_PS ? out.shiftedDz : out.dz));
+    }
+    const float miniDelta = mi.gap;
+    const float resIn = 0.02f / miniDelta * (inB ? 1.0f : fabsF(in.a.z / in.a.rt));
+    const float resOut = 0.02f / miniDelta * (outB ? 1.0f : fabsF(out.a.z / out.a.rt));
+    const float res = resIn + resOut;
+    const double base = sq(res) + sq(sdMuls);
+    const float thIO = float(double(dAlphaB) + sqrt(base));
+    const float thIn = (inB && mi.side == SIDE_CENTER) ? thIO : float(double(dAlphaB) + sqrt(base + sq(lumIn)));
+    const float thOut = (outB && mo.side == SIDE_CENTER) ? thIO : float(double(dAlphaB) + sqrt(base + sq(lumOut)));
+    if (!(fabsF(in.dphichange - sgAlpha) < thIn)) return false;
+    if (!(fabsF(out.dphichange - sgAlpha) < thOut)) return false;
+    return fabsF(out.dphichange - in.dphichange) < thIO;
+}
+
+// runSegmentDefaultAlgo{Barrel,Endcap} (Segment.cc:337-859).  connFound: the
+// outer module is in the connection list of the inner one (Segment.cc:627-634).
+__device__ __forceinline__ bool sgAlgo(const MDView& in, const MDView& out, const ModDev& mi, const ModDev& mo,
+                                       bool connFound, const HitView& outHi, const HitView& outLo, SGOut& o) {
+    o.dphi = o.dphimin = o.dphimax = o.dphichange = o.dphichangemin = o.dphichangemax = 0.f;
+    const float rtIn = in.a.rt, rtOut = out.a.rt, zIn = in.a.z, zOut = out.a.z;
+    const float sdSlope = asinF(fmin2(rtOut * k2Rinv() / kPtCut, kSinAlphaMax));
+    if (mo.subdet == SUB_BARREL) {  // barrel-barrel, or endcap inner with barrel outer
+        const float mulsB[6] = {0.0052f, 0.0038f, 0.0034f, 0.0034f, 0.0032f, 0.0034f};
+        const float sdMuls = mulsB[mi.layer - 1] * 3.f / kPtCut;
+        const float sdPVoff = 0.1f / rtOut;
+        const float dzDrtScale = tanF(sdSlope) / sdSlope;
+        const float zGeom = mi.layer <= 2 ? 2.f * kPixelPSZpitch : 2.f * kStrip2SZpitch;
+        const float zLo = zIn + (zIn - kDeltaZLum) * (rtOut / rtIn - 1.f) * (zIn > 0.f ? 1.f : dzDrtScale) - zGeom;
+        const float zHi = zIn + (zIn + kDeltaZLum) * (rtOut / rtIn - 1.f) * (zIn < 0.f ? 1.f : dzDrtScale) + zGeom;
+        if (!(zOut >= zLo && zOut <= zHi)) return false;
+        const float sdCut = float(double(sdSlope) + sqrt(double(sdMuls * sdMuls + sdPVoff * sdPVoff)));
+        if (!(fabsF(dPhi(in.a.phi, out.a.phi)) <= sdCut)) return false;
+        o.dphichange = dPhiChange(in.a.x, in.a.y, in.a.phi, out.a.x, out.a.y);
+        if (!(fabsF(o.dphichange) <= sdCut)) return false;
+        return sgAlphaCuts(in, out, mi, mo, o.dphichange);
+    }
+    const bool l2r123 = (mi.ring == 1 || mi.ring == 2 || mi.ring == 3) && mi.layer == 2;
+    if (!(connFound || l2r123)) return false;
+    const float rtGeom = (rtIn < 60.f && rtOut < 60.f) ? 2.f * kPixelPSZpitch
+                         : ((rtIn < 60.f || rtOut < 60.f) ? (kPixelPSZpitch + kStrip2SZpitch) : 2.f * kStrip2SZpitch);
+    if (zIn * zOut < 0) return false;
+    const float dz = zOut - zIn;
+    const float dLum = copysignf(kDeltaZLum, zIn);
+    const float drtDzScale = sdSlope / tanF(sdSlope);
+    const float rtLo = fmax2(rtIn * (1.f + dz / (zIn + dLum) * drtDzScale) - rtGeom, rtIn - 0.5f * rtGeom);
+    const float rtHi = rtIn * (zOut - dLum) / (zIn - dLum) + rtGeom;
+    if (!(rtOut >= rtLo && rtOut <= rtHi)) return false;
+    const float dPhiPos = dPhi(in.a.phi, out.a.phi);
+    const float dHigh = dPhi(in.a.phi, outHi.phi);
+    const float dLow = dPhi(in.a.phi, outLo.phi);
+    const bool hiBigger = absTrunc(dHigh) > absTrunc(dLow);  // int abs (A-3)
+    o.dphi = dPhiPos;
+    o.dphimax = hiBigger ? dHigh : dLow;
+    o.dphimin = hiBigger ? dLow : dHigh;
+    if (!(fabsF(dPhiPos) <= sdSlope)) return false;
+    const float dzFrac = dz / zIn;
+    o.dphichange = dPhiPos / dzFrac * (1.f + dzFrac);
+    o.dphichangemin = o.dphimin / dzFrac * (1.f + dzFrac);
+    o.dphichangemax = o.dphimax / dzFrac * (1.f + dzFrac);
+    if (!(fabsF(o.dphichange) <= sdSlope)) return false;
+    return sgAlphaCuts(in, out, mi, mo, o.dphichange);
+}
+
+// ------------------------------------------------------------------ TL
+// Tracklet::runDeltaBetaIterations (Tracklet.cc:3181-3324): the values the cuts read.
+__device__ __forceinline__ void deltaBetaIter(float& betaIn, float& betaOut, float& betaAv, float& ptBeta, float sdInDr,
+                                              float sdOutDr, float dr, int lIn) {
+    const float ptBetaMax = 7.0f;
+    const float k2 = k2Rinv();
+    auto corr = [&](float len, float pt, float signOf) {
+        return copysign(double(asinF(fmin2(len * k2 / fabsF(pt), kSinAlphaMax))), double(signOf));
+    };
+    auto ptOf = [&](float beta) { return float(double(dr * k2) / sin(double(beta))); };
+    if (betaIn * betaOut > 0.f &&
+        (fabsF(ptBeta) < 4.f * ptBetaMax || (lIn >= 11 && fabsF(ptBeta) < 8.f * ptBetaMax))) {
+        const float inUpd = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
+        const float outUpd = float(double(betaOut) + corr(sdOutDr, ptBeta, betaOut));
+        betaAv = 0.5f * (inUpd + outUpd);
+        ptBeta = ptOf(betaAv);
+        betaIn = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
+        betaOut = float(double(betaOut) + corr(sdOutDr, ptBeta, betaOut));
+        betaAv = 0.5f * (betaIn + betaOut);
+        ptBeta = ptOf(betaAv);
+    } else if (lIn < 11 && double(fabsF(betaOut)) < 0.2 * double(fabsF(betaIn)) && fabsF(ptBeta) < 12.f * ptBetaMax) {
+        const float ptBetaIn = ptOf(betaIn);
+        const float inUpd = float(double(betaIn) + corr(sdInDr, ptBetaIn, betaIn));
+        const float outUpd = float(double(betaOut) + corr(sdOutDr, ptBetaIn, betaIn));
+        betaAv = (fabsF(betaOut) > 0.2f * fabsF(betaIn)) ? (0.5f * (inUpd + outUpd)) : inUpd;
+        ptBeta = ptOf(betaAv);
+        betaIn = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
+        betaOut = float(double(betaOut) + corr(sdOutDr, ptBeta, betaIn));
+        betaAv = 0.5f * (betaIn + betaOut);
+        ptBeta = ptOf(betaAv);
+    }
+}
+
+__device__ __forceinline__ void rescaleRH(float beta, float& mn, float& mx) {
+    const float s = mn + mx;
+    const float sf = (fabsF(s) > 0) ? (2.f * beta / fabsF(s)) : 0.f;
+    mn *= sf;
+    mx *= sf;
+}
+
+// Hit() += a, += b, /= 2 (Tracklet.cc:1131-1134) -> phi of the midpoint
+__device__ __forceinline__ float midPhi(const HitView& a, const HitView& b) {
+    const float mx = ((0.f + a.x) + b.x) / 2.0f, my = ((0.f + a.y) + b.y) / 2.0f;
+    return phiOf(mx, my);
+}
+
+struct SGView {
+    MDView in, out;
+    float dphi, dphimin, dphimax, dphichange, dphichangemin, dphichangemax;
+};
+
+// Edge hit of GeometryUtil::stripEdgeHit (GeometryUtil.cc:3-23)
+__device__ __forceinline__ HitView edgeOf(const HitView& h, float centroidPhi, bool up) {
+    const double s = up ? 1.0 : -1.0;
+    HitView e;
+    e.x = float(s * 2.5 * cos(double(centroidPhi))) + h.x;
+    e.y = float(s * 2.5 * sin(double(centroidPhi))) + h.y;
+    e.z = 0.f + h.z;
+    e.rt = rtOf(e.x, e.y);
+    e.r3 = r3Of(e.x, e.y, e.z);
+    e.phi = phiOf(e.x, e.y);
+    return e;
+}
+
+// runTrackletDefaultAlgoBarrelBarrelBarrelBarrel_v2 (Tracklet.cc:902-1538)
+__device__ __noinline__ bool tlBBBB(const SGView& sIn, const SGView& sOut, const ModDev* mods) {
+    const HitView& hInLo = sIn.in.a;
+    const HitView& hInUp = sIn.out.a;
+    const HitView& hOutLo = sOut.in.a;
+    const HitView& hOutUp = sOut.out.a;
+    const float k2 = k2Rinv();
+    const bool isPS_InLo = mods[sIn.in.mod].type == TYPE_PS;
+    const bool isPS_OutLo = mods[sOut.in.mod].type == TYPE_PS;
+    const float rt_InLo = hInLo.rt, rt_OutLo = hOutLo.rt, z_InLo = hInLo.z, z_OutLo = hOutLo.z;
+    const float alpha1GeV_OutLo = asinF(fmin2(rt_OutLo * k2 / kPtCut, kSinAlphaMax));
+    const float rtRatio = rt_OutLo / rt_InLo;
+    const float dzDrtScale = float(tan(double(alpha1GeV_OutLo)) / double(alpha1GeV_OutLo));
+    const float zpIn = isPS_InLo ? kPixelPSZpitch : kStrip2SZpitch;
+    const float zpOut = isPS_OutLo ? kPixelPSZpitch : kStrip2SZpitch;
+    const float zHi = z_InLo + (z_InLo + kDeltaZLum) * (rtRatio - 1.f) * (z_InLo < 0.f ? 1.f : dzDrtScale) + (zpIn + zpOut);
+    const float zLo = z_InLo + (z_InLo - kDeltaZLum) * (rtRatio - 1.f) * (z_InLo > 0.f ? 1.f : dzDrtScale) - (zpIn + zpOut);
+    if (!(z_OutLo >= zLo && z_OutLo <= zHi)) return false;
+    const float drt_OutLo_InLo = rt_OutLo - rt_InLo;
+    const float r3_InLo = sqrtF(z_InLo * z_InLo + rt_InLo * rt_InLo);
+    const float drt_InSeg = hInUp.rt - hInLo.rt;
+    const float dz_InSeg = hInUp.z - hInLo.z;
+    const float dr3_InSeg = hInUp.r3 - hInLo.r3;
+    const float coshEta = dr3_InSeg / drt_InSeg;
+    float dzErr = (zpIn + zpOut) * (zpIn + zpOut) * 2.f;
+    const float thetaMuls = float(double(0.015f) * sqrt(double(0.1f) + 0.2 * double(rt_OutLo - rt_InLo) / double(50.f)) *
+                                  sqrt(double(r3_InLo / rt_InLo)));
+    const float sdlMuls = thetaMuls * 3.f / kPtCut * 4.f;
+    dzErr += sdlMuls * sdlMuls * drt_OutLo_InLo * drt_OutLo_InLo / 3.f * coshEta * coshEta;
+    dzErr = sqrtF(dzErr);
+    const float dzMean = dz_InSeg / drt_InSeg * drt_OutLo_InLo;
+    const float zWindow = dzErr / drt_InSeg * drt_OutLo_InLo + (zpIn + zpOut);
+    const float zLoP = z_InLo + dzMean * (z_InLo > 0.f ? 1.f : dzDrtScale) - zWindow;
+    const float zHiP = z_InLo + dzMean * (z_InLo < 0.f ? 1.f : dzDrtScale) + zWindow;
+    if (!(z_OutLo >= zLoP && z_OutLo <= zHiP)) return false;
+    const float sdlPVoff = 0.1f / rt_OutLo;
+    const float sdlCut = float(double(alpha1GeV_OutLo) + sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
+    if (!(fabsF(dPhi(hInUp.phi, hOutUp.phi)) <= sdlCut)) return false;
+    const float dPhiMid = phiMpiPi(phiOf(hOutLo.x - hInLo.x, hOutLo.y - hInLo.y) - midPhi(hInLo, hOutLo));
+    if (!(fabsF(dPhiMid) <= sdlCut)) return false;
+    const float alpha_InLo = sIn.dphichange;
+    const ModDev& mOU = mods[sOut.out.mod];
+    const bool isEC_lastLayer = mOU.subdet == SUB_ENDCAP && mOU.type == TYPE_2S;
+    const HitView hiE = isEC_lastLayer ? edgeOf(hOutUp, mOU.centroidPhi, true) : hOutUp;
+    const HitView loE = isEC_lastLayer ? edgeOf(hOutUp, mOU.centroidPhi, false) : hOutUp;
+    // alpha_OutUp = deltaPhi(h, h - hOutLo) for the anchor and the two edges
+    const float aOU = phiMpiPi(phiOf(hOutUp.x - hOutLo.x, hOutUp.y - hOutLo.y) - hOutUp.phi);
+    const float aOUhi = phiMpiPi(phiOf(hiE.x - hOutLo.x, hiE.y - hOutLo.y) - hiE.phi);
+    const float aOUlo = phiMpiPi(phiOf(loE.x - hOutLo.x, loE.y - hOutLo.y) - loE.phi);
+    const float axX = hOutUp.x - hInLo.x, axY = hOutUp.y - hInLo.y;
+    const float axPhi = phiOf(axX, axY);
+    const float axHiPhi = phiOf(hiE.x - hInLo.x, hiE.y - hInLo.y);
+    const float axLoPhi = phiOf(loE.x - hInLo.x, loE.y - hInLo.y);
+    float betaIn = alpha_InLo - phiMpiPi(axPhi - hInLo.phi);
+    float betaInRHmin = betaIn, betaInRHmax = betaIn;
+    float betaOut = -aOU + phiMpiPi(axPhi - hOutUp.phi);
+    float betaOutRHmin = -aOUhi + phiMpiPi(axHiPhi - hiE.phi);
+    float betaOutRHmax = -aOUlo + phiMpiPi(axLoPhi - loE.phi);
+    const float drt_tl_axis = rtOf(axX, axY);
+    const float rt_InSeg = rtOf(hInUp.x - hInLo.x, hInUp.y - hInLo.y);
+    const float betaIn_cut = asinF(fmin2((-rt_InSeg * 1.f + drt_tl_axis) * k2 / kPtCut, kSinAlphaMax)) + (0.02f / drt_InSeg);
+    if (!(fabsF(betaInRHmin) < betaIn_cut)) return false;
+    float betaAv = 0.5f * (betaIn + betaOut);
+    float pt_beta = float(double(drt_tl_axis * k2) / sin(double(betaAv)));
+    const int lOut = isEC_lastLayer ? 11 : 5;
+    const float sdOut_dr = rtOf(hOutUp.x - hOutLo.x, hOutUp.y - hOutLo.y);
+    const float sdOut_d = hOutUp.rt - hOutLo.rt;
+    deltaBetaIter(betaIn, betaOut, betaAv, pt_beta, rt_InSeg, sdOut_dr, drt_tl_axis, 5);
+    rescaleRH(betaIn, betaInRHmin, betaInRHmax);
+    rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
+    const float dBetaMuls = thetaMuls * 4.f / fmin2(fabsF(pt_beta), 7.0f);
+    const float alpha_OutLo = sOut.dphichange;
+    const float alphaOutAbsReg = fmax2(fabsF(alpha_OutLo), asinF(fmin2(hOutLo.rt * k2 / 3.0f, kSinAlphaMax)));
+    const float dBetaOutLum = lOut < 11 ? 0.0f : fabsF(alphaOutAbsReg * kDeltaZLum / hOutLo.z);
+    const float dBetaLum2 = (0.0f + dBetaOutLum) * (0.0f + dBetaOutLum);
+    const float sinDPhi = sinF(dPhiMid);
+    const float dBetaROut2 = float(sq(double((hiE.rt - loE.rt) * sinDPhi / drt_tl_axis)));
+    const float betaOut_cut = float(double(asinF(fmin2(drt_tl_axis * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
+                                    sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
+    if (!(fabsF(betaOut) < betaOut_cut)) return false;
+    const float dBetaRes = 0.02f / fmin2(sdOut_d, drt_InSeg);
+    const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
+                                  0.25 * sq(double(fabsF(betaInRHmin - betaInRHmax) + fabsF(betaOutRHmin - betaOutRHmax))));
+    const float dBeta = betaIn - betaOut;
+    return dBeta * dBeta <= dBetaCut2;
+}
+
+// runTrackletDefaultAlgoBarrelBarrelEndcapEndcap (Tracklet.cc:1540-2016) and
+// ...EndcapEndcapEndcapEndcap (Tracklet.cc:2018-2665), which share their skeleton.
+template <bool EEEE>
+__device__ __noinline__ bool tlEndcapFamily(const SGView& sIn, const SGView& sOut, const ModDev* mods,
+                                            const HitView& outUpHi, const HitView& outUpLo) {
+    const HitView& hInLo = sIn.in.a;
+    const HitView& hInUp = sIn.out.a;
+    const HitView& hOutLo = sOut.in.a;
+    const HitView& hOutUp = sOut.out.a;
+    const float k2 = k2Rinv();
+    const bool psInLo = mods[sIn.in.mod].type == TYPE_PS;
+    const bool psOutLo = mods[sOut.in.mod].type == TYPE_PS;
+    const bool psInUp = mods[sIn.out.mod].type == TYPE_PS;
+    const float rtIn = hInLo.rt, rtOut = hOutLo.rt, zIn = hInLo.z, zOut = hOutLo.z;
+    const float sdlSlope = asinF(fmin2(rtOut * k2 / kPtCut, kSinAlphaMax));
+    const float dzDrtScale = float(tan(double(sdlSlope)) / double(sdlSlope));
+    if (!(zIn * zOut > 0)) return false;
+    const float dLum = copysignf(kDeltaZLum, zIn);
+    float rtLo, rtHi;
+    if (!EEEE) {
+        const float zGeom = (psInLo ? kPixelPSZpitch : kStrip2SZpitch) + (psOutLo ? kPixelPSZpitch : kStrip2SZpitch);
+        const float rtGeom1 = psOutLo ? kPixelPSZpitch : kStrip2SZpitch;
+        const float zGeom1 = copysignf(zGeom, zIn);
+        rtLo = rtIn * (1.f + (zOut - zIn - zGeom1) / (zIn + zGeom1 + dLum) / dzDrtScale) - rtGeom1;
+        if (!(rtOut >= rtLo)) return false;
+        float zInForHi = zIn - zGeom1 - dLum;
+        if (zInForHi * zIn < 0) zInForHi = copysignf(0.1f, zIn);
+        rtHi = rtIn * (1.f + (zOut - zIn + zGeom1) / zInForHi) + rtGeom1;
+    } else {
+        const float rtGeom = (psInLo && psOutLo) ? 2.f * kPixelPSZpitch
+                             : (psInLo || psOutLo) ? (kPixelPSZpitch + kStrip2SZpitch) : 2.f * kStrip2SZpitch;
+        const float dz = zOut - zIn;
+        rtLo = rtIn * (1.f + dz / (zIn + dLum) / dzDrtScale) - rtGeom;
+        if (!(rtOut >= rtLo)) return false;
+        rtHi = rtIn * (1.f + dz / (zIn - dLum)) + rtGeom;
+    }
+    if (!(rtOut >= rtLo && rtOut <= rtHi)) return false;
+    const float drtSDIn = hInUp.rt - hInLo.rt;
+    const float dzSDIn = hInUp.z - hInLo.z;
+    const float dr3SDIn = hInUp.r3 - hInLo.r3;
+    const float coshEta = dr3SDIn / drtSDIn;
+    const float dzOutInAbs = fabsF(zOut - zIn);
+    const float kZ = (zOut - zIn) / dzSDIn;
+    float thetaMuls;
+    if (!EEEE) {
+        const float rIn = sqrtF(zIn * zIn + rtIn * rtIn);
+        thetaMuls = float(double(0.015f) * sqrt(double(0.1f) + 0.2 * double(rtOut - rtIn) / double(50.f)) *
+                          sqrt(double(rIn / rtIn)));
+        if (!(kZ >= 0 && rtOut >= rtLo && rtOut <= rtHi)) return false;
+    } else {
+        thetaMuls = float(double(0.015f) * sqrt(double(0.1f) + 0.2 * double(rtOut - rtIn) / double(50.f)));
+    }
+    const float sdlMuls = thetaMuls * 3.f / kPtCut * 4.f;
+    if (EEEE && psInLo && psInUp) {
+        const float multDzDr = dzOutInAbs * coshEta / (coshEta * coshEta - 1.f);
+        const float rtGeom = (psInLo && psOutLo) ? 2.f * kPixelPSZpitch
+                             : (psInLo || psOutLo) ? (kPixelPSZpitch + kStrip2SZpitch) : 2.f * kStrip2SZpitch;
+        float drtErr = kPixelPSZpitch * kPixelPSZpitch * 2.f / dzSDIn / dzSDIn * dzOutInAbs * dzOutInAbs;
+        drtErr += sdlMuls * sdlMuls * multDzDr * multDzDr / 3.f * coshEta * coshEta;
+        drtErr = sqrtF(drtErr);
+        const float drtMean = drtSDIn * dzOutInAbs / fabsF(dzSDIn);
+        const float rtWindow = drtErr + rtGeom;
+        const float rtLoP = rtIn + drtMean / dzDrtScale - rtWindow;
+        const float rtHiP = rtIn + drtMean + rtWindow;
+        if (!(kZ >= 0 && rtOut >= rtLoP && rtOut <= rtHiP)) return false;
+    }
+    const float sdlPVoff = 0.1f / rtOut;
+    const float sdlCut = float(double(sdlSlope) + sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
+    if (!(fabsF(dPhi(hInUp.phi, hOutUp.phi)) <= sdlCut)) return false;
+    const float dPhiMid = phiMpiPi(phiOf(hOutLo.x - hInLo.x, hOutLo.y - hInLo.y) - midPhi(hInLo, hOutLo));
+    if (!(fabsF(dPhiMid) <= sdlCut)) return false;
+    const float sdIn_alpha = sIn.dphichange;
+    const float sdOut_alpha = sIn.dphichange;  // A-8
+    const float dr3x = hOutUp.x - hInLo.x, dr3y = hOutUp.y - hInLo.y;
+    const float dr3Phi = phiOf(dr3x, dr3y);
+    float betaIn = sdIn_alpha - phiMpiPi(dr3Phi - hInLo.phi);
+    float betaInRHmin, betaInRHmax, betaOutRHmin, betaOutRHmax, betaOut;
+    if (!EEEE) {
+        const float sdOut_alphaOut = phiMpiPi(phiOf(hOutUp.x - hOutLo.x, hOutUp.y - hOutLo.y) - hOutUp.phi);
+        const float aMin = phiMpiPi(sOut.dphichangemin - sOut.dphimin);
+        const float aMax = phiMpiPi(sOut.dphichangemax - sOut.dphimax);
+        betaOut = -sdOut_alphaOut + phiMpiPi(dr3Phi - hOutUp.phi);
+        betaInRHmin = betaIn;  // the 2S test never holds (A-9)
+        betaInRHmax = betaIn;
+        betaOutRHmin = betaOut - aMin + sdOut_alphaOut;
+        betaOutRHmax = betaOut - aMax + sdOut_alphaOut;
+    } else {
+        const float sdOut_dPhiPos = dPhi(hOutLo.phi, hOutUp.phi);
+        const float aMin = phiMpiPi(sOut.dphichangemin - sdOut_dPhiPos);
+        const float aMax = phiMpiPi(sOut.dphichangemax - sdOut_dPhiPos);
+        const float sdOut_alphaOut = phiMpiPi(sOut.dphichange - sdOut_dPhiPos);
+        betaOut = -sdOut_alphaOut + phiMpiPi(dr3Phi - hOutUp.phi);
+        betaInRHmin = betaIn + sIn.dphichangemin - sdIn_alpha;
+        betaInRHmax = betaIn + sIn.dphichangemax - sdIn_alpha;
+        betaOutRHmin = betaOut - aMin + sdOut_alphaOut;
+        betaOutRHmax = betaOut - aMax + sdOut_alphaOut;
+    }
+    if (!EEEE) {
+        if (fabsF(betaOutRHmin) > fabsF(betaOutRHmax)) { float t = betaOutRHmin; betaOutRHmin = betaOutRHmax; betaOutRHmax = t; }
+        if (fabsF(betaInRHmin) > fabsF(betaInRHmax)) { float t = betaInRHmin; betaInRHmin = betaInRHmax; betaInRHmax = t; }
+    } else {
+        if (fabsF(betaInRHmin) > fabsF(betaInRHmax)) { float t = betaInRHmin; betaInRHmin = betaInRHmax; betaInRHmax = t; }
+        if (fabsF(betaOutRHmin) > fabsF(betaOutRHmax)) { float t = betaOutRHmin; betaOutRHmin = betaOutRHmax; betaOutRHmax = t; }
+    }
+    const float sdIn_dr = rtOf(hInUp.x - hInLo.x, hInUp.y - hInLo.y);
+    const float sdIn_d = hInUp.rt - hInLo.rt;
+    const float dr = rtOf(dr3x, dr3y);
+    if (!EEEE) {
+        const float betaIn_cut = asinF(fmin2((-sdIn_dr * 1.f + dr) * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdIn_d);
+        if (!(fabsF(betaInRHmin) < betaIn_cut)) return false;
+    }  // in EEEE the betaIn cut never rejects (A-11)
+    float betaAv = 0.5f * (betaIn + betaOut);
+    float pt_beta = float(double(dr * k2) / sin(double(betaAv)));
+    const int lIn = EEEE ? 11 : 5;
+    const float sdOut_dr = rtOf(hOutUp.x - hOutLo.x, hOutUp.y - hOutLo.y);
+    const float sdOut_d = hOutUp.rt - hOutLo.rt;
+    deltaBetaIter(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn);
+    rescaleRH(betaIn, betaInRHmin, betaInRHmax);
+    rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
+    const float dBetaMuls = thetaMuls * 4.f / fmin2(fabsF(pt_beta), 7.0f);
+    const float alphaOutAbsReg = fmax2(fabsF(sdOut_alpha), asinF(fmin2(hOutLo.rt * k2 / 3.0f, kSinAlphaMax)));
+    float dBetaInLum = 0.0f;
+    if (EEEE) {
+        const float alphaInAbsReg = fmax2(fabsF(sdIn_alpha), asinF(fmin2(hInLo.rt * k2 / 3.0f, kSinAlphaMax)));
+        dBetaInLum = fabsF(alphaInAbsReg * kDeltaZLum / hInLo.z);
+    }
+    const float dBetaOutLum = fabsF(alphaOutAbsReg * kDeltaZLum / hOutLo.z);
+    const float dBetaLum2 = (dBetaInLum + dBetaOutLum) * (dBetaInLum + dBetaOutLum);
+    float dBetaROut2 = 0.f;
+    if (!EEEE) {
+        const float sinDPhi = sinF(dPhiMid);
+        const bool out2S = mods[sOut.out.mod].type == TYPE_2S;
+        const float hiRt = out2S ? outUpHi.rt : hOutUp.rt;
+        const float loRt = out2S ? outUpLo.rt : hOutUp.rt;
+        dBetaROut2 = float(sq(double((hiRt - loRt) * sinDPhi / dr)));
+    }
+    const float betaOut_cut = float(double(asinF(fmin2(dr * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
+                                    sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
+    if (!(fabsF(betaOut) < betaOut_cut)) return false;
+    const float dBetaRes = 0.02f / fmin2(sdOut_d, sdIn_d);
+    const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
+                                  0.25 * sq(double(fabsF(betaInRHmin - betaInRHmax) + fabsF(betaOutRHmin - betaOutRHmax))));
+    const float dBeta = betaIn - betaOut;
+    return dBeta * dBeta <= dBetaCut2;
+}
+
+// Tracklet::runTrackletDefaultAlgo dispatch on the four anchor subdets (Tracklet.cc:144-296)
+__device__ __forceinline__ bool tlAlgo(const SGView& sIn, const SGView& sOut, const ModDev* mods,
+                                       const HitView& outUpHi, const HitView& outUpLo) {
+    const bool b0 = mods[sIn.in.mod].subdet == SUB_BARREL;
+    const bool b1 = mods[sIn.out.mod].subdet == SUB_BARREL;
+    const bool b2 = mods[sOut.in.mod].subdet == SUB_BARREL;
+    const bool b3 = mods[sOut.out.mod].subdet == SUB_BARREL;
+    if (b0 && b1 && b2) return tlBBBB(sIn, sOut, mods);
+    if (b0 && !b2 && !b3) return tlEndcapFamily<false>(sIn, sOut, mods, outUpHi, outUpLo);  // BBEE, BEEE
+    if (!b0 && !b1 && !b2 && !b3) return tlEndcapFamily<true>(sIn, sOut, mods, outUpHi, outUpLo);
+    return false;  // other mixes fail (A-14)
+}
+
+}  // namespace sdl

@@ -1,0 +1,168 @@
+"""ctypes binding of the C-ABI in include/sdl_amd.h (libsdl_amd.so).
+
+Host plumbing only: every computation runs in the HIP kernels of the library.
+Loading fails loudly when the library has not been built."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libsdl_amd.so")
+
+STAGE_MD, STAGE_SG, STAGE_TP, STAGE_TL, STAGE_TCP, STAGE_TCL = range(6)
+STAGES = ("MD", "SG", "TP", "TL", "TCP", "TCL")
+WIDTH = {"MD": 2, "SG": 4, "TP": 8, "TL": 8, "TCP": 16, "TCL": 16}
+COUNTER_NAMES = ("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")
+
+# every entry point declared in include/sdl_amd.h
+EXPORTS = (
+    "sdl_last_error", "sdl_geom_load", "sdl_geom_free", "sdl_geom_num_modules", "sdl_event_create",
+    "sdl_event_destroy", "sdl_event_reset", "sdl_event_add_hits", "sdl_event_next", "sdl_event_set_hits_device",
+    "sdl_event_run", "sdl_event_run_chain", "sdl_event_sync", "sdl_event_counts", "sdl_event_layer_counts",
+    "sdl_event_export", "sdl_event_num_events", "sdl_event_stream", "sdl_event_set_timing", "sdl_event_kernel_ms",
+)
+TIMERS = ("prep", "md", "sg", "tl")
+
+_lib = None
+
+
+class SDLError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SDLError(f"{LIB_PATH} is missing: the HIP library must be built (__graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
+        L.sdl_last_error.restype = ctypes.c_char_p
+        L.sdl_geom_load.argtypes = [ctypes.c_char_p] * 3 + [ctypes.POINTER(vp)]
+        L.sdl_geom_free.argtypes = [vp]
+        L.sdl_geom_num_modules.argtypes = [vp]
+        L.sdl_event_create.argtypes = [vp, ctypes.POINTER(vp)]
+        L.sdl_event_destroy.argtypes = [vp]
+        L.sdl_event_reset.argtypes = [vp]
+        L.sdl_event_add_hits.argtypes = [vp, i] + [vp] * 5
+        L.sdl_event_next.argtypes = [vp]
+        L.sdl_event_set_hits_device.argtypes = [vp, i, vp] + [vp] * 5
+        L.sdl_event_run.argtypes = [vp, i, i]
+        L.sdl_event_run_chain.argtypes = [vp]
+        L.sdl_event_sync.argtypes = [vp]
+        L.sdl_event_counts.argtypes = [vp, i, vp]
+        L.sdl_event_layer_counts.argtypes = [vp, i, vp]
+        L.sdl_event_export.argtypes = [vp, i, i, vp, ll, ctypes.POINTER(ll)]
+        L.sdl_event_num_events.argtypes = [vp]
+        L.sdl_event_stream.argtypes = [vp]
+        L.sdl_event_stream.restype = vp
+        L.sdl_event_set_timing.argtypes = [vp, i]
+        L.sdl_event_kernel_ms.argtypes = [vp, vp, i]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SDLError(f"sdl error {rc}: {lib().sdl_last_error().decode()}")
+
+
+class Geometry:
+    def __init__(self, tilted, endcap, cmap):
+        self.h = ctypes.c_void_p()
+        _check(lib().sdl_geom_load(tilted.encode(), endcap.encode(), cmap.encode(), ctypes.byref(self.h)))
+
+    @property
+    def num_modules(self):
+        return lib().sdl_geom_num_modules(self.h)
+
+    def close(self):
+        if self.h:
+            lib().sdl_geom_free(self.h)
+            self.h = ctypes.c_void_p()
+
+
+class Event:
+    """A batch of one or more events on one HIP stream."""
+
+    def __init__(self, geom):
+        self.geom = geom
+        self.h = ctypes.c_void_p()
+        _check(lib().sdl_event_create(geom.h, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().sdl_event_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        _check(lib().sdl_event_reset(self.h))
+
+    def add_hits(self, x, y, z, idx, detId):
+        """Host arrays (float32 x3, int32, uint32), appended to the current event."""
+        arrs = [np.ascontiguousarray(a, dt) for a, dt in
+                ((x, np.float32), (y, np.float32), (z, np.float32), (idx, np.int32), (detId, np.uint32))]
+        self._keep = arrs
+        _check(lib().sdl_event_add_hits(self.h, len(arrs[0]), *[a.ctypes.data for a in arrs]))
+
+    def next_event(self):
+        _check(lib().sdl_event_next(self.h))
+
+    def set_hits_device(self, offsets, x_ptr, y_ptr, z_ptr, idx_ptr, det_ptr):
+        off = np.ascontiguousarray(offsets, np.int64)
+        self._off = off
+        _check(lib().sdl_event_set_hits_device(self.h, len(off) - 1, off.ctypes.data, x_ptr, y_ptr, z_ptr, idx_ptr, det_ptr))
+
+    def run(self, stage, algo=0):
+        _check(lib().sdl_event_run(self.h, stage, algo))
+
+    def run_chain(self):
+        _check(lib().sdl_event_run_chain(self.h))
+
+    def sync(self):
+        _check(lib().sdl_event_sync(self.h))
+
+    @property
+    def num_events(self):
+        return lib().sdl_event_num_events(self.h)
+
+    @property
+    def stream(self):
+        return lib().sdl_event_stream(self.h)
+
+    def counts(self, e=-1):
+        out = np.zeros(10, np.uint64)
+        _check(lib().sdl_event_counts(self.h, e, out.ctypes.data))
+        return dict(zip(COUNTER_NAMES, [int(v) for v in out]))
+
+    def layer_counts(self, e=0):
+        out = np.zeros(110, np.uint64)
+        _check(lib().sdl_event_layer_counts(self.h, e, out.ctypes.data))
+        return out.reshape(10, 11)
+
+    def export(self, stage, e=0):
+        name = STAGES[stage]
+        n = ctypes.c_longlong(0)
+        _check(lib().sdl_event_export(self.h, e, stage, None, 0, ctypes.byref(n)))
+        out = np.empty((n.value, WIDTH[name]), np.int32)
+        if n.value:
+            _check(lib().sdl_event_export(self.h, e, stage, out.ctypes.data, out.size, ctypes.byref(n)))
+        return out
+
+    def set_timing(self, on=True):
+        _check(lib().sdl_event_set_timing(self.h, 1 if on else 0))
+
+    def kernel_ms(self):
+        out = np.zeros(len(TIMERS), np.float64)
+        _check(lib().sdl_event_kernel_ms(self.h, out.ctypes.data, len(TIMERS)))
+        return dict(zip(TIMERS, out.tolist()))
+
+    def export_all(self, e=0):
+        return {s: self.export(i, e) for i, s in enumerate(STAGES)}

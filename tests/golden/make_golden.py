@@ -1,0 +1,43 @@
+"""Writes the golden fixtures of the SDL chain from the compiled reference.
+
+Runs in the build container only (it needs /root/reference): builds
+oracle/_ref (the unmodified reference sources + this repository's driver,
+oracle/ref_driver.cc) through oracle/Makefile, runs the reference's module-map
+chain on seeded generated events and stores every object set as hit-index
+tuples plus the stage counters in ``tests/golden/ref_sets.npz``.
+
+Usage:  python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle  # noqa: E402
+
+# (workload, first event, number of events)
+CASES = [(1, 0, 2), (12, 0, 2), (15, 0, 1)]
+
+
+def main():
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for wl, e0, ne in CASES:
+            dump = os.path.join(tmp, f"w{wl}.txt")
+            subprocess.run([oracle.REF_DRIVER, "dump", str(wl), tmp, str(e0), str(ne), dump], check=True)
+            for e, (sets, counters) in oracle.parse_ref_dump(dump).items():
+                for s in oracle.STAGES:
+                    out[f"w{wl}_e{e}_{s}"] = oracle.as_multiset(sets[s])
+                out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in (
+                    "nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")], np.int64)
+    np.savez_compressed(os.path.join(HERE, "ref_sets.npz"), **out)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,147 @@
+"""Parity of the HIP chain (through the C-ABI) with the CPU oracle.
+
+The device evaluates the float transcendentals correctly rounded, so it is
+compared bit for bit with the oracle's CR policy; test_oracle.py relates that
+policy to the compiled reference.  Objects are compared as multisets of
+hit-index tuples (MD 2, SG 4, TP/TL 8, TC 16 indices)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from sdl_amd import gen, runtime
+
+pytestmark = pytest.mark.gpu
+
+SMALL = [(1, 0), (1, 1), (1, 2), (12, 0), (12, 1), (15, 0), (15, 1)]
+
+
+def _geom(kind, cache={}):
+    if kind not in cache:
+        cache[kind] = runtime.Geometry(*gen.geometry(kind)[1])
+    return cache[kind]
+
+
+def _run_gpu(wl, events):
+    g = _geom(gen.geom_kind(wl))
+    ev = runtime.Event(g)
+    for i, e in enumerate(events):
+        if i:
+            ev.next_event()
+        ev.add_hits(e.x, e.y, e.z, e.idx, e.detId)
+    ev.run_chain()
+    ev.sync()
+    return ev
+
+
+def _assert_same(got_sets, ref_sets, what):
+    for s in oracle.STAGES:
+        a = oracle.as_multiset(got_sets[s])
+        b = oracle.as_multiset(ref_sets[s])
+        assert a.shape == b.shape, f"{what} {s}: gpu {a.shape} oracle {b.shape}"
+        assert np.array_equal(a, b), f"{what} {s} differs"
+
+
+@pytest.mark.parametrize("wl,e", SMALL)
+def test_chain_matches_oracle(wl, e):
+    print(f"case workload={wl} event={e}", flush=True)
+    ev_in = gen.event(wl, e)
+    ev = _run_gpu(wl, [ev_in])
+    got = ev.export_all(0)
+    ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], ev_in, oracle.CR)
+    _assert_same(got, ref, f"w{wl}e{e}")
+    assert ev.counts(0) == cnt
+
+
+@pytest.mark.parametrize("wl,e", [(1, 0), (12, 0), (15, 0)])
+def test_chain_matches_reference_golden(wl, e, golden):
+    """Where the two libm policies agree (checked on the CPU), the device must
+    reproduce the compiled reference's sets directly."""
+    print(f"case golden workload={wl} event={e}", flush=True)
+    ev = _run_gpu(wl, [gen.event(wl, e)])
+    got = ev.export_all(0)
+    for s in oracle.STAGES:
+        assert np.array_equal(oracle.as_multiset(got[s]), golden[f"w{wl}_e{e}_{s}"]), s
+
+
+def test_batch_equals_single_events():
+    print("case batch of 4 workload-12 events", flush=True)
+    evs = [gen.event(12, e) for e in range(4)]
+    batch = _run_gpu(12, evs)
+    assert batch.num_events == 4
+    for i, e in enumerate(evs):
+        single = _run_gpu(12, [e])
+        _assert_same(batch.export_all(i), single.export_all(0), f"batch event {i}")
+        assert batch.counts(i) == single.counts(0)
+        assert np.array_equal(batch.layer_counts(i), single.layer_counts(0))
+
+
+def test_device_resident_input_and_rerun():
+    import torch
+    print("case device-resident input, workload 15 event 1", flush=True)
+    e = gen.event(15, 1)
+    dev = [torch.from_numpy(a).cuda() for a in (e.x, e.y, e.z, e.idx, e.detId.view(np.int32))]
+    torch.cuda.synchronize()
+    g = _geom(2)
+    ev = runtime.Event(g)
+    ev.set_hits_device([0, e.nhits], *[t.data_ptr() for t in dev])
+    ref = _run_gpu(15, [e]).export_all(0)
+    for rep in range(2):  # the handle reuses its device buffers
+        ev.run_chain()
+        ev.sync()
+        _assert_same(ev.export_all(0), ref, f"device input rep {rep}")
+
+
+def test_stage_order_and_errors():
+    print("case error paths", flush=True)
+    g = _geom(1)
+    ev = runtime.Event(g)
+    e = gen.event(1, 0)
+    ev.add_hits(e.x, e.y, e.z, e.idx, e.detId)
+    with pytest.raises(runtime.SDLError):
+        ev.run(runtime.STAGE_TL)  # stages must run in order
+    ev.run(runtime.STAGE_MD)
+    ev.run(runtime.STAGE_SG)
+    with pytest.raises(runtime.SDLError):
+        ev.export(runtime.STAGE_TL)  # not run yet
+    bad = runtime.Event(g)
+    bad.add_hits(e.x[:5], e.y[:5], e.z[:5], e.idx[:5], np.full(5, 7, np.uint32))
+    with pytest.raises(runtime.SDLError, match="unknown"):
+        bad.run_chain()
+
+
+def test_empty_event():
+    print("case empty event", flush=True)
+    g = _geom(1)
+    ev = runtime.Event(g)
+    e = gen.event(1, 0)
+    ev.add_hits(e.x[:0], e.y[:0], e.z[:0], e.idx[:0], e.detId[:0])
+    ev.run_chain()
+    ev.sync()
+    assert all(len(v) == 0 for v in ev.export_all(0).values())
+    assert all(v == 0 for v in ev.counts(0).values())
+
+
+def test_duplicate_connection_rejected(tmp_path):
+    print("case duplicate connection", flush=True)
+    t, e, m = gen.geometry(1)[1]
+    lines = open(m).read().splitlines()
+    first = lines[0].split()
+    first = [first[0], str(int(first[1]) + 1)] + first[2:] + [first[2]]
+    bad = tmp_path / "map.txt"
+    bad.write_text("\n".join([" ".join(first)] + lines[1:]) + "\n")
+    with pytest.raises(runtime.SDLError, match="repeats"):
+        runtime.Geometry(t, e, str(bad))
+
+
+def test_pu200_event_full_parity():
+    """One full PU200-density event (workload 2, ~1.1e5 hits): every object set
+    and counter equals the oracle's."""
+    print("case full PU200 event, workload 2 event 0", flush=True)
+    e = gen.event(2, 0)
+    ev = _run_gpu(2, [e])
+    got = ev.export_all(0)
+    ref, cnt = oracle.run(gen.geometry(2)[1], e, oracle.CR)
+    _assert_same(got, ref, "pu200")
+    assert ev.counts(0) == cnt

@@ -1,0 +1,76 @@
+"""The CPU oracle against the compiled reference's golden object sets, and
+the libm policy comparison (CPU only)."""
+import numpy as np
+import pytest
+
+import oracle
+from sdl_amd import gen
+
+CASES = [(1, 0), (1, 1), (12, 0), (12, 1), (15, 0)]
+COUNTERS = ("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")
+
+
+@pytest.mark.parametrize("wl,e", CASES)
+def test_oracle_matches_reference(wl, e, geom_paths, golden):
+    ev = gen.event(wl, e)
+    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC)
+    for s in oracle.STAGES:
+        ref = golden[f"w{wl}_e{e}_{s}"]
+        got = oracle.as_multiset(sets[s])
+        assert got.shape == ref.shape, (s, got.shape, ref.shape)
+        assert np.array_equal(got, ref), s
+        assert len(ref) > 0, f"{s}: empty golden set makes the check vacuous"
+    assert [counters[k] for k in COUNTERS] == list(golden[f"w{wl}_e{e}_counters"])
+
+
+@pytest.mark.parametrize("wl,e", CASES)
+def test_libm_policies(wl, e, geom_paths):
+    """Correctly rounded float transcendentals (the device's) against glibc's
+    (the reference's): any object that differs must have its deciding cut
+    variable within 4 FP32 ulp of the threshold."""
+    paths = geom_paths[gen.geom_kind(wl)]
+    ev = gen.event(wl, e)
+    a, _ = oracle.run(paths, ev, oracle.GLIBC)
+    b, _ = oracle.run(paths, ev, oracle.CR)
+    common = {}
+    for s, kind in (("MD", 0), ("SG", 1), ("TL", 2)):
+        sa = {tuple(r) for r in a[s]}
+        sb = {tuple(r) for r in b[s]}
+        common[s] = sa & sb
+        for t in sa ^ sb:
+            # an object whose inputs already differ is a consequence, not a new difference
+            if kind == 1 and not (t[:2] in common["MD"] and t[2:] in common["MD"]):
+                continue
+            if kind == 2 and not (t[:4] in common["SG"] and t[4:] in common["SG"]):
+                continue
+            if kind == 0:
+                info = oracle.explain(paths, ev, 0, t)
+            elif kind == 1:
+                info = oracle.explain(paths, ev, 1, t)
+            else:
+                info = oracle.explain(paths, ev, 2, t[:4], t[4:])
+            assert info["ulps"] <= 4.0, (s, t, info)
+
+
+def test_generator_is_deterministic():
+    a = gen.event(12, 3)
+    b = gen.event(12, 3)
+    c = gen.event(12, 4)
+    for f in ("x", "y", "z", "detId", "idx", "sim"):
+        assert np.array_equal(getattr(a, f), getattr(b, f))
+    assert not np.array_equal(a.x[:100], c.x[:100])
+    assert np.array_equal(a.idx, np.arange(a.nhits))
+
+
+def test_pow_square_is_exact():
+    """pow(float, 2) in double equals the exact product, which the device uses."""
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.pow.restype = ctypes.c_double
+    libm.pow.argtypes = [ctypes.c_double, ctypes.c_double]
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.standard_normal(2000).astype(np.float32) * 10 ** rng.uniform(-8, 4, 2000).astype(np.float32),
+                         np.float32([0.0, -0.0, 1e-30, 3.4e18])])
+    for x in xs:
+        d = float(x)
+        assert libm.pow(d, 2.0) == d * d
