@@ -57,24 +57,21 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the chain shards by event, "
               "see the design notes, but no multi-GPU run is part of this build.", file=sys.stderr)
         sys.exit(2)
-    import torch
-    from sdl_amd import gen, runtime
+    from sdl_amd import gen, hip, runtime
 
-    torch.cuda.set_device(0)
     B = a.events_per_step
     evs = [gen.event(a.workload, i) for i in range(B)]
     off = np.zeros(B + 1, np.int64)
     off[1:] = np.cumsum([e.nhits for e in evs])
     cat = lambda f: np.ascontiguousarray(np.concatenate([getattr(e, f) for e in evs]))
-    dev = [torch.from_numpy(cat(f)).cuda() for f in ("x", "y", "z", "idx")]
-    dev.append(torch.from_numpy(cat("detId").view(np.int32)).cuda())
-    torch.cuda.synchronize()
+    dev = [hip.DeviceArray(cat(f)) for f in ("x", "y", "z", "idx", "detId")]
+    hip.synchronize()
 
     geom = runtime.Geometry(*gen.geometry(gen.geom_kind(a.workload))[1])
     ev = runtime.Event(geom)
     ev.set_timing(True)
-    ptrs = [t.data_ptr() for t in dev]
-    stream = torch.cuda.ExternalStream(ev.stream)
+    ptrs = [d.address for d in dev]
+    stream = ev.stream
 
     def step():
         ev.set_hits_device(off, *ptrs)
@@ -83,10 +80,9 @@ def main():
     for _ in range(a.warmup):
         step()
     ev.sync()
-    torch.cuda.synchronize()
+    hip.synchronize()
     kms = {k: 0.0 for k in runtime.TIMERS}
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
+    start, stop = hip.Event(), hip.Event()
     t0 = time.perf_counter()
     start.record(stream)
     for _ in range(a.steps):
@@ -95,9 +91,9 @@ def main():
             kms[k] += v
     stop.record(stream)
     ev.sync()
-    torch.cuda.synchronize()
+    hip.synchronize()
     wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(stop)
+    gpu_ms = start.elapsed_ms(stop)
     ms_per_step = 1000.0 * wall / a.steps
     counts = ev.counts(-1)
     for k in kms:

@@ -78,14 +78,14 @@ def test_batch_equals_single_events():
 
 
 def test_device_resident_input_and_rerun():
-    import torch
+    from sdl_amd import hip
     print("case device-resident input, workload 15 event 1", flush=True)
     e = gen.event(15, 1)
-    dev = [torch.from_numpy(a).cuda() for a in (e.x, e.y, e.z, e.idx, e.detId.view(np.int32))]
-    torch.cuda.synchronize()
+    dev = [hip.DeviceArray(a) for a in (e.x, e.y, e.z, e.idx, e.detId)]
+    hip.synchronize()
     g = _geom(2)
     ev = runtime.Event(g)
-    ev.set_hits_device([0, e.nhits], *[t.data_ptr() for t in dev])
+    ev.set_hits_device([0, e.nhits], *[d.address for d in dev])
     ref = _run_gpu(15, [e]).export_all(0)
     for rep in range(2):  # the handle reuses its device buffers
         ev.run_chain()
