@@ -62,9 +62,17 @@ def build_shim():
     return so
 
 
+def build_calib():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "copy_kernel")
+    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "calib", "copy_kernel.hip"), "-o", out])
+    return out
+
+
 def build_all(with_shim=True):
     build_gen()
     build_hip()
+    build_calib()
     if with_shim and os.path.exists(os.path.join(CSRC, "shim", "sdl_shim.cc")):
         build_shim()
 

@@ -150,8 +150,8 @@ int scanExclusive(const TIn* in, long long* out, long long n, ScanScratch& sc, h
 // ------------------------------------------------------------------ hits
 __global__ void k_hit_prep(int nh, const float* x, const float* y, const float* z, const unsigned* detId,
                            const long long* evOff, int E, const uint32_t* sortedDet, int NM, const ModDev* mods,
-                           float* rt, float* r3, float* phi, int* slot, float4* eHi, float4* eLo, int* slotCount,
-                           int* unknown) {
+                           float* rt, float* r3, float* phi, float* alpha, int* slot, float4* eHi, float4* eLo,
+                           int* slotCount, int* unknown) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nh) return;
     // event of the hit
@@ -174,14 +174,16 @@ __global__ void k_hit_prep(int nh, const float* x, const float* y, const float* 
         return;
     }
     const float xx = x[i], yy = y[i], zz = z[i];
-    rt[i] = rtOf(xx, yy);
+    const float rtv = rtOf(xx, yy);
+    rt[i] = rtv;
     r3[i] = r3Of(xx, yy, zz);
     phi[i] = phiOf(xx, yy);
+    alpha[i] = asinF(fmin2(rtv * k2Rinv() / kPtCut, kSinAlphaMax));
     const int s = e * NM + a;
     slot[i] = s;
     const ModDev m = mods[a];
     if (m.subdet == SUB_ENDCAP && m.type == TYPE_2S) {
-        HitView h{xx, yy, zz, 0, 0, 0};
+        HitView h{xx, yy, zz, 0, 0, 0, 0};
         HitView hh = edgeOf(h, m.centroidPhi, true);
         HitView hl = edgeOf(h, m.centroidPhi, false);
         eHi[i] = make_float4(hh.x, hh.y, hh.rt, hh.phi);
@@ -199,11 +201,28 @@ __global__ void k_hit_scatter(int nh, const int* slot, const long long* slotStar
     sortedHit[slotStart[s] + pos] = i;
 }
 
-// Restores insertion order inside each slot (Module::addHit appends, Module.cc:128-137).
+// Sorts each segment ascending: restores insertion order inside each slot
+// (Module::addHit appends, Module.cc:128-137) and makes atomically scattered
+// lists deterministic.  One wave per segment; up to 64 elements are ranked in
+// registers, longer segments are sorted by the wave's first lane.
 __global__ void k_sort_segments(int nseg, const long long* start, int* vals) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (s >= nseg) return;
     const long long a = start[s], b = start[s + 1];
+    const int n = int(b - a);
+    if (n <= 1) return;
+    if (n <= kWave) {
+        const int v = lane < n ? vals[a + lane] : INT_MAX;
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const int o = __shfl(v, j);
+            rank += (o < v) || (o == v && j < lane);
+        }
+        if (lane < n) vals[a + rank] = v;
+        return;
+    }
+    if (lane != 0) return;
     for (long long i = a + 1; i < b; ++i) {
         int v = vals[i];
         long long j = i - 1;
@@ -216,8 +235,8 @@ __global__ void k_sort_segments(int nseg, const long long* start, int* vals) {
 }
 
 __device__ __forceinline__ HitView loadHit(int h, const float* x, const float* y, const float* z, const float* rt,
-                                           const float* r3, const float* phi) {
-    return HitView{x[h], y[h], z[h], rt[h], r3[h], phi[h]};
+                                           const float* r3, const float* phi, const float* alpha) {
+    return HitView{x[h], y[h], z[h], rt[h], r3[h], phi[h], alpha[h]};
 }
 
 // ------------------------------------------------------------------ MD
@@ -238,7 +257,7 @@ __global__ void k_md_cap(int NS, int NM, const ModDev* mods, const long long* sl
 }
 
 struct HitsDev {
-    const float *x, *y, *z, *rt, *r3, *phi;
+    const float *x, *y, *z, *rt, *r3, *phi, *alpha;
     const int* idx;
     const float4 *eHi, *eLo;
 };
@@ -271,8 +290,8 @@ __global__ void __launch_bounds__(kBlock) k_md(int NS, int NM, const ModDev* mod
             const int i = k / nUp, j = k - (k / nUp) * nUp;
             lo = sortedHit[loStart + i];
             up = sortedHit[upStart + j];
-            const HitView hl = loadHit(lo, H.x, H.y, H.z, H.rt, H.r3, H.phi);
-            const HitView hu = loadHit(up, H.x, H.y, H.z, H.rt, H.r3, H.phi);
+            const HitView hl = loadHit(lo, H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
+            const HitView hu = loadHit(up, H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
             pass = mdAlgo(hl, hu, md, o);
         }
         const unsigned long long mask = __ballot(pass);
@@ -291,6 +310,7 @@ struct MDDev {
     float4* a;   // anchor x, y, z, rt
     float4* b;   // anchor r3, phi; dz, shiftedDz
     float2* c;   // dphi, dphichange
+    float4* p;   // alpha1 = asin(min(rt k2, .95)), tanf(alpha1), tan(alpha1)/alpha1 (double), asin(min(rt k2 / 3, .95))
 };
 
 // Dense repack; the anchor is the pixel hit of a PS module, else the lower hit
@@ -315,23 +335,34 @@ __global__ void k_md_repack(int NS, int NM, const ModDev* mods, HitsDev H, const
         D.up[o] = h.y;
         D.anchor[o] = an;
         D.slot[o] = item;
-        D.a[o] = make_float4(H.x[an], H.y[an], H.z[an], H.rt[an]);
+        const float rta = H.rt[an];
+        const float a1 = H.alpha[an];
+        D.a[o] = make_float4(H.x[an], H.y[an], H.z[an], rta);
         D.b[o] = make_float4(H.r3[an], H.phi[an], f.x, f.y);
         D.c[o] = make_float2(f.z, f.w);
+        D.p[o] = make_float4(a1, tanF(a1), float(tan(double(a1)) / double(a1)),
+                             asinF(fmin2(rta * k2Rinv() / 3.0f, kSinAlphaMax)));
     }
 }
 
 __device__ __forceinline__ MDView loadMD(int i, const MDDev& D) {
     const float4 a = D.a[i], b = D.b[i];
     const float2 c = D.c[i];
+    const float4 p = D.p[i];
     MDView v;
-    v.a = HitView{a.x, a.y, a.z, a.w, b.x, b.y};
+    v.a = HitView{a.x, a.y, a.z, a.w, b.x, b.y, p.x};
     v.dz = b.z;
     v.shiftedDz = b.w;
     v.dphi = c.x;
     v.dphichange = c.y;
+    v.alpha1 = p.x;
+    v.tanAlpha1 = p.y;
     v.mod = 0;
     return v;
+}
+__device__ __forceinline__ HitView loadAnchor(int i, const MDDev& D) {
+    const float4 a = D.a[i], b = D.b[i];
+    return HitView{a.x, a.y, a.z, a.w, b.x, b.y, 0.f};
 }
 
 // ------------------------------------------------------------------ SG
@@ -436,7 +467,32 @@ __global__ void __launch_bounds__(kBlock) k_sg(int nMD, int NM, const ModDev* mo
 struct SGDev {
     int *in, *out;
     SGF* f;
+    float4* x;    // rt(out - in), out.rt - in.rt, deltaPhi(out, out - in), deltaPhi(in, out)   (anchors)
+    float2* e;    // deltaPhi(edge, edge - in) for the high / low strip edges of an endcap-2S outer anchor
+    int* flags;   // 1 inner barrel, 2 outer barrel, 4 inner PS, 8 outer PS, 16 outer endcap 2S
 };
+
+// Per-segment quantities that the tracklet algebra reads for either role.
+__global__ void k_sg_extra(int nSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSG) return;
+    const int mi = S.in[s], mo = S.out[s];
+    const HitView a = loadAnchor(mi, D), b = loadAnchor(mo, D);
+    const ModDev Mi = mods[D.slot[mi] % NM], Mo = mods[D.slot[mo] % NM];
+    const float dx = b.x - a.x, dy = b.y - a.y;
+    S.x[s] = make_float4(rtOf(dx, dy), b.rt - a.rt, phiMpiPi(phiOf(dx, dy) - b.phi), dPhi(a.phi, b.phi));
+    const bool ec2s = Mo.subdet == SUB_ENDCAP && Mo.type == TYPE_2S;
+    S.flags[s] = (Mi.subdet == SUB_BARREL ? 1 : 0) | (Mo.subdet == SUB_BARREL ? 2 : 0) | (Mi.type == TYPE_PS ? 4 : 0) |
+                 (Mo.type == TYPE_PS ? 8 : 0) | (ec2s ? 16 : 0);
+    float2 e = make_float2(0.f, 0.f);
+    if (ec2s) {
+        const int an = D.anchor[mo];
+        const float4 hi = H.eHi[an], lo = H.eLo[an];
+        e.x = phiMpiPi(phiOf(hi.x - a.x, hi.y - a.y) - hi.w);
+        e.y = phiMpiPi(phiOf(lo.x - a.x, lo.y - a.y) - lo.w);
+    }
+    S.e[s] = e;
+}
 
 __global__ void k_sg_repack(int nMD, const long long* sgCapOff, const int* sgCount, const long long* sgStart,
                             const int* slabOut, const SGF* slabF, SGDev S) {
@@ -512,24 +568,41 @@ __global__ void k_tl_cap(int nSG, int NM, const int* sgOut, const int* mdSlot, c
     tlCap[s] = c;
 }
 
-__device__ __forceinline__ SGView loadSG(int s, const SGDev& S, const MDDev& D, int NM) {
-    SGView v;
-    const int a = S.in[s], b = S.out[s];
-    v.in = loadMD(a, D);
-    v.out = loadMD(b, D);
-    v.in.mod = D.slot[a] % NM;
-    v.out.mod = D.slot[b] % NM;
-    const SGF f = S.f[s];
-    v.dphi = f.v[0]; v.dphimin = f.v[1]; v.dphimax = f.v[2];
-    v.dphichange = f.v[3]; v.dphichangemin = f.v[4]; v.dphichangemax = f.v[5];
-    return v;
+// One wave per inner segment.  Phase A walks the MDs of every module connected to
+// the inner segment's outer-MD module (the outer segments' possible inner MDs)
+// and applies the first-stage cuts once per (inner segment, MD) pair; the
+// surviving MDs go to a per-wave LDS queue.  Phase B expands the queue into the
+// outer segments starting at those MDs and applies the second stage with all
+// lanes busy.  Survivors are written in the reference's candidate order.
+struct TLQueue {
+    int mLo[kWave], sgA[kWave], nS[kWave], variant[kWave];
+    float sdlCut[kWave], thetaMuls[kWave], dPhiMid[kWave];
+};
+
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
-__global__ void __launch_bounds__(kBlock) k_tl(int nSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
-                                               const int* connStart, const int* connList, const long long* slotSgStart,
-                                               const long long* tlCap, const long long* tlCapOff, int* slabOut,
-                                               int* tlCount) {
-    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+__device__ __forceinline__ TLOutLo loadOutLo(int mLo, int flags, const MDDev& D) {
+    TLOutLo L;
+    L.a = loadAnchor(mLo, D);
+    const float4 p = D.p[mLo];
+    L.alpha1 = p.x;
+    L.dzScale = p.z;
+    L.alphaReg3 = p.w;
+    L.barrel = flags & 1;
+    L.ps = flags & 4;
+    return L;
+}
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_tl(int nSG, int NM, HitsDev H, MDDev D, SGDev S, const int* connStart,
+                                               const int* connList, const long long* mdStart, const int* mdCount,
+                                               const long long* sgStart, const long long* tlCap,
+                                               const long long* tlCapOff, int* slabOut, int* tlCount) {
+    __shared__ TLQueue queues[kWavesPerBlock];
+    TLQueue& Q = queues[threadIdx.x >> 6];
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     if (item >= nSG) return;
     const long long cap = tlCap[item];
@@ -537,8 +610,21 @@ __global__ void __launch_bounds__(kBlock) k_tl(int nSG, int NM, const ModDev* mo
         if (lane == 0) tlCount[item] = 0;
         return;
     }
-    const SGView sIn = loadSG(item, S, D, NM);
-    const int b = D.slot[S.out[item]];
+    TLInner I;
+    const int mi = S.in[item], mo = S.out[item];
+    I.lo = loadAnchor(mi, D);
+    I.up = loadAnchor(mo, D);
+    const SGF fi = S.f[item];
+    I.dphichange = fi.v[3];
+    I.dphichangemin = fi.v[4];
+    I.dphichangemax = fi.v[5];
+    const float4 xi = S.x[item];
+    I.dr = xi.x;
+    I.d = xi.y;
+    I.alphaReg3Lo = D.p[mi].w;
+    const int fl = S.flags[item];
+    I.bLo = fl & 1; I.bUp = fl & 2; I.psLo = fl & 4; I.psUp = fl & 8;
+    const int b = D.slot[mo];
     const int m = b % NM, eb = b - m;
     const int c0 = connStart[m], nc = connStart[m + 1] - c0;
     const long long off = tlCapOff[item];
@@ -546,42 +632,88 @@ __global__ void __launch_bounds__(kBlock) k_tl(int nSG, int NM, const ModDev* mo
     for (int cb = 0; cb < nc; cb += kWave) {
         const int ne = min(kWave, nc - cb);
         int cnt = 0;
-        if (lane < ne) {
-            const int t = eb + connList[c0 + cb + lane];
-            cnt = int(slotSgStart[t + 1] - slotSgStart[t]);
-        }
+        if (lane < ne) cnt = mdCount[eb + connList[c0 + cb + lane]];
         int pre = cnt;
         for (int o = 1; o < kWave; o <<= 1) {
             const int t = __shfl_up(pre, o);
             if (lane >= o) pre += t;
         }
-        const int total = __shfl(pre, kWave - 1);
-        for (int k0 = 0; k0 < total; k0 += kWave) {
-            const int k = k0 + lane;
-            bool pass = false;
-            int so = 0;
+        const int totalA = __shfl(pre, kWave - 1);
+        for (int kA0 = 0; kA0 < totalA; kA0 += kWave) {
+            // ---- phase A: (inner segment, outer-inner MD) pairs
+            const int k = kA0 + lane;
             int inner;
             const int e = findEntry(k, pre, ne, inner);
-            if (k < total) {
+            bool ok = false;
+            int mLo = 0, sgA = 0, nS = 0, variant = TL_NONE;
+            TLCarry c{0.f, 0.f, 0.f};
+            if (k < totalA) {
                 const int t = eb + connList[c0 + cb + e];
-                so = int(slotSgStart[t]) + inner;
-                const SGView sOut = loadSG(so, S, D, NM);
-                HitView hiE = sOut.out.a, loE = sOut.out.a;
-                const ModDev mOU = mods[sOut.out.mod];
-                if (mOU.subdet == SUB_ENDCAP && mOU.type == TYPE_2S) {
-                    const int an = D.anchor[S.out[so]];
-                    const float4 a = H.eHi[an], c = H.eLo[an];
-                    hiE.rt = a.z;
-                    loE.rt = c.z;
+                mLo = int(mdStart[t]) + inner;
+                sgA = int(sgStart[mLo]);
+                nS = int(sgStart[mLo + 1]) - sgA;
+                if (nS > 0) {
+                    const TLOutLo L = loadOutLo(mLo, S.flags[sgA], D);
+                    variant = tlVariant(I.bLo, I.bUp, L.barrel);
+                    ok = tlStage1(variant, I, L, c);
                 }
-                pass = tlAlgo(sIn, sOut, mods, hiE, loE);
             }
-            const unsigned long long mask = __ballot(pass);
-            if (pass) {
-                const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
-                slabOut[off + pos] = so;
+            const unsigned long long maskA = __ballot(ok);
+            const int nq = __popcll(maskA);
+            if (nq == 0) continue;
+            if (ok) {
+                const int q = __popcll(maskA & ((1ull << lane) - 1ull));
+                Q.mLo[q] = mLo; Q.sgA[q] = sgA; Q.nS[q] = nS; Q.variant[q] = variant;
+                Q.sdlCut[q] = c.sdlCut; Q.thetaMuls[q] = c.thetaMuls; Q.dPhiMid[q] = c.dPhiMid;
             }
-            base += __popcll(mask);
+            waveSync();
+            // ---- phase B: the outer segments of the queued MDs
+            const int cntB = lane < nq ? Q.nS[lane] : 0;
+            int preB = cntB;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int t = __shfl_up(preB, o);
+                if (lane >= o) preB += t;
+            }
+            const int totalB = __shfl(preB, kWave - 1);
+            for (int kB0 = 0; kB0 < totalB; kB0 += kWave) {
+                const int kb = kB0 + lane;
+                int innerB;
+                const int eB = findEntry(kb, preB, nq, innerB);
+                bool pass = false;
+                int so = 0;
+                if (kb < totalB) {
+                    so = Q.sgA[eB] + innerB;
+                    const int flo = S.flags[so];
+                    const TLCarry cc{Q.sdlCut[eB], Q.thetaMuls[eB], Q.dPhiMid[eB]};
+                    const TLOutLo L = loadOutLo(Q.mLo[eB], flo, D);
+                    TLOutUp U;
+                    const int mUp = S.out[so];
+                    U.a = loadAnchor(mUp, D);
+                    const SGF g = S.f[so];
+                    U.dphimin = g.v[1]; U.dphimax = g.v[2];
+                    U.dphichange = g.v[3]; U.dphichangemin = g.v[4]; U.dphichangemax = g.v[5];
+                    const float4 sx = S.x[so];
+                    U.dr = sx.x; U.d = sx.y; U.aOU = sx.z; U.dPhiPos = sx.w;
+                    U.ec2s = flo & 16;
+                    U.hiX = U.hiY = U.hiRt = U.hiPhi = U.loX = U.loY = U.loRt = U.loPhi = U.aOUhi = U.aOUlo = 0.f;
+                    if (U.ec2s) {
+                        const int an = D.anchor[mUp];
+                        const float4 hi = H.eHi[an], lo = H.eLo[an];
+                        const float2 ee = S.e[so];
+                        U.hiX = hi.x; U.hiY = hi.y; U.hiRt = hi.z; U.hiPhi = hi.w;
+                        U.loX = lo.x; U.loY = lo.y; U.loRt = lo.z; U.loPhi = lo.w;
+                        U.aOUhi = ee.x; U.aOUlo = ee.y;
+                    }
+                    pass = tlStage2(Q.variant[eB], (flo & 2) != 0, I, L, U, cc);
+                }
+                const unsigned long long mask = __ballot(pass);
+                if (pass) {
+                    const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+                    slabOut[off + pos] = so;
+                }
+                base += __popcll(mask);
+            }
+            waveSync();  // the queue is rewritten by the next phase A
         }
     }
     if (lane == 0) tlCount[item] = base;
@@ -650,26 +782,36 @@ __global__ void k_tp_end_count(int nMD, const long long* sgEndStart, const int* 
     }
     tpEndCount[m] = c;
 }
+// The TPs ending at each MD, in (s2, s1) order: written once per MD.
+__global__ void k_tp_end_write(int nMD, const long long* sgEndStart, const int* sgEndList, const int* sgIn,
+                               const long long* sgStart, const long long* tpStart, const long long* tpEndStart,
+                               int* tpEndList) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nMD) return;
+    long long d = tpEndStart[m];
+    for (long long k = sgEndStart[m]; k < sgEndStart[m + 1]; ++k) {
+        const int s2 = sgEndList[k];
+        const int mx = sgIn[s2];
+        for (long long q = sgEndStart[mx]; q < sgEndStart[mx + 1]; ++q) {
+            const int s1 = sgEndList[q];
+            tpEndList[d++] = int(tpStart[s1] + (s2 - sgStart[mx]));
+        }
+    }
+}
+// One wave per tracklet copies the TPs ending at its first MD next to TP(s3, s4).
 __global__ void k_tcp_write(int nTL, const int* tlIn, const int* tlOutArr, const int* sgIn, const long long* sgStart,
-                            const long long* sgEndStart, const int* sgEndList, const long long* tpStart,
+                            const long long* tpStart, const long long* tpEndStart, const int* tpEndList,
                             const int* s3Of, const long long* start, int2* tc) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (l >= nTL) return;
     const int s3 = s3Of[l];
     if (s3 < 0) return;
     const int b = tlIn[l], s4 = tlOutArr[l];
     const int mdC = sgIn[b];
-    const long long tOuter = tpStart[s3] + (s4 - sgStart[sgIn[s4]]);
-    long long d = start[l];
-    for (long long k = sgEndStart[mdC]; k < sgEndStart[mdC + 1]; ++k) {
-        const int s2 = sgEndList[k];
-        const int mx = sgIn[s2];
-        for (long long q = sgEndStart[mx]; q < sgEndStart[mx + 1]; ++q) {
-            const int s1 = sgEndList[q];
-            const long long tInner = tpStart[s1] + (s2 - sgStart[mx]);
-            tc[d++] = make_int2(int(tInner), int(tOuter));
-        }
-    }
+    const int tOuter = int(tpStart[s3] + (s4 - sgStart[sgIn[s4]]));
+    const long long a = tpEndStart[mdC], n = tpEndStart[mdC + 1] - a, d = start[l];
+    for (long long k = lane; k < n; k += kWave) tc[d + k] = make_int2(tpEndList[a + k], tOuter);
 }
 
 // ------------------------------------------------------------------ counters
@@ -908,7 +1050,7 @@ struct sdl_event {
     DBuf<int> bidx;
     DBuf<unsigned> bdet;
     DBuf<long long> dEvOff;
-    DBuf<float> rt, r3, phi;
+    DBuf<float> rt, r3, phi, halpha;
     DBuf<int> hslot;
     DBuf<float4> eHi, eLo;
     DBuf<int> slotHitCount, slotFill, sortedHit, unknown;
@@ -918,14 +1060,18 @@ struct sdl_event {
     DBuf<int2> mdSlabHits;
     DBuf<float4> mdSlabF;
     DBuf<int> mdLo, mdUp, mdAnchor, mdSlot;
-    DBuf<float4> mdA, mdB;
+    DBuf<float4> mdA, mdB, mdP;
     DBuf<float2> mdC;
     DBuf<int> sgCap, sgCount, sgSlabOut, sgIn, sgOut;
     DBuf<long long> sgCapOff, sgStart, slotSgStart;
     DBuf<SGF> sgSlabF, sgF;
+    DBuf<float4> sgX;
+    DBuf<float2> sgE;
+    DBuf<int> sgFlags;
     DBuf<int> sgEndCount, sgEndFill, sgEndList;
     DBuf<long long> sgEndStart;
-    DBuf<int> tpCount, tpIn, tpOut, tpEndCount;
+    DBuf<int> tpCount, tpIn, tpOut, tpEndCount, tpEndList;
+    DBuf<long long> tpEndStart;
     DBuf<long long> tpStart, slotTpStart;
     DBuf<long long> tlCap, tlCapOff, tlStart, slotTlStart;
     DBuf<int> tlCount, tlSlabOut, tlIn, tlOut;
@@ -1056,6 +1202,7 @@ int ensureHits(sdl_event* ev) {
     if (int r = ev->rt.ensure(NH)) return r;
     if (int r = ev->r3.ensure(NH)) return r;
     if (int r = ev->phi.ensure(NH)) return r;
+    if (int r = ev->halpha.ensure(NH)) return r;
     if (int r = ev->hslot.ensure(NH)) return r;
     if (int r = ev->eHi.ensure(NH)) return r;
     if (int r = ev->eLo.ensure(NH)) return r;
@@ -1069,12 +1216,14 @@ int ensureHits(sdl_event* ev) {
 }
 
 HitsDev hitsView(sdl_event* ev) {
-    return HitsDev{ev->px, ev->py, ev->pz, ev->rt.p, ev->r3.p, ev->phi.p, ev->pidx, ev->eHi.p, ev->eLo.p};
+    return HitsDev{ev->px, ev->py, ev->pz, ev->rt.p, ev->r3.p, ev->phi.p, ev->halpha.p, ev->pidx, ev->eHi.p, ev->eLo.p};
 }
 MDDev mdView(sdl_event* ev) {
-    return MDDev{ev->mdLo.p, ev->mdUp.p, ev->mdAnchor.p, ev->mdSlot.p, ev->mdA.p, ev->mdB.p, ev->mdC.p};
+    return MDDev{ev->mdLo.p, ev->mdUp.p, ev->mdAnchor.p, ev->mdSlot.p, ev->mdA.p, ev->mdB.p, ev->mdC.p, ev->mdP.p};
 }
-SGDev sgView(sdl_event* ev) { return SGDev{ev->sgIn.p, ev->sgOut.p, ev->sgF.p}; }
+SGDev sgView(sdl_event* ev) {
+    return SGDev{ev->sgIn.p, ev->sgOut.p, ev->sgF.p, ev->sgX.p, ev->sgE.p, ev->sgFlags.p};
+}
 
 template <class T>
 int accumulate(sdl_event* ev, long long n, const T* vals, const int* slotOf, int which) {
@@ -1102,14 +1251,14 @@ int runPrep(sdl_event* ev) {
     if (nh > 0)
         hipLaunchKernelGGL(k_hit_prep, dim3(gridFor(nh, kBlock)), dim3(kBlock), 0, st, nh, ev->px, ev->py, ev->pz,
                            ev->pdet, ev->dEvOff.p, ev->E, G->dDet, G->NM, G->dMods, ev->rt.p, ev->r3.p, ev->phi.p,
-                           ev->hslot.p, ev->eHi.p, ev->eLo.p, ev->slotHitCount.p, ev->unknown.p);
+                           ev->halpha.p, ev->hslot.p, ev->eHi.p, ev->eLo.p, ev->slotHitCount.p, ev->unknown.p);
     timerEnd(ev, T_PREP);
     if (int r = scanExclusive<int>(ev->slotHitCount.p, ev->slotHitStart.p, NS, ev->scan, st)) return r;
     if (nh > 0) {
         hipLaunchKernelGGL(k_hit_scatter, dim3(gridFor(nh, kBlock)), dim3(kBlock), 0, st, nh, ev->hslot.p,
                            ev->slotHitStart.p, ev->slotFill.p, ev->sortedHit.p);
-        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, NS, ev->slotHitStart.p,
-                           ev->sortedHit.p);
+        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, NS,
+                           ev->slotHitStart.p, ev->sortedHit.p);
     }
     int unknown = 0;
     HIP_OK(hipMemcpyAsync(&unknown, ev->unknown.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1151,6 +1300,7 @@ int runMD(sdl_event* ev) {
     if (int r = ev->mdA.ensure(n)) return r;
     if (int r = ev->mdB.ensure(n)) return r;
     if (int r = ev->mdC.ensure(n)) return r;
+    if (int r = ev->mdP.ensure(n)) return r;
     hipLaunchKernelGGL(k_md_repack, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods, H,
                        ev->mdCapOff.p, ev->mdCount.p, ev->mdStart.p, ev->mdSlabHits.p, ev->mdSlabF.p, mdView(ev));
     accumulate<int>(ev, NS, ev->mdCap.p, nullptr, 0);
@@ -1190,9 +1340,15 @@ int runSG(sdl_event* ev) {
     if (int r = ev->sgIn.ensure(ns)) return r;
     if (int r = ev->sgOut.ensure(ns)) return r;
     if (int r = ev->sgF.ensure(ns)) return r;
+    if (int r = ev->sgX.ensure(ns)) return r;
+    if (int r = ev->sgE.ensure(ns)) return r;
+    if (int r = ev->sgFlags.ensure(ns)) return r;
     if (nMD > 0)
         hipLaunchKernelGGL(k_sg_repack, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, int(nMD),
                            ev->sgCapOff.p, ev->sgCount.p, ev->sgStart.p, ev->sgSlabOut.p, ev->sgSlabF.p, sgView(ev));
+    if (ev->nSG > 0)
+        hipLaunchKernelGGL(k_sg_extra, dim3(gridFor(ev->nSG, kBlock)), dim3(kBlock), 0, st, int(ev->nSG), G->NM,
+                           G->dMods, hitsView(ev), mdView(ev), sgView(ev));
     hipLaunchKernelGGL(k_slot_range, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, NS, ev->mdStart.p,
                        ev->sgStart.p, ev->slotSgStart.p);
     // reverse index: SGs by outer MD
@@ -1209,7 +1365,7 @@ int runSG(sdl_event* ev) {
     if (ev->nSG > 0) {
         hipLaunchKernelGGL(k_scatter_by, dim3(gridFor(ev->nSG, kBlock)), dim3(kBlock), 0, st, int(ev->nSG),
                            ev->sgOut.p, ev->sgEndStart.p, ev->sgEndFill.p, ev->sgEndList.p);
-        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, int(nMD),
+        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, int(nMD),
                            ev->sgEndStart.p, ev->sgEndList.p);
     }
     accumulate<int>(ev, nMD, ev->sgCap.p, ev->mdSlot.p, 1);
@@ -1273,9 +1429,9 @@ int runTL(sdl_event* ev) {
     if (int r = ev->tlSlabOut.ensure(capTotal)) return r;
     timerBegin(ev, T_TL);
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tl, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, int(nSG), G->NM, G->dMods,
-                           hitsView(ev), mdView(ev), sgView(ev), G->dConnStart, G->dConnList, ev->slotSgStart.p,
-                           ev->tlCap.p, ev->tlCapOff.p, ev->tlSlabOut.p, ev->tlCount.p);
+        hipLaunchKernelGGL(k_tl, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, int(nSG), G->NM,
+                           hitsView(ev), mdView(ev), sgView(ev), G->dConnStart, G->dConnList, ev->mdStart.p,
+                           ev->mdCount.p, ev->sgStart.p, ev->tlCap.p, ev->tlCapOff.p, ev->tlSlabOut.p, ev->tlCount.p);
     timerEnd(ev, T_TL);
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, nSG, ev->scan, st)) return r;
     ev->nTL = readLL(ev->tlStart.p + nSG, st);
@@ -1309,6 +1465,13 @@ int runTCP(sdl_event* ev) {
     if (nMD > 0)
         hipLaunchKernelGGL(k_tp_end_count, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, int(nMD),
                            ev->sgEndStart.p, ev->sgEndList.p, ev->sgIn.p, ev->tpEndCount.p);
+    if (int r = ev->tpEndStart.ensure(std::max(nMD, 1LL) + 1)) return r;
+    if (int r = ev->tpEndList.ensure(std::max(ev->nTP, 1LL))) return r;
+    if (int r = scanExclusive<int>(ev->tpEndCount.p, ev->tpEndStart.p, nMD, ev->scan, st)) return r;
+    if (nMD > 0)
+        hipLaunchKernelGGL(k_tp_end_write, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, int(nMD),
+                           ev->sgEndStart.p, ev->sgEndList.p, ev->sgIn.p, ev->sgStart.p, ev->tpStart.p,
+                           ev->tpEndStart.p, ev->tpEndList.p);
     if (nTL > 0)
         hipLaunchKernelGGL(k_tcp_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlIn.p,
                            ev->tlOut.p, ev->sgIn.p, ev->sgOut.p, ev->sgStart.p, ev->tpEndCount.p, ev->s3Of.p,
@@ -1318,8 +1481,8 @@ int runTCP(sdl_event* ev) {
     if (ev->nTCP < 0) return fail(SDL_ERR_HIP, "readback failed");
     if (int r = ev->tcp.ensure(std::max(ev->nTCP, 1LL))) return r;
     if (nTL > 0)
-        hipLaunchKernelGGL(k_tcp_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlIn.p,
-                           ev->tlOut.p, ev->sgIn.p, ev->sgStart.p, ev->sgEndStart.p, ev->sgEndList.p, ev->tpStart.p,
+        hipLaunchKernelGGL(k_tcp_write, dim3(gridFor(nTL, kWavesPerBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlIn.p,
+                           ev->tlOut.p, ev->sgIn.p, ev->sgStart.p, ev->tpStart.p, ev->tpEndStart.p, ev->tpEndList.p,
                            ev->s3Of.p, ev->tcpStart.p, ev->tcp.p);
     // counters: a TC made from triplets counts at its inner TP's module (Event.cc:961)
     if (ev->nTCP > 0) {
@@ -1475,7 +1638,8 @@ void sdl_event_destroy(sdl_event* ev) {
     // DBuf members free on release
     auto rel = [](auto& b) { b.release(); };
     rel(ev->bx); rel(ev->by); rel(ev->bz); rel(ev->bidx); rel(ev->bdet); rel(ev->dEvOff);
-    rel(ev->rt); rel(ev->r3); rel(ev->phi); rel(ev->hslot); rel(ev->eHi); rel(ev->eLo);
+    rel(ev->rt); rel(ev->r3); rel(ev->phi); rel(ev->halpha); rel(ev->mdP); rel(ev->sgX); rel(ev->sgE);
+    rel(ev->sgFlags); rel(ev->tpEndList); rel(ev->tpEndStart); rel(ev->hslot); rel(ev->eHi); rel(ev->eLo);
     rel(ev->slotHitCount); rel(ev->slotFill); rel(ev->sortedHit); rel(ev->unknown); rel(ev->slotHitStart);
     rel(ev->mdCap); rel(ev->mdCount); rel(ev->mdCapOff); rel(ev->mdStart); rel(ev->mdSlabHits); rel(ev->mdSlabF);
     rel(ev->mdLo); rel(ev->mdUp); rel(ev->mdAnchor); rel(ev->mdSlot); rel(ev->mdA); rel(ev->mdB); rel(ev->mdC);

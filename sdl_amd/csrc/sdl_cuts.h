@@ -94,22 +94,27 @@ __device__ __forceinline__ float dPhiChange(float ax, float ay, float phiA, floa
 
 struct HitView {
     float x, y, z, rt, r3, phi;
+    float alpha;   // asin(min(rt k2 / ptCut, 0.95)), used as the MD threshold slope
 };
 
 // ------------------------------------------------------------------ MD
 // MiniDoublet::moduleGapSize is precomputed per module (ModDev::gap).
 
 // MiniDoublet::dPhiThreshold (MiniDoublet.cc:741-810)
-__device__ __forceinline__ float dPhiThreshold(float rt, const ModDev& m, float dPhiArg, float dzArg) {
-    const float mulsB[6] = {0.0052f, 0.0038f, 0.0034f, 0.0034f, 0.0032f, 0.0034f};
-    const float rB[6] = {21.8f, 34.6f, 49.6f, 67.4f, 87.6f, 106.8f};
-    const float rE[5] = {131.4f, 156.2f, 185.6f, 220.3f, 261.5f};
+// miniSlope = asin(min(rt k2 / ptCut, 0.95)) of the threshold hit is precomputed per hit.
+__device__ __forceinline__ float mulsPtScaleBarrel(int iL) {
+    return iL == 0 ? 0.0052f : iL == 1 ? 0.0038f : iL == 4 ? 0.0032f : 0.0034f;
+}
+__device__ __forceinline__ float rLayNominal(bool barrel, int iL) {
+    if (barrel) return iL == 0 ? 21.8f : iL == 1 ? 34.6f : iL == 2 ? 49.6f : iL == 3 ? 67.4f : iL == 4 ? 87.6f : 106.8f;
+    return iL == 0 ? 131.4f : iL == 1 ? 156.2f : iL == 2 ? 185.6f : iL == 3 ? 220.3f : 261.5f;
+}
+__device__ __forceinline__ float dPhiThreshold(float miniSlope, const ModDev& m, float dPhiArg, float dzArg) {
     const int iL = m.layer - 1;
     const bool barrel = m.subdet == SUB_BARREL;
-    const float miniSlope = asinF(fmin2(rt * k2Rinv() / 1.f, kSinAlphaMax));
-    const float rLay = barrel ? rB[iL] : rE[iL];
+    const float rLay = rLayNominal(barrel, iL);
     const float miniPVoff = float(0.1 / double(rLay));
-    const float miniMuls = barrel ? mulsB[iL] * 3.f / 1.f : 0.006f * 3.f / 1.f;
+    const float miniMuls = barrel ? mulsPtScaleBarrel(iL) * 3.f / 1.f : 0.006f * 3.f / 1.f;
     if (barrel && m.side == SIDE_CENTER) return float(double(miniSlope) + sqrt(sq(miniMuls) + sq(miniPVoff)));
     if (barrel) {
         const float d = m.drdzStrip;
@@ -172,7 +177,7 @@ __device__ __forceinline__ bool mdAlgo(const HitView& lo, const HitView& up, con
         const float sign = float(((dz > 0) - (dz < 0)) * ((lo.z > 0) - (lo.z < 0)));
         const float invertedcrossercut = float(absTrunc(dz) > 2) * sign;
         if (!(fabsF(dz) < dzCut && invertedcrossercut <= 0)) return false;
-        const float miniCut = pixLower ? dPhiThreshold(lo.rt, m, 0.f, 1.f) : dPhiThreshold(up.rt, m, 0.f, 1.f);
+        const float miniCut = pixLower ? dPhiThreshold(lo.alpha, m, 0.f, 1.f) : dPhiThreshold(up.alpha, m, 0.f, 1.f);
         if (m.side != SIDE_CENTER) {
             float xn, yn, zn;
             if (m.type == TYPE_PS) {
@@ -212,7 +217,7 @@ __device__ __forceinline__ bool mdAlgo(const HitView& lo, const HitView& up, con
         o.shiftedDz = pixLower ? lo.z - zn : up.z - zn;
         dz = o.shiftedDz;
     }
-    const float miniCut = pixLower ? dPhiThreshold(lo.rt, m, o.dphi, dz) : dPhiThreshold(up.rt, m, o.dphi, dz);
+    const float miniCut = pixLower ? dPhiThreshold(lo.alpha, m, o.dphi, dz) : dPhiThreshold(up.alpha, m, o.dphi, dz);
     if (!(fabsF(o.dphi) < miniCut)) return false;
     const float dzFrac = float(double(fabsF(dz)) / double(fabsF(lo.z)));
     o.dphichange = o.dphi / dzFrac * (1.f + dzFrac);
@@ -220,10 +225,12 @@ __device__ __forceinline__ bool mdAlgo(const HitView& lo, const HitView& up, con
 }
 
 // ------------------------------------------------------------------ SG
-// What the segment and tracklet algebra reads of a mini-doublet.
+// What the segment algebra reads of a mini-doublet.  alpha1 = asin(min(rt k2, 0.95))
+// and tanAlpha1 = tanf(alpha1) of the anchor are precomputed per MD.
 struct MDView {
     HitView a;          // anchor hit
     float dz, shiftedDz, dphi, dphichange;
+    float alpha1, tanAlpha1;
     int mod;            // lower module index
 };
 
@@ -235,9 +242,8 @@ struct SGOut {
 // (Segment.cc:538-592 / 789-853).
 __device__ __forceinline__ bool sgAlphaCuts(const MDView& in, const MDView& out, const ModDev& mi, const ModDev& mo,
                                             float sgAlpha) {
-    const float mulsB[6] = {0.0052f, 0.0038f, 0.0034f, 0.0034f, 0.0032f, 0.0034f};
     const bool inB = mi.subdet == SUB_BARREL, outB = mo.subdet == SUB_BARREL;
-    const float sdMuls = inB ? mulsB[mi.layer - 1] * 3.f / kPtCut : 0.006f * 3.f / kPtCut;
+    const float sdMuls = inB ? mulsPtScaleBarrel(mi.layer - 1) * 3.f / kPtCut : 0.006f * 3.f / kPtCut;
     const float segmentDr = float(sqrt(sq(out.a.y - in.a.y) + sq(out.a.x - in.a.x)));
     const float dAlphaB = asinF(fmin2(segmentDr * k2Rinv() / kPtCut, kSinAlphaMax));
     float lumIn, lumOut;
@@ -280,12 +286,11 @@ __device__ __forceinline__ bool sgAlgo(const MDView& in, const MDView& out, cons
                                        bool connFound, const HitView& outHi, const HitView& outLo, SGOut& o) {
     o.dphi = o.dphimin = o.dphimax = o.dphichange = o.dphichangemin = o.dphichangemax = 0.f;
     const float rtIn = in.a.rt, rtOut = out.a.rt, zIn = in.a.z, zOut = out.a.z;
-    const float sdSlope = asinF(fmin2(rtOut * k2Rinv() / kPtCut, kSinAlphaMax));
+    const float sdSlope = out.alpha1;  // asin(min(rtOut k2 / ptCut, 0.95))
     if (mo.subdet == SUB_BARREL) {  // barrel-barrel, or endcap inner with barrel outer
-        const float mulsB[6] = {0.0052f, 0.0038f, 0.0034f, 0.0034f, 0.0032f, 0.0034f};
-        const float sdMuls = mulsB[mi.layer - 1] * 3.f / kPtCut;
+        const float sdMuls = mulsPtScaleBarrel(mi.layer - 1) * 3.f / kPtCut;
         const float sdPVoff = 0.1f / rtOut;
-        const float dzDrtScale = tanF(sdSlope) / sdSlope;
+        const float dzDrtScale = out.tanAlpha1 / sdSlope;
         const float zGeom = mi.layer <= 2 ? 2.f * kPixelPSZpitch : 2.f * kStrip2SZpitch;
         const float zLo = zIn + (zIn - kDeltaZLum) * (rtOut / rtIn - 1.f) * (zIn > 0.f ? 1.f : dzDrtScale) - zGeom;
         const float zHi = zIn + (zIn + kDeltaZLum) * (rtOut / rtIn - 1.f) * (zIn < 0.f ? 1.f : dzDrtScale) + zGeom;
@@ -303,7 +308,7 @@ __device__ __forceinline__ bool sgAlgo(const MDView& in, const MDView& out, cons
     if (zIn * zOut < 0) return false;
     const float dz = zOut - zIn;
     const float dLum = copysignf(kDeltaZLum, zIn);
-    const float drtDzScale = sdSlope / tanF(sdSlope);
+    const float drtDzScale = sdSlope / out.tanAlpha1;
     const float rtLo = fmax2(rtIn * (1.f + dz / (zIn + dLum) * drtDzScale) - rtGeom, rtIn - 0.5f * rtGeom);
     const float rtHi = rtIn * (zOut - dLum) / (zIn - dLum) + rtGeom;
     if (!(rtOut >= rtLo && rtOut <= rtHi)) return false;
@@ -369,15 +374,11 @@ __device__ __forceinline__ float midPhi(const HitView& a, const HitView& b) {
     return phiOf(mx, my);
 }
 
-struct SGView {
-    MDView in, out;
-    float dphi, dphimin, dphimax, dphichange, dphichangemin, dphichangemax;
-};
-
 // Edge hit of GeometryUtil::stripEdgeHit (GeometryUtil.cc:3-23)
 __device__ __forceinline__ HitView edgeOf(const HitView& h, float centroidPhi, bool up) {
     const double s = up ? 1.0 : -1.0;
     HitView e;
+    e.alpha = 0.f;
     e.x = float(s * 2.5 * cos(double(centroidPhi))) + h.x;
     e.y = float(s * 2.5 * sin(double(centroidPhi))) + h.y;
     e.z = 0.f + h.z;
@@ -387,27 +388,74 @@ __device__ __forceinline__ HitView edgeOf(const HitView& h, float centroidPhi, b
     return e;
 }
 
-// runTrackletDefaultAlgoBarrelBarrelBarrelBarrel_v2 (Tracklet.cc:902-1538)
-__device__ __noinline__ bool tlBBBB(const SGView& sIn, const SGView& sOut, const ModDev* mods) {
-    const HitView& hInLo = sIn.in.a;
-    const HitView& hInUp = sIn.out.a;
-    const HitView& hOutLo = sOut.in.a;
-    const HitView& hOutUp = sOut.out.a;
-    const float k2 = k2Rinv();
-    const bool isPS_InLo = mods[sIn.in.mod].type == TYPE_PS;
-    const bool isPS_OutLo = mods[sOut.in.mod].type == TYPE_PS;
+// --- Tracklets.  The kernel evaluates a candidate in two stages: the cuts that
+// need only the inner segment (wave-uniform) and the outer segment's inner MD
+// run first; the outer segment's outer MD and the per-segment quantities are
+// loaded only for candidates that survive them.  All cuts must pass, so their
+// order does not change the decision.  Quantities that depend on a single MD
+// or segment are precomputed once per object with the same expressions:
+//   MD:  alpha1 = asin(min(rt k2, 0.95)), dzScale = tan(alpha1)/alpha1 (double),
+//        alphaReg3 = asin(min(rt k2 / 3, 0.95))               (anchor rt)
+//   SG:  dr = rt(out - in), d = out.rt - in.rt, aOU = deltaPhi(out, out - in),
+//        dPhiPos = deltaPhi(in, out), and aOU with the strip edges of out.
+
+// wave-uniform view of the inner segment
+struct TLInner {
+    HitView lo, up;
+    float dphichange, dphichangemin, dphichangemax;
+    float dr, d, alphaReg3Lo;
+    bool bLo, bUp, psLo, psUp;
+};
+// outer segment, first stage: its inner MD
+struct TLOutLo {
+    HitView a;
+    float alpha1, dzScale, alphaReg3;
+    bool barrel, ps;
+};
+// outer segment, second stage
+struct TLOutUp {
+    HitView a;
+    float dphichange, dphichangemin, dphichangemax, dphimin, dphimax;
+    float dr, d, aOU, dPhiPos;
+    bool ec2s;
+    float hiX, hiY, hiRt, hiPhi, loX, loY, loRt, loPhi, aOUhi, aOUlo;
+};
+
+// Values of the first stage that the second stage reads.
+struct TLCarry {
+    float sdlCut, thetaMuls, dPhiMid;
+};
+
+enum { TL_BBBB = 0, TL_BBEE = 1, TL_EEEE = 2, TL_NONE = 3 };
+
+// Variant of runTrackletDefaultAlgo (Tracklet.cc:144-296) from the first three anchor
+// subdets; the fourth (outer segment's outer MD) is checked in the second stage:
+// BBBB and BBBE run _v2, BBEE and BEEE the barrel-endcap algorithm, EEEE the endcap
+// one, every other combination fails.
+__device__ __forceinline__ int tlVariant(bool b0, bool b1, bool b2) {
+    if (b0 && b1 && b2) return TL_BBBB;
+    if (b0 && !b2) return TL_BBEE;
+    if (!b0 && !b1 && !b2) return TL_EEEE;
+    return TL_NONE;
+}
+
+// ---- first stage: cuts on the inner segment and the outer segment's inner MD
+// _v2: Tracklet.cc:902-1153 (z window, pointed z window, deltaPhi of the midpoint)
+__device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L, TLCarry& c) {
+    const HitView& hInLo = I.lo;
+    const HitView& hInUp = I.up;
+    const HitView& hOutLo = L.a;
     const float rt_InLo = hInLo.rt, rt_OutLo = hOutLo.rt, z_InLo = hInLo.z, z_OutLo = hOutLo.z;
-    const float alpha1GeV_OutLo = asinF(fmin2(rt_OutLo * k2 / kPtCut, kSinAlphaMax));
     const float rtRatio = rt_OutLo / rt_InLo;
-    const float dzDrtScale = float(tan(double(alpha1GeV_OutLo)) / double(alpha1GeV_OutLo));
-    const float zpIn = isPS_InLo ? kPixelPSZpitch : kStrip2SZpitch;
-    const float zpOut = isPS_OutLo ? kPixelPSZpitch : kStrip2SZpitch;
+    const float dzDrtScale = L.dzScale;
+    const float zpIn = I.psLo ? kPixelPSZpitch : kStrip2SZpitch;
+    const float zpOut = L.ps ? kPixelPSZpitch : kStrip2SZpitch;
     const float zHi = z_InLo + (z_InLo + kDeltaZLum) * (rtRatio - 1.f) * (z_InLo < 0.f ? 1.f : dzDrtScale) + (zpIn + zpOut);
     const float zLo = z_InLo + (z_InLo - kDeltaZLum) * (rtRatio - 1.f) * (z_InLo > 0.f ? 1.f : dzDrtScale) - (zpIn + zpOut);
     if (!(z_OutLo >= zLo && z_OutLo <= zHi)) return false;
     const float drt_OutLo_InLo = rt_OutLo - rt_InLo;
     const float r3_InLo = sqrtF(z_InLo * z_InLo + rt_InLo * rt_InLo);
-    const float drt_InSeg = hInUp.rt - hInLo.rt;
+    const float drt_InSeg = I.d;
     const float dz_InSeg = hInUp.z - hInLo.z;
     const float dr3_InSeg = hInUp.r3 - hInLo.r3;
     const float coshEta = dr3_InSeg / drt_InSeg;
@@ -423,73 +471,24 @@ __device__ __noinline__ bool tlBBBB(const SGView& sIn, const SGView& sOut, const
     const float zHiP = z_InLo + dzMean * (z_InLo < 0.f ? 1.f : dzDrtScale) + zWindow;
     if (!(z_OutLo >= zLoP && z_OutLo <= zHiP)) return false;
     const float sdlPVoff = 0.1f / rt_OutLo;
-    const float sdlCut = float(double(alpha1GeV_OutLo) + sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
-    if (!(fabsF(dPhi(hInUp.phi, hOutUp.phi)) <= sdlCut)) return false;
+    const float sdlCut = float(double(L.alpha1) + sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
     const float dPhiMid = phiMpiPi(phiOf(hOutLo.x - hInLo.x, hOutLo.y - hInLo.y) - midPhi(hInLo, hOutLo));
     if (!(fabsF(dPhiMid) <= sdlCut)) return false;
-    const float alpha_InLo = sIn.dphichange;
-    const ModDev& mOU = mods[sOut.out.mod];
-    const bool isEC_lastLayer = mOU.subdet == SUB_ENDCAP && mOU.type == TYPE_2S;
-    const HitView hiE = isEC_lastLayer ? edgeOf(hOutUp, mOU.centroidPhi, true) : hOutUp;
-    const HitView loE = isEC_lastLayer ? edgeOf(hOutUp, mOU.centroidPhi, false) : hOutUp;
-    // alpha_OutUp = deltaPhi(h, h - hOutLo) for the anchor and the two edges
-    const float aOU = phiMpiPi(phiOf(hOutUp.x - hOutLo.x, hOutUp.y - hOutLo.y) - hOutUp.phi);
-    const float aOUhi = phiMpiPi(phiOf(hiE.x - hOutLo.x, hiE.y - hOutLo.y) - hiE.phi);
-    const float aOUlo = phiMpiPi(phiOf(loE.x - hOutLo.x, loE.y - hOutLo.y) - loE.phi);
-    const float axX = hOutUp.x - hInLo.x, axY = hOutUp.y - hInLo.y;
-    const float axPhi = phiOf(axX, axY);
-    const float axHiPhi = phiOf(hiE.x - hInLo.x, hiE.y - hInLo.y);
-    const float axLoPhi = phiOf(loE.x - hInLo.x, loE.y - hInLo.y);
-    float betaIn = alpha_InLo - phiMpiPi(axPhi - hInLo.phi);
-    float betaInRHmin = betaIn, betaInRHmax = betaIn;
-    float betaOut = -aOU + phiMpiPi(axPhi - hOutUp.phi);
-    float betaOutRHmin = -aOUhi + phiMpiPi(axHiPhi - hiE.phi);
-    float betaOutRHmax = -aOUlo + phiMpiPi(axLoPhi - loE.phi);
-    const float drt_tl_axis = rtOf(axX, axY);
-    const float rt_InSeg = rtOf(hInUp.x - hInLo.x, hInUp.y - hInLo.y);
-    const float betaIn_cut = asinF(fmin2((-rt_InSeg * 1.f + drt_tl_axis) * k2 / kPtCut, kSinAlphaMax)) + (0.02f / drt_InSeg);
-    if (!(fabsF(betaInRHmin) < betaIn_cut)) return false;
-    float betaAv = 0.5f * (betaIn + betaOut);
-    float pt_beta = float(double(drt_tl_axis * k2) / sin(double(betaAv)));
-    const int lOut = isEC_lastLayer ? 11 : 5;
-    const float sdOut_dr = rtOf(hOutUp.x - hOutLo.x, hOutUp.y - hOutLo.y);
-    const float sdOut_d = hOutUp.rt - hOutLo.rt;
-    deltaBetaIter(betaIn, betaOut, betaAv, pt_beta, rt_InSeg, sdOut_dr, drt_tl_axis, 5);
-    rescaleRH(betaIn, betaInRHmin, betaInRHmax);
-    rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
-    const float dBetaMuls = thetaMuls * 4.f / fmin2(fabsF(pt_beta), 7.0f);
-    const float alpha_OutLo = sOut.dphichange;
-    const float alphaOutAbsReg = fmax2(fabsF(alpha_OutLo), asinF(fmin2(hOutLo.rt * k2 / 3.0f, kSinAlphaMax)));
-    const float dBetaOutLum = lOut < 11 ? 0.0f : fabsF(alphaOutAbsReg * kDeltaZLum / hOutLo.z);
-    const float dBetaLum2 = (0.0f + dBetaOutLum) * (0.0f + dBetaOutLum);
-    const float sinDPhi = sinF(dPhiMid);
-    const float dBetaROut2 = float(sq(double((hiE.rt - loE.rt) * sinDPhi / drt_tl_axis)));
-    const float betaOut_cut = float(double(asinF(fmin2(drt_tl_axis * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
-                                    sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
-    if (!(fabsF(betaOut) < betaOut_cut)) return false;
-    const float dBetaRes = 0.02f / fmin2(sdOut_d, drt_InSeg);
-    const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
-                                  0.25 * sq(double(fabsF(betaInRHmin - betaInRHmax) + fabsF(betaOutRHmin - betaOutRHmax))));
-    const float dBeta = betaIn - betaOut;
-    return dBeta * dBeta <= dBetaCut2;
+    c.sdlCut = sdlCut;
+    c.thetaMuls = thetaMuls;
+    c.dPhiMid = dPhiMid;
+    return true;
 }
 
-// runTrackletDefaultAlgoBarrelBarrelEndcapEndcap (Tracklet.cc:1540-2016) and
-// ...EndcapEndcapEndcapEndcap (Tracklet.cc:2018-2665), which share their skeleton.
+// BBEE / EEEE: Tracklet.cc:1540-1771 and 2018-2270 (rt windows, pointing, midpoint deltaPhi)
 template <bool EEEE>
-__device__ __noinline__ bool tlEndcapFamily(const SGView& sIn, const SGView& sOut, const ModDev* mods,
-                                            const HitView& outUpHi, const HitView& outUpLo) {
-    const HitView& hInLo = sIn.in.a;
-    const HitView& hInUp = sIn.out.a;
-    const HitView& hOutLo = sOut.in.a;
-    const HitView& hOutUp = sOut.out.a;
-    const float k2 = k2Rinv();
-    const bool psInLo = mods[sIn.in.mod].type == TYPE_PS;
-    const bool psOutLo = mods[sOut.in.mod].type == TYPE_PS;
-    const bool psInUp = mods[sIn.out.mod].type == TYPE_PS;
+__device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, TLCarry& c) {
+    const HitView& hInLo = I.lo;
+    const HitView& hInUp = I.up;
+    const HitView& hOutLo = L.a;
+    const bool psInLo = I.psLo, psOutLo = L.ps, psInUp = I.psUp;
     const float rtIn = hInLo.rt, rtOut = hOutLo.rt, zIn = hInLo.z, zOut = hOutLo.z;
-    const float sdlSlope = asinF(fmin2(rtOut * k2 / kPtCut, kSinAlphaMax));
-    const float dzDrtScale = float(tan(double(sdlSlope)) / double(sdlSlope));
+    const float dzDrtScale = L.dzScale;
     if (!(zIn * zOut > 0)) return false;
     const float dLum = copysignf(kDeltaZLum, zIn);
     float rtLo, rtHi;
@@ -511,7 +510,7 @@ __device__ __noinline__ bool tlEndcapFamily(const SGView& sIn, const SGView& sOu
         rtHi = rtIn * (1.f + dz / (zIn - dLum)) + rtGeom;
     }
     if (!(rtOut >= rtLo && rtOut <= rtHi)) return false;
-    const float drtSDIn = hInUp.rt - hInLo.rt;
+    const float drtSDIn = I.d;
     const float dzSDIn = hInUp.z - hInLo.z;
     const float dr3SDIn = hInUp.r3 - hInLo.r3;
     const float coshEta = dr3SDIn / drtSDIn;
@@ -541,45 +540,115 @@ __device__ __noinline__ bool tlEndcapFamily(const SGView& sIn, const SGView& sOu
         if (!(kZ >= 0 && rtOut >= rtLoP && rtOut <= rtHiP)) return false;
     }
     const float sdlPVoff = 0.1f / rtOut;
-    const float sdlCut = float(double(sdlSlope) + sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
-    if (!(fabsF(dPhi(hInUp.phi, hOutUp.phi)) <= sdlCut)) return false;
+    const float sdlCut = float(double(L.alpha1) + sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
     const float dPhiMid = phiMpiPi(phiOf(hOutLo.x - hInLo.x, hOutLo.y - hInLo.y) - midPhi(hInLo, hOutLo));
     if (!(fabsF(dPhiMid) <= sdlCut)) return false;
-    const float sdIn_alpha = sIn.dphichange;
-    const float sdOut_alpha = sIn.dphichange;  // A-8
+    c.sdlCut = sdlCut;
+    c.thetaMuls = thetaMuls;
+    c.dPhiMid = dPhiMid;
+    return true;
+}
+
+__device__ __forceinline__ bool tlStage1(int variant, const TLInner& I, const TLOutLo& L, TLCarry& c) {
+    if (variant == TL_BBBB) return tlStage1BBBB(I, L, c);
+    if (variant == TL_BBEE) return tlStage1EC<false>(I, L, c);
+    if (variant == TL_EEEE) return tlStage1EC<true>(I, L, c);
+    return false;
+}
+
+// ---- second stage: cuts that involve the outer segment's outer MD
+// _v2: Tracklet.cc:1097-1100 and 1156-1501
+__device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L, const TLOutUp& U, const TLCarry& c) {
+    const HitView& hInLo = I.lo;
+    const HitView& hInUp = I.up;
+    const HitView& hOutLo = L.a;
+    const HitView& hOutUp = U.a;
+    const float k2 = k2Rinv();
+    if (!(fabsF(dPhi(hInUp.phi, hOutUp.phi)) <= c.sdlCut)) return false;
+    const float alpha_InLo = I.dphichange;
+    const bool isEC_lastLayer = U.ec2s;
+    const float axX = hOutUp.x - hInLo.x, axY = hOutUp.y - hInLo.y;
+    const float axPhi = phiOf(axX, axY);
+    float betaIn = alpha_InLo - phiMpiPi(axPhi - hInLo.phi);
+    float betaInRHmin = betaIn, betaInRHmax = betaIn;
+    float betaOut = -U.aOU + phiMpiPi(axPhi - hOutUp.phi);
+    float betaOutRHmin = betaOut, betaOutRHmax = betaOut;
+    float hiRt = hOutUp.rt, loRt = hOutUp.rt;
+    if (isEC_lastLayer) {
+        betaOutRHmin = -U.aOUhi + phiMpiPi(phiOf(U.hiX - hInLo.x, U.hiY - hInLo.y) - U.hiPhi);
+        betaOutRHmax = -U.aOUlo + phiMpiPi(phiOf(U.loX - hInLo.x, U.loY - hInLo.y) - U.loPhi);
+        hiRt = U.hiRt;
+        loRt = U.loRt;
+    }
+    const float drt_tl_axis = rtOf(axX, axY);
+    const float rt_InSeg = I.dr;
+    const float drt_InSeg = I.d;
+    const float betaIn_cut = asinF(fmin2((-rt_InSeg * 1.f + drt_tl_axis) * k2 / kPtCut, kSinAlphaMax)) + (0.02f / drt_InSeg);
+    if (!(fabsF(betaInRHmin) < betaIn_cut)) return false;
+    float betaAv = 0.5f * (betaIn + betaOut);
+    float pt_beta = float(double(drt_tl_axis * k2) / sin(double(betaAv)));
+    const int lOut = isEC_lastLayer ? 11 : 5;
+    const float sdOut_dr = U.dr;
+    const float sdOut_d = U.d;
+    deltaBetaIter(betaIn, betaOut, betaAv, pt_beta, rt_InSeg, sdOut_dr, drt_tl_axis, 5);
+    rescaleRH(betaIn, betaInRHmin, betaInRHmax);
+    rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
+    const float dBetaMuls = c.thetaMuls * 4.f / fmin2(fabsF(pt_beta), 7.0f);
+    const float alphaOutAbsReg = fmax2(fabsF(U.dphichange), L.alphaReg3);
+    const float dBetaOutLum = lOut < 11 ? 0.0f : fabsF(alphaOutAbsReg * kDeltaZLum / hOutLo.z);
+    const float dBetaLum2 = (0.0f + dBetaOutLum) * (0.0f + dBetaOutLum);
+    const float sinDPhi = sinF(c.dPhiMid);
+    const float dBetaROut2 = float(sq(double((hiRt - loRt) * sinDPhi / drt_tl_axis)));
+    const float betaOut_cut = float(double(asinF(fmin2(drt_tl_axis * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
+                                    sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
+    if (!(fabsF(betaOut) < betaOut_cut)) return false;
+    const float dBetaRes = 0.02f / fmin2(sdOut_d, drt_InSeg);
+    const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
+                                  0.25 * sq(double(fabsF(betaInRHmin - betaInRHmax) + fabsF(betaOutRHmin - betaOutRHmax))));
+    const float dBeta = betaIn - betaOut;
+    return dBeta * dBeta <= dBetaCut2;
+}
+
+// BBEE: Tracklet.cc:1774-2016; EEEE: Tracklet.cc:2273-2665
+template <bool EEEE>
+__device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, const TLOutUp& U, const TLCarry& c) {
+    const HitView& hInLo = I.lo;
+    const HitView& hInUp = I.up;
+    const HitView& hOutLo = L.a;
+    const HitView& hOutUp = U.a;
+    const float k2 = k2Rinv();
+    if (!(fabsF(dPhi(hInUp.phi, hOutUp.phi)) <= c.sdlCut)) return false;
+    const float sdIn_alpha = I.dphichange;
+    const float sdOut_alpha = I.dphichange;  // A-8
     const float dr3x = hOutUp.x - hInLo.x, dr3y = hOutUp.y - hInLo.y;
     const float dr3Phi = phiOf(dr3x, dr3y);
     float betaIn = sdIn_alpha - phiMpiPi(dr3Phi - hInLo.phi);
     float betaInRHmin, betaInRHmax, betaOutRHmin, betaOutRHmax, betaOut;
     if (!EEEE) {
-        const float sdOut_alphaOut = phiMpiPi(phiOf(hOutUp.x - hOutLo.x, hOutUp.y - hOutLo.y) - hOutUp.phi);
-        const float aMin = phiMpiPi(sOut.dphichangemin - sOut.dphimin);
-        const float aMax = phiMpiPi(sOut.dphichangemax - sOut.dphimax);
+        const float sdOut_alphaOut = U.aOU;
+        const float aMin = phiMpiPi(U.dphichangemin - U.dphimin);
+        const float aMax = phiMpiPi(U.dphichangemax - U.dphimax);
         betaOut = -sdOut_alphaOut + phiMpiPi(dr3Phi - hOutUp.phi);
         betaInRHmin = betaIn;  // the 2S test never holds (A-9)
         betaInRHmax = betaIn;
         betaOutRHmin = betaOut - aMin + sdOut_alphaOut;
         betaOutRHmax = betaOut - aMax + sdOut_alphaOut;
+        if (fabsF(betaOutRHmin) > fabsF(betaOutRHmax)) { float t = betaOutRHmin; betaOutRHmin = betaOutRHmax; betaOutRHmax = t; }
     } else {
-        const float sdOut_dPhiPos = dPhi(hOutLo.phi, hOutUp.phi);
-        const float aMin = phiMpiPi(sOut.dphichangemin - sdOut_dPhiPos);
-        const float aMax = phiMpiPi(sOut.dphichangemax - sdOut_dPhiPos);
-        const float sdOut_alphaOut = phiMpiPi(sOut.dphichange - sdOut_dPhiPos);
+        const float sdOut_dPhiPos = U.dPhiPos;
+        const float aMin = phiMpiPi(U.dphichangemin - sdOut_dPhiPos);
+        const float aMax = phiMpiPi(U.dphichangemax - sdOut_dPhiPos);
+        const float sdOut_alphaOut = phiMpiPi(U.dphichange - sdOut_dPhiPos);
         betaOut = -sdOut_alphaOut + phiMpiPi(dr3Phi - hOutUp.phi);
-        betaInRHmin = betaIn + sIn.dphichangemin - sdIn_alpha;
-        betaInRHmax = betaIn + sIn.dphichangemax - sdIn_alpha;
+        betaInRHmin = betaIn + I.dphichangemin - sdIn_alpha;
+        betaInRHmax = betaIn + I.dphichangemax - sdIn_alpha;
+        if (fabsF(betaInRHmin) > fabsF(betaInRHmax)) { float t = betaInRHmin; betaInRHmin = betaInRHmax; betaInRHmax = t; }
         betaOutRHmin = betaOut - aMin + sdOut_alphaOut;
         betaOutRHmax = betaOut - aMax + sdOut_alphaOut;
-    }
-    if (!EEEE) {
-        if (fabsF(betaOutRHmin) > fabsF(betaOutRHmax)) { float t = betaOutRHmin; betaOutRHmin = betaOutRHmax; betaOutRHmax = t; }
-        if (fabsF(betaInRHmin) > fabsF(betaInRHmax)) { float t = betaInRHmin; betaInRHmin = betaInRHmax; betaInRHmax = t; }
-    } else {
-        if (fabsF(betaInRHmin) > fabsF(betaInRHmax)) { float t = betaInRHmin; betaInRHmin = betaInRHmax; betaInRHmax = t; }
         if (fabsF(betaOutRHmin) > fabsF(betaOutRHmax)) { float t = betaOutRHmin; betaOutRHmin = betaOutRHmax; betaOutRHmax = t; }
     }
-    const float sdIn_dr = rtOf(hInUp.x - hInLo.x, hInUp.y - hInLo.y);
-    const float sdIn_d = hInUp.rt - hInLo.rt;
+    const float sdIn_dr = I.dr;
+    const float sdIn_d = I.d;
     const float dr = rtOf(dr3x, dr3y);
     if (!EEEE) {
         const float betaIn_cut = asinF(fmin2((-sdIn_dr * 1.f + dr) * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdIn_d);
@@ -588,26 +657,25 @@ __device__ __noinline__ bool tlEndcapFamily(const SGView& sIn, const SGView& sOu
     float betaAv = 0.5f * (betaIn + betaOut);
     float pt_beta = float(double(dr * k2) / sin(double(betaAv)));
     const int lIn = EEEE ? 11 : 5;
-    const float sdOut_dr = rtOf(hOutUp.x - hOutLo.x, hOutUp.y - hOutLo.y);
-    const float sdOut_d = hOutUp.rt - hOutLo.rt;
+    const float sdOut_dr = U.dr;
+    const float sdOut_d = U.d;
     deltaBetaIter(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn);
     rescaleRH(betaIn, betaInRHmin, betaInRHmax);
     rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
-    const float dBetaMuls = thetaMuls * 4.f / fmin2(fabsF(pt_beta), 7.0f);
-    const float alphaOutAbsReg = fmax2(fabsF(sdOut_alpha), asinF(fmin2(hOutLo.rt * k2 / 3.0f, kSinAlphaMax)));
+    const float dBetaMuls = c.thetaMuls * 4.f / fmin2(fabsF(pt_beta), 7.0f);
+    const float alphaOutAbsReg = fmax2(fabsF(sdOut_alpha), L.alphaReg3);
     float dBetaInLum = 0.0f;
     if (EEEE) {
-        const float alphaInAbsReg = fmax2(fabsF(sdIn_alpha), asinF(fmin2(hInLo.rt * k2 / 3.0f, kSinAlphaMax)));
+        const float alphaInAbsReg = fmax2(fabsF(sdIn_alpha), I.alphaReg3Lo);
         dBetaInLum = fabsF(alphaInAbsReg * kDeltaZLum / hInLo.z);
     }
     const float dBetaOutLum = fabsF(alphaOutAbsReg * kDeltaZLum / hOutLo.z);
     const float dBetaLum2 = (dBetaInLum + dBetaOutLum) * (dBetaInLum + dBetaOutLum);
     float dBetaROut2 = 0.f;
     if (!EEEE) {
-        const float sinDPhi = sinF(dPhiMid);
-        const bool out2S = mods[sOut.out.mod].type == TYPE_2S;
-        const float hiRt = out2S ? outUpHi.rt : hOutUp.rt;
-        const float loRt = out2S ? outUpLo.rt : hOutUp.rt;
+        const float sinDPhi = sinF(c.dPhiMid);
+        const float hiRt = U.ec2s ? U.hiRt : hOutUp.rt;
+        const float loRt = U.ec2s ? U.loRt : hOutUp.rt;
         dBetaROut2 = float(sq(double((hiRt - loRt) * sinDPhi / dr)));
     }
     const float betaOut_cut = float(double(asinF(fmin2(dr * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
@@ -620,17 +688,13 @@ __device__ __noinline__ bool tlEndcapFamily(const SGView& sIn, const SGView& sOu
     return dBeta * dBeta <= dBetaCut2;
 }
 
-// Tracklet::runTrackletDefaultAlgo dispatch on the four anchor subdets (Tracklet.cc:144-296)
-__device__ __forceinline__ bool tlAlgo(const SGView& sIn, const SGView& sOut, const ModDev* mods,
-                                       const HitView& outUpHi, const HitView& outUpLo) {
-    const bool b0 = mods[sIn.in.mod].subdet == SUB_BARREL;
-    const bool b1 = mods[sIn.out.mod].subdet == SUB_BARREL;
-    const bool b2 = mods[sOut.in.mod].subdet == SUB_BARREL;
-    const bool b3 = mods[sOut.out.mod].subdet == SUB_BARREL;
-    if (b0 && b1 && b2) return tlBBBB(sIn, sOut, mods);
-    if (b0 && !b2 && !b3) return tlEndcapFamily<false>(sIn, sOut, mods, outUpHi, outUpLo);  // BBEE, BEEE
-    if (!b0 && !b1 && !b2 && !b3) return tlEndcapFamily<true>(sIn, sOut, mods, outUpHi, outUpLo);
-    return false;  // other mixes fail (A-14)
+// bOutUp: the outer segment's outer MD is in the barrel (the fourth subdet of the dispatch)
+__device__ __forceinline__ bool tlStage2(int variant, bool bOutUp, const TLInner& I, const TLOutLo& L,
+                                         const TLOutUp& U, const TLCarry& c) {
+    if (variant == TL_BBBB) return tlStage2BBBB(I, L, U, c);
+    if (bOutUp) return false;
+    if (variant == TL_BBEE) return tlStage2EC<false>(I, L, U, c);
+    return tlStage2EC<true>(I, L, U, c);
 }
 
 }  // namespace sdl

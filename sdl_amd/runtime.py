@@ -8,7 +8,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libsdl_amd.so")
+# SDL_AMD_LIB selects another build of the same library (used to compare builds)
+LIB_PATH = os.environ.get("SDL_AMD_LIB", os.path.join(_HERE, "_lib", "libsdl_amd.so"))
 
 STAGE_MD, STAGE_SG, STAGE_TP, STAGE_TL, STAGE_TCP, STAGE_TCL = range(6)
 STAGES = ("MD", "SG", "TP", "TL", "TCP", "TCL")
