@@ -53,6 +53,10 @@ template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
     int ensure(size_t n) {
         if (n <= cap && p) return 0;
         size_t want = std::max<size_t>(n, 1);
@@ -74,76 +78,189 @@ struct DBuf {
 };
 
 // ------------------------------------------------------------------ scans
-// Exclusive scan of n counts into n+1 offsets (out[n] = total), three phases.
+// Exclusive scan of n counts into n+1 offsets (out[n] = total) in one launch:
+// single-pass chained scan with decoupled look-back.  Tiles take their index
+// from a ticket counter, so every predecessor of a tile has started when the
+// tile looks back, and a predecessor publishes its aggregate before it waits
+// on anything: every wait ends.  A tile's state word packs
+// [epoch:20][flag:2][value:42]; the epoch changes after every scan (the last
+// tile to finish advances it and re-arms the ticket), so the state array needs
+// no clearing between scans.  Totals of 2^42 or more are reported as -1.
 constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+constexpr unsigned long long kScanValMask = (1ULL << 42) - 1;
+constexpr unsigned kScanEpochMax = (1u << 20) - 1;
+enum : unsigned { SCAN_AGG = 1, SCAN_INCL = 2 };
+
+struct ScanCtl {
+    unsigned long long ticket;
+    unsigned long long done;
+    unsigned long long epoch;
+};
+
+__device__ __forceinline__ unsigned long long scanPack(unsigned epoch, unsigned flag, long long v) {
+    const unsigned long long val = v < 0 || (unsigned long long)v >= kScanValMask ? kScanValMask : (unsigned long long)v;
+    return ((unsigned long long)epoch << 44) | ((unsigned long long)flag << 42) | val;
+}
 
 template <class TIn>
-__global__ void k_scan_local(const TIn* in, long long* out, long long* blockSums, long long n) {
-    __shared__ long long sh[kBlock];
-    const long long base = (long long)blockIdx.x * kBlock * kScanItems + threadIdx.x * kScanItems;
+__global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, long long n, int nTiles,
+                                                 unsigned long long* state, unsigned long long stateCap,
+                                                 ScanCtl* ctl) {
+    __shared__ long long waveSum[kWavesPerBlock];
+    __shared__ long long prefixSh;
+    __shared__ int tileSh;
+    __shared__ unsigned epochSh;
+    if (threadIdx.x == 0) {
+        tileSh = int(atomicAdd(&ctl->ticket, 1ULL));
+        epochSh = unsigned(__hip_atomic_load(&ctl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    __syncthreads();
+    const int tile = tileSh;
+    const unsigned epoch = epochSh;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const long long base = (long long)tile * kScanTile + (long long)threadIdx.x * kScanItems;
     long long v[kScanItems];
     long long s = 0;
+#pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        long long i = base + k;
+        const long long i = base + k;
         v[k] = i < n ? (long long)in[i] : 0;
         s += v[k];
     }
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-        long long t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
+    long long x = s;  // inclusive within the wave
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const long long y = __shfl_up(x, off);
+        if (lane >= off) x += y;
     }
-    long long run = sh[threadIdx.x] - s;
+    if (lane == kWave - 1) waveSum[wave] = x;
+    __syncthreads();
+    long long wPrefix = 0, agg = 0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        if (w < wave) wPrefix += waveSum[w];
+        agg += waveSum[w];
+    }
+    if (wave == 0) {
+        long long excl = 0;
+        if (tile == 0) {
+            if (lane == 0) __hip_atomic_store(&state[0], scanPack(epoch, SCAN_INCL, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0) __hip_atomic_store(&state[tile], scanPack(epoch, SCAN_AGG, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int j = tile - 1;
+            for (;;) {
+                const int idx = j - lane;
+                unsigned long long w = 0;
+                bool ready = idx < 0;
+                for (;;) {
+                    if (!ready) {
+                        w = __hip_atomic_load(&state[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                        ready = unsigned(w >> 44) == epoch && ((w >> 42) & 3u) != 0;
+                    }
+                    if (__all(ready)) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                const bool incl = idx >= 0 && ((w >> 42) & 3u) == SCAN_INCL;
+                const long long val = idx >= 0 ? (long long)(w & kScanValMask) : 0;
+                const unsigned long long inclMask = __ballot(incl);
+                // lanes up to and including the nearest inclusive predecessor
+                const int stop = inclMask ? __builtin_ctzll(inclMask) : kWave - 1;
+                long long part = lane <= stop ? val : 0;
+#pragma unroll
+                for (int off = 1; off < kWave; off <<= 1) part += __shfl_xor(part, off);
+                excl += part;
+                if (inclMask) break;
+                j -= kWave;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&state[tile], scanPack(epoch, SCAN_INCL, excl + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) prefixSh = excl;
+    }
+    __syncthreads();
+    const long long prefix = prefixSh;
+    long long run = prefix + wPrefix + x - s;
+#pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        long long i = base + k;
+        const long long i = base + k;
         if (i < n) out[i] = run;
         run += v[k];
     }
-    if (threadIdx.x == kBlock - 1) blockSums[blockIdx.x] = sh[kBlock - 1];
-}
-
-__global__ void k_scan_add(long long* out, const long long* blockOff, long long n) {
-    const long long base = (long long)blockIdx.x * kBlock * kScanItems;
-    const long long add = blockOff[blockIdx.x];
-    for (int k = threadIdx.x; k < kBlock * kScanItems; k += kBlock) {
-        long long i = base + k;
-        if (i < n) out[i] += add;
+    if (tile == nTiles - 1 && threadIdx.x == 0) {
+        const long long total = prefix + agg;
+        out[n] = (unsigned long long)prefix >= kScanValMask || (unsigned long long)total >= kScanValMask ? -1 : total;
+    }
+    // the last tile to finish re-arms the ticket and advances the epoch
+    __syncthreads();
+    __shared__ bool lastSh;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        lastSh = atomicAdd(&ctl->done, 1ULL) == (unsigned long long)(nTiles - 1);
+    }
+    __syncthreads();
+    if (!lastSh) return;
+    if (epoch == kScanEpochMax) {
+        for (unsigned long long i = threadIdx.x; i < stateCap; i += blockDim.x) state[i] = 0;
+        __threadfence();
+    }
+    if (threadIdx.x == 0) {
+        ctl->ticket = 0;
+        ctl->done = 0;
+        ctl->epoch = epoch == kScanEpochMax ? 1 : epoch + 1;
     }
 }
 
-__global__ void k_set_total(long long* out, long long n, const long long* blockOff, long long nb) {
-    out[n] = blockOff[nb];
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long n, unsigned long long v) {
+    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
+__global__ void k_scan_empty(long long* out) { out[0] = 0; }
 
 struct ScanScratch {
-    DBuf<long long> sums[4], offs[4];
+    unsigned long long* state = nullptr;
+    unsigned long long cap = 0;
+    ScanCtl* ctl = nullptr;
+    ~ScanScratch() {
+        if (state) (void)hipFree(state);
+        if (ctl) (void)hipFree(ctl);
+    }
+    // Makes room for nTiles tile states; new memory starts at epoch 0 (never current).
+    int ensure(unsigned long long nTiles, hipStream_t st) {
+        if (!ctl) {
+            if (hipMalloc(&ctl, sizeof(ScanCtl)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
+            const ScanCtl init{0, 0, 1};
+            if (hipMemcpyAsync(ctl, &init, sizeof(init), hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return fail(SDL_ERR_HIP, "scan state init failed");
+        }
+        if (nTiles <= cap) return 0;
+        const unsigned long long want = std::max<unsigned long long>(nTiles, cap * 2);
+        if (state) {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(state);
+        }
+        state = nullptr;
+        cap = 0;
+        if (hipMalloc(&state, want * sizeof(unsigned long long)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
+        hipLaunchKernelGGL(k_fill_u64, dim3(unsigned((want + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, state, want, 0ULL);
+        cap = want;
+        return 0;
+    }
 };
 
-// out must hold n+1 entries; out[n] receives the total.
+// out must hold n+1 entries; out[n] receives the total (-1 if it does not fit 42 bits).
 template <class TIn>
-int scanExclusive(const TIn* in, long long* out, long long n, ScanScratch& sc, hipStream_t st, int level = 0) {
+int scanExclusive(const TIn* in, long long* out, long long n, ScanScratch& sc, hipStream_t st) {
     if (n <= 0) {
-        HIP_OK(hipMemsetAsync(out, 0, sizeof(long long), st));
+        hipLaunchKernelGGL(k_scan_empty, dim3(1), dim3(1), 0, st, out);
         return 0;
     }
-    if (level >= 4) return fail(SDL_ERR_CAPACITY, "scan too deep");
-    const long long per = (long long)kBlock * kScanItems;
-    const long long nb = (n + per - 1) / per;
-    if (int r = sc.sums[level].ensure(nb + 1)) return r;
-    if (int r = sc.offs[level].ensure(nb + 1)) return r;
-    long long* sums = sc.sums[level].p;
-    long long* offs = sc.offs[level].p;
-    hipLaunchKernelGGL(k_scan_local<TIn>, dim3(unsigned(nb)), dim3(kBlock), 0, st, in, out, sums, n);
-    if (nb == 1) {
-        hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, st, out, n, sums, 0);  // total = sums[0]
-        return 0;
-    }
-    if (int r = scanExclusive<long long>(sums, offs, nb, sc, st, level + 1)) return r;
-    hipLaunchKernelGGL(k_scan_add, dim3(unsigned(nb)), dim3(kBlock), 0, st, out, offs, n);
-    hipLaunchKernelGGL(k_set_total, dim3(1), dim3(1), 0, st, out, n, offs, nb);
+    const long long nTiles = (n + kScanTile - 1) / kScanTile;
+    if (nTiles > (1LL << 30)) return fail(SDL_ERR_CAPACITY, "scan too large");
+    if (int r = sc.ensure((unsigned long long)nTiles, st)) return r;
+    hipLaunchKernelGGL(k_scan<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, n, int(nTiles), sc.state,
+                       sc.cap, sc.ctl);
     return 0;
 }
 
@@ -757,61 +874,57 @@ __global__ void k_tcl_write(int nTL, const int* tlOutArr, const long long* tlSta
 // exactly these candidates (D's module is connected to C's because s3..s4 and the
 // bridge are module-map objects), so: for every tracklet (b, s4) with b = C->D,
 // the TCs are {TPs ending at C} x {TP(s3, s4)} when the SG s3 = D->s4.inner exists.
+//
+// The TPs ending at each MD are listed once.  Entry k of the SG-end lists is the
+// pair (MD m, SG s2 ending at m); it owns the TPs (s1, s2) for the SGs s1 ending
+// at s2's inner MD, so the list of MD m is the range [tpeOff[sgEndStart[m]],
+// tpeOff[sgEndStart[m+1]]) of the scan tpeOff of the per-entry counts.
+__global__ void k_tpe_count(int nSG, const int* sgEndList, const long long* sgEndStart, const int* sgIn, int* cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nSG) return;
+    const int mx = sgIn[sgEndList[k]];
+    cnt[k] = int(sgEndStart[mx + 1] - sgEndStart[mx]);
+}
+// Largest k in [0, n) with off[k] <= j, for off[0] <= j < off[n].
+__device__ __forceinline__ long long rangeOf(const long long* off, long long n, long long j) {
+    long long lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const long long mid = (lo + hi) >> 1;
+        if (off[mid] <= j) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+// One thread per listed TP (nTP of them in total).
+__global__ void k_tpe_write(long long nTP, int nSG, const long long* tpeOff, const int* sgEndList,
+                            const long long* sgEndStart, const int* sgIn, const long long* sgStart,
+                            const long long* tpStart, int* tpEndList) {
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nTP || j >= tpeOff[nSG]) return;  // the two totals are equal (every TP (s1, s2) is listed once)
+    const long long k = rangeOf(tpeOff, nSG, j);
+    const int s2 = sgEndList[k];
+    const int mx = sgIn[s2];
+    const int s1 = sgEndList[sgEndStart[mx] + (j - tpeOff[k])];
+    tpEndList[j] = int(tpStart[s1] + (s2 - sgStart[mx]));
+}
+// Per tracklet: the bridge test (s3 exists), the number of TCs and where their
+// inner TPs are listed; info = (first listed inner TP, outer TP).
 __global__ void k_tcp_count(int nTL, const int* tlIn, const int* tlOutArr, const int* sgIn, const int* sgOut,
-                            const long long* sgStart, const int* tpEndCount, int* s3Of, int* cnt) {
+                            const long long* sgStart, const long long* sgEndStart, const long long* tpeOff,
+                            const long long* tpStart, int2* info, int* cnt) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nTL) return;
     const int b = tlIn[l], s4 = tlOutArr[l];
     const int mdC = sgIn[b], mdD = sgOut[b], mdE = sgIn[s4];
     int s3 = -1;
     for (long long k = sgStart[mdD]; k < sgStart[mdD + 1]; ++k)
-        if (sgOut[k] == mdE) { s3 = int(k); break; }
-    s3Of[l] = s3;
-    cnt[l] = s3 >= 0 ? tpEndCount[mdC] : 0;
-}
-// # TPs ending at MD m = sum over SGs s2 ending at m of # SGs ending at s2's inner MD
-__global__ void k_tp_end_count(int nMD, const long long* sgEndStart, const int* sgEndList, const int* sgIn,
-                               int* tpEndCount) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nMD) return;
-    int c = 0;
-    for (long long k = sgEndStart[m]; k < sgEndStart[m + 1]; ++k) {
-        const int s2 = sgEndList[k];
-        const int mx = sgIn[s2];
-        c += int(sgEndStart[mx + 1] - sgEndStart[mx]);
-    }
-    tpEndCount[m] = c;
-}
-// The TPs ending at each MD, in (s2, s1) order: written once per MD.
-__global__ void k_tp_end_write(int nMD, const long long* sgEndStart, const int* sgEndList, const int* sgIn,
-                               const long long* sgStart, const long long* tpStart, const long long* tpEndStart,
-                               int* tpEndList) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nMD) return;
-    long long d = tpEndStart[m];
-    for (long long k = sgEndStart[m]; k < sgEndStart[m + 1]; ++k) {
-        const int s2 = sgEndList[k];
-        const int mx = sgIn[s2];
-        for (long long q = sgEndStart[mx]; q < sgEndStart[mx + 1]; ++q) {
-            const int s1 = sgEndList[q];
-            tpEndList[d++] = int(tpStart[s1] + (s2 - sgStart[mx]));
+        if (sgOut[k] == mdE) {
+            s3 = int(k);
+            break;
         }
-    }
-}
-// One wave per tracklet copies the TPs ending at its first MD next to TP(s3, s4).
-__global__ void k_tcp_write(int nTL, const int* tlIn, const int* tlOutArr, const int* sgIn, const long long* sgStart,
-                            const long long* tpStart, const long long* tpEndStart, const int* tpEndList,
-                            const int* s3Of, const long long* start, int2* tc) {
-    const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (l >= nTL) return;
-    const int s3 = s3Of[l];
-    if (s3 < 0) return;
-    const int b = tlIn[l], s4 = tlOutArr[l];
-    const int mdC = sgIn[b];
-    const int tOuter = int(tpStart[s3] + (s4 - sgStart[sgIn[s4]]));
-    const long long a = tpEndStart[mdC], n = tpEndStart[mdC + 1] - a, d = start[l];
-    for (long long k = lane; k < n; k += kWave) tc[d + k] = make_int2(tpEndList[a + k], tOuter);
+    const long long a = tpeOff[sgEndStart[mdC]], e = tpeOff[sgEndStart[mdC + 1]];
+    cnt[l] = s3 >= 0 ? int(e - a) : 0;
+    info[l] = make_int2(int(a), s3 >= 0 ? int(tpStart[s3] + (s4 - sgStart[mdE])) : -1);
 }
 
 // ------------------------------------------------------------------ counters
@@ -852,11 +965,32 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(long long n, const T* val
             if (bins[b]) atomicAdd(&counters[((b / kNBins) * kNCounters + which) * kNBins + (b % kNBins)], bins[b]);
 }
 
-// slot of the inner TP's first MD for every TC made from triplets
-__global__ void k_tcp_slot(long long n, const int2* tcp, const int* tpIn, const int* sgIn, const int* mdSlot, int* out) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[i] = mdSlot[sgIn[tpIn[tcp[i].x]]];
+// TCs from triplets, one thread per TC (grid-stride).  Each TC counts at the
+// module of its inner TP (Event.cc:893-897,965) in a per-block LDS histogram.
+__global__ void __launch_bounds__(kBlock) k_tcp_write(long long nTC, int nTL, const long long* start, const int2* info,
+                                                      const int* tpEndList, const int* tpIn, const int* sgIn,
+                                                      const int* mdSlot, int NM, int E, const ModDev* mods, int2* tc,
+                                                      unsigned long long* counters) {
+    __shared__ unsigned long long bins[kMaxLdsBins];
+    const int nb = E * kNBins;
+    const bool lds = nb <= kMaxLdsBins;
+    if (lds)
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0;
+    __syncthreads();
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nTC; i += (long long)gridDim.x * blockDim.x) {
+        const long long l = rangeOf(start, nTL, i);
+        const int2 f = info[l];
+        const int t = tpEndList[f.x + (i - start[l])];
+        tc[i] = make_int2(t, f.y);
+        const int sl = mdSlot[sgIn[tpIn[t]]];
+        const int e = sl / NM, bin = layerBin(mods[sl % NM]);
+        if (lds) atomicAdd(&bins[e * kNBins + bin], 1ULL);
+        else atomicAdd(&counters[(e * kNCounters + 9) * kNBins + bin], 1ULL);
+    }
+    __syncthreads();
+    if (lds)
+        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+            if (bins[b]) atomicAdd(&counters[((b / kNBins) * kNCounters + 9) * kNBins + (b % kNBins)], bins[b]);
 }
 
 // TP candidates of an inner SG at module A: SGs stored at every module connected to A (Event.cc:539-597)
@@ -1070,12 +1204,13 @@ struct sdl_event {
     DBuf<int> sgFlags;
     DBuf<int> sgEndCount, sgEndFill, sgEndList;
     DBuf<long long> sgEndStart;
-    DBuf<int> tpCount, tpIn, tpOut, tpEndCount, tpEndList;
-    DBuf<long long> tpEndStart;
+    DBuf<int> tpCount, tpIn, tpOut, tpeCount, tpEndList;
+    DBuf<long long> tpeOff;
     DBuf<long long> tpStart, slotTpStart;
     DBuf<long long> tlCap, tlCapOff, tlStart, slotTlStart;
     DBuf<int> tlCount, tlSlabOut, tlIn, tlOut;
-    DBuf<int> tcCount, s3Of;
+    DBuf<int> tcCount;
+    DBuf<int2> tcpInfo;
     DBuf<long long> tcpStart, tclStart;
     DBuf<int2> tcp, tcl;
     DBuf<long long> cand;
@@ -1456,41 +1591,34 @@ int runTL(sdl_event* ev) {
 int runTCP(sdl_event* ev) {
     sdl_geom* G = ev->g;
     hipStream_t st = ev->st;
-    const long long nTL = ev->nTL, nMD = ev->nMD;
+    const long long nTL = ev->nTL, nSG = ev->nSG;
     const long long nl = std::max(nTL, 1LL);
-    if (int r = ev->tpEndCount.ensure(std::max(nMD, 1LL))) return r;
-    if (int r = ev->tcCount.ensure(nl)) return r;
-    if (int r = ev->s3Of.ensure(nl)) return r;
-    if (int r = ev->tcpStart.ensure(nl + 1)) return r;
-    if (nMD > 0)
-        hipLaunchKernelGGL(k_tp_end_count, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, int(nMD),
-                           ev->sgEndStart.p, ev->sgEndList.p, ev->sgIn.p, ev->tpEndCount.p);
-    if (int r = ev->tpEndStart.ensure(std::max(nMD, 1LL) + 1)) return r;
+    if (int r = ev->tpeCount.ensure(std::max(nSG, 1LL))) return r;
+    if (int r = ev->tpeOff.ensure(std::max(nSG, 1LL) + 1)) return r;
     if (int r = ev->tpEndList.ensure(std::max(ev->nTP, 1LL))) return r;
-    if (int r = scanExclusive<int>(ev->tpEndCount.p, ev->tpEndStart.p, nMD, ev->scan, st)) return r;
-    if (nMD > 0)
-        hipLaunchKernelGGL(k_tp_end_write, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, int(nMD),
-                           ev->sgEndStart.p, ev->sgEndList.p, ev->sgIn.p, ev->sgStart.p, ev->tpStart.p,
-                           ev->tpEndStart.p, ev->tpEndList.p);
+    if (int r = ev->tcCount.ensure(nl)) return r;
+    if (int r = ev->tcpInfo.ensure(nl)) return r;
+    if (int r = ev->tcpStart.ensure(nl + 1)) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tpe_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), ev->sgEndList.p,
+                           ev->sgEndStart.p, ev->sgIn.p, ev->tpeCount.p);
+    if (int r = scanExclusive<int>(ev->tpeCount.p, ev->tpeOff.p, nSG, ev->scan, st)) return r;
+    if (ev->nTP > 0)
+        hipLaunchKernelGGL(k_tpe_write, dim3(gridFor(ev->nTP, kBlock)), dim3(kBlock), 0, st, ev->nTP, int(nSG),
+                           ev->tpeOff.p, ev->sgEndList.p, ev->sgEndStart.p, ev->sgIn.p, ev->sgStart.p, ev->tpStart.p,
+                           ev->tpEndList.p);
     if (nTL > 0)
         hipLaunchKernelGGL(k_tcp_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlIn.p,
-                           ev->tlOut.p, ev->sgIn.p, ev->sgOut.p, ev->sgStart.p, ev->tpEndCount.p, ev->s3Of.p,
-                           ev->tcCount.p);
+                           ev->tlOut.p, ev->sgIn.p, ev->sgOut.p, ev->sgStart.p, ev->sgEndStart.p, ev->tpeOff.p,
+                           ev->tpStart.p, ev->tcpInfo.p, ev->tcCount.p);
     if (int r = scanExclusive<int>(ev->tcCount.p, ev->tcpStart.p, nTL, ev->scan, st)) return r;
     ev->nTCP = readLL(ev->tcpStart.p + nTL, st);
     if (ev->nTCP < 0) return fail(SDL_ERR_HIP, "readback failed");
     if (int r = ev->tcp.ensure(std::max(ev->nTCP, 1LL))) return r;
-    if (nTL > 0)
-        hipLaunchKernelGGL(k_tcp_write, dim3(gridFor(nTL, kWavesPerBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlIn.p,
-                           ev->tlOut.p, ev->sgIn.p, ev->sgStart.p, ev->tpStart.p, ev->tpEndStart.p, ev->tpEndList.p,
-                           ev->s3Of.p, ev->tcpStart.p, ev->tcp.p);
-    // counters: a TC made from triplets counts at its inner TP's module (Event.cc:961)
-    if (ev->nTCP > 0) {
-        if (int r = ev->itemSlot.ensure(ev->nTCP)) return r;
-        hipLaunchKernelGGL(k_tcp_slot, dim3(gridFor(ev->nTCP, kBlock)), dim3(kBlock), 0, st, ev->nTCP, ev->tcp.p,
-                           ev->tpIn.p, ev->sgIn.p, ev->mdSlot.p, ev->itemSlot.p);
-        accumulate<int>(ev, ev->nTCP, (const int*)nullptr, ev->itemSlot.p, 9);
-    }
+    if (ev->nTCP > 0)
+        hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(ev->nTCP, kBlock), 4096u)), dim3(kBlock), 0, st, ev->nTCP,
+                           int(nTL), ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p, ev->sgIn.p,
+                           ev->mdSlot.p, G->NM, ev->E, G->dMods, ev->tcp.p, ev->counters.p);
     const long long nTP = ev->nTP;
     if (nTP > 0) {
         if (int r = ev->cand.ensure(nTP)) return r;
@@ -1635,26 +1763,10 @@ int sdl_event_create(sdl_geom* g, sdl_event** out) {
 void sdl_event_destroy(sdl_event* ev) {
     if (!ev) return;
     (void)hipStreamSynchronize(ev->st);
-    // DBuf members free on release
-    auto rel = [](auto& b) { b.release(); };
-    rel(ev->bx); rel(ev->by); rel(ev->bz); rel(ev->bidx); rel(ev->bdet); rel(ev->dEvOff);
-    rel(ev->rt); rel(ev->r3); rel(ev->phi); rel(ev->halpha); rel(ev->mdP); rel(ev->sgX); rel(ev->sgE);
-    rel(ev->sgFlags); rel(ev->tpEndList); rel(ev->tpEndStart); rel(ev->hslot); rel(ev->eHi); rel(ev->eLo);
-    rel(ev->slotHitCount); rel(ev->slotFill); rel(ev->sortedHit); rel(ev->unknown); rel(ev->slotHitStart);
-    rel(ev->mdCap); rel(ev->mdCount); rel(ev->mdCapOff); rel(ev->mdStart); rel(ev->mdSlabHits); rel(ev->mdSlabF);
-    rel(ev->mdLo); rel(ev->mdUp); rel(ev->mdAnchor); rel(ev->mdSlot); rel(ev->mdA); rel(ev->mdB); rel(ev->mdC);
-    rel(ev->sgCap); rel(ev->sgCount); rel(ev->sgSlabOut); rel(ev->sgIn); rel(ev->sgOut); rel(ev->sgCapOff);
-    rel(ev->sgStart); rel(ev->slotSgStart); rel(ev->sgSlabF); rel(ev->sgF); rel(ev->sgEndCount); rel(ev->sgEndFill);
-    rel(ev->sgEndList); rel(ev->sgEndStart); rel(ev->tpCount); rel(ev->tpIn); rel(ev->tpOut); rel(ev->tpEndCount);
-    rel(ev->tpStart); rel(ev->slotTpStart); rel(ev->tlCap); rel(ev->tlCapOff); rel(ev->tlStart); rel(ev->slotTlStart);
-    rel(ev->tlCount); rel(ev->tlSlabOut); rel(ev->tlIn); rel(ev->tlOut); rel(ev->tcCount); rel(ev->s3Of);
-    rel(ev->tcpStart); rel(ev->tclStart); rel(ev->tcp); rel(ev->tcl); rel(ev->cand); rel(ev->itemSlot);
-    rel(ev->counters); rel(ev->tmp64); rel(ev->exportBuf);
+    // device buffers are released by their destructors
     for (auto& t : ev->tev)
         for (auto& x : t)
             if (x) (void)hipEventDestroy(x);
-    for (auto& l : ev->scan.sums) l.release();
-    for (auto& l : ev->scan.offs) l.release();
     (void)hipStreamDestroy(ev->st);
     delete ev;
 }

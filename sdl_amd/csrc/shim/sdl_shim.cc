@@ -1,0 +1,167 @@
+// SDL::Event shim over the C-ABI (see sdl_amd/include/SDL/Event.h).
+#include "SDL/Event.h"
+
+#include <cstdio>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+
+#include "sdl_amd.h"
+
+namespace SDL {
+
+TiltedGeometry tiltedGeometry;
+EndcapGeometry endcapGeometry;
+ModuleConnectionMap moduleConnectionMap;
+
+namespace {
+
+std::mutex g_geomMutex;
+sdl_geom* g_geom = nullptr;
+std::string g_geomKey;
+
+// The device geometry for the current three files (rebuilt after a new load()).
+sdl_geom* currentGeometry() {
+    std::lock_guard<std::mutex> lock(g_geomMutex);
+    const std::string key = tiltedGeometry.file() + "\n" + endcapGeometry.file() + "\n" + moduleConnectionMap.file();
+    if (g_geom && key == g_geomKey) return g_geom;
+    sdl_geom* g = nullptr;
+    if (sdl_geom_load(tiltedGeometry.file().c_str(), endcapGeometry.file().c_str(), moduleConnectionMap.file().c_str(),
+                      &g) != SDL_OK)
+        throw std::runtime_error(std::string("SDL: geometry load failed: ") + sdl_last_error());
+    // the previous geometry may still be used by live events: it is kept
+    g_geom = g;
+    g_geomKey = key;
+    return g_geom;
+}
+
+void check(int rc, const char* what) {
+    if (rc != SDL_OK) throw std::runtime_error(std::string("SDL: ") + what + ": " + sdl_last_error());
+}
+
+// Module layer bin of a detId (Module.cc:159-235): barrel 0..5, endcap 6..10.
+int layerBin(unsigned int detId) {
+    const unsigned subdet = (detId >> 25) & 7;
+    if (subdet == 5) return int((detId >> 20) & 7) - 1;
+    return 6 + int((detId >> 18) & 7) - 1;
+}
+bool isLowerSensor(unsigned int d) {
+    const unsigned subdet = (d >> 25) & 7, mod = (d >> 2) & 127;
+    bool inv = false;
+    if (subdet == 4) {
+        const unsigned side = (d >> 23) & 3;
+        inv = side == 1 ? mod % 2 == 1 : (side == 2 ? mod % 2 == 0 : false);
+    } else if (subdet == 5) {
+        const unsigned side = (d >> 18) & 3, layer = (d >> 20) & 7;
+        if (side == 3) inv = layer <= 3 ? mod % 2 == 1 : mod % 2 == 0;
+        else inv = layer <= 2 ? mod % 2 == 1 : (layer == 3 ? mod % 2 == 0 : false);
+    }
+    return inv ? !(d & 1) : (d & 1);
+}
+
+}  // namespace
+
+void TiltedGeometry::load(std::string filename) { file_ = filename; }
+void EndcapGeometry::load(std::string filename) { file_ = filename; }
+void ModuleConnectionMap::load(std::string filename) { file_ = filename; }
+
+Event::Event() { check(sdl_event_create(currentGeometry(), &ev_), "event creation"); }
+
+Event::~Event() {
+    if (ev_) sdl_event_destroy(ev_);
+}
+
+void Event::addHitToModule(Hit hit, unsigned int detId) {
+    const float x = hit.x(), y = hit.y(), z = hit.z();
+    const int idx = hit.idx();
+    check(sdl_event_add_hits(ev_, 1, &x, &y, &z, &idx, &detId), "addHitToModule");
+    ++nHits_;
+    // lower and upper sensors are counted apart (Event.cc:1170-1190)
+    const int b = layerBin(detId);
+    if (b >= 0 && b < 11) ++(isLowerSensor(detId) ? hitsByLayer_ : hitsByLayerUpper_)[b];
+    lastStage_ = -1;
+}
+
+bool Event::run(int stage, int algo, const char* what) {
+    if (algo != 0) {
+        std::printf("SDL::  Warning: %s algorithm %d has no device kernels; stage left empty\n", what, algo);
+        return false;
+    }
+    if (stage != lastStage_ + 1) throw std::logic_error(std::string("SDL: ") + what + " called out of order");
+    check(sdl_event_run(ev_, stage, algo), what);
+    lastStage_ = stage;
+    return true;
+}
+
+void Event::createMiniDoublets(MDAlgo algo) { run(SDL_STAGE_MD, algo, "createMiniDoublets"); }
+void Event::createSegmentsWithModuleMap(SGAlgo algo) { run(SDL_STAGE_SG, algo, "createSegmentsWithModuleMap"); }
+void Event::createTriplets(TPAlgo algo) { run(SDL_STAGE_TP, algo, "createTriplets"); }
+void Event::createTrackletsWithModuleMap(TLAlgo algo) { run(SDL_STAGE_TL, algo, "createTrackletsWithModuleMap"); }
+void Event::createTrackCandidatesFromTriplets(TCAlgo algo) { run(SDL_STAGE_TCP, algo, "createTrackCandidatesFromTriplets"); }
+void Event::createTrackCandidatesFromTracklets(TCAlgo algo) { run(SDL_STAGE_TCL, algo, "createTrackCandidatesFromTracklets"); }
+
+unsigned int Event::counter(int which, int bin) {
+    if (lastStage_ < 0) return 0;
+    unsigned long long c[110];
+    check(sdl_event_layer_counts(ev_, 0, c), "counters");
+    if (bin >= 0) return (unsigned int)c[which * 11 + bin];
+    unsigned long long s = 0;
+    for (int b = 0; b < 11; ++b) s += c[which * 11 + b];
+    return (unsigned int)s;
+}
+
+unsigned int Event::getNumberOfHits() { return nHits_; }
+unsigned int Event::getNumberOfHitsByLayerBarrel(unsigned int l) { return hitsByLayer_.at(l); }
+unsigned int Event::getNumberOfHitsByLayerEndcap(unsigned int l) { return hitsByLayer_.at(6 + l); }
+unsigned int Event::getNumberOfHitsByLayerBarrelUpperModule(unsigned int l) { return hitsByLayerUpper_.at(l); }
+unsigned int Event::getNumberOfHitsByLayerEndcapUpperModule(unsigned int l) { return hitsByLayerUpper_.at(6 + l); }
+unsigned int Event::getNumberOfMiniDoublets() { return counter(5, -1); }
+unsigned int Event::getNumberOfSegments() { return counter(6, -1); }
+unsigned int Event::getNumberOfTriplets() { return counter(7, -1); }
+unsigned int Event::getNumberOfTracklets() { return counter(8, -1); }
+unsigned int Event::getNumberOfTrackCandidates() { return counter(9, -1); }
+unsigned int Event::getNumberOfMiniDoubletCandidates() { return counter(0, -1); }
+unsigned int Event::getNumberOfSegmentCandidates() { return counter(1, -1); }
+unsigned int Event::getNumberOfTripletCandidates() { return counter(2, -1); }
+unsigned int Event::getNumberOfTrackletCandidates() { return counter(3, -1); }
+unsigned int Event::getNumberOfTrackCandidateCandidates() { return counter(4, -1); }
+unsigned int Event::getNumberOfMiniDoubletCandidatesByLayerBarrel(unsigned int l) { return counter(0, int(l)); }
+unsigned int Event::getNumberOfSegmentCandidatesByLayerBarrel(unsigned int l) { return counter(1, int(l)); }
+unsigned int Event::getNumberOfTripletCandidatesByLayerBarrel(unsigned int l) { return counter(2, int(l)); }
+unsigned int Event::getNumberOfTrackletCandidatesByLayerBarrel(unsigned int l) { return counter(3, int(l)); }
+unsigned int Event::getNumberOfTrackCandidateCandidatesByLayerBarrel(unsigned int l) { return counter(4, int(l)); }
+unsigned int Event::getNumberOfMiniDoubletCandidatesByLayerEndcap(unsigned int l) { return counter(0, 6 + int(l)); }
+unsigned int Event::getNumberOfSegmentCandidatesByLayerEndcap(unsigned int l) { return counter(1, 6 + int(l)); }
+unsigned int Event::getNumberOfTripletCandidatesByLayerEndcap(unsigned int l) { return counter(2, 6 + int(l)); }
+unsigned int Event::getNumberOfTrackletCandidatesByLayerEndcap(unsigned int l) { return counter(3, 6 + int(l)); }
+unsigned int Event::getNumberOfTrackCandidateCandidatesByLayerEndcap(unsigned int l) { return counter(4, 6 + int(l)); }
+unsigned int Event::getNumberOfMiniDoubletsByLayerBarrel(unsigned int l) { return counter(5, int(l)); }
+unsigned int Event::getNumberOfSegmentsByLayerBarrel(unsigned int l) { return counter(6, int(l)); }
+unsigned int Event::getNumberOfTripletsByLayerBarrel(unsigned int l) { return counter(7, int(l)); }
+unsigned int Event::getNumberOfTrackletsByLayerBarrel(unsigned int l) { return counter(8, int(l)); }
+unsigned int Event::getNumberOfTrackCandidatesByLayerBarrel(unsigned int l) { return counter(9, int(l)); }
+unsigned int Event::getNumberOfMiniDoubletsByLayerEndcap(unsigned int l) { return counter(5, 6 + int(l)); }
+unsigned int Event::getNumberOfSegmentsByLayerEndcap(unsigned int l) { return counter(6, 6 + int(l)); }
+unsigned int Event::getNumberOfTripletsByLayerEndcap(unsigned int l) { return counter(7, 6 + int(l)); }
+unsigned int Event::getNumberOfTrackletsByLayerEndcap(unsigned int l) { return counter(8, 6 + int(l)); }
+unsigned int Event::getNumberOfTrackCandidatesByLayerEndcap(unsigned int l) { return counter(9, 6 + int(l)); }
+
+template <int W>
+std::vector<std::array<int, W>> Event::objects(int stage) {
+    std::vector<std::array<int, W>> out;
+    if (stage > lastStage_) return out;
+    long long n = 0;
+    check(sdl_event_export(ev_, 0, stage, nullptr, 0, &n), "export");
+    out.resize(size_t(n));
+    if (n) check(sdl_event_export(ev_, 0, stage, out.front().data(), n * W, &n), "export");
+    return out;
+}
+
+std::vector<std::array<int, 2>> Event::getMiniDoublets() { return objects<2>(SDL_STAGE_MD); }
+std::vector<std::array<int, 4>> Event::getSegments() { return objects<4>(SDL_STAGE_SG); }
+std::vector<std::array<int, 8>> Event::getTriplets() { return objects<8>(SDL_STAGE_TP); }
+std::vector<std::array<int, 8>> Event::getTracklets() { return objects<8>(SDL_STAGE_TL); }
+std::vector<std::array<int, 16>> Event::getTrackCandidatesFromTriplets() { return objects<16>(SDL_STAGE_TCP); }
+std::vector<std::array<int, 16>> Event::getTrackCandidatesFromTracklets() { return objects<16>(SDL_STAGE_TCL); }
+
+}  // namespace SDL

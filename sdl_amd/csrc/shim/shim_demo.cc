@@ -1,0 +1,56 @@
+// A host program written against the SDL::Event API (as a user of the reference
+// would), running one generated event through the MI355X library.
+// Usage: sdl_shim_demo <workload> <event> <geomdir>
+// Prints the counters and a checksum of every object set as one line each.
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "SDL/Event.h"
+#include "sdl_gen.h"
+
+template <class V>
+unsigned long long checksum(const V& v) {
+    unsigned long long s = 0;
+    for (const auto& a : v) {
+        unsigned long long h = 1469598103934665603ULL;
+        for (int x : a) h = (h ^ (unsigned)x) * 1099511628211ULL;
+        s += h;  // order-independent
+    }
+    return s;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 4) return 2;
+    const int workload = std::atoi(argv[1]);
+    const int ievent = std::atoi(argv[2]);
+    const std::string dir = argv[3];
+    const int kind = (workload == 1 || workload == 4) ? 1 : 2;
+    sdlgen::Geometry g;
+    sdlgen::buildGeometry(kind, g, kind == 1 ? 50000 : 200000);
+    const std::string t = dir + "/shim_tilted.txt", e = dir + "/shim_endcap.txt", m = dir + "/shim_map.txt";
+    sdlgen::writeGeometryFiles(g, t, e, m);
+    SDL::tiltedGeometry.load(t);
+    SDL::endcapGeometry.load(e);
+    SDL::moduleConnectionMap.load(m);
+    sdlgen::Event ev;
+    sdlgen::generateEvent(g, sdlgen::presetConfig(workload), ievent, ev);
+
+    SDL::Event event;
+    for (size_t i = 0; i < ev.x.size(); ++i) event.addHitToModule(SDL::Hit(ev.x[i], ev.y[i], ev.z[i], ev.idx[i]), ev.detId[i]);
+    event.createMiniDoublets();
+    event.createSegmentsWithModuleMap();
+    event.createTriplets();
+    event.createTrackletsWithModuleMap();
+    event.createTrackCandidatesFromTriplets();
+    event.createTrackCandidatesFromTracklets();
+    std::printf("counts %u %u %u %u %u %u %u %u %u %u\n", event.getNumberOfMiniDoubletCandidates(),
+                event.getNumberOfSegmentCandidates(), event.getNumberOfTripletCandidates(),
+                event.getNumberOfTrackletCandidates(), event.getNumberOfTrackCandidateCandidates(),
+                event.getNumberOfMiniDoublets(), event.getNumberOfSegments(), event.getNumberOfTriplets(),
+                event.getNumberOfTracklets(), event.getNumberOfTrackCandidates());
+    std::printf("checksums %llu %llu %llu %llu %llu %llu\n", checksum(event.getMiniDoublets()),
+                checksum(event.getSegments()), checksum(event.getTriplets()), checksum(event.getTracklets()),
+                checksum(event.getTrackCandidatesFromTriplets()), checksum(event.getTrackCandidatesFromTracklets()));
+    return 0;
+}

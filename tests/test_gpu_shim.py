@@ -1,0 +1,40 @@
+"""The C++ SDL::Event shim (sdl_amd/include/SDL/Event.h) driven the way a user
+of the reference drives SDL::Event (Event.h:110-300): hits added one by one
+with addHitToModule, the six stages called in order, counters and objects read
+back.  Its results must equal the oracle's (CR libm policy, as for the C-ABI)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from sdl_amd import build, gen
+
+pytestmark = pytest.mark.gpu
+
+DEMO = os.path.join(build.LIB, "sdl_shim_demo")
+
+
+def _checksum(rows):
+    s = 0
+    for r in np.asarray(rows, dtype=np.int64).reshape(len(rows), -1) if len(rows) else []:
+        h = 1469598103934665603
+        for x in r:
+            h = ((h ^ (int(x) & 0xFFFFFFFF)) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+        s = (s + h) & 0xFFFFFFFFFFFFFFFF
+    return s
+
+
+@pytest.mark.parametrize("wl,e", [(12, 0), (15, 1)])
+def test_shim_matches_oracle(wl, e, tmp_path):
+    print(f"case shim workload={wl} event={e}", flush=True)
+    assert os.path.exists(DEMO), "shim demo not built"
+    out = subprocess.run([DEMO, str(wl), str(e), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = dict(l.split(" ", 1) for l in out.stdout.strip().splitlines())
+    counts = [int(v) for v in lines["counts"].split()]
+    sums = [int(v) for v in lines["checksums"].split()]
+    ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], gen.event(wl, e), oracle.CR)
+    assert counts == list(cnt.values())
+    assert sums == [_checksum(ref[s]) for s in oracle.STAGES]
