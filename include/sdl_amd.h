@@ -85,7 +85,11 @@ int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offse
  * stage whose inputs are missing returns SDL_ERR_STATE).  algo: 0 = Default,
  * the only algorithm with device kernels. */
 int sdl_event_run(sdl_event* ev, int stage, int algo);
-/* Runs MD, SG, TP, TL, TCP and TCL back to back. */
+/* Runs MD, SG, TP, TL, TCP and TCL back to back.  After a first call has
+ * measured the batch's object counts, later calls size their buffers from
+ * them (scaled by the hit count, 25 % headroom) and enqueue the whole chain
+ * with one host round trip; if the kernels find a buffer too small, the
+ * chain resumes from that stage with buffers sized exactly (same results). */
 int sdl_event_run_chain(sdl_event* ev);
 /* Waits for the handle's stream. */
 int sdl_event_sync(sdl_event* ev);
@@ -114,6 +118,10 @@ void* sdl_event_stream(sdl_event* ev);
  * (ms, in that order, n <= 4) for the last run. */
 int sdl_event_set_timing(sdl_event* ev, int on);
 int sdl_event_kernel_ms(sdl_event* ev, double* out, int n);
+/* Tuning knob with no counterpart in the reference: the headroom factor of the
+ * predicted buffer sizes for the stages >= fromStage (default 1.25 for all).
+ * A factor below 1 forces the resume path; the tests use it to check it. */
+int sdl_event_set_headroom(sdl_event* ev, int fromStage, double factor);
 
 #ifdef __cplusplus
 }

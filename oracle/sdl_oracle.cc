@@ -1093,6 +1093,9 @@ void fillEvent(Event& ev, const Geometry* g, int libm, int n, const float* x, co
 
 }  // namespace oracle
 
+// The op-counting build (opcount.cc) includes this file with float / double
+// replaced by counting types and provides its own entry point instead.
+#ifndef ORACLE_OPCOUNT
 using namespace oracle;
 
 extern "C" {
@@ -1216,3 +1219,4 @@ double oracle_time(void* geom, int nevents, const long* offsets, const float* x,
 }
 
 }  // extern "C"
+#endif  // ORACLE_OPCOUNT

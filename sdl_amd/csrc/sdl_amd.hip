@@ -77,6 +77,45 @@ struct DBuf {
     }
 };
 
+// ------------------------------------------------------------------ run totals
+// Totals of one run, written on the device by the scans (and read back by the
+// host once per call).  Kernels take their item counts from here, so a whole
+// chain can be enqueued without a host round trip.  Buffers sized ahead of
+// time are checked against the totals by the kernel that would fill them
+// (Guard): on overflow it raises TOT_ABORT and writes nothing, every later
+// kernel of the call then sees zero items, and the host runs the call again
+// with buffers sized from the totals.
+enum {
+    TOT_MDCAP = 0, TOT_MD, TOT_SGCAP, TOT_SG, TOT_TP, TOT_TLCAP, TOT_TL, TOT_TPE, TOT_TCP, TOT_TCL,
+    TOT_ABORT, TOT_UNKNOWN, TOT_N
+};
+
+// Number of work items: fixed by the host (idx < 0) or a run total.
+struct Count {
+    long long host;
+    const long long* tot;
+    int idx;
+    __device__ __forceinline__ long long get() const {
+        if (tot[TOT_ABORT]) return 0;
+        return idx < 0 ? host : tot[idx];
+    }
+};
+// A buffer of `cap` entries about to receive tot[idx] entries.
+struct Guard {
+    long long* tot;
+    int idx;
+    long long cap;
+    int stage;
+    // true (and the run aborted) when the total does not fit; uniform over the grid
+    __device__ __forceinline__ bool over() const {
+        if (tot[TOT_ABORT]) return true;
+        const long long t = tot[idx];
+        if (t >= 0 && t <= cap) return false;
+        if (threadIdx.x == 0) tot[TOT_ABORT] = 1 + stage;
+        return true;
+    }
+};
+
 // ------------------------------------------------------------------ scans
 // Exclusive scan of n counts into n+1 offsets (out[n] = total) in one launch:
 // single-pass chained scan with decoupled look-back.  Tiles take their index
@@ -86,6 +125,9 @@ struct DBuf {
 // [epoch:20][flag:2][value:42]; the epoch changes after every scan (the last
 // tile to finish advances it and re-arms the ticket), so the state array needs
 // no clearing between scans.  Totals of 2^42 or more are reported as -1.
+// A state word carries its own value, so publishing and polling are relaxed
+// device-scope atomics: no fence (which would write back or invalidate the
+// XCD's L2) is needed.
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr unsigned long long kScanValMask = (1ULL << 42) - 1;
@@ -104,20 +146,27 @@ __device__ __forceinline__ unsigned long long scanPack(unsigned epoch, unsigned 
 }
 
 template <class TIn>
-__global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, long long n, int nTiles,
+__global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, Count cn, int nTiles,
                                                  unsigned long long* state, unsigned long long stateCap,
-                                                 ScanCtl* ctl) {
+                                                 ScanCtl* ctl, long long* tot, int totIdx) {
     __shared__ long long waveSum[kWavesPerBlock];
     __shared__ long long prefixSh;
     __shared__ int tileSh;
     __shared__ unsigned epochSh;
     if (threadIdx.x == 0) {
-        tileSh = int(atomicAdd(&ctl->ticket, 1ULL));
+        tileSh = int(__hip_atomic_fetch_add(&ctl->ticket, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         epochSh = unsigned(__hip_atomic_load(&ctl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
     __syncthreads();
     const int tile = tileSh;
     const unsigned epoch = epochSh;
+    // every tile of the launch takes part (the ticket protocol needs all of
+    // them); tiles past the end of the input scan zeros
+    long long n = cn.get();
+    if (n > (long long)nTiles * kScanTile) {  // launched for fewer items than the total: abort the run
+        if (threadIdx.x == 0) tot[TOT_ABORT] = 1 + 100;
+        n = 0;
+    }
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const long long base = (long long)tile * kScanTile + (long long)threadIdx.x * kScanItems;
     long long v[kScanItems];
@@ -145,9 +194,9 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
     if (wave == 0) {
         long long excl = 0;
         if (tile == 0) {
-            if (lane == 0) __hip_atomic_store(&state[0], scanPack(epoch, SCAN_INCL, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) __hip_atomic_store(&state[0], scanPack(epoch, SCAN_INCL, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            if (lane == 0) __hip_atomic_store(&state[tile], scanPack(epoch, SCAN_AGG, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) __hip_atomic_store(&state[tile], scanPack(epoch, SCAN_AGG, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int j = tile - 1;
             for (;;) {
                 const int idx = j - lane;
@@ -155,7 +204,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
                 bool ready = idx < 0;
                 for (;;) {
                     if (!ready) {
-                        w = __hip_atomic_load(&state[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                        w = __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         ready = unsigned(w >> 44) == epoch && ((w >> 42) & 3u) != 0;
                     }
                     if (__all(ready)) break;
@@ -174,7 +223,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
                 j -= kWave;
             }
             if (lane == 0)
-                __hip_atomic_store(&state[tile], scanPack(epoch, SCAN_INCL, excl + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&state[tile], scanPack(epoch, SCAN_INCL, excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) prefixSh = excl;
     }
@@ -189,20 +238,20 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
     }
     if (tile == nTiles - 1 && threadIdx.x == 0) {
         const long long total = prefix + agg;
-        out[n] = (unsigned long long)prefix >= kScanValMask || (unsigned long long)total >= kScanValMask ? -1 : total;
+        const long long t = (unsigned long long)prefix >= kScanValMask || (unsigned long long)total >= kScanValMask ? -1 : total;
+        out[n] = t;
+        if (totIdx >= 0) tot[totIdx] = t;
     }
     // the last tile to finish re-arms the ticket and advances the epoch
     __syncthreads();
     __shared__ bool lastSh;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        lastSh = atomicAdd(&ctl->done, 1ULL) == (unsigned long long)(nTiles - 1);
-    }
+    if (threadIdx.x == 0)
+        lastSh = __hip_atomic_fetch_add(&ctl->done, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                 (unsigned long long)(nTiles - 1);
     __syncthreads();
     if (!lastSh) return;
     if (epoch == kScanEpochMax) {
         for (unsigned long long i = threadIdx.x; i < stateCap; i += blockDim.x) state[i] = 0;
-        __threadfence();
     }
     if (threadIdx.x == 0) {
         ctl->ticket = 0;
@@ -215,7 +264,6 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long n, unsigned
     const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
-__global__ void k_scan_empty(long long* out) { out[0] = 0; }
 
 struct ScanScratch {
     unsigned long long* state = nullptr;
@@ -249,18 +297,17 @@ struct ScanScratch {
     }
 };
 
-// out must hold n+1 entries; out[n] receives the total (-1 if it does not fit 42 bits).
+// out must hold n+1 entries; out[n] (and tot[totIdx], if totIdx >= 0) receives
+// the total (-1 if it does not fit 42 bits).  nMax bounds the item count on
+// the host (it sizes the launch); the count itself may live on the device.
 template <class TIn>
-int scanExclusive(const TIn* in, long long* out, long long n, ScanScratch& sc, hipStream_t st) {
-    if (n <= 0) {
-        hipLaunchKernelGGL(k_scan_empty, dim3(1), dim3(1), 0, st, out);
-        return 0;
-    }
-    const long long nTiles = (n + kScanTile - 1) / kScanTile;
+int scanExclusive(const TIn* in, long long* out, Count cn, long long nMax, ScanScratch& sc, hipStream_t st,
+                  long long* tot, int totIdx) {
+    const long long nTiles = std::max(1LL, (nMax + kScanTile - 1) / kScanTile);
     if (nTiles > (1LL << 30)) return fail(SDL_ERR_CAPACITY, "scan too large");
     if (int r = sc.ensure((unsigned long long)nTiles, st)) return r;
-    hipLaunchKernelGGL(k_scan<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, n, int(nTiles), sc.state,
-                       sc.cap, sc.ctl);
+    hipLaunchKernelGGL(k_scan<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, cn, int(nTiles), sc.state,
+                       sc.cap, sc.ctl, tot, totIdx);
     return 0;
 }
 
@@ -268,7 +315,7 @@ int scanExclusive(const TIn* in, long long* out, long long n, ScanScratch& sc, h
 __global__ void k_hit_prep(int nh, const float* x, const float* y, const float* z, const unsigned* detId,
                            const long long* evOff, int E, const uint32_t* sortedDet, int NM, const ModDev* mods,
                            float* rt, float* r3, float* phi, float* alpha, int* slot, float4* eHi, float4* eLo,
-                           int* slotCount, int* unknown) {
+                           int* slotCount, long long* tot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nh) return;
     // event of the hit
@@ -286,7 +333,7 @@ __global__ void k_hit_prep(int nh, const float* x, const float* y, const float* 
         if (sortedDet[mid] < d) a = mid + 1; else b = mid;
     }
     if (a >= NM || sortedDet[a] != d) {
-        atomicAdd(unknown, 1);
+        atomicAdd((unsigned long long*)&tot[TOT_UNKNOWN], 1ULL);
         slot[i] = -1;
         return;
     }
@@ -322,7 +369,8 @@ __global__ void k_hit_scatter(int nh, const int* slot, const long long* slotStar
 // (Module::addHit appends, Module.cc:128-137) and makes atomically scattered
 // lists deterministic.  One wave per segment; up to 64 elements are ranked in
 // registers, longer segments are sorted by the wave's first lane.
-__global__ void k_sort_segments(int nseg, const long long* start, int* vals) {
+__global__ void k_sort_segments(Count cseg, const long long* start, int* vals) {
+    const int nseg = int(cseg.get());
     const int s = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (s >= nseg) return;
@@ -381,7 +429,8 @@ struct HitsDev {
 
 __global__ void __launch_bounds__(kBlock) k_md(int NS, int NM, const ModDev* mods, HitsDev H, const long long* slotHitStart,
                                                const int* sortedHit, const int* mdCap, const long long* mdCapOff,
-                                               int2* slabHits, float4* slabF, int* mdCount) {
+                                               int2* slabHits, float4* slabF, int* mdCount, Guard g) {
+    if (g.over()) return;
     const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (item >= NS) return;
@@ -434,7 +483,8 @@ struct MDDev {
 // (MiniDoublet.cc:60-81).
 __global__ void k_md_repack(int NS, int NM, const ModDev* mods, HitsDev H, const long long* mdCapOff,
                             const int* mdCount, const long long* mdStart, const int2* slabHits, const float4* slabF,
-                            MDDev D) {
+                            MDDev D, Guard g) {
+    if (g.over()) return;
     const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (item >= NS) return;
@@ -484,8 +534,9 @@ __device__ __forceinline__ HitView loadAnchor(int i, const MDDev& D) {
 
 // ------------------------------------------------------------------ SG
 // SG candidates of one inner MD: MDs of every connected lower module (Event.cc:457-522).
-__global__ void k_sg_cap(int nMD, int NM, const int* mdSlot, const int* connStart, const int* connList,
+__global__ void k_sg_cap(Count cMD, int NM, const int* mdSlot, const int* connStart, const int* connList,
                          const int* mdCount, int* sgCap) {
+    const int nMD = int(cMD.get());
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nMD) return;
     const int s = mdSlot[i];
@@ -514,10 +565,12 @@ __device__ __forceinline__ int findEntry(int k, int pre, int nEntries, int& inne
     return e;
 }
 
-__global__ void __launch_bounds__(kBlock) k_sg(int nMD, int NM, const ModDev* mods, HitsDev H, MDDev D,
+__global__ void __launch_bounds__(kBlock) k_sg(Count cMD, int NM, const ModDev* mods, HitsDev H, MDDev D,
                                                const int* connStart, const int* connList, const long long* mdStart,
                                                const int* mdCount, const int* sgCap, const long long* sgCapOff,
-                                               int* slabOut, SGF* slabF, int* sgCount) {
+                                               int* slabOut, SGF* slabF, int* sgCount, Guard g) {
+    if (g.over()) return;
+    const int nMD = int(cMD.get());
     const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (item >= nMD) return;
@@ -590,7 +643,8 @@ struct SGDev {
 };
 
 // Per-segment quantities that the tracklet algebra reads for either role.
-__global__ void k_sg_extra(int nSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S) {
+__global__ void k_sg_extra(Count cSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S) {
+    const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSG) return;
     const int mi = S.in[s], mo = S.out[s];
@@ -611,8 +665,10 @@ __global__ void k_sg_extra(int nSG, int NM, const ModDev* mods, HitsDev H, MDDev
     S.e[s] = e;
 }
 
-__global__ void k_sg_repack(int nMD, const long long* sgCapOff, const int* sgCount, const long long* sgStart,
-                            const int* slabOut, const SGF* slabF, SGDev S) {
+__global__ void k_sg_repack(Count cMD, const long long* sgCapOff, const int* sgCount, const long long* sgStart,
+                            const int* slabOut, const SGF* slabF, SGDev S, Guard g) {
+    if (g.over()) return;
+    const int nMD = int(cMD.get());
     const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (item >= nMD) return;
@@ -626,19 +682,22 @@ __global__ void k_sg_repack(int nMD, const long long* sgCapOff, const int* sgCou
 }
 
 // Per-slot ranges of objects ordered by an inner MD: first = start[mdStart[slot]].
-__global__ void k_slot_range(int NS, const long long* mdStart, const long long* objStartByMd, long long* slotObjStart) {
+__global__ void k_slot_range(Count cNS, const long long* mdStart, const long long* objStartByMd, long long* slotObjStart) {
+    const int NS = int(cNS.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s > NS) return;
     slotObjStart[s] = objStartByMd[mdStart[s]];
 }
 
 // SGs ending at each MD (reverse index, used by the joins)
-__global__ void k_count_by(int n, const int* key, int* count) {
+__global__ void k_count_by(Count cn, const int* key, int* count) {
+    const int n = int(cn.get());
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     atomicAdd(&count[key[i]], 1);
 }
-__global__ void k_scatter_by(int n, const int* key, const long long* start, int* fill, int* list) {
+__global__ void k_scatter_by(Count cn, const int* key, const long long* start, int* fill, int* list) {
+    const int n = int(cn.get());
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int k = key[i];
@@ -650,14 +709,17 @@ __global__ void k_scatter_by(int n, const int* key, const long long* start, int*
 // Triplet = two SGs sharing the middle MD (Triplet.cc:50-122, Segment.cc:861-866).
 // The inner SG s ends at MD m; the outer SGs stored at m's module with inner MD m
 // are exactly the dense range of SGs whose inner MD is m.
-__global__ void k_tp_count(int nSG, const int* sgOut, const long long* sgStart, int* tpCount) {
+__global__ void k_tp_count(Count cSG, const int* sgOut, const long long* sgStart, int* tpCount) {
+    const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSG) return;
     const int m = sgOut[s];
     tpCount[s] = int(sgStart[m + 1] - sgStart[m]);
 }
-__global__ void k_tp_write(int nSG, const int* sgOut, const long long* sgStart, const long long* tpStart, int* tpIn,
-                           int* tpOutArr) {
+__global__ void k_tp_write(Count cSG, const int* sgOut, const long long* sgStart, const long long* tpStart, int* tpIn,
+                           int* tpOutArr, Guard g) {
+    if (g.over()) return;
+    const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSG) return;
     const int m = sgOut[s];
@@ -671,8 +733,9 @@ __global__ void k_tp_write(int nSG, const int* sgOut, const long long* sgStart, 
 // ------------------------------------------------------------------ TL
 // TL candidates of one inner SG: SGs stored at every module connected to the
 // inner SG's outer-MD module (Event.cc:627-693).
-__global__ void k_tl_cap(int nSG, int NM, const int* sgOut, const int* mdSlot, const int* connStart,
+__global__ void k_tl_cap(Count cSG, int NM, const int* sgOut, const int* mdSlot, const int* connStart,
                          const int* connList, const long long* slotSgStart, long long* tlCap) {
+    const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSG) return;
     const int b = mdSlot[sgOut[s]];
@@ -713,10 +776,12 @@ __device__ __forceinline__ TLOutLo loadOutLo(int mLo, int flags, const MDDev& D)
     return L;
 }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_tl(int nSG, int NM, HitsDev H, MDDev D, SGDev S, const int* connStart,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_tl(Count cSG, int NM, HitsDev H, MDDev D, SGDev S, const int* connStart,
                                                const int* connList, const long long* mdStart, const int* mdCount,
                                                const long long* sgStart, const long long* tlCap,
-                                               const long long* tlCapOff, int* slabOut, int* tlCount) {
+                                               const long long* tlCapOff, int* slabOut, int* tlCount, Guard g) {
+    if (g.over()) return;
+    const int nSG = int(cSG.get());
     __shared__ TLQueue queues[kWavesPerBlock];
     TLQueue& Q = queues[threadIdx.x >> 6];
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
@@ -806,9 +871,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
                     TLOutUp U;
                     const int mUp = S.out[so];
                     U.a = loadAnchor(mUp, D);
-                    const SGF g = S.f[so];
-                    U.dphimin = g.v[1]; U.dphimax = g.v[2];
-                    U.dphichange = g.v[3]; U.dphichangemin = g.v[4]; U.dphichangemax = g.v[5];
+                    const SGF sf = S.f[so];
+                    U.dphimin = sf.v[1]; U.dphimax = sf.v[2];
+                    U.dphichange = sf.v[3]; U.dphichangemin = sf.v[4]; U.dphichangemax = sf.v[5];
                     const float4 sx = S.x[so];
                     U.dr = sx.x; U.d = sx.y; U.aOU = sx.z; U.dPhiPos = sx.w;
                     U.ec2s = flo & 16;
@@ -836,8 +901,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
     if (lane == 0) tlCount[item] = base;
 }
 
-__global__ void k_tl_repack(int nSG, const long long* tlCapOff, const int* tlCount, const long long* tlStart,
-                            const int* slabOut, int* tlIn, int* tlOutArr) {
+__global__ void k_tl_repack(Count cSG, const long long* tlCapOff, const int* tlCount, const long long* tlStart,
+                            const int* slabOut, int* tlIn, int* tlOutArr, Guard g) {
+    if (g.over()) return;
+    const int nSG = int(cSG.get());
     const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (item >= nSG) return;
@@ -853,13 +920,16 @@ __global__ void k_tl_repack(int nSG, const long long* tlCapOff, const int* tlCou
 // TC from tracklets (Event.cc:999-1062, TrackCandidate.cc:143-226): the dR cut
 // never rejects, so the pair passes iff the outer SG of the inner TL is the
 // inner SG of the outer TL; those TLs are the dense range of TLs of that SG.
-__global__ void k_tcl_count(int nTL, const int* tlOutArr, const long long* tlStart, int* cnt) {
+__global__ void k_tcl_count(Count cTL, const int* tlOutArr, const long long* tlStart, int* cnt) {
+    const int nTL = int(cTL.get());
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nTL) return;
     const int s = tlOutArr[l];
     cnt[l] = int(tlStart[s + 1] - tlStart[s]);
 }
-__global__ void k_tcl_write(int nTL, const int* tlOutArr, const long long* tlStart, const long long* start, int2* tc) {
+__global__ void k_tcl_write(Count cTL, const int* tlOutArr, const long long* tlStart, const long long* start, int2* tc, Guard g) {
+    if (g.over()) return;
+    const int nTL = int(cTL.get());
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nTL) return;
     const int s = tlOutArr[l];
@@ -879,7 +949,8 @@ __global__ void k_tcl_write(int nTL, const int* tlOutArr, const long long* tlSta
 // pair (MD m, SG s2 ending at m); it owns the TPs (s1, s2) for the SGs s1 ending
 // at s2's inner MD, so the list of MD m is the range [tpeOff[sgEndStart[m]],
 // tpeOff[sgEndStart[m+1]]) of the scan tpeOff of the per-entry counts.
-__global__ void k_tpe_count(int nSG, const int* sgEndList, const long long* sgEndStart, const int* sgIn, int* cnt) {
+__global__ void k_tpe_count(Count cSG, const int* sgEndList, const long long* sgEndStart, const int* sgIn, int* cnt) {
+    const int nSG = int(cSG.get());
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nSG) return;
     const int mx = sgIn[sgEndList[k]];
@@ -896,9 +967,12 @@ __device__ __forceinline__ long long rangeOf(const long long* off, long long n, 
     return lo;
 }
 // One thread per listed TP (nTP of them in total).
-__global__ void k_tpe_write(long long nTP, int nSG, const long long* tpeOff, const int* sgEndList,
+__global__ void k_tpe_write(Count cTP, Count cSG, const long long* tpeOff, const int* sgEndList,
                             const long long* sgEndStart, const int* sgIn, const long long* sgStart,
-                            const long long* tpStart, int* tpEndList) {
+                            const long long* tpStart, int* tpEndList, Guard g) {
+    if (g.over()) return;
+    const long long nTP = cTP.get();
+    const int nSG = int(cSG.get());
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nTP || j >= tpeOff[nSG]) return;  // the two totals are equal (every TP (s1, s2) is listed once)
     const long long k = rangeOf(tpeOff, nSG, j);
@@ -909,9 +983,10 @@ __global__ void k_tpe_write(long long nTP, int nSG, const long long* tpeOff, con
 }
 // Per tracklet: the bridge test (s3 exists), the number of TCs and where their
 // inner TPs are listed; info = (first listed inner TP, outer TP).
-__global__ void k_tcp_count(int nTL, const int* tlIn, const int* tlOutArr, const int* sgIn, const int* sgOut,
+__global__ void k_tcp_count(Count cTL, const int* tlIn, const int* tlOutArr, const int* sgIn, const int* sgOut,
                             const long long* sgStart, const long long* sgEndStart, const long long* tpeOff,
                             const long long* tpStart, int2* info, int* cnt) {
+    const int nTL = int(cTL.get());
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nTL) return;
     const int b = tlIn[l], s4 = tlOutArr[l];
@@ -942,8 +1017,9 @@ constexpr int kNBins = 11;
 constexpr int kMaxLdsBins = 4096;
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_accumulate(long long n, const T* vals, const int* slotOf, int NM, int E,
+__global__ void __launch_bounds__(kBlock) k_accumulate(Count cn, const T* vals, const int* slotOf, int NM, int E,
                                                        const ModDev* mods, int which, unsigned long long* counters) {
+    const long long n = cn.get();
     __shared__ unsigned long long bins[kMaxLdsBins];
     const int nb = E * kNBins;
     const bool lds = nb <= kMaxLdsBins;
@@ -967,10 +1043,13 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(long long n, const T* val
 
 // TCs from triplets, one thread per TC (grid-stride).  Each TC counts at the
 // module of its inner TP (Event.cc:893-897,965) in a per-block LDS histogram.
-__global__ void __launch_bounds__(kBlock) k_tcp_write(long long nTC, int nTL, const long long* start, const int2* info,
+__global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTC, Count cTL, const long long* start, const int2* info,
                                                       const int* tpEndList, const int* tpIn, const int* sgIn,
                                                       const int* mdSlot, int NM, int E, const ModDev* mods, int2* tc,
-                                                      unsigned long long* counters) {
+                                                      unsigned long long* counters, Guard g) {
+    if (g.over()) return;
+    const long long nTC = cTC.get();
+    const int nTL = int(cTL.get());
     __shared__ unsigned long long bins[kMaxLdsBins];
     const int nb = E * kNBins;
     const bool lds = nb <= kMaxLdsBins;
@@ -994,8 +1073,9 @@ __global__ void __launch_bounds__(kBlock) k_tcp_write(long long nTC, int nTL, co
 }
 
 // TP candidates of an inner SG at module A: SGs stored at every module connected to A (Event.cc:539-597)
-__global__ void k_tp_cand(int nSG, int NM, const int* sgIn, const int* mdSlot, const int* connStart,
+__global__ void k_tp_cand(Count cSG, int NM, const int* sgIn, const int* mdSlot, const int* connStart,
                           const int* connList, const long long* slotSgStart, long long* cand) {
+    const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSG) return;
     const int a = mdSlot[sgIn[s]];
@@ -1010,9 +1090,10 @@ __global__ void k_tp_cand(int nSG, int NM, const int* sgIn, const int* mdSlot, c
 // TC candidates: TC(TP) per inner TP (TPs at modules connected to its outermost MD
 // module, Event.cc:889-982) and TC(TL) per inner TL (TLs at modules connected to its
 // second MD module, Event.cc:999-1062).
-__global__ void k_tc_cand_tp(long long nTP, int NM, const int* tpIn, const int* tpOutArr, const int* sgIn,
+__global__ void k_tc_cand_tp(Count cTP, int NM, const int* tpIn, const int* tpOutArr, const int* sgIn,
                              const int* sgOut, const int* mdSlot, const int* connStart, const int* connList,
                              const long long* slotTpStart, long long* cand, int* slotOfItem) {
+    const long long nTP = cTP.get();
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nTP) return;
     const int c = mdSlot[sgOut[tpOutArr[t]]];
@@ -1025,9 +1106,10 @@ __global__ void k_tc_cand_tp(long long nTP, int NM, const int* tpIn, const int* 
     cand[t] = n;
     slotOfItem[t] = mdSlot[sgIn[tpIn[t]]];
 }
-__global__ void k_tc_cand_tl(long long nTL, int NM, const int* tlIn, const int* sgIn, const int* sgOut,
+__global__ void k_tc_cand_tl(Count cTL, int NM, const int* tlIn, const int* sgIn, const int* sgOut,
                              const int* mdSlot, const int* connStart, const int* connList,
                              const long long* slotTlStart, long long* cand, int* slotOfItem) {
+    const long long nTL = cTL.get();
     const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nTL) return;
     const int b = mdSlot[sgOut[tlIn[l]]];
@@ -1040,15 +1122,17 @@ __global__ void k_tc_cand_tl(long long nTL, int NM, const int* tlIn, const int* 
     cand[l] = n;
     slotOfItem[l] = mdSlot[sgIn[tlIn[l]]];
 }
-__global__ void k_gather_slot(long long n, const int* idxA, const int* idxB, const int* mdSlot, int* out) {
+__global__ void k_gather_slot(Count cn, const int* idxA, const int* idxB, const int* mdSlot, int* out) {
+    const long long n = cn.get();
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     out[i] = mdSlot[idxB ? idxB[idxA[i]] : idxA[i]];
 }
 
 // Per-slot object starts for objects ordered by inner SG: slotStart[s] = start[slotSgStart[s]]
-__global__ void k_slot_range_sg(int NS, const long long* slotSgStart, const long long* objStartBySg,
+__global__ void k_slot_range_sg(Count cNS, const long long* slotSgStart, const long long* objStartBySg,
                                 long long* slotObjStart) {
+    const int NS = int(cNS.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s > NS) return;
     slotObjStart[s] = objStartBySg[slotSgStart[s]];
@@ -1179,6 +1263,17 @@ struct sdl_event {
     int stage = -1;   // last completed stage
     bool hitsReady = false;
     long long nMD = 0, nSG = 0, nTP = 0, nTL = 0, nTCP = 0, nTCL = 0;
+    // run totals on the device; sizes of the current call (exact or predicted)
+    DBuf<long long> tot;
+    bool spec = false;
+    long long need[TOT_N] = {};
+    // totals of the last completed call, to predict buffer sizes from
+    bool haveHistory = false;
+    long long lastTot[TOT_N] = {};
+    long long lastNH = 0;
+    // prediction headroom: factor for the totals of stages >= headroomFrom
+    int headroomFrom = SDL_STAGE_TCL + 1;
+    double headroomFactor = 1.25;
     // device buffers
     DBuf<float> bx, by, bz;
     DBuf<int> bidx;
@@ -1187,7 +1282,7 @@ struct sdl_event {
     DBuf<float> rt, r3, phi, halpha;
     DBuf<int> hslot;
     DBuf<float4> eHi, eLo;
-    DBuf<int> slotHitCount, slotFill, sortedHit, unknown;
+    DBuf<int> slotHitCount, slotFill, sortedHit;
     DBuf<long long> slotHitStart;
     DBuf<int> mdCap, mdCount;
     DBuf<long long> mdCapOff, mdStart;
@@ -1345,7 +1440,6 @@ int ensureHits(sdl_event* ev) {
     if (int r = ev->slotHitCount.ensure(NS)) return r;
     if (int r = ev->slotFill.ensure(NS)) return r;
     if (int r = ev->slotHitStart.ensure(NS + 1)) return r;
-    if (int r = ev->unknown.ensure(1)) return r;
     if (int r = ev->counters.ensure(size_t(ev->E) * kNCounters * kNBins)) return r;
     return 0;
 }
@@ -1360,24 +1454,48 @@ SGDev sgView(sdl_event* ev) {
     return SGDev{ev->sgIn.p, ev->sgOut.p, ev->sgF.p, ev->sgX.p, ev->sgE.p, ev->sgFlags.p};
 }
 
+Count hostCount(sdl_event* ev, long long n) { return Count{n, ev->tot.p, -1}; }
+Count devCount(sdl_event* ev, int k) { return Count{0, ev->tot.p, k}; }
+Guard guardOf(sdl_event* ev, int k, long long cap, int stage) { return Guard{ev->tot.p, k, cap, stage}; }
+
+// Host-side size of run total k for this call: read back (exact mode) or
+// predicted from the last call, scaled by the hit count, with 25 % headroom
+// (speculative mode; kernels check it and the call runs again on overflow).
+long long sizeOf(sdl_event* ev, int k) {
+    long long v;
+    if (!ev->spec) {
+        v = readLL(ev->tot.p + k, ev->st);
+    } else {
+        static const int stageOf[TOT_N] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
+                                           SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
+                                           0, 0};
+        const double scale = ev->lastNH > 0 ? double(ev->NH) / double(ev->lastNH) : 1.0;
+        const bool custom = stageOf[k] >= ev->headroomFrom;
+        const double f = custom ? ev->headroomFactor : 1.25;
+        v = (long long)(double(ev->lastTot[k]) * scale * f) + (custom && f < 1.0 ? 0 : 1024);
+    }
+    ev->need[k] = v;
+    return v;
+}
+
 template <class T>
-int accumulate(sdl_event* ev, long long n, const T* vals, const int* slotOf, int which) {
-    if (n <= 0) return 0;
-    const unsigned grid = std::min<unsigned>(gridFor(n, kBlock), 1024u);
-    hipLaunchKernelGGL(k_accumulate<T>, dim3(grid), dim3(kBlock), 0, ev->st, n, vals, slotOf, ev->g->NM, ev->E,
+void accumulate(sdl_event* ev, Count cn, long long nMax, const T* vals, const int* slotOf, int which) {
+    if (nMax <= 0) return;
+    const unsigned grid = std::min<unsigned>(gridFor(nMax, kBlock), 1024u);
+    hipLaunchKernelGGL(k_accumulate<T>, dim3(grid), dim3(kBlock), 0, ev->st, cn, vals, slotOf, ev->g->NM, ev->E,
                        ev->g->dMods, which, ev->counters.p);
-    return 0;
 }
 
 // Hit ingestion: module lookup, derived quantities, counting sort by slot.
+// Hits on modules unknown to the geometry are counted (TOT_UNKNOWN) and left out.
 int runPrep(sdl_event* ev) {
     sdl_geom* G = ev->g;
     hipStream_t st = ev->st;
     const int NS = ev->E * G->NM;
     if (int r = ensureHits(ev)) return r;
+    HIP_OK(hipMemsetAsync(ev->tot.p, 0, sizeof(long long) * TOT_N, st));
     HIP_OK(hipMemsetAsync(ev->slotHitCount.p, 0, sizeof(int) * NS, st));
     HIP_OK(hipMemsetAsync(ev->slotFill.p, 0, sizeof(int) * NS, st));
-    HIP_OK(hipMemsetAsync(ev->unknown.p, 0, sizeof(int), st));
     HIP_OK(hipMemsetAsync(ev->counters.p, 0, sizeof(unsigned long long) * ev->E * kNCounters * kNBins, st));
     if (int r = ev->dEvOff.ensure(ev->E + 1)) return r;
     HIP_OK(hipMemcpyAsync(ev->dEvOff.p, ev->evOff.data(), sizeof(long long) * (ev->E + 1), hipMemcpyHostToDevice, st));
@@ -1386,19 +1504,17 @@ int runPrep(sdl_event* ev) {
     if (nh > 0)
         hipLaunchKernelGGL(k_hit_prep, dim3(gridFor(nh, kBlock)), dim3(kBlock), 0, st, nh, ev->px, ev->py, ev->pz,
                            ev->pdet, ev->dEvOff.p, ev->E, G->dDet, G->NM, G->dMods, ev->rt.p, ev->r3.p, ev->phi.p,
-                           ev->halpha.p, ev->hslot.p, ev->eHi.p, ev->eLo.p, ev->slotHitCount.p, ev->unknown.p);
+                           ev->halpha.p, ev->hslot.p, ev->eHi.p, ev->eLo.p, ev->slotHitCount.p, ev->tot.p);
     timerEnd(ev, T_PREP);
-    if (int r = scanExclusive<int>(ev->slotHitCount.p, ev->slotHitStart.p, NS, ev->scan, st)) return r;
+    if (int r = scanExclusive<int>(ev->slotHitCount.p, ev->slotHitStart.p, hostCount(ev, NS), NS, ev->scan, st,
+                                   ev->tot.p, -1))
+        return r;
     if (nh > 0) {
         hipLaunchKernelGGL(k_hit_scatter, dim3(gridFor(nh, kBlock)), dim3(kBlock), 0, st, nh, ev->hslot.p,
                            ev->slotHitStart.p, ev->slotFill.p, ev->sortedHit.p);
-        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, NS,
-                           ev->slotHitStart.p, ev->sortedHit.p);
+        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st,
+                           hostCount(ev, NS), ev->slotHitStart.p, ev->sortedHit.p);
     }
-    int unknown = 0;
-    HIP_OK(hipMemcpyAsync(&unknown, ev->unknown.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    if (unknown) return fail(SDL_ERR_UNKNOWN_DETID, std::to_string(unknown) + " hits have a detId unknown to the geometry");
     ev->hitsReady = true;
     return 0;
 }
@@ -1413,33 +1529,35 @@ int runMD(sdl_event* ev) {
     if (int r = ev->mdStart.ensure(NS + 1)) return r;
     hipLaunchKernelGGL(k_md_cap, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods,
                        ev->slotHitStart.p, ev->mdCap.p);
-    if (int r = scanExclusive<int>(ev->mdCap.p, ev->mdCapOff.p, NS, ev->scan, st)) return r;
-    const long long capTotal = readLL(ev->mdCapOff.p + NS, st);
-    if (capTotal < 0) return fail(SDL_ERR_HIP, "readback failed");
-    if (int r = ev->mdSlabHits.ensure(capTotal)) return r;
-    if (int r = ev->mdSlabF.ensure(capTotal)) return r;
+    if (int r = scanExclusive<int>(ev->mdCap.p, ev->mdCapOff.p, hostCount(ev, NS), NS, ev->scan, st, ev->tot.p,
+                                   TOT_MDCAP))
+        return r;
+    const long long slab = sizeOf(ev, TOT_MDCAP);
+    if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->mdSlabHits.ensure(slab)) return r;
+    if (int r = ev->mdSlabF.ensure(slab)) return r;
     HitsDev H = hitsView(ev);
     timerBegin(ev, T_MD);
     hipLaunchKernelGGL(k_md, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods, H,
                        ev->slotHitStart.p, ev->sortedHit.p, ev->mdCap.p, ev->mdCapOff.p, ev->mdSlabHits.p,
-                       ev->mdSlabF.p, ev->mdCount.p);
+                       ev->mdSlabF.p, ev->mdCount.p, guardOf(ev, TOT_MDCAP, slab, SDL_STAGE_MD));
     timerEnd(ev, T_MD);
-    if (int r = scanExclusive<int>(ev->mdCount.p, ev->mdStart.p, NS, ev->scan, st)) return r;
-    ev->nMD = readLL(ev->mdStart.p + NS, st);
-    if (ev->nMD < 0) return fail(SDL_ERR_HIP, "readback failed");
-    const long long n = std::max(ev->nMD, 1LL);
-    if (int r = ev->mdLo.ensure(n)) return r;
-    if (int r = ev->mdUp.ensure(n)) return r;
-    if (int r = ev->mdAnchor.ensure(n)) return r;
-    if (int r = ev->mdSlot.ensure(n)) return r;
-    if (int r = ev->mdA.ensure(n)) return r;
-    if (int r = ev->mdB.ensure(n)) return r;
+    if (int r = scanExclusive<int>(ev->mdCount.p, ev->mdStart.p, hostCount(ev, NS), NS, ev->scan, st, ev->tot.p, TOT_MD))
+        return r;
+    const long long nMD = sizeOf(ev, TOT_MD);
+    if (nMD < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nMD = nMD;
+    const long long n = std::max(nMD, 1LL);
+    for (DBuf<int>* b : {&ev->mdLo, &ev->mdUp, &ev->mdAnchor, &ev->mdSlot})
+        if (int r = b->ensure(n)) return r;
+    for (DBuf<float4>* b : {&ev->mdA, &ev->mdB, &ev->mdP})
+        if (int r = b->ensure(n)) return r;
     if (int r = ev->mdC.ensure(n)) return r;
-    if (int r = ev->mdP.ensure(n)) return r;
     hipLaunchKernelGGL(k_md_repack, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods, H,
-                       ev->mdCapOff.p, ev->mdCount.p, ev->mdStart.p, ev->mdSlabHits.p, ev->mdSlabF.p, mdView(ev));
-    accumulate<int>(ev, NS, ev->mdCap.p, nullptr, 0);
-    accumulate<int>(ev, NS, ev->mdCount.p, nullptr, 5);
+                       ev->mdCapOff.p, ev->mdCount.p, ev->mdStart.p, ev->mdSlabHits.p, ev->mdSlabF.p, mdView(ev),
+                       guardOf(ev, TOT_MD, nMD, SDL_STAGE_MD));
+    accumulate<int>(ev, hostCount(ev, NS), NS, ev->mdCap.p, nullptr, 0);
+    accumulate<int>(ev, hostCount(ev, NS), NS, ev->mdCount.p, nullptr, 5);
     return 0;
 }
 
@@ -1449,63 +1567,69 @@ int runSG(sdl_event* ev) {
     const int NS = ev->E * G->NM;
     const long long nMD = ev->nMD;
     const long long nm = std::max(nMD, 1LL);
-    if (int r = ev->sgCap.ensure(nm)) return r;
-    if (int r = ev->sgCount.ensure(nm)) return r;
-    if (int r = ev->sgCapOff.ensure(nm + 1)) return r;
-    if (int r = ev->sgStart.ensure(nm + 1)) return r;
+    for (DBuf<int>* b : {&ev->sgCap, &ev->sgCount, &ev->sgEndCount, &ev->sgEndFill})
+        if (int r = b->ensure(nm)) return r;
+    for (DBuf<long long>* b : {&ev->sgCapOff, &ev->sgStart, &ev->sgEndStart})
+        if (int r = b->ensure(nm + 1)) return r;
     if (int r = ev->slotSgStart.ensure(NS + 1)) return r;
+    const Count cMD = devCount(ev, TOT_MD);
     if (nMD > 0)
-        hipLaunchKernelGGL(k_sg_cap, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, int(nMD), G->NM, ev->mdSlot.p,
+        hipLaunchKernelGGL(k_sg_cap, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, cMD, G->NM, ev->mdSlot.p,
                            G->dConnStart, G->dConnList, ev->mdCount.p, ev->sgCap.p);
-    if (int r = scanExclusive<int>(ev->sgCap.p, ev->sgCapOff.p, nMD, ev->scan, st)) return r;
-    const long long capTotal = readLL(ev->sgCapOff.p + nMD, st);
-    if (capTotal < 0) return fail(SDL_ERR_HIP, "readback failed");
-    if (int r = ev->sgSlabOut.ensure(capTotal)) return r;
-    if (int r = ev->sgSlabF.ensure(capTotal)) return r;
+    if (int r = scanExclusive<int>(ev->sgCap.p, ev->sgCapOff.p, cMD, nMD, ev->scan, st, ev->tot.p, TOT_SGCAP)) return r;
+    const long long slab = sizeOf(ev, TOT_SGCAP);
+    if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->sgSlabOut.ensure(slab)) return r;
+    if (int r = ev->sgSlabF.ensure(slab)) return r;
     timerBegin(ev, T_SG);
     if (nMD > 0)
-        hipLaunchKernelGGL(k_sg, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, int(nMD), G->NM, G->dMods,
+        hipLaunchKernelGGL(k_sg, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, cMD, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), G->dConnStart, G->dConnList, ev->mdStart.p, ev->mdCount.p,
-                           ev->sgCap.p, ev->sgCapOff.p, ev->sgSlabOut.p, ev->sgSlabF.p, ev->sgCount.p);
+                           ev->sgCap.p, ev->sgCapOff.p, ev->sgSlabOut.p, ev->sgSlabF.p, ev->sgCount.p,
+                           guardOf(ev, TOT_SGCAP, slab, SDL_STAGE_SG));
     timerEnd(ev, T_SG);
-    if (int r = scanExclusive<int>(ev->sgCount.p, ev->sgStart.p, nMD, ev->scan, st)) return r;
-    ev->nSG = readLL(ev->sgStart.p + nMD, st);
-    if (ev->nSG < 0) return fail(SDL_ERR_HIP, "readback failed");
-    const long long ns = std::max(ev->nSG, 1LL);
-    if (int r = ev->sgIn.ensure(ns)) return r;
-    if (int r = ev->sgOut.ensure(ns)) return r;
+    if (int r = scanExclusive<int>(ev->sgCount.p, ev->sgStart.p, cMD, nMD, ev->scan, st, ev->tot.p, TOT_SG)) return r;
+    const long long nSG = sizeOf(ev, TOT_SG);
+    if (nSG < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nSG = nSG;
+    const long long ns = std::max(nSG, 1LL);
+    for (DBuf<int>* b : {&ev->sgIn, &ev->sgOut, &ev->sgFlags, &ev->sgEndList})
+        if (int r = b->ensure(ns)) return r;
     if (int r = ev->sgF.ensure(ns)) return r;
     if (int r = ev->sgX.ensure(ns)) return r;
     if (int r = ev->sgE.ensure(ns)) return r;
-    if (int r = ev->sgFlags.ensure(ns)) return r;
+    const Count cSG = devCount(ev, TOT_SG);
     if (nMD > 0)
-        hipLaunchKernelGGL(k_sg_repack, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, int(nMD),
-                           ev->sgCapOff.p, ev->sgCount.p, ev->sgStart.p, ev->sgSlabOut.p, ev->sgSlabF.p, sgView(ev));
-    if (ev->nSG > 0)
-        hipLaunchKernelGGL(k_sg_extra, dim3(gridFor(ev->nSG, kBlock)), dim3(kBlock), 0, st, int(ev->nSG), G->NM,
-                           G->dMods, hitsView(ev), mdView(ev), sgView(ev));
-    hipLaunchKernelGGL(k_slot_range, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, NS, ev->mdStart.p,
-                       ev->sgStart.p, ev->slotSgStart.p);
+        hipLaunchKernelGGL(k_sg_repack, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, cMD,
+                           ev->sgCapOff.p, ev->sgCount.p, ev->sgStart.p, ev->sgSlabOut.p, ev->sgSlabF.p, sgView(ev),
+                           guardOf(ev, TOT_SG, nSG, SDL_STAGE_SG));
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_sg_extra, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
+                           hitsView(ev), mdView(ev), sgView(ev));
+    hipLaunchKernelGGL(k_slot_range, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+                       ev->mdStart.p, ev->sgStart.p, ev->slotSgStart.p);
     // reverse index: SGs by outer MD
-    if (int r = ev->sgEndCount.ensure(nm)) return r;
-    if (int r = ev->sgEndFill.ensure(nm)) return r;
-    if (int r = ev->sgEndStart.ensure(nm + 1)) return r;
-    if (int r = ev->sgEndList.ensure(ns)) return r;
     HIP_OK(hipMemsetAsync(ev->sgEndCount.p, 0, sizeof(int) * nm, st));
     HIP_OK(hipMemsetAsync(ev->sgEndFill.p, 0, sizeof(int) * nm, st));
-    if (ev->nSG > 0)
-        hipLaunchKernelGGL(k_count_by, dim3(gridFor(ev->nSG, kBlock)), dim3(kBlock), 0, st, int(ev->nSG), ev->sgOut.p,
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_count_by, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
                            ev->sgEndCount.p);
-    if (int r = scanExclusive<int>(ev->sgEndCount.p, ev->sgEndStart.p, nMD, ev->scan, st)) return r;
-    if (ev->nSG > 0) {
-        hipLaunchKernelGGL(k_scatter_by, dim3(gridFor(ev->nSG, kBlock)), dim3(kBlock), 0, st, int(ev->nSG),
-                           ev->sgOut.p, ev->sgEndStart.p, ev->sgEndFill.p, ev->sgEndList.p);
-        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, int(nMD),
+    if (int r = scanExclusive<int>(ev->sgEndCount.p, ev->sgEndStart.p, cMD, nMD, ev->scan, st, ev->tot.p, -1)) return r;
+    if (nSG > 0) {
+        hipLaunchKernelGGL(k_scatter_by, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
+                           ev->sgEndStart.p, ev->sgEndFill.p, ev->sgEndList.p);
+        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, cMD,
                            ev->sgEndStart.p, ev->sgEndList.p);
     }
-    accumulate<int>(ev, nMD, ev->sgCap.p, ev->mdSlot.p, 1);
-    accumulate<int>(ev, nMD, ev->sgCount.p, ev->mdSlot.p, 6);
+    accumulate<int>(ev, cMD, nMD, ev->sgCap.p, ev->mdSlot.p, 1);
+    accumulate<int>(ev, cMD, nMD, ev->sgCount.p, ev->mdSlot.p, 6);
     return 0;
+}
+
+// The per-item scratch (candidate counts, slots) serves SG-, TP- and TL-indexed passes.
+int ensureItemScratch(sdl_event* ev, long long n) {
+    if (int r = ev->cand.ensure(std::max(n, 1LL))) return r;
+    return ev->itemSlot.ensure(std::max(n, 1LL));
 }
 
 int runTP(sdl_event* ev) {
@@ -1517,29 +1641,30 @@ int runTP(sdl_event* ev) {
     if (int r = ev->tpCount.ensure(ns)) return r;
     if (int r = ev->tpStart.ensure(ns + 1)) return r;
     if (int r = ev->slotTpStart.ensure(NS + 1)) return r;
+    const Count cSG = devCount(ev, TOT_SG);
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tp_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), ev->sgOut.p,
+        hipLaunchKernelGGL(k_tp_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
                            ev->sgStart.p, ev->tpCount.p);
-    if (int r = scanExclusive<int>(ev->tpCount.p, ev->tpStart.p, nSG, ev->scan, st)) return r;
-    ev->nTP = readLL(ev->tpStart.p + nSG, st);
-    if (ev->nTP < 0) return fail(SDL_ERR_HIP, "readback failed");
-    if (int r = ev->tpIn.ensure(std::max(ev->nTP, 1LL))) return r;
-    if (int r = ev->tpOut.ensure(std::max(ev->nTP, 1LL))) return r;
+    if (int r = scanExclusive<int>(ev->tpCount.p, ev->tpStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TP)) return r;
+    const long long nTP = sizeOf(ev, TOT_TP);
+    if (nTP < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nTP = nTP;
+    if (int r = ev->tpIn.ensure(std::max(nTP, 1LL))) return r;
+    if (int r = ev->tpOut.ensure(std::max(nTP, 1LL))) return r;
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tp_write, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), ev->sgOut.p,
-                           ev->sgStart.p, ev->tpStart.p, ev->tpIn.p, ev->tpOut.p);
-    hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, NS, ev->slotSgStart.p,
-                       ev->tpStart.p, ev->slotTpStart.p);
+        hipLaunchKernelGGL(k_tp_write, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
+                           ev->sgStart.p, ev->tpStart.p, ev->tpIn.p, ev->tpOut.p, guardOf(ev, TOT_TP, nTP, SDL_STAGE_TP));
+    hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+                       ev->slotSgStart.p, ev->tpStart.p, ev->slotTpStart.p);
     // counters: candidates per inner SG, objects per inner SG
-    if (int r = ev->cand.ensure(ns)) return r;
+    if (int r = ensureItemScratch(ev, nSG)) return r;
     if (nSG > 0) {
-        hipLaunchKernelGGL(k_tp_cand, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), G->NM, ev->sgIn.p,
+        hipLaunchKernelGGL(k_tp_cand, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, ev->sgIn.p,
                            ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->cand.p);
-        if (int r = ev->itemSlot.ensure(ns)) return r;
-        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, nSG, ev->sgIn.p,
+        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgIn.p,
                            (const int*)nullptr, ev->mdSlot.p, ev->itemSlot.p);
-        accumulate<long long>(ev, nSG, ev->cand.p, ev->itemSlot.p, 2);
-        accumulate<int>(ev, nSG, ev->tpCount.p, ev->itemSlot.p, 7);
+        accumulate<long long>(ev, cSG, nSG, ev->cand.p, ev->itemSlot.p, 2);
+        accumulate<int>(ev, cSG, nSG, ev->tpCount.p, ev->itemSlot.p, 7);
     }
     return 0;
 }
@@ -1555,35 +1680,40 @@ int runTL(sdl_event* ev) {
     if (int r = ev->tlCount.ensure(ns)) return r;
     if (int r = ev->tlStart.ensure(ns + 1)) return r;
     if (int r = ev->slotTlStart.ensure(NS + 1)) return r;
+    const Count cSG = devCount(ev, TOT_SG);
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), G->NM, ev->sgOut.p,
+        hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, ev->sgOut.p,
                            ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->tlCap.p);
-    if (int r = scanExclusive<long long>(ev->tlCap.p, ev->tlCapOff.p, nSG, ev->scan, st)) return r;
-    const long long capTotal = readLL(ev->tlCapOff.p + nSG, st);
-    if (capTotal < 0) return fail(SDL_ERR_HIP, "readback failed");
-    if (int r = ev->tlSlabOut.ensure(capTotal)) return r;
+    if (int r = scanExclusive<long long>(ev->tlCap.p, ev->tlCapOff.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TLCAP))
+        return r;
+    const long long slab = sizeOf(ev, TOT_TLCAP);
+    if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->tlSlabOut.ensure(slab)) return r;
     timerBegin(ev, T_TL);
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tl, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, int(nSG), G->NM,
-                           hitsView(ev), mdView(ev), sgView(ev), G->dConnStart, G->dConnList, ev->mdStart.p,
-                           ev->mdCount.p, ev->sgStart.p, ev->tlCap.p, ev->tlCapOff.p, ev->tlSlabOut.p, ev->tlCount.p);
+        hipLaunchKernelGGL(k_tl, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM, hitsView(ev),
+                           mdView(ev), sgView(ev), G->dConnStart, G->dConnList, ev->mdStart.p, ev->mdCount.p,
+                           ev->sgStart.p, ev->tlCap.p, ev->tlCapOff.p, ev->tlSlabOut.p, ev->tlCount.p,
+                           guardOf(ev, TOT_TLCAP, slab, SDL_STAGE_TL));
     timerEnd(ev, T_TL);
-    if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, nSG, ev->scan, st)) return r;
-    ev->nTL = readLL(ev->tlStart.p + nSG, st);
-    if (ev->nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
-    if (int r = ev->tlIn.ensure(std::max(ev->nTL, 1LL))) return r;
-    if (int r = ev->tlOut.ensure(std::max(ev->nTL, 1LL))) return r;
+    if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
+    const long long nTL = sizeOf(ev, TOT_TL);
+    if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nTL = nTL;
+    if (int r = ev->tlIn.ensure(std::max(nTL, 1LL))) return r;
+    if (int r = ev->tlOut.ensure(std::max(nTL, 1LL))) return r;
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_repack, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, int(nSG),
-                           ev->tlCapOff.p, ev->tlCount.p, ev->tlStart.p, ev->tlSlabOut.p, ev->tlIn.p, ev->tlOut.p);
-    hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, NS, ev->slotSgStart.p,
-                       ev->tlStart.p, ev->slotTlStart.p);
+        hipLaunchKernelGGL(k_tl_repack, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, ev->tlCapOff.p,
+                           ev->tlCount.p, ev->tlStart.p, ev->tlSlabOut.p, ev->tlIn.p, ev->tlOut.p,
+                           guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
+    hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+                       ev->slotSgStart.p, ev->tlStart.p, ev->slotTlStart.p);
+    if (int r = ensureItemScratch(ev, nSG)) return r;
     if (nSG > 0) {
-        if (int r = ev->itemSlot.ensure(ns)) return r;
-        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, nSG, ev->sgIn.p,
+        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgIn.p,
                            (const int*)nullptr, ev->mdSlot.p, ev->itemSlot.p);
-        accumulate<long long>(ev, nSG, ev->tlCap.p, ev->itemSlot.p, 3);
-        accumulate<int>(ev, nSG, ev->tlCount.p, ev->itemSlot.p, 8);
+        accumulate<long long>(ev, cSG, nSG, ev->tlCap.p, ev->itemSlot.p, 3);
+        accumulate<int>(ev, cSG, nSG, ev->tlCount.p, ev->itemSlot.p, 8);
     }
     return 0;
 }
@@ -1591,42 +1721,43 @@ int runTL(sdl_event* ev) {
 int runTCP(sdl_event* ev) {
     sdl_geom* G = ev->g;
     hipStream_t st = ev->st;
-    const long long nTL = ev->nTL, nSG = ev->nSG;
+    const long long nTL = ev->nTL, nSG = ev->nSG, nTP = ev->nTP;
     const long long nl = std::max(nTL, 1LL);
     if (int r = ev->tpeCount.ensure(std::max(nSG, 1LL))) return r;
     if (int r = ev->tpeOff.ensure(std::max(nSG, 1LL) + 1)) return r;
-    if (int r = ev->tpEndList.ensure(std::max(ev->nTP, 1LL))) return r;
+    if (int r = ev->tpEndList.ensure(std::max(nTP, 1LL))) return r;
     if (int r = ev->tcCount.ensure(nl)) return r;
     if (int r = ev->tcpInfo.ensure(nl)) return r;
     if (int r = ev->tcpStart.ensure(nl + 1)) return r;
+    const Count cSG = devCount(ev, TOT_SG), cTP = devCount(ev, TOT_TP), cTL = devCount(ev, TOT_TL);
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tpe_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, int(nSG), ev->sgEndList.p,
+        hipLaunchKernelGGL(k_tpe_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgEndList.p,
                            ev->sgEndStart.p, ev->sgIn.p, ev->tpeCount.p);
-    if (int r = scanExclusive<int>(ev->tpeCount.p, ev->tpeOff.p, nSG, ev->scan, st)) return r;
-    if (ev->nTP > 0)
-        hipLaunchKernelGGL(k_tpe_write, dim3(gridFor(ev->nTP, kBlock)), dim3(kBlock), 0, st, ev->nTP, int(nSG),
-                           ev->tpeOff.p, ev->sgEndList.p, ev->sgEndStart.p, ev->sgIn.p, ev->sgStart.p, ev->tpStart.p,
-                           ev->tpEndList.p);
+    if (int r = scanExclusive<int>(ev->tpeCount.p, ev->tpeOff.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TPE)) return r;
+    if (nTP > 0)
+        hipLaunchKernelGGL(k_tpe_write, dim3(gridFor(nTP, kBlock)), dim3(kBlock), 0, st, cTP, cSG, ev->tpeOff.p,
+                           ev->sgEndList.p, ev->sgEndStart.p, ev->sgIn.p, ev->sgStart.p, ev->tpStart.p, ev->tpEndList.p,
+                           guardOf(ev, TOT_TPE, nTP, SDL_STAGE_TCP));
     if (nTL > 0)
-        hipLaunchKernelGGL(k_tcp_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlIn.p,
+        hipLaunchKernelGGL(k_tcp_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlIn.p,
                            ev->tlOut.p, ev->sgIn.p, ev->sgOut.p, ev->sgStart.p, ev->sgEndStart.p, ev->tpeOff.p,
                            ev->tpStart.p, ev->tcpInfo.p, ev->tcCount.p);
-    if (int r = scanExclusive<int>(ev->tcCount.p, ev->tcpStart.p, nTL, ev->scan, st)) return r;
-    ev->nTCP = readLL(ev->tcpStart.p + nTL, st);
-    if (ev->nTCP < 0) return fail(SDL_ERR_HIP, "readback failed");
-    if (int r = ev->tcp.ensure(std::max(ev->nTCP, 1LL))) return r;
-    if (ev->nTCP > 0)
-        hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(ev->nTCP, kBlock), 4096u)), dim3(kBlock), 0, st, ev->nTCP,
-                           int(nTL), ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p, ev->sgIn.p,
-                           ev->mdSlot.p, G->NM, ev->E, G->dMods, ev->tcp.p, ev->counters.p);
-    const long long nTP = ev->nTP;
+    if (int r = scanExclusive<int>(ev->tcCount.p, ev->tcpStart.p, cTL, nTL, ev->scan, st, ev->tot.p, TOT_TCP)) return r;
+    const long long nTCP = sizeOf(ev, TOT_TCP);
+    if (nTCP < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nTCP = nTCP;
+    if (int r = ev->tcp.ensure(std::max(nTCP, 1LL))) return r;
+    if (nTCP > 0)
+        hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(nTCP, kBlock), 4096u)), dim3(kBlock), 0, st,
+                           devCount(ev, TOT_TCP), cTL, ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p,
+                           ev->sgIn.p, ev->mdSlot.p, G->NM, ev->E, G->dMods, ev->tcp.p, ev->counters.p,
+                           guardOf(ev, TOT_TCP, nTCP, SDL_STAGE_TCP));
+    if (int r = ensureItemScratch(ev, nTP)) return r;
     if (nTP > 0) {
-        if (int r = ev->cand.ensure(nTP)) return r;
-        if (int r = ev->itemSlot.ensure(nTP)) return r;
-        hipLaunchKernelGGL(k_tc_cand_tp, dim3(gridFor(nTP, kBlock)), dim3(kBlock), 0, st, nTP, G->NM, ev->tpIn.p,
+        hipLaunchKernelGGL(k_tc_cand_tp, dim3(gridFor(nTP, kBlock)), dim3(kBlock), 0, st, cTP, G->NM, ev->tpIn.p,
                            ev->tpOut.p, ev->sgIn.p, ev->sgOut.p, ev->mdSlot.p, G->dConnStart, G->dConnList,
                            ev->slotTpStart.p, ev->cand.p, ev->itemSlot.p);
-        accumulate<long long>(ev, nTP, ev->cand.p, ev->itemSlot.p, 4);
+        accumulate<long long>(ev, cTP, nTP, ev->cand.p, ev->itemSlot.p, 4);
     }
     return 0;
 }
@@ -1638,25 +1769,83 @@ int runTCL(sdl_event* ev) {
     const long long nl = std::max(nTL, 1LL);
     if (int r = ev->tcCount.ensure(nl)) return r;
     if (int r = ev->tclStart.ensure(nl + 1)) return r;
+    const Count cTL = devCount(ev, TOT_TL);
     if (nTL > 0)
-        hipLaunchKernelGGL(k_tcl_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlOut.p,
+        hipLaunchKernelGGL(k_tcl_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlOut.p,
                            ev->tlStart.p, ev->tcCount.p);
-    if (int r = scanExclusive<int>(ev->tcCount.p, ev->tclStart.p, nTL, ev->scan, st)) return r;
-    ev->nTCL = readLL(ev->tclStart.p + nTL, st);
-    if (ev->nTCL < 0) return fail(SDL_ERR_HIP, "readback failed");
-    if (int r = ev->tcl.ensure(std::max(ev->nTCL, 1LL))) return r;
+    if (int r = scanExclusive<int>(ev->tcCount.p, ev->tclStart.p, cTL, nTL, ev->scan, st, ev->tot.p, TOT_TCL)) return r;
+    const long long nTCL = sizeOf(ev, TOT_TCL);
+    if (nTCL < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nTCL = nTCL;
+    if (int r = ev->tcl.ensure(std::max(nTCL, 1LL))) return r;
+    if (int r = ensureItemScratch(ev, nTL)) return r;
     if (nTL > 0) {
-        hipLaunchKernelGGL(k_tcl_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, int(nTL), ev->tlOut.p,
-                           ev->tlStart.p, ev->tclStart.p, ev->tcl.p);
-        if (int r = ev->itemSlot.ensure(nl)) return r;
-        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, nTL, ev->tlIn.p,
+        hipLaunchKernelGGL(k_tcl_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlOut.p,
+                           ev->tlStart.p, ev->tclStart.p, ev->tcl.p, guardOf(ev, TOT_TCL, nTCL, SDL_STAGE_TCL));
+        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlIn.p,
                            ev->sgIn.p, ev->mdSlot.p, ev->itemSlot.p);
-        accumulate<int>(ev, nTL, ev->tcCount.p, ev->itemSlot.p, 9);
-        if (int r = ev->cand.ensure(nl)) return r;
-        hipLaunchKernelGGL(k_tc_cand_tl, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, nTL, G->NM, ev->tlIn.p,
+        accumulate<int>(ev, cTL, nTL, ev->tcCount.p, ev->itemSlot.p, 9);
+        hipLaunchKernelGGL(k_tc_cand_tl, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, G->NM, ev->tlIn.p,
                            ev->sgIn.p, ev->sgOut.p, ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotTlStart.p,
                            ev->cand.p, ev->itemSlot.p);
-        accumulate<long long>(ev, nTL, ev->cand.p, ev->itemSlot.p, 4);
+        accumulate<long long>(ev, cTL, nTL, ev->cand.p, ev->itemSlot.p, 4);
+    }
+    return 0;
+}
+
+int runStage(sdl_event* ev, int stage) {
+    switch (stage) {
+        case SDL_STAGE_MD: return runMD(ev);
+        case SDL_STAGE_SG: return runSG(ev);
+        case SDL_STAGE_TP: return runTP(ev);
+        case SDL_STAGE_TL: return runTL(ev);
+        case SDL_STAGE_TCP: return runTCP(ev);
+        case SDL_STAGE_TCL: return runTCL(ev);
+        default: return fail(SDL_ERR_ARG, "bad stage");
+    }
+}
+
+// Runs stages s0..s1 of the batch with one host round trip at the end when the
+// last call's totals predict the buffer sizes; otherwise (first call, or a
+// prediction that fell short) each stage reads back the totals it sizes from.
+int runStages(sdl_event* ev, int s0, int s1) {
+    hipStream_t st = ev->st;
+    if (int r = ev->tot.ensure(TOT_N)) return r;
+    ev->spec = ev->haveHistory;
+    if (!ev->hitsReady) {
+        if (int r = runPrep(ev)) return r;
+    } else {
+        HIP_OK(hipMemsetAsync(ev->tot.p + TOT_ABORT, 0, sizeof(long long), st));
+    }
+    for (int s = s0; s <= s1; ++s)
+        if (int r = runStage(ev, s)) return r;
+    long long t[TOT_N];
+    HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (t[TOT_UNKNOWN] > 0)
+        return fail(SDL_ERR_UNKNOWN_DETID, std::to_string(t[TOT_UNKNOWN]) + " hits have a detId unknown to the geometry");
+    if (t[TOT_ABORT] != 0) {
+        if (!ev->spec) return fail(SDL_ERR_CAPACITY, "device capacity check failed on an exactly sized run");
+        // rerun from the stage that overflowed, sized exactly
+        const int from = t[TOT_ABORT] > SDL_STAGE_TCL + 1 ? s0 : int(t[TOT_ABORT] - 1);
+        ev->spec = false;
+        HIP_OK(hipMemsetAsync(ev->tot.p + TOT_ABORT, 0, sizeof(long long), st));
+        for (int s = std::max(from, s0); s <= s1; ++s)
+            if (int r = runStage(ev, s)) return r;
+        HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (t[TOT_ABORT] != 0) return fail(SDL_ERR_CAPACITY, "device capacity check failed on an exactly sized run");
+    }
+    ev->nMD = t[TOT_MD]; ev->nSG = t[TOT_SG]; ev->nTP = t[TOT_TP];
+    ev->nTL = t[TOT_TL]; ev->nTCP = t[TOT_TCP]; ev->nTCL = t[TOT_TCL];
+    // remember the sizes of the stages that ran (the prediction for the next call)
+    const int ran[SDL_STAGE_TCL + 1][2] = {{TOT_MDCAP, TOT_MD}, {TOT_SGCAP, TOT_SG}, {TOT_TP, TOT_TPE},
+                                           {TOT_TLCAP, TOT_TL}, {TOT_TCP, TOT_TCP}, {TOT_TCL, TOT_TCL}};
+    for (int s = s0; s <= s1; ++s)
+        for (int k : ran[s]) ev->lastTot[k] = t[k];
+    if (s1 == SDL_STAGE_TCL) {
+        ev->lastNH = ev->NH;
+        ev->haveHistory = true;
     }
     return 0;
 }
@@ -1822,25 +2011,15 @@ int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offse
 int sdl_event_run(sdl_event* ev, int stage, int algo) {
     if (!ev) return fail(SDL_ERR_ARG, "null event");
     if (algo != 0) return fail(SDL_ERR_ARG, "only the default algorithm has device kernels");
+    if (stage < SDL_STAGE_MD || stage > SDL_STAGE_TCL) return fail(SDL_ERR_ARG, "bad stage");
     if (stage != ev->stage + 1) return fail(SDL_ERR_STATE, "stages must run in order");
     if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
-    int r = 0;
     if (!ev->hitsReady) {
         if (ev->hostStaged && ev->hoff.size() == 1) ev->hoff.push_back(0);
-        if ((r = beginBatch(ev))) return r;
+        if (int r = beginBatch(ev)) return r;
         if (ev->E <= 0) return fail(SDL_ERR_STATE, "no event");
-        if ((r = runPrep(ev))) return r;
     }
-    switch (stage) {
-        case SDL_STAGE_MD: r = runMD(ev); break;
-        case SDL_STAGE_SG: r = runSG(ev); break;
-        case SDL_STAGE_TP: r = runTP(ev); break;
-        case SDL_STAGE_TL: r = runTL(ev); break;
-        case SDL_STAGE_TCP: r = runTCP(ev); break;
-        case SDL_STAGE_TCL: r = runTCL(ev); break;
-        default: return fail(SDL_ERR_ARG, "bad stage");
-    }
-    if (r) return r;
+    if (int r = runStages(ev, stage, stage)) return r;
     if (hipGetLastError() != hipSuccess) return fail(SDL_ERR_HIP, "kernel launch failed");
     ev->stage = stage;
     return 0;
@@ -1848,10 +2027,15 @@ int sdl_event_run(sdl_event* ev, int stage, int algo) {
 
 int sdl_event_run_chain(sdl_event* ev) {
     if (!ev) return fail(SDL_ERR_ARG, "null event");
+    if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
     ev->stage = -1;
     ev->hitsReady = false;
-    for (int s = 0; s <= SDL_STAGE_TCL; ++s)
-        if (int r = sdl_event_run(ev, s, 0)) return r;
+    if (ev->hostStaged && ev->hoff.size() == 1) ev->hoff.push_back(0);
+    if (int r = beginBatch(ev)) return r;
+    if (ev->E <= 0) return fail(SDL_ERR_STATE, "no event");
+    if (int r = runStages(ev, SDL_STAGE_MD, SDL_STAGE_TCL)) return r;
+    if (hipGetLastError() != hipSuccess) return fail(SDL_ERR_HIP, "kernel launch failed");
+    ev->stage = SDL_STAGE_TCL;
     return 0;
 }
 
@@ -1929,5 +2113,12 @@ int sdl_event_kernel_ms(sdl_event* ev, double* out, int n) {
 }
 
 void* sdl_event_stream(sdl_event* ev) { return ev ? (void*)ev->st : nullptr; }
+
+int sdl_event_set_headroom(sdl_event* ev, int fromStage, double factor) {
+    if (!ev || factor <= 0.0) return fail(SDL_ERR_ARG, "bad arguments");
+    ev->headroomFrom = fromStage;
+    ev->headroomFactor = factor;
+    return 0;
+}
 
 }  // extern "C"

@@ -22,6 +22,7 @@ EXPORTS = (
     "sdl_event_destroy", "sdl_event_reset", "sdl_event_add_hits", "sdl_event_next", "sdl_event_set_hits_device",
     "sdl_event_run", "sdl_event_run_chain", "sdl_event_sync", "sdl_event_counts", "sdl_event_layer_counts",
     "sdl_event_export", "sdl_event_num_events", "sdl_event_stream", "sdl_event_set_timing", "sdl_event_kernel_ms",
+    "sdl_event_set_headroom",
 )
 TIMERS = ("prep", "md", "sg", "tl")
 
@@ -60,6 +61,7 @@ def lib():
         L.sdl_event_stream.restype = vp
         L.sdl_event_set_timing.argtypes = [vp, i]
         L.sdl_event_kernel_ms.argtypes = [vp, vp, i]
+        L.sdl_event_set_headroom.argtypes = [vp, i, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -159,6 +161,10 @@ class Event:
 
     def set_timing(self, on=True):
         _check(lib().sdl_event_set_timing(self.h, 1 if on else 0))
+
+    def set_headroom(self, from_stage, factor):
+        """Headroom factor of the predicted buffer sizes for stages >= from_stage."""
+        _check(lib().sdl_event_set_headroom(self.h, from_stage, factor))
 
     def kernel_ms(self):
         out = np.zeros(len(TIMERS), np.float64)

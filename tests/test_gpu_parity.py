@@ -145,3 +145,41 @@ def test_pu200_event_full_parity():
     ref, cnt = oracle.run(gen.geometry(2)[1], e, oracle.CR)
     _assert_same(got, ref, "pu200")
     assert ev.counts(0) == cnt
+
+
+@pytest.mark.parametrize("stage", range(6))
+def test_resume_after_short_prediction(stage):
+    """Buffers predicted too small from the stage on: the kernels detect it, the
+    chain resumes from that stage sized exactly, and the results are unchanged."""
+    print(f"case resume from stage {stage}", flush=True)
+    wl, e = 12, 1
+    ev_in = gen.event(wl, e)
+    ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], ev_in, oracle.CR)
+    g = _geom(gen.geom_kind(wl))
+    ev = runtime.Event(g)
+    for k in range(3):
+        if k == 1:
+            ev.set_headroom(stage, 0.5)
+        ev.reset()
+        ev.add_hits(ev_in.x, ev_in.y, ev_in.z, ev_in.idx, ev_in.detId)
+        ev.run_chain()
+        ev.sync()
+        _assert_same(ev.export_all(0), ref, f"resume{stage} pass{k}")
+        assert ev.counts(0) == cnt
+
+
+def test_speculative_sizing_across_events():
+    """One handle runs a small event, a PU200-size batch and a small event again:
+    every call matches the oracle whichever sizing path it took."""
+    print("case speculative sizing across events", flush=True)
+    g = _geom(2)
+    ev = runtime.Event(g)
+    for wl, e in ((12, 0), (15, 0), (12, 1), (15, 1), (12, 0)):
+        ev_in = gen.event(wl, e)
+        ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], ev_in, oracle.CR)
+        ev.reset()
+        ev.add_hits(ev_in.x, ev_in.y, ev_in.z, ev_in.idx, ev_in.detId)
+        ev.run_chain()
+        ev.sync()
+        _assert_same(ev.export_all(0), ref, f"w{wl}e{e}")
+        assert ev.counts(0) == cnt
