@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 REF_DRIVER = os.path.join(_HERE, "_ref", "sdl_ref_driver")
 
 GLIBC, CR = 0, 1
+NUDGED = 2  # test only: CR made 0.4 % too large (a planted libm error)
 STAGES = ("MD", "SG", "TP", "TL", "TCP", "TCL")
 WIDTH = {"MD": 2, "SG": 4, "TP": 8, "TL": 8, "TCP": 16, "TCL": 16}
 
@@ -35,7 +36,7 @@ def lib():
         L.oracle_result_counters.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_result_free.argtypes = [ctypes.c_void_p]
         L.oracle_explain.restype = ctypes.c_int
-        L.oracle_explain.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 3
+        L.oracle_explain.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 2 + [ctypes.c_int, ctypes.c_void_p]
         L.oracle_time.restype = ctypes.c_double
         L.oracle_time.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_int, ctypes.c_int]
         _lib = L
@@ -81,15 +82,17 @@ def run(paths, ev, libm=GLIBC, stages=6):
         L.oracle_result_free(r)
 
 
-def explain(paths, ev, kind, a, b=None):
+def explain(paths, ev, kind, a, b=None, policy=None):
     """Re-evaluates one MD (kind 0, a = (lo, up)), SG (kind 1, a = 4 idx) or TL
-    (kind 2, a, b = 8 idx) candidate under both libm policies."""
+    (kind 2, a, b = 8 idx) candidate under the GLIBC policy and `policy` (CR by
+    default)."""
     L = lib()
     g = geometry(paths)
     aa = np.asarray(a, np.int32)
     bb = np.asarray(b if b is not None else [0] * 4, np.int32)
     out = np.zeros(9, np.float64)
-    L.oracle_explain(g, ev.nhits, *_ptrs(ev), kind, aa.ctypes.data, bb.ctypes.data, out.ctypes.data)
+    L.oracle_explain(g, ev.nhits, *_ptrs(ev), kind, aa.ctypes.data, bb.ctypes.data, CR if policy is None else policy,
+                     out.ctypes.data)
     keys = ("cut_glibc", "cut_cr", "var", "lo", "hi", "var_cr", "lo_cr", "hi_cr", "ulps")
     return dict(zip(keys, out.tolist()))
 

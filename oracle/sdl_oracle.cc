@@ -33,12 +33,18 @@ constexpr float kStrip2SZpitch = 5.0f;
 
 // ------------------------------------------------------------------ libm policy
 struct Libm {
-    bool cr;
-    float asinF(float v) const { return cr ? float(std::asin(double(v))) : std::asin(v); }
-    float atanF(float v) const { return cr ? float(std::atan(double(v))) : std::atan(v); }
-    float sinF(float v) const { return cr ? float(std::sin(double(v))) : std::sin(v); }
-    float cosF(float v) const { return cr ? float(std::cos(double(v))) : std::cos(v); }
-    float tanF(float v) const { return cr ? float(std::tan(double(v))) : std::tan(v); }
+    int mode;  // ORACLE_LIBM_*
+    float pick(float glibc, double exact) const {
+        if (mode == ORACLE_LIBM_GLIBC) return glibc;
+        const float r = float(exact);
+        if (mode == ORACLE_LIBM_CR) return r;
+        return r * (1.f + 1.f / 256.f);  // planted error (tests): 0.4 % too large
+    }
+    float asinF(float v) const { return mode == ORACLE_LIBM_GLIBC ? std::asin(v) : pick(0.f, std::asin(double(v))); }
+    float atanF(float v) const { return mode == ORACLE_LIBM_GLIBC ? std::atan(v) : pick(0.f, std::atan(double(v))); }
+    float sinF(float v) const { return mode == ORACLE_LIBM_GLIBC ? std::sin(v) : pick(0.f, std::sin(double(v))); }
+    float cosF(float v) const { return mode == ORACLE_LIBM_GLIBC ? std::cos(v) : pick(0.f, std::cos(double(v))); }
+    float tanF(float v) const { return mode == ORACLE_LIBM_GLIBC ? std::tan(v) : pick(0.f, std::tan(double(v))); }
 };
 
 // std::min / std::max semantics (NaN handling follows the argument order).
@@ -1086,7 +1092,7 @@ struct Result {
 void fillEvent(Event& ev, const Geometry* g, int libm, int n, const float* x, const float* y, const float* z,
                const unsigned* detId, const int* idx) {
     ev.g = g;
-    ev.lm.cr = libm == ORACLE_LIBM_CR;
+    ev.lm.mode = libm;
     ev.hits.reserve(n);
     for (int i = 0; i < n; ++i) ev.addHit(x[i], y[i], z[i], idx[i], detId[i]);
 }
@@ -1139,13 +1145,13 @@ void oracle_result_counters(void* res, unsigned long long* out10) {
 void oracle_result_free(void* res) { delete static_cast<Result*>(res); }
 
 int oracle_explain(void* geom, int nhits, const float* x, const float* y, const float* z, const unsigned* detId,
-                   const int* idx, int kind, const int* a, const int* b, double* out) {
+                   const int* idx, int kind, const int* a, const int* b, int libmB, double* out) {
     Geometry* g = static_cast<Geometry*>(geom);
     Trace trs[2];
     int decided[2] = {-1, -1};
     for (int mode = 0; mode < 2; ++mode) {
         Event ev;
-        fillEvent(ev, g, mode, nhits, x, y, z, detId, idx);
+        fillEvent(ev, g, mode == 0 ? ORACLE_LIBM_GLIBC : libmB, nhits, x, y, z, detId, idx);
         std::unordered_map<int, int> pos;  // user idx -> hit position
         for (int i = 0; i < nhits; ++i) pos[idx[i]] = i;
         auto makeMD = [&](int lo, int up) -> int {

@@ -25,7 +25,10 @@
 
 extern "C" {
 
-enum { ORACLE_LIBM_GLIBC = 0, ORACLE_LIBM_CR = 1 };
+enum { ORACLE_LIBM_GLIBC = 0, ORACLE_LIBM_CR = 1,
+       // test only: the CR value made 0.4 % too large -- a planted libm
+       // error that the tests of the difference listing must detect
+       ORACLE_LIBM_NUDGED = 2 };
 
 // Stage indices used by the result accessors.
 enum { ORACLE_MD = 0, ORACLE_SG = 1, ORACLE_TP = 2, ORACLE_TL = 3, ORACLE_TCP = 4, ORACLE_TCL = 5 };
@@ -45,13 +48,13 @@ void oracle_result_copy(void* res, int stage, int* out);
 void oracle_result_counters(void* res, unsigned long long* out10);
 void oracle_result_free(void* res);
 
-// Re-evaluates one candidate under both libm policies and reports the cut where
-// the decisions differ.  kind: 0 MD (a,b = lower/upper hit positions in the
+// Re-evaluates one candidate under the GLIBC policy and under policy libmB and
+// reports the cut where the decisions differ.  kind: 0 MD (a,b = lower/upper hit positions in the
 // event arrays), 1 SG (a,b = MD tuples (4 ints) looked up in the GLIBC result),
-// 2 TL (a,b = SG tuples (8 ints)).  out: [cutGlibc, cutCR, var, lo, hi,
-// var_cr, lo_cr, hi_cr, ulpDistance].  Returns 0 on success.
+// 2 TL (a,b = SG tuples (8 ints)).  out: [cutGlibc, cutB, var, lo, hi,
+// var_b, lo_b, hi_b, ulpDistance].  Returns 0 on success.
 int oracle_explain(void* geom, int nhits, const float* x, const float* y, const float* z, const unsigned* detId,
-                   const int* idx, int kind, const int* a, const int* b, double* out);
+                   const int* idx, int kind, const int* a, const int* b, int libmB, double* out);
 
 // Times `nevents` generated-elsewhere events run by `threads` threads (one event
 // per thread at a time); events are passed as concatenated hit arrays with

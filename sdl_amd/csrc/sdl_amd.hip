@@ -87,6 +87,7 @@ struct DBuf {
 // with buffers sized from the totals.
 enum {
     TOT_MDCAP = 0, TOT_MD, TOT_SGCAP, TOT_SG, TOT_TP, TOT_TLCAP, TOT_TL, TOT_TPE, TOT_TCP, TOT_TCL,
+    TOT_TLC0, TOT_TLC1, TOT_TLC2,  // second-stage tracklet candidates per variant
     TOT_ABORT, TOT_UNKNOWN, TOT_N
 };
 
@@ -427,7 +428,7 @@ struct HitsDev {
     const float4 *eHi, *eLo;
 };
 
-__global__ void __launch_bounds__(kBlock) k_md(int NS, int NM, const ModDev* mods, HitsDev H, const long long* slotHitStart,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_md(int NS, int NM, const ModDev* mods, HitsDev H, const long long* slotHitStart,
                                                const int* sortedHit, const int* mdCap, const long long* mdCapOff,
                                                int2* slabHits, float4* slabF, int* mdCount, Guard g) {
     if (g.over()) return;
@@ -565,7 +566,7 @@ __device__ __forceinline__ int findEntry(int k, int pre, int nEntries, int& inne
     return e;
 }
 
-__global__ void __launch_bounds__(kBlock) k_sg(Count cMD, int NM, const ModDev* mods, HitsDev H, MDDev D,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_sg(Count cMD, int NM, const ModDev* mods, HitsDev H, MDDev D,
                                                const int* connStart, const int* connList, const long long* mdStart,
                                                const int* mdCount, const int* sgCap, const long long* sgCapOff,
                                                int* slabOut, SGF* slabF, int* sgCount, Guard g) {
@@ -748,22 +749,17 @@ __global__ void k_tl_cap(Count cSG, int NM, const int* sgOut, const int* mdSlot,
     tlCap[s] = c;
 }
 
-// One wave per inner segment.  Phase A walks the MDs of every module connected to
-// the inner segment's outer-MD module (the outer segments' possible inner MDs)
-// and applies the first-stage cuts once per (inner segment, MD) pair; the
-// surviving MDs go to a per-wave LDS queue.  Phase B expands the queue into the
-// outer segments starting at those MDs and applies the second stage with all
-// lanes busy.  Survivors are written in the reference's candidate order.
-struct TLQueue {
-    int mLo[kWave], sgA[kWave], nS[kWave], variant[kWave];
-    float sdlCut[kWave], thetaMuls[kWave], dPhiMid[kWave];
-};
-
-__device__ __forceinline__ void waveSync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
+// The tracklet algorithm runs as a cascade of passes, so that the expensive
+// second stage sees dense work:
+//  1. k_tl_pairs, one wave per inner SG: the first-stage cuts, which depend only
+//     on the inner SG and on the outer SG's inner MD, once per (inner SG, MD of
+//     a connected module) pair; surviving pairs, with the values the second
+//     stage reads, go to a slab at the inner SG's candidate offset;
+//  2. k_tl_expand, one wave per inner SG: the outer SGs starting at the surviving
+//     pairs' MDs, listed densely per algorithm variant (BBBB, BBEE, EEEE);
+//  3. k_tl_stage2<variant>, one thread per listed candidate: the second stage;
+//  4. k_tl_count and k_tl_scatter, one wave per inner SG: the survivors, in
+//     (variant, reference candidate) order within each inner SG.
 __device__ __forceinline__ TLOutLo loadOutLo(int mLo, int flags, const MDDev& D) {
     TLOutLo L;
     L.a = loadAnchor(mLo, D);
@@ -776,143 +772,224 @@ __device__ __forceinline__ TLOutLo loadOutLo(int mLo, int flags, const MDDev& D)
     return L;
 }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_tl(Count cSG, int NM, HitsDev H, MDDev D, SGDev S, const int* connStart,
-                                               const int* connList, const long long* mdStart, const int* mdCount,
-                                               const long long* sgStart, const long long* tlCap,
-                                               const long long* tlCapOff, int* slabOut, int* tlCount, Guard g) {
-    if (g.over()) return;
-    const int nSG = int(cSG.get());
-    __shared__ TLQueue queues[kWavesPerBlock];
-    TLQueue& Q = queues[threadIdx.x >> 6];
-    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
-    const int lane = threadIdx.x & 63;
-    if (item >= nSG) return;
-    const long long cap = tlCap[item];
-    if (cap == 0) {
-        if (lane == 0) tlCount[item] = 0;
-        return;
-    }
+__device__ __forceinline__ TLInner loadInner(int s, const MDDev& D, const SGDev& S) {
     TLInner I;
-    const int mi = S.in[item], mo = S.out[item];
+    const int mi = S.in[s], mo = S.out[s];
     I.lo = loadAnchor(mi, D);
     I.up = loadAnchor(mo, D);
-    const SGF fi = S.f[item];
+    const SGF fi = S.f[s];
     I.dphichange = fi.v[3];
     I.dphichangemin = fi.v[4];
     I.dphichangemax = fi.v[5];
-    const float4 xi = S.x[item];
+    const float4 xi = S.x[s];
     I.dr = xi.x;
     I.d = xi.y;
     I.alphaReg3Lo = D.p[mi].w;
-    const int fl = S.flags[item];
+    const int fl = S.flags[s];
     I.bLo = fl & 1; I.bUp = fl & 2; I.psLo = fl & 4; I.psUp = fl & 8;
-    const int b = D.slot[mo];
-    const int m = b % NM, eb = b - m;
-    const int c0 = connStart[m], nc = connStart[m + 1] - c0;
-    const long long off = tlCapOff[item];
-    int base = 0;
-    for (int cb = 0; cb < nc; cb += kWave) {
-        const int ne = min(kWave, nc - cb);
-        int cnt = 0;
-        if (lane < ne) cnt = mdCount[eb + connList[c0 + cb + lane]];
-        int pre = cnt;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int t = __shfl_up(pre, o);
-            if (lane >= o) pre += t;
-        }
-        const int totalA = __shfl(pre, kWave - 1);
-        for (int kA0 = 0; kA0 < totalA; kA0 += kWave) {
-            // ---- phase A: (inner segment, outer-inner MD) pairs
-            const int k = kA0 + lane;
-            int inner;
-            const int e = findEntry(k, pre, ne, inner);
-            bool ok = false;
-            int mLo = 0, sgA = 0, nS = 0, variant = TL_NONE;
-            TLCarry c{0.f, 0.f, 0.f};
-            if (k < totalA) {
-                const int t = eb + connList[c0 + cb + e];
-                mLo = int(mdStart[t]) + inner;
-                sgA = int(sgStart[mLo]);
-                nS = int(sgStart[mLo + 1]) - sgA;
-                if (nS > 0) {
-                    const TLOutLo L = loadOutLo(mLo, S.flags[sgA], D);
-                    variant = tlVariant(I.bLo, I.bUp, L.barrel);
-                    ok = tlStage1(variant, I, L, c);
-                }
-            }
-            const unsigned long long maskA = __ballot(ok);
-            const int nq = __popcll(maskA);
-            if (nq == 0) continue;
-            if (ok) {
-                const int q = __popcll(maskA & ((1ull << lane) - 1ull));
-                Q.mLo[q] = mLo; Q.sgA[q] = sgA; Q.nS[q] = nS; Q.variant[q] = variant;
-                Q.sdlCut[q] = c.sdlCut; Q.thetaMuls[q] = c.thetaMuls; Q.dPhiMid[q] = c.dPhiMid;
-            }
-            waveSync();
-            // ---- phase B: the outer segments of the queued MDs
-            const int cntB = lane < nq ? Q.nS[lane] : 0;
-            int preB = cntB;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int t = __shfl_up(preB, o);
-                if (lane >= o) preB += t;
-            }
-            const int totalB = __shfl(preB, kWave - 1);
-            for (int kB0 = 0; kB0 < totalB; kB0 += kWave) {
-                const int kb = kB0 + lane;
-                int innerB;
-                const int eB = findEntry(kb, preB, nq, innerB);
-                bool pass = false;
-                int so = 0;
-                if (kb < totalB) {
-                    so = Q.sgA[eB] + innerB;
-                    const int flo = S.flags[so];
-                    const TLCarry cc{Q.sdlCut[eB], Q.thetaMuls[eB], Q.dPhiMid[eB]};
-                    const TLOutLo L = loadOutLo(Q.mLo[eB], flo, D);
-                    TLOutUp U;
-                    const int mUp = S.out[so];
-                    U.a = loadAnchor(mUp, D);
-                    const SGF sf = S.f[so];
-                    U.dphimin = sf.v[1]; U.dphimax = sf.v[2];
-                    U.dphichange = sf.v[3]; U.dphichangemin = sf.v[4]; U.dphichangemax = sf.v[5];
-                    const float4 sx = S.x[so];
-                    U.dr = sx.x; U.d = sx.y; U.aOU = sx.z; U.dPhiPos = sx.w;
-                    U.ec2s = flo & 16;
-                    U.hiX = U.hiY = U.hiRt = U.hiPhi = U.loX = U.loY = U.loRt = U.loPhi = U.aOUhi = U.aOUlo = 0.f;
-                    if (U.ec2s) {
-                        const int an = D.anchor[mUp];
-                        const float4 hi = H.eHi[an], lo = H.eLo[an];
-                        const float2 ee = S.e[so];
-                        U.hiX = hi.x; U.hiY = hi.y; U.hiRt = hi.z; U.hiPhi = hi.w;
-                        U.loX = lo.x; U.loY = lo.y; U.loRt = lo.z; U.loPhi = lo.w;
-                        U.aOUhi = ee.x; U.aOUlo = ee.y;
-                    }
-                    pass = tlStage2(Q.variant[eB], (flo & 2) != 0, I, L, U, cc);
-                }
-                const unsigned long long mask = __ballot(pass);
-                if (pass) {
-                    const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
-                    slabOut[off + pos] = so;
-                }
-                base += __popcll(mask);
-            }
-            waveSync();  // the queue is rewritten by the next phase A
-        }
-    }
-    if (lane == 0) tlCount[item] = base;
+    return I;
 }
 
-__global__ void k_tl_repack(Count cSG, const long long* tlCapOff, const int* tlCount, const long long* tlStart,
-                            const int* slabOut, int* tlIn, int* tlOutArr, Guard g) {
+__device__ __forceinline__ TLOutUp loadOutUp(int so, int flo, HitsDev H, const MDDev& D, const SGDev& S) {
+    TLOutUp U;
+    const int mUp = S.out[so];
+    U.a = loadAnchor(mUp, D);
+    const SGF sf = S.f[so];
+    U.dphimin = sf.v[1]; U.dphimax = sf.v[2];
+    U.dphichange = sf.v[3]; U.dphichangemin = sf.v[4]; U.dphichangemax = sf.v[5];
+    const float4 sx = S.x[so];
+    U.dr = sx.x; U.d = sx.y; U.aOU = sx.z; U.dPhiPos = sx.w;
+    U.ec2s = flo & 16;
+    U.hiX = U.hiY = U.hiRt = U.hiPhi = U.loX = U.loY = U.loRt = U.loPhi = U.aOUhi = U.aOUlo = 0.f;
+    if (U.ec2s) {
+        const int an = D.anchor[mUp];
+        const float4 hi = H.eHi[an], lo = H.eLo[an];
+        const float2 ee = S.e[so];
+        U.hiX = hi.x; U.hiY = hi.y; U.hiRt = hi.z; U.hiPhi = hi.w;
+        U.loX = lo.x; U.loY = lo.y; U.loRt = lo.z; U.loPhi = lo.w;
+        U.aOUhi = ee.x; U.aOUlo = ee.y;
+    }
+    return U;
+}
+
+__device__ __forceinline__ int waveSum(int v) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Pair record: (outer-inner MD, inner SG | variant << 30) and the carried values.
+constexpr int kVariantShift = 30;
+constexpr int kSgMask = (1 << kVariantShift) - 1;
+constexpr int kNVariants = 3;
+
+__global__ void __launch_bounds__(kBlock) k_tl_pairs(Count cSG, int NM, MDDev D, SGDev S, const int* connStart,
+                                                     const int* connList, const long long* mdStart, const int* mdCount,
+                                                     const long long* sgStart, const long long* tlCap,
+                                                     const long long* tlCapOff, int2* prRec, float4* prCarry,
+                                                     int* prCount, int* nCand, long long nCandStride, Guard g) {
+    if (g.over()) return;
+    const int nSG = int(cSG.get());
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    if (item >= nSG) return;
+    int cnt[kNVariants] = {0, 0, 0};
+    int base = 0;
+    if (tlCap[item] > 0) {
+        const TLInner I = loadInner(item, D, S);
+        const int b = D.slot[S.out[item]];
+        const int m = b % NM, eb = b - m;
+        const int c0 = connStart[m], nc = connStart[m + 1] - c0;
+        const long long off = tlCapOff[item];
+        for (int cb = 0; cb < nc; cb += kWave) {
+            const int ne = min(kWave, nc - cb);
+            int mc = 0;
+            if (lane < ne) mc = mdCount[eb + connList[c0 + cb + lane]];
+            int pre = mc;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int t = __shfl_up(pre, o);
+                if (lane >= o) pre += t;
+            }
+            const int totalA = __shfl(pre, kWave - 1);
+            for (int kA0 = 0; kA0 < totalA; kA0 += kWave) {
+                const int k = kA0 + lane;
+                int inner;
+                const int e = findEntry(k, pre, ne, inner);
+                bool ok = false;
+                int mLo = 0, nS = 0, variant = TL_NONE;
+                TLCarry c{0.f, 0.f, 0.f};
+                if (k < totalA) {
+                    const int t = eb + connList[c0 + cb + e];
+                    mLo = int(mdStart[t]) + inner;
+                    const int sgA = int(sgStart[mLo]);
+                    nS = int(sgStart[mLo + 1]) - sgA;
+                    if (nS > 0) {
+                        const TLOutLo L = loadOutLo(mLo, S.flags[sgA], D);
+                        variant = tlVariant(I.bLo, I.bUp, L.barrel);
+                        ok = tlStage1(variant, I, L, c);
+                    }
+                }
+                const unsigned long long mask = __ballot(ok);
+                if (ok) {
+                    const long long q = off + base + __popcll(mask & ((1ull << lane) - 1ull));
+                    prRec[q] = make_int2(mLo, item | (variant << kVariantShift));
+                    prCarry[q] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);
+                }
+                base += __popcll(mask);
+#pragma unroll
+                for (int v = 0; v < kNVariants; ++v) cnt[v] += waveSum(ok && variant == v ? nS : 0);
+            }
+        }
+    }
+    if (lane == 0) {
+        prCount[item] = base;
+        for (int v = 0; v < kNVariants; ++v) nCand[v * nCandStride + item] = cnt[v];
+    }
+}
+
+// The outer SGs of each surviving pair, as (pair slab index, outer SG), per variant.
+struct TLLists {
+    const long long* off[kNVariants];   // per-SG offsets into each list
+    int2* cand[kNVariants];
+};
+
+__global__ void __launch_bounds__(kBlock) k_tl_expand(Count cSG, const long long* sgStart, const long long* tlCapOff,
+                                                      const int2* prRec, const int* prCount, TLLists Ls, Guard g0,
+                                                      Guard g1, Guard g2) {
+    if (g0.over() || g1.over() || g2.over()) return;
+    const int nSG = int(cSG.get());
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    if (item >= nSG) return;
+    const int n = prCount[item];
+    const long long off = tlCapOff[item];
+    long long base[kNVariants];
+    for (int v = 0; v < kNVariants; ++v) base[v] = Ls.off[v][item];
+    for (int p0 = 0; p0 < n; p0 += kWave) {
+        const int ne = min(kWave, n - p0);
+        int variant = TL_NONE, sgA = 0, nS = 0;
+        if (lane < ne) {
+            const int2 r = prRec[off + p0 + lane];
+            variant = int((unsigned)r.y >> kVariantShift);
+            sgA = int(sgStart[r.x]);
+            nS = int(sgStart[r.x + 1]) - sgA;
+        }
+#pragma unroll
+        for (int v = 0; v < kNVariants; ++v) {
+            int pre = variant == v ? nS : 0;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int t = __shfl_up(pre, o);
+                if (lane >= o) pre += t;
+            }
+            const int total = __shfl(pre, kWave - 1);
+            for (int k0 = 0; k0 < total; k0 += kWave) {
+                const int k = k0 + lane;
+                int inner;
+                const int e = findEntry(k, pre, ne, inner);
+                const int a = __shfl(sgA, e & (kWave - 1));
+                if (k < total) Ls.cand[v][base[v] + k] = make_int2(int(off) + p0 + e, a + inner);
+            }
+            base[v] += total;
+        }
+    }
+}
+
+template <int V>
+__global__ void __launch_bounds__(kBlock) k_tl_stage2(Count cC, HitsDev H, MDDev D, SGDev S, const int2* cand,
+                                                      const int2* prRec, const float4* prCarry, unsigned char* flag) {
+    const long long n = cC.get();
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 c = cand[i];
+    const int2 r = prRec[c.x];
+    const float4 cf = prCarry[c.x];
+    const int so = c.y;
+    const int flo = S.flags[so];
+    const TLInner I = loadInner(r.y & kSgMask, D, S);
+    const TLOutLo L = loadOutLo(r.x, flo, D);
+    const TLOutUp U = loadOutUp(so, flo, H, D, S);
+    const TLCarry cc{cf.x, cf.y, cf.z};
+    flag[i] = tlStage2(V, (flo & 2) != 0, I, L, U, cc) ? 1 : 0;
+}
+
+struct TLFlags {
+    const long long* off[kNVariants];
+    const unsigned char* flag[kNVariants];
+    const int2* cand[kNVariants];
+};
+
+__global__ void k_tl_count(Count cSG, TLFlags F, int* tlCount) {
+    const int nSG = int(cSG.get());
+    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= nSG) return;
+    int c = 0;
+    for (int v = 0; v < kNVariants; ++v)
+        for (long long k = F.off[v][item] + lane; k < F.off[v][item + 1]; k += kWave) c += F.flag[v][k];
+    c = waveSum(c);
+    if (lane == 0) tlCount[item] = c;
+}
+
+__global__ void k_tl_scatter(Count cSG, TLFlags F, const long long* tlStart, int* tlIn, int* tlOutArr, Guard g) {
     if (g.over()) return;
     const int nSG = int(cSG.get());
     const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (item >= nSG) return;
-    const int n = tlCount[item];
-    const long long src = tlCapOff[item], dst = tlStart[item];
-    for (int k = lane; k < n; k += kWave) {
-        tlIn[dst + k] = item;
-        tlOutArr[dst + k] = slabOut[src + k];
+    long long pos = tlStart[item];
+    for (int v = 0; v < kNVariants; ++v) {
+        const long long a = F.off[v][item], e = F.off[v][item + 1];
+        for (long long k0 = a; k0 < e; k0 += kWave) {
+            const long long k = k0 + lane;
+            const bool pass = k < e && F.flag[v][k];
+            const unsigned long long mask = __ballot(pass);
+            if (pass) {
+                const long long d = pos + __popcll(mask & ((1ull << lane) - 1ull));
+                tlIn[d] = item;
+                tlOutArr[d] = F.cand[v][k].y;
+            }
+            pos += __popcll(mask);
+        }
     }
 }
 
@@ -1303,7 +1380,13 @@ struct sdl_event {
     DBuf<long long> tpeOff;
     DBuf<long long> tpStart, slotTpStart;
     DBuf<long long> tlCap, tlCapOff, tlStart, slotTlStart;
-    DBuf<int> tlCount, tlSlabOut, tlIn, tlOut;
+    DBuf<int> tlCount, tlIn, tlOut;
+    DBuf<int2> prRec;
+    DBuf<float4> prCarry;
+    DBuf<int> prCount, tlNC;
+    DBuf<long long> tlCOff[3];
+    DBuf<int2> tlCand[3];
+    DBuf<unsigned char> tlFlag[3];
     DBuf<int> tcCount;
     DBuf<int2> tcpInfo;
     DBuf<long long> tcpStart, tclStart;
@@ -1468,7 +1551,7 @@ long long sizeOf(sdl_event* ev, int k) {
     } else {
         static const int stageOf[TOT_N] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                            SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
-                                           0, 0};
+                                           SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, 0, 0};
         const double scale = ev->lastNH > 0 ? double(ev->NH) / double(ev->lastNH) : 1.0;
         const bool custom = stageOf[k] >= ev->headroomFrom;
         const double f = custom ? ev->headroomFactor : 1.25;
@@ -1680,6 +1763,10 @@ int runTL(sdl_event* ev) {
     if (int r = ev->tlCount.ensure(ns)) return r;
     if (int r = ev->tlStart.ensure(ns + 1)) return r;
     if (int r = ev->slotTlStart.ensure(NS + 1)) return r;
+    if (int r = ev->prCount.ensure(ns)) return r;
+    if (int r = ev->tlNC.ensure(3 * ns)) return r;
+    for (int v = 0; v < kNVariants; ++v)
+        if (int r = ev->tlCOff[v].ensure(ns + 1)) return r;
     const Count cSG = devCount(ev, TOT_SG);
     if (nSG > 0)
         hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, ev->sgOut.p,
@@ -1688,14 +1775,54 @@ int runTL(sdl_event* ev) {
         return r;
     const long long slab = sizeOf(ev, TOT_TLCAP);
     if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
-    if (int r = ev->tlSlabOut.ensure(slab)) return r;
+    if (int r = ev->prRec.ensure(std::max(slab, 1LL))) return r;
+    if (int r = ev->prCarry.ensure(std::max(slab, 1LL))) return r;
     timerBegin(ev, T_TL);
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tl, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM, hitsView(ev),
+        hipLaunchKernelGGL(k_tl_pairs, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM,
                            mdView(ev), sgView(ev), G->dConnStart, G->dConnList, ev->mdStart.p, ev->mdCount.p,
-                           ev->sgStart.p, ev->tlCap.p, ev->tlCapOff.p, ev->tlSlabOut.p, ev->tlCount.p,
-                           guardOf(ev, TOT_TLCAP, slab, SDL_STAGE_TL));
-    timerEnd(ev, T_TL);
+                           ev->sgStart.p, ev->tlCap.p, ev->tlCapOff.p, ev->prRec.p, ev->prCarry.p, ev->prCount.p,
+                           ev->tlNC.p, ns, guardOf(ev, TOT_TLCAP, slab, SDL_STAGE_TL));
+    long long nc[kNVariants];
+    for (int v = 0; v < kNVariants; ++v)
+        if (int r = scanExclusive<int>(ev->tlNC.p + v * ns, ev->tlCOff[v].p, cSG, nSG, ev->scan, st, ev->tot.p,
+                                       TOT_TLC0 + v))
+            return r;
+    for (int v = 0; v < kNVariants; ++v) {
+        nc[v] = sizeOf(ev, TOT_TLC0 + v);
+        if (nc[v] < 0) return fail(SDL_ERR_HIP, "readback failed");
+        if (int r = ev->tlCand[v].ensure(std::max(nc[v], 1LL))) return r;
+        if (int r = ev->tlFlag[v].ensure(std::max(nc[v], 1LL))) return r;
+    }
+    TLLists Ls;
+    TLFlags F;
+    for (int v = 0; v < kNVariants; ++v) {
+        Ls.off[v] = F.off[v] = ev->tlCOff[v].p;
+        Ls.cand[v] = ev->tlCand[v].p;
+        F.cand[v] = ev->tlCand[v].p;
+        F.flag[v] = ev->tlFlag[v].p;
+    }
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tl_expand, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, ev->sgStart.p,
+                           ev->tlCapOff.p, ev->prRec.p, ev->prCount.p, Ls,
+                           guardOf(ev, TOT_TLC0, nc[0], SDL_STAGE_TL), guardOf(ev, TOT_TLC1, nc[1], SDL_STAGE_TL),
+                           guardOf(ev, TOT_TLC2, nc[2], SDL_STAGE_TL));
+    const HitsDev H = hitsView(ev);
+    if (nc[0] > 0)
+        hipLaunchKernelGGL(k_tl_stage2<TL_BBBB>, dim3(gridFor(nc[0], kBlock)), dim3(kBlock), 0, st,
+                           devCount(ev, TOT_TLC0), H, mdView(ev), sgView(ev), ev->tlCand[0].p, ev->prRec.p,
+                           ev->prCarry.p, ev->tlFlag[0].p);
+    if (nc[1] > 0)
+        hipLaunchKernelGGL(k_tl_stage2<TL_BBEE>, dim3(gridFor(nc[1], kBlock)), dim3(kBlock), 0, st,
+                           devCount(ev, TOT_TLC1), H, mdView(ev), sgView(ev), ev->tlCand[1].p, ev->prRec.p,
+                           ev->prCarry.p, ev->tlFlag[1].p);
+    if (nc[2] > 0)
+        hipLaunchKernelGGL(k_tl_stage2<TL_EEEE>, dim3(gridFor(nc[2], kBlock)), dim3(kBlock), 0, st,
+                           devCount(ev, TOT_TLC2), H, mdView(ev), sgView(ev), ev->tlCand[2].p, ev->prRec.p,
+                           ev->prCarry.p, ev->tlFlag[2].p);
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tl_count, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, F,
+                           ev->tlCount.p);
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
     const long long nTL = sizeOf(ev, TOT_TL);
     if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
@@ -1703,9 +1830,9 @@ int runTL(sdl_event* ev) {
     if (int r = ev->tlIn.ensure(std::max(nTL, 1LL))) return r;
     if (int r = ev->tlOut.ensure(std::max(nTL, 1LL))) return r;
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_repack, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, ev->tlCapOff.p,
-                           ev->tlCount.p, ev->tlStart.p, ev->tlSlabOut.p, ev->tlIn.p, ev->tlOut.p,
-                           guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
+        hipLaunchKernelGGL(k_tl_scatter, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, F,
+                           ev->tlStart.p, ev->tlIn.p, ev->tlOut.p, guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
+    timerEnd(ev, T_TL);
     hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->slotSgStart.p, ev->tlStart.p, ev->slotTlStart.p);
     if (int r = ensureItemScratch(ev, nSG)) return r;
@@ -1839,10 +1966,11 @@ int runStages(sdl_event* ev, int s0, int s1) {
     ev->nMD = t[TOT_MD]; ev->nSG = t[TOT_SG]; ev->nTP = t[TOT_TP];
     ev->nTL = t[TOT_TL]; ev->nTCP = t[TOT_TCP]; ev->nTCL = t[TOT_TCL];
     // remember the sizes of the stages that ran (the prediction for the next call)
-    const int ran[SDL_STAGE_TCL + 1][2] = {{TOT_MDCAP, TOT_MD}, {TOT_SGCAP, TOT_SG}, {TOT_TP, TOT_TPE},
-                                           {TOT_TLCAP, TOT_TL}, {TOT_TCP, TOT_TCP}, {TOT_TCL, TOT_TCL}};
-    for (int s = s0; s <= s1; ++s)
-        for (int k : ran[s]) ev->lastTot[k] = t[k];
+    static const int stageOfTot[TOT_ABORT] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
+                                              SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
+                                              SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL};
+    for (int k = 0; k < TOT_ABORT; ++k)
+        if (stageOfTot[k] >= s0 && stageOfTot[k] <= s1) ev->lastTot[k] = t[k];
     if (s1 == SDL_STAGE_TCL) {
         ev->lastNH = ev->NH;
         ev->haveHistory = true;

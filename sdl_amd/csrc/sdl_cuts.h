@@ -423,7 +423,7 @@ struct TLOutUp {
 
 // Values of the first stage that the second stage reads.
 struct TLCarry {
-    float sdlCut, thetaMuls, dPhiMid;
+    float sdlCut, thetaMuls, sinDPhiMid;  // sin of the midpoint deltaPhi (BBBB, BBEE)
 };
 
 enum { TL_BBBB = 0, TL_BBEE = 1, TL_EEEE = 2, TL_NONE = 3 };
@@ -476,7 +476,7 @@ __device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L,
     if (!(fabsF(dPhiMid) <= sdlCut)) return false;
     c.sdlCut = sdlCut;
     c.thetaMuls = thetaMuls;
-    c.dPhiMid = dPhiMid;
+    c.sinDPhiMid = sinF(dPhiMid);
     return true;
 }
 
@@ -545,7 +545,7 @@ __device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, T
     if (!(fabsF(dPhiMid) <= sdlCut)) return false;
     c.sdlCut = sdlCut;
     c.thetaMuls = thetaMuls;
-    c.dPhiMid = dPhiMid;
+    c.sinDPhiMid = EEEE ? 0.f : sinF(dPhiMid);  // the EEEE second stage has no dBetaROut term (Tracklet.cc:2493)
     return true;
 }
 
@@ -597,7 +597,7 @@ __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L,
     const float alphaOutAbsReg = fmax2(fabsF(U.dphichange), L.alphaReg3);
     const float dBetaOutLum = lOut < 11 ? 0.0f : fabsF(alphaOutAbsReg * kDeltaZLum / hOutLo.z);
     const float dBetaLum2 = (0.0f + dBetaOutLum) * (0.0f + dBetaOutLum);
-    const float sinDPhi = sinF(c.dPhiMid);
+    const float sinDPhi = c.sinDPhiMid;
     const float dBetaROut2 = float(sq(double((hiRt - loRt) * sinDPhi / drt_tl_axis)));
     const float betaOut_cut = float(double(asinF(fmin2(drt_tl_axis * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
                                     sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
@@ -673,7 +673,7 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
     const float dBetaLum2 = (dBetaInLum + dBetaOutLum) * (dBetaInLum + dBetaOutLum);
     float dBetaROut2 = 0.f;
     if (!EEEE) {
-        const float sinDPhi = sinF(c.dPhiMid);
+        const float sinDPhi = c.sinDPhiMid;
         const float hiRt = U.ec2s ? U.hiRt : hOutUp.rt;
         const float loRt = U.ec2s ? U.loRt : hOutUp.rt;
         dBetaROut2 = float(sq(double((hiRt - loRt) * sinDPhi / dr)));

@@ -74,3 +74,30 @@ def test_pow_square_is_exact():
     for x in xs:
         d = float(x)
         assert libm.pow(d, 2.0) == d * d
+
+
+def test_libm_listing_detects_planted_error():
+    """The difference listing finds and explains objects when the second policy
+    carries a planted error (float transcendentals 0.4 % too large), and those
+    objects would fail the 4-ulp rule: the listing and its rule are not vacuous."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import list_libm_differences as L
+    listed, consequences = L.differences(12, 0, oracle.NUDGED)
+    assert len(listed) > 0
+    assert all(d["cut"] >= 0 for d in listed)
+    assert max(d["ulps"] for d in listed) > 4.0
+
+
+def test_libm_listing_committed():
+    """The committed listing covers the PU200 events and lists no object: on
+    those events the device's arithmetic (CR) and the reference's (glibc) give
+    identical object sets."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "libm_differences.json")) as f:
+        d = json.load(f)
+    assert "2:0" in d["cases"] and "2:1" in d["cases"]
+    assert all(x["ulps"] <= 4.0 for x in d["differences"])
+    assert d["differences"] == []
