@@ -33,6 +33,9 @@ def parse():
     p.add_argument("--workload", type=int, default=2)
     p.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
+    p.add_argument("--learned", default=os.path.join(ROOT, "sdl_amd", "data", "proposal_w2.json"),
+                   help="trained proposal (skipped if the file is absent)")
+    p.add_argument("--recall", default="0.995", help="target TC recall whose threshold the learned leg uses")
     return p.parse_args()
 
 
@@ -76,6 +79,37 @@ def tl_roofline(workload, nevents, tl_ms):
             "kernel": "k_tl", "kernel_ms": round(tl_ms, 4),
             "peak_note": "FP32 peak 157.3 TFLOP/s (vector and matrix); k_tl issues FP32/FP64 VALU, no MFMA",
             "flops_per_launch": flops}
+
+
+def learned_leg(a, ev, step, plain_counts, plain_ms):
+    """The same steps with the learned proposal at the threshold chosen for the
+    target TC recall; recall = TCs kept / TCs of the plain chain on the step's
+    events (the learned objects are a subset of the plain ones)."""
+    from sdl_amd import hip, proposal
+    m, meta = proposal.MLP.load(a.learned)
+    t = (meta.get("thresholds") or {}).get(a.recall)
+    if t is None:
+        return {"skipped": f"no threshold for recall {a.recall} in {os.path.basename(a.learned)}"}
+    proposal.set_proposal(ev, m, t)
+    for _ in range(a.warmup):
+        step()
+    ev.sync()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    ev.sync()
+    hip.synchronize()
+    wall = time.perf_counter() - t0
+    c = ev.counts(-1)
+    proposal.set_proposal(ev, None, 0.5)
+    ms = 1000.0 * wall / a.steps
+    return {"value": round(a.events_per_step * a.steps / wall, 3), "ms_per_step": round(ms, 4),
+            "speedup_vs_plain": round(plain_ms / ms, 4), "threshold": t, "target_tc_recall": float(a.recall),
+            "tc_recall": round(c["nTC"] / max(plain_counts["nTC"], 1), 6),
+            "tl_recall": round(c["nTL"] / max(plain_counts["nTL"], 1), 6),
+            "weights": os.path.relpath(a.learned, ROOT), "trained_on": meta.get("train_seeds"),
+            "validated_on": meta.get("val_seeds")}
 
 
 def main():
@@ -136,6 +170,7 @@ def main():
         kms[k] /= a.steps
 
     roofline = tl_roofline(a.workload, B, kms["tl"])
+    learned = learned_leg(a, ev, step, counts, ms_per_step) if os.path.exists(a.learned) else None
     out = {
         "metric": "events/s on 1 MI355X at PU200 density: plain vs learned, >=99.5% TC recall",
         "value": round(B * a.steps / wall, 3),
@@ -157,6 +192,8 @@ def main():
         "objects_per_step": counts,
         "roofline": roofline,
     }
+    if learned is not None:
+        out["learned"] = learned
     if a.cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(a.workload, threads)

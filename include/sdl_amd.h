@@ -23,6 +23,7 @@ extern "C" {
 
 typedef struct sdl_geom sdl_geom;
 typedef struct sdl_event sdl_event;
+typedef struct sdl_mlp sdl_mlp;
 
 enum {
     SDL_OK = 0,
@@ -118,10 +119,44 @@ void* sdl_event_stream(sdl_event* ev);
  * (ms, in that order, n <= 4) for the last run. */
 int sdl_event_set_timing(sdl_event* ev, int on);
 int sdl_event_kernel_ms(sdl_event* ev, double* out, int n);
+/* With no counterpart in the reference: whether run_chain may capture its
+ * speculative form into a HIP graph and replay it (default on). */
+int sdl_event_set_graphs(sdl_event* ev, int on);
 /* Tuning knob with no counterpart in the reference: the headroom factor of the
  * predicted buffer sizes for the stages >= fromStage (default 1.25 for all).
  * A factor below 1 forces the resume path; the tests use it to check it. */
 int sdl_event_set_headroom(sdl_event* ev, int fromStage, double factor);
+
+/* ---------------------------------------------------------------- learned proposal
+ * No counterpart in the reference.  A 24-64-64-64-1 MLP scores the tracklet
+ * stage's (inner segment, outer-inner mini-doublet) pairs before the exact
+ * tracklet cuts; pairs scored below the threshold are dropped, every object
+ * that is kept still passes the exact cuts.  Inference runs in BF16 MFMA with
+ * FP32 accumulation; training runs FP32 forward and backward kernels with Adam.
+ * Parameters: float32, layout W1[64][24], b1[64], W2[64][64], b2[64],
+ * W3[64][64], b3[64], w4[64], b4 (sdl_mlp_num_params() values). */
+int sdl_mlp_create(sdl_mlp** out);
+void sdl_mlp_free(sdl_mlp* m);
+int sdl_mlp_num_params(void);
+int sdl_mlp_set_params(sdl_mlp* m, const float* params, int n);
+int sdl_mlp_get_params(sdl_mlp* m, float* params, int n);
+/* Gradient of the mean weighted binary cross entropy (positives weighted by
+ * posWeight) over n DEVICE rows dX (n x 24) with labels dY, without an update. */
+int sdl_mlp_gradient(sdl_mlp* m, const float* dX, const float* dY, long long n, float posWeight, float* grad,
+                     float* loss);
+/* `steps` Adam steps on consecutive batches of rows dIdx[start + k*batch ...]
+ * (device indices; NULL: the rows themselves); *lastLoss: the last batch's loss. */
+int sdl_mlp_train(sdl_mlp* m, const float* dX, const float* dY, const int* dIdx, long long start, int batch, int steps,
+                  float lr, float posWeight, float* lastLoss);
+/* Logits of n DEVICE feature rows through the BF16 MFMA inference path. */
+int sdl_mlp_forward(sdl_mlp* m, const float* dX, long long n, float* dLogit);
+/* Uses m to drop tracklet pairs whose score is below threshold (0 < t < 1);
+ * m = NULL restores the plain chain. */
+int sdl_event_set_proposal(sdl_event* ev, sdl_mlp* m, float threshold);
+/* Training rows from the last plain chain: features (24 per pair, DEVICE dX)
+ * and labels (1 if the pair yields a tracklet, DEVICE dY) of every pair; with
+ * dX = NULL only *n (the number of pairs) is returned. */
+int sdl_event_tl_pair_rows(sdl_event* ev, float* dX, float* dY, long long cap, long long* n);
 
 #ifdef __cplusplus
 }

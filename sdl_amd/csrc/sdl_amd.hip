@@ -27,6 +27,7 @@
 
 #include "../../include/sdl_amd.h"
 #include "sdl_cuts.h"
+#include "sdl_mlp.h"
 
 using namespace sdl;
 
@@ -48,6 +49,15 @@ int fail(int code, const std::string& msg) {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kNVariantsTL = 3;  // tracklet algorithm variants with device kernels: BBBB, BBEE, EEEE
+
+// Set while a chain is being captured into a HIP graph: an allocation is not
+// allowed then, so a buffer that must grow ends the capture (the chain then
+// runs uncaptured once, which grows it).
+thread_local bool g_capturing = false;
+thread_local bool g_captureBlocked = false;
+// Bumped by every (re)allocation, so a captured graph knows its buffers moved.
+thread_local unsigned long long g_allocEpoch = 0;
 
 template <class T>
 struct DBuf {
@@ -59,6 +69,11 @@ struct DBuf {
     ~DBuf() { release(); }
     int ensure(size_t n) {
         if (n <= cap && p) return 0;
+        if (g_capturing) {
+            g_captureBlocked = true;
+            return fail(SDL_ERR_STATE, "allocation during graph capture");
+        }
+        ++g_allocEpoch;
         size_t want = std::max<size_t>(n, 1);
         want = std::max(want, cap + cap / 2);
         // queued work may still read the old buffer
@@ -88,6 +103,7 @@ struct DBuf {
 enum {
     TOT_MDCAP = 0, TOT_MD, TOT_SGCAP, TOT_SG, TOT_TP, TOT_TLCAP, TOT_TL, TOT_TPE, TOT_TCP, TOT_TCL,
     TOT_TLC0, TOT_TLC1, TOT_TLC2,  // second-stage tracklet candidates per variant
+    TOT_TLP0, TOT_TLP1, TOT_TLP2,  // first-stage tracklet pairs per variant
     TOT_ABORT, TOT_UNKNOWN, TOT_N
 };
 
@@ -117,12 +133,23 @@ struct Guard {
     }
 };
 
+// Largest k in [0, n) with off[k] <= j, for off[0] <= j < off[n].
+__device__ __forceinline__ long long rangeOf(const long long* off, long long n, long long j) {
+    long long lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const long long mid = (lo + hi) >> 1;
+        if (off[mid] <= j) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // ------------------------------------------------------------------ scans
 // Exclusive scan of n counts into n+1 offsets (out[n] = total) in one launch:
-// single-pass chained scan with decoupled look-back.  Tiles take their index
-// from a ticket counter, so every predecessor of a tile has started when the
-// tile looks back, and a predecessor publishes its aggregate before it waits
-// on anything: every wait ends.  A tile's state word packs
+// chained scan with decoupled look-back.  Tiles take their index from a ticket
+// counter, so every predecessor of a tile has started when the tile looks
+// back, and a predecessor publishes its aggregate before it waits on anything:
+// every wait ends.  A tile's state word packs
 // [epoch:20][flag:2][value:42]; the epoch changes after every scan (the last
 // tile to finish advances it and re-arms the ticket), so the state array needs
 // no clearing between scans.  Totals of 2^42 or more are reported as -1.
@@ -146,8 +173,65 @@ __device__ __forceinline__ unsigned long long scanPack(unsigned epoch, unsigned 
     return ((unsigned long long)epoch << 44) | ((unsigned long long)flag << 42) | val;
 }
 
+// Sum over the block of one value per thread (waveSum: kWavesPerBlock shared slots).
+__device__ __forceinline__ long long blockSum(long long v, long long* waveSum) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) v += __shfl_xor(v, off);
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) waveSum[threadIdx.x / kWave] = v;
+    __syncthreads();
+    long long t = 0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) t += waveSum[w];
+    return t;
+}
+
+// Exclusive scan of the kScanTile items at [base, base + kScanTile) (zeros past
+// n) written as run + prefix; returns the items' total.
 template <class TIn>
-__global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, Count cn, int nTiles,
+__device__ __forceinline__ long long scanSubTile(const TIn* in, long long* out, long long base, long long n,
+                                                 long long run, long long* waveSum) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const long long b = base + (long long)threadIdx.x * kScanItems;
+    long long v[kScanItems];
+    long long s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long i = b + k;
+        v[k] = i < n ? (long long)in[i] : 0;
+        s += v[k];
+    }
+    long long x = s;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const long long y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    __syncthreads();
+    if (lane == kWave - 1) waveSum[wave] = x;
+    __syncthreads();
+    long long wPrefix = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        if (w < wave) wPrefix += waveSum[w];
+        total += waveSum[w];
+    }
+    long long r = run + wPrefix + x - s;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long i = b + k;
+        if (i < n) out[i] = r;
+        r += v[k];
+    }
+    return total;
+}
+
+// Tiles own `chunk` consecutive items (a multiple of kScanTile) and there are
+// at most kWave of them, so one look-back window covers every predecessor: a
+// tile sums its chunk, publishes the sum, reads its predecessors' words in one
+// round and scans its chunk a second time writing the offsets.
+template <class TIn>
+__global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, Count cn, int nTiles, long long chunk,
                                                  unsigned long long* state, unsigned long long stateCap,
                                                  ScanCtl* ctl, long long* tot, int totIdx) {
     __shared__ long long waveSum[kWavesPerBlock];
@@ -161,37 +245,27 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
     __syncthreads();
     const int tile = tileSh;
     const unsigned epoch = epochSh;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     // every tile of the launch takes part (the ticket protocol needs all of
     // them); tiles past the end of the input scan zeros
     long long n = cn.get();
-    if (n > (long long)nTiles * kScanTile) {  // launched for fewer items than the total: abort the run
+    if (n > (long long)nTiles * chunk) {  // launched for fewer items than the total: abort the run
         if (threadIdx.x == 0) tot[TOT_ABORT] = 1 + 100;
         n = 0;
     }
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const long long base = (long long)tile * kScanTile + (long long)threadIdx.x * kScanItems;
-    long long v[kScanItems];
-    long long s = 0;
+    const long long begin = (long long)tile * chunk;
+    const long long end = min(begin + chunk, n);
+    // pass 1: the chunk's sum
+    long long part = 0;
+    for (long long sub = begin; sub < end; sub += kScanTile) {
+        const long long b = sub + (long long)threadIdx.x * kScanItems;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const long long i = base + k;
-        v[k] = i < n ? (long long)in[i] : 0;
-        s += v[k];
+        for (int k = 0; k < kScanItems; ++k) {
+            const long long i = b + k;
+            if (i < end) part += (long long)in[i];
+        }
     }
-    long long x = s;  // inclusive within the wave
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const long long y = __shfl_up(x, off);
-        if (lane >= off) x += y;
-    }
-    if (lane == kWave - 1) waveSum[wave] = x;
-    __syncthreads();
-    long long wPrefix = 0, agg = 0;
-#pragma unroll
-    for (int w = 0; w < kWavesPerBlock; ++w) {
-        if (w < wave) wPrefix += waveSum[w];
-        agg += waveSum[w];
-    }
+    const long long agg = blockSum(part, waveSum);
     if (wave == 0) {
         long long excl = 0;
         if (tile == 0) {
@@ -216,10 +290,10 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
                 const unsigned long long inclMask = __ballot(incl);
                 // lanes up to and including the nearest inclusive predecessor
                 const int stop = inclMask ? __builtin_ctzll(inclMask) : kWave - 1;
-                long long part = lane <= stop ? val : 0;
+                long long pv = lane <= stop ? val : 0;
 #pragma unroll
-                for (int off = 1; off < kWave; off <<= 1) part += __shfl_xor(part, off);
-                excl += part;
+                for (int off = 1; off < kWave; off <<= 1) pv += __shfl_xor(pv, off);
+                excl += pv;
                 if (inclMask) break;
                 j -= kWave;
             }
@@ -230,13 +304,9 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
     }
     __syncthreads();
     const long long prefix = prefixSh;
-    long long run = prefix + wPrefix + x - s;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const long long i = base + k;
-        if (i < n) out[i] = run;
-        run += v[k];
-    }
+    // pass 2: offsets
+    long long run = prefix;
+    for (long long sub = begin; sub < end; sub += kScanTile) run += scanSubTile(in, out, sub, end, run, waveSum);
     if (tile == nTiles - 1 && threadIdx.x == 0) {
         const long long total = prefix + agg;
         const long long t = (unsigned long long)prefix >= kScanValMask || (unsigned long long)total >= kScanValMask ? -1 : total;
@@ -261,6 +331,73 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
     }
 }
 
+// Large scans: reduce-then-scan over tiles of kScanTile items, fully parallel.
+template <class TIn>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const TIn* in, Count cn, long long* tileSum) {
+    __shared__ long long waveSum[kWavesPerBlock];
+    const long long n = cn.get();
+    const long long b = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    long long part = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (b + k < n) part += (long long)in[b + k];
+    const long long t = blockSum(part, waveSum);
+    if (threadIdx.x == 0) tileSum[blockIdx.x] = t;
+}
+
+// One block: exclusive scan of the tile sums; the grand total goes to out[n] and tot[totIdx].
+__global__ void __launch_bounds__(1024) k_scan_tiles(const long long* tileSum, int nTiles, long long* tileOff,
+                                                     long long* out, Count cn, long long* tot, int totIdx) {
+    __shared__ long long sh[1024];
+    long long n = cn.get();
+    if (n > (long long)nTiles * kScanTile) {  // launched for fewer items than the total: abort the run
+        if (threadIdx.x == 0) tot[TOT_ABORT] = 1 + 100;
+        n = 0;
+    }
+    const int per = (nTiles + 1023) / 1024;
+    const int t0 = threadIdx.x * per;
+    long long s = 0;
+    for (int k = 0; k < per; ++k)
+        if (t0 + k < nTiles) s += tileSum[t0 + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    long long run = sh[threadIdx.x] - s;
+    for (int k = 0; k < per; ++k)
+        if (t0 + k < nTiles) {
+            tileOff[t0 + k] = run;
+            run += tileSum[t0 + k];
+        }
+    if (threadIdx.x == 1023) {
+        const long long total = sh[1023];
+        const long long t = (unsigned long long)total >= kScanValMask ? -1 : total;
+        out[n] = t;
+        if (totIdx >= 0) tot[totIdx] = t;
+    }
+}
+
+template <class TIn>
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const TIn* in, long long* out, Count cn,
+                                                       const long long* tileOff) {
+    __shared__ long long waveSum[kWavesPerBlock];
+    const long long n = cn.get();
+    const long long base = (long long)blockIdx.x * kScanTile;
+    if (base >= n) return;  // uniform over the block
+    scanSubTile(in, out, base, n, tileOff[blockIdx.x], waveSum);
+}
+
+// Zero-fill (a kernel rather than a memset, so that captured graphs replay it reliably).
+template <class T>
+__global__ void k_zero(T* p, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long n, unsigned long long v) {
     const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -270,12 +407,18 @@ struct ScanScratch {
     unsigned long long* state = nullptr;
     unsigned long long cap = 0;
     ScanCtl* ctl = nullptr;
+    DBuf<long long> tileSum, tileOff;
     ~ScanScratch() {
         if (state) (void)hipFree(state);
         if (ctl) (void)hipFree(ctl);
     }
     // Makes room for nTiles tile states; new memory starts at epoch 0 (never current).
     int ensure(unsigned long long nTiles, hipStream_t st) {
+        if (g_capturing && (!ctl || nTiles > cap)) {
+            g_captureBlocked = true;
+            return fail(SDL_ERR_STATE, "allocation during graph capture");
+        }
+        if (!ctl || nTiles > cap) ++g_allocEpoch;
         if (!ctl) {
             if (hipMalloc(&ctl, sizeof(ScanCtl)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
             const ScanCtl init{0, 0, 1};
@@ -305,18 +448,29 @@ template <class TIn>
 int scanExclusive(const TIn* in, long long* out, Count cn, long long nMax, ScanScratch& sc, hipStream_t st,
                   long long* tot, int totIdx) {
     const long long nTiles = std::max(1LL, (nMax + kScanTile - 1) / kScanTile);
+    if (nTiles <= kWave) {
+        // one launch; every predecessor of a tile lies in its first look-back window
+        if (int r = sc.ensure((unsigned long long)nTiles, st)) return r;
+        hipLaunchKernelGGL(k_scan<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, cn, int(nTiles),
+                           (long long)kScanTile, sc.state, sc.cap, sc.ctl, tot, totIdx);
+        return 0;
+    }
     if (nTiles > (1LL << 30)) return fail(SDL_ERR_CAPACITY, "scan too large");
-    if (int r = sc.ensure((unsigned long long)nTiles, st)) return r;
-    hipLaunchKernelGGL(k_scan<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, cn, int(nTiles), sc.state,
-                       sc.cap, sc.ctl, tot, totIdx);
+    if (int r = sc.tileSum.ensure(nTiles)) return r;
+    if (int r = sc.tileOff.ensure(nTiles)) return r;
+    hipLaunchKernelGGL(k_scan_reduce<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, cn, sc.tileSum.p);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, sc.tileSum.p, int(nTiles), sc.tileOff.p, out, cn, tot,
+                       totIdx);
+    hipLaunchKernelGGL(k_scan_apply<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, cn, sc.tileOff.p);
     return 0;
 }
 
 // ------------------------------------------------------------------ hits
-__global__ void k_hit_prep(int nh, const float* x, const float* y, const float* z, const unsigned* detId,
+__global__ void k_hit_prep(const float* x, const float* y, const float* z, const unsigned* detId,
                            const long long* evOff, int E, const uint32_t* sortedDet, int NM, const ModDev* mods,
                            float* rt, float* r3, float* phi, float* alpha, int* slot, float4* eHi, float4* eLo,
                            int* slotCount, long long* tot) {
+    const long long nh = evOff[E];  // hits of the batch (uploaded with the event offsets)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nh) return;
     // event of the hit
@@ -357,7 +511,9 @@ __global__ void k_hit_prep(int nh, const float* x, const float* y, const float* 
     atomicAdd(&slotCount[s], 1);
 }
 
-__global__ void k_hit_scatter(int nh, const int* slot, const long long* slotStart, int* slotFill, int* sortedHit) {
+__global__ void k_hit_scatter(const long long* evOff, int E, const int* slot, const long long* slotStart, int* slotFill,
+                              int* sortedHit) {
+    const long long nh = evOff[E];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nh) return;
     const int s = slot[i];
@@ -428,48 +584,38 @@ struct HitsDev {
     const float4 *eHi, *eLo;
 };
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_md(int NS, int NM, const ModDev* mods, HitsDev H, const long long* slotHitStart,
-                                               const int* sortedHit, const int* mdCap, const long long* mdCapOff,
-                                               int2* slabHits, float4* slabF, int* mdCount, Guard g) {
+// One thread per candidate (lower hit x upper hit of a lower slot, in the
+// reference's loop order): the MD algorithm, its cached values at the
+// candidate's index, a pass flag, and a count per slot.
+__global__ void __launch_bounds__(kBlock)
+k_md(Count cCand, int NS, int NM, const ModDev* mods, HitsDev H, const long long* slotHitStart, const int* sortedHit,
+     const long long* mdCapOff, int2* slabHits, float4* slabF, int* slabSlot, unsigned char* flag, int* mdCount,
+     Guard g) {
     if (g.over()) return;
-    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (item >= NS) return;
-    const int cap = mdCap[item];
-    if (cap == 0) {
-        if (lane == 0) mdCount[item] = 0;
-        return;
-    }
+    const long long nCand = cCand.get();
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nCand) return;
+    const int item = int(rangeOf(mdCapOff, NS, j));
     const int m = item % NM;
     const ModDev md = mods[m];
     const int ps = item - m + md.partner;
-    const long long loStart = slotHitStart[item];
     const long long upStart = slotHitStart[ps];
     const int nUp = int(slotHitStart[ps + 1] - upStart);
-    const long long off = mdCapOff[item];
-    int base = 0;
-    for (int k0 = 0; k0 < cap; k0 += kWave) {
-        const int k = k0 + lane;
-        bool pass = false;
-        MDOut o;
-        int lo = 0, up = 0;
-        if (k < cap) {
-            const int i = k / nUp, j = k - (k / nUp) * nUp;
-            lo = sortedHit[loStart + i];
-            up = sortedHit[upStart + j];
-            const HitView hl = loadHit(lo, H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
-            const HitView hu = loadHit(up, H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
-            pass = mdAlgo(hl, hu, md, o);
-        }
-        const unsigned long long mask = __ballot(pass);
-        if (pass) {
-            const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
-            slabHits[off + pos] = make_int2(lo, up);
-            slabF[off + pos] = make_float4(o.dz, o.shiftedDz, o.dphi, o.dphichange);
-        }
-        base += __popcll(mask);
+    const int k = int(j - mdCapOff[item]);
+    const int i = k / nUp, jj = k - i * nUp;
+    const int lo = sortedHit[slotHitStart[item] + i];
+    const int up = sortedHit[upStart + jj];
+    const HitView hl = loadHit(lo, H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
+    const HitView hu = loadHit(up, H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
+    MDOut o;
+    const bool pass = mdAlgo(hl, hu, md, o);
+    flag[j] = pass ? 1 : 0;
+    if (pass) {
+        slabHits[j] = make_int2(lo, up);
+        slabF[j] = make_float4(o.dz, o.shiftedDz, o.dphi, o.dphichange);
+        slabSlot[j] = item;
+        atomicAdd(&mdCount[item], 1);
     }
-    if (lane == 0) mdCount[item] = base;
 }
 
 struct MDDev {
@@ -478,39 +624,39 @@ struct MDDev {
     float4* b;   // anchor r3, phi; dz, shiftedDz
     float2* c;   // dphi, dphichange
     float4* p;   // alpha1 = asin(min(rt k2, .95)), tanf(alpha1), tan(alpha1)/alpha1 (double), asin(min(rt k2 / 3, .95))
+    double* q;   // mulsFactor(z, rt) of the anchor
 };
 
 // Dense repack; the anchor is the pixel hit of a PS module, else the lower hit
 // (MiniDoublet.cc:60-81).
-__global__ void k_md_repack(int NS, int NM, const ModDev* mods, HitsDev H, const long long* mdCapOff,
-                            const int* mdCount, const long long* mdStart, const int2* slabHits, const float4* slabF,
+// One thread per candidate: survivors to their dense position (the exclusive
+// scan of the pass flags, i.e. slot order then the reference's loop order).
+__global__ void k_md_repack(Count cCand, const ModDev* mods, int NM, HitsDev H, const unsigned char* flag,
+                            const long long* pos, const int2* slabHits, const float4* slabF, const int* slabSlot,
                             MDDev D, Guard g) {
     if (g.over()) return;
-    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (item >= NS) return;
-    const int n = mdCount[item];
-    if (n == 0) return;
+    const long long nCand = cCand.get();
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nCand || !flag[j]) return;
+    const long long o = pos[j];
+    const int2 h = slabHits[j];
+    const float4 f = slabF[j];
+    const int item = slabSlot[j];
     const ModDev md = mods[item % NM];
     const bool upAnchor = md.type == TYPE_PS && md.layerType != LT_PIXEL;
-    const long long src = mdCapOff[item], dst = mdStart[item];
-    for (int k = lane; k < n; k += kWave) {
-        const int2 h = slabHits[src + k];
-        const float4 f = slabF[src + k];
-        const int an = upAnchor ? h.y : h.x;
-        const long long o = dst + k;
-        D.lo[o] = h.x;
-        D.up[o] = h.y;
-        D.anchor[o] = an;
-        D.slot[o] = item;
-        const float rta = H.rt[an];
-        const float a1 = H.alpha[an];
-        D.a[o] = make_float4(H.x[an], H.y[an], H.z[an], rta);
-        D.b[o] = make_float4(H.r3[an], H.phi[an], f.x, f.y);
-        D.c[o] = make_float2(f.z, f.w);
-        D.p[o] = make_float4(a1, tanF(a1), float(tan(double(a1)) / double(a1)),
-                             asinF(fmin2(rta * k2Rinv() / 3.0f, kSinAlphaMax)));
-    }
+    const int an = upAnchor ? h.y : h.x;
+    D.lo[o] = h.x;
+    D.up[o] = h.y;
+    D.anchor[o] = an;
+    D.slot[o] = item;
+    const float rta = H.rt[an];
+    const float a1 = H.alpha[an];
+    D.a[o] = make_float4(H.x[an], H.y[an], H.z[an], rta);
+    D.b[o] = make_float4(H.r3[an], H.phi[an], f.x, f.y);
+    D.c[o] = make_float2(f.z, f.w);
+    D.p[o] = make_float4(a1, tanF(a1), float(tan(double(a1)) / double(a1)),
+                         asinF(fmin2(rta * k2Rinv() / 3.0f, kSinAlphaMax)));
+    D.q[o] = mulsFactor(H.z[an], rta);
 }
 
 __device__ __forceinline__ MDView loadMD(int i, const MDDev& D) {
@@ -733,33 +879,92 @@ __global__ void k_tp_write(Count cSG, const int* sgOut, const long long* sgStart
 
 // ------------------------------------------------------------------ TL
 // TL candidates of one inner SG: SGs stored at every module connected to the
-// inner SG's outer-MD module (Event.cc:627-693).
-__global__ void k_tl_cap(Count cSG, int NM, const int* sgOut, const int* mdSlot, const int* connStart,
-                         const int* connList, const long long* slotSgStart, long long* tlCap) {
+// inner SG's outer-MD module (Event.cc:627-693).  The tracklet algorithm runs
+// as a cascade of dense passes:
+//  1. pairs: (inner SG, MD of a connected module that starts an SG), listed per
+//     algorithm variant (BBBB, BBEE, EEEE; uniform over a connected module);
+//  2. k_tl_stage1<V>, one thread per pair: the first-stage cuts, which involve
+//     only the inner SG and the outer SG's inner MD (Tracklet.cc:902-1153,
+//     1540-1771, 2018-2270), once per pair instead of once per candidate;
+//  3. k_tl_expand<V>, one thread per surviving pair: its outer SGs, densely;
+//  4. k_tl_stage2<V>, one thread per candidate: the remaining cuts;
+//  5. survivors grouped by inner SG, sorted by outer SG inside each group.
+__device__ __forceinline__ int tlModuleVariant(int innerFlags, const ModDev& outLoModule) {
+    return tlVariant(innerFlags & 1, innerFlags & 2, outLoModule.subdet == SUB_BARREL);
+}
+
+// MDs of a slot that start at least one SG
+__global__ void k_slot_md_sg(Count cNS, const long long* mdStart, const long long* sgStart, int* out) {
+    const int NS = int(cNS.get());
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NS) return;
+    int c = 0;
+    for (long long m = mdStart[t]; m < mdStart[t + 1]; ++m) c += sgStart[m + 1] > sgStart[m];
+    out[t] = c;
+}
+
+// Per inner SG: TL candidates (for the counters) and pairs per variant.
+__global__ void k_tl_cap(Count cSG, int NM, const ModDev* mods, const int* sgOut, const int* sgFlags,
+                         const int* mdSlot, const int* connStart, const int* connList, const long long* slotSgStart,
+                         const int* slotMdSg, long long* tlCap, int* nPair, long long stride) {
     const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSG) return;
     const int b = mdSlot[sgOut[s]];
     const int m = b % NM, eb = b - m;
+    const int fl = sgFlags[s];
     long long c = 0;
+    int np[kNVariantsTL] = {0, 0, 0};
     for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
         const int t = eb + connList[k];
         c += slotSgStart[t + 1] - slotSgStart[t];
+        const int v = tlModuleVariant(fl, mods[connList[k]]);
+        if (v < kNVariantsTL) np[v] += slotMdSg[t];
     }
     tlCap[s] = c;
+    for (int v = 0; v < kNVariantsTL; ++v) nPair[v * stride + s] = np[v];
 }
 
-// The tracklet algorithm runs as a cascade of passes, so that the expensive
-// second stage sees dense work:
-//  1. k_tl_pairs, one wave per inner SG: the first-stage cuts, which depend only
-//     on the inner SG and on the outer SG's inner MD, once per (inner SG, MD of
-//     a connected module) pair; surviving pairs, with the values the second
-//     stage reads, go to a slab at the inner SG's candidate offset;
-//  2. k_tl_expand, one wave per inner SG: the outer SGs starting at the surviving
-//     pairs' MDs, listed densely per algorithm variant (BBBB, BBEE, EEEE);
-//  3. k_tl_stage2<variant>, one thread per listed candidate: the second stage;
-//  4. k_tl_count and k_tl_scatter, one wave per inner SG: the survivors, in
-//     (variant, reference candidate) order within each inner SG.
+struct TLPairs {
+    const long long* off[kNVariantsTL];  // per inner SG
+    int2* pair[kNVariantsTL];            // (inner SG, outer-inner MD)
+};
+
+// The MDs of each slot that start at least one SG, in MD order.
+__global__ void k_slot_md_sg_list(Count cNS, const long long* mdStart, const long long* sgStart,
+                                  const long long* listStart, int* list) {
+    const int NS = int(cNS.get());
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NS) return;
+    long long d = listStart[t];
+    for (long long m = mdStart[t]; m < mdStart[t + 1]; ++m)
+        if (sgStart[m + 1] > sgStart[m]) list[d++] = int(m);
+}
+
+// One thread per inner SG: its pairs, in connected-module and MD order.
+__global__ void k_tl_enum(Count cSG, int NM, const ModDev* mods, const int* sgOut, const int* sgFlags,
+                          const int* mdSlot, const int* connStart, const int* connList, const long long* listStart,
+                          const int* list, TLPairs P, Guard g0, Guard g1, Guard g2) {
+    if (g0.over() || g1.over() || g2.over()) return;
+    const int nSG = int(cSG.get());
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSG) return;
+    const int b = mdSlot[sgOut[s]];
+    const int m = b % NM, eb = b - m;
+    const int fl = sgFlags[s];
+    long long pos[kNVariantsTL];
+    for (int v = 0; v < kNVariantsTL; ++v) pos[v] = P.off[v][s];
+    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
+        const int v = tlModuleVariant(fl, mods[connList[k]]);
+        if (v >= kNVariantsTL) continue;
+        const int t = eb + connList[k];
+        int2* dst = P.pair[v];
+        long long p = pos[v];
+        for (long long q = listStart[t]; q < listStart[t + 1]; ++q) dst[p++] = make_int2(s, list[q]);
+        pos[v] = p;
+    }
+}
+
 __device__ __forceinline__ TLOutLo loadOutLo(int mLo, int flags, const MDDev& D) {
     TLOutLo L;
     L.a = loadAnchor(mLo, D);
@@ -785,6 +990,7 @@ __device__ __forceinline__ TLInner loadInner(int s, const MDDev& D, const SGDev&
     I.dr = xi.x;
     I.d = xi.y;
     I.alphaReg3Lo = D.p[mi].w;
+    I.sqrtR3Rt = D.q[mi];
     const int fl = S.flags[s];
     I.bLo = fl & 1; I.bUp = fl & 2; I.psLo = fl & 4; I.psUp = fl & 8;
     return I;
@@ -812,184 +1018,167 @@ __device__ __forceinline__ TLOutUp loadOutUp(int so, int flo, HitsDev H, const M
     return U;
 }
 
-__device__ __forceinline__ int waveSum(int v) {
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-// Pair record: (outer-inner MD, inner SG | variant << 30) and the carried values.
-constexpr int kVariantShift = 30;
-constexpr int kSgMask = (1 << kVariantShift) - 1;
-constexpr int kNVariants = 3;
-
-__global__ void __launch_bounds__(kBlock) k_tl_pairs(Count cSG, int NM, MDDev D, SGDev S, const int* connStart,
-                                                     const int* connList, const long long* mdStart, const int* mdCount,
-                                                     const long long* sgStart, const long long* tlCap,
-                                                     const long long* tlCapOff, int2* prRec, float4* prCarry,
-                                                     int* prCount, int* nCand, long long nCandStride, Guard g) {
-    if (g.over()) return;
-    const int nSG = int(cSG.get());
-    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
-    const int lane = threadIdx.x & 63;
-    if (item >= nSG) return;
-    int cnt[kNVariants] = {0, 0, 0};
-    int base = 0;
-    if (tlCap[item] > 0) {
-        const TLInner I = loadInner(item, D, S);
-        const int b = D.slot[S.out[item]];
-        const int m = b % NM, eb = b - m;
-        const int c0 = connStart[m], nc = connStart[m + 1] - c0;
-        const long long off = tlCapOff[item];
-        for (int cb = 0; cb < nc; cb += kWave) {
-            const int ne = min(kWave, nc - cb);
-            int mc = 0;
-            if (lane < ne) mc = mdCount[eb + connList[c0 + cb + lane]];
-            int pre = mc;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int t = __shfl_up(pre, o);
-                if (lane >= o) pre += t;
-            }
-            const int totalA = __shfl(pre, kWave - 1);
-            for (int kA0 = 0; kA0 < totalA; kA0 += kWave) {
-                const int k = kA0 + lane;
-                int inner;
-                const int e = findEntry(k, pre, ne, inner);
-                bool ok = false;
-                int mLo = 0, nS = 0, variant = TL_NONE;
-                TLCarry c{0.f, 0.f, 0.f};
-                if (k < totalA) {
-                    const int t = eb + connList[c0 + cb + e];
-                    mLo = int(mdStart[t]) + inner;
-                    const int sgA = int(sgStart[mLo]);
-                    nS = int(sgStart[mLo + 1]) - sgA;
-                    if (nS > 0) {
-                        const TLOutLo L = loadOutLo(mLo, S.flags[sgA], D);
-                        variant = tlVariant(I.bLo, I.bUp, L.barrel);
-                        ok = tlStage1(variant, I, L, c);
-                    }
-                }
-                const unsigned long long mask = __ballot(ok);
-                if (ok) {
-                    const long long q = off + base + __popcll(mask & ((1ull << lane) - 1ull));
-                    prRec[q] = make_int2(mLo, item | (variant << kVariantShift));
-                    prCarry[q] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);
-                }
-                base += __popcll(mask);
-#pragma unroll
-                for (int v = 0; v < kNVariants; ++v) cnt[v] += waveSum(ok && variant == v ? nS : 0);
-            }
-        }
-    }
-    if (lane == 0) {
-        prCount[item] = base;
-        for (int v = 0; v < kNVariants; ++v) nCand[v * nCandStride + item] = cnt[v];
-    }
-}
-
-// The outer SGs of each surviving pair, as (pair slab index, outer SG), per variant.
-struct TLLists {
-    const long long* off[kNVariants];   // per-SG offsets into each list
-    int2* cand[kNVariants];
-};
-
-__global__ void __launch_bounds__(kBlock) k_tl_expand(Count cSG, const long long* sgStart, const long long* tlCapOff,
-                                                      const int2* prRec, const int* prCount, TLLists Ls, Guard g0,
-                                                      Guard g1, Guard g2) {
-    if (g0.over() || g1.over() || g2.over()) return;
-    const int nSG = int(cSG.get());
-    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
-    const int lane = threadIdx.x & 63;
-    if (item >= nSG) return;
-    const int n = prCount[item];
-    const long long off = tlCapOff[item];
-    long long base[kNVariants];
-    for (int v = 0; v < kNVariants; ++v) base[v] = Ls.off[v][item];
-    for (int p0 = 0; p0 < n; p0 += kWave) {
-        const int ne = min(kWave, n - p0);
-        int variant = TL_NONE, sgA = 0, nS = 0;
-        if (lane < ne) {
-            const int2 r = prRec[off + p0 + lane];
-            variant = int((unsigned)r.y >> kVariantShift);
-            sgA = int(sgStart[r.x]);
-            nS = int(sgStart[r.x + 1]) - sgA;
-        }
-#pragma unroll
-        for (int v = 0; v < kNVariants; ++v) {
-            int pre = variant == v ? nS : 0;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int t = __shfl_up(pre, o);
-                if (lane >= o) pre += t;
-            }
-            const int total = __shfl(pre, kWave - 1);
-            for (int k0 = 0; k0 < total; k0 += kWave) {
-                const int k = k0 + lane;
-                int inner;
-                const int e = findEntry(k, pre, ne, inner);
-                const int a = __shfl(sgA, e & (kWave - 1));
-                if (k < total) Ls.cand[v][base[v] + k] = make_int2(int(off) + p0 + e, a + inner);
-            }
-            base[v] += total;
-        }
-    }
+// ---- learned proposal (pairs scored before the first stage)
+// Features of an (inner SG, outer-inner MD) pair, all from stored records.
+__device__ __forceinline__ void tlPairFeatures(const TLInner& I, const TLOutLo& L, int variant, float* f) {
+    const float drIn = fmaxf(I.up.rt - I.lo.rt, 1e-3f);
+    const float drOut = fmaxf(L.a.rt - I.lo.rt, 1e-3f);
+    const float pointIn = (I.up.z - I.lo.z) / drIn;
+    const float pointOut = (L.a.z - I.lo.z) / drOut;
+    f[0] = I.dphichange;
+    f[1] = I.dphichangemax - I.dphichangemin;
+    f[2] = I.dr * 0.05f;
+    f[3] = I.d * 0.05f;
+    f[4] = I.lo.rt * 0.01f;
+    f[5] = I.lo.z * 0.01f;
+    f[6] = I.up.rt * 0.01f;
+    f[7] = I.up.z * 0.01f;
+    f[8] = L.a.rt * 0.01f;
+    f[9] = L.a.z * 0.01f;
+    f[10] = L.alpha1;
+    f[11] = I.alphaReg3Lo;
+    f[12] = L.alphaReg3;
+    f[13] = phiMpiPi(L.a.phi - I.up.phi);
+    f[14] = phiMpiPi(L.a.phi - I.lo.phi);
+    f[15] = fminf(fmaxf(pointOut - pointIn, -5.f), 5.f);
+    f[16] = (L.a.rt - I.up.rt) * 0.02f;
+    f[17] = phiMpiPi(L.a.phi - I.up.phi) - I.dphichange;
+    f[18] = I.psLo ? 1.f : 0.f;
+    f[19] = I.psUp ? 1.f : 0.f;
+    f[20] = L.ps ? 1.f : 0.f;
+    f[21] = L.barrel ? 1.f : 0.f;
+    f[22] = variant == TL_BBEE ? 1.f : 0.f;
+    f[23] = variant == TL_EEEE ? 1.f : 0.f;
 }
 
 template <int V>
+__device__ __forceinline__ void pairFeatures(long long i, const int2* pair, const long long* sgStart, MDDev D,
+                                             SGDev S, float* f) {
+    const int2 pr = pair[i];
+    const TLInner I = loadInner(pr.x, D, S);
+    const TLOutLo L = loadOutLo(pr.y, S.flags[sgStart[pr.y]], D);
+    tlPairFeatures(I, L, V, f);
+}
+
+// Keep flag of every pair: the network's logit against the threshold, one wave per 16 pairs.
+template <int V>
+__global__ void __launch_bounds__(kBlock) k_tl_score(Count cP, MDDev D, SGDev S, const long long* sgStart,
+                                                     const int2* pair, const mlp::Packed* P, float logitCut,
+                                                     unsigned char* keep) {
+    const long long n = cP.get();
+    const int lane = threadIdx.x & 63;
+    const long long nTiles = (n + 15) / 16;
+    const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    if (w0 >= nTiles) return;  // uniform over the wave
+    mlp::Frags F;
+    mlp::loadFrags(P, lane, F);
+    for (long long tile = w0; tile < nTiles; tile += nw) {
+        const long long i = tile * 16 + (lane & 15);
+        float f[mlp::kIn];
+        if (i < n) pairFeatures<V>(i, pair, sgStart, D, S, f);
+        else
+            for (int k = 0; k < mlp::kIn; ++k) f[k] = 0.f;
+        const float z = mlp::logit16(f, lane, F);
+        if (lane < 16 && i < n) keep[i] = z >= logitCut ? 1 : 0;
+    }
+}
+
+// Training rows of the plain chain: features of every pair and its label (the
+// pair yields at least one tracklet), written at row offset `row0`.
+template <int V>
+__global__ void k_tl_pair_rows(Count cP, MDDev D, SGDev S, const long long* sgStart, const int2* pair,
+                               const long long* candOff, const unsigned char* flag, long long row0, float* X,
+                               float* Y) {
+    const long long n = cP.get();
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float f[mlp::kIn];
+    pairFeatures<V>(i, pair, sgStart, D, S, f);
+    for (int k = 0; k < mlp::kIn; ++k) X[(row0 + i) * mlp::kIn + k] = f[k];
+    bool any = false;
+    for (long long c = candOff[i]; c < candOff[i + 1] && !any; ++c) any = flag[c] != 0;
+    Y[row0 + i] = any ? 1.f : 0.f;
+}
+
+// First stage, one thread per pair: carry = (sdlCut, thetaMuls, sin of the
+// midpoint deltaPhi); nOut = number of outer SGs to try (0 if the pair failed).
+template <int V>
+__global__ void __launch_bounds__(kBlock) k_tl_stage1(Count cP, MDDev D, SGDev S, const long long* sgStart,
+                                                      const int2* pair, const unsigned char* keep, float4* carry,
+                                                      int* nOut) {
+    const long long n = cP.get();
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (keep && !keep[i]) {  // dropped by the learned proposal
+        nOut[i] = 0;
+        return;
+    }
+    const int2 pr = pair[i];
+    const int sgA = int(sgStart[pr.y]);
+    const TLInner I = loadInner(pr.x, D, S);
+    const TLOutLo L = loadOutLo(pr.y, S.flags[sgA], D);
+    TLCarry c{0.f, 0.f, 0.f};
+    const bool ok = tlStage1(V, I, L, c);
+    carry[i] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);
+    nOut[i] = ok ? int(sgStart[pr.y + 1]) - sgA : 0;
+}
+
+// Candidates (pair, outer SG) of the surviving pairs, one thread per pair.
+__global__ void k_tl_expand(Count cP, const long long* sgStart, const int2* pair, const int* nOut,
+                            const long long* candOff, int2* cand, Guard g) {
+    if (g.over()) return;
+    const long long n = cP.get();
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = nOut[i];
+    if (c == 0) return;
+    const int a = int(sgStart[pair[i].y]);
+    const long long d = candOff[i];
+    for (int q = 0; q < c; ++q) cand[d + q] = make_int2(int(i), a + q);
+}
+
+// Second stage, one thread per candidate; survivors are counted per inner SG.
+template <int V>
 __global__ void __launch_bounds__(kBlock) k_tl_stage2(Count cC, HitsDev H, MDDev D, SGDev S, const int2* cand,
-                                                      const int2* prRec, const float4* prCarry, unsigned char* flag) {
+                                                      const int2* pair, const float4* carry, unsigned char* flag,
+                                                      int* tlCount) {
     const long long n = cC.get();
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int2 c = cand[i];
-    const int2 r = prRec[c.x];
-    const float4 cf = prCarry[c.x];
+    const int2 pr = pair[c.x];
+    const float4 cf = carry[c.x];
     const int so = c.y;
     const int flo = S.flags[so];
-    const TLInner I = loadInner(r.y & kSgMask, D, S);
-    const TLOutLo L = loadOutLo(r.x, flo, D);
+    const TLInner I = loadInner(pr.x, D, S);
+    const TLOutLo L = loadOutLo(pr.y, flo, D);
     const TLOutUp U = loadOutUp(so, flo, H, D, S);
     const TLCarry cc{cf.x, cf.y, cf.z};
-    flag[i] = tlStage2(V, (flo & 2) != 0, I, L, U, cc) ? 1 : 0;
+    const bool pass = tlStage2(V, (flo & 2) != 0, I, L, U, cc);
+    flag[i] = pass ? 1 : 0;
+    if (pass) atomicAdd(&tlCount[pr.x], 1);
 }
 
-struct TLFlags {
-    const long long* off[kNVariants];
-    const unsigned char* flag[kNVariants];
-    const int2* cand[kNVariants];
-};
-
-__global__ void k_tl_count(Count cSG, TLFlags F, int* tlCount) {
-    const int nSG = int(cSG.get());
-    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (item >= nSG) return;
-    int c = 0;
-    for (int v = 0; v < kNVariants; ++v)
-        for (long long k = F.off[v][item] + lane; k < F.off[v][item + 1]; k += kWave) c += F.flag[v][k];
-    c = waveSum(c);
-    if (lane == 0) tlCount[item] = c;
-}
-
-__global__ void k_tl_scatter(Count cSG, TLFlags F, const long long* tlStart, int* tlIn, int* tlOutArr, Guard g) {
+// Survivors to their inner SG's range (the order inside a range is fixed
+// afterwards by sorting on the outer SG); four candidates per thread, whose
+// flags are read as one word (flag arrays are padded to a multiple of four).
+__global__ void k_tl_scatter(Count cC, const int2* cand, const int2* pair, const unsigned char* flag,
+                             const long long* tlStart, int* tlFill, int* tlIn, int* tlOutArr, Guard g) {
     if (g.over()) return;
-    const int nSG = int(cSG.get());
-    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (item >= nSG) return;
-    long long pos = tlStart[item];
-    for (int v = 0; v < kNVariants; ++v) {
-        const long long a = F.off[v][item], e = F.off[v][item + 1];
-        for (long long k0 = a; k0 < e; k0 += kWave) {
-            const long long k = k0 + lane;
-            const bool pass = k < e && F.flag[v][k];
-            const unsigned long long mask = __ballot(pass);
-            if (pass) {
-                const long long d = pos + __popcll(mask & ((1ull << lane) - 1ull));
-                tlIn[d] = item;
-                tlOutArr[d] = F.cand[v][k].y;
-            }
-            pos += __popcll(mask);
-        }
+    const long long n = cC.get();
+    const long long i0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i0 >= n) return;
+    const unsigned word = *reinterpret_cast<const unsigned*>(flag + i0);
+    if (word == 0) return;
+    for (int q = 0; q < 4; ++q) {
+        const long long i = i0 + q;
+        if (i >= n || !((word >> (8 * q)) & 0xffu)) continue;
+        const int2 c = cand[i];
+        const int s = pair[c.x].x;
+        const long long d = tlStart[s] + atomicAdd(&tlFill[s], 1);
+        tlIn[d] = s;
+        tlOutArr[d] = c.y;
     }
 }
 
@@ -1032,16 +1221,6 @@ __global__ void k_tpe_count(Count cSG, const int* sgEndList, const long long* sg
     if (k >= nSG) return;
     const int mx = sgIn[sgEndList[k]];
     cnt[k] = int(sgEndStart[mx + 1] - sgEndStart[mx]);
-}
-// Largest k in [0, n) with off[k] <= j, for off[0] <= j < off[n].
-__device__ __forceinline__ long long rangeOf(const long long* off, long long n, long long j) {
-    long long lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const long long mid = (lo + hi) >> 1;
-        if (off[mid] <= j) lo = mid;
-        else hi = mid;
-    }
-    return lo;
 }
 // One thread per listed TP (nTP of them in total).
 __global__ void k_tpe_write(Count cTP, Count cSG, const long long* tpeOff, const int* sgEndList,
@@ -1118,14 +1297,14 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(Count cn, const T* vals, 
             if (bins[b]) atomicAdd(&counters[((b / kNBins) * kNCounters + which) * kNBins + (b % kNBins)], bins[b]);
 }
 
-// TCs from triplets, one thread per TC (grid-stride).  Each TC counts at the
-// module of its inner TP (Event.cc:893-897,965) in a per-block LDS histogram.
-__global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTC, Count cTL, const long long* start, const int2* info,
+// TCs from triplets, one wave per tracklet (grid-stride over tracklets): the
+// lanes copy the tracklet's inner TPs.  Each TC counts at the module of its
+// inner TP (Event.cc:893-897,965) in a per-block LDS histogram.
+__global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long* start, const int2* info,
                                                       const int* tpEndList, const int* tpIn, const int* sgIn,
                                                       const int* mdSlot, int NM, int E, const ModDev* mods, int2* tc,
                                                       unsigned long long* counters, Guard g) {
     if (g.over()) return;
-    const long long nTC = cTC.get();
     const int nTL = int(cTL.get());
     __shared__ unsigned long long bins[kMaxLdsBins];
     const int nb = E * kNBins;
@@ -1133,15 +1312,20 @@ __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTC, Count cTL, cons
     if (lds)
         for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0;
     __syncthreads();
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nTC; i += (long long)gridDim.x * blockDim.x) {
-        const long long l = rangeOf(start, nTL, i);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w0 = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6), nw = gridDim.x * kWavesPerBlock;
+    for (int l = w0; l < nTL; l += nw) {
+        const long long a = start[l], n = start[l + 1] - a;
+        if (n == 0) continue;
         const int2 f = info[l];
-        const int t = tpEndList[f.x + (i - start[l])];
-        tc[i] = make_int2(t, f.y);
-        const int sl = mdSlot[sgIn[tpIn[t]]];
-        const int e = sl / NM, bin = layerBin(mods[sl % NM]);
-        if (lds) atomicAdd(&bins[e * kNBins + bin], 1ULL);
-        else atomicAdd(&counters[(e * kNCounters + 9) * kNBins + bin], 1ULL);
+        for (long long q = lane; q < n; q += kWave) {
+            const int t = tpEndList[f.x + q];
+            tc[a + q] = make_int2(t, f.y);
+            const int sl = mdSlot[sgIn[tpIn[t]]];
+            const int e = sl / NM, bin = layerBin(mods[sl % NM]);
+            if (lds) atomicAdd(&bins[e * kNBins + bin], 1ULL);
+            else atomicAdd(&counters[(e * kNCounters + 9) * kNBins + bin], 1ULL);
+        }
     }
     __syncthreads();
     if (lds)
@@ -1321,6 +1505,18 @@ struct sdl_geom {
     int *dConnStart = nullptr, *dConnList = nullptr;
 };
 
+struct sdl_mlp {
+    int device = 0;
+    hipStream_t st = nullptr;
+    float* dParams = nullptr;   // float32 master copy
+    float* dGrad = nullptr;
+    float* dM = nullptr;        // Adam moments
+    float* dV = nullptr;
+    float* dLoss = nullptr;
+    sdl::mlp::Packed* dPacked = nullptr;  // BF16 fragments for inference
+    long long adamSteps = 0;
+};
+
 struct sdl_event {
     sdl_geom* g = nullptr;
     hipStream_t st = nullptr;
@@ -1348,6 +1544,19 @@ struct sdl_event {
     bool haveHistory = false;
     long long lastTot[TOT_N] = {};
     long long lastNH = 0;
+    // captured chain (speculative mode)
+    bool useGraphs = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    struct {
+        int E = -1;
+        const void *px = nullptr, *py = nullptr, *pz = nullptr, *pidx = nullptr, *pdet = nullptr;
+        unsigned long long allocEpoch = 0;
+        int headroomFrom = 0;
+        double headroomFactor = 0;
+        const void* proposal = nullptr;
+        float proposalLogit = 0.f;
+    } graphKey;
     // prediction headroom: factor for the totals of stages >= headroomFrom
     int headroomFrom = SDL_STAGE_TCL + 1;
     double headroomFactor = 1.25;
@@ -1364,9 +1573,13 @@ struct sdl_event {
     DBuf<int> mdCap, mdCount;
     DBuf<long long> mdCapOff, mdStart;
     DBuf<int2> mdSlabHits;
+    DBuf<unsigned char> mdFlag;
+    DBuf<int> mdSlabSlot;
+    DBuf<long long> mdPos;
     DBuf<float4> mdSlabF;
     DBuf<int> mdLo, mdUp, mdAnchor, mdSlot;
     DBuf<float4> mdA, mdB, mdP;
+    DBuf<double> mdQ;
     DBuf<float2> mdC;
     DBuf<int> sgCap, sgCount, sgSlabOut, sgIn, sgOut;
     DBuf<long long> sgCapOff, sgStart, slotSgStart;
@@ -1381,12 +1594,19 @@ struct sdl_event {
     DBuf<long long> tpStart, slotTpStart;
     DBuf<long long> tlCap, tlCapOff, tlStart, slotTlStart;
     DBuf<int> tlCount, tlIn, tlOut;
-    DBuf<int2> prRec;
-    DBuf<float4> prCarry;
-    DBuf<int> prCount, tlNC;
-    DBuf<long long> tlCOff[3];
-    DBuf<int2> tlCand[3];
-    DBuf<unsigned char> tlFlag[3];
+    DBuf<int> slotMdSg, slotMdSgList, tlNP, tlFill;
+    DBuf<long long> slotMdSgStart;
+    DBuf<long long> tlPOff[kNVariantsTL];
+    DBuf<int2> tlPair[kNVariantsTL];
+    DBuf<float4> tlCarry[kNVariantsTL];
+    DBuf<int> tlNOut[kNVariantsTL];
+    DBuf<long long> tlCOff[kNVariantsTL];
+    DBuf<int2> tlCand[kNVariantsTL];
+    DBuf<unsigned char> tlFlag[kNVariantsTL];
+    DBuf<unsigned char> tlKeep[kNVariantsTL];
+    // learned proposal (null: the plain chain)
+    sdl_mlp* proposal = nullptr;
+    float proposalLogit = 0.f;
     DBuf<int> tcCount;
     DBuf<int2> tcpInfo;
     DBuf<long long> tcpStart, tclStart;
@@ -1531,7 +1751,8 @@ HitsDev hitsView(sdl_event* ev) {
     return HitsDev{ev->px, ev->py, ev->pz, ev->rt.p, ev->r3.p, ev->phi.p, ev->halpha.p, ev->pidx, ev->eHi.p, ev->eLo.p};
 }
 MDDev mdView(sdl_event* ev) {
-    return MDDev{ev->mdLo.p, ev->mdUp.p, ev->mdAnchor.p, ev->mdSlot.p, ev->mdA.p, ev->mdB.p, ev->mdC.p, ev->mdP.p};
+    return MDDev{ev->mdLo.p, ev->mdUp.p, ev->mdAnchor.p, ev->mdSlot.p, ev->mdA.p, ev->mdB.p, ev->mdC.p, ev->mdP.p,
+                 ev->mdQ.p};
 }
 SGDev sgView(sdl_event* ev) {
     return SGDev{ev->sgIn.p, ev->sgOut.p, ev->sgF.p, ev->sgX.p, ev->sgE.p, ev->sgFlags.p};
@@ -1551,7 +1772,8 @@ long long sizeOf(sdl_event* ev, int k) {
     } else {
         static const int stageOf[TOT_N] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                            SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
-                                           SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, 0, 0};
+                                           SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL,
+                                           SDL_STAGE_TL, 0, 0};
         const double scale = ev->lastNH > 0 ? double(ev->NH) / double(ev->lastNH) : 1.0;
         const bool custom = stageOf[k] >= ev->headroomFrom;
         const double f = custom ? ev->headroomFactor : 1.25;
@@ -1571,33 +1793,42 @@ void accumulate(sdl_event* ev, Count cn, long long nMax, const T* vals, const in
 
 // Hit ingestion: module lookup, derived quantities, counting sort by slot.
 // Hits on modules unknown to the geometry are counted (TOT_UNKNOWN) and left out.
+template <class T>
+void zeroAsync(T* p, long long n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_zero<T>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0, st, p, n);
+}
+
+// Uploads the batch's event offsets (outside any captured graph: the host array
+// is read when the copy runs).
+int uploadOffsets(sdl_event* ev) {
+    if (int r = ev->dEvOff.ensure(ev->E + 1)) return r;
+    HIP_OK(hipMemcpyAsync(ev->dEvOff.p, ev->evOff.data(), sizeof(long long) * (ev->E + 1), hipMemcpyHostToDevice, ev->st));
+    return 0;
+}
+
 int runPrep(sdl_event* ev) {
     sdl_geom* G = ev->g;
     hipStream_t st = ev->st;
     const int NS = ev->E * G->NM;
     if (int r = ensureHits(ev)) return r;
-    HIP_OK(hipMemsetAsync(ev->tot.p, 0, sizeof(long long) * TOT_N, st));
-    HIP_OK(hipMemsetAsync(ev->slotHitCount.p, 0, sizeof(int) * NS, st));
-    HIP_OK(hipMemsetAsync(ev->slotFill.p, 0, sizeof(int) * NS, st));
-    HIP_OK(hipMemsetAsync(ev->counters.p, 0, sizeof(unsigned long long) * ev->E * kNCounters * kNBins, st));
-    if (int r = ev->dEvOff.ensure(ev->E + 1)) return r;
-    HIP_OK(hipMemcpyAsync(ev->dEvOff.p, ev->evOff.data(), sizeof(long long) * (ev->E + 1), hipMemcpyHostToDevice, st));
-    const int nh = int(ev->NH);
+    zeroAsync(ev->tot.p, TOT_N, st);
+    zeroAsync(ev->slotHitCount.p, NS, st);
+    zeroAsync(ev->slotFill.p, NS, st);
+    zeroAsync(ev->counters.p, (long long)ev->E * kNCounters * kNBins, st);
+    // launches cover the hit buffers' capacity; the kernels read the hit count
+    const long long nhGrid = (long long)ev->rt.cap;
     timerBegin(ev, T_PREP);
-    if (nh > 0)
-        hipLaunchKernelGGL(k_hit_prep, dim3(gridFor(nh, kBlock)), dim3(kBlock), 0, st, nh, ev->px, ev->py, ev->pz,
-                           ev->pdet, ev->dEvOff.p, ev->E, G->dDet, G->NM, G->dMods, ev->rt.p, ev->r3.p, ev->phi.p,
-                           ev->halpha.p, ev->hslot.p, ev->eHi.p, ev->eLo.p, ev->slotHitCount.p, ev->tot.p);
+    hipLaunchKernelGGL(k_hit_prep, dim3(gridFor(nhGrid, kBlock)), dim3(kBlock), 0, st, ev->px, ev->py, ev->pz, ev->pdet,
+                       ev->dEvOff.p, ev->E, G->dDet, G->NM, G->dMods, ev->rt.p, ev->r3.p, ev->phi.p, ev->halpha.p,
+                       ev->hslot.p, ev->eHi.p, ev->eLo.p, ev->slotHitCount.p, ev->tot.p);
     timerEnd(ev, T_PREP);
     if (int r = scanExclusive<int>(ev->slotHitCount.p, ev->slotHitStart.p, hostCount(ev, NS), NS, ev->scan, st,
                                    ev->tot.p, -1))
         return r;
-    if (nh > 0) {
-        hipLaunchKernelGGL(k_hit_scatter, dim3(gridFor(nh, kBlock)), dim3(kBlock), 0, st, nh, ev->hslot.p,
-                           ev->slotHitStart.p, ev->slotFill.p, ev->sortedHit.p);
-        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st,
-                           hostCount(ev, NS), ev->slotHitStart.p, ev->sortedHit.p);
-    }
+    hipLaunchKernelGGL(k_hit_scatter, dim3(gridFor(nhGrid, kBlock)), dim3(kBlock), 0, st, ev->dEvOff.p, ev->E,
+                       ev->hslot.p, ev->slotHitStart.p, ev->slotFill.p, ev->sortedHit.p);
+    hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+                       ev->slotHitStart.p, ev->sortedHit.p);
     ev->hitsReady = true;
     return 0;
 }
@@ -1619,11 +1850,20 @@ int runMD(sdl_event* ev) {
     if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
     if (int r = ev->mdSlabHits.ensure(slab)) return r;
     if (int r = ev->mdSlabF.ensure(slab)) return r;
+    if (int r = ev->mdFlag.ensure(std::max(slab, 1LL))) return r;
+    if (int r = ev->mdSlabSlot.ensure(std::max(slab, 1LL))) return r;
+    if (int r = ev->mdPos.ensure(std::max(slab, 1LL) + 1)) return r;
     HitsDev H = hitsView(ev);
+    zeroAsync(ev->mdCount.p, NS, st);
     timerBegin(ev, T_MD);
-    hipLaunchKernelGGL(k_md, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods, H,
-                       ev->slotHitStart.p, ev->sortedHit.p, ev->mdCap.p, ev->mdCapOff.p, ev->mdSlabHits.p,
-                       ev->mdSlabF.p, ev->mdCount.p, guardOf(ev, TOT_MDCAP, slab, SDL_STAGE_MD));
+    if (slab > 0)
+        hipLaunchKernelGGL(k_md, dim3(gridFor(slab, kBlock)), dim3(kBlock), 0, st, devCount(ev, TOT_MDCAP), NS, G->NM,
+                           G->dMods, H, ev->slotHitStart.p, ev->sortedHit.p, ev->mdCapOff.p, ev->mdSlabHits.p,
+                           ev->mdSlabF.p, ev->mdSlabSlot.p, ev->mdFlag.p, ev->mdCount.p,
+                           guardOf(ev, TOT_MDCAP, slab, SDL_STAGE_MD));
+    if (int r = scanExclusive<unsigned char>(ev->mdFlag.p, ev->mdPos.p, devCount(ev, TOT_MDCAP), slab, ev->scan, st,
+                                             ev->tot.p, -1))
+        return r;
     timerEnd(ev, T_MD);
     if (int r = scanExclusive<int>(ev->mdCount.p, ev->mdStart.p, hostCount(ev, NS), NS, ev->scan, st, ev->tot.p, TOT_MD))
         return r;
@@ -1636,9 +1876,11 @@ int runMD(sdl_event* ev) {
     for (DBuf<float4>* b : {&ev->mdA, &ev->mdB, &ev->mdP})
         if (int r = b->ensure(n)) return r;
     if (int r = ev->mdC.ensure(n)) return r;
-    hipLaunchKernelGGL(k_md_repack, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods, H,
-                       ev->mdCapOff.p, ev->mdCount.p, ev->mdStart.p, ev->mdSlabHits.p, ev->mdSlabF.p, mdView(ev),
-                       guardOf(ev, TOT_MD, nMD, SDL_STAGE_MD));
+    if (int r = ev->mdQ.ensure(n)) return r;
+    if (slab > 0)
+        hipLaunchKernelGGL(k_md_repack, dim3(gridFor(slab, kBlock)), dim3(kBlock), 0, st, devCount(ev, TOT_MDCAP),
+                           G->dMods, G->NM, H, ev->mdFlag.p, ev->mdPos.p, ev->mdSlabHits.p, ev->mdSlabF.p,
+                           ev->mdSlabSlot.p, mdView(ev), guardOf(ev, TOT_MD, nMD, SDL_STAGE_MD));
     accumulate<int>(ev, hostCount(ev, NS), NS, ev->mdCap.p, nullptr, 0);
     accumulate<int>(ev, hostCount(ev, NS), NS, ev->mdCount.p, nullptr, 5);
     return 0;
@@ -1692,8 +1934,8 @@ int runSG(sdl_event* ev) {
     hipLaunchKernelGGL(k_slot_range, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->mdStart.p, ev->sgStart.p, ev->slotSgStart.p);
     // reverse index: SGs by outer MD
-    HIP_OK(hipMemsetAsync(ev->sgEndCount.p, 0, sizeof(int) * nm, st));
-    HIP_OK(hipMemsetAsync(ev->sgEndFill.p, 0, sizeof(int) * nm, st));
+    zeroAsync(ev->sgEndCount.p, nm, st);
+    zeroAsync(ev->sgEndFill.p, nm, st);
     if (nSG > 0)
         hipLaunchKernelGGL(k_count_by, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
                            ev->sgEndCount.p);
@@ -1761,77 +2003,133 @@ int runTL(sdl_event* ev) {
     if (int r = ev->tlCap.ensure(ns)) return r;
     if (int r = ev->tlCapOff.ensure(ns + 1)) return r;
     if (int r = ev->tlCount.ensure(ns)) return r;
+    if (int r = ev->tlFill.ensure(ns)) return r;
     if (int r = ev->tlStart.ensure(ns + 1)) return r;
     if (int r = ev->slotTlStart.ensure(NS + 1)) return r;
-    if (int r = ev->prCount.ensure(ns)) return r;
-    if (int r = ev->tlNC.ensure(3 * ns)) return r;
-    for (int v = 0; v < kNVariants; ++v)
-        if (int r = ev->tlCOff[v].ensure(ns + 1)) return r;
+    if (int r = ev->slotMdSg.ensure(NS)) return r;
+    if (int r = ev->tlNP.ensure(kNVariantsTL * ns)) return r;
+    for (int v = 0; v < kNVariantsTL; ++v)
+        if (int r = ev->tlPOff[v].ensure(ns + 1)) return r;
     const Count cSG = devCount(ev, TOT_SG);
+    timerBegin(ev, T_TL);
+    hipLaunchKernelGGL(k_slot_md_sg, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS), ev->mdStart.p,
+                       ev->sgStart.p, ev->slotMdSg.p);
+    if (int r = ev->slotMdSgStart.ensure(NS + 1)) return r;
+    if (int r = ev->slotMdSgList.ensure(std::max(ev->nMD, 1LL))) return r;
+    if (int r = scanExclusive<int>(ev->slotMdSg.p, ev->slotMdSgStart.p, hostCount(ev, NS), NS, ev->scan, st, ev->tot.p,
+                                   -1))
+        return r;
+    hipLaunchKernelGGL(k_slot_md_sg_list, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+                       ev->mdStart.p, ev->sgStart.p, ev->slotMdSgStart.p, ev->slotMdSgList.p);
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, ev->sgOut.p,
-                           ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->tlCap.p);
+        hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
+                           ev->sgOut.p, ev->sgFlags.p, ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p,
+                           ev->slotMdSg.p, ev->tlCap.p, ev->tlNP.p, ns);
     if (int r = scanExclusive<long long>(ev->tlCap.p, ev->tlCapOff.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TLCAP))
         return r;
-    const long long slab = sizeOf(ev, TOT_TLCAP);
-    if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
-    if (int r = ev->prRec.ensure(std::max(slab, 1LL))) return r;
-    if (int r = ev->prCarry.ensure(std::max(slab, 1LL))) return r;
-    timerBegin(ev, T_TL);
-    if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_pairs, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM,
-                           mdView(ev), sgView(ev), G->dConnStart, G->dConnList, ev->mdStart.p, ev->mdCount.p,
-                           ev->sgStart.p, ev->tlCap.p, ev->tlCapOff.p, ev->prRec.p, ev->prCarry.p, ev->prCount.p,
-                           ev->tlNC.p, ns, guardOf(ev, TOT_TLCAP, slab, SDL_STAGE_TL));
-    long long nc[kNVariants];
-    for (int v = 0; v < kNVariants; ++v)
-        if (int r = scanExclusive<int>(ev->tlNC.p + v * ns, ev->tlCOff[v].p, cSG, nSG, ev->scan, st, ev->tot.p,
-                                       TOT_TLC0 + v))
+    // pairs per variant
+    long long np[kNVariantsTL];
+    TLPairs P;
+    for (int v = 0; v < kNVariantsTL; ++v)
+        if (int r = scanExclusive<int>(ev->tlNP.p + v * ns, ev->tlPOff[v].p, cSG, nSG, ev->scan, st, ev->tot.p,
+                                       TOT_TLP0 + v))
             return r;
-    for (int v = 0; v < kNVariants; ++v) {
+    for (int v = 0; v < kNVariantsTL; ++v) {
+        np[v] = sizeOf(ev, TOT_TLP0 + v);
+        if (np[v] < 0) return fail(SDL_ERR_HIP, "readback failed");
+        const long long n = std::max(np[v], 1LL);
+        if (int r = ev->tlPair[v].ensure(n)) return r;
+        if (int r = ev->tlCarry[v].ensure(n)) return r;
+        if (int r = ev->tlNOut[v].ensure(n)) return r;
+        if (int r = ev->tlCOff[v].ensure(n + 1)) return r;
+        P.off[v] = ev->tlPOff[v].p;
+        P.pair[v] = ev->tlPair[v].p;
+    }
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tl_enum, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
+                           ev->sgOut.p, ev->sgFlags.p, ev->mdSlot.p, G->dConnStart, G->dConnList,
+                           ev->slotMdSgStart.p, ev->slotMdSgList.p, P, guardOf(ev, TOT_TLP0, np[0], SDL_STAGE_TL),
+                           guardOf(ev, TOT_TLP1, np[1], SDL_STAGE_TL), guardOf(ev, TOT_TLP2, np[2], SDL_STAGE_TL));
+    // first stage per pair, after the learned proposal when one is set
+    const MDDev D = mdView(ev);
+    const SGDev S = sgView(ev);
+    const bool learned = ev->proposal != nullptr;
+    for (int v = 0; v < kNVariantsTL; ++v) {
+        if (learned)
+            if (int r = ev->tlKeep[v].ensure(std::max(np[v], 1LL))) return r;
+        if (np[v] == 0) continue;
+        const Count cP = devCount(ev, TOT_TLP0 + v);
+        const unsigned char* keep = learned ? ev->tlKeep[v].p : nullptr;
+        if (learned) {
+            const unsigned grid = std::min<unsigned>(gridFor((np[v] + 15) / 16, kWavesPerBlock), 8192u);
+            if (v == 0)
+                hipLaunchKernelGGL(k_tl_score<TL_BBBB>, dim3(grid), dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
+                                   ev->tlPair[0].p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep[0].p);
+            else if (v == 1)
+                hipLaunchKernelGGL(k_tl_score<TL_BBEE>, dim3(grid), dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
+                                   ev->tlPair[1].p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep[1].p);
+            else
+                hipLaunchKernelGGL(k_tl_score<TL_EEEE>, dim3(grid), dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
+                                   ev->tlPair[2].p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep[2].p);
+        }
+        const dim3 grid(gridFor(np[v], kBlock));
+        if (v == 0)
+            hipLaunchKernelGGL(k_tl_stage1<TL_BBBB>, grid, dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
+                               ev->tlPair[0].p, keep, ev->tlCarry[0].p, ev->tlNOut[0].p);
+        else if (v == 1)
+            hipLaunchKernelGGL(k_tl_stage1<TL_BBEE>, grid, dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
+                               ev->tlPair[1].p, keep, ev->tlCarry[1].p, ev->tlNOut[1].p);
+        else
+            hipLaunchKernelGGL(k_tl_stage1<TL_EEEE>, grid, dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
+                               ev->tlPair[2].p, keep, ev->tlCarry[2].p, ev->tlNOut[2].p);
+    }
+    // candidates of the surviving pairs
+    long long nc[kNVariantsTL];
+    for (int v = 0; v < kNVariantsTL; ++v)
+        if (int r = scanExclusive<int>(ev->tlNOut[v].p, ev->tlCOff[v].p, devCount(ev, TOT_TLP0 + v), np[v], ev->scan,
+                                       st, ev->tot.p, TOT_TLC0 + v))
+            return r;
+    for (int v = 0; v < kNVariantsTL; ++v) {
         nc[v] = sizeOf(ev, TOT_TLC0 + v);
         if (nc[v] < 0) return fail(SDL_ERR_HIP, "readback failed");
         if (int r = ev->tlCand[v].ensure(std::max(nc[v], 1LL))) return r;
-        if (int r = ev->tlFlag[v].ensure(std::max(nc[v], 1LL))) return r;
+        if (int r = ev->tlFlag[v].ensure((std::max(nc[v], 1LL) + 3) / 4 * 4)) return r;
+        if (np[v] > 0)
+            hipLaunchKernelGGL(k_tl_expand, dim3(gridFor(np[v], kBlock)), dim3(kBlock), 0, st,
+                               devCount(ev, TOT_TLP0 + v), ev->sgStart.p, ev->tlPair[v].p, ev->tlNOut[v].p,
+                               ev->tlCOff[v].p, ev->tlCand[v].p, guardOf(ev, TOT_TLC0 + v, nc[v], SDL_STAGE_TL));
     }
-    TLLists Ls;
-    TLFlags F;
-    for (int v = 0; v < kNVariants; ++v) {
-        Ls.off[v] = F.off[v] = ev->tlCOff[v].p;
-        Ls.cand[v] = ev->tlCand[v].p;
-        F.cand[v] = ev->tlCand[v].p;
-        F.flag[v] = ev->tlFlag[v].p;
-    }
-    if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_expand, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, ev->sgStart.p,
-                           ev->tlCapOff.p, ev->prRec.p, ev->prCount.p, Ls,
-                           guardOf(ev, TOT_TLC0, nc[0], SDL_STAGE_TL), guardOf(ev, TOT_TLC1, nc[1], SDL_STAGE_TL),
-                           guardOf(ev, TOT_TLC2, nc[2], SDL_STAGE_TL));
+    // second stage per candidate
+    zeroAsync(ev->tlCount.p, ns, st);
+    zeroAsync(ev->tlFill.p, ns, st);
     const HitsDev H = hitsView(ev);
     if (nc[0] > 0)
         hipLaunchKernelGGL(k_tl_stage2<TL_BBBB>, dim3(gridFor(nc[0], kBlock)), dim3(kBlock), 0, st,
-                           devCount(ev, TOT_TLC0), H, mdView(ev), sgView(ev), ev->tlCand[0].p, ev->prRec.p,
-                           ev->prCarry.p, ev->tlFlag[0].p);
+                           devCount(ev, TOT_TLC0), H, D, S, ev->tlCand[0].p, ev->tlPair[0].p, ev->tlCarry[0].p,
+                           ev->tlFlag[0].p, ev->tlCount.p);
     if (nc[1] > 0)
         hipLaunchKernelGGL(k_tl_stage2<TL_BBEE>, dim3(gridFor(nc[1], kBlock)), dim3(kBlock), 0, st,
-                           devCount(ev, TOT_TLC1), H, mdView(ev), sgView(ev), ev->tlCand[1].p, ev->prRec.p,
-                           ev->prCarry.p, ev->tlFlag[1].p);
+                           devCount(ev, TOT_TLC1), H, D, S, ev->tlCand[1].p, ev->tlPair[1].p, ev->tlCarry[1].p,
+                           ev->tlFlag[1].p, ev->tlCount.p);
     if (nc[2] > 0)
         hipLaunchKernelGGL(k_tl_stage2<TL_EEEE>, dim3(gridFor(nc[2], kBlock)), dim3(kBlock), 0, st,
-                           devCount(ev, TOT_TLC2), H, mdView(ev), sgView(ev), ev->tlCand[2].p, ev->prRec.p,
-                           ev->prCarry.p, ev->tlFlag[2].p);
-    if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_count, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, F,
-                           ev->tlCount.p);
+                           devCount(ev, TOT_TLC2), H, D, S, ev->tlCand[2].p, ev->tlPair[2].p, ev->tlCarry[2].p,
+                           ev->tlFlag[2].p, ev->tlCount.p);
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
     const long long nTL = sizeOf(ev, TOT_TL);
     if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
     ev->nTL = nTL;
     if (int r = ev->tlIn.ensure(std::max(nTL, 1LL))) return r;
     if (int r = ev->tlOut.ensure(std::max(nTL, 1LL))) return r;
+    for (int v = 0; v < kNVariantsTL; ++v)
+        if (nc[v] > 0)
+            hipLaunchKernelGGL(k_tl_scatter, dim3(gridFor((nc[v] + 3) / 4, kBlock)), dim3(kBlock), 0, st,
+                               devCount(ev, TOT_TLC0 + v), ev->tlCand[v].p, ev->tlPair[v].p, ev->tlFlag[v].p,
+                               ev->tlStart.p, ev->tlFill.p, ev->tlIn.p, ev->tlOut.p,
+                               guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_scatter, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, F,
-                           ev->tlStart.p, ev->tlIn.p, ev->tlOut.p, guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
+        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG,
+                           ev->tlStart.p, ev->tlOut.p);
     timerEnd(ev, T_TL);
     hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->slotSgStart.p, ev->tlStart.p, ev->slotTlStart.p);
@@ -1875,10 +2173,9 @@ int runTCP(sdl_event* ev) {
     ev->nTCP = nTCP;
     if (int r = ev->tcp.ensure(std::max(nTCP, 1LL))) return r;
     if (nTCP > 0)
-        hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(nTCP, kBlock), 4096u)), dim3(kBlock), 0, st,
-                           devCount(ev, TOT_TCP), cTL, ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p,
-                           ev->sgIn.p, ev->mdSlot.p, G->NM, ev->E, G->dMods, ev->tcp.p, ev->counters.p,
-                           guardOf(ev, TOT_TCP, nTCP, SDL_STAGE_TCP));
+        hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(nTL, kWavesPerBlock), 4096u)), dim3(kBlock), 0, st, cTL,
+                           ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p, ev->sgIn.p, ev->mdSlot.p, G->NM,
+                           ev->E, G->dMods, ev->tcp.p, ev->counters.p, guardOf(ev, TOT_TCP, nTCP, SDL_STAGE_TCP));
     if (int r = ensureItemScratch(ev, nTP)) return r;
     if (nTP > 0) {
         hipLaunchKernelGGL(k_tc_cand_tp, dim3(gridFor(nTP, kBlock)), dim3(kBlock), 0, st, cTP, G->NM, ev->tpIn.p,
@@ -1932,20 +2229,91 @@ int runStage(sdl_event* ev, int stage) {
     }
 }
 
+// Enqueues hit preparation (if pending) and stages s0..s1.
+int enqueueStages(sdl_event* ev, int s0, int s1) {
+    if (!ev->hitsReady)
+        if (int r = runPrep(ev)) return r;
+    for (int s = s0; s <= s1; ++s)
+        if (int r = runStage(ev, s)) return r;
+    return 0;
+}
+
+bool sameGraphKey(const sdl_event* ev) {
+    const auto& k = ev->graphKey;
+    return k.E == ev->E && k.px == ev->px && k.py == ev->py && k.pz == ev->pz && k.pidx == ev->pidx &&
+           k.pdet == ev->pdet && k.allocEpoch == g_allocEpoch && k.headroomFrom == ev->headroomFrom &&
+           k.headroomFactor == ev->headroomFactor && k.proposal == ev->proposal &&
+           k.proposalLogit == ev->proposalLogit;
+}
+
+void dropGraph(sdl_event* ev) {
+    if (ev->graphExec) (void)hipGraphExecDestroy(ev->graphExec);
+    if (ev->graph) (void)hipGraphDestroy(ev->graph);
+    ev->graphExec = nullptr;
+    ev->graph = nullptr;
+}
+
+// Speculative whole chain as a HIP graph: replayed while the batch shape, the
+// input pointers and every buffer are unchanged, captured again otherwise.
+// Returns 1 if the chain was launched, 0 if it must run uncaptured.
+int launchChainGraph(sdl_event* ev) {
+    hipStream_t st = ev->st;
+    if (ev->graphExec && sameGraphKey(ev)) {
+        HIP_OK(hipGraphLaunch(ev->graphExec, st));
+        ev->hitsReady = true;
+        return 1;
+    }
+    dropGraph(ev);
+    HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    g_capturing = true;
+    g_captureBlocked = false;
+    const int r = enqueueStages(ev, SDL_STAGE_MD, SDL_STAGE_TCL);
+    g_capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    ev->hitsReady = false;
+    if (r != 0 || e != hipSuccess || !graph || g_captureBlocked) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (hipGraphInstantiate(&ev->graphExec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        ev->graphExec = nullptr;
+        (void)hipGetLastError();
+        return 0;
+    }
+    ev->graph = graph;
+    auto& k = ev->graphKey;
+    k.E = ev->E; k.px = ev->px; k.py = ev->py; k.pz = ev->pz; k.pidx = ev->pidx; k.pdet = ev->pdet;
+    k.allocEpoch = g_allocEpoch; k.headroomFrom = ev->headroomFrom; k.headroomFactor = ev->headroomFactor;
+    k.proposal = ev->proposal; k.proposalLogit = ev->proposalLogit;
+    HIP_OK(hipGraphLaunch(ev->graphExec, st));
+    ev->hitsReady = true;
+    return 1;
+}
+
 // Runs stages s0..s1 of the batch with one host round trip at the end when the
-// last call's totals predict the buffer sizes; otherwise (first call, or a
-// prediction that fell short) each stage reads back the totals it sizes from.
+// last call's totals predict the buffer sizes (as a replayed graph for a whole
+// chain); otherwise (first call, or a prediction that fell short) each stage
+// reads back the totals it sizes from.
 int runStages(sdl_event* ev, int s0, int s1) {
     hipStream_t st = ev->st;
     if (int r = ev->tot.ensure(TOT_N)) return r;
-    ev->spec = ev->haveHistory;
     if (!ev->hitsReady) {
-        if (int r = runPrep(ev)) return r;
+        if (int r = uploadOffsets(ev)) return r;
     } else {
-        HIP_OK(hipMemsetAsync(ev->tot.p + TOT_ABORT, 0, sizeof(long long), st));
+        zeroAsync(ev->tot.p + TOT_ABORT, 1, st);
     }
-    for (int s = s0; s <= s1; ++s)
-        if (int r = runStage(ev, s)) return r;
+    ev->spec = ev->haveHistory;
+    bool launched = false;
+    if (ev->spec && ev->useGraphs && !ev->timing && !ev->hitsReady && s0 == SDL_STAGE_MD && s1 == SDL_STAGE_TCL) {
+        const int r = launchChainGraph(ev);
+        if (r < 0) return r;
+        launched = r == 1;
+    }
+    if (!launched)
+        if (int r = enqueueStages(ev, s0, s1)) return r;
     long long t[TOT_N];
     HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -1956,7 +2324,7 @@ int runStages(sdl_event* ev, int s0, int s1) {
         // rerun from the stage that overflowed, sized exactly
         const int from = t[TOT_ABORT] > SDL_STAGE_TCL + 1 ? s0 : int(t[TOT_ABORT] - 1);
         ev->spec = false;
-        HIP_OK(hipMemsetAsync(ev->tot.p + TOT_ABORT, 0, sizeof(long long), st));
+        zeroAsync(ev->tot.p + TOT_ABORT, 1, st);
         for (int s = std::max(from, s0); s <= s1; ++s)
             if (int r = runStage(ev, s)) return r;
         HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, st));
@@ -1968,7 +2336,8 @@ int runStages(sdl_event* ev, int s0, int s1) {
     // remember the sizes of the stages that ran (the prediction for the next call)
     static const int stageOfTot[TOT_ABORT] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                               SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
-                                              SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL};
+                                              SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL,
+                                              SDL_STAGE_TL};
     for (int k = 0; k < TOT_ABORT; ++k)
         if (stageOfTot[k] >= s0 && stageOfTot[k] <= s1) ev->lastTot[k] = t[k];
     if (s1 == SDL_STAGE_TCL) {
@@ -2028,6 +2397,54 @@ int beginBatch(sdl_event* ev) {
     return 0;
 }
 
+// float -> bf16 bits, round to nearest even (NaN kept quiet)
+short bf16Host(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return short((u >> 16) | 0x40u);
+    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+    return short(r >> 16);
+}
+
+// Parameters (layout of sdl_mlp.h) -> MFMA operand fragments in lane order.
+void packMlp(const float* P, sdl::mlp::Packed& K) {
+    using namespace sdl::mlp;
+    for (int t = 0; t < kTiles; ++t)
+        for (int l = 0; l < 64; ++l) {
+            const int row = t * 16 + (l % 16);  // A operand row: output neuron
+            for (int v = 0; v < 4; ++v) {
+                const int kq = 4 * (l / 16) + v;  // A operand column within a K step
+                for (int ks = 0; ks < 2; ++ks) {
+                    const int k = ks * 16 + kq;
+                    const float w = k < kIn ? P[kW1 + row * kIn + k] : (k == kIn ? P[kB1 + row] : 0.f);
+                    K.a1[t][ks][l][v] = bf16Host(w);
+                }
+                for (int ks = 0; ks < kTiles; ++ks) {
+                    K.a2[t][ks][l][v] = bf16Host(P[kW2 + row * kH + ks * 16 + kq]);
+                    K.a3[t][ks][l][v] = bf16Host(P[kW3 + row * kH + ks * 16 + kq]);
+                }
+                const int acc = t * 16 + 4 * (l / 16) + v;  // accumulator row of the lane
+                K.c2[t][l][v] = P[kB2 + acc];
+                K.c3[t][l][v] = P[kB3 + acc];
+                K.w4[t][l][v] = P[kW4 + acc];
+            }
+        }
+    K.b4 = P[kB4];
+}
+
+int repackMlp(sdl_mlp* m) {
+    std::vector<float> p(sdl::mlp::kNParams);
+    HIP_OK(hipMemcpyAsync(p.data(), m->dParams, sizeof(float) * p.size(), hipMemcpyDeviceToHost, m->st));
+    HIP_OK(hipStreamSynchronize(m->st));
+    auto* K = new sdl::mlp::Packed();
+    packMlp(p.data(), *K);
+    const hipError_t e = hipMemcpyAsync(m->dPacked, K, sizeof(*K), hipMemcpyHostToDevice, m->st);
+    const hipError_t e2 = hipStreamSynchronize(m->st);
+    delete K;
+    if (e != hipSuccess || e2 != hipSuccess) return fail(SDL_ERR_HIP, "MLP upload failed");
+    return 0;
+}
+
 }  // namespace
 
 // ====================================================================== C-ABI
@@ -2081,6 +2498,7 @@ void sdl_event_destroy(sdl_event* ev) {
     if (!ev) return;
     (void)hipStreamSynchronize(ev->st);
     // device buffers are released by their destructors
+    dropGraph(ev);
     for (auto& t : ev->tev)
         for (auto& x : t)
             if (x) (void)hipEventDestroy(x);
@@ -2242,10 +2660,156 @@ int sdl_event_kernel_ms(sdl_event* ev, double* out, int n) {
 
 void* sdl_event_stream(sdl_event* ev) { return ev ? (void*)ev->st : nullptr; }
 
+int sdl_event_set_graphs(sdl_event* ev, int on) {
+    if (!ev) return fail(SDL_ERR_ARG, "null event");
+    ev->useGraphs = on != 0;
+    if (!ev->useGraphs) dropGraph(ev);
+    return 0;
+}
+
 int sdl_event_set_headroom(sdl_event* ev, int fromStage, double factor) {
     if (!ev || factor <= 0.0) return fail(SDL_ERR_ARG, "bad arguments");
     ev->headroomFrom = fromStage;
     ev->headroomFactor = factor;
+    return 0;
+}
+
+// ---------------------------------------------------------------- learned proposal
+int sdl_mlp_create(sdl_mlp** out) {
+    if (!out) return fail(SDL_ERR_ARG, "null argument");
+    auto* m = new sdl_mlp();
+    (void)hipGetDevice(&m->device);
+    const size_t pb = sizeof(float) * sdl::mlp::kNParams;
+    if (hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking) != hipSuccess || hipMalloc(&m->dParams, pb) != hipSuccess ||
+        hipMalloc(&m->dGrad, pb) != hipSuccess || hipMalloc(&m->dM, pb) != hipSuccess ||
+        hipMalloc(&m->dV, pb) != hipSuccess || hipMalloc(&m->dLoss, sizeof(float)) != hipSuccess ||
+        hipMalloc(&m->dPacked, sizeof(sdl::mlp::Packed)) != hipSuccess) {
+        sdl_mlp_free(m);
+        return fail(SDL_ERR_HIP, "MLP allocation failed");
+    }
+    for (float* p : {m->dParams, m->dGrad, m->dM, m->dV})
+        hipLaunchKernelGGL(k_zero<float>, dim3(gridFor(sdl::mlp::kNParams, kBlock)), dim3(kBlock), 0, m->st, p,
+                           (long long)sdl::mlp::kNParams);
+    if (int r = repackMlp(m)) {
+        sdl_mlp_free(m);
+        return r;
+    }
+    *out = m;
+    return 0;
+}
+
+void sdl_mlp_free(sdl_mlp* m) {
+    if (!m) return;
+    if (m->st) (void)hipStreamSynchronize(m->st);
+    for (void* p : {(void*)m->dParams, (void*)m->dGrad, (void*)m->dM, (void*)m->dV, (void*)m->dLoss, (void*)m->dPacked})
+        if (p) (void)hipFree(p);
+    if (m->st) (void)hipStreamDestroy(m->st);
+    delete m;
+}
+
+int sdl_mlp_num_params(void) { return sdl::mlp::kNParams; }
+
+int sdl_mlp_set_params(sdl_mlp* m, const float* params, int n) {
+    if (!m || !params || n != sdl::mlp::kNParams) return fail(SDL_ERR_ARG, "bad arguments");
+    HIP_OK(hipMemcpyAsync(m->dParams, params, sizeof(float) * n, hipMemcpyHostToDevice, m->st));
+    for (float* p : {m->dGrad, m->dM, m->dV})
+        hipLaunchKernelGGL(k_zero<float>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0, m->st, p, (long long)n);
+    m->adamSteps = 0;
+    return repackMlp(m);
+}
+
+int sdl_mlp_get_params(sdl_mlp* m, float* params, int n) {
+    if (!m || !params || n != sdl::mlp::kNParams) return fail(SDL_ERR_ARG, "bad arguments");
+    HIP_OK(hipMemcpyAsync(params, m->dParams, sizeof(float) * n, hipMemcpyDeviceToHost, m->st));
+    HIP_OK(hipStreamSynchronize(m->st));
+    return 0;
+}
+
+int sdl_mlp_gradient(sdl_mlp* m, const float* dX, const float* dY, long long n, float posWeight, float* grad,
+                     float* loss) {
+    using namespace sdl::mlp;
+    if (!m || !dX || !dY || n <= 0 || !grad || !loss) return fail(SDL_ERR_ARG, "bad arguments");
+    hipLaunchKernelGGL(k_zero<float>, dim3(gridFor(kNParams, kBlock)), dim3(kBlock), 0, m->st, m->dGrad,
+                       (long long)kNParams);
+    hipLaunchKernelGGL(k_zero<float>, dim3(1), dim3(kBlock), 0, m->st, m->dLoss, 1LL);
+    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(gridFor(n, kTrainSamples)), dim3(kTrainThreads), 0, m->st, dX, dY,
+                       (const int*)nullptr, 0LL, n, n, m->dParams, posWeight, m->dGrad, m->dLoss);
+    HIP_OK(hipMemcpyAsync(grad, m->dGrad, sizeof(float) * kNParams, hipMemcpyDeviceToHost, m->st));
+    HIP_OK(hipMemcpyAsync(loss, m->dLoss, sizeof(float), hipMemcpyDeviceToHost, m->st));
+    hipLaunchKernelGGL(k_zero<float>, dim3(gridFor(kNParams, kBlock)), dim3(kBlock), 0, m->st, m->dGrad,
+                       (long long)kNParams);
+    HIP_OK(hipStreamSynchronize(m->st));
+    return 0;
+}
+
+int sdl_mlp_train(sdl_mlp* m, const float* dX, const float* dY, const int* dIdx, long long start, int batch, int steps,
+                  float lr, float posWeight, float* lastLoss) {
+    using namespace sdl::mlp;
+    if (!m || !dX || !dY || batch <= 0 || steps < 0 || start < 0) return fail(SDL_ERR_ARG, "bad arguments");
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (int k = 0; k < steps; ++k) {
+        hipLaunchKernelGGL(k_zero<float>, dim3(1), dim3(kBlock), 0, m->st, m->dLoss, 1LL);
+        hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(gridFor(batch, kTrainSamples)), dim3(kTrainThreads), 0, m->st, dX, dY,
+                           dIdx, start + (long long)k * batch, (long long)batch, (long long)batch, m->dParams,
+                           posWeight, m->dGrad, m->dLoss);
+        ++m->adamSteps;
+        const float bc1 = 1.f - std::pow(b1, float(m->adamSteps)), bc2 = 1.f - std::pow(b2, float(m->adamSteps));
+        hipLaunchKernelGGL(k_mlp_adam, dim3(gridFor(kNParams, kBlock)), dim3(kBlock), 0, m->st, m->dParams, m->dGrad,
+                           m->dM, m->dV, lr, b1, b2, 1e-8f, bc1, bc2);
+    }
+    if (lastLoss) HIP_OK(hipMemcpyAsync(lastLoss, m->dLoss, sizeof(float), hipMemcpyDeviceToHost, m->st));
+    HIP_OK(hipStreamSynchronize(m->st));
+    return repackMlp(m);
+}
+
+int sdl_mlp_forward(sdl_mlp* m, const float* dX, long long n, float* dLogit) {
+    if (!m || !dX || !dLogit || n < 0) return fail(SDL_ERR_ARG, "bad arguments");
+    if (n == 0) return 0;
+    const unsigned grid = std::min<unsigned>(gridFor((n + 15) / 16, kWavesPerBlock), 8192u);
+    hipLaunchKernelGGL(sdl::mlp::k_mlp_forward, dim3(grid), dim3(kBlock), 0, m->st, dX, n, m->dPacked, dLogit);
+    HIP_OK(hipStreamSynchronize(m->st));
+    return 0;
+}
+
+int sdl_event_set_proposal(sdl_event* ev, sdl_mlp* m, float threshold) {
+    if (!ev || (m && !(threshold > 0.f && threshold < 1.f))) return fail(SDL_ERR_ARG, "bad arguments");
+    ev->proposal = m;
+    ev->proposalLogit = m ? std::log(threshold / (1.f - threshold)) : 0.f;
+    return 0;
+}
+
+int sdl_event_tl_pair_rows(sdl_event* ev, float* dX, float* dY, long long cap, long long* n) {
+    if (!ev || !n) return fail(SDL_ERR_ARG, "bad arguments");
+    if (ev->stage < SDL_STAGE_TL) return fail(SDL_ERR_STATE, "the tracklet stage has not run");
+    if (ev->proposal) return fail(SDL_ERR_STATE, "training rows come from a plain chain (no proposal set)");
+    long long t[TOT_N];
+    HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, ev->st));
+    HIP_OK(hipStreamSynchronize(ev->st));
+    const long long total = t[TOT_TLP0] + t[TOT_TLP1] + t[TOT_TLP2];
+    *n = total;
+    if (!dX) return 0;
+    if (!dY || cap < total) return fail(SDL_ERR_CAPACITY, "row buffer too small");
+    const MDDev D = mdView(ev);
+    const SGDev S = sgView(ev);
+    long long row0 = 0;
+    for (int v = 0; v < kNVariantsTL; ++v) {
+        const long long np = t[TOT_TLP0 + v];
+        if (np > 0) {
+            const dim3 grid(gridFor(np, kBlock));
+            const Count cP = devCount(ev, TOT_TLP0 + v);
+            if (v == 0)
+                hipLaunchKernelGGL(k_tl_pair_rows<TL_BBBB>, grid, dim3(kBlock), 0, ev->st, cP, D, S, ev->sgStart.p,
+                                   ev->tlPair[0].p, ev->tlCOff[0].p, ev->tlFlag[0].p, row0, dX, dY);
+            else if (v == 1)
+                hipLaunchKernelGGL(k_tl_pair_rows<TL_BBEE>, grid, dim3(kBlock), 0, ev->st, cP, D, S, ev->sgStart.p,
+                                   ev->tlPair[1].p, ev->tlCOff[1].p, ev->tlFlag[1].p, row0, dX, dY);
+            else
+                hipLaunchKernelGGL(k_tl_pair_rows<TL_EEEE>, grid, dim3(kBlock), 0, ev->st, cP, D, S, ev->sgStart.p,
+                                   ev->tlPair[2].p, ev->tlCOff[2].p, ev->tlFlag[2].p, row0, dX, dY);
+        }
+        row0 += np;
+    }
+    HIP_OK(hipStreamSynchronize(ev->st));
     return 0;
 }
 

@@ -404,6 +404,7 @@ struct TLInner {
     HitView lo, up;
     float dphichange, dphichangemin, dphichangemax;
     float dr, d, alphaReg3Lo;
+    double sqrtR3Rt;  // sqrt(double(sqrtf(z^2 + rt^2) / rt)) of the inner anchor (the muls factor)
     bool bLo, bUp, psLo, psUp;
 };
 // outer segment, first stage: its inner MD
@@ -439,6 +440,14 @@ __device__ __forceinline__ int tlVariant(bool b0, bool b1, bool b2) {
     return TL_NONE;
 }
 
+// sqrt(double(r / rt)) with r = sqrtf(z^2 + rt^2) of an anchor: the factor of
+// the multiple-scattering term that depends on the inner anchor only
+// (Tracklet.cc:1050, 1716)
+__device__ __forceinline__ double mulsFactor(float z, float rt) {
+    const float r = sqrtF(z * z + rt * rt);
+    return sqrt(double(r / rt));
+}
+
 // ---- first stage: cuts on the inner segment and the outer segment's inner MD
 // _v2: Tracklet.cc:902-1153 (z window, pointed z window, deltaPhi of the midpoint)
 __device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L, TLCarry& c) {
@@ -454,14 +463,13 @@ __device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L,
     const float zLo = z_InLo + (z_InLo - kDeltaZLum) * (rtRatio - 1.f) * (z_InLo > 0.f ? 1.f : dzDrtScale) - (zpIn + zpOut);
     if (!(z_OutLo >= zLo && z_OutLo <= zHi)) return false;
     const float drt_OutLo_InLo = rt_OutLo - rt_InLo;
-    const float r3_InLo = sqrtF(z_InLo * z_InLo + rt_InLo * rt_InLo);
     const float drt_InSeg = I.d;
     const float dz_InSeg = hInUp.z - hInLo.z;
     const float dr3_InSeg = hInUp.r3 - hInLo.r3;
     const float coshEta = dr3_InSeg / drt_InSeg;
     float dzErr = (zpIn + zpOut) * (zpIn + zpOut) * 2.f;
     const float thetaMuls = float(double(0.015f) * sqrt(double(0.1f) + 0.2 * double(rt_OutLo - rt_InLo) / double(50.f)) *
-                                  sqrt(double(r3_InLo / rt_InLo)));
+                                  I.sqrtR3Rt);
     const float sdlMuls = thetaMuls * 3.f / kPtCut * 4.f;
     dzErr += sdlMuls * sdlMuls * drt_OutLo_InLo * drt_OutLo_InLo / 3.f * coshEta * coshEta;
     dzErr = sqrtF(dzErr);
@@ -518,9 +526,8 @@ __device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, T
     const float kZ = (zOut - zIn) / dzSDIn;
     float thetaMuls;
     if (!EEEE) {
-        const float rIn = sqrtF(zIn * zIn + rtIn * rtIn);
         thetaMuls = float(double(0.015f) * sqrt(double(0.1f) + 0.2 * double(rtOut - rtIn) / double(50.f)) *
-                          sqrt(double(rIn / rtIn)));
+                          I.sqrtR3Rt);
         if (!(kZ >= 0 && rtOut >= rtLo && rtOut <= rtHi)) return false;
     } else {
         thetaMuls = float(double(0.015f) * sqrt(double(0.1f) + 0.2 * double(rtOut - rtIn) / double(50.f)));

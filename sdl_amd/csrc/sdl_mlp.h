@@ -1,0 +1,265 @@
+// Learned proposal for the tracklet stage: a 24-64-64-64-1 MLP (ReLU hidden
+// layers, sigmoid output) that scores (inner SG, outer-inner MD) pairs before
+// the exact tracklet cuts.  This component has no counterpart in the reference;
+// the exact cuts still decide every object, the network only drops pairs.
+//
+//  * inference: BF16 MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation,
+//    one wave per 16 pairs.  Layers are computed transposed, D = W . X^T, so a
+//    layer's accumulator fragment (rows 4q..4q+3 of a 16-neuron tile for the
+//    pair in column l % 16, q = l / 16) is exactly the B fragment the next layer
+//    needs for that tile's K step: activations never leave registers.
+//  * training: FP32 forward and backward kernels (weighted binary cross
+//    entropy on logits, gradients reduced per block in LDS, Adam update).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sdl {
+namespace mlp {
+
+constexpr int kIn = 24;    // features
+constexpr int kH = 64;     // hidden width
+constexpr int kTiles = kH / 16;
+// parameter layout (float32, weights as [out][in])
+constexpr int kW1 = 0;
+constexpr int kB1 = kW1 + kH * kIn;
+constexpr int kW2 = kB1 + kH;
+constexpr int kB2 = kW2 + kH * kH;
+constexpr int kW3 = kB2 + kH;
+constexpr int kB3 = kW3 + kH * kH;
+constexpr int kW4 = kB3 + kH;
+constexpr int kB4 = kW4 + kH;
+constexpr int kNParams = kB4 + 1;  // 9985
+
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+// Weight fragments in the lane order of the MFMA operands (packed on the host).
+struct Packed {
+    short4v a1[kTiles][2][64];      // layer 1: K = 32 (24 features, the constant 1 at 24 carries b1, zeros)
+    short4v a2[kTiles][kTiles][64]; // layer 2: [output tile][K step][lane]
+    short4v a3[kTiles][kTiles][64];
+    float4v c2[kTiles][64];         // layer-2 bias at the accumulator rows of the lane
+    float4v c3[kTiles][64];
+    float4v w4[kTiles][64];         // output weights at the accumulator rows of the lane
+    float b4;
+};
+
+__device__ __forceinline__ short bf16Bits(float f) {
+    const __bf16 h = (__bf16)f;  // round to nearest even
+    return __builtin_bit_cast(short, h);
+}
+
+__device__ __forceinline__ short4v toBf16(float a, float b, float c, float d) {
+    short4v r;
+    r[0] = bf16Bits(a);
+    r[1] = bf16Bits(b);
+    r[2] = bf16Bits(c);
+    r[3] = bf16Bits(d);
+    return r;
+}
+
+__device__ __forceinline__ short4v reluBf16(float4v d) {
+    return toBf16(fmaxf(d[0], 0.f), fmaxf(d[1], 0.f), fmaxf(d[2], 0.f), fmaxf(d[3], 0.f));
+}
+
+// Per-wave copy of the fragments it needs (one lane's share of every operand).
+struct Frags {
+    short4v a1[kTiles][2], a2[kTiles][kTiles], a3[kTiles][kTiles];
+    float4v c2[kTiles], c3[kTiles], w4[kTiles];
+    float b4;
+};
+
+__device__ __forceinline__ void loadFrags(const Packed* P, int lane, Frags& F) {
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        F.a1[t][0] = P->a1[t][0][lane];
+        F.a1[t][1] = P->a1[t][1][lane];
+#pragma unroll
+        for (int k = 0; k < kTiles; ++k) {
+            F.a2[t][k] = P->a2[t][k][lane];
+            F.a3[t][k] = P->a3[t][k][lane];
+        }
+        F.c2[t] = P->c2[t][lane];
+        F.c3[t] = P->c3[t][lane];
+        F.w4[t] = P->w4[t][lane];
+    }
+    F.b4 = P->b4;
+}
+
+// Logit of the sample in column lane % 16.  f holds that sample's 24 features
+// (every lane of the column passes the same values).  All 64 lanes must call.
+__device__ __forceinline__ float logit16(const float* f, int lane, const Frags& F) {
+    const int q = lane >> 4;
+    // layer-1 B fragments: features 4q..4q+3 and 16+4q..16+4q+3 (24 is the constant 1)
+    float x1[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int k = 16 + 4 * q + v;
+        x1[v] = k < kIn ? f[k] : (k == kIn ? 1.f : 0.f);
+    }
+    const short4v b0 = toBf16(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+    const short4v b1 = toBf16(x1[0], x1[1], x1[2], x1[3]);
+    short4v h1[kTiles], h2[kTiles];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        float4v d = {0.f, 0.f, 0.f, 0.f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a1[t][0], b0, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a1[t][1], b1, d, 0, 0, 0);
+        h1[t] = reluBf16(d);
+    }
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        float4v d = F.c2[t];
+#pragma unroll
+        for (int k = 0; k < kTiles; ++k) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a2[t][k], h1[k], d, 0, 0, 0);
+        h2[t] = reluBf16(d);
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        float4v d = F.c3[t];
+#pragma unroll
+        for (int k = 0; k < kTiles; ++k) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a3[t][k], h2[k], d, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) part += F.w4[t][v] * fmaxf(d[v], 0.f);
+    }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    return part + F.b4;
+}
+
+// Inference on stored features (row-major n x 24): one wave per 16 rows.
+__global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, const Packed* P, float* logit) {
+    const int lane = threadIdx.x & 63;
+    Frags F;
+    loadFrags(P, lane, F);
+    const long long nTiles = (n + 15) / 16;
+    const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long tile = w0; tile < nTiles; tile += nw) {
+        const long long i = tile * 16 + (lane & 15);
+        float f[kIn];
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) f[k] = i < n ? X[i * kIn + k] : 0.f;
+        const float z = logit16(f, lane, F);
+        if (lane < 16 && i < n) logit[i] = z;
+    }
+}
+
+// ---------------------------------------------------------------- training
+constexpr int kTrainSamples = 32;   // samples per block
+constexpr int kTrainThreads = 256;  // 8 threads per sample
+constexpr int kPerSample = kTrainThreads / kTrainSamples;
+
+// One block: forward and backward of kTrainSamples samples (rows idx[base + s],
+// or base + s when idx is null) of the mean weighted BCE over `nTotal` samples;
+// gradients and loss are added to grad[] / loss[0].
+__global__ void __launch_bounds__(kTrainThreads) k_mlp_fwd_bwd(const float* X, const float* Y, const int* idx,
+                                                               long long base, long long nBatch, long long nTotal,
+                                                               const float* params, float posWeight, float* grad,
+                                                               float* loss) {
+    __shared__ float W[kNParams];
+    __shared__ float x[kTrainSamples][kIn];
+    __shared__ float h1[kTrainSamples][kH], h2[kTrainSamples][kH], h3[kTrainSamples][kH];
+    __shared__ float d1[kTrainSamples][kH], d2[kTrainSamples][kH], d3[kTrainSamples][kH];
+    __shared__ float d4[kTrainSamples];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < kNParams; k += kTrainThreads) W[k] = params[k];
+    const int s = tid / kPerSample, r = tid % kPerSample;  // sample, part
+    const long long row = (long long)blockIdx.x * kTrainSamples + s;
+    const bool valid = row < nBatch;
+    const long long src = valid ? (idx ? (long long)idx[base + row] : base + row) : 0;
+    for (int k = r; k < kIn; k += kPerSample) x[s][k] = valid ? X[src * kIn + k] : 0.f;
+    __syncthreads();
+    // forward: each part computes kH / kPerSample neurons of its sample
+    constexpr int kNpp = kH / kPerSample;
+    for (int j = r * kNpp; j < (r + 1) * kNpp; ++j) {
+        float a = W[kB1 + j];
+        for (int k = 0; k < kIn; ++k) a += W[kW1 + j * kIn + k] * x[s][k];
+        h1[s][j] = fmaxf(a, 0.f);
+    }
+    __syncthreads();
+    for (int j = r * kNpp; j < (r + 1) * kNpp; ++j) {
+        float a = W[kB2 + j];
+        for (int k = 0; k < kH; ++k) a += W[kW2 + j * kH + k] * h1[s][k];
+        h2[s][j] = fmaxf(a, 0.f);
+    }
+    __syncthreads();
+    float zp = 0.f;
+    for (int j = r * kNpp; j < (r + 1) * kNpp; ++j) {
+        float a = W[kB3 + j];
+        for (int k = 0; k < kH; ++k) a += W[kW3 + j * kH + k] * h2[s][k];
+        h3[s][j] = fmaxf(a, 0.f);
+        zp += W[kW4 + j] * h3[s][j];
+    }
+    // the kPerSample parts of a sample are consecutive lanes
+    for (int o = 1; o < kPerSample; o <<= 1) zp += __shfl_xor(zp, o);
+    if (r == 0) {
+        const float z = zp + W[kB4];
+        const float y = valid ? Y[src] : 0.f;
+        const float w = valid ? (y > 0.5f ? posWeight : 1.f) / float(nTotal) : 0.f;
+        const float p = 1.f / (1.f + expf(-z));
+        d4[s] = w * (p - y);
+        const float l = w * (fmaxf(z, 0.f) - z * y + log1pf(expf(-fabsf(z))));
+        if (valid) atomicAdd(loss, l);
+    }
+    __syncthreads();
+    // backward through the hidden layers (ReLU masks from the activations)
+    for (int j = r * kNpp; j < (r + 1) * kNpp; ++j) d3[s][j] = h3[s][j] > 0.f ? d4[s] * W[kW4 + j] : 0.f;
+    __syncthreads();
+    for (int j = r * kNpp; j < (r + 1) * kNpp; ++j) {
+        float a = 0.f;
+        for (int i = 0; i < kH; ++i) a += W[kW3 + i * kH + j] * d3[s][i];
+        d2[s][j] = h2[s][j] > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    for (int j = r * kNpp; j < (r + 1) * kNpp; ++j) {
+        float a = 0.f;
+        for (int i = 0; i < kH; ++i) a += W[kW2 + i * kH + j] * d2[s][i];
+        d1[s][j] = h1[s][j] > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    // parameter gradients: sums over the block's samples, one entry per thread at a time
+    for (int e = tid; e < kNParams; e += kTrainThreads) {
+        float g = 0.f;
+        if (e < kB1) {
+            const int j = e / kIn, k = e % kIn;
+            for (int q = 0; q < kTrainSamples; ++q) g += d1[q][j] * x[q][k];
+        } else if (e < kW2) {
+            for (int q = 0; q < kTrainSamples; ++q) g += d1[q][e - kB1];
+        } else if (e < kB2) {
+            const int j = (e - kW2) / kH, k = (e - kW2) % kH;
+            for (int q = 0; q < kTrainSamples; ++q) g += d2[q][j] * h1[q][k];
+        } else if (e < kW3) {
+            for (int q = 0; q < kTrainSamples; ++q) g += d2[q][e - kB2];
+        } else if (e < kB3) {
+            const int j = (e - kW3) / kH, k = (e - kW3) % kH;
+            for (int q = 0; q < kTrainSamples; ++q) g += d3[q][j] * h2[q][k];
+        } else if (e < kW4) {
+            for (int q = 0; q < kTrainSamples; ++q) g += d3[q][e - kB3];
+        } else if (e < kB4) {
+            for (int q = 0; q < kTrainSamples; ++q) g += d4[q] * h3[q][e - kW4];
+        } else {
+            for (int q = 0; q < kTrainSamples; ++q) g += d4[q];
+        }
+        if (g != 0.f) atomicAdd(&grad[e], g);
+    }
+}
+
+// Adam step on every parameter; clears the gradient.
+__global__ void k_mlp_adam(float* params, float* grad, float* m, float* v, float lr, float beta1, float beta2,
+                           float eps, float bc1, float bc2) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kNParams) return;
+    const float g = grad[e];
+    const float mm = beta1 * m[e] + (1.f - beta1) * g;
+    const float vv = beta2 * v[e] + (1.f - beta2) * g * g;
+    m[e] = mm;
+    v[e] = vv;
+    params[e] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    grad[e] = 0.f;
+}
+
+}  // namespace mlp
+}  // namespace sdl

@@ -55,6 +55,16 @@ class DeviceArray:
         if self.nbytes:
             _ok(hip().hipMemcpy(self.ptr, host.ctypes.data, self.nbytes, H2D), "hipMemcpy")
 
+    @classmethod
+    def empty(cls, n, dtype):
+        """An uninitialised device array of n elements."""
+        a = cls.__new__(cls)
+        a.dtype, a.shape = np.dtype(dtype), (int(n),)
+        a.nbytes = int(n) * a.dtype.itemsize
+        a.ptr = ctypes.c_void_p()
+        _ok(hip().hipMalloc(ctypes.byref(a.ptr), max(a.nbytes, 4)), "hipMalloc")
+        return a
+
     @property
     def address(self):
         return self.ptr.value

@@ -22,7 +22,9 @@ EXPORTS = (
     "sdl_event_destroy", "sdl_event_reset", "sdl_event_add_hits", "sdl_event_next", "sdl_event_set_hits_device",
     "sdl_event_run", "sdl_event_run_chain", "sdl_event_sync", "sdl_event_counts", "sdl_event_layer_counts",
     "sdl_event_export", "sdl_event_num_events", "sdl_event_stream", "sdl_event_set_timing", "sdl_event_kernel_ms",
-    "sdl_event_set_headroom",
+    "sdl_event_set_headroom", "sdl_event_set_graphs",
+    "sdl_mlp_create", "sdl_mlp_free", "sdl_mlp_num_params", "sdl_mlp_set_params", "sdl_mlp_get_params",
+    "sdl_mlp_gradient", "sdl_mlp_train", "sdl_mlp_forward", "sdl_event_set_proposal", "sdl_event_tl_pair_rows",
 )
 TIMERS = ("prep", "md", "sg", "tl")
 
@@ -62,6 +64,19 @@ def lib():
         L.sdl_event_set_timing.argtypes = [vp, i]
         L.sdl_event_kernel_ms.argtypes = [vp, vp, i]
         L.sdl_event_set_headroom.argtypes = [vp, i, ctypes.c_double]
+        L.sdl_event_set_graphs.argtypes = [vp, i]
+        f = ctypes.c_float
+        L.sdl_mlp_create.argtypes = [ctypes.POINTER(vp)]
+        L.sdl_mlp_free.argtypes = [vp]
+        L.sdl_mlp_free.restype = None
+        L.sdl_mlp_num_params.argtypes = []
+        L.sdl_mlp_set_params.argtypes = [vp, vp, i]
+        L.sdl_mlp_get_params.argtypes = [vp, vp, i]
+        L.sdl_mlp_gradient.argtypes = [vp, vp, vp, ll, f, vp, vp]
+        L.sdl_mlp_train.argtypes = [vp, vp, vp, vp, ll, i, i, f, f, vp]
+        L.sdl_mlp_forward.argtypes = [vp, vp, ll, vp]
+        L.sdl_event_set_proposal.argtypes = [vp, vp, f]
+        L.sdl_event_tl_pair_rows.argtypes = [vp, vp, vp, ll, ctypes.POINTER(ll)]
         _lib = L
     return _lib
 
@@ -165,6 +180,10 @@ class Event:
     def set_headroom(self, from_stage, factor):
         """Headroom factor of the predicted buffer sizes for stages >= from_stage."""
         _check(lib().sdl_event_set_headroom(self.h, from_stage, factor))
+
+    def set_graphs(self, on=True):
+        """Allow the speculative chain to be captured into a HIP graph and replayed."""
+        _check(lib().sdl_event_set_graphs(self.h, 1 if on else 0))
 
     def kernel_ms(self):
         out = np.zeros(len(TIMERS), np.float64)
