@@ -81,7 +81,7 @@ def tl_roofline(workload, nevents, tl_ms):
             "flops_per_launch": flops}
 
 
-def learned_leg(a, ev, step, plain_counts, plain_ms):
+def learned_leg(a, ev, step, plain_counts, plain_ms, plain_work):
     """The same steps with the learned proposal at the threshold chosen for the
     target TC recall; recall = TCs kept / TCs of the plain chain on the step's
     events (the learned objects are a subset of the plain ones)."""
@@ -102,12 +102,14 @@ def learned_leg(a, ev, step, plain_counts, plain_ms):
     hip.synchronize()
     wall = time.perf_counter() - t0
     c = ev.counts(-1)
+    work = ev.tl_work()
     proposal.set_proposal(ev, None, 0.5)
     ms = 1000.0 * wall / a.steps
     return {"value": round(a.events_per_step * a.steps / wall, 3), "ms_per_step": round(ms, 4),
             "speedup_vs_plain": round(plain_ms / ms, 4), "threshold": t, "target_tc_recall": float(a.recall),
             "tc_recall": round(c["nTC"] / max(plain_counts["nTC"], 1), 6),
             "tl_recall": round(c["nTL"] / max(plain_counts["nTL"], 1), 6),
+            "tl_work_learned": work, "tl_work_plain": plain_work,
             "weights": os.path.relpath(a.learned, ROOT), "trained_on": meta.get("train_seeds"),
             "validated_on": meta.get("val_seeds")}
 
@@ -170,7 +172,8 @@ def main():
         kms[k] /= a.steps
 
     roofline = tl_roofline(a.workload, B, kms["tl"])
-    learned = learned_leg(a, ev, step, counts, ms_per_step) if os.path.exists(a.learned) else None
+    plain_work = ev.tl_work()
+    learned = learned_leg(a, ev, step, counts, ms_per_step, plain_work) if os.path.exists(a.learned) else None
     out = {
         "metric": "events/s on 1 MI355X at PU200 density: plain vs learned, >=99.5% TC recall",
         "value": round(B * a.steps / wall, 3),

@@ -157,6 +157,9 @@ int sdl_event_set_proposal(sdl_event* ev, sdl_mlp* m, float threshold);
  * and labels (1 if the pair yields a tracklet, DEVICE dY) of every pair; with
  * dX = NULL only *n (the number of pairs) is returned. */
 int sdl_event_tl_pair_rows(sdl_event* ev, float* dX, float* dY, long long cap, long long* n);
+/* Diagnostics of the last tracklet stage: out[0] (inner segment, MD) pairs,
+ * out[1] candidates that reached the second-stage cuts, out[2] tracklets. */
+int sdl_event_tl_work(sdl_event* ev, long long out[3]);
 
 #ifdef __cplusplus
 }

@@ -104,6 +104,7 @@ enum {
     TOT_MDCAP = 0, TOT_MD, TOT_SGCAP, TOT_SG, TOT_TP, TOT_TLCAP, TOT_TL, TOT_TPE, TOT_TCP, TOT_TCL,
     TOT_TLC0, TOT_TLC1, TOT_TLC2,  // second-stage tracklet candidates per variant
     TOT_TLP0, TOT_TLP1, TOT_TLP2,  // first-stage tracklet pairs per variant
+    TOT_TLPAIRS, TOT_TLCANDS,      // all first-stage pairs, all second-stage candidates
     TOT_ABORT, TOT_UNKNOWN, TOT_N
 };
 
@@ -909,7 +910,11 @@ __global__ void k_tl_cap(Count cSG, int NM, const ModDev* mods, const int* sgOut
                          const int* slotMdSg, long long* tlCap, int* nPair, long long stride) {
     const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nSG) return;
+    if (s >= stride) return;
+    if (s >= nSG) {  // the three per-variant rows are scanned as one array
+        for (int v = 0; v < kNVariantsTL; ++v) nPair[v * stride + s] = 0;
+        return;
+    }
     const int b = mdSlot[sgOut[s]];
     const int m = b % NM, eb = b - m;
     const int fl = sgFlags[s];
@@ -925,9 +930,32 @@ __global__ void k_tl_cap(Count cSG, int NM, const ModDev* mods, const int* sgOut
     for (int v = 0; v < kNVariantsTL; ++v) nPair[v * stride + s] = np[v];
 }
 
-struct TLPairs {
-    const long long* off[kNVariantsTL];  // per inner SG
-    int2* pair[kNVariantsTL];            // (inner SG, outer-inner MD)
+// The items of one variant in a list ordered by variant: [b, e) with
+// b = off1[v * stride] (then off2[b] when off2 is set), e likewise for v + 1.
+// A launch sized for `cap` items aborts the run if the variant has more.
+struct VarRange {
+    const long long* off1;
+    const long long* off2;
+    long long stride;
+    int v;
+    long long cap;
+    long long* tot;
+    __device__ __forceinline__ bool get(long long& b, long long& e) const {
+        b = e = 0;
+        if (tot[TOT_ABORT]) return false;
+        long long p0 = off1[v * stride], p1 = off1[(v + 1) * stride];
+        if (off2) {
+            p0 = off2[p0];
+            p1 = off2[p1];
+        }
+        if (p1 - p0 > cap) {
+            if (threadIdx.x == 0) tot[TOT_ABORT] = 1 + SDL_STAGE_TL;
+            return false;
+        }
+        b = p0;
+        e = p1;
+        return true;
+    }
 };
 
 // The MDs of each slot that start at least one SG, in MD order.
@@ -941,11 +969,12 @@ __global__ void k_slot_md_sg_list(Count cNS, const long long* mdStart, const lon
         if (sgStart[m + 1] > sgStart[m]) list[d++] = int(m);
 }
 
-// One thread per inner SG: its pairs, in connected-module and MD order.
+// One thread per inner SG: its pairs, in connected-module and MD order, into
+// one list ordered by variant (offsets pairOff[v * stride + s]).
 __global__ void k_tl_enum(Count cSG, int NM, const ModDev* mods, const int* sgOut, const int* sgFlags,
                           const int* mdSlot, const int* connStart, const int* connList, const long long* listStart,
-                          const int* list, TLPairs P, Guard g0, Guard g1, Guard g2) {
-    if (g0.over() || g1.over() || g2.over()) return;
+                          const int* list, const long long* pairOff, long long stride, int2* pair, Guard g) {
+    if (g.over()) return;
     const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSG) return;
@@ -953,14 +982,13 @@ __global__ void k_tl_enum(Count cSG, int NM, const ModDev* mods, const int* sgOu
     const int m = b % NM, eb = b - m;
     const int fl = sgFlags[s];
     long long pos[kNVariantsTL];
-    for (int v = 0; v < kNVariantsTL; ++v) pos[v] = P.off[v][s];
+    for (int v = 0; v < kNVariantsTL; ++v) pos[v] = pairOff[v * stride + s];
     for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
         const int v = tlModuleVariant(fl, mods[connList[k]]);
         if (v >= kNVariantsTL) continue;
         const int t = eb + connList[k];
-        int2* dst = P.pair[v];
         long long p = pos[v];
-        for (long long q = listStart[t]; q < listStart[t + 1]; ++q) dst[p++] = make_int2(s, list[q]);
+        for (long long q = listStart[t]; q < listStart[t + 1]; ++q) pair[p++] = make_int2(s, list[q]);
         pos[v] = p;
     }
 }
@@ -1060,56 +1088,63 @@ __device__ __forceinline__ void pairFeatures(long long i, const int2* pair, cons
     tlPairFeatures(I, L, V, f);
 }
 
-// Keep flag of every pair: the network's logit against the threshold, one wave per 16 pairs.
+// Keep flag of every pair: the network's logit against the threshold, one wave per 64 pairs.
 template <int V>
-__global__ void __launch_bounds__(kBlock) k_tl_score(Count cP, MDDev D, SGDev S, const long long* sgStart,
-                                                     const int2* pair, const mlp::Packed* P, float logitCut,
-                                                     unsigned char* keep) {
-    const long long n = cP.get();
+__global__ void __launch_bounds__(kBlock) k_tl_score(VarRange R, MDDev D, SGDev S, const long long* sgStart,
+                                                     const int2* pairAll, const mlp::Packed* P, float logitCut,
+                                                     unsigned char* keepAll) {
+    __shared__ short tiles[kWavesPerBlock][64 * 32];
+    long long b0, e0;
+    R.get(b0, e0);
+    const long long n = e0 - b0;
+    const int2* pair = pairAll + b0;
+    unsigned char* keep = keepAll + b0;
     const int lane = threadIdx.x & 63;
-    const long long nTiles = (n + 15) / 16;
+    short* tile = tiles[threadIdx.x >> 6];
+    const long long nTiles = (n + 63) / 64;
     const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
     if (w0 >= nTiles) return;  // uniform over the wave
     mlp::Frags F;
     mlp::loadFrags(P, lane, F);
-    for (long long tile = w0; tile < nTiles; tile += nw) {
-        const long long i = tile * 16 + (lane & 15);
+    for (long long t = w0; t < nTiles; t += nw) {
+        const long long i = t * 64 + lane;
         float f[mlp::kIn];
         if (i < n) pairFeatures<V>(i, pair, sgStart, D, S, f);
         else
             for (int k = 0; k < mlp::kIn; ++k) f[k] = 0.f;
-        const float z = mlp::logit16(f, lane, F);
-        if (lane < 16 && i < n) keep[i] = z >= logitCut ? 1 : 0;
+        const float z = mlp::logit64(f, lane, F, tile);
+        if (i < n) keep[i] = z >= logitCut ? 1 : 0;
     }
 }
 
 // Training rows of the plain chain: features of every pair and its label (the
 // pair yields at least one tracklet), written at row offset `row0`.
 template <int V>
-__global__ void k_tl_pair_rows(Count cP, MDDev D, SGDev S, const long long* sgStart, const int2* pair,
-                               const long long* candOff, const unsigned char* flag, long long row0, float* X,
-                               float* Y) {
-    const long long n = cP.get();
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void k_tl_pair_rows(VarRange R, MDDev D, SGDev S, const long long* sgStart, const int2* pair,
+                               const long long* candOff, const unsigned char* flag, float* X, float* Y) {
+    long long b0, e0;
+    R.get(b0, e0);
+    const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e0) return;
     float f[mlp::kIn];
     pairFeatures<V>(i, pair, sgStart, D, S, f);
-    for (int k = 0; k < mlp::kIn; ++k) X[(row0 + i) * mlp::kIn + k] = f[k];
+    for (int k = 0; k < mlp::kIn; ++k) X[i * mlp::kIn + k] = f[k];
     bool any = false;
     for (long long c = candOff[i]; c < candOff[i + 1] && !any; ++c) any = flag[c] != 0;
-    Y[row0 + i] = any ? 1.f : 0.f;
+    Y[i] = any ? 1.f : 0.f;
 }
 
 // First stage, one thread per pair: carry = (sdlCut, thetaMuls, sin of the
 // midpoint deltaPhi); nOut = number of outer SGs to try (0 if the pair failed).
 template <int V>
-__global__ void __launch_bounds__(kBlock) k_tl_stage1(Count cP, MDDev D, SGDev S, const long long* sgStart,
+__global__ void __launch_bounds__(kBlock) k_tl_stage1(VarRange R, MDDev D, SGDev S, const long long* sgStart,
                                                       const int2* pair, const unsigned char* keep, float4* carry,
                                                       int* nOut) {
-    const long long n = cP.get();
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    long long b0, e0;
+    R.get(b0, e0);
+    const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e0) return;
     if (keep && !keep[i]) {  // dropped by the learned proposal
         nOut[i] = 0;
         return;
@@ -1140,12 +1175,13 @@ __global__ void k_tl_expand(Count cP, const long long* sgStart, const int2* pair
 
 // Second stage, one thread per candidate; survivors are counted per inner SG.
 template <int V>
-__global__ void __launch_bounds__(kBlock) k_tl_stage2(Count cC, HitsDev H, MDDev D, SGDev S, const int2* cand,
+__global__ void __launch_bounds__(kBlock) k_tl_stage2(VarRange R, HitsDev H, MDDev D, SGDev S, const int2* cand,
                                                       const int2* pair, const float4* carry, unsigned char* flag,
                                                       int* tlCount) {
-    const long long n = cC.get();
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    long long b0, e0;
+    R.get(b0, e0);
+    const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e0) return;
     const int2 c = cand[i];
     const int2 pr = pair[c.x];
     const float4 cf = carry[c.x];
@@ -1158,6 +1194,16 @@ __global__ void __launch_bounds__(kBlock) k_tl_stage2(Count cC, HitsDev H, MDDev
     const bool pass = tlStage2(V, (flo & 2) != 0, I, L, U, cc);
     flag[i] = pass ? 1 : 0;
     if (pass) atomicAdd(&tlCount[pr.x], 1);
+}
+
+// Pairs and candidates per variant, for the next call's size predictions.
+__global__ void k_tl_bounds(const long long* pairOff, const long long* candOff, long long stride, long long* tot) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || tot[TOT_ABORT]) return;
+    for (int v = 0; v < kNVariantsTL; ++v) {
+        const long long p0 = pairOff[v * stride], p1 = pairOff[(v + 1) * stride];
+        tot[TOT_TLP0 + v] = p1 - p0;
+        tot[TOT_TLC0 + v] = candOff[p1] - candOff[p0];
+    }
 }
 
 // Survivors to their inner SG's range (the order inside a range is fixed
@@ -1267,34 +1313,100 @@ __device__ __forceinline__ int layerBin(const ModDev& m) {
 constexpr int kNCounters = 10;
 constexpr int kNBins = 11;
 
-// counters[(e * 10 + which) * 11 + bin] += value of each item (1 when vals is
-// null); item -> slot via slotOf, or the item is a slot itself when slotOf is
-// null.  Bins are pre-aggregated in LDS, one global atomic per touched bin.
+// All per-layer counters of the stages in `stages` (bit s = stage s), one
+// grid-stride pass over slots, MDs, SGs, TPs and TLs (Event.cc:1169-1344).  A
+// candidate or object counts at the lower module of its innermost MD (TCs
+// from triplets are counted where they are written).  Bins are pre-aggregated
+// in LDS, one global atomic per touched bin and block.
 constexpr int kMaxLdsBins = 4096;
 
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_accumulate(Count cn, const T* vals, const int* slotOf, int NM, int E,
-                                                       const ModDev* mods, int which, unsigned long long* counters) {
-    const long long n = cn.get();
+struct CounterIn {
+    int NS, NM, E, stages;
+    const ModDev* mods;
+    const int *connStart, *connList;
+    const int *mdCap, *mdCount;                              // per slot
+    const int *mdSlot, *sgCap, *sgCount;                     // per MD
+    const int *sgIn, *sgOut, *tpCount, *tlCount;             // per SG
+    const long long* tlCap;                                  // per SG
+    const long long *slotSgStart, *slotTpStart, *slotTlStart;
+    const int *tpIn, *tpOut;                                 // per TP
+    const int *tlIn, *tclCount;                              // per TL
+};
+
+__device__ __forceinline__ long long connected(const CounterIn& C, int slot, const long long* slotStart) {
+    const int m = slot % C.NM, eb = slot - m;
+    long long c = 0;
+    for (int k = C.connStart[m]; k < C.connStart[m + 1]; ++k) {
+        const int t = eb + C.connList[k];
+        c += slotStart[t + 1] - slotStart[t];
+    }
+    return c;
+}
+
+__global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long long* tot, unsigned long long* counters) {
     __shared__ unsigned long long bins[kMaxLdsBins];
-    const int nb = E * kNBins;
+    const int nb = C.E * kNCounters * kNBins;
     const bool lds = nb <= kMaxLdsBins;
     if (lds)
         for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0;
     __syncthreads();
+    const bool aborted = tot[TOT_ABORT] != 0;
+    const long long nSlot = aborted || !(C.stages & 1) ? 0 : C.NS;
+    const long long nMd = aborted || !(C.stages & 2) ? 0 : tot[TOT_MD];
+    const long long nSg = aborted || !(C.stages & (4 | 8)) ? 0 : tot[TOT_SG];
+    const long long nTp = aborted || !(C.stages & 16) ? 0 : tot[TOT_TP];
+    const long long nTl = aborted || !(C.stages & 32) ? 0 : tot[TOT_TL];
+    const long long n = nSlot + nMd + nSg + nTp + nTl;
+    auto add = [&](int slot, int which, long long v) {
+        if (v == 0) return;
+        const int e = slot / C.NM, bin = layerBin(C.mods[slot % C.NM]);
+        const int k = (e * kNCounters + which) * kNBins + bin;
+        if (lds) atomicAdd(&bins[k], (unsigned long long)v);
+        else atomicAdd(&counters[k], (unsigned long long)v);
+    };
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const long long v = vals ? (long long)vals[i] : 1;
-        if (v == 0) continue;
-        const int s = slotOf ? slotOf[i] : int(i);
-        const int e = s / NM;
-        const int bin = layerBin(mods[s % NM]);
-        if (lds) atomicAdd(&bins[e * kNBins + bin], (unsigned long long)v);
-        else atomicAdd(&counters[(e * kNCounters + which) * kNBins + bin], (unsigned long long)v);
+        long long j = i;
+        if (j < nSlot) {  // MD candidates / objects per lower slot (Event.cc:257-305)
+            add(int(j), 0, C.mdCap[j]);
+            add(int(j), 5, C.mdCount[j]);
+            continue;
+        }
+        j -= nSlot;
+        if (j < nMd) {  // SG candidates / objects per inner MD (Event.cc:457-522)
+            const int slot = C.mdSlot[j];
+            add(slot, 1, C.sgCap[j]);
+            add(slot, 6, C.sgCount[j]);
+            continue;
+        }
+        j -= nMd;
+        if (j < nSg) {  // TP (Event.cc:539-597) and TL (Event.cc:627-693) per inner SG
+            const int slot = C.mdSlot[C.sgIn[j]];
+            if (C.stages & 4) {
+                add(slot, 2, connected(C, slot, C.slotSgStart));
+                add(slot, 7, C.tpCount[j]);
+            }
+            if (C.stages & 8) {
+                add(slot, 3, C.tlCap[j]);
+                add(slot, 8, C.tlCount[j]);
+            }
+            continue;
+        }
+        j -= nSg;
+        if (j < nTp) {  // TC candidates per inner TP: TPs at modules connected to its outermost MD (Event.cc:889-982)
+            add(C.mdSlot[C.sgIn[C.tpIn[j]]], 4, connected(C, C.mdSlot[C.sgOut[C.tpOut[j]]], C.slotTpStart));
+            continue;
+        }
+        j -= nTp;
+        {  // TC candidates and objects per inner TL (Event.cc:999-1062)
+            const int slot = C.mdSlot[C.sgIn[C.tlIn[j]]];
+            add(slot, 4, connected(C, C.mdSlot[C.sgOut[C.tlIn[j]]], C.slotTlStart));
+            add(slot, 9, C.tclCount[j]);
+        }
     }
     __syncthreads();
     if (lds)
         for (int b = threadIdx.x; b < nb; b += blockDim.x)
-            if (bins[b]) atomicAdd(&counters[((b / kNBins) * kNCounters + which) * kNBins + (b % kNBins)], bins[b]);
+            if (bins[b]) atomicAdd(&counters[b], bins[b]);
 }
 
 // TCs from triplets, one wave per tracklet (grid-stride over tracklets): the
@@ -1333,62 +1445,6 @@ __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long
             if (bins[b]) atomicAdd(&counters[((b / kNBins) * kNCounters + 9) * kNBins + (b % kNBins)], bins[b]);
 }
 
-// TP candidates of an inner SG at module A: SGs stored at every module connected to A (Event.cc:539-597)
-__global__ void k_tp_cand(Count cSG, int NM, const int* sgIn, const int* mdSlot, const int* connStart,
-                          const int* connList, const long long* slotSgStart, long long* cand) {
-    const int nSG = int(cSG.get());
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nSG) return;
-    const int a = mdSlot[sgIn[s]];
-    const int m = a % NM, eb = a - m;
-    long long c = 0;
-    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
-        const int t = eb + connList[k];
-        c += slotSgStart[t + 1] - slotSgStart[t];
-    }
-    cand[s] = c;
-}
-// TC candidates: TC(TP) per inner TP (TPs at modules connected to its outermost MD
-// module, Event.cc:889-982) and TC(TL) per inner TL (TLs at modules connected to its
-// second MD module, Event.cc:999-1062).
-__global__ void k_tc_cand_tp(Count cTP, int NM, const int* tpIn, const int* tpOutArr, const int* sgIn,
-                             const int* sgOut, const int* mdSlot, const int* connStart, const int* connList,
-                             const long long* slotTpStart, long long* cand, int* slotOfItem) {
-    const long long nTP = cTP.get();
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nTP) return;
-    const int c = mdSlot[sgOut[tpOutArr[t]]];
-    const int m = c % NM, eb = c - m;
-    long long n = 0;
-    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
-        const int d = eb + connList[k];
-        n += slotTpStart[d + 1] - slotTpStart[d];
-    }
-    cand[t] = n;
-    slotOfItem[t] = mdSlot[sgIn[tpIn[t]]];
-}
-__global__ void k_tc_cand_tl(Count cTL, int NM, const int* tlIn, const int* sgIn, const int* sgOut,
-                             const int* mdSlot, const int* connStart, const int* connList,
-                             const long long* slotTlStart, long long* cand, int* slotOfItem) {
-    const long long nTL = cTL.get();
-    const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nTL) return;
-    const int b = mdSlot[sgOut[tlIn[l]]];
-    const int m = b % NM, eb = b - m;
-    long long n = 0;
-    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
-        const int d = eb + connList[k];
-        n += slotTlStart[d + 1] - slotTlStart[d];
-    }
-    cand[l] = n;
-    slotOfItem[l] = mdSlot[sgIn[tlIn[l]]];
-}
-__global__ void k_gather_slot(Count cn, const int* idxA, const int* idxB, const int* mdSlot, int* out) {
-    const long long n = cn.get();
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[i] = mdSlot[idxB ? idxB[idxA[i]] : idxA[i]];
-}
 
 // Per-slot object starts for objects ordered by inner SG: slotStart[s] = start[slotSgStart[s]]
 __global__ void k_slot_range_sg(Count cNS, const long long* slotSgStart, const long long* objStartBySg,
@@ -1596,14 +1652,12 @@ struct sdl_event {
     DBuf<int> tlCount, tlIn, tlOut;
     DBuf<int> slotMdSg, slotMdSgList, tlNP, tlFill;
     DBuf<long long> slotMdSgStart;
-    DBuf<long long> tlPOff[kNVariantsTL];
-    DBuf<int2> tlPair[kNVariantsTL];
-    DBuf<float4> tlCarry[kNVariantsTL];
-    DBuf<int> tlNOut[kNVariantsTL];
-    DBuf<long long> tlCOff[kNVariantsTL];
-    DBuf<int2> tlCand[kNVariantsTL];
-    DBuf<unsigned char> tlFlag[kNVariantsTL];
-    DBuf<unsigned char> tlKeep[kNVariantsTL];
+    // tracklet pairs and candidates, one list each, ordered by variant
+    DBuf<long long> tlPOff, tlCOff;
+    DBuf<int2> tlPair, tlCand;
+    DBuf<float4> tlCarry;
+    DBuf<int> tlNOut;
+    DBuf<unsigned char> tlFlag, tlKeep;
     // learned proposal (null: the plain chain)
     sdl_mlp* proposal = nullptr;
     float proposalLogit = 0.f;
@@ -1611,8 +1665,6 @@ struct sdl_event {
     DBuf<int2> tcpInfo;
     DBuf<long long> tcpStart, tclStart;
     DBuf<int2> tcp, tcl;
-    DBuf<long long> cand;
-    DBuf<int> itemSlot;
     DBuf<unsigned long long> counters;
     DBuf<long long> tmp64;
     DBuf<int> exportBuf;
@@ -1773,7 +1825,7 @@ long long sizeOf(sdl_event* ev, int k) {
         static const int stageOf[TOT_N] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                            SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
                                            SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL,
-                                           SDL_STAGE_TL, 0, 0};
+                                           SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, 0, 0};
         const double scale = ev->lastNH > 0 ? double(ev->NH) / double(ev->lastNH) : 1.0;
         const bool custom = stageOf[k] >= ev->headroomFrom;
         const double f = custom ? ev->headroomFactor : 1.25;
@@ -1783,13 +1835,6 @@ long long sizeOf(sdl_event* ev, int k) {
     return v;
 }
 
-template <class T>
-void accumulate(sdl_event* ev, Count cn, long long nMax, const T* vals, const int* slotOf, int which) {
-    if (nMax <= 0) return;
-    const unsigned grid = std::min<unsigned>(gridFor(nMax, kBlock), 1024u);
-    hipLaunchKernelGGL(k_accumulate<T>, dim3(grid), dim3(kBlock), 0, ev->st, cn, vals, slotOf, ev->g->NM, ev->E,
-                       ev->g->dMods, which, ev->counters.p);
-}
 
 // Hit ingestion: module lookup, derived quantities, counting sort by slot.
 // Hits on modules unknown to the geometry are counted (TOT_UNKNOWN) and left out.
@@ -1881,8 +1926,6 @@ int runMD(sdl_event* ev) {
         hipLaunchKernelGGL(k_md_repack, dim3(gridFor(slab, kBlock)), dim3(kBlock), 0, st, devCount(ev, TOT_MDCAP),
                            G->dMods, G->NM, H, ev->mdFlag.p, ev->mdPos.p, ev->mdSlabHits.p, ev->mdSlabF.p,
                            ev->mdSlabSlot.p, mdView(ev), guardOf(ev, TOT_MD, nMD, SDL_STAGE_MD));
-    accumulate<int>(ev, hostCount(ev, NS), NS, ev->mdCap.p, nullptr, 0);
-    accumulate<int>(ev, hostCount(ev, NS), NS, ev->mdCount.p, nullptr, 5);
     return 0;
 }
 
@@ -1946,16 +1989,9 @@ int runSG(sdl_event* ev) {
         hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, cMD,
                            ev->sgEndStart.p, ev->sgEndList.p);
     }
-    accumulate<int>(ev, cMD, nMD, ev->sgCap.p, ev->mdSlot.p, 1);
-    accumulate<int>(ev, cMD, nMD, ev->sgCount.p, ev->mdSlot.p, 6);
     return 0;
 }
 
-// The per-item scratch (candidate counts, slots) serves SG-, TP- and TL-indexed passes.
-int ensureItemScratch(sdl_event* ev, long long n) {
-    if (int r = ev->cand.ensure(std::max(n, 1LL))) return r;
-    return ev->itemSlot.ensure(std::max(n, 1LL));
-}
 
 int runTP(sdl_event* ev) {
     sdl_geom* G = ev->g;
@@ -1981,17 +2017,20 @@ int runTP(sdl_event* ev) {
                            ev->sgStart.p, ev->tpStart.p, ev->tpIn.p, ev->tpOut.p, guardOf(ev, TOT_TP, nTP, SDL_STAGE_TP));
     hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->slotSgStart.p, ev->tpStart.p, ev->slotTpStart.p);
-    // counters: candidates per inner SG, objects per inner SG
-    if (int r = ensureItemScratch(ev, nSG)) return r;
-    if (nSG > 0) {
-        hipLaunchKernelGGL(k_tp_cand, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, ev->sgIn.p,
-                           ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->cand.p);
-        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgIn.p,
-                           (const int*)nullptr, ev->mdSlot.p, ev->itemSlot.p);
-        accumulate<long long>(ev, cSG, nSG, ev->cand.p, ev->itemSlot.p, 2);
-        accumulate<int>(ev, cSG, nSG, ev->tpCount.p, ev->itemSlot.p, 7);
-    }
     return 0;
+}
+
+// Host-side item count of variant v in a variant-ordered list: read back
+// (exact mode) or predicted (speculative; the kernels check it).
+long long variantSize(sdl_event* ev, int totK, const long long* off1, const long long* off2, long long stride, int v) {
+    if (ev->spec) return sizeOf(ev, totK);
+    long long p0 = readLL(off1 + v * stride, ev->st), p1 = readLL(off1 + (v + 1) * stride, ev->st);
+    if (p0 < 0 || p1 < 0) return -1;
+    if (off2) {
+        p0 = readLL(off2 + p0, ev->st);
+        p1 = readLL(off2 + p1, ev->st);
+    }
+    return p1 - p0;
 }
 
 int runTL(sdl_event* ev) {
@@ -2001,15 +2040,13 @@ int runTL(sdl_event* ev) {
     const long long ns = std::max(nSG, 1LL);
     const int NS = ev->E * G->NM;
     if (int r = ev->tlCap.ensure(ns)) return r;
-    if (int r = ev->tlCapOff.ensure(ns + 1)) return r;
     if (int r = ev->tlCount.ensure(ns)) return r;
     if (int r = ev->tlFill.ensure(ns)) return r;
     if (int r = ev->tlStart.ensure(ns + 1)) return r;
     if (int r = ev->slotTlStart.ensure(NS + 1)) return r;
     if (int r = ev->slotMdSg.ensure(NS)) return r;
     if (int r = ev->tlNP.ensure(kNVariantsTL * ns)) return r;
-    for (int v = 0; v < kNVariantsTL; ++v)
-        if (int r = ev->tlPOff[v].ensure(ns + 1)) return r;
+    if (int r = ev->tlPOff.ensure(kNVariantsTL * ns + 1)) return r;
     const Count cSG = devCount(ev, TOT_SG);
     timerBegin(ev, T_TL);
     hipLaunchKernelGGL(k_slot_md_sg, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS), ev->mdStart.p,
@@ -2021,125 +2058,115 @@ int runTL(sdl_event* ev) {
         return r;
     hipLaunchKernelGGL(k_slot_md_sg_list, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->mdStart.p, ev->sgStart.p, ev->slotMdSgStart.p, ev->slotMdSgList.p);
-    if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
-                           ev->sgOut.p, ev->sgFlags.p, ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p,
-                           ev->slotMdSg.p, ev->tlCap.p, ev->tlNP.p, ns);
-    if (int r = scanExclusive<long long>(ev->tlCap.p, ev->tlCapOff.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TLCAP))
+    hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(ns, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods, ev->sgOut.p,
+                       ev->sgFlags.p, ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->slotMdSg.p,
+                       ev->tlCap.p, ev->tlNP.p, ns);
+    // pairs: one list, variant-major
+    if (int r = scanExclusive<int>(ev->tlNP.p, ev->tlPOff.p, hostCount(ev, kNVariantsTL * ns), kNVariantsTL * ns,
+                                   ev->scan, st, ev->tot.p, TOT_TLPAIRS))
         return r;
-    // pairs per variant
-    long long np[kNVariantsTL];
-    TLPairs P;
-    for (int v = 0; v < kNVariantsTL; ++v)
-        if (int r = scanExclusive<int>(ev->tlNP.p + v * ns, ev->tlPOff[v].p, cSG, nSG, ev->scan, st, ev->tot.p,
-                                       TOT_TLP0 + v))
-            return r;
-    for (int v = 0; v < kNVariantsTL; ++v) {
-        np[v] = sizeOf(ev, TOT_TLP0 + v);
-        if (np[v] < 0) return fail(SDL_ERR_HIP, "readback failed");
-        const long long n = std::max(np[v], 1LL);
-        if (int r = ev->tlPair[v].ensure(n)) return r;
-        if (int r = ev->tlCarry[v].ensure(n)) return r;
-        if (int r = ev->tlNOut[v].ensure(n)) return r;
-        if (int r = ev->tlCOff[v].ensure(n + 1)) return r;
-        P.off[v] = ev->tlPOff[v].p;
-        P.pair[v] = ev->tlPair[v].p;
-    }
+    const long long nPairs = sizeOf(ev, TOT_TLPAIRS);
+    if (nPairs < 0) return fail(SDL_ERR_HIP, "readback failed");
+    const long long npc = std::max(nPairs, 1LL);
+    if (int r = ev->tlPair.ensure(npc)) return r;
+    if (int r = ev->tlCarry.ensure(npc)) return r;
+    if (int r = ev->tlNOut.ensure(npc)) return r;
+    if (int r = ev->tlCOff.ensure(npc + 1)) return r;
     if (nSG > 0)
         hipLaunchKernelGGL(k_tl_enum, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
                            ev->sgOut.p, ev->sgFlags.p, ev->mdSlot.p, G->dConnStart, G->dConnList,
-                           ev->slotMdSgStart.p, ev->slotMdSgList.p, P, guardOf(ev, TOT_TLP0, np[0], SDL_STAGE_TL),
-                           guardOf(ev, TOT_TLP1, np[1], SDL_STAGE_TL), guardOf(ev, TOT_TLP2, np[2], SDL_STAGE_TL));
+                           ev->slotMdSgStart.p, ev->slotMdSgList.p, ev->tlPOff.p, ns, ev->tlPair.p,
+                           guardOf(ev, TOT_TLPAIRS, nPairs, SDL_STAGE_TL));
+    long long np[kNVariantsTL];
+    for (int v = 0; v < kNVariantsTL; ++v) {
+        np[v] = variantSize(ev, TOT_TLP0 + v, ev->tlPOff.p, nullptr, ns, v);
+        if (np[v] < 0) return fail(SDL_ERR_HIP, "readback failed");
+    }
     // first stage per pair, after the learned proposal when one is set
     const MDDev D = mdView(ev);
     const SGDev S = sgView(ev);
     const bool learned = ev->proposal != nullptr;
+    if (learned)
+        if (int r = ev->tlKeep.ensure(npc)) return r;
+    const unsigned char* keep = learned ? ev->tlKeep.p : nullptr;
     for (int v = 0; v < kNVariantsTL; ++v) {
-        if (learned)
-            if (int r = ev->tlKeep[v].ensure(std::max(np[v], 1LL))) return r;
         if (np[v] == 0) continue;
-        const Count cP = devCount(ev, TOT_TLP0 + v);
-        const unsigned char* keep = learned ? ev->tlKeep[v].p : nullptr;
+        const VarRange R{ev->tlPOff.p, nullptr, ns, v, np[v], ev->tot.p};
         if (learned) {
-            const unsigned grid = std::min<unsigned>(gridFor((np[v] + 15) / 16, kWavesPerBlock), 8192u);
+            const unsigned grid = std::min<unsigned>(gridFor((np[v] + 63) / 64, kWavesPerBlock), 8192u);
             if (v == 0)
-                hipLaunchKernelGGL(k_tl_score<TL_BBBB>, dim3(grid), dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
-                                   ev->tlPair[0].p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep[0].p);
+                hipLaunchKernelGGL(k_tl_score<TL_BBBB>, dim3(grid), dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
+                                   ev->tlPair.p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep.p);
             else if (v == 1)
-                hipLaunchKernelGGL(k_tl_score<TL_BBEE>, dim3(grid), dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
-                                   ev->tlPair[1].p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep[1].p);
+                hipLaunchKernelGGL(k_tl_score<TL_BBEE>, dim3(grid), dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
+                                   ev->tlPair.p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep.p);
             else
-                hipLaunchKernelGGL(k_tl_score<TL_EEEE>, dim3(grid), dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
-                                   ev->tlPair[2].p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep[2].p);
+                hipLaunchKernelGGL(k_tl_score<TL_EEEE>, dim3(grid), dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
+                                   ev->tlPair.p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep.p);
         }
         const dim3 grid(gridFor(np[v], kBlock));
         if (v == 0)
-            hipLaunchKernelGGL(k_tl_stage1<TL_BBBB>, grid, dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
-                               ev->tlPair[0].p, keep, ev->tlCarry[0].p, ev->tlNOut[0].p);
+            hipLaunchKernelGGL(k_tl_stage1<TL_BBBB>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
+                               keep, ev->tlCarry.p, ev->tlNOut.p);
         else if (v == 1)
-            hipLaunchKernelGGL(k_tl_stage1<TL_BBEE>, grid, dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
-                               ev->tlPair[1].p, keep, ev->tlCarry[1].p, ev->tlNOut[1].p);
+            hipLaunchKernelGGL(k_tl_stage1<TL_BBEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
+                               keep, ev->tlCarry.p, ev->tlNOut.p);
         else
-            hipLaunchKernelGGL(k_tl_stage1<TL_EEEE>, grid, dim3(kBlock), 0, st, cP, D, S, ev->sgStart.p,
-                               ev->tlPair[2].p, keep, ev->tlCarry[2].p, ev->tlNOut[2].p);
+            hipLaunchKernelGGL(k_tl_stage1<TL_EEEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
+                               keep, ev->tlCarry.p, ev->tlNOut.p);
     }
-    // candidates of the surviving pairs
+    // candidates of the surviving pairs: one list, variant-major
+    const Count cPairs = devCount(ev, TOT_TLPAIRS);
+    if (int r = scanExclusive<int>(ev->tlNOut.p, ev->tlCOff.p, cPairs, nPairs, ev->scan, st, ev->tot.p, TOT_TLCANDS))
+        return r;
+    const long long nCands = sizeOf(ev, TOT_TLCANDS);
+    if (nCands < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->tlCand.ensure(std::max(nCands, 1LL))) return r;
+    if (int r = ev->tlFlag.ensure((std::max(nCands, 1LL) + 3) / 4 * 4)) return r;
+    if (nPairs > 0)
+        hipLaunchKernelGGL(k_tl_expand, dim3(gridFor(nPairs, kBlock)), dim3(kBlock), 0, st, cPairs, ev->sgStart.p,
+                           ev->tlPair.p, ev->tlNOut.p, ev->tlCOff.p, ev->tlCand.p,
+                           guardOf(ev, TOT_TLCANDS, nCands, SDL_STAGE_TL));
     long long nc[kNVariantsTL];
-    for (int v = 0; v < kNVariantsTL; ++v)
-        if (int r = scanExclusive<int>(ev->tlNOut[v].p, ev->tlCOff[v].p, devCount(ev, TOT_TLP0 + v), np[v], ev->scan,
-                                       st, ev->tot.p, TOT_TLC0 + v))
-            return r;
     for (int v = 0; v < kNVariantsTL; ++v) {
-        nc[v] = sizeOf(ev, TOT_TLC0 + v);
+        nc[v] = variantSize(ev, TOT_TLC0 + v, ev->tlPOff.p, ev->tlCOff.p, ns, v);
         if (nc[v] < 0) return fail(SDL_ERR_HIP, "readback failed");
-        if (int r = ev->tlCand[v].ensure(std::max(nc[v], 1LL))) return r;
-        if (int r = ev->tlFlag[v].ensure((std::max(nc[v], 1LL) + 3) / 4 * 4)) return r;
-        if (np[v] > 0)
-            hipLaunchKernelGGL(k_tl_expand, dim3(gridFor(np[v], kBlock)), dim3(kBlock), 0, st,
-                               devCount(ev, TOT_TLP0 + v), ev->sgStart.p, ev->tlPair[v].p, ev->tlNOut[v].p,
-                               ev->tlCOff[v].p, ev->tlCand[v].p, guardOf(ev, TOT_TLC0 + v, nc[v], SDL_STAGE_TL));
     }
     // second stage per candidate
     zeroAsync(ev->tlCount.p, ns, st);
     zeroAsync(ev->tlFill.p, ns, st);
     const HitsDev H = hitsView(ev);
-    if (nc[0] > 0)
-        hipLaunchKernelGGL(k_tl_stage2<TL_BBBB>, dim3(gridFor(nc[0], kBlock)), dim3(kBlock), 0, st,
-                           devCount(ev, TOT_TLC0), H, D, S, ev->tlCand[0].p, ev->tlPair[0].p, ev->tlCarry[0].p,
-                           ev->tlFlag[0].p, ev->tlCount.p);
-    if (nc[1] > 0)
-        hipLaunchKernelGGL(k_tl_stage2<TL_BBEE>, dim3(gridFor(nc[1], kBlock)), dim3(kBlock), 0, st,
-                           devCount(ev, TOT_TLC1), H, D, S, ev->tlCand[1].p, ev->tlPair[1].p, ev->tlCarry[1].p,
-                           ev->tlFlag[1].p, ev->tlCount.p);
-    if (nc[2] > 0)
-        hipLaunchKernelGGL(k_tl_stage2<TL_EEEE>, dim3(gridFor(nc[2], kBlock)), dim3(kBlock), 0, st,
-                           devCount(ev, TOT_TLC2), H, D, S, ev->tlCand[2].p, ev->tlPair[2].p, ev->tlCarry[2].p,
-                           ev->tlFlag[2].p, ev->tlCount.p);
+    for (int v = 0; v < kNVariantsTL; ++v) {
+        if (nc[v] == 0) continue;
+        const VarRange R{ev->tlPOff.p, ev->tlCOff.p, ns, v, nc[v], ev->tot.p};
+        const dim3 grid(gridFor(nc[v], kBlock));
+        if (v == 0)
+            hipLaunchKernelGGL(k_tl_stage2<TL_BBBB>, grid, dim3(kBlock), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
+                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p);
+        else if (v == 1)
+            hipLaunchKernelGGL(k_tl_stage2<TL_BBEE>, grid, dim3(kBlock), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
+                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p);
+        else
+            hipLaunchKernelGGL(k_tl_stage2<TL_EEEE>, grid, dim3(kBlock), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
+                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p);
+    }
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
     const long long nTL = sizeOf(ev, TOT_TL);
     if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
     ev->nTL = nTL;
     if (int r = ev->tlIn.ensure(std::max(nTL, 1LL))) return r;
     if (int r = ev->tlOut.ensure(std::max(nTL, 1LL))) return r;
-    for (int v = 0; v < kNVariantsTL; ++v)
-        if (nc[v] > 0)
-            hipLaunchKernelGGL(k_tl_scatter, dim3(gridFor((nc[v] + 3) / 4, kBlock)), dim3(kBlock), 0, st,
-                               devCount(ev, TOT_TLC0 + v), ev->tlCand[v].p, ev->tlPair[v].p, ev->tlFlag[v].p,
-                               ev->tlStart.p, ev->tlFill.p, ev->tlIn.p, ev->tlOut.p,
-                               guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
+    if (nCands > 0)
+        hipLaunchKernelGGL(k_tl_scatter, dim3(gridFor((nCands + 3) / 4, kBlock)), dim3(kBlock), 0, st,
+                           devCount(ev, TOT_TLCANDS), ev->tlCand.p, ev->tlPair.p, ev->tlFlag.p, ev->tlStart.p,
+                           ev->tlFill.p, ev->tlIn.p, ev->tlOut.p, guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
     if (nSG > 0)
         hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG,
                            ev->tlStart.p, ev->tlOut.p);
+    hipLaunchKernelGGL(k_tl_bounds, dim3(1), dim3(64), 0, st, ev->tlPOff.p, ev->tlCOff.p, ns, ev->tot.p);
     timerEnd(ev, T_TL);
     hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->slotSgStart.p, ev->tlStart.p, ev->slotTlStart.p);
-    if (int r = ensureItemScratch(ev, nSG)) return r;
-    if (nSG > 0) {
-        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgIn.p,
-                           (const int*)nullptr, ev->mdSlot.p, ev->itemSlot.p);
-        accumulate<long long>(ev, cSG, nSG, ev->tlCap.p, ev->itemSlot.p, 3);
-        accumulate<int>(ev, cSG, nSG, ev->tlCount.p, ev->itemSlot.p, 8);
-    }
     return 0;
 }
 
@@ -2176,13 +2203,6 @@ int runTCP(sdl_event* ev) {
         hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(nTL, kWavesPerBlock), 4096u)), dim3(kBlock), 0, st, cTL,
                            ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p, ev->sgIn.p, ev->mdSlot.p, G->NM,
                            ev->E, G->dMods, ev->tcp.p, ev->counters.p, guardOf(ev, TOT_TCP, nTCP, SDL_STAGE_TCP));
-    if (int r = ensureItemScratch(ev, nTP)) return r;
-    if (nTP > 0) {
-        hipLaunchKernelGGL(k_tc_cand_tp, dim3(gridFor(nTP, kBlock)), dim3(kBlock), 0, st, cTP, G->NM, ev->tpIn.p,
-                           ev->tpOut.p, ev->sgIn.p, ev->sgOut.p, ev->mdSlot.p, G->dConnStart, G->dConnList,
-                           ev->slotTpStart.p, ev->cand.p, ev->itemSlot.p);
-        accumulate<long long>(ev, cTP, nTP, ev->cand.p, ev->itemSlot.p, 4);
-    }
     return 0;
 }
 
@@ -2202,18 +2222,9 @@ int runTCL(sdl_event* ev) {
     if (nTCL < 0) return fail(SDL_ERR_HIP, "readback failed");
     ev->nTCL = nTCL;
     if (int r = ev->tcl.ensure(std::max(nTCL, 1LL))) return r;
-    if (int r = ensureItemScratch(ev, nTL)) return r;
-    if (nTL > 0) {
+    if (nTL > 0)
         hipLaunchKernelGGL(k_tcl_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlOut.p,
                            ev->tlStart.p, ev->tclStart.p, ev->tcl.p, guardOf(ev, TOT_TCL, nTCL, SDL_STAGE_TCL));
-        hipLaunchKernelGGL(k_gather_slot, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlIn.p,
-                           ev->sgIn.p, ev->mdSlot.p, ev->itemSlot.p);
-        accumulate<int>(ev, cTL, nTL, ev->tcCount.p, ev->itemSlot.p, 9);
-        hipLaunchKernelGGL(k_tc_cand_tl, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, G->NM, ev->tlIn.p,
-                           ev->sgIn.p, ev->sgOut.p, ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotTlStart.p,
-                           ev->cand.p, ev->itemSlot.p);
-        accumulate<long long>(ev, cTL, nTL, ev->cand.p, ev->itemSlot.p, 4);
-    }
     return 0;
 }
 
@@ -2229,12 +2240,35 @@ int runStage(sdl_event* ev, int stage) {
     }
 }
 
-// Enqueues hit preparation (if pending) and stages s0..s1.
+// Per-layer counters of stages s0..s1 (one pass, after the stages ran).
+void launchCounters(sdl_event* ev, int s0, int s1) {
+    sdl_geom* G = ev->g;
+    CounterIn C;
+    C.NS = ev->E * G->NM;
+    C.NM = G->NM;
+    C.E = ev->E;
+    C.stages = 0;
+    for (int s = s0; s <= s1; ++s) C.stages |= 1 << s;
+    C.mods = G->dMods;
+    C.connStart = G->dConnStart;
+    C.connList = G->dConnList;
+    C.mdCap = ev->mdCap.p; C.mdCount = ev->mdCount.p;
+    C.mdSlot = ev->mdSlot.p; C.sgCap = ev->sgCap.p; C.sgCount = ev->sgCount.p;
+    C.sgIn = ev->sgIn.p; C.sgOut = ev->sgOut.p; C.tpCount = ev->tpCount.p; C.tlCount = ev->tlCount.p;
+    C.tlCap = ev->tlCap.p;
+    C.slotSgStart = ev->slotSgStart.p; C.slotTpStart = ev->slotTpStart.p; C.slotTlStart = ev->slotTlStart.p;
+    C.tpIn = ev->tpIn.p; C.tpOut = ev->tpOut.p;
+    C.tlIn = ev->tlIn.p; C.tclCount = ev->tcCount.p;
+    hipLaunchKernelGGL(k_counters, dim3(2048), dim3(kBlock), 0, ev->st, C, ev->tot.p, ev->counters.p);
+}
+
+// Enqueues hit preparation (if pending), stages s0..s1 and their counters.
 int enqueueStages(sdl_event* ev, int s0, int s1) {
     if (!ev->hitsReady)
         if (int r = runPrep(ev)) return r;
     for (int s = s0; s <= s1; ++s)
         if (int r = runStage(ev, s)) return r;
+    launchCounters(ev, s0, s1);
     return 0;
 }
 
@@ -2327,6 +2361,7 @@ int runStages(sdl_event* ev, int s0, int s1) {
         zeroAsync(ev->tot.p + TOT_ABORT, 1, st);
         for (int s = std::max(from, s0); s <= s1; ++s)
             if (int r = runStage(ev, s)) return r;
+        launchCounters(ev, s0, s1);
         HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         if (t[TOT_ABORT] != 0) return fail(SDL_ERR_CAPACITY, "device capacity check failed on an exactly sized run");
@@ -2337,7 +2372,7 @@ int runStages(sdl_event* ev, int s0, int s1) {
     static const int stageOfTot[TOT_ABORT] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                               SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
                                               SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL,
-                                              SDL_STAGE_TL};
+                                              SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL};
     for (int k = 0; k < TOT_ABORT; ++k)
         if (stageOfTot[k] >= s0 && stageOfTot[k] <= s1) ev->lastTot[k] = t[k];
     if (s1 == SDL_STAGE_TCL) {
@@ -2765,7 +2800,7 @@ int sdl_mlp_train(sdl_mlp* m, const float* dX, const float* dY, const int* dIdx,
 int sdl_mlp_forward(sdl_mlp* m, const float* dX, long long n, float* dLogit) {
     if (!m || !dX || !dLogit || n < 0) return fail(SDL_ERR_ARG, "bad arguments");
     if (n == 0) return 0;
-    const unsigned grid = std::min<unsigned>(gridFor((n + 15) / 16, kWavesPerBlock), 8192u);
+    const unsigned grid = std::min<unsigned>(gridFor((n + 63) / 64, kWavesPerBlock), 8192u);
     hipLaunchKernelGGL(sdl::mlp::k_mlp_forward, dim3(grid), dim3(kBlock), 0, m->st, dX, n, m->dPacked, dLogit);
     HIP_OK(hipStreamSynchronize(m->st));
     return 0;
@@ -2778,6 +2813,17 @@ int sdl_event_set_proposal(sdl_event* ev, sdl_mlp* m, float threshold) {
     return 0;
 }
 
+int sdl_event_tl_work(sdl_event* ev, long long out[3]) {
+    if (!ev || !out) return fail(SDL_ERR_ARG, "bad arguments");
+    long long t[TOT_N];
+    HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, ev->st));
+    HIP_OK(hipStreamSynchronize(ev->st));
+    out[0] = t[TOT_TLPAIRS];
+    out[1] = t[TOT_TLCANDS];
+    out[2] = t[TOT_TL];
+    return 0;
+}
+
 int sdl_event_tl_pair_rows(sdl_event* ev, float* dX, float* dY, long long cap, long long* n) {
     if (!ev || !n) return fail(SDL_ERR_ARG, "bad arguments");
     if (ev->stage < SDL_STAGE_TL) return fail(SDL_ERR_STATE, "the tracklet stage has not run");
@@ -2785,29 +2831,27 @@ int sdl_event_tl_pair_rows(sdl_event* ev, float* dX, float* dY, long long cap, l
     long long t[TOT_N];
     HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, ev->st));
     HIP_OK(hipStreamSynchronize(ev->st));
-    const long long total = t[TOT_TLP0] + t[TOT_TLP1] + t[TOT_TLP2];
+    const long long total = t[TOT_TLPAIRS];
     *n = total;
     if (!dX) return 0;
     if (!dY || cap < total) return fail(SDL_ERR_CAPACITY, "row buffer too small");
     const MDDev D = mdView(ev);
     const SGDev S = sgView(ev);
-    long long row0 = 0;
+    const long long ns = std::max(ev->nSG, 1LL);
     for (int v = 0; v < kNVariantsTL; ++v) {
         const long long np = t[TOT_TLP0 + v];
-        if (np > 0) {
-            const dim3 grid(gridFor(np, kBlock));
-            const Count cP = devCount(ev, TOT_TLP0 + v);
-            if (v == 0)
-                hipLaunchKernelGGL(k_tl_pair_rows<TL_BBBB>, grid, dim3(kBlock), 0, ev->st, cP, D, S, ev->sgStart.p,
-                                   ev->tlPair[0].p, ev->tlCOff[0].p, ev->tlFlag[0].p, row0, dX, dY);
-            else if (v == 1)
-                hipLaunchKernelGGL(k_tl_pair_rows<TL_BBEE>, grid, dim3(kBlock), 0, ev->st, cP, D, S, ev->sgStart.p,
-                                   ev->tlPair[1].p, ev->tlCOff[1].p, ev->tlFlag[1].p, row0, dX, dY);
-            else
-                hipLaunchKernelGGL(k_tl_pair_rows<TL_EEEE>, grid, dim3(kBlock), 0, ev->st, cP, D, S, ev->sgStart.p,
-                                   ev->tlPair[2].p, ev->tlCOff[2].p, ev->tlFlag[2].p, row0, dX, dY);
-        }
-        row0 += np;
+        if (np == 0) continue;
+        const VarRange R{ev->tlPOff.p, nullptr, ns, v, np, ev->tot.p};
+        const dim3 grid(gridFor(np, kBlock));
+        if (v == 0)
+            hipLaunchKernelGGL(k_tl_pair_rows<TL_BBBB>, grid, dim3(kBlock), 0, ev->st, R, D, S, ev->sgStart.p,
+                               ev->tlPair.p, ev->tlCOff.p, ev->tlFlag.p, dX, dY);
+        else if (v == 1)
+            hipLaunchKernelGGL(k_tl_pair_rows<TL_BBEE>, grid, dim3(kBlock), 0, ev->st, R, D, S, ev->sgStart.p,
+                               ev->tlPair.p, ev->tlCOff.p, ev->tlFlag.p, dX, dY);
+        else
+            hipLaunchKernelGGL(k_tl_pair_rows<TL_EEEE>, grid, dim3(kBlock), 0, ev->st, R, D, S, ev->sgStart.p,
+                               ev->tlPair.p, ev->tlCOff.p, ev->tlFlag.p, dX, dY);
     }
     HIP_OK(hipStreamSynchronize(ev->st));
     return 0;

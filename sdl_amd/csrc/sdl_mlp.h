@@ -4,7 +4,8 @@
 // the exact cuts still decide every object, the network only drops pairs.
 //
 //  * inference: BF16 MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation,
-//    one wave per 16 pairs.  Layers are computed transposed, D = W . X^T, so a
+//    one wave per 64 pairs (four 16-pair MFMA tiles).  Layers are computed
+//    transposed, D = W . X^T, so a
 //    layer's accumulator fragment (rows 4q..4q+3 of a 16-neuron tile for the
 //    pair in column l % 16, q = l / 16) is exactly the B fragment the next layer
 //    needs for that tile's K step: activations never leave registers.
@@ -87,63 +88,76 @@ __device__ __forceinline__ void loadFrags(const Packed* P, int lane, Frags& F) {
     F.b4 = P->b4;
 }
 
-// Logit of the sample in column lane % 16.  f holds that sample's 24 features
-// (every lane of the column passes the same values).  All 64 lanes must call.
-__device__ __forceinline__ float logit16(const float* f, int lane, const Frags& F) {
-    const int q = lane >> 4;
-    // layer-1 B fragments: features 4q..4q+3 and 16+4q..16+4q+3 (24 is the constant 1)
-    float x1[4];
+// Logits of 64 samples, one per lane: each lane passes its sample's 24
+// features; they go through the wave's LDS tile (64 rows x 32 bf16, the
+// constant 1 at column 24 carries b1) so that every MFMA column is one sample.
+// All 64 lanes must call; tile is the wave's own 4 KB of LDS.
+__device__ __forceinline__ float logit64(const float* f, int lane, const Frags& F, short* tile) {
+    short* row = tile + lane * 32;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int k = 16 + 4 * q + v;
-        x1[v] = k < kIn ? f[k] : (k == kIn ? 1.f : 0.f);
+    for (int k = 0; k < 32; ++k) row[k] = bf16Bits(k < kIn ? f[k] : (k == kIn ? 1.f : 0.f));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int q = lane >> 4, c = lane & 15;
+    float mine = 0.f;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const short* src = tile + (st * 16 + c) * 32;
+        const short4v b0 = *reinterpret_cast<const short4v*>(src + 4 * q);
+        const short4v b1 = *reinterpret_cast<const short4v*>(src + 16 + 4 * q);
+        short4v h1[kTiles], h2[kTiles];
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+            float4v d = {0.f, 0.f, 0.f, 0.f};
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a1[t][0], b0, d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a1[t][1], b1, d, 0, 0, 0);
+            h1[t] = reluBf16(d);
+        }
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+            float4v d = F.c2[t];
+#pragma unroll
+            for (int k = 0; k < kTiles; ++k) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a2[t][k], h1[k], d, 0, 0, 0);
+            h2[t] = reluBf16(d);
+        }
+        float part = 0.f;
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+            float4v d = F.c3[t];
+#pragma unroll
+            for (int k = 0; k < kTiles; ++k) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a3[t][k], h2[k], d, 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) part += F.w4[t][v] * fmaxf(d[v], 0.f);
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (q == st) mine = part + F.b4;  // lane st * 16 + c holds sample st * 16 + c
     }
-    const short4v b0 = toBf16(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
-    const short4v b1 = toBf16(x1[0], x1[1], x1[2], x1[3]);
-    short4v h1[kTiles], h2[kTiles];
-#pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
-        float4v d = {0.f, 0.f, 0.f, 0.f};
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a1[t][0], b0, d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a1[t][1], b1, d, 0, 0, 0);
-        h1[t] = reluBf16(d);
-    }
-#pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
-        float4v d = F.c2[t];
-#pragma unroll
-        for (int k = 0; k < kTiles; ++k) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a2[t][k], h1[k], d, 0, 0, 0);
-        h2[t] = reluBf16(d);
-    }
-    float part = 0.f;
-#pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
-        float4v d = F.c3[t];
-#pragma unroll
-        for (int k = 0; k < kTiles; ++k) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a3[t][k], h2[k], d, 0, 0, 0);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) part += F.w4[t][v] * fmaxf(d[v], 0.f);
-    }
-    part += __shfl_xor(part, 16);
-    part += __shfl_xor(part, 32);
-    return part + F.b4;
+    // the tile is rewritten by the next call
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return mine;
 }
 
-// Inference on stored features (row-major n x 24): one wave per 16 rows.
+// Inference on stored features (row-major n x 24): one wave per 64 rows.
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, const Packed* P, float* logit) {
+    __shared__ short tiles[4][64 * 32];
     const int lane = threadIdx.x & 63;
+    short* tile = tiles[threadIdx.x >> 6];
     Frags F;
     loadFrags(P, lane, F);
-    const long long nTiles = (n + 15) / 16;
+    const long long nTiles = (n + 63) / 64;
     const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
-    for (long long tile = w0; tile < nTiles; tile += nw) {
-        const long long i = tile * 16 + (lane & 15);
+    for (long long t = w0; t < nTiles; t += nw) {
+        const long long i = t * 64 + lane;
         float f[kIn];
 #pragma unroll
         for (int k = 0; k < kIn; ++k) f[k] = i < n ? X[i * kIn + k] : 0.f;
-        const float z = logit16(f, lane, F);
-        if (lane < 16 && i < n) logit[i] = z;
+        const float z = logit64(f, lane, F, tile);
+        if (i < n) logit[i] = z;
     }
 }
 

@@ -24,7 +24,7 @@ EXPORTS = (
     "sdl_event_export", "sdl_event_num_events", "sdl_event_stream", "sdl_event_set_timing", "sdl_event_kernel_ms",
     "sdl_event_set_headroom", "sdl_event_set_graphs",
     "sdl_mlp_create", "sdl_mlp_free", "sdl_mlp_num_params", "sdl_mlp_set_params", "sdl_mlp_get_params",
-    "sdl_mlp_gradient", "sdl_mlp_train", "sdl_mlp_forward", "sdl_event_set_proposal", "sdl_event_tl_pair_rows",
+    "sdl_mlp_gradient", "sdl_mlp_train", "sdl_mlp_forward", "sdl_event_set_proposal", "sdl_event_tl_pair_rows", "sdl_event_tl_work",
 )
 TIMERS = ("prep", "md", "sg", "tl")
 
@@ -77,6 +77,7 @@ def lib():
         L.sdl_mlp_forward.argtypes = [vp, vp, ll, vp]
         L.sdl_event_set_proposal.argtypes = [vp, vp, f]
         L.sdl_event_tl_pair_rows.argtypes = [vp, vp, vp, ll, ctypes.POINTER(ll)]
+        L.sdl_event_tl_work.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -180,6 +181,12 @@ class Event:
     def set_headroom(self, from_stage, factor):
         """Headroom factor of the predicted buffer sizes for stages >= from_stage."""
         _check(lib().sdl_event_set_headroom(self.h, from_stage, factor))
+
+    def tl_work(self):
+        """(pairs, second-stage candidates, tracklets) of the last tracklet stage."""
+        out = np.zeros(3, np.int64)
+        _check(lib().sdl_event_tl_work(self.h, out.ctypes.data))
+        return dict(zip(("pairs", "stage2", "tracklets"), out.tolist()))
 
     def set_graphs(self, on=True):
         """Allow the speculative chain to be captured into a HIP graph and replayed."""

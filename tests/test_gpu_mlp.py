@@ -84,7 +84,10 @@ def test_proposal_only_removes_objects():
     ref, _ = oracle.run(gen.geometry(gen.geom_kind(wl))[1], e, oracle.CR)
     X, Y, n = P.pair_rows(ev)
     y = Y.to_host()[:n]
-    assert n > 0 and 0 < y.sum() <= len(plain["TL"])
+    assert n == ev.tl_work()["pairs"]
+    # a pair (inner SG, outer SG's inner MD) is positive iff some tracklet starts with it
+    positive_pairs = {tuple(t[:6]) for t in plain["TL"]}
+    assert int(y.sum()) == len(positive_pairs)
     # keep everything: identical to the plain chain
     keep_all = np.zeros(P.N_PARAMS, np.float32)
     keep_all[-1] = 10.0
