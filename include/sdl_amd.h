@@ -129,9 +129,10 @@ int sdl_event_set_headroom(sdl_event* ev, int fromStage, double factor);
 
 /* ---------------------------------------------------------------- learned proposal
  * No counterpart in the reference.  A 24-64-64-64-1 MLP scores the tracklet
- * stage's (inner segment, outer-inner mini-doublet) pairs before the exact
- * tracklet cuts; pairs scored below the threshold are dropped, every object
- * that is kept still passes the exact cuts.  Inference runs in BF16 MFMA with
+ * stage's (inner segment, outer-inner mini-doublet) pairs that passed the
+ * first-stage cuts, before the second-stage candidates are formed; pairs scored
+ * below the threshold are dropped, every object that is kept still passes all
+ * exact cuts.  Inference runs in BF16 MFMA with
  * FP32 accumulation; training runs FP32 forward and backward kernels with Adam.
  * Parameters: float32, layout W1[64][24], b1[64], W2[64][64], b2[64],
  * W3[64][64], b3[64], w4[64], b4 (sdl_mlp_num_params() values). */
@@ -154,12 +155,14 @@ int sdl_mlp_forward(sdl_mlp* m, const float* dX, long long n, float* dLogit);
  * m = NULL restores the plain chain. */
 int sdl_event_set_proposal(sdl_event* ev, sdl_mlp* m, float threshold);
 /* Training rows from the last plain chain: features (24 per pair, DEVICE dX)
- * and labels (1 if the pair yields a tracklet, DEVICE dY) of every pair; with
- * dX = NULL only *n (the number of pairs) is returned. */
+ * and labels (DEVICE dY: 1 if the pair yields a tracklet, 0 if it passed the
+ * first-stage cuts but yields none, -1 if it failed them and is never scored)
+ * of every pair; with dX = NULL only *n (the number of pairs) is returned. */
 int sdl_event_tl_pair_rows(sdl_event* ev, float* dX, float* dY, long long cap, long long* n);
 /* Diagnostics of the last tracklet stage: out[0] (inner segment, MD) pairs,
- * out[1] candidates that reached the second-stage cuts, out[2] tracklets. */
-int sdl_event_tl_work(sdl_event* ev, long long out[3]);
+ * out[1] candidates that reached the second-stage cuts, out[2] tracklets,
+ * out[3] pairs scored by the learned proposal (0 without one). */
+int sdl_event_tl_work(sdl_event* ev, long long out[4]);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,7 @@ enum {
     TOT_TLC0, TOT_TLC1, TOT_TLC2,  // second-stage tracklet candidates per variant
     TOT_TLP0, TOT_TLP1, TOT_TLP2,  // first-stage tracklet pairs per variant
     TOT_TLPAIRS, TOT_TLCANDS,      // all first-stage pairs, all second-stage candidates
+    TOT_TLSCORED,                  // pairs scored by the learned proposal
     TOT_ABORT, TOT_UNKNOWN, TOT_N
 };
 
@@ -1046,15 +1047,37 @@ __device__ __forceinline__ TLOutUp loadOutUp(int so, int flo, HitsDev H, const M
     return U;
 }
 
-// ---- learned proposal (pairs scored before the first stage)
-// Features of an (inner SG, outer-inner MD) pair, all from stored records.
-__device__ __forceinline__ void tlPairFeatures(const TLInner& I, const TLOutLo& L, int variant, float* f) {
+// First stage, one thread per pair: carry = (sdlCut, thetaMuls, sin of the
+// midpoint deltaPhi); nOut = number of outer SGs to try (0 if the pair failed).
+template <int V>
+__global__ void __launch_bounds__(kBlock) k_tl_stage1(VarRange R, MDDev D, SGDev S, const long long* sgStart,
+                                                      const int2* pair, float4* carry, int* nOut) {
+    long long b0, e0;
+    R.get(b0, e0);
+    const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e0) return;
+    const int2 pr = pair[i];
+    const int sgA = int(sgStart[pr.y]);
+    const TLInner I = loadInner(pr.x, D, S);
+    const TLOutLo L = loadOutLo(pr.y, S.flags[sgA], D);
+    TLCarry c{0.f, 0.f, 0.f};
+    const bool ok = tlStage1(V, I, L, c);
+    carry[i] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);
+    nOut[i] = ok ? int(sgStart[pr.y + 1]) - sgA : 0;
+}
+
+// ---- learned proposal (pairs that passed the first stage are scored)
+// Features of a pair that passed the first stage: the stored records of the
+// inner SG and the outer SG's inner MD, and where the pair lies in the
+// first-stage windows.
+__device__ __forceinline__ void tlPairFeatures(const TLInner& I, const TLOutLo& L, const TLCarry& c,
+                                               const TLMargins& m, int variant, float* f) {
     const float drIn = fmaxf(I.up.rt - I.lo.rt, 1e-3f);
     const float drOut = fmaxf(L.a.rt - I.lo.rt, 1e-3f);
     const float pointIn = (I.up.z - I.lo.z) / drIn;
     const float pointOut = (L.a.z - I.lo.z) / drOut;
-    f[0] = I.dphichange;
-    f[1] = I.dphichangemax - I.dphichangemin;
+    f[0] = I.dphichange * 10.f;
+    f[1] = (I.dphichangemax - I.dphichangemin) * 10.f;
     f[2] = I.dr * 0.05f;
     f[3] = I.d * 0.05f;
     f[4] = I.lo.rt * 0.01f;
@@ -1066,60 +1089,133 @@ __device__ __forceinline__ void tlPairFeatures(const TLInner& I, const TLOutLo& 
     f[10] = L.alpha1;
     f[11] = I.alphaReg3Lo;
     f[12] = L.alphaReg3;
-    f[13] = phiMpiPi(L.a.phi - I.up.phi);
-    f[14] = phiMpiPi(L.a.phi - I.lo.phi);
-    f[15] = fminf(fmaxf(pointOut - pointIn, -5.f), 5.f);
-    f[16] = (L.a.rt - I.up.rt) * 0.02f;
-    f[17] = phiMpiPi(L.a.phi - I.up.phi) - I.dphichange;
-    f[18] = I.psLo ? 1.f : 0.f;
-    f[19] = I.psUp ? 1.f : 0.f;
-    f[20] = L.ps ? 1.f : 0.f;
-    f[21] = L.barrel ? 1.f : 0.f;
+    f[13] = phiMpiPi(L.a.phi - I.up.phi) * 10.f;
+    f[14] = fminf(fmaxf(pointOut - pointIn, -5.f), 5.f);
+    f[15] = m.w1 - 0.5f;
+    f[16] = m.w2 - 0.5f;
+    f[17] = m.phi;
+    f[18] = c.sdlCut * 10.f;
+    f[19] = I.psLo ? 1.f : 0.f;
+    f[20] = I.psUp ? 1.f : 0.f;
+    f[21] = L.ps ? 1.f : 0.f;
     f[22] = variant == TL_BBEE ? 1.f : 0.f;
     f[23] = variant == TL_EEEE ? 1.f : 0.f;
 }
 
+// First stage of pair i with the proposal's features (ok = passed the cuts).
 template <int V>
-__device__ __forceinline__ void pairFeatures(long long i, const int2* pair, const long long* sgStart, MDDev D,
-                                             SGDev S, float* f) {
+__device__ __forceinline__ bool stage1Features(long long i, const int2* pair, const long long* sgStart, MDDev D,
+                                               SGDev S, TLCarry& c, int& nOuter, float* f) {
     const int2 pr = pair[i];
+    const int sgA = int(sgStart[pr.y]);
     const TLInner I = loadInner(pr.x, D, S);
-    const TLOutLo L = loadOutLo(pr.y, S.flags[sgStart[pr.y]], D);
-    tlPairFeatures(I, L, V, f);
+    const TLOutLo L = loadOutLo(pr.y, S.flags[sgA], D);
+    TLMargins m{0.f, 0.f, 0.f};
+    c = TLCarry{0.f, 0.f, 0.f};
+    const bool ok = tlStage1(V, I, L, c, &m);
+    nOuter = int(sgStart[pr.y + 1]) - sgA;
+    if (ok) tlPairFeatures(I, L, c, m, V, f);
+    return ok;
 }
 
-// Keep flag of every pair: the network's logit against the threshold, one wave per 64 pairs.
+// First stage with the learned proposal, one thread per pair: as k_tl_stage1,
+// and every pair that passes writes its BF16 feature row at its own index.
 template <int V>
-__global__ void __launch_bounds__(kBlock) k_tl_score(VarRange R, MDDev D, SGDev S, const long long* sgStart,
-                                                     const int2* pairAll, const mlp::Packed* P, float logitCut,
-                                                     unsigned char* keepAll) {
-    __shared__ short tiles[kWavesPerBlock][64 * 32];
+__global__ void __launch_bounds__(kBlock) k_tl_stage1_emit(VarRange R, MDDev D, SGDev S, const long long* sgStart,
+                                                           const int2* pair, float4* carry, int* nOut,
+                                                           mlp::short8v* rows) {
     long long b0, e0;
     R.get(b0, e0);
-    const long long n = e0 - b0;
-    const int2* pair = pairAll + b0;
-    unsigned char* keep = keepAll + b0;
-    const int lane = threadIdx.x & 63;
-    short* tile = tiles[threadIdx.x >> 6];
+    const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e0) return;
+    TLCarry c;
+    int nOuter;
+    float f[mlp::kIn];
+    const bool ok = stage1Features<V>(i, pair, sgStart, D, S, c, nOuter, f);
+    carry[i] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);
+    nOut[i] = ok ? nOuter : 0;
+    if (!ok) return;
+    mlp::short8v v[4];
+    mlp::featureRow(f, v);
+    for (int q = 0; q < 4; ++q) rows[i * 4 + q] = v[q];
+}
+
+// Scores the pairs that passed the first stage (nOut > 0), 64 at a time per
+// wave: each wave walks tiles of 64 pairs, queues the indices of the passing
+// ones in LDS (compacted with a ballot) and scores every 64 queued rows with
+// the BF16 MFMA network; a pair whose logit is below the threshold gets
+// nOut = 0 (its outer SGs are not tried).  Adds the number scored to
+// tot[TOT_TLSCORED] (one atomic per block).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) k_tl_score_rows(const long long* tot, const mlp::short8v* rows,
+                                                          const mlp::Packed* P, float logitCut, int* nOut,
+                                                          long long* totOut) {
+    constexpr int kQ = 128;  // < 64 left over + 64 new
+    __shared__ __align__(16) mlp::Vecs Vs;
+    __shared__ int queueAll[kWavesPerBlock][kQ];
+    __shared__ int scoredBlock;
+    if (threadIdx.x == 0) scoredBlock = 0;
+    mlp::loadVecs(P, Vs);  // includes a block barrier
+    if (tot[TOT_ABORT]) return;
+    const long long n = tot[TOT_TLPAIRS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = lane >> 4, c = lane & 15;
+    int* queue = queueAll[w];
     const long long nTiles = (n + 63) / 64;
     const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
-    if (w0 >= nTiles) return;  // uniform over the wave
     mlp::Frags F;
     mlp::loadFrags(P, lane, F);
-    for (long long t = w0; t < nTiles; t += nw) {
-        const long long i = t * 64 + lane;
-        float f[mlp::kIn];
-        if (i < n) pairFeatures<V>(i, pair, sgStart, D, S, f);
-        else
-            for (int k = 0; k < mlp::kIn; ++k) f[k] = 0.f;
-        const float z = mlp::logit64(f, lane, F, tile);
-        if (i < n) keep[i] = z >= logitCut ? 1 : 0;
+    int q = 0, scored = 0;  // uniform over the wave
+    auto outerOf = [&](long long t) { return t < nTiles && t * 64 + lane < n ? nOut[t * 64 + lane] : 0; };
+    int cur = outerOf(w0);
+    for (long long t = w0;; t += nw) {
+        const bool more = t < nTiles;
+        const int next = outerOf(t + nw);  // in flight while this tile is handled
+        if (more) {
+            const bool ok = cur > 0;
+            const unsigned long long mask = __ballot(ok);
+            if (ok) queue[q + __popcll(mask & ((1ull << lane) - 1))] = int(t * 64 + lane);
+            q += __popcll(mask);
+            mlp::waveSync();
+        }
+        cur = next;
+        if (q >= 64 || (!more && q > 0)) {
+            mlp::short8v x[4];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const int r = st * 16 + c;
+                const long long idx = queue[r < q ? r : 0];  // rows past the queue repeat a valid row
+                x[st] = rows[idx * 4 + g];
+            }
+            float z = 0.f;
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const float zs = mlp::logit16(__builtin_bit_cast(mlp::bf16x8, x[st]), lane, F, Vs);
+                if (g == st) z = zs;  // lane st * 16 + c holds row st * 16 + c
+            }
+            const int take = q < 64 ? q : 64;
+            if (lane < take && z < logitCut) nOut[queue[lane]] = 0;
+            scored += take;
+            mlp::waveSync();
+            const int left = q - take;
+            int moved = 0;
+            if (lane < left) moved = queue[64 + lane];
+            mlp::waveSync();
+            if (lane < left) queue[lane] = moved;
+            mlp::waveSync();
+            q = left;
+        }
+        if (!more && q == 0) break;
     }
+    if (lane == 0) atomicAdd(&scoredBlock, scored);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long*>(totOut + TOT_TLSCORED),
+                                    (unsigned long long)scoredBlock);
 }
 
-// Training rows of the plain chain: features of every pair and its label (the
-// pair yields at least one tracklet), written at row offset `row0`.
+// Training rows of the plain chain: for every pair, its features and label
+// (1: the pair yields at least one tracklet, 0: it passed the first stage but
+// yields none, -1: it failed the first stage and is never scored).
 template <int V>
 __global__ void k_tl_pair_rows(VarRange R, MDDev D, SGDev S, const long long* sgStart, const int2* pair,
                                const long long* candOff, const unsigned char* flag, float* X, float* Y) {
@@ -1128,35 +1224,13 @@ __global__ void k_tl_pair_rows(VarRange R, MDDev D, SGDev S, const long long* sg
     const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= e0) return;
     float f[mlp::kIn];
-    pairFeatures<V>(i, pair, sgStart, D, S, f);
-    for (int k = 0; k < mlp::kIn; ++k) X[i * mlp::kIn + k] = f[k];
+    TLCarry c;
+    int nOuter;
+    const bool ok = stage1Features<V>(i, pair, sgStart, D, S, c, nOuter, f);
+    for (int k = 0; k < mlp::kIn; ++k) X[i * mlp::kIn + k] = ok ? f[k] : 0.f;
     bool any = false;
-    for (long long c = candOff[i]; c < candOff[i + 1] && !any; ++c) any = flag[c] != 0;
-    Y[i] = any ? 1.f : 0.f;
-}
-
-// First stage, one thread per pair: carry = (sdlCut, thetaMuls, sin of the
-// midpoint deltaPhi); nOut = number of outer SGs to try (0 if the pair failed).
-template <int V>
-__global__ void __launch_bounds__(kBlock) k_tl_stage1(VarRange R, MDDev D, SGDev S, const long long* sgStart,
-                                                      const int2* pair, const unsigned char* keep, float4* carry,
-                                                      int* nOut) {
-    long long b0, e0;
-    R.get(b0, e0);
-    const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= e0) return;
-    if (keep && !keep[i]) {  // dropped by the learned proposal
-        nOut[i] = 0;
-        return;
-    }
-    const int2 pr = pair[i];
-    const int sgA = int(sgStart[pr.y]);
-    const TLInner I = loadInner(pr.x, D, S);
-    const TLOutLo L = loadOutLo(pr.y, S.flags[sgA], D);
-    TLCarry c{0.f, 0.f, 0.f};
-    const bool ok = tlStage1(V, I, L, c);
-    carry[i] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);
-    nOut[i] = ok ? int(sgStart[pr.y + 1]) - sgA : 0;
+    for (long long cc = candOff[i]; cc < candOff[i + 1] && !any; ++cc) any = flag[cc] != 0;
+    Y[i] = ok ? (any ? 1.f : 0.f) : -1.f;
 }
 
 // Candidates (pair, outer SG) of the surviving pairs, one thread per pair.
@@ -1657,7 +1731,8 @@ struct sdl_event {
     DBuf<int2> tlPair, tlCand;
     DBuf<float4> tlCarry;
     DBuf<int> tlNOut;
-    DBuf<unsigned char> tlFlag, tlKeep;
+    DBuf<unsigned char> tlFlag;
+    DBuf<mlp::short8v> tlRows;  // learned proposal: BF16 feature rows of the pairs that pass the first stage
     // learned proposal (null: the plain chain)
     sdl_mlp* proposal = nullptr;
     float proposalLogit = 0.f;
@@ -1825,7 +1900,7 @@ long long sizeOf(sdl_event* ev, int k) {
         static const int stageOf[TOT_N] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                            SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
                                            SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL,
-                                           SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, 0, 0};
+                                           SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, 0, 0};
         const double scale = ev->lastNH > 0 ? double(ev->NH) / double(ev->lastNH) : 1.0;
         const bool custom = stageOf[k] >= ev->headroomFrom;
         const double f = custom ? ev->headroomFactor : 1.25;
@@ -2082,38 +2157,45 @@ int runTL(sdl_event* ev) {
         np[v] = variantSize(ev, TOT_TLP0 + v, ev->tlPOff.p, nullptr, ns, v);
         if (np[v] < 0) return fail(SDL_ERR_HIP, "readback failed");
     }
-    // first stage per pair, after the learned proposal when one is set
+    // first stage per pair; with the learned proposal, the pairs that pass are listed and scored
     const MDDev D = mdView(ev);
     const SGDev S = sgView(ev);
     const bool learned = ev->proposal != nullptr;
-    if (learned)
-        if (int r = ev->tlKeep.ensure(npc)) return r;
-    const unsigned char* keep = learned ? ev->tlKeep.p : nullptr;
+    if (learned) {
+        if (int r = ev->tlRows.ensure(std::max(nPairs, 1LL) * 4)) return r;
+        zeroAsync(ev->tot.p + TOT_TLSCORED, 1, st);
+    }
     for (int v = 0; v < kNVariantsTL; ++v) {
         if (np[v] == 0) continue;
         const VarRange R{ev->tlPOff.p, nullptr, ns, v, np[v], ev->tot.p};
-        if (learned) {
-            const unsigned grid = std::min<unsigned>(gridFor((np[v] + 63) / 64, kWavesPerBlock), 8192u);
-            if (v == 0)
-                hipLaunchKernelGGL(k_tl_score<TL_BBBB>, dim3(grid), dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
-                                   ev->tlPair.p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep.p);
-            else if (v == 1)
-                hipLaunchKernelGGL(k_tl_score<TL_BBEE>, dim3(grid), dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
-                                   ev->tlPair.p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep.p);
-            else
-                hipLaunchKernelGGL(k_tl_score<TL_EEEE>, dim3(grid), dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
-                                   ev->tlPair.p, ev->proposal->dPacked, ev->proposalLogit, ev->tlKeep.p);
-        }
         const dim3 grid(gridFor(np[v], kBlock));
+        if (learned) {
+            mlp::short8v* rows = ev->tlRows.p;
+            if (v == 0)
+                hipLaunchKernelGGL(k_tl_stage1_emit<TL_BBBB>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
+                                   ev->tlPair.p, ev->tlCarry.p, ev->tlNOut.p, rows);
+            else if (v == 1)
+                hipLaunchKernelGGL(k_tl_stage1_emit<TL_BBEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
+                                   ev->tlPair.p, ev->tlCarry.p, ev->tlNOut.p, rows);
+            else
+                hipLaunchKernelGGL(k_tl_stage1_emit<TL_EEEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
+                                   ev->tlPair.p, ev->tlCarry.p, ev->tlNOut.p, rows);
+            continue;
+        }
         if (v == 0)
             hipLaunchKernelGGL(k_tl_stage1<TL_BBBB>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
-                               keep, ev->tlCarry.p, ev->tlNOut.p);
+                               ev->tlCarry.p, ev->tlNOut.p);
         else if (v == 1)
             hipLaunchKernelGGL(k_tl_stage1<TL_BBEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
-                               keep, ev->tlCarry.p, ev->tlNOut.p);
+                               ev->tlCarry.p, ev->tlNOut.p);
         else
             hipLaunchKernelGGL(k_tl_stage1<TL_EEEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
-                               keep, ev->tlCarry.p, ev->tlNOut.p);
+                               ev->tlCarry.p, ev->tlNOut.p);
+    }
+    if (learned && nPairs > 0) {
+        const dim3 grid(std::min<unsigned>(gridFor((nPairs + 63) / 64, kWavesPerBlock), 1024u));
+        hipLaunchKernelGGL(k_tl_score_rows, grid, dim3(kBlock), 0, st, ev->tot.p, ev->tlRows.p,
+                           ev->proposal->dPacked, ev->proposalLogit, ev->tlNOut.p, ev->tot.p);
     }
     // candidates of the surviving pairs: one list, variant-major
     const Count cPairs = devCount(ev, TOT_TLPAIRS);
@@ -2372,7 +2454,7 @@ int runStages(sdl_event* ev, int s0, int s1) {
     static const int stageOfTot[TOT_ABORT] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                               SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
                                               SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL,
-                                              SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL};
+                                              SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL};
     for (int k = 0; k < TOT_ABORT; ++k)
         if (stageOfTot[k] >= s0 && stageOfTot[k] <= s1) ev->lastTot[k] = t[k];
     if (s1 == SDL_STAGE_TCL) {
@@ -2446,24 +2528,22 @@ void packMlp(const float* P, sdl::mlp::Packed& K) {
     using namespace sdl::mlp;
     for (int t = 0; t < kTiles; ++t)
         for (int l = 0; l < 64; ++l) {
-            const int row = t * 16 + (l % 16);  // A operand row: output neuron
-            for (int v = 0; v < 4; ++v) {
-                const int kq = 4 * (l / 16) + v;  // A operand column within a K step
-                for (int ks = 0; ks < 2; ++ks) {
-                    const int k = ks * 16 + kq;
-                    const float w = k < kIn ? P[kW1 + row * kIn + k] : (k == kIn ? P[kB1 + row] : 0.f);
-                    K.a1[t][ks][l][v] = bf16Host(w);
+            const int row = t * 16 + (l % 16), g = l / 16;  // A operand row: output neuron
+            for (int i = 0; i < 8; ++i) {
+                const int k = 8 * g + i;  // layer 1: K slot = input feature (24: the bias, then zeros)
+                K.a1[t][l][i] = bf16Host(k < kIn ? P[kW1 + row * kIn + k] : (k == kIn ? P[kB1 + row] : 0.f));
+                for (int u = 0; u < kSteps; ++u) {
+                    const int h = hiddenOfSlot(u, g, i);
+                    K.a2[t][u][l][i] = bf16Host(P[kW2 + row * kH + h]);
+                    K.a3[t][u][l][i] = bf16Host(P[kW3 + row * kH + h]);
                 }
-                for (int ks = 0; ks < kTiles; ++ks) {
-                    K.a2[t][ks][l][v] = bf16Host(P[kW2 + row * kH + ks * 16 + kq]);
-                    K.a3[t][ks][l][v] = bf16Host(P[kW3 + row * kH + ks * 16 + kq]);
-                }
-                const int acc = t * 16 + 4 * (l / 16) + v;  // accumulator row of the lane
-                K.c2[t][l][v] = P[kB2 + acc];
-                K.c3[t][l][v] = P[kB3 + acc];
-                K.w4[t][l][v] = P[kW4 + acc];
             }
         }
+    for (int k = 0; k < kH; ++k) {
+        K.b2[k] = P[kB2 + k];
+        K.b3[k] = P[kB3 + k];
+        K.w4[k] = P[kW4 + k];
+    }
     K.b4 = P[kB4];
 }
 
@@ -2813,7 +2893,7 @@ int sdl_event_set_proposal(sdl_event* ev, sdl_mlp* m, float threshold) {
     return 0;
 }
 
-int sdl_event_tl_work(sdl_event* ev, long long out[3]) {
+int sdl_event_tl_work(sdl_event* ev, long long out[4]) {
     if (!ev || !out) return fail(SDL_ERR_ARG, "bad arguments");
     long long t[TOT_N];
     HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, ev->st));
@@ -2821,6 +2901,7 @@ int sdl_event_tl_work(sdl_event* ev, long long out[3]) {
     out[0] = t[TOT_TLPAIRS];
     out[1] = t[TOT_TLCANDS];
     out[2] = t[TOT_TL];
+    out[3] = ev->proposal ? t[TOT_TLSCORED] : 0;
     return 0;
 }
 

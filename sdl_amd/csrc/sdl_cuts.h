@@ -427,6 +427,14 @@ struct TLCarry {
     float sdlCut, thetaMuls, sinDPhiMid;  // sin of the midpoint deltaPhi (BBBB, BBEE)
 };
 
+// Where a pair that passed the first stage lies inside its windows (features of
+// the learned proposal): w1 = position in the z (BBBB) or rt (BBEE, EEEE)
+// window, w2 = position in the pointed window (0.5 where the variant has none),
+// both in [0, 1]; phi = midpoint deltaPhi / sdlCut, in [-1, 1].
+struct TLMargins {
+    float w1, w2, phi;
+};
+
 enum { TL_BBBB = 0, TL_BBEE = 1, TL_EEEE = 2, TL_NONE = 3 };
 
 // Variant of runTrackletDefaultAlgo (Tracklet.cc:144-296) from the first three anchor
@@ -450,7 +458,7 @@ __device__ __forceinline__ double mulsFactor(float z, float rt) {
 
 // ---- first stage: cuts on the inner segment and the outer segment's inner MD
 // _v2: Tracklet.cc:902-1153 (z window, pointed z window, deltaPhi of the midpoint)
-__device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L, TLCarry& c) {
+__device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L, TLCarry& c, TLMargins* m) {
     const HitView& hInLo = I.lo;
     const HitView& hInUp = I.up;
     const HitView& hOutLo = L.a;
@@ -485,12 +493,17 @@ __device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L,
     c.sdlCut = sdlCut;
     c.thetaMuls = thetaMuls;
     c.sinDPhiMid = sinF(dPhiMid);
+    if (m) {
+        m->w1 = (z_OutLo - zLo) / (zHi - zLo);
+        m->w2 = (z_OutLo - zLoP) / (zHiP - zLoP);
+        m->phi = dPhiMid / sdlCut;
+    }
     return true;
 }
 
 // BBEE / EEEE: Tracklet.cc:1540-1771 and 2018-2270 (rt windows, pointing, midpoint deltaPhi)
 template <bool EEEE>
-__device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, TLCarry& c) {
+__device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, TLCarry& c, TLMargins* m) {
     const HitView& hInLo = I.lo;
     const HitView& hInUp = I.up;
     const HitView& hOutLo = L.a;
@@ -533,6 +546,7 @@ __device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, T
         thetaMuls = float(double(0.015f) * sqrt(double(0.1f) + 0.2 * double(rtOut - rtIn) / double(50.f)));
     }
     const float sdlMuls = thetaMuls * 3.f / kPtCut * 4.f;
+    float w2 = 0.5f;
     if (EEEE && psInLo && psInUp) {
         const float multDzDr = dzOutInAbs * coshEta / (coshEta * coshEta - 1.f);
         const float rtGeom = (psInLo && psOutLo) ? 2.f * kPixelPSZpitch
@@ -545,6 +559,7 @@ __device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, T
         const float rtLoP = rtIn + drtMean / dzDrtScale - rtWindow;
         const float rtHiP = rtIn + drtMean + rtWindow;
         if (!(kZ >= 0 && rtOut >= rtLoP && rtOut <= rtHiP)) return false;
+        if (m) w2 = (rtOut - rtLoP) / (rtHiP - rtLoP);
     }
     const float sdlPVoff = 0.1f / rtOut;
     const float sdlCut = float(double(L.alpha1) + sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
@@ -553,13 +568,19 @@ __device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, T
     c.sdlCut = sdlCut;
     c.thetaMuls = thetaMuls;
     c.sinDPhiMid = EEEE ? 0.f : sinF(dPhiMid);  // the EEEE second stage has no dBetaROut term (Tracklet.cc:2493)
+    if (m) {
+        m->w1 = (rtOut - rtLo) / (rtHi - rtLo);
+        m->w2 = w2;
+        m->phi = dPhiMid / sdlCut;
+    }
     return true;
 }
 
-__device__ __forceinline__ bool tlStage1(int variant, const TLInner& I, const TLOutLo& L, TLCarry& c) {
-    if (variant == TL_BBBB) return tlStage1BBBB(I, L, c);
-    if (variant == TL_BBEE) return tlStage1EC<false>(I, L, c);
-    if (variant == TL_EEEE) return tlStage1EC<true>(I, L, c);
+__device__ __forceinline__ bool tlStage1(int variant, const TLInner& I, const TLOutLo& L, TLCarry& c,
+                                         TLMargins* m = nullptr) {
+    if (variant == TL_BBBB) return tlStage1BBBB(I, L, c, m);
+    if (variant == TL_BBEE) return tlStage1EC<false>(I, L, c, m);
+    if (variant == TL_EEEE) return tlStage1EC<true>(I, L, c, m);
     return false;
 }
 

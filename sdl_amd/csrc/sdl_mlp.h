@@ -1,14 +1,18 @@
 // Learned proposal for the tracklet stage: a 24-64-64-64-1 MLP (ReLU hidden
-// layers, sigmoid output) that scores (inner SG, outer-inner MD) pairs before
-// the exact tracklet cuts.  This component has no counterpart in the reference;
-// the exact cuts still decide every object, the network only drops pairs.
+// layers, sigmoid output) that scores (inner SG, outer-inner MD) pairs that
+// passed the first-stage tracklet cuts.  This component has no counterpart in
+// the reference; the exact cuts still decide every object, the network only
+// drops pairs.
 //
-//  * inference: BF16 MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation,
-//    one wave per 64 pairs (four 16-pair MFMA tiles).  Layers are computed
-//    transposed, D = W . X^T, so a
-//    layer's accumulator fragment (rows 4q..4q+3 of a 16-neuron tile for the
-//    pair in column l % 16, q = l / 16) is exactly the B fragment the next layer
-//    needs for that tile's K step: activations never leave registers.
+//  * inference: BF16 MFMA (v_mfma_f32_16x16x32_bf16, gfx950) with FP32
+//    accumulation on 64-row tiles of BF16 feature rows in LDS.  Layers are
+//    computed transposed, D = W . H^T, 16 neurons x 16 samples per product.
+//    Lane l = 16 g + c holds accumulator rows 4g..4g+3 of sample c; the B
+//    operand of a 32-deep K step holds 8 K values per lane (slots 8g..8g+7).
+//    The hidden units are assigned to K slots so that the accumulators of
+//    output tiles 2u and 2u+1 are, after ReLU and rounding, exactly the lane's
+//    B operand of the next layer's K step u: activations never leave registers
+//    (the host packs the weights in that order).
 //  * training: FP32 forward and backward kernels (weighted binary cross
 //    entropy on logits, gradients reduced per block in LDS, Adam update).
 #pragma once
@@ -21,6 +25,7 @@ namespace mlp {
 constexpr int kIn = 24;    // features
 constexpr int kH = 64;     // hidden width
 constexpr int kTiles = kH / 16;
+constexpr int kSteps = kH / 32;  // K steps of a hidden layer
 // parameter layout (float32, weights as [out][in])
 constexpr int kW1 = 0;
 constexpr int kB1 = kW1 + kH * kIn;
@@ -33,16 +38,20 @@ constexpr int kB4 = kW4 + kH;
 constexpr int kNParams = kB4 + 1;  // 9985
 
 typedef short short4v __attribute__((ext_vector_type(4)));
+typedef short short8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
-// Weight fragments in the lane order of the MFMA operands (packed on the host).
+// The hidden unit in K slot (step u, lane group g, element i) of layers 2 and 3.
+__host__ __device__ constexpr int hiddenOfSlot(int u, int g, int i) { return 16 * (2 * u + i / 4) + 4 * g + (i % 4); }
+
+// Weight fragments in the lane order of the MFMA operands (packed on the host),
+// and the vectors that every lane group reads at its own accumulator rows.
 struct Packed {
-    short4v a1[kTiles][2][64];      // layer 1: K = 32 (24 features, the constant 1 at 24 carries b1, zeros)
-    short4v a2[kTiles][kTiles][64]; // layer 2: [output tile][K step][lane]
-    short4v a3[kTiles][kTiles][64];
-    float4v c2[kTiles][64];         // layer-2 bias at the accumulator rows of the lane
-    float4v c3[kTiles][64];
-    float4v w4[kTiles][64];         // output weights at the accumulator rows of the lane
+    short8v a1[kTiles][64];          // layer 1: [output tile][lane], K = 32
+    short8v a2[kTiles][kSteps][64];  // layer 2: [output tile][K step][lane]
+    short8v a3[kTiles][kSteps][64];
+    float b2[kH], b3[kH], w4[kH];
     float b4;
 };
 
@@ -51,112 +60,158 @@ __device__ __forceinline__ short bf16Bits(float f) {
     return __builtin_bit_cast(short, h);
 }
 
-__device__ __forceinline__ short4v toBf16(float a, float b, float c, float d) {
+__device__ __forceinline__ short4v reluBf16(float4v d) {
     short4v r;
-    r[0] = bf16Bits(a);
-    r[1] = bf16Bits(b);
-    r[2] = bf16Bits(c);
-    r[3] = bf16Bits(d);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = bf16Bits(fmaxf(d[j], 0.f));
     return r;
 }
 
-__device__ __forceinline__ short4v reluBf16(float4v d) {
-    return toBf16(fmaxf(d[0], 0.f), fmaxf(d[1], 0.f), fmaxf(d[2], 0.f), fmaxf(d[3], 0.f));
+__device__ __forceinline__ bf16x8 join(short4v lo, short4v hi) {
+    const short8v v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    return __builtin_bit_cast(bf16x8, v);
 }
 
-// Per-wave copy of the fragments it needs (one lane's share of every operand).
+__device__ __forceinline__ float4v mfma(short8v a, bf16x8 b, float4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), b, c, 0, 0, 0);
+}
+
+// Per-wave copy of the layer-2 weight fragments (one lane's share of the A
+// operands); layers 1 and 3 are read from the block's LDS copy (Vecs) at each
+// use, which keeps the register count low enough for four waves per SIMD.
 struct Frags {
-    short4v a1[kTiles][2], a2[kTiles][kTiles], a3[kTiles][kTiles];
-    float4v c2[kTiles], c3[kTiles], w4[kTiles];
+    short8v a2[kTiles][kSteps];
     float b4;
 };
 
 __device__ __forceinline__ void loadFrags(const Packed* P, int lane, Frags& F) {
 #pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
-        F.a1[t][0] = P->a1[t][0][lane];
-        F.a1[t][1] = P->a1[t][1][lane];
+    for (int t = 0; t < kTiles; ++t)
 #pragma unroll
-        for (int k = 0; k < kTiles; ++k) {
-            F.a2[t][k] = P->a2[t][k][lane];
-            F.a3[t][k] = P->a3[t][k][lane];
-        }
-        F.c2[t] = P->c2[t][lane];
-        F.c3[t] = P->c3[t][lane];
-        F.w4[t] = P->w4[t][lane];
-    }
+        for (int u = 0; u < kSteps; ++u) F.a2[t][u] = P->a2[t][u][lane];
     F.b4 = P->b4;
 }
 
-// Logits of 64 samples, one per lane: each lane passes its sample's 24
-// features; they go through the wave's LDS tile (64 rows x 32 bf16, the
-// constant 1 at column 24 carries b1) so that every MFMA column is one sample.
-// All 64 lanes must call; tile is the wave's own 4 KB of LDS.
-__device__ __forceinline__ float logit64(const float* f, int lane, const Frags& F, short* tile) {
-    short* row = tile + lane * 32;
-#pragma unroll
-    for (int k = 0; k < 32; ++k) row[k] = bf16Bits(k < kIn ? f[k] : (k == kIn ? 1.f : 0.f));
+// Layer-1 and layer-3 fragments, biases of layers 2 and 3 and the output
+// weights in LDS (one copy per block).  All threads must call.
+struct Vecs {
+    short8v a1[kTiles][64];
+    short8v a3[kTiles][kSteps][64];
+    float b2[kH], b3[kH], w4[kH];
+};
+
+__device__ __forceinline__ void loadVecs(const Packed* P, Vecs& V) {
+    for (int k = threadIdx.x; k < kTiles * 64; k += blockDim.x) {
+        const int t = k / 64, l = k % 64;
+        V.a1[t][l] = P->a1[t][l];
+        for (int u = 0; u < kSteps; ++u) V.a3[t][u][l] = P->a3[t][u][l];
+    }
+    for (int k = threadIdx.x; k < kH; k += blockDim.x) {
+        V.b2[k] = P->b2[k];
+        V.b3[k] = P->b3[k];
+        V.w4[k] = P->w4[k];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ float4v at4(const float* v, int i) { return *reinterpret_cast<const float4v*>(v + i); }
+
+__device__ __forceinline__ void waveSync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int q = lane >> 4, c = lane & 15;
+}
+
+// A feature row: 32 BF16 (24 features, the constant 1 at 24 carries b1, zeros).
+__device__ __forceinline__ void featureRow(const float* f, short8v (&v)[4]) {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v[k / 8][k % 8] = bf16Bits(k < kIn ? f[k] : (k == kIn ? 1.f : 0.f));
+}
+
+// Logits of 16 rows: x is this lane's B fragment (elements 8g..8g+7 of row c,
+// lane = 16 g + c); every lane returns the logit of row c.  All 64 lanes must
+// call.  A row only affects its own logit (every MFMA column is one row), so
+// rows past the end may hold anything.
+__device__ __forceinline__ float logit16(bf16x8 x, int lane, const Frags& F, const Vecs& V) {
+    const int g = lane >> 4;
+    // an opaque copy of the lane index: the LDS fragments are read at each use
+    // instead of being hoisted into registers by the compiler
+    int l = lane;
+    asm volatile("" : "+v"(l));
+    short4v h1[kTiles], h2[kTiles];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) h1[t] = reluBf16(mfma(V.a1[t][l], x, float4v{0.f, 0.f, 0.f, 0.f}));
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        float4v d = at4(V.b2, 16 * t + 4 * g);
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u) d = mfma(F.a2[t][u], join(h1[2 * u], h1[2 * u + 1]), d);
+        h2[t] = reluBf16(d);
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        float4v d = at4(V.b3, 16 * t + 4 * g);
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u) d = mfma(V.a3[t][u][l], join(h2[2 * u], h2[2 * u + 1]), d);
+        const float4v w = at4(V.w4, 16 * t + 4 * g);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) part += w[v] * fmaxf(d[v], 0.f);
+    }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    return part + F.b4;
+}
+
+// Logits of 64 rows: x[st] is this lane's B fragment of rows st * 16 + c.
+// Lane l returns row l's logit.  All 64 lanes must call.
+__device__ __forceinline__ float logit64(const bf16x8 (&x)[4], int lane, const Frags& F, const Vecs& V) {
     float mine = 0.f;
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
-        const short* src = tile + (st * 16 + c) * 32;
-        const short4v b0 = *reinterpret_cast<const short4v*>(src + 4 * q);
-        const short4v b1 = *reinterpret_cast<const short4v*>(src + 16 + 4 * q);
-        short4v h1[kTiles], h2[kTiles];
-#pragma unroll
-        for (int t = 0; t < kTiles; ++t) {
-            float4v d = {0.f, 0.f, 0.f, 0.f};
-            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a1[t][0], b0, d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a1[t][1], b1, d, 0, 0, 0);
-            h1[t] = reluBf16(d);
-        }
-#pragma unroll
-        for (int t = 0; t < kTiles; ++t) {
-            float4v d = F.c2[t];
-#pragma unroll
-            for (int k = 0; k < kTiles; ++k) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a2[t][k], h1[k], d, 0, 0, 0);
-            h2[t] = reluBf16(d);
-        }
-        float part = 0.f;
-#pragma unroll
-        for (int t = 0; t < kTiles; ++t) {
-            float4v d = F.c3[t];
-#pragma unroll
-            for (int k = 0; k < kTiles; ++k) d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(F.a3[t][k], h2[k], d, 0, 0, 0);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) part += F.w4[t][v] * fmaxf(d[v], 0.f);
-        }
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
-        if (q == st) mine = part + F.b4;  // lane st * 16 + c holds sample st * 16 + c
+        const float z = logit16(x[st], lane, F, V);
+        if ((lane >> 4) == st) mine = z;  // lane st * 16 + c holds row st * 16 + c
     }
-    // the tile is rewritten by the next call
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     return mine;
 }
 
-// Inference on stored features (row-major n x 24): one wave per 64 rows.
+// Logits of n rows of 4 short8v each (row-major, 64 B per row), one wave per 64
+// rows; the row buffer holds at least a whole number of 64-row tiles.
+__device__ __forceinline__ void loadTile(const short8v* rows, long long tile, int lane, bf16x8 (&x)[4]) {
+    const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) x[st] = __builtin_bit_cast(bf16x8, rows[(tile * 64 + st * 16 + c) * 4 + g]);
+}
+
+// Inference on stored features (row-major n x 24, FP32): one wave per 64 rows;
+// the rows are rounded to BF16 into the wave's LDS tile.
 __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, const Packed* P, float* logit) {
-    __shared__ short tiles[4][64 * 32];
+    __shared__ __align__(16) short8v tiles[4][64 * 5];  // 64 rows of 4 short8v, padded to 5
+    __shared__ __align__(16) Vecs V;
+    loadVecs(P, V);
     const int lane = threadIdx.x & 63;
-    short* tile = tiles[threadIdx.x >> 6];
+    short8v* tile = tiles[threadIdx.x >> 6];
     Frags F;
     loadFrags(P, lane, F);
     const long long nTiles = (n + 63) / 64;
     const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    const int g = lane >> 4, c = lane & 15;
     for (long long t = w0; t < nTiles; t += nw) {
         const long long i = t * 64 + lane;
         float f[kIn];
 #pragma unroll
         for (int k = 0; k < kIn; ++k) f[k] = i < n ? X[i * kIn + k] : 0.f;
-        const float z = logit64(f, lane, F, tile);
+        short8v v[4];
+        featureRow(f, v);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tile[lane * 5 + q] = v[q];
+        waveSync();
+        bf16x8 x[4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) x[st] = __builtin_bit_cast(bf16x8, tile[(st * 16 + c) * 5 + g]);
+        waveSync();  // the tile is rewritten by the next iteration
+        const float z = logit64(x, lane, F, V);
         if (i < n) logit[i] = z;
     }
 }

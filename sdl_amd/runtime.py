@@ -183,10 +183,11 @@ class Event:
         _check(lib().sdl_event_set_headroom(self.h, from_stage, factor))
 
     def tl_work(self):
-        """(pairs, second-stage candidates, tracklets) of the last tracklet stage."""
-        out = np.zeros(3, np.int64)
+        """(pairs, second-stage candidates, tracklets, pairs scored by the proposal)
+        of the last tracklet stage."""
+        out = np.zeros(4, np.int64)
         _check(lib().sdl_event_tl_work(self.h, out.ctypes.data))
-        return dict(zip(("pairs", "stage2", "tracklets"), out.tolist()))
+        return dict(zip(("pairs", "stage2", "tracklets", "scored"), out.tolist()))
 
     def set_graphs(self, on=True):
         """Allow the speculative chain to be captured into a HIP graph and replayed."""

@@ -5,9 +5,10 @@ score thresholds.
                                      [--out sdl_amd/data/proposal_w2.json]
 
 1. The plain chain runs on seeded training events (seeds 100, 101, ...); every
-   (inner segment, outer-inner MD) pair of its tracklet stage becomes a row:
-   24 features and the label "the pair yields at least one tracklet" -- labels
-   the exact cuts generate.
+   (inner segment, outer-inner MD) pair of its tracklet stage that passes the
+   first-stage cuts (the pairs the proposal scores) becomes a row: 24 features
+   and the label "the pair yields at least one tracklet" -- labels the exact
+   cuts generate.
 2. The network (24-64-64-64-1) trains on the GPU with this library's FP32
    forward/backward kernels and Adam (weighted cross entropy; positives are
    rare, so they are up-weighted).
@@ -68,10 +69,12 @@ def main():
         run(ev, gen.event(a.workload, 100 + k))
         X, Y, n = P.pair_rows(ev)
         y = Y.to_host()[:n]
-        npos += int(y.sum())
-        nneg += int(n - y.sum())
-        rows.append((X, Y, n))
-        print(f"train event {100 + k}: {n} pairs, {int(y.sum())} positive", flush=True)
+        scored = np.flatnonzero(y >= 0).astype(np.int32)  # pairs that pass the first stage
+        pos = int((y == 1).sum())
+        npos += pos
+        nneg += len(scored) - pos
+        rows.append((X, Y, scored))
+        print(f"train event {100 + k}: {n} pairs, {len(scored)} scored, {pos} positive", flush=True)
     pos_weight = float(max(1.0, nneg / max(npos, 1) / 8.0))
 
     # 2. training
@@ -82,11 +85,11 @@ def main():
         lr = a.lr * (0.5 ** epoch)
         losses = []
         for k in rng.permutation(len(rows)):
-            X, Y, n = rows[k]
-            steps = n // a.batch
+            X, Y, scored = rows[k]
+            steps = len(scored) // a.batch
             if steps == 0:
                 continue
-            perm = hip.DeviceArray(rng.permutation(n).astype(np.int32))
+            perm = hip.DeviceArray(rng.permutation(scored))
             losses.append(m.train(X.address, Y.address, perm.address, 0, a.batch, steps, lr, pos_weight))
             perm.free()
         history.append(float(np.mean(losses)))

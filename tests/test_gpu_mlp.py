@@ -87,7 +87,9 @@ def test_proposal_only_removes_objects():
     assert n == ev.tl_work()["pairs"]
     # a pair (inner SG, outer SG's inner MD) is positive iff some tracklet starts with it
     positive_pairs = {tuple(t[:6]) for t in plain["TL"]}
-    assert int(y.sum()) == len(positive_pairs)
+    assert int((y == 1).sum()) == len(positive_pairs)
+    # rows of pairs that fail the first stage are marked -1 (never scored)
+    assert set(np.unique(y).tolist()) <= {-1.0, 0.0, 1.0}
     # keep everything: identical to the plain chain
     keep_all = np.zeros(P.N_PARAMS, np.float32)
     keep_all[-1] = 10.0
