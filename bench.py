@@ -55,13 +55,16 @@ def cpu_baseline(workload, threads):
 
 
 def tl_roofline(workload, nevents, tl_ms):
-    """Roofline of the dominant kernel, the tracklet kernel k_tl (one launch per
-    step).  Its work is floating-point algebra on a few MB of records, so its
-    ceiling is the FP32 peak: achieved = algorithmic FP operations of the
-    tracklet stage on the step's events (bench_work.json, counted by the
-    oracle's op-counting build) / measured k_tl duration.  traffic = HBM bytes
-    per launch from the FETCH_SIZE / WRITE_SIZE counters of the profiled run
-    recorded in bench_work.json, scaled by the copy-kernel calibration."""
+    """Roofline of the dominant cost, the tracklet stage: the k_tl_* launches of
+    one chain (pair lists, first-stage cuts, expansion, second-stage cuts,
+    scatter, and their scans), timed together with HIP events.  Its work is
+    floating-point cut algebra, so its ceiling is the FP32 peak: achieved =
+    algorithmic FP operations of the tracklet stage on the step's events
+    (bench_work.json, counted by the oracle's op-counting build along each
+    candidate's path in the reference's loops) / measured stage duration.
+    traffic = HBM bytes of the stage per chain from the FETCH_SIZE / WRITE_SIZE
+    counters of the profiled run recorded in bench_work.json, scaled by the
+    copy-kernel calibration."""
     path = os.path.join(ROOT, "bench_work.json")
     table = {}
     if os.path.exists(path):
@@ -76,8 +79,9 @@ def tl_roofline(workload, nevents, tl_ms):
             "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": None if achieved is None else round(achieved / PEAK_FP32_TFLOPS, 6),
             "traffic": None if traffic is None else traffic["bytes"],
-            "kernel": "k_tl", "kernel_ms": round(tl_ms, 4),
-            "peak_note": "FP32 peak 157.3 TFLOP/s (vector and matrix); k_tl issues FP32/FP64 VALU, no MFMA",
+            "kernel": "tracklet stage (k_tl_* launches of one chain)", "kernel_ms": round(tl_ms, 4),
+            "peak_note": "FP32 peak 157.3 TFLOP/s; the stage issues FP32/FP64 VALU and no MFMA, and is "
+                         "VALU-issue bound (profiles/chain_pu200_counters.md)",
             "flops_per_launch": flops}
 
 
