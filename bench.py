@@ -36,6 +36,16 @@ def parse():
     p.add_argument("--learned", default=os.path.join(ROOT, "sdl_amd", "data", "proposal_w2.json"),
                    help="trained proposal (skipped if the file is absent)")
     p.add_argument("--recall", default="0.995", help="target TC recall whose threshold the learned leg uses")
+    p.add_argument("--batch", type=int, default=64,
+                   help="events of the uneven-batch leg (workload --batch-workload), 0 skips it")
+    p.add_argument("--batch-workload", type=int, default=3)
+    p.add_argument("--batch-steps", type=int, default=3)
+    p.add_argument("--autotune", type=int, default=1, help="0 skips the autotuner of the uneven-batch leg")
+    p.add_argument("--hard-mix", type=int, default=256,
+                   help="events of the hard-mix leg (workload 5, plain and learned, in sub-batches), 0 skips it")
+    p.add_argument("--hard-mix-batch", type=int, default=64)
+    p.add_argument("--learned-batch", type=int, default=32,
+                   help="events per batch of the batched plain-vs-learned leg on the main workload, 0 skips it")
     return p.parse_args()
 
 
@@ -116,6 +126,120 @@ def learned_leg(a, ev, step, plain_counts, plain_ms, plain_work):
             "tl_work_learned": work, "tl_work_plain": plain_work,
             "weights": os.path.relpath(a.learned, ROOT), "trained_on": meta.get("train_seeds"),
             "validated_on": meta.get("val_seeds")}
+
+
+def generate(workload, first, n):
+    """Events first .. first+n-1 of a workload, generated on host threads."""
+    from concurrent.futures import ThreadPoolExecutor
+    from sdl_amd import gen
+    gen.geometry(gen.geom_kind(workload))
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+        return list(pool.map(lambda i: gen.event(workload, i), range(first, first + n)))
+
+
+class Batch:
+    """A batch of generated events resident in HBM, fed to an event handle."""
+
+    def __init__(self, evs):
+        from sdl_amd import hip
+        self.n = len(evs)
+        self.off = np.zeros(self.n + 1, np.int64)
+        self.off[1:] = np.cumsum([e.nhits for e in evs])
+        cat = lambda f: np.ascontiguousarray(np.concatenate([getattr(e, f) for e in evs]))
+        self.dev = [hip.DeviceArray(cat(f)) for f in ("x", "y", "z", "idx", "detId")]
+        self.tracks = [e.ntracks for e in evs]
+        hip.synchronize()
+
+    def run(self, ev):
+        ev.set_hits_device(self.off, *[d.address for d in self.dev])
+        ev.run_chain()
+
+    def free(self):
+        for d in self.dev:
+            d.free()
+
+
+def timed(ev, fn, warmup, steps):
+    from sdl_amd import hip
+    for _ in range(warmup):
+        fn()
+    ev.sync()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    ev.sync()
+    hip.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def batch_leg(a, geom):
+    """The plain chain on one batch of events of uneven size (workload
+    --batch-workload: 2,000-10,000 tracks per event) as one set of launches,
+    after the autotuner has chosen the launch shapes on this batch."""
+    from sdl_amd import runtime
+    evs = generate(a.batch_workload, 0, a.batch)
+    b = Batch(evs)
+    ev = runtime.Event(geom)
+    b.run(ev)  # the batch the tuner times
+    tune = None
+    if a.autotune:
+        tune = ev.autotune(reps=3)
+    sec = timed(ev, lambda: b.run(ev), 1, a.batch_steps)
+    counts = ev.counts(-1)
+    out = {"workload": f"workload {a.batch_workload}: {a.batch} events, tracks per event uniform in "
+                       f"[2000, 10000] + 20 hard, one batch per step",
+           "events_per_step": a.batch, "value": round(a.batch / sec, 3), "unit": "events/s",
+           "ms_per_step": round(1000 * sec, 3), "hits_per_step": int(b.off[-1]),
+           "tracks_min_max": [min(b.tracks), max(b.tracks)], "objects_per_step": counts,
+           "tuning": ev.tuning(), "autotune": tune}
+    b.free()
+    return out
+
+
+def plain_vs_learned(a, geom, workload, nevents, batch, steps):
+    """Events 0 .. nevents-1 of a workload in batches of `batch` events: events/s
+    of the plain chain and of the learned proposal (at the threshold chosen for
+    the target TC recall) on the same batches, and the per-stage recall of the
+    learned chain against the plain one (its objects are a subset of the plain
+    chain's)."""
+    from sdl_amd import proposal, runtime
+    m, meta = proposal.MLP.load(a.learned)
+    t = (meta.get("thresholds") or {}).get(a.recall)
+    if t is None:
+        return {"skipped": f"no threshold for recall {a.recall} in {os.path.basename(a.learned)}"}
+    ev = runtime.Event(geom)
+    keys = ("nMD", "nSG", "nTP", "nTL", "nTC")
+    tot = {"plain": dict.fromkeys(keys, 0), "learned": dict.fromkeys(keys, 0)}
+    sec = {"plain": 0.0, "learned": 0.0}
+    work = {"plain": None, "learned": None}
+    for first in range(0, nevents, batch):
+        b = Batch(generate(workload, first, min(batch, nevents - first)))
+        for form in ("plain", "learned"):
+            proposal.set_proposal(ev, m if form == "learned" else None, t if form == "learned" else 0.5)
+            sec[form] += timed(ev, lambda: b.run(ev), 1, steps)
+            c = ev.counts(-1)
+            for k in keys:
+                tot[form][k] += c[k]
+            work[form] = ev.tl_work()
+        proposal.set_proposal(ev, None, 0.5)
+        b.free()
+    rec = {k[1:]: round(tot["learned"][k] / max(tot["plain"][k], 1), 6) for k in keys}
+    return {"events": nevents, "events_per_step": batch,
+            "plain_events_per_s": round(nevents / sec["plain"], 3),
+            "learned_events_per_s": round(nevents / sec["learned"], 3),
+            "speedup_vs_plain": round(sec["plain"] / sec["learned"], 4), "threshold": t,
+            "target_tc_recall": float(a.recall), "recall_per_stage": rec,
+            "objects_plain": tot["plain"], "objects_learned": tot["learned"],
+            "tl_work_last_batch": work}
+
+
+def hard_mix_leg(a, geom):
+    """Workload 5: 15 % low-pT and 5 % displaced tracks, 10 % noise, 2,000-10,000
+    tracks per event; --hard-mix events in sub-batches, plain and learned."""
+    out = plain_vs_learned(a, geom, 5, a.hard_mix, a.hard_mix_batch, 2)
+    out["workload"] = f"workload 5 (hard mix): {a.hard_mix} events in batches of {a.hard_mix_batch}"
+    return out
 
 
 def main():
@@ -201,6 +325,14 @@ def main():
     }
     if learned is not None:
         out["learned"] = learned
+    if a.learned_batch > 0 and os.path.exists(a.learned):
+        lb = plain_vs_learned(a, geom, a.workload, a.learned_batch, a.learned_batch, 3)
+        lb["workload"] = f"workload {a.workload}: {a.learned_batch} events as one batch per step"
+        out["learned_batched"] = lb
+    if a.batch > 0:
+        out["batched"] = batch_leg(a, runtime.Geometry(*gen.geometry(gen.geom_kind(a.batch_workload))[1]))
+    if a.hard_mix > 0 and os.path.exists(a.learned):
+        out["hard_mix"] = hard_mix_leg(a, runtime.Geometry(*gen.geometry(gen.geom_kind(5))[1]))
     if a.cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(a.workload, threads)

@@ -126,6 +126,19 @@ int sdl_event_set_graphs(sdl_event* ev, int on);
  * predicted buffer sizes for the stages >= fromStage (default 1.25 for all).
  * A factor below 1 forces the resume path; the tests use it to check it. */
 int sdl_event_set_headroom(sdl_event* ev, int fromStage, double factor);
+/* Autotuner with no counterpart in the reference.  Launch shapes of the main
+ * kernels (values in this order): md_block, sg_block, tl1_block, tl2_block,
+ * tl_expand_block (block sizes 64, 128 or 256 of the mini-doublet, segment,
+ * first- and second-stage tracklet and candidate-expansion kernels) and
+ * score_blocks (grid of the proposal's scoring kernel: 256 ... 2048).
+ * sdl_event_autotune times run_chain on the handle's current batch (median of
+ * `reps` runs per setting), tries each knob's values in turn and keeps the
+ * fastest (a change must gain 1 %); the timings and choice are written to
+ * `report` as JSON.  Object sets do not depend on the launch shapes.
+ * get/set return or restore a choice (n = 6 values; get returns 6). */
+int sdl_event_autotune(sdl_event* ev, int reps, char* report, int cap);
+int sdl_event_get_tuning(sdl_event* ev, int* values, int n);
+int sdl_event_set_tuning(sdl_event* ev, const int* values, int n);
 
 /* ---------------------------------------------------------------- learned proposal
  * No counterpart in the reference.  A 24-64-64-64-1 MLP scores the tracklet

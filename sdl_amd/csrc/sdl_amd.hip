@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -720,7 +721,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
                                                int* slabOut, SGF* slabF, int* sgCount, Guard g) {
     if (g.over()) return;
     const int nMD = int(cMD.get());
-    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (item >= nMD) return;
     const int cap = sgCap[item];
@@ -1647,6 +1648,24 @@ struct sdl_mlp {
     long long adamSteps = 0;
 };
 
+// Launch shapes of the kernels whose best shape depends on the batch and the
+// device: block sizes of the thread-per-item kernels, the wave-per-MD segment
+// kernel's block, and the grid of the persistent scoring kernel.
+enum { TUNE_MD_BLOCK = 0, TUNE_SG_BLOCK, TUNE_TL1_BLOCK, TUNE_TL2_BLOCK, TUNE_TL_EXPAND_BLOCK, TUNE_SCORE_BLOCKS, TUNE_N };
+struct Tuning {
+    int v[TUNE_N] = {256, 256, 256, 256, 256, 1024};
+    bool operator==(const Tuning& o) const {
+        for (int k = 0; k < TUNE_N; ++k)
+            if (v[k] != o.v[k]) return false;
+        return true;
+    }
+};
+const char* const kTuneNames[TUNE_N] = {"md_block", "sg_block", "tl1_block", "tl2_block", "tl_expand_block",
+                                        "score_blocks"};
+// Candidate values the autotuner tries for each knob (0 ends a list).
+const int kTuneValues[TUNE_N][5] = {{64, 128, 256, 0}, {64, 128, 256, 0}, {64, 128, 256, 0}, {64, 128, 256, 0},
+                                    {64, 128, 256, 0}, {256, 512, 1024, 2048, 0}};
+
 struct sdl_event {
     sdl_geom* g = nullptr;
     hipStream_t st = nullptr;
@@ -1674,6 +1693,8 @@ struct sdl_event {
     bool haveHistory = false;
     long long lastTot[TOT_N] = {};
     long long lastNH = 0;
+    // launch shapes (chosen by sdl_event_autotune; defaults below)
+    Tuning tune;
     // captured chain (speculative mode)
     bool useGraphs = true;
     hipGraph_t graph = nullptr;
@@ -1686,6 +1707,7 @@ struct sdl_event {
         double headroomFactor = 0;
         const void* proposal = nullptr;
         float proposalLogit = 0.f;
+        Tuning tune;
     } graphKey;
     // prediction headroom: factor for the totals of stages >= headroomFrom
     int headroomFrom = SDL_STAGE_TCL + 1;
@@ -1977,7 +1999,8 @@ int runMD(sdl_event* ev) {
     zeroAsync(ev->mdCount.p, NS, st);
     timerBegin(ev, T_MD);
     if (slab > 0)
-        hipLaunchKernelGGL(k_md, dim3(gridFor(slab, kBlock)), dim3(kBlock), 0, st, devCount(ev, TOT_MDCAP), NS, G->NM,
+        hipLaunchKernelGGL(k_md, dim3(gridFor(slab, ev->tune.v[TUNE_MD_BLOCK])), dim3(ev->tune.v[TUNE_MD_BLOCK]), 0, st,
+                           devCount(ev, TOT_MDCAP), NS, G->NM,
                            G->dMods, H, ev->slotHitStart.p, ev->sortedHit.p, ev->mdCapOff.p, ev->mdSlabHits.p,
                            ev->mdSlabF.p, ev->mdSlabSlot.p, ev->mdFlag.p, ev->mdCount.p,
                            guardOf(ev, TOT_MDCAP, slab, SDL_STAGE_MD));
@@ -2026,7 +2049,8 @@ int runSG(sdl_event* ev) {
     if (int r = ev->sgSlabF.ensure(slab)) return r;
     timerBegin(ev, T_SG);
     if (nMD > 0)
-        hipLaunchKernelGGL(k_sg, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, cMD, G->NM, G->dMods,
+        hipLaunchKernelGGL(k_sg, dim3(gridFor(nMD, ev->tune.v[TUNE_SG_BLOCK] / kWave)), dim3(ev->tune.v[TUNE_SG_BLOCK]), 0,
+                           st, cMD, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), G->dConnStart, G->dConnList, ev->mdStart.p, ev->mdCount.p,
                            ev->sgCap.p, ev->sgCapOff.p, ev->sgSlabOut.p, ev->sgSlabF.p, ev->sgCount.p,
                            guardOf(ev, TOT_SGCAP, slab, SDL_STAGE_SG));
@@ -2168,32 +2192,34 @@ int runTL(sdl_event* ev) {
     for (int v = 0; v < kNVariantsTL; ++v) {
         if (np[v] == 0) continue;
         const VarRange R{ev->tlPOff.p, nullptr, ns, v, np[v], ev->tot.p};
-        const dim3 grid(gridFor(np[v], kBlock));
+        const int b1 = ev->tune.v[TUNE_TL1_BLOCK];
+        const dim3 grid(gridFor(np[v], b1));
         if (learned) {
             mlp::short8v* rows = ev->tlRows.p;
             if (v == 0)
-                hipLaunchKernelGGL(k_tl_stage1_emit<TL_BBBB>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
+                hipLaunchKernelGGL(k_tl_stage1_emit<TL_BBBB>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p,
                                    ev->tlPair.p, ev->tlCarry.p, ev->tlNOut.p, rows);
             else if (v == 1)
-                hipLaunchKernelGGL(k_tl_stage1_emit<TL_BBEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
+                hipLaunchKernelGGL(k_tl_stage1_emit<TL_BBEE>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p,
                                    ev->tlPair.p, ev->tlCarry.p, ev->tlNOut.p, rows);
             else
-                hipLaunchKernelGGL(k_tl_stage1_emit<TL_EEEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p,
+                hipLaunchKernelGGL(k_tl_stage1_emit<TL_EEEE>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p,
                                    ev->tlPair.p, ev->tlCarry.p, ev->tlNOut.p, rows);
             continue;
         }
         if (v == 0)
-            hipLaunchKernelGGL(k_tl_stage1<TL_BBBB>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
+            hipLaunchKernelGGL(k_tl_stage1<TL_BBBB>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
                                ev->tlCarry.p, ev->tlNOut.p);
         else if (v == 1)
-            hipLaunchKernelGGL(k_tl_stage1<TL_BBEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
+            hipLaunchKernelGGL(k_tl_stage1<TL_BBEE>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
                                ev->tlCarry.p, ev->tlNOut.p);
         else
-            hipLaunchKernelGGL(k_tl_stage1<TL_EEEE>, grid, dim3(kBlock), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
+            hipLaunchKernelGGL(k_tl_stage1<TL_EEEE>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
                                ev->tlCarry.p, ev->tlNOut.p);
     }
     if (learned && nPairs > 0) {
-        const dim3 grid(std::min<unsigned>(gridFor((nPairs + 63) / 64, kWavesPerBlock), 1024u));
+        const dim3 grid(std::min<unsigned>(gridFor((nPairs + 63) / 64, kWavesPerBlock),
+                                           unsigned(ev->tune.v[TUNE_SCORE_BLOCKS])));
         hipLaunchKernelGGL(k_tl_score_rows, grid, dim3(kBlock), 0, st, ev->tot.p, ev->tlRows.p,
                            ev->proposal->dPacked, ev->proposalLogit, ev->tlNOut.p, ev->tot.p);
     }
@@ -2206,7 +2232,8 @@ int runTL(sdl_event* ev) {
     if (int r = ev->tlCand.ensure(std::max(nCands, 1LL))) return r;
     if (int r = ev->tlFlag.ensure((std::max(nCands, 1LL) + 3) / 4 * 4)) return r;
     if (nPairs > 0)
-        hipLaunchKernelGGL(k_tl_expand, dim3(gridFor(nPairs, kBlock)), dim3(kBlock), 0, st, cPairs, ev->sgStart.p,
+        hipLaunchKernelGGL(k_tl_expand, dim3(gridFor(nPairs, ev->tune.v[TUNE_TL_EXPAND_BLOCK])),
+                           dim3(ev->tune.v[TUNE_TL_EXPAND_BLOCK]), 0, st, cPairs, ev->sgStart.p,
                            ev->tlPair.p, ev->tlNOut.p, ev->tlCOff.p, ev->tlCand.p,
                            guardOf(ev, TOT_TLCANDS, nCands, SDL_STAGE_TL));
     long long nc[kNVariantsTL];
@@ -2221,15 +2248,16 @@ int runTL(sdl_event* ev) {
     for (int v = 0; v < kNVariantsTL; ++v) {
         if (nc[v] == 0) continue;
         const VarRange R{ev->tlPOff.p, ev->tlCOff.p, ns, v, nc[v], ev->tot.p};
-        const dim3 grid(gridFor(nc[v], kBlock));
+        const int b2 = ev->tune.v[TUNE_TL2_BLOCK];
+        const dim3 grid(gridFor(nc[v], b2));
         if (v == 0)
-            hipLaunchKernelGGL(k_tl_stage2<TL_BBBB>, grid, dim3(kBlock), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
+            hipLaunchKernelGGL(k_tl_stage2<TL_BBBB>, grid, dim3(b2), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
                                ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p);
         else if (v == 1)
-            hipLaunchKernelGGL(k_tl_stage2<TL_BBEE>, grid, dim3(kBlock), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
+            hipLaunchKernelGGL(k_tl_stage2<TL_BBEE>, grid, dim3(b2), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
                                ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p);
         else
-            hipLaunchKernelGGL(k_tl_stage2<TL_EEEE>, grid, dim3(kBlock), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
+            hipLaunchKernelGGL(k_tl_stage2<TL_EEEE>, grid, dim3(b2), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
                                ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p);
     }
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
@@ -2359,7 +2387,7 @@ bool sameGraphKey(const sdl_event* ev) {
     return k.E == ev->E && k.px == ev->px && k.py == ev->py && k.pz == ev->pz && k.pidx == ev->pidx &&
            k.pdet == ev->pdet && k.allocEpoch == g_allocEpoch && k.headroomFrom == ev->headroomFrom &&
            k.headroomFactor == ev->headroomFactor && k.proposal == ev->proposal &&
-           k.proposalLogit == ev->proposalLogit;
+           k.proposalLogit == ev->proposalLogit && k.tune == ev->tune;
 }
 
 void dropGraph(sdl_event* ev) {
@@ -2403,7 +2431,7 @@ int launchChainGraph(sdl_event* ev) {
     auto& k = ev->graphKey;
     k.E = ev->E; k.px = ev->px; k.py = ev->py; k.pz = ev->pz; k.pidx = ev->pidx; k.pdet = ev->pdet;
     k.allocEpoch = g_allocEpoch; k.headroomFrom = ev->headroomFrom; k.headroomFactor = ev->headroomFactor;
-    k.proposal = ev->proposal; k.proposalLogit = ev->proposalLogit;
+    k.proposal = ev->proposal; k.proposalLogit = ev->proposalLogit; k.tune = ev->tune;
     HIP_OK(hipGraphLaunch(ev->graphExec, st));
     ev->hitsReady = true;
     return 1;
@@ -2779,6 +2807,87 @@ int sdl_event_set_graphs(sdl_event* ev, int on) {
     if (!ev) return fail(SDL_ERR_ARG, "null event");
     ev->useGraphs = on != 0;
     if (!ev->useGraphs) dropGraph(ev);
+    return 0;
+}
+
+int sdl_event_get_tuning(sdl_event* ev, int* values, int n) {
+    if (!ev || !values || n < TUNE_N) return fail(SDL_ERR_ARG, "bad arguments");
+    for (int k = 0; k < TUNE_N; ++k) values[k] = ev->tune.v[k];
+    return TUNE_N;
+}
+
+int sdl_event_set_tuning(sdl_event* ev, const int* values, int n) {
+    if (!ev || !values || n != TUNE_N) return fail(SDL_ERR_ARG, "bad arguments");
+    for (int k = 0; k < TUNE_N; ++k) {
+        bool ok = false;
+        for (int j = 0; kTuneValues[k][j]; ++j) ok = ok || values[k] == kTuneValues[k][j];
+        if (!ok) return fail(SDL_ERR_ARG, std::string("unsupported value for ") + kTuneNames[k]);
+    }
+    for (int k = 0; k < TUNE_N; ++k) ev->tune.v[k] = values[k];
+    return 0;
+}
+
+int sdl_event_autotune(sdl_event* ev, int reps, char* report, int cap) {
+    if (!ev || reps < 1) return fail(SDL_ERR_ARG, "bad arguments");
+    using clk = std::chrono::steady_clock;
+    // one timed chain: wall time of run_chain, which ends with its host round trip
+    auto timeChain = [&](double& ms) -> int {
+        const auto t0 = clk::now();
+        if (int r = sdl_event_run_chain(ev)) return r;
+        ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        return 0;
+    };
+    auto median = [&](double& out) -> int {
+        double warm;
+        if (int r = timeChain(warm)) return r;  // sizes, graph capture
+        std::vector<double> t(reps);
+        for (int k = 0; k < reps; ++k)
+            if (int r = timeChain(t[k])) return r;
+        std::sort(t.begin(), t.end());
+        out = t[reps / 2];
+        return 0;
+    };
+    std::string rep = "{";
+    double best;
+    if (int r = median(best)) return r;
+    if (ev->NH <= 0) return fail(SDL_ERR_STATE, "autotune needs a batch with hits");
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "\"start_ms\": %.4f", best);
+    rep += buf;
+    // one knob at a time, in launch order, keeping the fastest value
+    for (int k = 0; k < TUNE_N; ++k) {
+        const int keep = ev->tune.v[k];
+        int bestV = keep;
+        for (int j = 0; kTuneValues[k][j]; ++j) {
+            const int v = kTuneValues[k][j];
+            if (v == keep) continue;
+            ev->tune.v[k] = v;
+            double ms;
+            if (int r = median(ms)) {
+                ev->tune.v[k] = keep;
+                return r;
+            }
+            std::snprintf(buf, sizeof(buf), ", \"%s=%d\": %.4f", kTuneNames[k], v, ms);
+            rep += buf;
+            if (ms < best * 0.99) {  // a change must gain at least 1 %
+                best = ms;
+                bestV = v;
+            }
+        }
+        ev->tune.v[k] = bestV;
+    }
+    std::snprintf(buf, sizeof(buf), ", \"best_ms\": %.4f", best);
+    rep += buf;
+    for (int k = 0; k < TUNE_N; ++k) {
+        std::snprintf(buf, sizeof(buf), ", \"%s\": %d", kTuneNames[k], ev->tune.v[k]);
+        rep += buf;
+    }
+    rep += "}";
+    if (report && cap > 0) {
+        const size_t m = std::min<size_t>(rep.size(), size_t(cap - 1));
+        std::memcpy(report, rep.data(), m);
+        report[m] = 0;
+    }
     return 0;
 }
 

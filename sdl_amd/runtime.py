@@ -3,6 +3,7 @@
 Host plumbing only: every computation runs in the HIP kernels of the library.
 Loading fails loudly when the library has not been built."""
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -25,7 +26,9 @@ EXPORTS = (
     "sdl_event_set_headroom", "sdl_event_set_graphs",
     "sdl_mlp_create", "sdl_mlp_free", "sdl_mlp_num_params", "sdl_mlp_set_params", "sdl_mlp_get_params",
     "sdl_mlp_gradient", "sdl_mlp_train", "sdl_mlp_forward", "sdl_event_set_proposal", "sdl_event_tl_pair_rows", "sdl_event_tl_work",
+    "sdl_event_autotune", "sdl_event_get_tuning", "sdl_event_set_tuning",
 )
+TUNING = ("md_block", "sg_block", "tl1_block", "tl2_block", "tl_expand_block", "score_blocks")
 TIMERS = ("prep", "md", "sg", "tl")
 
 _lib = None
@@ -78,6 +81,9 @@ def lib():
         L.sdl_event_set_proposal.argtypes = [vp, vp, f]
         L.sdl_event_tl_pair_rows.argtypes = [vp, vp, vp, ll, ctypes.POINTER(ll)]
         L.sdl_event_tl_work.argtypes = [vp, vp]
+        L.sdl_event_autotune.argtypes = [vp, i, ctypes.c_char_p, i]
+        L.sdl_event_get_tuning.argtypes = [vp, vp, i]
+        L.sdl_event_set_tuning.argtypes = [vp, vp, i]
         _lib = L
     return _lib
 
@@ -181,6 +187,24 @@ class Event:
     def set_headroom(self, from_stage, factor):
         """Headroom factor of the predicted buffer sizes for stages >= from_stage."""
         _check(lib().sdl_event_set_headroom(self.h, from_stage, factor))
+
+    def autotune(self, reps=3):
+        """Times run_chain on the current batch for each launch-shape knob and
+        keeps the fastest values; returns the report (timings and choice)."""
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().sdl_event_autotune(self.h, reps, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def tuning(self):
+        v = np.zeros(len(TUNING), np.int32)
+        rc = lib().sdl_event_get_tuning(self.h, v.ctypes.data, len(v))
+        if rc != len(TUNING):
+            _check(rc if rc < 0 else -1)
+        return dict(zip(TUNING, v.tolist()))
+
+    def set_tuning(self, t):
+        v = np.asarray([t[k] for k in TUNING], np.int32)
+        _check(lib().sdl_event_set_tuning(self.h, v.ctypes.data, len(v)))
 
     def tl_work(self):
         """(pairs, second-stage candidates, tracklets, pairs scored by the proposal)

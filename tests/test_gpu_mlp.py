@@ -116,3 +116,22 @@ def test_proposal_only_removes_objects():
         b = {tuple(r) for r in ref[s]}
         assert a <= b, s
     P.set_proposal(ev, None, 0.5)
+
+
+def test_proposal_result_independent_of_launch_shape():
+    print("case proposal launch shapes", flush=True)
+    wl = 12
+    e = gen.event(wl, 1)
+    g = runtime.Geometry(*gen.geometry(gen.geom_kind(wl))[1])
+    ev = runtime.Event(g)
+    m = P.MLP(P.init_params(seed=13))
+    P.set_proposal(ev, m, 0.4)
+    base = ev.tuning()
+    got = []
+    for blocks, b1 in ((256, 64), (2048, 256), (512, 128)):
+        ev.set_tuning(dict(base, score_blocks=blocks, tl1_block=b1))
+        got.append(_chain(ev, e))
+    for s in ("TL", "TCP", "TCL"):
+        assert np.array_equal(oracle.as_multiset(got[0][s]), oracle.as_multiset(got[1][s])), s
+        assert np.array_equal(oracle.as_multiset(got[0][s]), oracle.as_multiset(got[2][s])), s
+    P.set_proposal(ev, None, 0.5)

@@ -206,3 +206,43 @@ def test_graph_replays_match_eager():
         _assert_same(eager.export_all(0), ref, f"eager pass {k}")
         assert ev.counts(0) == cnt
         assert np.array_equal(ev.layer_counts(0), eager.layer_counts(0))
+
+
+def test_autotune_keeps_objects():
+    """The autotuner only changes launch shapes: after tuning, and with every
+    knob at another allowed value, a batch of uneven events gives the same
+    objects; the choice is one of the allowed values; bad values are rejected."""
+    from sdl_amd import hip
+    print("case autotune, batch of workload 15 events 0-1 + workload 12 event 0", flush=True)
+    evs = [gen.event(15, 0), gen.event(15, 1), gen.event(12, 0)]
+    off = np.zeros(len(evs) + 1, np.int64)
+    off[1:] = np.cumsum([e.nhits for e in evs])
+    dev = [hip.DeviceArray(np.ascontiguousarray(np.concatenate([getattr(e, f) for e in evs])))
+           for f in ("x", "y", "z", "idx", "detId")]
+    hip.synchronize()
+    ev = runtime.Event(_geom(2))
+    ev.set_hits_device(off, *[d.address for d in dev])
+    ev.run_chain()
+    ev.sync()
+    before = [ev.export_all(i) for i in range(len(evs))]
+    ref, _ = oracle.run(gen.geometry(2)[1], evs[0], oracle.CR)
+    _assert_same(before[0], ref, "before tuning")
+    rep = ev.autotune(reps=1)
+    chosen = ev.tuning()
+    assert set(runtime.TUNING) <= set(rep)
+    assert rep["best_ms"] <= rep["start_ms"]
+    allowed = {"md_block": {64, 128, 256}, "sg_block": {64, 128, 256}, "tl1_block": {64, 128, 256},
+               "tl2_block": {64, 128, 256}, "tl_expand_block": {64, 128, 256},
+               "score_blocks": {256, 512, 1024, 2048}}
+    for k, v in chosen.items():
+        assert v in allowed[k] and rep[k] == v, k
+    for setting in (chosen, {k: min(v) for k, v in allowed.items()}, {k: max(v) for k, v in allowed.items()}):
+        ev.set_tuning(setting)
+        ev.run_chain()
+        ev.sync()
+        for i in range(len(evs)):
+            _assert_same(ev.export_all(i), before[i], f"tuning {setting} event {i}")
+    with pytest.raises(runtime.SDLError):
+        ev.set_tuning(dict(chosen, md_block=100))
+    for d in dev:
+        d.free()
