@@ -60,6 +60,20 @@ int sdl_geom_load(const char* tiltedPath, const char* endcapPath, const char* ma
 void sdl_geom_free(sdl_geom* g);
 /* Number of sensor modules known to the geometry. */
 int sdl_geom_num_modules(const sdl_geom* g);
+/* Binary geometry cache (no counterpart in the reference; it replaces the text
+ * parsing of TiltedGeometry::load, EndcapGeometry::load and
+ * ModuleConnectionMap::load, TiltedGeometry.cc:18-41, EndcapGeometry.cc:18-61,
+ * ModuleConnectionMap.cc:18-47, at start-up).  The cache holds the parsed module
+ * table and CSR connection map exactly as the device uses them, with a version,
+ * the record size and a checksum; a file that fails any check is rejected with
+ * SDL_ERR_IO.  sdl_geom_convert parses the three text files and writes the
+ * cache without touching the GPU; sdl_geom_save writes a loaded geometry's
+ * cache; sdl_geom_load_binary loads a cache onto the current HIP device;
+ * sdl_geom_check_binary validates a cache on the host only. */
+int sdl_geom_convert(const char* tiltedPath, const char* endcapPath, const char* mapPath, const char* binaryPath);
+int sdl_geom_save(const sdl_geom* g, const char* binaryPath);
+int sdl_geom_load_binary(const char* binaryPath, sdl_geom** out);
+int sdl_geom_check_binary(const char* binaryPath, int* numModules, long long* numConnections);
 
 /* Creates an event handle (batch of events) on the geometry's device. */
 int sdl_event_create(sdl_geom* g, sdl_event** out);
@@ -130,7 +144,8 @@ int sdl_event_set_headroom(sdl_event* ev, int fromStage, double factor);
  * kernels (values in this order): md_block, sg_block, tl1_block, tl2_block,
  * tl_expand_block (block sizes 64, 128 or 256 of the mini-doublet, segment,
  * first- and second-stage tracklet and candidate-expansion kernels) and
- * score_blocks (grid of the proposal's scoring kernel: 256 ... 2048).
+ * scored_waves (waves per block of the learned proposal's first stage: 2, 4
+ * or 8).
  * sdl_event_autotune times run_chain on the handle's current batch (median of
  * `reps` runs per setting), tries each knob's values in turn and keeps the
  * fastest (a change must gain 1 %); the timings and choice are written to

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 #include <string>
 #include <unordered_map>
@@ -106,7 +107,6 @@ enum {
     TOT_TLC0, TOT_TLC1, TOT_TLC2,  // second-stage tracklet candidates per variant
     TOT_TLP0, TOT_TLP1, TOT_TLP2,  // first-stage tracklet pairs per variant
     TOT_TLPAIRS, TOT_TLCANDS,      // all first-stage pairs, all second-stage candidates
-    TOT_TLSCORED,                  // pairs scored by the learned proposal
     TOT_ABORT, TOT_UNKNOWN, TOT_N
 };
 
@@ -1063,7 +1063,7 @@ __global__ void __launch_bounds__(kBlock) k_tl_stage1(VarRange R, MDDev D, SGDev
     const TLOutLo L = loadOutLo(pr.y, S.flags[sgA], D);
     TLCarry c{0.f, 0.f, 0.f};
     const bool ok = tlStage1(V, I, L, c);
-    carry[i] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);
+    if (ok) carry[i] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);  // read for surviving pairs only
     nOut[i] = ok ? int(sgStart[pr.y + 1]) - sgA : 0;
 }
 
@@ -1119,99 +1119,65 @@ __device__ __forceinline__ bool stage1Features(long long i, const int2* pair, co
     return ok;
 }
 
-// First stage with the learned proposal, one thread per pair: as k_tl_stage1,
-// and every pair that passes writes its BF16 feature row at its own index.
+// First stage with the learned proposal: one wave per tile of 64 pairs,
+// kScoreWaves waves per block (the block loads the network's LDS operands once).
+// Each lane runs the first-stage cuts for its pair; the pairs that pass write
+// their BF16 feature rows to the wave's LDS tile (ballot compaction) and are
+// scored 16 rows at a time (one MFMA column block) with the BF16 network.  A
+// pair keeps its outer SGs (nOut) only if its logit reaches the threshold;
+// pairs that fail the cuts get nOut = 0.  scoredPerWave[wave] = the number of
+// pairs the wave scored (a diagnostic; no atomics on a shared counter).  The
+// waves per block are a launch parameter (dynamic LDS: kScoredLdsPerWave bytes
+// per wave).
+constexpr int kScoredLdsPerWave = 64 * 4 * 16 + 64 * 4 + 64 * 4;  // rows, pair indices, outer SG counts
 template <int V>
-__global__ void __launch_bounds__(kBlock) k_tl_stage1_emit(VarRange R, MDDev D, SGDev S, const long long* sgStart,
-                                                           const int2* pair, float4* carry, int* nOut,
-                                                           mlp::short8v* rows) {
+__global__ void __launch_bounds__(1024)
+k_tl_stage1_scored(VarRange R, MDDev D, SGDev S, const long long* sgStart, const int2* pairAll, const mlp::Packed* P,
+                   float logitCut, float4* carryAll, int* nOutAll, int* scoredPerWave) {
+    __shared__ __align__(16) mlp::Vecs Vs;
+    extern __shared__ __align__(16) char dynLds[];
+    mlp::loadVecs(P, Vs);  // includes a block barrier
     long long b0, e0;
     R.get(b0, e0);
-    const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= e0) return;
-    TLCarry c;
-    int nOuter;
-    float f[mlp::kIn];
-    const bool ok = stage1Features<V>(i, pair, sgStart, D, S, c, nOuter, f);
-    carry[i] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);
-    nOut[i] = ok ? nOuter : 0;
-    if (!ok) return;
-    mlp::short8v v[4];
-    mlp::featureRow(f, v);
-    for (int q = 0; q < 4; ++q) rows[i * 4 + q] = v[q];
-}
-
-// Scores the pairs that passed the first stage (nOut > 0), 64 at a time per
-// wave: each wave walks tiles of 64 pairs, queues the indices of the passing
-// ones in LDS (compacted with a ballot) and scores every 64 queued rows with
-// the BF16 MFMA network; a pair whose logit is below the threshold gets
-// nOut = 0 (its outer SGs are not tried).  Adds the number scored to
-// tot[TOT_TLSCORED] (one atomic per block).
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) k_tl_score_rows(const long long* tot, const mlp::short8v* rows,
-                                                          const mlp::Packed* P, float logitCut, int* nOut,
-                                                          long long* totOut) {
-    constexpr int kQ = 128;  // < 64 left over + 64 new
-    __shared__ __align__(16) mlp::Vecs Vs;
-    __shared__ int queueAll[kWavesPerBlock][kQ];
-    __shared__ int scoredBlock;
-    if (threadIdx.x == 0) scoredBlock = 0;
-    mlp::loadVecs(P, Vs);  // includes a block barrier
-    if (tot[TOT_ABORT]) return;
-    const long long n = tot[TOT_TLPAIRS];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long n = e0 - b0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nWaves = blockDim.x >> 6;
     const int g = lane >> 4, c = lane & 15;
-    int* queue = queueAll[w];
-    const long long nTiles = (n + 63) / 64;
-    const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    mlp::short8v* rows = reinterpret_cast<mlp::short8v*>(dynLds) + w * 64 * 4;
+    int* qPair = reinterpret_cast<int*>(dynLds + nWaves * 64 * 4 * 16) + w * 64;
+    int* qOut = reinterpret_cast<int*>(dynLds + nWaves * (64 * 4 * 16 + 64 * 4)) + w * 64;
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x);  // this lane's pair
+    bool ok = false;
+    int nOuter = 0;
+    if (i < n) {
+        TLCarry cr;
+        float f[mlp::kIn];
+        ok = stage1Features<V>(i, pairAll + b0, sgStart, D, S, cr, nOuter, f);
+        if (ok) {
+            carryAll[b0 + i] = make_float4(cr.sdlCut, cr.thetaMuls, cr.sinDPhiMid, 0.f);
+            const unsigned long long m = __ballot(1);
+            const int pos = __popcll(m & ((1ull << lane) - 1));
+            mlp::short8v v[4];
+            mlp::featureRow(f, v);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) rows[pos * 4 + k] = v[k];
+            qPair[pos] = int(i);
+            qOut[pos] = nOuter;
+        } else {
+            nOutAll[b0 + i] = 0;
+        }
+    }
+    const int q = __popcll(__ballot(ok));  // rows queued by this wave
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (lane == 0 && wave * 64 < n) scoredPerWave[wave] = q;
+    if (q == 0) return;  // uniform over the wave (no block barrier follows)
+    mlp::waveSync();
     mlp::Frags F;
     mlp::loadFrags(P, lane, F);
-    int q = 0, scored = 0;  // uniform over the wave
-    auto outerOf = [&](long long t) { return t < nTiles && t * 64 + lane < n ? nOut[t * 64 + lane] : 0; };
-    int cur = outerOf(w0);
-    for (long long t = w0;; t += nw) {
-        const bool more = t < nTiles;
-        const int next = outerOf(t + nw);  // in flight while this tile is handled
-        if (more) {
-            const bool ok = cur > 0;
-            const unsigned long long mask = __ballot(ok);
-            if (ok) queue[q + __popcll(mask & ((1ull << lane) - 1))] = int(t * 64 + lane);
-            q += __popcll(mask);
-            mlp::waveSync();
-        }
-        cur = next;
-        if (q >= 64 || (!more && q > 0)) {
-            mlp::short8v x[4];
-#pragma unroll
-            for (int st = 0; st < 4; ++st) {
-                const int r = st * 16 + c;
-                const long long idx = queue[r < q ? r : 0];  // rows past the queue repeat a valid row
-                x[st] = rows[idx * 4 + g];
-            }
-            float z = 0.f;
-#pragma unroll
-            for (int st = 0; st < 4; ++st) {
-                const float zs = mlp::logit16(__builtin_bit_cast(mlp::bf16x8, x[st]), lane, F, Vs);
-                if (g == st) z = zs;  // lane st * 16 + c holds row st * 16 + c
-            }
-            const int take = q < 64 ? q : 64;
-            if (lane < take && z < logitCut) nOut[queue[lane]] = 0;
-            scored += take;
-            mlp::waveSync();
-            const int left = q - take;
-            int moved = 0;
-            if (lane < left) moved = queue[64 + lane];
-            mlp::waveSync();
-            if (lane < left) queue[lane] = moved;
-            mlp::waveSync();
-            q = left;
-        }
-        if (!more && q == 0) break;
+    for (int base = 0; base < q; base += 16) {  // rows past q hold stale data: they only affect their own logits
+        const int r = base + c;
+        const float z = mlp::logit16(__builtin_bit_cast(mlp::bf16x8, rows[r * 4 + g]), lane, F, Vs);
+        if (g == 0 && r < q) nOutAll[b0 + qPair[r]] = z >= logitCut ? qOut[r] : 0;
     }
-    if (lane == 0) atomicAdd(&scoredBlock, scored);
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long*>(totOut + TOT_TLSCORED),
-                                    (unsigned long long)scoredBlock);
 }
 
 // Training rows of the plain chain: for every pair, its features and label
@@ -1650,10 +1616,10 @@ struct sdl_mlp {
 
 // Launch shapes of the kernels whose best shape depends on the batch and the
 // device: block sizes of the thread-per-item kernels, the wave-per-MD segment
-// kernel's block, and the grid of the persistent scoring kernel.
-enum { TUNE_MD_BLOCK = 0, TUNE_SG_BLOCK, TUNE_TL1_BLOCK, TUNE_TL2_BLOCK, TUNE_TL_EXPAND_BLOCK, TUNE_SCORE_BLOCKS, TUNE_N };
+// kernel's block, and the waves per block of the learned first stage.
+enum { TUNE_MD_BLOCK = 0, TUNE_SG_BLOCK, TUNE_TL1_BLOCK, TUNE_TL2_BLOCK, TUNE_TL_EXPAND_BLOCK, TUNE_SCORED_WAVES, TUNE_N };
 struct Tuning {
-    int v[TUNE_N] = {256, 256, 256, 256, 256, 1024};
+    int v[TUNE_N] = {256, 256, 256, 256, 256, 8};
     bool operator==(const Tuning& o) const {
         for (int k = 0; k < TUNE_N; ++k)
             if (v[k] != o.v[k]) return false;
@@ -1661,10 +1627,10 @@ struct Tuning {
     }
 };
 const char* const kTuneNames[TUNE_N] = {"md_block", "sg_block", "tl1_block", "tl2_block", "tl_expand_block",
-                                        "score_blocks"};
+                                        "scored_waves"};
 // Candidate values the autotuner tries for each knob (0 ends a list).
 const int kTuneValues[TUNE_N][5] = {{64, 128, 256, 0}, {64, 128, 256, 0}, {64, 128, 256, 0}, {64, 128, 256, 0},
-                                    {64, 128, 256, 0}, {256, 512, 1024, 2048, 0}};
+                                    {64, 128, 256, 0}, {2, 4, 8, 0}};
 
 struct sdl_event {
     sdl_geom* g = nullptr;
@@ -1754,7 +1720,8 @@ struct sdl_event {
     DBuf<float4> tlCarry;
     DBuf<int> tlNOut;
     DBuf<unsigned char> tlFlag;
-    DBuf<mlp::short8v> tlRows;  // learned proposal: BF16 feature rows of the pairs that pass the first stage
+    DBuf<int> tlScored;  // learned proposal: pairs scored per wave, per variant (diagnostic)
+    long long tlScoredStride = 0;
     // learned proposal (null: the plain chain)
     sdl_mlp* proposal = nullptr;
     float proposalLogit = 0.f;
@@ -1922,7 +1889,7 @@ long long sizeOf(sdl_event* ev, int k) {
         static const int stageOf[TOT_N] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                            SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
                                            SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL,
-                                           SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, 0, 0};
+                                           SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, 0, 0};
         const double scale = ev->lastNH > 0 ? double(ev->NH) / double(ev->lastNH) : 1.0;
         const bool custom = stageOf[k] >= ev->headroomFrom;
         const double f = custom ? ev->headroomFactor : 1.25;
@@ -2185,26 +2152,30 @@ int runTL(sdl_event* ev) {
     const MDDev D = mdView(ev);
     const SGDev S = sgView(ev);
     const bool learned = ev->proposal != nullptr;
-    if (learned) {
-        if (int r = ev->tlRows.ensure(std::max(nPairs, 1LL) * 4)) return r;
-        zeroAsync(ev->tot.p + TOT_TLSCORED, 1, st);
-    }
+    const long long waveStride = (nPairs + 63) / 64 + 1;  // per-variant region of the per-wave scored counts
+    ev->tlScoredStride = waveStride;
+    if (learned)
+        if (int r = ev->tlScored.ensure(kNVariantsTL * waveStride)) return r;
     for (int v = 0; v < kNVariantsTL; ++v) {
         if (np[v] == 0) continue;
         const VarRange R{ev->tlPOff.p, nullptr, ns, v, np[v], ev->tot.p};
         const int b1 = ev->tune.v[TUNE_TL1_BLOCK];
         const dim3 grid(gridFor(np[v], b1));
         if (learned) {
-            mlp::short8v* rows = ev->tlRows.p;
+            const int sw = ev->tune.v[TUNE_SCORED_WAVES];
+            const dim3 sgrid(gridFor(np[v], sw * kWave));
+            const size_t lds = size_t(sw) * kScoredLdsPerWave;
+            const mlp::Packed* P = ev->proposal->dPacked;
+            const float cut = ev->proposalLogit;
             if (v == 0)
-                hipLaunchKernelGGL(k_tl_stage1_emit<TL_BBBB>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p,
-                                   ev->tlPair.p, ev->tlCarry.p, ev->tlNOut.p, rows);
+                hipLaunchKernelGGL(k_tl_stage1_scored<TL_BBBB>, sgrid, dim3(sw * kWave), lds, st, R, D, S, ev->sgStart.p,
+                                   ev->tlPair.p, P, cut, ev->tlCarry.p, ev->tlNOut.p, ev->tlScored.p + 0 * waveStride);
             else if (v == 1)
-                hipLaunchKernelGGL(k_tl_stage1_emit<TL_BBEE>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p,
-                                   ev->tlPair.p, ev->tlCarry.p, ev->tlNOut.p, rows);
+                hipLaunchKernelGGL(k_tl_stage1_scored<TL_BBEE>, sgrid, dim3(sw * kWave), lds, st, R, D, S, ev->sgStart.p,
+                                   ev->tlPair.p, P, cut, ev->tlCarry.p, ev->tlNOut.p, ev->tlScored.p + 1 * waveStride);
             else
-                hipLaunchKernelGGL(k_tl_stage1_emit<TL_EEEE>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p,
-                                   ev->tlPair.p, ev->tlCarry.p, ev->tlNOut.p, rows);
+                hipLaunchKernelGGL(k_tl_stage1_scored<TL_EEEE>, sgrid, dim3(sw * kWave), lds, st, R, D, S, ev->sgStart.p,
+                                   ev->tlPair.p, P, cut, ev->tlCarry.p, ev->tlNOut.p, ev->tlScored.p + 2 * waveStride);
             continue;
         }
         if (v == 0)
@@ -2216,12 +2187,6 @@ int runTL(sdl_event* ev) {
         else
             hipLaunchKernelGGL(k_tl_stage1<TL_EEEE>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
                                ev->tlCarry.p, ev->tlNOut.p);
-    }
-    if (learned && nPairs > 0) {
-        const dim3 grid(std::min<unsigned>(gridFor((nPairs + 63) / 64, kWavesPerBlock),
-                                           unsigned(ev->tune.v[TUNE_SCORE_BLOCKS])));
-        hipLaunchKernelGGL(k_tl_score_rows, grid, dim3(kBlock), 0, st, ev->tot.p, ev->tlRows.p,
-                           ev->proposal->dPacked, ev->proposalLogit, ev->tlNOut.p, ev->tot.p);
     }
     // candidates of the surviving pairs: one list, variant-major
     const Count cPairs = devCount(ev, TOT_TLPAIRS);
@@ -2482,7 +2447,7 @@ int runStages(sdl_event* ev, int s0, int s1) {
     static const int stageOfTot[TOT_ABORT] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                               SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
                                               SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL,
-                                              SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL};
+                                              SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TL};
     for (int k = 0; k < TOT_ABORT; ++k)
         if (stageOfTot[k] >= s0 && stageOfTot[k] <= s1) ev->lastTot[k] = t[k];
     if (s1 == SDL_STAGE_TCL) {
@@ -2588,6 +2553,108 @@ int repackMlp(sdl_mlp* m) {
     return 0;
 }
 
+// ---- geometry: device upload and the binary cache
+int uploadGeom(sdl_geom* G) {
+    (void)hipGetDevice(&G->device);
+    auto up = [&](void** d, const void* h, size_t bytes) -> int {
+        if (hipMalloc(d, std::max<size_t>(bytes, 4)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
+        if (bytes && hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(SDL_ERR_HIP, "hipMemcpy failed");
+        return 0;
+    };
+    int r = up((void**)&G->dDet, G->detIds.data(), sizeof(uint32_t) * G->detIds.size());
+    if (!r) r = up((void**)&G->dMods, G->mods.data(), sizeof(ModDev) * G->mods.size());
+    if (!r) r = up((void**)&G->dConnStart, G->connStart.data(), sizeof(int) * G->connStart.size());
+    if (!r) r = up((void**)&G->dConnList, G->connList.data(), sizeof(int) * G->connList.size());
+    return r;
+}
+
+// Binary cache: the parsed geometry exactly as the device holds it.
+//   "SDLGEOM1" | u32 version | u32 sizeof(ModDev) | i32 NM | i64 number of connections
+//   | u32 detIds[NM] (sorted) | ModDev mods[NM] | i32 connStart[NM + 1] | i32 connList[]
+//   | u64 FNV-1a hash of everything before it
+constexpr char kGeomMagic[8] = {'S', 'D', 'L', 'G', 'E', 'O', 'M', '1'};
+constexpr uint32_t kGeomVersion = 1;
+
+uint64_t fnv1a(const std::string& b) {
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char ch : b) {
+        h ^= ch;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+template <class T>
+void putRaw(std::string& b, const T* p, size_t n) {
+    b.append(reinterpret_cast<const char*>(p), sizeof(T) * n);
+}
+
+int writeGeomBinary(const sdl_geom& G, const char* path) {
+    std::string b(kGeomMagic, 8);
+    const uint32_t ver = kGeomVersion, msz = sizeof(ModDev);
+    const int32_t nm = G.NM;
+    const int64_t nc = (int64_t)G.connList.size();
+    putRaw(b, &ver, 1);
+    putRaw(b, &msz, 1);
+    putRaw(b, &nm, 1);
+    putRaw(b, &nc, 1);
+    putRaw(b, G.detIds.data(), G.detIds.size());
+    putRaw(b, G.mods.data(), G.mods.size());
+    putRaw(b, G.connStart.data(), G.connStart.size());
+    putRaw(b, G.connList.data(), G.connList.size());
+    const uint64_t h = fnv1a(b);
+    putRaw(b, &h, 1);
+    const std::string tmp = std::string(path) + ".tmp";
+    std::ofstream f(tmp, std::ios::binary);
+    if (!f || !f.write(b.data(), (std::streamsize)b.size())) return fail(SDL_ERR_IO, std::string("cannot write ") + tmp);
+    f.close();
+    if (std::rename(tmp.c_str(), path) != 0) return fail(SDL_ERR_IO, std::string("cannot rename to ") + path);
+    return 0;
+}
+
+int readGeomBinary(const char* path, sdl_geom& G) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return fail(SDL_ERR_IO, std::string("cannot open ") + path);
+    const std::string b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    const size_t head = 8 + 4 + 4 + 4 + 8;
+    auto bad = [&](const char* why) { return fail(SDL_ERR_IO, std::string(path) + ": not a valid geometry cache (" + why + ")"); };
+    if (b.size() < head + 8 || std::memcmp(b.data(), kGeomMagic, 8) != 0) return bad("magic");
+    uint32_t ver, msz;
+    int32_t nm;
+    int64_t nc;
+    std::memcpy(&ver, b.data() + 8, 4);
+    std::memcpy(&msz, b.data() + 12, 4);
+    std::memcpy(&nm, b.data() + 16, 4);
+    std::memcpy(&nc, b.data() + 20, 8);
+    if (ver != kGeomVersion) return bad("version");
+    if (msz != sizeof(ModDev)) return bad("module record size");
+    if (nm < 0 || nc < 0 || nc > (int64_t(1) << 31)) return bad("sizes");
+    const size_t need = head + 4 * size_t(nm) + sizeof(ModDev) * size_t(nm) + 4 * (size_t(nm) + 1) + 4 * size_t(nc) + 8;
+    if (b.size() != need) return bad("length");
+    uint64_t h;
+    std::memcpy(&h, b.data() + need - 8, 8);
+    if (fnv1a(b.substr(0, need - 8)) != h) return bad("checksum");
+    size_t o = head;
+    G.NM = nm;
+    G.detIds.resize(nm);
+    G.mods.resize(nm);
+    G.connStart.resize(size_t(nm) + 1);
+    G.connList.resize(size_t(nc));
+    std::memcpy(G.detIds.data(), b.data() + o, 4 * size_t(nm)); o += 4 * size_t(nm);
+    std::memcpy(G.mods.data(), b.data() + o, sizeof(ModDev) * size_t(nm)); o += sizeof(ModDev) * size_t(nm);
+    std::memcpy(G.connStart.data(), b.data() + o, 4 * (size_t(nm) + 1)); o += 4 * (size_t(nm) + 1);
+    std::memcpy(G.connList.data(), b.data() + o, 4 * size_t(nc));
+    // structure: sorted detIds, a monotone CSR covering the list, module indices in range
+    for (int k = 1; k < nm; ++k)
+        if (G.detIds[k - 1] >= G.detIds[k]) return bad("detIds not sorted");
+    if (G.connStart[0] != 0 || G.connStart[nm] != nc) return bad("connection offsets");
+    for (int k = 0; k < nm; ++k)
+        if (G.connStart[k] > G.connStart[k + 1]) return bad("connection offsets");
+    for (int64_t k = 0; k < nc; ++k)
+        if (G.connList[k] < 0 || G.connList[k] >= nm) return bad("connection target");
+    return 0;
+}
+
 }  // namespace
 
 // ====================================================================== C-ABI
@@ -2599,17 +2666,37 @@ int sdl_geom_load(const char* tiltedPath, const char* endcapPath, const char* ma
     if (!tiltedPath || !endcapPath || !mapPath || !out) return fail(SDL_ERR_ARG, "null argument");
     auto* G = new sdl_geom();
     if (int r = loadGeomFiles(tiltedPath, endcapPath, mapPath, *G)) { delete G; return r; }
-    (void)hipGetDevice(&G->device);
-    auto up = [&](void** d, const void* h, size_t bytes) -> int {
-        if (hipMalloc(d, std::max<size_t>(bytes, 4)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
-        if (bytes && hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(SDL_ERR_HIP, "hipMemcpy failed");
-        return 0;
-    };
-    int r = up((void**)&G->dDet, G->detIds.data(), sizeof(uint32_t) * G->detIds.size());
-    if (!r) r = up((void**)&G->dMods, G->mods.data(), sizeof(ModDev) * G->mods.size());
-    if (!r) r = up((void**)&G->dConnStart, G->connStart.data(), sizeof(int) * G->connStart.size());
-    if (!r) r = up((void**)&G->dConnList, G->connList.data(), sizeof(int) * G->connList.size());
-    if (r) { sdl_geom_free(G); return r; }
+    if (int r = uploadGeom(G)) { sdl_geom_free(G); return r; }
+    *out = G;
+    return 0;
+}
+
+int sdl_geom_convert(const char* tiltedPath, const char* endcapPath, const char* mapPath, const char* binaryPath) {
+    if (!tiltedPath || !endcapPath || !mapPath || !binaryPath) return fail(SDL_ERR_ARG, "null argument");
+    sdl_geom G;
+    if (int r = loadGeomFiles(tiltedPath, endcapPath, mapPath, G)) return r;
+    return writeGeomBinary(G, binaryPath);
+}
+
+int sdl_geom_save(const sdl_geom* g, const char* binaryPath) {
+    if (!g || !binaryPath) return fail(SDL_ERR_ARG, "null argument");
+    return writeGeomBinary(*g, binaryPath);
+}
+
+int sdl_geom_check_binary(const char* binaryPath, int* numModules, long long* numConnections) {
+    if (!binaryPath) return fail(SDL_ERR_ARG, "null argument");
+    sdl_geom G;
+    if (int r = readGeomBinary(binaryPath, G)) return r;
+    if (numModules) *numModules = G.NM;
+    if (numConnections) *numConnections = (long long)G.connList.size();
+    return 0;
+}
+
+int sdl_geom_load_binary(const char* binaryPath, sdl_geom** out) {
+    if (!binaryPath || !out) return fail(SDL_ERR_ARG, "null argument");
+    auto* G = new sdl_geom();
+    if (int r = readGeomBinary(binaryPath, *G)) { delete G; return r; }
+    if (int r = uploadGeom(G)) { sdl_geom_free(G); return r; }
     *out = G;
     return 0;
 }
@@ -3010,7 +3097,16 @@ int sdl_event_tl_work(sdl_event* ev, long long out[4]) {
     out[0] = t[TOT_TLPAIRS];
     out[1] = t[TOT_TLCANDS];
     out[2] = t[TOT_TL];
-    out[3] = ev->proposal ? t[TOT_TLSCORED] : 0;
+    out[3] = 0;
+    if (ev->proposal && ev->tlScoredStride > 0) {
+        std::vector<int> w(size_t(kNVariantsTL * ev->tlScoredStride));
+        HIP_OK(hipMemcpyAsync(w.data(), ev->tlScored.p, sizeof(int) * w.size(), hipMemcpyDeviceToHost, ev->st));
+        HIP_OK(hipStreamSynchronize(ev->st));
+        for (int v = 0; v < kNVariantsTL; ++v) {
+            const long long nw = (t[TOT_TLP0 + v] + 63) / 64;
+            for (long long k = 0; k < nw && k < ev->tlScoredStride; ++k) out[3] += w[size_t(v * ev->tlScoredStride + k)];
+        }
+    }
     return 0;
 }
 

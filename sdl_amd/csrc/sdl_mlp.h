@@ -133,11 +133,12 @@ __device__ __forceinline__ void featureRow(const float* f, short8v (&v)[4]) {
 // call.  A row only affects its own logit (every MFMA column is one row), so
 // rows past the end may hold anything.
 __device__ __forceinline__ float logit16(bf16x8 x, int lane, const Frags& F, const Vecs& V) {
-    const int g = lane >> 4;
-    // an opaque copy of the lane index: the LDS fragments are read at each use
-    // instead of being hoisted into registers by the compiler
+    // an opaque copy of the lane index: the LDS operands (fragments, biases,
+    // output weights) are read at each use instead of being hoisted into
+    // registers by the compiler
     int l = lane;
     asm volatile("" : "+v"(l));
+    const int g = l >> 4;
     short4v h1[kTiles], h2[kTiles];
 #pragma unroll
     for (int t = 0; t < kTiles; ++t) h1[t] = reluBf16(mfma(V.a1[t][l], x, float4v{0.f, 0.f, 0.f, 0.f}));

@@ -27,8 +27,9 @@ EXPORTS = (
     "sdl_mlp_create", "sdl_mlp_free", "sdl_mlp_num_params", "sdl_mlp_set_params", "sdl_mlp_get_params",
     "sdl_mlp_gradient", "sdl_mlp_train", "sdl_mlp_forward", "sdl_event_set_proposal", "sdl_event_tl_pair_rows", "sdl_event_tl_work",
     "sdl_event_autotune", "sdl_event_get_tuning", "sdl_event_set_tuning",
+    "sdl_geom_convert", "sdl_geom_save", "sdl_geom_load_binary", "sdl_geom_check_binary",
 )
-TUNING = ("md_block", "sg_block", "tl1_block", "tl2_block", "tl_expand_block", "score_blocks")
+TUNING = ("md_block", "sg_block", "tl1_block", "tl2_block", "tl_expand_block", "scored_waves")
 TIMERS = ("prep", "md", "sg", "tl")
 
 _lib = None
@@ -82,6 +83,10 @@ def lib():
         L.sdl_event_tl_pair_rows.argtypes = [vp, vp, vp, ll, ctypes.POINTER(ll)]
         L.sdl_event_tl_work.argtypes = [vp, vp]
         L.sdl_event_autotune.argtypes = [vp, i, ctypes.c_char_p, i]
+        L.sdl_geom_convert.argtypes = [ctypes.c_char_p] * 4
+        L.sdl_geom_save.argtypes = [vp, ctypes.c_char_p]
+        L.sdl_geom_load_binary.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.sdl_geom_check_binary.argtypes = [ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(ll)]
         L.sdl_event_get_tuning.argtypes = [vp, vp, i]
         L.sdl_event_set_tuning.argtypes = [vp, vp, i]
         _lib = L
@@ -94,9 +99,19 @@ def _check(rc):
 
 
 class Geometry:
-    def __init__(self, tilted, endcap, cmap):
+    """The three geometry text files (reference formats), or with binary=PATH a
+    binary geometry cache, loaded onto the current HIP device."""
+
+    def __init__(self, tilted=None, endcap=None, cmap=None, binary=None):
         self.h = ctypes.c_void_p()
-        _check(lib().sdl_geom_load(tilted.encode(), endcap.encode(), cmap.encode(), ctypes.byref(self.h)))
+        if binary is not None:
+            _check(lib().sdl_geom_load_binary(binary.encode(), ctypes.byref(self.h)))
+        else:
+            _check(lib().sdl_geom_load(tilted.encode(), endcap.encode(), cmap.encode(), ctypes.byref(self.h)))
+
+    def save(self, path):
+        """Writes the binary geometry cache of this geometry."""
+        _check(lib().sdl_geom_save(self.h, path.encode()))
 
     @property
     def num_modules(self):
@@ -106,6 +121,18 @@ class Geometry:
         if self.h:
             lib().sdl_geom_free(self.h)
             self.h = ctypes.c_void_p()
+
+
+def convert_geometry(tilted, endcap, cmap, binary):
+    """Parses the three text files and writes the binary geometry cache (host only)."""
+    _check(lib().sdl_geom_convert(tilted.encode(), endcap.encode(), cmap.encode(), binary.encode()))
+
+
+def check_geometry_cache(binary):
+    """Validates a binary geometry cache on the host: (modules, connections)."""
+    nm, nc = ctypes.c_int(), ctypes.c_longlong()
+    _check(lib().sdl_geom_check_binary(binary.encode(), ctypes.byref(nm), ctypes.byref(nc)))
+    return nm.value, nc.value
 
 
 class Event:

@@ -128,8 +128,8 @@ def test_proposal_result_independent_of_launch_shape():
     P.set_proposal(ev, m, 0.4)
     base = ev.tuning()
     got = []
-    for blocks, b1 in ((256, 64), (2048, 256), (512, 128)):
-        ev.set_tuning(dict(base, score_blocks=blocks, tl1_block=b1))
+    for waves, b1 in ((4, 64), (2, 256), (8, 128)):
+        ev.set_tuning(dict(base, scored_waves=waves, tl1_block=b1))
         got.append(_chain(ev, e))
     for s in ("TL", "TCP", "TCL"):
         assert np.array_equal(oracle.as_multiset(got[0][s]), oracle.as_multiset(got[1][s])), s

@@ -233,7 +233,7 @@ def test_autotune_keeps_objects():
     assert rep["best_ms"] <= rep["start_ms"]
     allowed = {"md_block": {64, 128, 256}, "sg_block": {64, 128, 256}, "tl1_block": {64, 128, 256},
                "tl2_block": {64, 128, 256}, "tl_expand_block": {64, 128, 256},
-               "score_blocks": {256, 512, 1024, 2048}}
+               "scored_waves": {2, 4, 8}}
     for k, v in chosen.items():
         assert v in allowed[k] and rep[k] == v, k
     for setting in (chosen, {k: min(v) for k, v in allowed.items()}, {k: max(v) for k, v in allowed.items()}):
@@ -246,3 +246,28 @@ def test_autotune_keeps_objects():
         ev.set_tuning(dict(chosen, md_block=100))
     for d in dev:
         d.free()
+
+
+def test_binary_geometry_cache(tmp_path):
+    """A geometry loaded from the binary cache gives the same objects as the
+    text files; a loaded geometry saves the same cache bytes as the converter."""
+    print("case binary geometry cache, workload 15 event 0 and workload 2 event 1", flush=True)
+    paths = gen.geometry(2)[1]
+    conv = str(tmp_path / "conv.bin")
+    runtime.convert_geometry(*paths, conv)
+    gb = runtime.Geometry(binary=conv)
+    saved = str(tmp_path / "saved.bin")
+    gb.save(saved)
+    assert open(conv, "rb").read() == open(saved, "rb").read()
+    gt = _geom(2)
+    assert gb.num_modules == gt.num_modules
+    for wl, e in ((15, 0), (2, 1)):
+        ev_in = gen.event(wl, e)
+        out = []
+        for g in (gt, gb):
+            ev = runtime.Event(g)
+            ev.add_hits(ev_in.x, ev_in.y, ev_in.z, ev_in.idx, ev_in.detId)
+            ev.run_chain()
+            ev.sync()
+            out.append(ev.export_all(0))
+        _assert_same(out[1], out[0], f"binary geometry, workload {wl} event {e}")
