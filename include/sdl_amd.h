@@ -43,7 +43,11 @@ enum {
     SDL_STAGE_TP = 2,   /* createTriplets                       Event.cc:524-597 */
     SDL_STAGE_TL = 3,   /* createTrackletsWithModuleMap         Event.cc:611-693 */
     SDL_STAGE_TCP = 4,  /* createTrackCandidatesFromTriplets    Event.cc:874-982 */
-    SDL_STAGE_TCL = 5   /* createTrackCandidatesFromTracklets   Event.cc:984-1062 */
+    SDL_STAGE_TCL = 5,  /* createTrackCandidatesFromTracklets   Event.cc:984-1062 */
+    /* layer-pair family (objects stored and exported as SG, TL and TCL) */
+    SDL_STAGE_SG_LAYERS = 6,  /* createSegments         Event.cc:432-443, 793-821 */
+    SDL_STAGE_TL_LAYERS = 7,  /* createTracklets        Event.cc:599-609, 755-790 */
+    SDL_STAGE_TC_LAYERS = 8   /* createTrackCandidates  Event.cc:823-872 */
 };
 
 /* Last error message of the calling thread. */
@@ -96,9 +100,14 @@ int sdl_event_next(sdl_event* ev);
 int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offsets, const float* dx, const float* dy,
                               const float* dz, const int* didx, const unsigned int* ddetId);
 
-/* Runs one stage on every event of the batch (stages must run in order; a
- * stage whose inputs are missing returns SDL_ERR_STATE).  algo: 0 = Default,
- * the only algorithm with device kernels. */
+/* Runs one stage on every event of the batch.  Each stage reads the objects of
+ * the stage before it, so stages run in order (else SDL_ERR_STATE): the
+ * module-map chain MD, SG, TP, TL, TCP, TCL; SG_LAYERS after MD; TL_LAYERS after
+ * SG or SG_LAYERS; TC_LAYERS after TL or TL_LAYERS (the module-map joins TP,
+ * TCP and TCL assume module-map inputs and do not follow a layer-pair stage).
+ * The layer-pair stages store their objects as the SG, TL and TCL objects of
+ * the batch.  algo: 0 = Default, the only algorithm with device
+ * kernels. */
 int sdl_event_run(sdl_event* ev, int stage, int algo);
 /* Runs MD, SG, TP, TL, TCP and TCL back to back.  After a first call has
  * measured the batch's object counts, later calls size their buffers from

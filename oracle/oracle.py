@@ -14,6 +14,7 @@ REF_DRIVER = os.path.join(_HERE, "_ref", "sdl_ref_driver")
 GLIBC, CR = 0, 1
 NUDGED = 2  # test only: CR made 0.4 % too large (a planted libm error)
 STAGES = ("MD", "SG", "TP", "TL", "TCP", "TCL")
+LAYER_CHAIN = 16  # stages value of run(): the layer-pair chain (its TCs are TCL objects)
 WIDTH = {"MD": 2, "SG": 4, "TP": 8, "TL": 8, "TCP": 16, "TCL": 16}
 
 _lib = None
@@ -30,6 +31,8 @@ def lib():
         L.oracle_geom_free.argtypes = [ctypes.c_void_p]
         L.oracle_run.restype = ctypes.c_void_p
         L.oracle_run.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int, ctypes.c_int]
+        L.oracle_run_sequence.restype = ctypes.c_void_p
+        L.oracle_run_sequence.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
         L.oracle_result_count.restype = ctypes.c_long
         L.oracle_result_count.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_result_copy.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
@@ -61,10 +64,17 @@ def _ptrs(ev):
 
 
 def run(paths, ev, libm=GLIBC, stages=6):
-    """Returns ({stage: int32 array [n, width]}, counters dict)."""
+    """Returns ({stage: int32 array [n, width]}, counters dict).  stages = 1..6
+    runs that many module-map stages; LAYER_CHAIN runs the layer-pair chain; a
+    list runs those stages in that order (0-5 module-map, 6-8 createSegments,
+    createTracklets, createTrackCandidates)."""
     L = lib()
     g = geometry(paths)
-    r = L.oracle_run(g, ev.nhits, *_ptrs(ev), libm, stages)
+    if isinstance(stages, (list, tuple)):
+        seq = np.asarray(stages, np.int32)
+        r = L.oracle_run_sequence(g, ev.nhits, *_ptrs(ev), libm, seq.ctypes.data, len(seq))
+    else:
+        r = L.oracle_run(g, ev.nhits, *_ptrs(ev), libm, stages)
     try:
         out = {}
         for i, s in enumerate(STAGES):
@@ -126,6 +136,8 @@ def parse_ref_dump(path):
                 events[cur][1].update(dict(zip(("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand"), map(int, parts[1:]))))
             elif tag == "C":
                 events[cur][1].update(dict(zip(("nMD", "nSG", "nTP", "nTL", "nTC"), map(int, parts[1:]))))
+            elif tag == "L":  # per-layer counters: 10 kinds x (barrel 1-6, endcap 1-5)
+                events[cur][1]["layers"] = np.asarray(list(map(int, parts[1:])), np.int64).reshape(10, 11)
     out = {}
     for e, (rows, cnt) in events.items():
         out[e] = ({s: np.asarray(rows[s], np.int32).reshape(-1, WIDTH[s]) for s in STAGES}, cnt)

@@ -10,10 +10,15 @@
 //     createTrackletsWithModuleMap, createTrackCandidatesFromTriplets,
 //     createTrackCandidatesFromTracklets                       (Event.h:240-300)
 // and writes every object as a tuple of hit indices, plus the per-stage counters.
+// Mode dumplayers runs the layer-pair chain instead
+//     createMiniDoublets, createSegments, createTracklets, createTrackCandidates
+//                                      (Event.cc:245-255, 432-443, 599-609, 823-837)
+// and adds the per-layer counters of every stage.
 //
 // Usage:
-//   sdl_ref_driver dump  <workload> <geomdir> <event0> <nevents> <outfile>
-//   sdl_ref_driver time  <workload> <geomdir> <event0> <nevents> <threads> [stages]
+//   sdl_ref_driver dump        <workload> <geomdir> <event0> <nevents> <outfile>
+//   sdl_ref_driver dumplayers  <workload> <geomdir> <event0> <nevents> <outfile>
+//   sdl_ref_driver time        <workload> <geomdir> <event0> <nevents> <threads> [stages]
 // `stages` limits the chain to its first k stages (1..6, default 6).
 #include <chrono>
 #include <cstdio>
@@ -85,6 +90,70 @@ void putSG(FILE* f, const SDL::Segment* sg) {
     putMD(f, sg->outerMiniDoubletPtr());
 }
 
+void runLayerChain(SDL::Event& ev) {
+    ev.createMiniDoublets();
+    ev.createSegments();
+    ev.createTracklets();
+    ev.createTrackCandidates();
+}
+
+// Layer-pair chain: MDs and SGs through the modules (addSegmentToEvent adds to
+// both, Event.cc:163-177), TLs and TCs through the layers (addTrackletToLowerLayer,
+// addTrackCandidateToLowerLayer, Event.cc:227-243), then the per-layer counters
+// (barrel 1-6, endcap 1-5) of the ten counter kinds.
+void dumpLayerEvent(FILE* f, SDL::Event& sev, long ievent) {
+    std::fprintf(f, "E %ld\n", ievent);
+    for (SDL::Module* m : sev.getModulePtrs()) {
+        for (SDL::MiniDoublet* md : m->getMiniDoubletPtrs()) { std::fprintf(f, "MD"); putMD(f, md); std::fprintf(f, "\n"); }
+        for (SDL::Segment* sg : m->getSegmentPtrs()) { std::fprintf(f, "SG"); putSG(f, sg); std::fprintf(f, "\n"); }
+    }
+    for (SDL::Layer* L : sev.getLayerPtrs()) {
+        for (SDL::Tracklet* tl : L->getTrackletPtrs()) {
+            std::fprintf(f, "TL"); putSG(f, tl->innerSegmentPtr()); putSG(f, tl->outerSegmentPtr()); std::fprintf(f, "\n");
+        }
+        for (SDL::TrackCandidate* tc : L->getTrackCandidatePtrs()) {
+            SDL::TrackletBase* a = tc->innerTrackletBasePtr();
+            SDL::TrackletBase* b = tc->outerTrackletBasePtr();
+            std::fprintf(f, "TCL");
+            putSG(f, a->innerSegmentPtr()); putSG(f, a->outerSegmentPtr());
+            putSG(f, b->innerSegmentPtr()); putSG(f, b->outerSegmentPtr());
+            std::fprintf(f, "\n");
+        }
+    }
+    std::fprintf(f, "N %u %u %u %u %u\n", sev.getNumberOfMiniDoubletCandidates(), sev.getNumberOfSegmentCandidates(),
+                 sev.getNumberOfTripletCandidates(), sev.getNumberOfTrackletCandidates(),
+                 sev.getNumberOfTrackCandidateCandidates());
+    std::fprintf(f, "C %u %u %u %u %u\n", sev.getNumberOfMiniDoublets(), sev.getNumberOfSegments(),
+                 sev.getNumberOfTriplets(), sev.getNumberOfTracklets(), sev.getNumberOfTrackCandidates());
+    typedef unsigned (SDL::Event::*Getter)(unsigned);
+    const Getter barrel[10] = {&SDL::Event::getNumberOfMiniDoubletCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfSegmentCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfTripletCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfTrackletCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfTrackCandidateCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfMiniDoubletsByLayerBarrel,
+                               &SDL::Event::getNumberOfSegmentsByLayerBarrel,
+                               &SDL::Event::getNumberOfTripletsByLayerBarrel,
+                               &SDL::Event::getNumberOfTrackletsByLayerBarrel,
+                               &SDL::Event::getNumberOfTrackCandidatesByLayerBarrel};
+    const Getter endcap[10] = {&SDL::Event::getNumberOfMiniDoubletCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfSegmentCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfTripletCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfTrackletCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfTrackCandidateCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfMiniDoubletsByLayerEndcap,
+                               &SDL::Event::getNumberOfSegmentsByLayerEndcap,
+                               &SDL::Event::getNumberOfTripletsByLayerEndcap,
+                               &SDL::Event::getNumberOfTrackletsByLayerEndcap,
+                               &SDL::Event::getNumberOfTrackCandidatesByLayerEndcap};
+    std::fprintf(f, "L");
+    for (int k = 0; k < 10; ++k) {
+        for (unsigned i = 0; i < 6; ++i) std::fprintf(f, " %u", (sev.*barrel[k])(i));
+        for (unsigned i = 0; i < 5; ++i) std::fprintf(f, " %u", (sev.*endcap[k])(i));
+    }
+    std::fprintf(f, "\n");
+}
+
 void dumpEvent(FILE* f, SDL::Event& sev, long ievent) {
     std::fprintf(f, "E %ld\n", ievent);
     // Objects are reached through the per-module pointer lists, which hold every
@@ -135,6 +204,20 @@ int main(int argc, char** argv) {
     loadGeometry(workload, geomdir, g);
     sdlgen::EventConfig cfg = sdlgen::presetConfig(workload);
 
+    if (mode == "dumplayers") {
+        FILE* f = std::fopen(argv[6], "w");
+        if (!f) return 3;
+        for (long ie = e0; ie < e0 + ne; ++ie) {
+            sdlgen::Event ev;
+            sdlgen::generateEvent(g, cfg, ie, ev);
+            SDL::Event sev;
+            fill(sev, ev);
+            runLayerChain(sev);
+            dumpLayerEvent(f, sev, ie);
+        }
+        std::fclose(f);
+        return 0;
+    }
     if (mode == "dump") {
         FILE* f = std::fopen(argv[6], "w");
         if (!f) return 3;

@@ -1008,6 +1008,7 @@ struct Event {
                         if (tlAlgo(sgs[si], sgs[so], nullptr)) {
                             slots[ls].tls.push_back(int(tls.size()));
                             tls.push_back({si, so});
+                            tlLayer.push_back(layerOfMD(sgs[si].mdIn));  // addTrackletToEvent (Event.cc:179-193)
                         }
                     }
                 }
@@ -1048,6 +1049,109 @@ struct Event {
                         if (sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo}});
                     }
                 }
+            }
+        }
+    }
+
+    // ---------------------------------------------------------- layer pairs
+    // Layer lists (Layer.h): objects are appended to the layer of their inner
+    // lower module as they are created (Event.cc:141-177); index 0-5 = barrel
+    // layers 1-6, 6-10 = endcap layers 1-5.
+    static int layerKey(int subdet, int layer) { return subdet == SUB_BARREL ? layer - 1 : 6 + layer - 1; }
+    int layerOfMD(int md) const { const ModInfo& m = slots[mds[md].slot].info; return layerKey(m.subdet, m.layer); }
+    std::vector<int> layerList(int key, int kind) const {  // kind 0: MDs, 1: SGs (creation order)
+        std::vector<int> out;
+        if (kind == 0) {
+            for (size_t i = 0; i < mds.size(); ++i)
+                if (layerOfMD(int(i)) == key) out.push_back(int(i));
+        } else {
+            for (size_t i = 0; i < sgs.size(); ++i)
+                if (layerOfMD(sgs[i].mdIn) == key) out.push_back(int(i));
+        }
+        return out;
+    }
+    // SDL::Layer::segment_compatible_layer_pairs_ (Layer.cc:35-52), as (inner, outer) keys
+    static const std::vector<std::pair<int, int>>& segmentLayerPairs() {
+        static const std::vector<std::pair<int, int>> v = {
+            {0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5},        // barrel 1-2 ... 5-6
+            {0, 6}, {1, 6}, {2, 6}, {3, 6}, {4, 6},        // barrel 1..5 - endcap 1
+            {6, 7}, {7, 8}, {7, 9}, {7, 10}, {8, 9}, {9, 10}};  // endcap 1-2, 2-3, 2-4, 2-5, 3-4, 4-5
+        return v;
+    }
+    // SDL::Layer::tracklet_compatible_layer_pairs_ (Layer.cc:3-33): barrel 1-3, 2-4, 3-5
+    static const std::vector<std::pair<int, int>>& trackletLayerPairs() {
+        static const std::vector<std::pair<int, int>> v = {{0, 2}, {1, 3}, {2, 4}};
+        return v;
+    }
+    // Event::createSegments + createSegmentsFromTwoLayers (Event.cc:432-443, 793-821):
+    // every MD of the inner layer against every MD of the outer layer; no counters.
+    void createSegmentsLayers() {
+        for (auto& pr : segmentLayerPairs()) {
+            const std::vector<int> in = layerList(pr.first, 0), out = layerList(pr.second, 0);
+            for (int mi : in)
+                for (int mo : out) {
+                    SGRec sg;
+                    if (sgAlgo(mi, mo, sg, nullptr)) {
+                        slots[mds[mi].slot].sgs.push_back(int(sgs.size()));
+                        sgs.push_back(sg);
+                    }
+                }
+        }
+    }
+    // Event::createTracklets + createTrackletsFromTwoLayers (Event.cc:599-609, 755-790):
+    // every SG of the inner layer against every SG of the outer layer; counted
+    // on the inner layer; the TLs go to the inner layer's list only.
+    std::vector<int> tlLayer;  // layer list of each TL (-1: none)
+    void createTrackletsLayers() {
+        for (auto& pr : trackletLayerPairs()) {
+            const std::vector<int> in = layerList(pr.first, 1), out = layerList(pr.second, 1);
+            for (int si : in)
+                for (int so : out) {
+                    nCand[3]++;
+                    if (tlAlgo(sgs[si], sgs[so], nullptr)) {
+                        tls.push_back({si, so});
+                        tlLayer.push_back(pr.first);
+                    }
+                }
+        }
+    }
+    // Event::createTrackCandidates + createTrackCandidatesFromTwoLayers (Event.cc:823-872):
+    // the TLs of barrel layer 1 against those of barrel layer 3 with the default
+    // algorithm, which passes iff the inner TL's outer SG is the outer TL's inner SG
+    // (TrackCandidate.cc:143-226, A-13).
+    void createTrackCandidatesLayers() {
+        std::vector<int> in, out;
+        for (size_t i = 0; i < tls.size(); ++i) {
+            if (tlLayer[i] == 0) in.push_back(int(i));
+            if (tlLayer[i] == 2) out.push_back(int(i));
+        }
+        for (int li : in)
+            for (int lo : out) {
+                nCand[4]++;
+                if (sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo}});
+            }
+    }
+    void runLayers() {
+        createMiniDoublets();
+        createSegmentsLayers();
+        createTrackletsLayers();
+        createTrackCandidatesLayers();
+    }
+    // Stages in the given order: 0-5 the module-map stages, 6-8 createSegments,
+    // createTracklets, createTrackCandidates.
+    void runSequence(const int* seq, int n) {
+        for (int k = 0; k < n; ++k) {
+            switch (seq[k]) {
+                case 0: createMiniDoublets(); break;
+                case 1: createSegments(); break;
+                case 2: createTriplets(); break;
+                case 3: createTracklets(); break;
+                case 4: createTCFromTriplets(); break;
+                case 5: createTCFromTracklets(); break;
+                case 6: createSegmentsLayers(); break;
+                case 7: createTrackletsLayers(); break;
+                case 8: createTrackCandidatesLayers(); break;
+                default: break;
             }
         }
     }
@@ -1118,7 +1222,24 @@ void* oracle_run(void* geom, int nhits, const float* x, const float* y, const fl
                  const int* idx, int libm, int stages) {
     Event ev;
     fillEvent(ev, static_cast<Geometry*>(geom), libm, nhits, x, y, z, detId, idx);
-    ev.run(stages);
+    if (stages == ORACLE_LAYER_CHAIN) ev.runLayers();
+    else ev.run(stages);
+    auto* r = new Result();
+    for (int s = 0; s < 6; ++s) r->t[s] = ev.tuples(s);
+    for (int s = 0; s < 5; ++s) r->counters[s] = ev.nCand[s];
+    r->counters[5] = ev.mds.size();
+    r->counters[6] = ev.sgs.size();
+    r->counters[7] = ev.tps.size();
+    r->counters[8] = ev.tls.size();
+    r->counters[9] = ev.tcs.size();
+    return r;
+}
+
+void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y, const float* z,
+                          const unsigned* detId, const int* idx, int libm, const int* seq, int n) {
+    Event ev;
+    fillEvent(ev, static_cast<Geometry*>(geom), libm, nhits, x, y, z, detId, idx);
+    ev.runSequence(seq, n);
     auto* r = new Result();
     for (int s = 0; s < 6; ++s) r->t[s] = ev.tuples(s);
     for (int s = 0; s < 5; ++s) r->counters[s] = ev.nCand[s];

@@ -38,9 +38,19 @@ enum { ORACLE_MD = 0, ORACLE_SG = 1, ORACLE_TP = 2, ORACLE_TL = 3, ORACLE_TCP = 
 void* oracle_geom_load(const char* tiltedPath, const char* endcapPath, const char* mapPath);
 void oracle_geom_free(void* geom);
 
-// Runs the first `stages` (1..6) module-map stages on one event.
+// stages value that runs the layer-pair chain instead: createMiniDoublets,
+// createSegments, createTracklets, createTrackCandidates (Event.cc:245-255,
+// 432-443, 599-609, 823-837); its TCs are reported as TCL objects.
+enum { ORACLE_LAYER_CHAIN = 16 };
+
+// Runs the first `stages` (1..6) module-map stages on one event (or, with
+// ORACLE_LAYER_CHAIN, the layer-pair chain).
 void* oracle_run(void* geom, int nhits, const float* x, const float* y, const float* z, const unsigned* detId,
                  const int* idx, int libm, int stages);
+// Runs the stages seq[0..n-1] in that order (0-5 module-map stages, 6-8
+// createSegments, createTracklets, createTrackCandidates); results as oracle_run.
+void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y, const float* z,
+                          const unsigned* detId, const int* idx, int libm, const int* seq, int n);
 // Tuple width: MD 2, SG 4, TP 8, TL 8, TCP 16, TCL 16 hit indices.
 long oracle_result_count(void* res, int stage);
 void oracle_result_copy(void* res, int stage, int* out);

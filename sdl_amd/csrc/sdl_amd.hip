@@ -700,6 +700,15 @@ struct SGF {
     float v[6];  // dphi, dphimin, dphimax, dphichange, dphichangemin, dphichangemax
 };
 
+// Layer bin of a module: barrel layers 0..5, endcap layers 6..10 (the layers
+// of Event.h's per-layer counters and of SDL::Layer).
+__host__ __device__ __forceinline__ int layerBin(const ModDev& m) {
+    return m.subdet == SUB_BARREL ? m.layer - 1 : 6 + m.layer - 1;
+}
+
+constexpr int kNCounters = 10;
+constexpr int kNBins = 11;
+
 // Flattened walk over the connected modules of one work item: returns the
 // connection entry holding candidate k and the index inside it.  pre holds the
 // wave-wide inclusive prefix of per-entry counts (entry j in lane j).  Must be
@@ -713,6 +722,36 @@ __device__ __forceinline__ int findEntry(int k, int pre, int nEntries, int& inne
     }
     inner = k - prev;
     return e;
+}
+
+// One SG candidate (inner MD `in` on module mi, outer MD mdOut on module mO):
+// the segment algorithm with the endcap-2S strip edges of the outer anchor.
+__device__ __forceinline__ bool sgCandidate(const MDView& in, const ModDev& mi, int mdOut, const ModDev& mO,
+                                            bool connFound, const HitsDev& H, const MDDev& D, SGOut& o) {
+    const MDView out = loadMD(mdOut, D);
+    HitView hiE = out.a, loE = out.a;
+    if (mO.subdet == SUB_ENDCAP && mO.type == TYPE_2S) {
+        const int an = D.anchor[mdOut];
+        const float4 a = H.eHi[an], b = H.eLo[an];
+        hiE.phi = a.w;
+        loE.phi = b.w;
+    }
+    return sgAlgo(in, out, mi, mO, connFound, hiE, loE, o);
+}
+
+// Appends the wave's passing SG candidates, in lane order, to the inner MD's slab.
+__device__ __forceinline__ void appendSG(bool pass, int lane, int& base, long long off, int mdOut, const SGOut& o,
+                                         int* slabOut, SGF* slabF) {
+    const unsigned long long mask = __ballot(pass);
+    if (pass) {
+        const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+        slabOut[off + pos] = mdOut;
+        SGF f;
+        f.v[0] = o.dphi; f.v[1] = o.dphimin; f.v[2] = o.dphimax;
+        f.v[3] = o.dphichange; f.v[4] = o.dphichangemin; f.v[5] = o.dphichangemax;
+        slabF[off + pos] = f;
+    }
+    base += __popcll(mask);
 }
 
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_sg(Count cMD, int NM, const ModDev* mods, HitsDev H, MDDev D,
@@ -758,27 +797,119 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
             if (k < total) {
                 const int modOut = connList[c0 + cb + e];
                 mdOut = int(mdStart[eb + modOut]) + inner;
+                pass = sgCandidate(in, mi, mdOut, mods[modOut], true, H, D, o);
+            }
+            appendSG(pass, lane, base, off, mdOut, o, slabOut, slabF);
+        }
+    }
+    if (lane == 0) sgCount[item] = base;
+}
+
+// ------------------------------------------------------- layer-pair form
+// createSegments / createTracklets / createTrackCandidates (Event.cc:432-443,
+// 599-609, 755-872): every object of an inner layer against every object of an
+// outer layer, for fixed lists of layer pairs (Layer.cc:3-52).  Layer bins:
+// barrel layers 1-6 = 0-5, endcap layers 1-5 = 6-10.
+//
+// Objects of one layer of one event: the geometry lists the modules by layer
+// bin (layerMods, with layerModStart[bin] .. [bin + 1]); a per-slot object array
+// (start) then gives each module's range, and a per-event list of the objects
+// of every layer is filled in that order (layerOff[e * NM + position] = offset).
+
+// createSegments' layer pairs (inner bin, outer bin), Layer.cc:35-52.
+constexpr int kNSgLayerPairs = 16;
+__constant__ int kSgLayerPairs[kNSgLayerPairs][2] = {
+    {0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5},  // barrel 1-2 ... 5-6
+    {0, 6}, {1, 6}, {2, 6}, {3, 6}, {4, 6},  // barrel 1 ... 5 - endcap 1
+    {6, 7}, {7, 8}, {7, 9}, {7, 10}, {8, 9}, {9, 10}};  // endcap 1-2, 2-3, 2-4, 2-5, 3-4, 4-5
+
+__global__ void k_layer_count(int E, int NM, const int* layerMods, const long long* start, int* cnt) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)E * NM) return;
+    const long long e = i / NM;
+    const int m = layerMods[i % NM];
+    cnt[i] = int(start[e * NM + m + 1] - start[e * NM + m]);
+}
+
+__global__ void k_layer_fill(int E, int NM, const int* layerMods, const long long* start, const long long* off,
+                             int* list) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)E * NM) return;
+    const long long e = i / NM;
+    const int m = layerMods[i % NM];
+    const long long a = start[e * NM + m], n = start[e * NM + m + 1] - a, d = off[i];
+    for (long long k = 0; k < n; ++k) list[d + k] = int(a + k);
+}
+
+// [a, b): the objects of layer bin L of event e in a list filled by k_layer_fill.
+__device__ __forceinline__ void layerRange(const long long* off, const int* layerModStart, int NM, int e, int L,
+                                           long long& a, long long& b) {
+    a = off[(long long)e * NM + layerModStart[L]];
+    b = off[(long long)e * NM + layerModStart[L + 1]];
+}
+
+// Per inner MD: SG candidates of the layer pairs that start at its layer.
+__global__ void k_sgl_cap(Count cMD, int NM, const ModDev* mods, const int* mdSlot, const long long* layerOff,
+                          const int* layerModStart, int* sgCap) {
+    const int nMD = int(cMD.get());
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nMD) return;
+    const int slot = mdSlot[j];
+    const int e = slot / NM, L = layerBin(mods[slot % NM]);
+    long long c = 0;
+    for (int p = 0; p < kNSgLayerPairs; ++p)
+        if (kSgLayerPairs[p][0] == L) {
+            long long a, b;
+            layerRange(layerOff, layerModStart, NM, e, kSgLayerPairs[p][1], a, b);
+            c += b - a;
+        }
+    sgCap[j] = int(c);
+}
+
+// One wave per inner MD: the SG algorithm against every MD of each outer layer
+// (Event.cc:793-821), survivors appended to the MD's slab as in k_sg.  The
+// endcap algorithm's connection test (Segment.cc:627-634) is evaluated here,
+// since layer pairs need not be connected modules.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_sgl(Count cMD, int NM, const ModDev* mods, HitsDev H, MDDev D, const int* connStart, const int* connList,
+      const long long* layerOff, const int* layerModStart, const int* layerMd, const int* sgCap,
+      const long long* sgCapOff, int* slabOut, SGF* slabF, int* sgCount, Guard g) {
+    if (g.over()) return;
+    const int nMD = int(cMD.get());
+    const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= nMD) return;
+    if (sgCap[item] == 0) {
+        if (lane == 0) sgCount[item] = 0;
+        return;
+    }
+    const int sIn = D.slot[item];
+    const int mIn = sIn % NM, e = sIn / NM;
+    const ModDev mi = mods[mIn];
+    const int L = layerBin(mi);
+    const MDView in = loadMD(item, D);
+    const int c0 = connStart[mIn], c1 = connStart[mIn + 1];
+    const long long off = sgCapOff[item];
+    int base = 0;
+    for (int p = 0; p < kNSgLayerPairs; ++p) {
+        if (kSgLayerPairs[p][0] != L) continue;
+        long long a, b;
+        layerRange(layerOff, layerModStart, NM, e, kSgLayerPairs[p][1], a, b);
+        for (long long k0 = a; k0 < b; k0 += kWave) {
+            const long long k = k0 + lane;
+            bool pass = false;
+            SGOut o;
+            int mdOut = 0;
+            if (k < b) {
+                mdOut = layerMd[k];
+                const int modOut = D.slot[mdOut] % NM;
                 const ModDev mO = mods[modOut];
-                const MDView out = loadMD(mdOut, D);
-                HitView hiE = out.a, loE = out.a;
-                if (mO.subdet == SUB_ENDCAP && mO.type == TYPE_2S) {
-                    const int an = D.anchor[mdOut];
-                    const float4 a = H.eHi[an], b = H.eLo[an];
-                    hiE.phi = a.w;
-                    loE.phi = b.w;
-                }
-                pass = sgAlgo(in, out, mi, mO, true, hiE, loE, o);
+                bool conn = false;
+                if (mO.subdet != SUB_BARREL)
+                    for (int q = c0; q < c1; ++q) conn = conn || connList[q] == modOut;
+                pass = sgCandidate(in, mi, mdOut, mO, conn, H, D, o);
             }
-            const unsigned long long mask = __ballot(pass);
-            if (pass) {
-                const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
-                slabOut[off + pos] = mdOut;
-                SGF f;
-                f.v[0] = o.dphi; f.v[1] = o.dphimin; f.v[2] = o.dphimax;
-                f.v[3] = o.dphichange; f.v[4] = o.dphichangemin; f.v[5] = o.dphichangemax;
-                slabF[off + pos] = f;
-            }
-            base += __popcll(mask);
+            appendSG(pass, lane, base, off, mdOut, o, slabOut, slabF);
         }
     }
     if (lane == 0) sgCount[item] = base;
@@ -1280,14 +1411,131 @@ __global__ void k_tcl_count(Count cTL, const int* tlOutArr, const long long* tlS
     const int s = tlOutArr[l];
     cnt[l] = int(tlStart[s + 1] - tlStart[s]);
 }
-__global__ void k_tcl_write(Count cTL, const int* tlOutArr, const long long* tlStart, const long long* start, int2* tc, Guard g) {
+// TCs of inner TL l: the cnt[l] TLs that start at l's outer SG.
+__global__ void k_tcl_write(Count cTL, const int* tlOutArr, const long long* tlStart, const int* cnt,
+                            const long long* start, int2* tc, Guard g) {
     if (g.over()) return;
     const int nTL = int(cTL.get());
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nTL) return;
-    const int s = tlOutArr[l];
-    const long long a = tlStart[s], n = tlStart[s + 1] - a, d = start[l];
+    const long long a = tlStart[tlOutArr[l]], n = cnt[l], d = start[l];
     for (long long k = 0; k < n; ++k) tc[d + k] = make_int2(l, int(a + k));
+}
+
+// Layer-pair tracklets (Event.cc:599-609, 755-790; tracklet layer pairs barrel
+// 1-3, 2-4, 3-5, Layer.cc:3-33): per inner SG of barrel layers 1-3, every SG of
+// the layer two further out (tlCap, also the candidate counter of the inner
+// layer, Event.cc:780).
+__global__ void k_tll_cap(Count cSG, int NM, const ModDev* mods, const int* sgIn, const int* mdSlot,
+                          const long long* layerOff, const int* layerModStart, long long* tlCap, int* capInt) {
+    const int nSG = int(cSG.get());
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSG) return;
+    const int slot = mdSlot[sgIn[s]];
+    const int e = slot / NM, L = layerBin(mods[slot % NM]);
+    long long c = 0;
+    if (L <= 2) {
+        long long a, b;
+        layerRange(layerOff, layerModStart, NM, e, L + 2, a, b);
+        c = b - a;
+    }
+    tlCap[s] = c;
+    capInt[s] = int(c);
+}
+
+// One wave per inner SG: both tracklet stages against every SG of the outer
+// layer (whose inner MDs are barrel modules, so the algorithm variant is the
+// inner SG's); survivors appended to the SG's slab in candidate order.
+__global__ void __launch_bounds__(kBlock) k_tll(Count cSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
+                                                const long long* layerOff, const int* layerModStart, const int* layerSg,
+                                                const int* capInt, const long long* capOff, int* slab, int* tlCount,
+                                                Guard g) {
+    if (g.over()) return;
+    const int nSG = int(cSG.get());
+    const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= nSG) return;
+    if (capInt[item] == 0) {
+        if (lane == 0) tlCount[item] = 0;
+        return;
+    }
+    const int slot = D.slot[S.in[item]];
+    const int e = slot / NM, L = layerBin(mods[slot % NM]);
+    long long a, b;
+    layerRange(layerOff, layerModStart, NM, e, L + 2, a, b);
+    const TLInner I = loadInner(item, D, S);
+    const int fl = S.flags[item];
+    const int V = tlVariant(fl & 1, fl & 2, true);
+    const long long off = capOff[item];
+    int base = 0;
+    for (long long k0 = a; k0 < b; k0 += kWave) {
+        const long long k = k0 + lane;
+        bool pass = false;
+        int so = 0;
+        if (k < b && V < kNVariantsTL) {
+            so = layerSg[k];
+            const int flo = S.flags[so];
+            const TLOutLo Lo = loadOutLo(S.in[so], flo, D);
+            TLCarry c{0.f, 0.f, 0.f};
+            if (tlStage1(V, I, Lo, c)) {
+                const TLOutUp U = loadOutUp(so, flo, H, D, S);
+                pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c);
+            }
+        }
+        const unsigned long long mask = __ballot(pass);
+        if (pass) slab[off + base + __popcll(mask & ((1ull << lane) - 1ull))] = so;
+        base += __popcll(mask);
+    }
+    if (lane == 0) tlCount[item] = base;
+}
+
+// Tracklets of inner SG s, in slab order, at tlStart[s].
+__global__ void k_tll_repack(Count cSG, const long long* capOff, const long long* tlStart, const int* slab, int* tlIn,
+                             int* tlOut, Guard g) {
+    if (g.over()) return;
+    const int nSG = int(cSG.get());
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSG) return;
+    const long long d = tlStart[s], n = tlStart[s + 1] - d, o = capOff[s];
+    for (long long k = 0; k < n; ++k) {
+        tlIn[d + k] = s;
+        tlOut[d + k] = slab[o + k];
+    }
+}
+
+// TLs per (event, layer bin of the inner SG): the layer lists of tracklets.
+__global__ void k_tl_per_layer(Count cTL, int NM, const ModDev* mods, const int* tlIn, const int* sgIn,
+                               const int* mdSlot, int* perLayer) {
+    const int nTL = int(cTL.get());
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nTL) return;
+    const int slot = mdSlot[sgIn[tlIn[l]]];
+    atomicAdd(&perLayer[(slot / NM) * kNBins + layerBin(mods[slot % NM])], 1);
+}
+
+// Layer-pair track candidates (Event.cc:823-872: barrel layer 1 against barrel
+// layer 3 only): inner TL l of layer bin 0 pairs with every TL of bin 2; the
+// default algorithm passes iff l's outer SG is the other TL's inner SG
+// (TrackCandidate.cc:143-226, A-13), i.e. the TLs that start at l's outer SG
+// when that SG lies in bin 2.  cand[l] = the candidates counted for l.
+__global__ void k_tcll_count(Count cTL, int NM, const ModDev* mods, const int* tlIn, const int* tlOut,
+                             const int* sgIn, const int* mdSlot, const long long* tlStart, const int* perLayer,
+                             int* cnt, long long* cand) {
+    const int nTL = int(cTL.get());
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nTL) return;
+    const int slot = mdSlot[sgIn[tlIn[l]]];
+    const int e = slot / NM;
+    int c = 0;
+    long long k = 0;
+    if (layerBin(mods[slot % NM]) == 0) {
+        k = perLayer[e * kNBins + 2];
+        const int so = tlOut[l];
+        const int slotO = mdSlot[sgIn[so]];
+        if (layerBin(mods[slotO % NM]) == 2) c = int(tlStart[so + 1] - tlStart[so]);
+    }
+    cnt[l] = c;
+    cand[l] = k;
 }
 
 // TC from triplets (Event.cc:889-982).  The candidate (inner TP t ending at MD C,
@@ -1346,13 +1594,6 @@ __global__ void k_tcp_count(Count cTL, const int* tlIn, const int* tlOutArr, con
 }
 
 // ------------------------------------------------------------------ counters
-// Layer bin of a module: barrel layers 0..5, endcap layers 6..10.
-__device__ __forceinline__ int layerBin(const ModDev& m) {
-    return m.subdet == SUB_BARREL ? m.layer - 1 : 6 + m.layer - 1;
-}
-
-constexpr int kNCounters = 10;
-constexpr int kNBins = 11;
 
 // All per-layer counters of the stages in `stages` (bit s = stage s), one
 // grid-stride pass over slots, MDs, SGs, TPs and TLs (Event.cc:1169-1344).  A
@@ -1372,6 +1613,7 @@ struct CounterIn {
     const long long *slotSgStart, *slotTpStart, *slotTlStart;
     const int *tpIn, *tpOut;                                 // per TP
     const int *tlIn, *tclCount;                              // per TL
+    const long long* tcCand;  // per TL: TC candidates of the layer-pair form (null: the module-map count)
 };
 
 __device__ __forceinline__ long long connected(const CounterIn& C, int slot, const long long* slotStart) {
@@ -1438,9 +1680,9 @@ __global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long lon
             continue;
         }
         j -= nTp;
-        {  // TC candidates and objects per inner TL (Event.cc:999-1062)
+        {  // TC candidates and objects per inner TL (Event.cc:999-1062; layer pairs: Event.cc:840-872)
             const int slot = C.mdSlot[C.sgIn[C.tlIn[j]]];
-            add(slot, 4, connected(C, C.mdSlot[C.sgOut[C.tlIn[j]]], C.slotTlStart));
+            add(slot, 4, C.tcCand ? C.tcCand[j] : connected(C, C.mdSlot[C.sgOut[C.tlIn[j]]], C.slotTlStart));
             add(slot, 9, C.tclCount[j]);
         }
     }
@@ -1600,6 +1842,8 @@ struct sdl_geom {
     uint32_t* dDet = nullptr;
     ModDev* dMods = nullptr;
     int *dConnStart = nullptr, *dConnList = nullptr;
+    // modules ordered by layer bin (then index), and where each bin starts
+    int *dLayerMods = nullptr, *dLayerModStart = nullptr;
 };
 
 struct sdl_mlp {
@@ -1659,6 +1903,7 @@ struct sdl_event {
     bool haveHistory = false;
     long long lastTot[TOT_N] = {};
     long long lastNH = 0;
+    unsigned ranMask = 0;  // stages run on the current batch (bit = stage)
     // launch shapes (chosen by sdl_event_autotune; defaults below)
     Tuning tune;
     // captured chain (speculative mode)
@@ -1720,6 +1965,9 @@ struct sdl_event {
     DBuf<float4> tlCarry;
     DBuf<int> tlNOut;
     DBuf<unsigned char> tlFlag;
+    // layer-pair stages: per-event layer lists of MDs and SGs, TL candidate slabs
+    DBuf<int> layerCnt, layerMd, layerSg, tlCapInt, tllSlab, tlPerLayer;
+    DBuf<long long> layerMdOff, layerSgOff, tcCand;
     DBuf<int> tlScored;  // learned proposal: pairs scored per wave, per variant (diagnostic)
     long long tlScoredStride = 0;
     // learned proposal (null: the plain chain)
@@ -1919,6 +2167,7 @@ int runPrep(sdl_event* ev) {
     sdl_geom* G = ev->g;
     hipStream_t st = ev->st;
     const int NS = ev->E * G->NM;
+    ev->ranMask = 0;
     if (int r = ensureHits(ev)) return r;
     zeroAsync(ev->tot.p, TOT_N, st);
     zeroAsync(ev->slotHitCount.p, NS, st);
@@ -1994,32 +2243,47 @@ int runMD(sdl_event* ev) {
     return 0;
 }
 
-int runSG(sdl_event* ev) {
+int fillLayerList(sdl_event* ev, const long long* start, DBuf<long long>& off, DBuf<int>& list, long long nObj);
+
+// Segments: module-map candidates (createSegmentsWithModuleMap), or with
+// layers = true the layer-pair candidates (createSegments); the objects are
+// stored alike, by inner MD.
+int runSG(sdl_event* ev, bool layers) {
     sdl_geom* G = ev->g;
     hipStream_t st = ev->st;
     const int NS = ev->E * G->NM;
     const long long nMD = ev->nMD;
     const long long nm = std::max(nMD, 1LL);
+    if (layers)
+        if (int r = fillLayerList(ev, ev->mdStart.p, ev->layerMdOff, ev->layerMd, nMD)) return r;
     for (DBuf<int>* b : {&ev->sgCap, &ev->sgCount, &ev->sgEndCount, &ev->sgEndFill})
         if (int r = b->ensure(nm)) return r;
     for (DBuf<long long>* b : {&ev->sgCapOff, &ev->sgStart, &ev->sgEndStart})
         if (int r = b->ensure(nm + 1)) return r;
     if (int r = ev->slotSgStart.ensure(NS + 1)) return r;
     const Count cMD = devCount(ev, TOT_MD);
-    if (nMD > 0)
+    if (nMD > 0 && !layers)
         hipLaunchKernelGGL(k_sg_cap, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, cMD, G->NM, ev->mdSlot.p,
                            G->dConnStart, G->dConnList, ev->mdCount.p, ev->sgCap.p);
+    if (nMD > 0 && layers)
+        hipLaunchKernelGGL(k_sgl_cap, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, cMD, G->NM, G->dMods,
+                           ev->mdSlot.p, ev->layerMdOff.p, G->dLayerModStart, ev->sgCap.p);
     if (int r = scanExclusive<int>(ev->sgCap.p, ev->sgCapOff.p, cMD, nMD, ev->scan, st, ev->tot.p, TOT_SGCAP)) return r;
     const long long slab = sizeOf(ev, TOT_SGCAP);
     if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
     if (int r = ev->sgSlabOut.ensure(slab)) return r;
     if (int r = ev->sgSlabF.ensure(slab)) return r;
     timerBegin(ev, T_SG);
-    if (nMD > 0)
+    if (nMD > 0 && !layers)
         hipLaunchKernelGGL(k_sg, dim3(gridFor(nMD, ev->tune.v[TUNE_SG_BLOCK] / kWave)), dim3(ev->tune.v[TUNE_SG_BLOCK]), 0,
                            st, cMD, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), G->dConnStart, G->dConnList, ev->mdStart.p, ev->mdCount.p,
                            ev->sgCap.p, ev->sgCapOff.p, ev->sgSlabOut.p, ev->sgSlabF.p, ev->sgCount.p,
+                           guardOf(ev, TOT_SGCAP, slab, SDL_STAGE_SG));
+    if (nMD > 0 && layers)
+        hipLaunchKernelGGL(k_sgl, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, cMD, G->NM, G->dMods,
+                           hitsView(ev), mdView(ev), G->dConnStart, G->dConnList, ev->layerMdOff.p, G->dLayerModStart,
+                           ev->layerMd.p, ev->sgCap.p, ev->sgCapOff.p, ev->sgSlabOut.p, ev->sgSlabF.p, ev->sgCount.p,
                            guardOf(ev, TOT_SGCAP, slab, SDL_STAGE_SG));
     timerEnd(ev, T_SG);
     if (int r = scanExclusive<int>(ev->sgCount.p, ev->sgStart.p, cMD, nMD, ev->scan, st, ev->tot.p, TOT_SG)) return r;
@@ -2299,18 +2563,115 @@ int runTCL(sdl_event* ev) {
     if (int r = ev->tcl.ensure(std::max(nTCL, 1LL))) return r;
     if (nTL > 0)
         hipLaunchKernelGGL(k_tcl_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlOut.p,
-                           ev->tlStart.p, ev->tclStart.p, ev->tcl.p, guardOf(ev, TOT_TCL, nTCL, SDL_STAGE_TCL));
+                           ev->tlStart.p, ev->tcCount.p, ev->tclStart.p, ev->tcl.p,
+                           guardOf(ev, TOT_TCL, nTCL, SDL_STAGE_TCL));
+    return 0;
+}
+
+// ------------------------------------------------------- layer-pair stages
+// The objects of every layer of every event, from a per-slot range array.
+int fillLayerList(sdl_event* ev, const long long* start, DBuf<long long>& off, DBuf<int>& list, long long nObj) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const long long NS = (long long)ev->E * G->NM;
+    if (int r = ev->layerCnt.ensure(NS)) return r;
+    if (int r = off.ensure(NS + 1)) return r;
+    if (int r = list.ensure(std::max(nObj, 1LL))) return r;
+    hipLaunchKernelGGL(k_layer_count, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, ev->E, G->NM, G->dLayerMods,
+                       start, ev->layerCnt.p);
+    if (int r = scanExclusive<int>(ev->layerCnt.p, off.p, hostCount(ev, NS), NS, ev->scan, st, ev->tot.p, -1)) return r;
+    hipLaunchKernelGGL(k_layer_fill, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, ev->E, G->NM, G->dLayerMods,
+                       start, off.p, list.p);
+    return 0;
+}
+
+int runTLLayers(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const int NS = ev->E * G->NM;
+    const long long nSG = ev->nSG;
+    const long long ns = std::max(nSG, 1LL);
+    if (int r = fillLayerList(ev, ev->slotSgStart.p, ev->layerSgOff, ev->layerSg, nSG)) return r;
+    if (int r = ev->tlCap.ensure(ns)) return r;
+    if (int r = ev->tlCapInt.ensure(ns)) return r;
+    if (int r = ev->tlCapOff.ensure(ns + 1)) return r;
+    if (int r = ev->tlCount.ensure(ns)) return r;
+    if (int r = ev->tlStart.ensure(ns + 1)) return r;
+    if (int r = ev->slotTlStart.ensure(NS + 1)) return r;
+    const Count cSG = devCount(ev, TOT_SG);
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tll_cap, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
+                           ev->sgIn.p, ev->mdSlot.p, ev->layerSgOff.p, G->dLayerModStart, ev->tlCap.p, ev->tlCapInt.p);
+    if (int r = scanExclusive<int>(ev->tlCapInt.p, ev->tlCapOff.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TLCANDS))
+        return r;
+    const long long slab = sizeOf(ev, TOT_TLCANDS);
+    if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->tllSlab.ensure(std::max(slab, 1LL))) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tll, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
+                           hitsView(ev), mdView(ev), sgView(ev), ev->layerSgOff.p, G->dLayerModStart, ev->layerSg.p,
+                           ev->tlCapInt.p, ev->tlCapOff.p, ev->tllSlab.p, ev->tlCount.p,
+                           guardOf(ev, TOT_TLCANDS, slab, SDL_STAGE_TL));
+    if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
+    const long long nTL = sizeOf(ev, TOT_TL);
+    if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nTL = nTL;
+    if (int r = ev->tlIn.ensure(std::max(nTL, 1LL))) return r;
+    if (int r = ev->tlOut.ensure(std::max(nTL, 1LL))) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tll_repack, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->tlCapOff.p,
+                           ev->tlStart.p, ev->tllSlab.p, ev->tlIn.p, ev->tlOut.p, guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
+    hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+                       ev->slotSgStart.p, ev->tlStart.p, ev->slotTlStart.p);
+    return 0;
+}
+
+int runTCLayers(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const long long nTL = ev->nTL;
+    const long long nl = std::max(nTL, 1LL);
+    if (int r = ev->tcCount.ensure(nl)) return r;
+    if (int r = ev->tcCand.ensure(nl)) return r;
+    if (int r = ev->tclStart.ensure(nl + 1)) return r;
+    if (int r = ev->tcpStart.ensure(nl + 1)) return r;
+    if (int r = ev->tlPerLayer.ensure((long long)ev->E * kNBins)) return r;
+    // no TCs from triplets in this form
+    zeroAsync(ev->tcpStart.p, nl + 1, st);
+    zeroAsync(ev->tot.p + TOT_TCP, 1, st);
+    ev->nTCP = 0;
+    zeroAsync(ev->tlPerLayer.p, (long long)ev->E * kNBins, st);
+    const Count cTL = devCount(ev, TOT_TL);
+    if (nTL > 0) {
+        hipLaunchKernelGGL(k_tl_per_layer, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, G->NM, G->dMods,
+                           ev->tlIn.p, ev->sgIn.p, ev->mdSlot.p, ev->tlPerLayer.p);
+        hipLaunchKernelGGL(k_tcll_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, G->NM, G->dMods,
+                           ev->tlIn.p, ev->tlOut.p, ev->sgIn.p, ev->mdSlot.p, ev->tlStart.p, ev->tlPerLayer.p,
+                           ev->tcCount.p, ev->tcCand.p);
+    }
+    if (int r = scanExclusive<int>(ev->tcCount.p, ev->tclStart.p, cTL, nTL, ev->scan, st, ev->tot.p, TOT_TCL)) return r;
+    const long long nTCL = sizeOf(ev, TOT_TCL);
+    if (nTCL < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nTCL = nTCL;
+    if (int r = ev->tcl.ensure(std::max(nTCL, 1LL))) return r;
+    if (nTL > 0)
+        hipLaunchKernelGGL(k_tcl_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlOut.p,
+                           ev->tlStart.p, ev->tcCount.p, ev->tclStart.p, ev->tcl.p,
+                           guardOf(ev, TOT_TCL, nTCL, SDL_STAGE_TCL));
     return 0;
 }
 
 int runStage(sdl_event* ev, int stage) {
     switch (stage) {
         case SDL_STAGE_MD: return runMD(ev);
-        case SDL_STAGE_SG: return runSG(ev);
+        case SDL_STAGE_SG: return runSG(ev, false);
         case SDL_STAGE_TP: return runTP(ev);
         case SDL_STAGE_TL: return runTL(ev);
         case SDL_STAGE_TCP: return runTCP(ev);
         case SDL_STAGE_TCL: return runTCL(ev);
+        case SDL_STAGE_SG_LAYERS: return runSG(ev, true);
+        case SDL_STAGE_TL_LAYERS: return runTLLayers(ev);
+        case SDL_STAGE_TC_LAYERS: return runTCLayers(ev);
         default: return fail(SDL_ERR_ARG, "bad stage");
     }
 }
@@ -2323,7 +2684,18 @@ void launchCounters(sdl_event* ev, int s0, int s1) {
     C.NM = G->NM;
     C.E = ev->E;
     C.stages = 0;
-    for (int s = s0; s <= s1; ++s) C.stages |= 1 << s;
+    C.tcCand = nullptr;
+    for (int s = s0; s <= s1; ++s) {
+        if (s == SDL_STAGE_SG_LAYERS) continue;  // createSegmentsFromTwoLayers counts nothing (Event.cc:793-821)
+        if (s == SDL_STAGE_TL_LAYERS) {           // counted on the inner layer (Event.cc:778-786)
+            C.stages |= 1 << SDL_STAGE_TL;
+        } else if (s == SDL_STAGE_TC_LAYERS) {    // counted on the inner layer (Event.cc:859-867)
+            C.stages |= 1 << SDL_STAGE_TCL;
+            C.tcCand = ev->tcCand.p;
+        } else {
+            C.stages |= 1 << s;
+        }
+    }
     C.mods = G->dMods;
     C.connStart = G->dConnStart;
     C.connList = G->dConnList;
@@ -2414,7 +2786,7 @@ int runStages(sdl_event* ev, int s0, int s1) {
     } else {
         zeroAsync(ev->tot.p + TOT_ABORT, 1, st);
     }
-    ev->spec = ev->haveHistory;
+    ev->spec = ev->haveHistory && s0 < SDL_STAGE_SG_LAYERS;  // the layer-pair stages are sized exactly
     bool launched = false;
     if (ev->spec && ev->useGraphs && !ev->timing && !ev->hitsReady && s0 == SDL_STAGE_MD && s1 == SDL_STAGE_TCL) {
         const int r = launchChainGraph(ev);
@@ -2443,6 +2815,7 @@ int runStages(sdl_event* ev, int s0, int s1) {
     }
     ev->nMD = t[TOT_MD]; ev->nSG = t[TOT_SG]; ev->nTP = t[TOT_TP];
     ev->nTL = t[TOT_TL]; ev->nTCP = t[TOT_TCP]; ev->nTCL = t[TOT_TCL];
+    for (int s = s0; s <= s1; ++s) ev->ranMask |= 1u << s;
     // remember the sizes of the stages that ran (the prediction for the next call)
     static const int stageOfTot[TOT_ABORT] = {SDL_STAGE_MD, SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_SG, SDL_STAGE_TP,
                                               SDL_STAGE_TL, SDL_STAGE_TL, SDL_STAGE_TCP, SDL_STAGE_TCP, SDL_STAGE_TCL,
@@ -2561,10 +2934,23 @@ int uploadGeom(sdl_geom* G) {
         if (bytes && hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(SDL_ERR_HIP, "hipMemcpy failed");
         return 0;
     };
+    std::vector<int> layerMods(G->NM), layerModStart(kNBins + 1, 0);
+    for (int m = 0; m < G->NM; ++m) {
+        const int b = layerBin(G->mods[m]);
+        if (b < 0 || b >= kNBins) return fail(SDL_ERR_IO, "module with a layer outside barrel 1-6 / endcap 1-5");
+        ++layerModStart[b + 1];
+    }
+    for (int b = 0; b < kNBins; ++b) layerModStart[b + 1] += layerModStart[b];
+    {
+        std::vector<int> fill(layerModStart.begin(), layerModStart.end() - 1);
+        for (int m = 0; m < G->NM; ++m) layerMods[fill[layerBin(G->mods[m])]++] = m;
+    }
     int r = up((void**)&G->dDet, G->detIds.data(), sizeof(uint32_t) * G->detIds.size());
     if (!r) r = up((void**)&G->dMods, G->mods.data(), sizeof(ModDev) * G->mods.size());
     if (!r) r = up((void**)&G->dConnStart, G->connStart.data(), sizeof(int) * G->connStart.size());
     if (!r) r = up((void**)&G->dConnList, G->connList.data(), sizeof(int) * G->connList.size());
+    if (!r) r = up((void**)&G->dLayerMods, layerMods.data(), sizeof(int) * layerMods.size());
+    if (!r) r = up((void**)&G->dLayerModStart, layerModStart.data(), sizeof(int) * layerModStart.size());
     return r;
 }
 
@@ -2707,6 +3093,8 @@ void sdl_geom_free(sdl_geom* g) {
     if (g->dMods) (void)hipFree(g->dMods);
     if (g->dConnStart) (void)hipFree(g->dConnStart);
     if (g->dConnList) (void)hipFree(g->dConnList);
+    if (g->dLayerMods) (void)hipFree(g->dLayerMods);
+    if (g->dLayerModStart) (void)hipFree(g->dLayerModStart);
     delete g;
 }
 
@@ -2787,13 +3175,23 @@ int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offse
 int sdl_event_run(sdl_event* ev, int stage, int algo) {
     if (!ev) return fail(SDL_ERR_ARG, "null event");
     if (algo != 0) return fail(SDL_ERR_ARG, "only the default algorithm has device kernels");
-    if (stage < SDL_STAGE_MD || stage > SDL_STAGE_TCL) return fail(SDL_ERR_ARG, "bad stage");
-    if (stage != ev->stage + 1) return fail(SDL_ERR_STATE, "stages must run in order");
+    if (stage < SDL_STAGE_MD || stage > SDL_STAGE_TC_LAYERS) return fail(SDL_ERR_ARG, "bad stage");
+    // each stage reads the objects of the stage before it: module-map chain MD,
+    // SG, TP, TL, TCP, TCL; layer-pair stages after MD (segments), after either
+    // segment stage (tracklets) and after either tracklet stage (track
+    // candidates).  The module-map joins (TP, TCP, TCL) assume module-map
+    // inputs, so they do not follow a layer-pair stage.
+    bool ok = stage == ev->stage + 1 && stage <= SDL_STAGE_TCL;
+    if (stage == SDL_STAGE_SG_LAYERS) ok = ev->stage == SDL_STAGE_MD;
+    if (stage == SDL_STAGE_TL_LAYERS) ok = ev->stage == SDL_STAGE_SG || ev->stage == SDL_STAGE_SG_LAYERS;
+    if (stage == SDL_STAGE_TC_LAYERS) ok = ev->stage == SDL_STAGE_TL || ev->stage == SDL_STAGE_TL_LAYERS;
+    if (!ok) return fail(SDL_ERR_STATE, "stages must run in order");
     if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
     if (!ev->hitsReady) {
         if (ev->hostStaged && ev->hoff.size() == 1) ev->hoff.push_back(0);
         if (int r = beginBatch(ev)) return r;
         if (ev->E <= 0) return fail(SDL_ERR_STATE, "no event");
+        ev->ranMask = 0;
     }
     if (int r = runStages(ev, stage, stage)) return r;
     if (hipGetLastError() != hipSuccess) return fail(SDL_ERR_HIP, "kernel launch failed");
@@ -2806,6 +3204,7 @@ int sdl_event_run_chain(sdl_event* ev) {
     if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
     ev->stage = -1;
     ev->hitsReady = false;
+    ev->ranMask = 0;
     if (ev->hostStaged && ev->hoff.size() == 1) ev->hoff.push_back(0);
     if (int r = beginBatch(ev)) return r;
     if (ev->E <= 0) return fail(SDL_ERR_STATE, "no event");
@@ -2837,15 +3236,44 @@ int sdl_event_counts(sdl_event* ev, int e, unsigned long long out[10]) {
     for (int k = 0; k < kNCounters; ++k) out[k] = 0;
     for (int ee = 0; ee < ev->E; ++ee) {
         if (e >= 0 && ee != e) continue;
-        for (int k = 0; k < kNCounters; ++k)
+        for (int k = 0; k < 5; ++k)
             for (int b = 0; b < kNBins; ++b) out[k] += all[(size_t(ee) * kNCounters + k) * kNBins + b];
+    }
+    // Object totals are the sizes of the object lists (Event.cc:1080-1092), not
+    // sums of the per-layer counters, which some stages leave untouched.
+    const unsigned m = ev->ranMask;
+    const bool ran[5] = {bool(m & (1u << SDL_STAGE_MD)),
+                         bool(m & ((1u << SDL_STAGE_SG) | (1u << SDL_STAGE_SG_LAYERS))),
+                         bool(m & (1u << SDL_STAGE_TP)),
+                         bool(m & ((1u << SDL_STAGE_TL) | (1u << SDL_STAGE_TL_LAYERS))),
+                         bool(m & ((1u << SDL_STAGE_TCP) | (1u << SDL_STAGE_TCL) | (1u << SDL_STAGE_TC_LAYERS)))};
+    const int stages[5] = {SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_TP, SDL_STAGE_TL, SDL_STAGE_TCP};
+    for (int k = 0; k < 5; ++k) {
+        if (!ran[k]) continue;
+        if (e < 0) {
+            const long long tot[5] = {ev->nMD, ev->nSG, ev->nTP, ev->nTL, ev->nTCP + ev->nTCL};
+            out[5 + k] = (unsigned long long)tot[k];
+            continue;
+        }
+        long long a, n;
+        if (int r = stageRange(ev, e, stages[k], a, n)) return r;
+        out[5 + k] = (unsigned long long)n;
+        if (k == 4) {
+            if (int r = stageRange(ev, e, SDL_STAGE_TCL, a, n)) return r;
+            out[9] += (unsigned long long)n;
+        }
     }
     return 0;
 }
 
 int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n) {
     if (!ev || !n || e < 0 || e >= ev->E || stage < 0 || stage > SDL_STAGE_TCL) return fail(SDL_ERR_ARG, "bad arguments");
-    if (stage > ev->stage) return fail(SDL_ERR_STATE, "stage has not run");
+    // the layer-pair stages store SG, TL and TCL objects
+    unsigned need = 1u << stage;
+    if (stage == SDL_STAGE_SG) need |= 1u << SDL_STAGE_SG_LAYERS;
+    if (stage == SDL_STAGE_TL) need |= 1u << SDL_STAGE_TL_LAYERS;
+    if (stage == SDL_STAGE_TCL) need |= 1u << SDL_STAGE_TC_LAYERS;
+    if (!(ev->ranMask & need)) return fail(SDL_ERR_STATE, "stage has not run");
     long long a = 0, cnt = 0;
     if (int r = stageRange(ev, e, stage, a, cnt)) return r;
     *n = cnt;

@@ -79,7 +79,7 @@ void Event::addHitToModule(Hit hit, unsigned int detId) {
     // lower and upper sensors are counted apart (Event.cc:1170-1190)
     const int b = layerBin(detId);
     if (b >= 0 && b < 11) ++(isLowerSensor(detId) ? hitsByLayer_ : hitsByLayerUpper_)[b];
-    lastStage_ = -1;
+    ran_ = 0;
 }
 
 bool Event::run(int stage, int algo, const char* what) {
@@ -87,9 +87,10 @@ bool Event::run(int stage, int algo, const char* what) {
         std::printf("SDL::  Warning: %s algorithm %d has no device kernels; stage left empty\n", what, algo);
         return false;
     }
-    if (stage != lastStage_ + 1) throw std::logic_error(std::string("SDL: ") + what + " called out of order");
-    check(sdl_event_run(ev_, stage, algo), what);
-    lastStage_ = stage;
+    const int rc = sdl_event_run(ev_, stage, algo);
+    if (rc == SDL_ERR_STATE) throw std::logic_error(std::string("SDL: ") + what + " called out of order");
+    check(rc, what);
+    ran_ |= 1u << stage;
     return true;
 }
 
@@ -99,15 +100,22 @@ void Event::createTriplets(TPAlgo algo) { run(SDL_STAGE_TP, algo, "createTriplet
 void Event::createTrackletsWithModuleMap(TLAlgo algo) { run(SDL_STAGE_TL, algo, "createTrackletsWithModuleMap"); }
 void Event::createTrackCandidatesFromTriplets(TCAlgo algo) { run(SDL_STAGE_TCP, algo, "createTrackCandidatesFromTriplets"); }
 void Event::createTrackCandidatesFromTracklets(TCAlgo algo) { run(SDL_STAGE_TCL, algo, "createTrackCandidatesFromTracklets"); }
+void Event::createSegments(SGAlgo algo) { run(SDL_STAGE_SG_LAYERS, algo, "createSegments"); }
+void Event::createTracklets(TLAlgo algo) { run(SDL_STAGE_TL_LAYERS, algo, "createTracklets"); }
+void Event::createTrackCandidates(TCAlgo algo) { run(SDL_STAGE_TC_LAYERS, algo, "createTrackCandidates"); }
 
+// Per-layer counters from the layer histogram; totals as Event.cc:1080-1112 define
+// them (candidates: sums over the layers; objects: sizes of the object lists).
 unsigned int Event::counter(int which, int bin) {
-    if (lastStage_ < 0) return 0;
-    unsigned long long c[110];
-    check(sdl_event_layer_counts(ev_, 0, c), "counters");
-    if (bin >= 0) return (unsigned int)c[which * 11 + bin];
-    unsigned long long s = 0;
-    for (int b = 0; b < 11; ++b) s += c[which * 11 + b];
-    return (unsigned int)s;
+    if (ran_ == 0) return 0;
+    if (bin >= 0) {
+        unsigned long long c[110];
+        check(sdl_event_layer_counts(ev_, 0, c), "counters");
+        return (unsigned int)c[which * 11 + bin];
+    }
+    unsigned long long c[10];
+    check(sdl_event_counts(ev_, 0, c), "counters");
+    return (unsigned int)c[which];
 }
 
 unsigned int Event::getNumberOfHits() { return nHits_; }
@@ -149,7 +157,12 @@ unsigned int Event::getNumberOfTrackCandidatesByLayerEndcap(unsigned int l) { re
 template <int W>
 std::vector<std::array<int, W>> Event::objects(int stage) {
     std::vector<std::array<int, W>> out;
-    if (stage > lastStage_) return out;
+    // the layer-pair stages store their objects as SG, TL and TCL objects
+    unsigned need = 1u << stage;
+    if (stage == SDL_STAGE_SG) need |= 1u << SDL_STAGE_SG_LAYERS;
+    if (stage == SDL_STAGE_TL) need |= 1u << SDL_STAGE_TL_LAYERS;
+    if (stage == SDL_STAGE_TCL) need |= 1u << SDL_STAGE_TC_LAYERS;
+    if (!(ran_ & need)) return out;
     long long n = 0;
     check(sdl_event_export(ev_, 0, stage, nullptr, 0, &n), "export");
     out.resize(size_t(n));

@@ -1,7 +1,9 @@
 // A host program written against the SDL::Event API (as a user of the reference
 // would), running one generated event through the MI355X library.
-// Usage: sdl_shim_demo <workload> <event> <geomdir>
-// Prints the counters and a checksum of every object set as one line each.
+// Usage: sdl_shim_demo <workload> <event> <geomdir> [layers]
+// Runs the module-map chain, or with "layers" the layer-pair chain
+// (createMiniDoublets, createSegments, createTracklets, createTrackCandidates);
+// prints the counters and a checksum of every object set as one line each.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -38,12 +40,19 @@ int main(int argc, char** argv) {
 
     SDL::Event event;
     for (size_t i = 0; i < ev.x.size(); ++i) event.addHitToModule(SDL::Hit(ev.x[i], ev.y[i], ev.z[i], ev.idx[i]), ev.detId[i]);
+    const bool layers = argc > 4 && std::string(argv[4]) == "layers";
     event.createMiniDoublets();
-    event.createSegmentsWithModuleMap();
-    event.createTriplets();
-    event.createTrackletsWithModuleMap();
-    event.createTrackCandidatesFromTriplets();
-    event.createTrackCandidatesFromTracklets();
+    if (layers) {
+        event.createSegments();
+        event.createTracklets();
+        event.createTrackCandidates();
+    } else {
+        event.createSegmentsWithModuleMap();
+        event.createTriplets();
+        event.createTrackletsWithModuleMap();
+        event.createTrackCandidatesFromTriplets();
+        event.createTrackCandidatesFromTracklets();
+    }
     std::printf("counts %u %u %u %u %u %u %u %u %u %u\n", event.getNumberOfMiniDoubletCandidates(),
                 event.getNumberOfSegmentCandidates(), event.getNumberOfTripletCandidates(),
                 event.getNumberOfTrackletCandidates(), event.getNumberOfTrackCandidateCandidates(),

@@ -89,6 +89,16 @@ class Event {
     void createTrackCandidatesFromTriplets(TCAlgo algo = Default_TCAlgo);
     void createTrackCandidatesFromTracklets(TCAlgo algo = Default_TCAlgo);
 
+    // Event.h:240-300, layer-pair family (Event.cc:432-443, 599-609, 823-837):
+    // every object of an inner layer against every object of an outer layer
+    // for the reference's fixed layer pairs; run after createMiniDoublets,
+    // after either segment stage and after either tracklet stage respectively.
+    // Their objects are returned by getSegments, getTracklets and
+    // getTrackCandidatesFromTracklets.
+    void createSegments(SGAlgo algo = Default_SGAlgo);
+    void createTracklets(TLAlgo algo = Default_TLAlgo);
+    void createTrackCandidates(TCAlgo algo = Default_TCAlgo);
+
     // Event.h:302-405
     unsigned int getNumberOfHits();
     unsigned int getNumberOfHitsByLayerBarrel(unsigned int);
@@ -143,7 +153,7 @@ class Event {
 
     sdl_event* ev_ = nullptr;
     LogLevel logLevel_ = Log_Nothing;
-    int lastStage_ = -1;
+    unsigned ran_ = 0;  // stages run since the last hit was added (bit = SDL_STAGE_*)
     unsigned int nHits_ = 0;
     std::array<unsigned int, 11> hitsByLayer_{};       // lower sensors
     std::array<unsigned int, 11> hitsByLayerUpper_{};  // upper sensors

@@ -14,6 +14,9 @@ LIB_PATH = os.environ.get("SDL_AMD_LIB", os.path.join(_HERE, "_lib", "libsdl_amd
 
 STAGE_MD, STAGE_SG, STAGE_TP, STAGE_TL, STAGE_TCP, STAGE_TCL = range(6)
 STAGES = ("MD", "SG", "TP", "TL", "TCP", "TCL")
+# layer-pair stages (createSegments, createTracklets, createTrackCandidates); their
+# objects are exported as SG, TL and TCL
+SG_LAYERS, TL_LAYERS, TC_LAYERS = 6, 7, 8
 WIDTH = {"MD": 2, "SG": 4, "TP": 8, "TL": 8, "TCP": 16, "TCL": 16}
 COUNTER_NAMES = ("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")
 

@@ -4,9 +4,12 @@ Runs in the build container only (it needs /root/reference): builds
 oracle/_ref (the unmodified reference sources + this repository's driver,
 oracle/ref_driver.cc) through oracle/Makefile, runs the reference's module-map
 chain on seeded generated events and stores every object set as hit-index
-tuples plus the stage counters in ``tests/golden/ref_sets.npz``.
+tuples plus the stage counters in ``tests/golden/ref_sets.npz``; and the
+layer-pair chain (createMiniDoublets, createSegments, createTracklets,
+createTrackCandidates) with the per-layer counters in
+``tests/golden/ref_layer_sets.npz``.
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [layers]   (layers: only the layer-pair fixtures)
 """
 import os
 import subprocess
@@ -22,10 +25,18 @@ import oracle  # noqa: E402
 
 # (workload, first event, number of events)
 CASES = [(1, 0, 2), (12, 0, 2), (15, 0, 1)]
+LAYER_CASES = [(1, 0, 2), (12, 0, 1)]
+COUNTERS = ("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")
 
 
 def main():
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    if len(sys.argv) < 2 or sys.argv[1] != "layers":
+        module_map_fixtures()
+    layer_fixtures()
+
+
+def module_map_fixtures():
     out = {}
     with tempfile.TemporaryDirectory() as tmp:
         for wl, e0, ne in CASES:
@@ -34,9 +45,22 @@ def main():
             for e, (sets, counters) in oracle.parse_ref_dump(dump).items():
                 for s in oracle.STAGES:
                     out[f"w{wl}_e{e}_{s}"] = oracle.as_multiset(sets[s])
-                out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in (
-                    "nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")], np.int64)
+                out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
     np.savez_compressed(os.path.join(HERE, "ref_sets.npz"), **out)
+
+
+def layer_fixtures():
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for wl, e0, ne in LAYER_CASES:
+            dump = os.path.join(tmp, f"w{wl}.txt")
+            subprocess.run([oracle.REF_DRIVER, "dumplayers", str(wl), tmp, str(e0), str(ne), dump], check=True)
+            for e, (sets, counters) in oracle.parse_ref_dump(dump).items():
+                for s in ("MD", "SG", "TL", "TCL"):
+                    out[f"w{wl}_e{e}_{s}"] = oracle.as_multiset(sets[s])
+                out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
+                out[f"w{wl}_e{e}_layers"] = counters["layers"]
+    np.savez_compressed(os.path.join(HERE, "ref_layer_sets.npz"), **out)
 
 
 if __name__ == "__main__":

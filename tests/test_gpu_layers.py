@@ -1,0 +1,120 @@
+"""The layer-pair family on the GPU (createSegments, createTracklets,
+createTrackCandidates; Event.cc:432-443, 599-609, 755-872) through the C-ABI,
+against the oracle (CR libm policy) and the compiled reference's golden
+per-layer counters.  Every object of an inner layer is paired with every
+object of an outer layer, so the checks run on small events."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from sdl_amd import gen, runtime
+
+pytestmark = pytest.mark.gpu
+
+LAYER_SEQ = [0, runtime.SG_LAYERS, runtime.TL_LAYERS, runtime.TC_LAYERS]
+
+
+def _geom(kind, cache={}):
+    if kind not in cache:
+        cache[kind] = runtime.Geometry(*gen.geometry(kind)[1])
+    return cache[kind]
+
+
+def _run(wl, events, seq):
+    ev = runtime.Event(_geom(gen.geom_kind(wl)))
+    for i, e in enumerate(events):
+        if i:
+            ev.next_event()
+        ev.add_hits(e.x, e.y, e.z, e.idx, e.detId)
+    for s in seq:
+        ev.run(s)
+    ev.sync()
+    return ev
+
+
+def _exported(ev, seq, e=0):
+    have = {"MD"}
+    for s in seq:
+        have.add({1: "SG", 2: "TP", 3: "TL", 4: "TCP", 5: "TCL", 6: "SG", 7: "TL", 8: "TCL"}.get(s, "MD"))
+    return {s: ev.export(i, e) if s in have else np.zeros((0, oracle.WIDTH[s]), np.int32)
+            for i, s in enumerate(oracle.STAGES)}
+
+
+def _assert_same(got, ref, what):
+    for s in oracle.STAGES:
+        a, b = oracle.as_multiset(got[s]), oracle.as_multiset(ref[s])
+        assert a.shape == b.shape, f"{what} {s}: gpu {a.shape} oracle {b.shape}"
+        assert np.array_equal(a, b), f"{what} {s} differs"
+
+
+@pytest.mark.parametrize("wl,e", [(1, 0), (1, 1), (1, 2), (12, 0), (15, 0)])
+def test_layer_chain_matches_oracle(wl, e):
+    print(f"case layer chain workload={wl} event={e}", flush=True)
+    ev_in = gen.event(wl, e)
+    ev = _run(wl, [ev_in], LAYER_SEQ)
+    ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], ev_in, oracle.CR, oracle.LAYER_CHAIN)
+    _assert_same(_exported(ev, LAYER_SEQ), ref, f"layer chain w{wl} e{e}")
+    assert ev.counts(0) == cnt
+    assert cnt["nTL"] > 0
+
+
+def test_layer_counters_match_reference_golden():
+    """Per-layer counters against the compiled reference (ref_layer_sets.npz)."""
+    print("case layer counters vs reference", flush=True)
+    path = os.path.join(os.path.dirname(__file__), "golden", "ref_layer_sets.npz")
+    with np.load(path) as f:
+        golden = {k: f[k] for k in f.files}
+    for wl, e in ((1, 0), (1, 1), (12, 0)):
+        ev = _run(wl, [gen.event(wl, e)], LAYER_SEQ)
+        assert np.array_equal(ev.layer_counts(0), golden[f"w{wl}_e{e}_layers"]), (wl, e)
+        got = ev.counts(0)
+        assert [got[k] for k in runtime.COUNTER_NAMES] == list(golden[f"w{wl}_e{e}_counters"]), (wl, e)
+
+
+@pytest.mark.parametrize("seq", [[0, 1, 7, 8], [0, 1, 2, 3, 8]])
+def test_mixed_stage_orders(seq):
+    """Module-map and layer-pair stages mixed as the reference allows: layer
+    tracklets over module-map segments, layer track candidates over module-map
+    tracklets."""
+    print(f"case mixed stages {seq}", flush=True)
+    wl, e = 12, 1
+    ev_in = gen.event(wl, e)
+    ev = _run(wl, [ev_in], seq)
+    ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], ev_in, oracle.CR, seq)
+    _assert_same(_exported(ev, seq), ref, f"stages {seq}")
+    assert ev.counts(0) == cnt
+
+
+def test_layer_batch_equals_single_events():
+    print("case layer chain batch of workload 1 events 0-1", flush=True)
+    evs = [gen.event(1, 0), gen.event(1, 1)]
+    batch = _run(1, evs, LAYER_SEQ)
+    for i, e in enumerate(evs):
+        single = _run(1, [e], LAYER_SEQ)
+        _assert_same(_exported(batch, LAYER_SEQ, i), _exported(single, LAYER_SEQ), f"batch event {i}")
+        assert batch.counts(i) == single.counts(0)
+        assert np.array_equal(batch.layer_counts(i), single.layer_counts(0))
+
+
+def test_layer_stage_order_rules():
+    print("case layer stage order", flush=True)
+    e = gen.event(1, 0)
+    ev = runtime.Event(_geom(1))
+    ev.add_hits(e.x, e.y, e.z, e.idx, e.detId)
+    with pytest.raises(runtime.SDLError):
+        ev.run(runtime.TC_LAYERS)  # before any tracklets
+    ev.run(0)
+    with pytest.raises(runtime.SDLError):
+        ev.run(runtime.TL_LAYERS)  # before any segments
+    ev.run(runtime.SG_LAYERS)
+    with pytest.raises(runtime.SDLError):
+        ev.run(2)  # module-map triplets do not follow layer-pair segments
+    ev.run(runtime.TL_LAYERS)
+    with pytest.raises(runtime.SDLError):
+        ev.run(4)  # TCs from triplets need module-map tracklets
+    with pytest.raises(runtime.SDLError):
+        ev.export(2)  # no triplets ran
+    ev.run(runtime.TC_LAYERS)
+    assert len(ev.export(5)) == ev.counts(0)["nTC"]

@@ -38,3 +38,20 @@ def test_shim_matches_oracle(wl, e, tmp_path):
     ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], gen.event(wl, e), oracle.CR)
     assert counts == list(cnt.values())
     assert sums == [_checksum(ref[s]) for s in oracle.STAGES]
+
+
+def test_shim_layer_chain_matches_oracle(tmp_path):
+    """createSegments, createTracklets and createTrackCandidates through the
+    shim on a barrel-only event: counters and object sets equal the oracle's."""
+    wl, e = 1, 2
+    print(f"case shim layer chain workload={wl} event={e}", flush=True)
+    out = subprocess.run([DEMO, str(wl), str(e), str(tmp_path), "layers"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = dict(l.split(" ", 1) for l in out.stdout.strip().splitlines())
+    counts = [int(v) for v in lines["counts"].split()]
+    sums = [int(v) for v in lines["checksums"].split()]
+    ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], gen.event(wl, e), oracle.CR, oracle.LAYER_CHAIN)
+    assert counts == list(cnt.values())
+    assert sums == [_checksum(ref[s]) for s in oracle.STAGES]
+    assert cnt["nTC"] > 0

@@ -101,3 +101,64 @@ def test_libm_listing_committed():
     assert "2:0" in d["cases"] and "2:1" in d["cases"]
     assert all(x["ulps"] <= 4.0 for x in d["differences"])
     assert d["differences"] == []
+
+
+LAYER_CASES = [(1, 0), (1, 1), (12, 0)]
+
+
+@pytest.fixture(scope="module")
+def layer_golden():
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "ref_layer_sets.npz")
+    with np.load(path) as f:
+        return {k: f[k] for k in f.files}
+
+
+@pytest.mark.parametrize("wl,e", LAYER_CASES)
+def test_oracle_layer_chain_matches_reference(wl, e, geom_paths, layer_golden):
+    """The layer-pair chain (createMiniDoublets, createSegments, createTracklets,
+    createTrackCandidates) against the compiled reference: object sets and the
+    ten totals (the object totals are list sizes, the candidate totals sums of
+    the per-layer counters, Event.cc:1080-1112)."""
+    ev = gen.event(wl, e)
+    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, oracle.LAYER_CHAIN)
+    for s in ("MD", "SG", "TL", "TCL"):
+        ref = layer_golden[f"w{wl}_e{e}_{s}"]
+        got = oracle.as_multiset(sets[s])
+        assert got.shape == ref.shape, (s, got.shape, ref.shape)
+        assert np.array_equal(got, ref), s
+        assert len(ref) > 0, f"{s}: empty golden set makes the check vacuous"
+    assert len(sets["TP"]) == 0 and len(sets["TCP"]) == 0
+    assert [counters[k] for k in COUNTERS] == list(layer_golden[f"w{wl}_e{e}_counters"])
+    # the per-layer counters of the reference: candidates and objects of the
+    # layer-pair tracklets and track candidates sit on the inner layer only
+    layers = layer_golden[f"w{wl}_e{e}_layers"]
+    assert layers[1].sum() == 0 and layers[6].sum() == 0  # createSegmentsFromTwoLayers counts nothing
+    assert layers[3][3:].sum() == 0 and layers[4][1:].sum() == 0 and layers[9][1:].sum() == 0
+
+
+def test_oracle_layer_chain_closed_forms(geom_paths):
+    """Invariants of the layer-pair chain that fix it independently of the
+    reference: the TL candidates are the products of the layer SG counts of the
+    three layer pairs, the TC candidates |TL(barrel 1)| x |TL(barrel 3)|, and a
+    TC exists for exactly the (TL, TL) pairs sharing the middle segment."""
+    wl, e = 1, 0
+    ev = gen.event(wl, e)
+    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, oracle.LAYER_CHAIN)
+    layer_of_hit = {}
+    for i, d in zip(ev.idx, ev.detId):
+        layer_of_hit[int(i)] = ((int(d) >> 20) & 7) - 1  # barrel-only workload: barrel layer bins
+    sg_layer = np.array([layer_of_hit[int(r[0])] for r in sets["SG"]])
+    n_sg = np.bincount(sg_layer, minlength=6)
+    assert counters["nTLCand"] == int(n_sg[0] * n_sg[2] + n_sg[1] * n_sg[3] + n_sg[2] * n_sg[4])
+    tl = sets["TL"]
+    tl_layer = np.array([layer_of_hit[int(r[0])] for r in tl])
+    assert counters["nTCCand"] == int((tl_layer == 0).sum() * (tl_layer == 2).sum())
+    inner = [tuple(r) for r, L in zip(tl, tl_layer) if L == 0]
+    outer = [tuple(r) for r, L in zip(tl, tl_layer) if L == 2]
+    by_first = {}
+    for o in outer:
+        by_first.setdefault(o[:4], []).append(o)
+    expect = sorted(i + o for i in inner for o in by_first.get(i[4:], []))
+    got = sorted(tuple(r) for r in sets["TCL"])
+    assert got == expect and len(got) > 0
