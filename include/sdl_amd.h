@@ -47,7 +47,12 @@ enum {
     /* layer-pair family (objects stored and exported as SG, TL and TCL) */
     SDL_STAGE_SG_LAYERS = 6,  /* createSegments         Event.cc:432-443, 793-821 */
     SDL_STAGE_TL_LAYERS = 7,  /* createTracklets        Event.cc:599-609, 755-790 */
-    SDL_STAGE_TC_LAYERS = 8   /* createTrackCandidates  Event.cc:823-872 */
+    SDL_STAGE_TC_LAYERS = 8,  /* createTrackCandidates  Event.cc:823-872 */
+    /* navigation: tracklets (inner SG, SG two hops out), stored as TL */
+    SDL_STAGE_TL_NAV = 9,     /* createTrackletsViaNavigation  Event.cc:696-751 */
+    /* pseudo mini-doublets, stored as MD: each anchor-side hit with the first
+     * hit of the partner sensor */
+    SDL_STAGE_MD_PSEUDO = 10  /* createPseudoMiniDoubletsFromAnchorModule  Event.cc:307-430 */
 };
 
 /* Last error message of the calling thread. */
@@ -102,9 +107,12 @@ int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offse
 
 /* Runs one stage on every event of the batch.  Each stage reads the objects of
  * the stage before it, so stages run in order (else SDL_ERR_STATE): the
- * module-map chain MD, SG, TP, TL, TCP, TCL; SG_LAYERS after MD; TL_LAYERS after
- * SG or SG_LAYERS; TC_LAYERS after TL or TL_LAYERS (the module-map joins TP,
- * TCP and TCL assume module-map inputs and do not follow a layer-pair stage).
+ * module-map chain MD (or MD_PSEUDO), SG, TP, TL, TCP, TCL; SG_LAYERS after MD
+ * or MD_PSEUDO; TL_LAYERS after
+ * SG or SG_LAYERS; TL_NAV after SG, SG_LAYERS or TP; TC_LAYERS after TL,
+ * TL_LAYERS or TL_NAV; TCL also after TL_NAV over module-map segments (the
+ * module-map joins TP, TCP and TCL assume module-map inputs and do not follow a
+ * layer-pair stage).
  * The layer-pair stages store their objects as the SG, TL and TCL objects of
  * the batch.  algo: 0 = Default, the only algorithm with device
  * kernels. */

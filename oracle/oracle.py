@@ -67,7 +67,8 @@ def run(paths, ev, libm=GLIBC, stages=6):
     """Returns ({stage: int32 array [n, width]}, counters dict).  stages = 1..6
     runs that many module-map stages; LAYER_CHAIN runs the layer-pair chain; a
     list runs those stages in that order (0-5 module-map, 6-8 createSegments,
-    createTracklets, createTrackCandidates)."""
+    createTracklets, createTrackCandidates, 9 createTrackletsViaNavigation,
+    10 createPseudoMiniDoubletsFromAnchorModule)."""
     L = lib()
     g = geometry(paths)
     if isinstance(stages, (list, tuple)):

@@ -18,6 +18,10 @@
 // Usage:
 //   sdl_ref_driver dump        <workload> <geomdir> <event0> <nevents> <outfile>
 //   sdl_ref_driver dumplayers  <workload> <geomdir> <event0> <nevents> <outfile>
+//   sdl_ref_driver dumpnav     <workload> <geomdir> <event0> <nevents> <outfile>
+//   sdl_ref_driver dumppseudo  <workload> <geomdir> <event0> <nevents> <outfile>
+//     (createPseudoMiniDoubletsFromAnchorModule, then the module-map chain)
+//     (createMiniDoublets, createSegmentsWithModuleMap, createTrackletsViaNavigation)
 //   sdl_ref_driver time        <workload> <geomdir> <event0> <nevents> <threads> [stages]
 // `stages` limits the chain to its first k stages (1..6, default 6).
 #include <chrono>
@@ -97,6 +101,37 @@ void runLayerChain(SDL::Event& ev) {
     ev.createTrackCandidates();
 }
 
+// Per-layer counters (barrel 1-6, endcap 1-5) of the ten counter kinds.
+void dumpLayerCounters(FILE* f, SDL::Event& sev) {
+    typedef unsigned (SDL::Event::*Getter)(unsigned);
+    const Getter barrel[10] = {&SDL::Event::getNumberOfMiniDoubletCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfSegmentCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfTripletCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfTrackletCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfTrackCandidateCandidatesByLayerBarrel,
+                               &SDL::Event::getNumberOfMiniDoubletsByLayerBarrel,
+                               &SDL::Event::getNumberOfSegmentsByLayerBarrel,
+                               &SDL::Event::getNumberOfTripletsByLayerBarrel,
+                               &SDL::Event::getNumberOfTrackletsByLayerBarrel,
+                               &SDL::Event::getNumberOfTrackCandidatesByLayerBarrel};
+    const Getter endcap[10] = {&SDL::Event::getNumberOfMiniDoubletCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfSegmentCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfTripletCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfTrackletCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfTrackCandidateCandidatesByLayerEndcap,
+                               &SDL::Event::getNumberOfMiniDoubletsByLayerEndcap,
+                               &SDL::Event::getNumberOfSegmentsByLayerEndcap,
+                               &SDL::Event::getNumberOfTripletsByLayerEndcap,
+                               &SDL::Event::getNumberOfTrackletsByLayerEndcap,
+                               &SDL::Event::getNumberOfTrackCandidatesByLayerEndcap};
+    std::fprintf(f, "L");
+    for (int k = 0; k < 10; ++k) {
+        for (unsigned i = 0; i < 6; ++i) std::fprintf(f, " %u", (sev.*barrel[k])(i));
+        for (unsigned i = 0; i < 5; ++i) std::fprintf(f, " %u", (sev.*endcap[k])(i));
+    }
+    std::fprintf(f, "\n");
+}
+
 // Layer-pair chain: MDs and SGs through the modules (addSegmentToEvent adds to
 // both, Event.cc:163-177), TLs and TCs through the layers (addTrackletToLowerLayer,
 // addTrackCandidateToLowerLayer, Event.cc:227-243), then the per-layer counters
@@ -125,33 +160,7 @@ void dumpLayerEvent(FILE* f, SDL::Event& sev, long ievent) {
                  sev.getNumberOfTrackCandidateCandidates());
     std::fprintf(f, "C %u %u %u %u %u\n", sev.getNumberOfMiniDoublets(), sev.getNumberOfSegments(),
                  sev.getNumberOfTriplets(), sev.getNumberOfTracklets(), sev.getNumberOfTrackCandidates());
-    typedef unsigned (SDL::Event::*Getter)(unsigned);
-    const Getter barrel[10] = {&SDL::Event::getNumberOfMiniDoubletCandidatesByLayerBarrel,
-                               &SDL::Event::getNumberOfSegmentCandidatesByLayerBarrel,
-                               &SDL::Event::getNumberOfTripletCandidatesByLayerBarrel,
-                               &SDL::Event::getNumberOfTrackletCandidatesByLayerBarrel,
-                               &SDL::Event::getNumberOfTrackCandidateCandidatesByLayerBarrel,
-                               &SDL::Event::getNumberOfMiniDoubletsByLayerBarrel,
-                               &SDL::Event::getNumberOfSegmentsByLayerBarrel,
-                               &SDL::Event::getNumberOfTripletsByLayerBarrel,
-                               &SDL::Event::getNumberOfTrackletsByLayerBarrel,
-                               &SDL::Event::getNumberOfTrackCandidatesByLayerBarrel};
-    const Getter endcap[10] = {&SDL::Event::getNumberOfMiniDoubletCandidatesByLayerEndcap,
-                               &SDL::Event::getNumberOfSegmentCandidatesByLayerEndcap,
-                               &SDL::Event::getNumberOfTripletCandidatesByLayerEndcap,
-                               &SDL::Event::getNumberOfTrackletCandidatesByLayerEndcap,
-                               &SDL::Event::getNumberOfTrackCandidateCandidatesByLayerEndcap,
-                               &SDL::Event::getNumberOfMiniDoubletsByLayerEndcap,
-                               &SDL::Event::getNumberOfSegmentsByLayerEndcap,
-                               &SDL::Event::getNumberOfTripletsByLayerEndcap,
-                               &SDL::Event::getNumberOfTrackletsByLayerEndcap,
-                               &SDL::Event::getNumberOfTrackCandidatesByLayerEndcap};
-    std::fprintf(f, "L");
-    for (int k = 0; k < 10; ++k) {
-        for (unsigned i = 0; i < 6; ++i) std::fprintf(f, " %u", (sev.*barrel[k])(i));
-        for (unsigned i = 0; i < 5; ++i) std::fprintf(f, " %u", (sev.*endcap[k])(i));
-    }
-    std::fprintf(f, "\n");
+    dumpLayerCounters(f, sev);
 }
 
 void dumpEvent(FILE* f, SDL::Event& sev, long ievent) {
@@ -204,6 +213,42 @@ int main(int argc, char** argv) {
     loadGeometry(workload, geomdir, g);
     sdlgen::EventConfig cfg = sdlgen::presetConfig(workload);
 
+    if (mode == "dumppseudo") {  // pseudo mini-doublets, then the module-map chain
+        FILE* f = std::fopen(argv[6], "w");
+        if (!f) return 3;
+        for (long ie = e0; ie < e0 + ne; ++ie) {
+            sdlgen::Event ev;
+            sdlgen::generateEvent(g, cfg, ie, ev);
+            SDL::Event sev;
+            fill(sev, ev);
+            sev.createPseudoMiniDoubletsFromAnchorModule();
+            sev.createSegmentsWithModuleMap();
+            sev.createTriplets();
+            sev.createTrackletsWithModuleMap();
+            sev.createTrackCandidatesFromTriplets();
+            sev.createTrackCandidatesFromTracklets();
+            dumpEvent(f, sev, ie);
+            dumpLayerCounters(f, sev);
+        }
+        std::fclose(f);
+        return 0;
+    }
+    if (mode == "dumpnav") {  // createMiniDoublets, createSegmentsWithModuleMap, createTrackletsViaNavigation
+        FILE* f = std::fopen(argv[6], "w");
+        if (!f) return 3;
+        for (long ie = e0; ie < e0 + ne; ++ie) {
+            sdlgen::Event ev;
+            sdlgen::generateEvent(g, cfg, ie, ev);
+            SDL::Event sev;
+            fill(sev, ev);
+            sev.createMiniDoublets();
+            sev.createSegmentsWithModuleMap();
+            sev.createTrackletsViaNavigation();
+            dumpLayerEvent(f, sev, ie);
+        }
+        std::fclose(f);
+        return 0;
+    }
     if (mode == "dumplayers") {
         FILE* f = std::fopen(argv[6], "w");
         if (!f) return 3;

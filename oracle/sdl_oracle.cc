@@ -958,6 +958,36 @@ struct Event {
                 }
         }
     }
+    // Event::createPseudoMiniDoubletsFromAnchorModule (Event.cc:307-430): the
+    // AllComb algorithm accepts every candidate (MiniDoublet.cc:223-226) and the
+    // loops stop at the first, so each hit on the anchor side (the lower sensor,
+    // or the upper one when the lower is the strip sensor of a PS module) pairs
+    // with the first hit of the other sensor; the cached values stay 0
+    // (MiniDoublet.cc:41-51).
+    void createPseudoMiniDoublets() {
+        std::vector<int> order = lowerOrder;
+        for (int ls : order) {
+            const ModInfo& m = slots[ls].info;
+            const int us = getSlot(m.partner);
+            const bool loopLower = !(m.type == TYPE_PS && m.layerType != LT_PIXEL);
+            const std::vector<int> loHits = slots[ls].hits;
+            const std::vector<int> upHits = slots[us].hits;
+            const std::vector<int>& outer = loopLower ? loHits : upHits;
+            const std::vector<int>& inner = loopLower ? upHits : loHits;
+            for (int h : outer) {
+                if (inner.empty()) break;
+                nCand[0]++;
+                MDRec md;
+                md.lo = loopLower ? h : inner[0];
+                md.up = loopLower ? inner[0] : h;
+                md.slot = ls;
+                md.dz = md.shiftedDz = md.dphi = md.dphichange = 0;
+                md.anchor = loopLower ? md.lo : md.up;  // MiniDoublet::setAnchorHit (MiniDoublet.cc:60-81)
+                slots[ls].mds.push_back(int(mds.size()));
+                mds.push_back(md);
+            }
+        }
+    }
     // Event::createSegmentsFromInnerLowerModule (Event.cc:457-522)
     void createSegments() {
         for (int ls : lowerOrder) {
@@ -1131,6 +1161,27 @@ struct Event {
                 if (sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo}});
             }
     }
+    // Event::createTrackletsViaNavigation (Event.cc:696-751): inner SG s1, every
+    // SG s2 starting at s1's outer MD, every SG s3 starting at s2's outer MD
+    // (the MDs' outward segment lists, filled in creation order by
+    // Segment::addSelfPtrToMiniDoublets); the tracklet is (s1, s3), counted and
+    // stored at s1's inner lower module.
+    void createTrackletsNavigation() {
+        std::vector<std::vector<int>> outward(mds.size());
+        for (size_t k = 0; k < sgs.size(); ++k) outward[sgs[k].mdIn].push_back(int(k));
+        for (int ls : lowerOrder)
+            for (int si : slots[ls].sgs)
+                for (int s2 : outward[sgs[si].mdOut])
+                    for (int s3 : outward[sgs[s2].mdOut]) {
+                        nCand[3]++;
+                        if (tlAlgo(sgs[si], sgs[s3], nullptr)) {
+                            slots[ls].tls.push_back(int(tls.size()));
+                            tls.push_back({si, s3});
+                            tlLayer.push_back(layerOfMD(sgs[si].mdIn));
+                        }
+                    }
+    }
+
     void runLayers() {
         createMiniDoublets();
         createSegmentsLayers();
@@ -1138,7 +1189,8 @@ struct Event {
         createTrackCandidatesLayers();
     }
     // Stages in the given order: 0-5 the module-map stages, 6-8 createSegments,
-    // createTracklets, createTrackCandidates.
+    // createTracklets, createTrackCandidates, 9 createTrackletsViaNavigation,
+    // 10 createPseudoMiniDoubletsFromAnchorModule.
     void runSequence(const int* seq, int n) {
         for (int k = 0; k < n; ++k) {
             switch (seq[k]) {
@@ -1151,6 +1203,8 @@ struct Event {
                 case 6: createSegmentsLayers(); break;
                 case 7: createTrackletsLayers(); break;
                 case 8: createTrackCandidatesLayers(); break;
+                case 9: createTrackletsNavigation(); break;
+                case 10: createPseudoMiniDoublets(); break;
                 default: break;
             }
         }

@@ -48,7 +48,9 @@ enum { ORACLE_LAYER_CHAIN = 16 };
 void* oracle_run(void* geom, int nhits, const float* x, const float* y, const float* z, const unsigned* detId,
                  const int* idx, int libm, int stages);
 // Runs the stages seq[0..n-1] in that order (0-5 module-map stages, 6-8
-// createSegments, createTracklets, createTrackCandidates); results as oracle_run.
+// createSegments, createTracklets, createTrackCandidates, 9
+// createTrackletsViaNavigation, 10 createPseudoMiniDoubletsFromAnchorModule);
+// results as oracle_run.
 void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y, const float* z,
                           const unsigned* detId, const int* idx, int libm, const int* seq, int n);
 // Tuple width: MD 2, SG 4, TP 8, TL 8, TCP 16, TCL 16 hit indices.

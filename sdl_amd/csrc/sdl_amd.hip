@@ -581,6 +581,27 @@ __global__ void k_md_cap(int NS, int NM, const ModDev* mods, const long long* sl
     mdCap[s] = cap;
 }
 
+// Pseudo mini-doublets (Event.cc:307-430): every hit on the anchor side (the
+// lower sensor, or the upper one when the lower is the strip sensor of a PS
+// module) paired with the first hit of the other sensor; one candidate each,
+// all accepted (AllComb, MiniDoublet.cc:223-226).
+__device__ __forceinline__ bool pseudoLoopsLower(const ModDev& md) { return !(md.type == TYPE_PS && md.layerType != LT_PIXEL); }
+
+__global__ void k_pmd_cap(int NS, int NM, const ModDev* mods, const long long* slotHitStart, int* mdCap) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= NS) return;
+    const int m = s % NM;
+    const ModDev md = mods[m];
+    int cap = 0;
+    if (md.isLower && md.partner >= 0) {
+        const int ps = s - m + md.partner;
+        const long long nLo = slotHitStart[s + 1] - slotHitStart[s];
+        const long long nUp = slotHitStart[ps + 1] - slotHitStart[ps];
+        cap = int(pseudoLoopsLower(md) ? (nUp > 0 ? nLo : 0) : (nLo > 0 ? nUp : 0));
+    }
+    mdCap[s] = cap;
+}
+
 struct HitsDev {
     const float *x, *y, *z, *rt, *r3, *phi, *alpha;
     const int* idx;
@@ -619,6 +640,30 @@ k_md(Count cCand, int NS, int NM, const ModDev* mods, HitsDev H, const long long
         slabSlot[j] = item;
         atomicAdd(&mdCount[item], 1);
     }
+}
+
+// One thread per pseudo MD, written in the slab format of k_md with the
+// cached values 0 (MiniDoublet.cc:41-51).
+__global__ void k_pmd(Count cCand, int NS, int NM, const ModDev* mods, const long long* slotHitStart,
+                      const int* sortedHit, const long long* mdCapOff, int2* slabHits, float4* slabF, int* slabSlot,
+                      unsigned char* flag, int* mdCount, Guard g) {
+    if (g.over()) return;
+    const long long nCand = cCand.get();
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nCand) return;
+    const int item = int(rangeOf(mdCapOff, NS, j));
+    const int m = item % NM;
+    const ModDev md = mods[m];
+    const int ps = item - m + md.partner;
+    const int k = int(j - mdCapOff[item]);
+    const bool loopLower = pseudoLoopsLower(md);
+    const int lo = sortedHit[slotHitStart[item] + (loopLower ? k : 0)];
+    const int up = sortedHit[slotHitStart[ps] + (loopLower ? 0 : k)];
+    flag[j] = 1;
+    slabHits[j] = make_int2(lo, up);
+    slabF[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    slabSlot[j] = item;
+    atomicAdd(&mdCount[item], 1);
 }
 
 struct MDDev {
@@ -1503,6 +1548,69 @@ __global__ void k_tll_repack(Count cSG, const long long* capOff, const long long
     }
 }
 
+// Tracklets via navigation (Event.cc:696-751): inner SG s1, every SG s2 that
+// starts at s1's outer MD, every SG s3 that starts at s2's outer MD; the
+// candidate is (s1, s3).  Per s1: the number of candidates (also its counter).
+__global__ void k_tln_cap(Count cSG, const int* sgOut, const long long* sgStart, long long* tlCap, int* capInt) {
+    const int nSG = int(cSG.get());
+    const int s1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s1 >= nSG) return;
+    const int m1 = sgOut[s1];
+    long long c = 0;
+    for (long long s2 = sgStart[m1]; s2 < sgStart[m1 + 1]; ++s2) {
+        const int m2 = sgOut[s2];
+        c += sgStart[m2 + 1] - sgStart[m2];
+    }
+    tlCap[s1] = c;
+    capInt[s1] = int(c);
+}
+
+// One wave per inner SG: the lanes take the candidates of each s2 in turn; both
+// tracklet stages with the variant of each candidate's anchors.
+__global__ void __launch_bounds__(kBlock) k_tln(Count cSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
+                                                const long long* sgStart, const int* capInt, const long long* capOff,
+                                                int* slab, int* tlCount, Guard g) {
+    if (g.over()) return;
+    const int nSG = int(cSG.get());
+    const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= nSG) return;
+    if (capInt[item] == 0) {
+        if (lane == 0) tlCount[item] = 0;
+        return;
+    }
+    const TLInner I = loadInner(item, D, S);
+    const int fl = S.flags[item];
+    const int m1 = S.out[item];
+    const long long off = capOff[item];
+    int base = 0;
+    for (long long s2 = sgStart[m1]; s2 < sgStart[m1 + 1]; ++s2) {
+        const int m2 = S.out[s2];
+        const long long a = sgStart[m2], b = sgStart[m2 + 1];
+        for (long long k0 = a; k0 < b; k0 += kWave) {
+            const long long s3 = k0 + lane;
+            bool pass = false;
+            if (s3 < b) {
+                const int flo = S.flags[s3];
+                const int mLo = S.in[s3];
+                const int V = tlModuleVariant(fl, mods[D.slot[mLo] % NM]);
+                if (V < kNVariantsTL) {
+                    const TLOutLo Lo = loadOutLo(mLo, flo, D);
+                    TLCarry c{0.f, 0.f, 0.f};
+                    if (tlStage1(V, I, Lo, c)) {
+                        const TLOutUp U = loadOutUp(int(s3), flo, H, D, S);
+                        pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c);
+                    }
+                }
+            }
+            const unsigned long long mask = __ballot(pass);
+            if (pass) slab[off + base + __popcll(mask & ((1ull << lane) - 1ull))] = int(s3);
+            base += __popcll(mask);
+        }
+    }
+    if (lane == 0) tlCount[item] = base;
+}
+
 // TLs per (event, layer bin of the inner SG): the layer lists of tracklets.
 __global__ void k_tl_per_layer(Count cTL, int NM, const ModDev* mods, const int* tlIn, const int* sgIn,
                                const int* mdSlot, int* perLayer) {
@@ -2191,7 +2299,10 @@ int runPrep(sdl_event* ev) {
     return 0;
 }
 
-int runMD(sdl_event* ev) {
+// Mini-doublets: the default algorithm over every (lower hit, upper hit) pair
+// (createMiniDoublets), or with pseudo = true the pseudo mini-doublets
+// (createPseudoMiniDoubletsFromAnchorModule); stored alike.
+int runMD(sdl_event* ev, bool pseudo) {
     sdl_geom* G = ev->g;
     hipStream_t st = ev->st;
     const int NS = ev->E * G->NM;
@@ -2199,8 +2310,12 @@ int runMD(sdl_event* ev) {
     if (int r = ev->mdCount.ensure(NS)) return r;
     if (int r = ev->mdCapOff.ensure(NS + 1)) return r;
     if (int r = ev->mdStart.ensure(NS + 1)) return r;
-    hipLaunchKernelGGL(k_md_cap, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods,
-                       ev->slotHitStart.p, ev->mdCap.p);
+    if (pseudo)
+        hipLaunchKernelGGL(k_pmd_cap, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods,
+                           ev->slotHitStart.p, ev->mdCap.p);
+    else
+        hipLaunchKernelGGL(k_md_cap, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods,
+                           ev->slotHitStart.p, ev->mdCap.p);
     if (int r = scanExclusive<int>(ev->mdCap.p, ev->mdCapOff.p, hostCount(ev, NS), NS, ev->scan, st, ev->tot.p,
                                    TOT_MDCAP))
         return r;
@@ -2214,7 +2329,12 @@ int runMD(sdl_event* ev) {
     HitsDev H = hitsView(ev);
     zeroAsync(ev->mdCount.p, NS, st);
     timerBegin(ev, T_MD);
-    if (slab > 0)
+    if (slab > 0 && pseudo)
+        hipLaunchKernelGGL(k_pmd, dim3(gridFor(slab, kBlock)), dim3(kBlock), 0, st, devCount(ev, TOT_MDCAP), NS, G->NM,
+                           G->dMods, ev->slotHitStart.p, ev->sortedHit.p, ev->mdCapOff.p, ev->mdSlabHits.p,
+                           ev->mdSlabF.p, ev->mdSlabSlot.p, ev->mdFlag.p, ev->mdCount.p,
+                           guardOf(ev, TOT_MDCAP, slab, SDL_STAGE_MD));
+    if (slab > 0 && !pseudo)
         hipLaunchKernelGGL(k_md, dim3(gridFor(slab, ev->tune.v[TUNE_MD_BLOCK])), dim3(ev->tune.v[TUNE_MD_BLOCK]), 0, st,
                            devCount(ev, TOT_MDCAP), NS, G->NM,
                            G->dMods, H, ev->slotHitStart.p, ev->sortedHit.p, ev->mdCapOff.p, ev->mdSlabHits.p,
@@ -2626,6 +2746,45 @@ int runTLLayers(sdl_event* ev) {
     return 0;
 }
 
+int runTLNavigation(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const int NS = ev->E * G->NM;
+    const long long nSG = ev->nSG;
+    const long long ns = std::max(nSG, 1LL);
+    if (int r = ev->tlCap.ensure(ns)) return r;
+    if (int r = ev->tlCapInt.ensure(ns)) return r;
+    if (int r = ev->tlCapOff.ensure(ns + 1)) return r;
+    if (int r = ev->tlCount.ensure(ns)) return r;
+    if (int r = ev->tlStart.ensure(ns + 1)) return r;
+    if (int r = ev->slotTlStart.ensure(NS + 1)) return r;
+    const Count cSG = devCount(ev, TOT_SG);
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tln_cap, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p, ev->sgStart.p,
+                           ev->tlCap.p, ev->tlCapInt.p);
+    if (int r = scanExclusive<int>(ev->tlCapInt.p, ev->tlCapOff.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TLCANDS))
+        return r;
+    const long long slab = sizeOf(ev, TOT_TLCANDS);
+    if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
+    if (int r = ev->tllSlab.ensure(std::max(slab, 1LL))) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tln, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
+                           hitsView(ev), mdView(ev), sgView(ev), ev->sgStart.p, ev->tlCapInt.p, ev->tlCapOff.p,
+                           ev->tllSlab.p, ev->tlCount.p, guardOf(ev, TOT_TLCANDS, slab, SDL_STAGE_TL));
+    if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
+    const long long nTL = sizeOf(ev, TOT_TL);
+    if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nTL = nTL;
+    if (int r = ev->tlIn.ensure(std::max(nTL, 1LL))) return r;
+    if (int r = ev->tlOut.ensure(std::max(nTL, 1LL))) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_tll_repack, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->tlCapOff.p,
+                           ev->tlStart.p, ev->tllSlab.p, ev->tlIn.p, ev->tlOut.p, guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
+    hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+                       ev->slotSgStart.p, ev->tlStart.p, ev->slotTlStart.p);
+    return 0;
+}
+
 int runTCLayers(sdl_event* ev) {
     sdl_geom* G = ev->g;
     hipStream_t st = ev->st;
@@ -2663,7 +2822,8 @@ int runTCLayers(sdl_event* ev) {
 
 int runStage(sdl_event* ev, int stage) {
     switch (stage) {
-        case SDL_STAGE_MD: return runMD(ev);
+        case SDL_STAGE_MD: return runMD(ev, false);
+        case SDL_STAGE_MD_PSEUDO: return runMD(ev, true);
         case SDL_STAGE_SG: return runSG(ev, false);
         case SDL_STAGE_TP: return runTP(ev);
         case SDL_STAGE_TL: return runTL(ev);
@@ -2672,6 +2832,7 @@ int runStage(sdl_event* ev, int stage) {
         case SDL_STAGE_SG_LAYERS: return runSG(ev, true);
         case SDL_STAGE_TL_LAYERS: return runTLLayers(ev);
         case SDL_STAGE_TC_LAYERS: return runTCLayers(ev);
+        case SDL_STAGE_TL_NAV: return runTLNavigation(ev);
         default: return fail(SDL_ERR_ARG, "bad stage");
     }
 }
@@ -2687,7 +2848,11 @@ void launchCounters(sdl_event* ev, int s0, int s1) {
     C.tcCand = nullptr;
     for (int s = s0; s <= s1; ++s) {
         if (s == SDL_STAGE_SG_LAYERS) continue;  // createSegmentsFromTwoLayers counts nothing (Event.cc:793-821)
-        if (s == SDL_STAGE_TL_LAYERS) {           // counted on the inner layer (Event.cc:778-786)
+        if (s == SDL_STAGE_MD_PSEUDO) {  // counted per lower module (Event.cc:354-364)
+            C.stages |= 1 << SDL_STAGE_MD;
+            continue;
+        }
+        if (s == SDL_STAGE_TL_LAYERS || s == SDL_STAGE_TL_NAV) {  // Event.cc:778-786, 720-734
             C.stages |= 1 << SDL_STAGE_TL;
         } else if (s == SDL_STAGE_TC_LAYERS) {    // counted on the inner layer (Event.cc:859-867)
             C.stages |= 1 << SDL_STAGE_TCL;
@@ -3175,16 +3340,23 @@ int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offse
 int sdl_event_run(sdl_event* ev, int stage, int algo) {
     if (!ev) return fail(SDL_ERR_ARG, "null event");
     if (algo != 0) return fail(SDL_ERR_ARG, "only the default algorithm has device kernels");
-    if (stage < SDL_STAGE_MD || stage > SDL_STAGE_TC_LAYERS) return fail(SDL_ERR_ARG, "bad stage");
+    if (stage < SDL_STAGE_MD || stage > SDL_STAGE_MD_PSEUDO) return fail(SDL_ERR_ARG, "bad stage");
     // each stage reads the objects of the stage before it: module-map chain MD,
     // SG, TP, TL, TCP, TCL; layer-pair stages after MD (segments), after either
     // segment stage (tracklets) and after either tracklet stage (track
     // candidates).  The module-map joins (TP, TCP, TCL) assume module-map
     // inputs, so they do not follow a layer-pair stage.
-    bool ok = stage == ev->stage + 1 && stage <= SDL_STAGE_TCL;
-    if (stage == SDL_STAGE_SG_LAYERS) ok = ev->stage == SDL_STAGE_MD;
+    const int prev = ev->stage == SDL_STAGE_MD_PSEUDO ? SDL_STAGE_MD : ev->stage;  // both MD forms are alike
+    bool ok = stage == prev + 1 && stage <= SDL_STAGE_TCL;
+    if (stage == SDL_STAGE_MD_PSEUDO) ok = ev->stage == -1;
+    if (stage == SDL_STAGE_SG_LAYERS) ok = prev == SDL_STAGE_MD;
     if (stage == SDL_STAGE_TL_LAYERS) ok = ev->stage == SDL_STAGE_SG || ev->stage == SDL_STAGE_SG_LAYERS;
-    if (stage == SDL_STAGE_TC_LAYERS) ok = ev->stage == SDL_STAGE_TL || ev->stage == SDL_STAGE_TL_LAYERS;
+    if (stage == SDL_STAGE_TL_NAV)
+        ok = ev->stage == SDL_STAGE_SG || ev->stage == SDL_STAGE_SG_LAYERS || ev->stage == SDL_STAGE_TP;
+    if (stage == SDL_STAGE_TC_LAYERS)
+        ok = ev->stage == SDL_STAGE_TL || ev->stage == SDL_STAGE_TL_LAYERS || ev->stage == SDL_STAGE_TL_NAV;
+    // navigation tracklets over module-map segments keep the TCL join exact
+    if (stage == SDL_STAGE_TCL && ev->stage == SDL_STAGE_TL_NAV) ok = !(ev->ranMask & (1u << SDL_STAGE_SG_LAYERS));
     if (!ok) return fail(SDL_ERR_STATE, "stages must run in order");
     if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
     if (!ev->hitsReady) {
@@ -3242,23 +3414,33 @@ int sdl_event_counts(sdl_event* ev, int e, unsigned long long out[10]) {
     // Object totals are the sizes of the object lists (Event.cc:1080-1092), not
     // sums of the per-layer counters, which some stages leave untouched.
     const unsigned m = ev->ranMask;
-    const bool ran[5] = {bool(m & (1u << SDL_STAGE_MD)),
+    const bool ran[5] = {bool(m & ((1u << SDL_STAGE_MD) | (1u << SDL_STAGE_MD_PSEUDO))),
                          bool(m & ((1u << SDL_STAGE_SG) | (1u << SDL_STAGE_SG_LAYERS))),
                          bool(m & (1u << SDL_STAGE_TP)),
-                         bool(m & ((1u << SDL_STAGE_TL) | (1u << SDL_STAGE_TL_LAYERS))),
+                         bool(m & ((1u << SDL_STAGE_TL) | (1u << SDL_STAGE_TL_LAYERS) | (1u << SDL_STAGE_TL_NAV))),
                          bool(m & ((1u << SDL_STAGE_TCP) | (1u << SDL_STAGE_TCL) | (1u << SDL_STAGE_TC_LAYERS)))};
     const int stages[5] = {SDL_STAGE_MD, SDL_STAGE_SG, SDL_STAGE_TP, SDL_STAGE_TL, SDL_STAGE_TCP};
     for (int k = 0; k < 5; ++k) {
         if (!ran[k]) continue;
         if (e < 0) {
-            const long long tot[5] = {ev->nMD, ev->nSG, ev->nTP, ev->nTL, ev->nTCP + ev->nTCL};
+            const long long tcp = (m & (1u << SDL_STAGE_TCP)) ? ev->nTCP : 0;
+            const long long tcl = (m & ((1u << SDL_STAGE_TCL) | (1u << SDL_STAGE_TC_LAYERS))) ? ev->nTCL : 0;
+            const long long tot[5] = {ev->nMD, ev->nSG, ev->nTP, ev->nTL, tcp + tcl};
             out[5 + k] = (unsigned long long)tot[k];
             continue;
         }
         long long a, n;
-        if (int r = stageRange(ev, e, stages[k], a, n)) return r;
-        out[5 + k] = (unsigned long long)n;
-        if (k == 4) {
+        if (k < 4) {
+            if (int r = stageRange(ev, e, stages[k], a, n)) return r;
+            out[5 + k] = (unsigned long long)n;
+            continue;
+        }
+        // TCs: from triplets and from tracklets, each only if its stage ran
+        if (m & (1u << SDL_STAGE_TCP)) {
+            if (int r = stageRange(ev, e, SDL_STAGE_TCP, a, n)) return r;
+            out[9] += (unsigned long long)n;
+        }
+        if (m & ((1u << SDL_STAGE_TCL) | (1u << SDL_STAGE_TC_LAYERS))) {
             if (int r = stageRange(ev, e, SDL_STAGE_TCL, a, n)) return r;
             out[9] += (unsigned long long)n;
         }
@@ -3270,8 +3452,9 @@ int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, l
     if (!ev || !n || e < 0 || e >= ev->E || stage < 0 || stage > SDL_STAGE_TCL) return fail(SDL_ERR_ARG, "bad arguments");
     // the layer-pair stages store SG, TL and TCL objects
     unsigned need = 1u << stage;
+    if (stage == SDL_STAGE_MD) need |= 1u << SDL_STAGE_MD_PSEUDO;
     if (stage == SDL_STAGE_SG) need |= 1u << SDL_STAGE_SG_LAYERS;
-    if (stage == SDL_STAGE_TL) need |= 1u << SDL_STAGE_TL_LAYERS;
+    if (stage == SDL_STAGE_TL) need |= (1u << SDL_STAGE_TL_LAYERS) | (1u << SDL_STAGE_TL_NAV);
     if (stage == SDL_STAGE_TCL) need |= 1u << SDL_STAGE_TC_LAYERS;
     if (!(ev->ranMask & need)) return fail(SDL_ERR_STATE, "stage has not run");
     long long a = 0, cnt = 0;

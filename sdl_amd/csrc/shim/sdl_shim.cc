@@ -95,6 +95,10 @@ bool Event::run(int stage, int algo, const char* what) {
 }
 
 void Event::createMiniDoublets(MDAlgo algo) { run(SDL_STAGE_MD, algo, "createMiniDoublets"); }
+// The reference runs the AllComb algorithm here whatever algo says (Event.cc:353).
+void Event::createPseudoMiniDoubletsFromAnchorModule(MDAlgo) {
+    run(SDL_STAGE_MD_PSEUDO, 0, "createPseudoMiniDoubletsFromAnchorModule");
+}
 void Event::createSegmentsWithModuleMap(SGAlgo algo) { run(SDL_STAGE_SG, algo, "createSegmentsWithModuleMap"); }
 void Event::createTriplets(TPAlgo algo) { run(SDL_STAGE_TP, algo, "createTriplets"); }
 void Event::createTrackletsWithModuleMap(TLAlgo algo) { run(SDL_STAGE_TL, algo, "createTrackletsWithModuleMap"); }
@@ -103,6 +107,7 @@ void Event::createTrackCandidatesFromTracklets(TCAlgo algo) { run(SDL_STAGE_TCL,
 void Event::createSegments(SGAlgo algo) { run(SDL_STAGE_SG_LAYERS, algo, "createSegments"); }
 void Event::createTracklets(TLAlgo algo) { run(SDL_STAGE_TL_LAYERS, algo, "createTracklets"); }
 void Event::createTrackCandidates(TCAlgo algo) { run(SDL_STAGE_TC_LAYERS, algo, "createTrackCandidates"); }
+void Event::createTrackletsViaNavigation(TLAlgo algo) { run(SDL_STAGE_TL_NAV, algo, "createTrackletsViaNavigation"); }
 
 // Per-layer counters from the layer histogram; totals as Event.cc:1080-1112 define
 // them (candidates: sums over the layers; objects: sizes of the object lists).
@@ -159,8 +164,9 @@ std::vector<std::array<int, W>> Event::objects(int stage) {
     std::vector<std::array<int, W>> out;
     // the layer-pair stages store their objects as SG, TL and TCL objects
     unsigned need = 1u << stage;
+    if (stage == SDL_STAGE_MD) need |= 1u << SDL_STAGE_MD_PSEUDO;
     if (stage == SDL_STAGE_SG) need |= 1u << SDL_STAGE_SG_LAYERS;
-    if (stage == SDL_STAGE_TL) need |= 1u << SDL_STAGE_TL_LAYERS;
+    if (stage == SDL_STAGE_TL) need |= (1u << SDL_STAGE_TL_LAYERS) | (1u << SDL_STAGE_TL_NAV);
     if (stage == SDL_STAGE_TCL) need |= 1u << SDL_STAGE_TC_LAYERS;
     if (!(ran_ & need)) return out;
     long long n = 0;

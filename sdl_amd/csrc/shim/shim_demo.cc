@@ -1,8 +1,9 @@
 // A host program written against the SDL::Event API (as a user of the reference
 // would), running one generated event through the MI355X library.
-// Usage: sdl_shim_demo <workload> <event> <geomdir> [layers]
-// Runs the module-map chain, or with "layers" the layer-pair chain
-// (createMiniDoublets, createSegments, createTracklets, createTrackCandidates);
+// Usage: sdl_shim_demo <workload> <event> <geomdir> [layers|pseudo]
+// Runs the module-map chain, with "layers" the layer-pair chain
+// (createMiniDoublets, createSegments, createTracklets, createTrackCandidates),
+// with "pseudo" the module-map chain on pseudo mini-doublets;
 // prints the counters and a checksum of every object set as one line each.
 #include <cstdio>
 #include <cstdlib>
@@ -41,7 +42,9 @@ int main(int argc, char** argv) {
     SDL::Event event;
     for (size_t i = 0; i < ev.x.size(); ++i) event.addHitToModule(SDL::Hit(ev.x[i], ev.y[i], ev.z[i], ev.idx[i]), ev.detId[i]);
     const bool layers = argc > 4 && std::string(argv[4]) == "layers";
-    event.createMiniDoublets();
+    const bool pseudo = argc > 4 && std::string(argv[4]) == "pseudo";
+    if (pseudo) event.createPseudoMiniDoubletsFromAnchorModule();
+    else event.createMiniDoublets();
     if (layers) {
         event.createSegments();
         event.createTracklets();

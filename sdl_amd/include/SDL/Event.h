@@ -83,6 +83,9 @@ class Event {
 
     // Event.h:240-300, module-map family
     void createMiniDoublets(MDAlgo algo = Default_MDAlgo);
+    // Event.cc:307-430: one pseudo mini-doublet per anchor-side hit (with the
+    // partner sensor's first hit), instead of createMiniDoublets.
+    void createPseudoMiniDoubletsFromAnchorModule(MDAlgo algo = Default_MDAlgo);
     void createSegmentsWithModuleMap(SGAlgo algo = Default_SGAlgo);
     void createTriplets(TPAlgo algo = Default_TPAlgo);
     void createTrackletsWithModuleMap(TLAlgo algo = Default_TLAlgo);
@@ -98,6 +101,9 @@ class Event {
     void createSegments(SGAlgo algo = Default_SGAlgo);
     void createTracklets(TLAlgo algo = Default_TLAlgo);
     void createTrackCandidates(TCAlgo algo = Default_TCAlgo);
+    // Event.cc:696-751: tracklets (inner SG, SG two segments further out along the
+    // MDs' outward segment lists); returned by getTracklets.
+    void createTrackletsViaNavigation(TLAlgo algo = Default_TLAlgo);
 
     // Event.h:302-405
     unsigned int getNumberOfHits();

@@ -17,6 +17,8 @@ STAGES = ("MD", "SG", "TP", "TL", "TCP", "TCL")
 # layer-pair stages (createSegments, createTracklets, createTrackCandidates); their
 # objects are exported as SG, TL and TCL
 SG_LAYERS, TL_LAYERS, TC_LAYERS = 6, 7, 8
+TL_NAV = 9  # createTrackletsViaNavigation; its objects are exported as TL
+MD_PSEUDO = 10  # createPseudoMiniDoubletsFromAnchorModule; exported as MD
 WIDTH = {"MD": 2, "SG": 4, "TP": 8, "TL": 8, "TCP": 16, "TCL": 16}
 COUNTER_NAMES = ("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")
 

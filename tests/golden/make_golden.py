@@ -7,9 +7,12 @@ chain on seeded generated events and stores every object set as hit-index
 tuples plus the stage counters in ``tests/golden/ref_sets.npz``; and the
 layer-pair chain (createMiniDoublets, createSegments, createTracklets,
 createTrackCandidates) with the per-layer counters in
-``tests/golden/ref_layer_sets.npz``.
+``tests/golden/ref_layer_sets.npz``; and createMiniDoublets,
+createSegmentsWithModuleMap, createTrackletsViaNavigation in
+``tests/golden/ref_nav_sets.npz``; and createPseudoMiniDoubletsFromAnchorModule
+followed by the module-map chain in ``tests/golden/ref_pseudo_sets.npz``.
 
-Usage:  python tests/golden/make_golden.py [layers]   (layers: only the layer-pair fixtures)
+Usage:  python tests/golden/make_golden.py [layers|nav|pseudo]   (only those fixtures)
 """
 import os
 import subprocess
@@ -26,14 +29,22 @@ import oracle  # noqa: E402
 # (workload, first event, number of events)
 CASES = [(1, 0, 2), (12, 0, 2), (15, 0, 1)]
 LAYER_CASES = [(1, 0, 2), (12, 0, 1)]
+NAV_CASES = [(1, 0, 2), (12, 0, 2), (15, 0, 1)]
+PSEUDO_CASES = [(1, 0, 2), (12, 0, 1), (15, 0, 1)]
 COUNTERS = ("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")
 
 
 def main():
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    if len(sys.argv) < 2 or sys.argv[1] != "layers":
+    which = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if which == "all":
         module_map_fixtures()
-    layer_fixtures()
+    if which in ("all", "layers"):
+        layer_fixtures()
+    if which in ("all", "nav"):
+        nav_fixtures()
+    if which in ("all", "pseudo"):
+        pseudo_fixtures()
 
 
 def module_map_fixtures():
@@ -61,6 +72,34 @@ def layer_fixtures():
                 out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
                 out[f"w{wl}_e{e}_layers"] = counters["layers"]
     np.savez_compressed(os.path.join(HERE, "ref_layer_sets.npz"), **out)
+
+
+def nav_fixtures():
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for wl, e0, ne in NAV_CASES:
+            dump = os.path.join(tmp, f"w{wl}.txt")
+            subprocess.run([oracle.REF_DRIVER, "dumpnav", str(wl), tmp, str(e0), str(ne), dump], check=True)
+            for e, (sets, counters) in oracle.parse_ref_dump(dump).items():
+                for s in ("MD", "SG", "TL"):
+                    out[f"w{wl}_e{e}_{s}"] = oracle.as_multiset(sets[s])
+                out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
+                out[f"w{wl}_e{e}_layers"] = counters["layers"]
+    np.savez_compressed(os.path.join(HERE, "ref_nav_sets.npz"), **out)
+
+
+def pseudo_fixtures():
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for wl, e0, ne in PSEUDO_CASES:
+            dump = os.path.join(tmp, f"w{wl}.txt")
+            subprocess.run([oracle.REF_DRIVER, "dumppseudo", str(wl), tmp, str(e0), str(ne), dump], check=True)
+            for e, (sets, counters) in oracle.parse_ref_dump(dump).items():
+                for s in oracle.STAGES:
+                    out[f"w{wl}_e{e}_{s}"] = oracle.as_multiset(sets[s])
+                out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
+                out[f"w{wl}_e{e}_layers"] = counters["layers"]
+    np.savez_compressed(os.path.join(HERE, "ref_pseudo_sets.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -37,7 +37,7 @@ def _run(wl, events, seq):
 def _exported(ev, seq, e=0):
     have = {"MD"}
     for s in seq:
-        have.add({1: "SG", 2: "TP", 3: "TL", 4: "TCP", 5: "TCL", 6: "SG", 7: "TL", 8: "TCL"}.get(s, "MD"))
+        have.add({1: "SG", 2: "TP", 3: "TL", 4: "TCP", 5: "TCL", 6: "SG", 7: "TL", 8: "TCL", 9: "TL"}.get(s, "MD"))
     return {s: ev.export(i, e) if s in have else np.zeros((0, oracle.WIDTH[s]), np.int32)
             for i, s in enumerate(oracle.STAGES)}
 
@@ -118,3 +118,70 @@ def test_layer_stage_order_rules():
         ev.export(2)  # no triplets ran
     ev.run(runtime.TC_LAYERS)
     assert len(ev.export(5)) == ev.counts(0)["nTC"]
+
+
+@pytest.mark.parametrize("seq", [[0, 1, 9], [0, 1, 2, 9, 5], [0, 6, 9, 8], [0, 1, 9, 8]])
+def test_navigation_matches_oracle(seq):
+    """createTrackletsViaNavigation over module-map or layer-pair segments, and
+    the track candidates that may follow it, against the oracle."""
+    print(f"case navigation stages {seq}", flush=True)
+    for wl, e in ((12, 0), (1, 1)):
+        ev_in = gen.event(wl, e)
+        ev = _run(wl, [ev_in], seq)
+        ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], ev_in, oracle.CR, seq)
+        have = {"MD", "SG", "TL"} | ({"TP"} if 2 in seq else set()) | ({"TCL"} if (5 in seq or 8 in seq) else set())
+        got = {s: ev.export(i, 0) if s in have else np.zeros((0, oracle.WIDTH[s]), np.int32)
+               for i, s in enumerate(oracle.STAGES)}
+        _assert_same(got, ref, f"stages {seq} w{wl} e{e}")
+        assert ev.counts(0) == cnt
+
+
+def test_navigation_counters_match_reference_golden():
+    print("case navigation counters vs reference", flush=True)
+    path = os.path.join(os.path.dirname(__file__), "golden", "ref_nav_sets.npz")
+    with np.load(path) as f:
+        golden = {k: f[k] for k in f.files}
+    for wl, e in ((1, 0), (12, 1), (15, 0)):
+        ev = _run(wl, [gen.event(wl, e)], [0, 1, runtime.TL_NAV])
+        assert np.array_equal(ev.layer_counts(0), golden[f"w{wl}_e{e}_layers"]), (wl, e)
+
+
+def test_navigation_order_rules():
+    print("case navigation order", flush=True)
+    e = gen.event(12, 0)
+    ev = runtime.Event(_geom(2))
+    ev.add_hits(e.x, e.y, e.z, e.idx, e.detId)
+    ev.run(0)
+    with pytest.raises(runtime.SDLError):
+        ev.run(runtime.TL_NAV)  # before any segments
+    ev.run(runtime.SG_LAYERS)
+    ev.run(runtime.TL_NAV)
+    with pytest.raises(runtime.SDLError):
+        ev.run(5)  # the TCL join needs module-map segments under navigation tracklets
+    ev.run(runtime.TC_LAYERS)
+
+
+@pytest.mark.parametrize("wl,e", [(1, 0), (1, 1), (12, 0), (15, 0)])
+def test_pseudo_chain_matches_oracle_and_reference(wl, e):
+    """createPseudoMiniDoubletsFromAnchorModule and the module-map chain on its
+    MDs (zero cached values: the segment algorithms meet 0/0) against the
+    oracle, and the per-layer counters against the compiled reference."""
+    print(f"case pseudo chain workload={wl} event={e}", flush=True)
+    seq = [runtime.MD_PSEUDO, 1, 2, 3, 4, 5]
+    ev_in = gen.event(wl, e)
+    ev = _run(wl, [ev_in], seq)
+    ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], ev_in, oracle.CR, seq)
+    _assert_same(ev.export_all(0), ref, f"pseudo chain w{wl} e{e}")
+    assert ev.counts(0) == cnt
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_pseudo_sets.npz")) as f:
+        assert np.array_equal(ev.layer_counts(0), f[f"w{wl}_e{e}_layers"])
+
+
+def test_pseudo_then_layer_pairs():
+    print("case pseudo MDs then layer-pair chain", flush=True)
+    seq = [runtime.MD_PSEUDO, runtime.SG_LAYERS, runtime.TL_LAYERS, runtime.TC_LAYERS]
+    ev_in = gen.event(1, 2)
+    ev = _run(1, [ev_in], seq)
+    ref, cnt = oracle.run(gen.geometry(1)[1], ev_in, oracle.CR, seq)
+    _assert_same(_exported(ev, seq), ref, "pseudo + layer pairs")
+    assert ev.counts(0) == cnt

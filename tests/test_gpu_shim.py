@@ -55,3 +55,18 @@ def test_shim_layer_chain_matches_oracle(tmp_path):
     assert counts == list(cnt.values())
     assert sums == [_checksum(ref[s]) for s in oracle.STAGES]
     assert cnt["nTC"] > 0
+
+
+def test_shim_pseudo_chain_matches_oracle(tmp_path):
+    """createPseudoMiniDoubletsFromAnchorModule, then the module-map chain, through the shim."""
+    wl, e = 12, 1
+    print(f"case shim pseudo chain workload={wl} event={e}", flush=True)
+    out = subprocess.run([DEMO, str(wl), str(e), str(tmp_path), "pseudo"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = dict(l.split(" ", 1) for l in out.stdout.strip().splitlines())
+    counts = [int(v) for v in lines["counts"].split()]
+    sums = [int(v) for v in lines["checksums"].split()]
+    ref, cnt = oracle.run(gen.geometry(gen.geom_kind(wl))[1], gen.event(wl, e), oracle.CR, [10, 1, 2, 3, 4, 5])
+    assert counts == list(cnt.values())
+    assert sums == [_checksum(ref[s]) for s in oracle.STAGES]

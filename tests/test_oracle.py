@@ -162,3 +162,95 @@ def test_oracle_layer_chain_closed_forms(geom_paths):
     expect = sorted(i + o for i in inner for o in by_first.get(i[4:], []))
     got = sorted(tuple(r) for r in sets["TCL"])
     assert got == expect and len(got) > 0
+
+
+NAV_CASES = [(1, 0), (1, 1), (12, 0), (12, 1), (15, 0)]
+
+
+@pytest.mark.parametrize("wl,e", NAV_CASES)
+def test_oracle_navigation_matches_reference(wl, e, geom_paths):
+    """createTrackletsViaNavigation after the module-map segments, against the
+    compiled reference (ref_nav_sets.npz): object sets and counters."""
+    import os
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_nav_sets.npz")) as f:
+        golden = {k: f[k] for k in f.files}
+    ev = gen.event(wl, e)
+    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, [0, 1, 9])
+    for s in ("MD", "SG", "TL"):
+        ref = golden[f"w{wl}_e{e}_{s}"]
+        got = oracle.as_multiset(sets[s])
+        assert got.shape == ref.shape and np.array_equal(got, ref), s
+    assert len(golden[f"w{wl}_e{e}_TL"]) > 0
+    assert [counters[k] for k in COUNTERS] == list(golden[f"w{wl}_e{e}_counters"])
+
+
+def test_oracle_navigation_closed_form(geom_paths):
+    """Navigation candidates by brute force over the SG set: (s1, s3) for every
+    SG s2 that starts at s1's outer MD and ends at s3's inner MD; the tracklets
+    are the candidates that pass, so the module-map TL set (connected modules)
+    and the navigation set agree on every candidate both consider."""
+    wl, e = 1, 0
+    ev = gen.event(wl, e)
+    paths = geom_paths[gen.geom_kind(wl)]
+    nav, cn = oracle.run(paths, ev, oracle.GLIBC, [0, 1, 9])
+    sg = [tuple(r) for r in nav["SG"]]
+    starts = {}
+    for r in sg:
+        starts.setdefault(r[:2], []).append(r)
+    cand = set()
+    for s1 in sg:
+        for s2 in starts.get(s1[2:], []):
+            for s3 in starts.get(s2[2:], []):
+                cand.add(s1 + s3)
+    assert cn["nTLCand"] == sum(len(starts.get(s2[2:], [])) for s1 in sg for s2 in starts.get(s1[2:], []))
+    tl = {tuple(r) for r in nav["TL"]}
+    assert tl <= cand and len(tl) > 0
+    mm, _ = oracle.run(paths, ev, oracle.GLIBC, [0, 1, 2, 3])
+    both = {tuple(r) for r in mm["TL"]} & cand
+    assert both <= tl
+
+
+PSEUDO_CASES = [(1, 0), (1, 1), (12, 0), (15, 0)]
+
+
+@pytest.mark.parametrize("wl,e", PSEUDO_CASES)
+def test_oracle_pseudo_chain_matches_reference(wl, e, geom_paths):
+    """createPseudoMiniDoubletsFromAnchorModule followed by the module-map chain
+    (whose algorithms then meet zero cached values and the 0/0 they produce)
+    against the compiled reference (ref_pseudo_sets.npz)."""
+    import os
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_pseudo_sets.npz")) as f:
+        golden = {k: f[k] for k in f.files}
+    ev = gen.event(wl, e)
+    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, [10, 1, 2, 3, 4, 5])
+    for s in oracle.STAGES:
+        ref = golden[f"w{wl}_e{e}_{s}"]
+        got = oracle.as_multiset(sets[s])
+        assert got.shape == ref.shape and np.array_equal(got, ref), s
+    assert [counters[k] for k in COUNTERS] == list(golden[f"w{wl}_e{e}_counters"])
+
+
+def test_oracle_pseudo_md_closed_form(geom_paths):
+    """Pseudo MDs by brute force, without the module decoding: per module pair,
+    either every hit of one sensor pairs with the first hit (input order) of
+    the other, or the reverse; nothing is rejected."""
+    wl, e = 12, 0
+    ev = gen.event(wl, e)
+    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, [10])
+    det_of = {int(i): int(d) for i, d in zip(ev.idx, ev.detId)}
+    hits_of = {}
+    for i, d in zip(ev.idx, ev.detId):
+        hits_of.setdefault(int(d), []).append(int(i))
+    groups = {}
+    for lo, up in sets["MD"]:
+        groups.setdefault((det_of[int(lo)], det_of[int(up)]), []).append((int(lo), int(up)))
+    assert len(groups) > 0
+    for (dlo, dup), mds in groups.items():
+        los = sorted({m[0] for m in mds})
+        ups = sorted({m[1] for m in mds})
+        if len(ups) == 1 and los == sorted(hits_of[dlo]):
+            assert ups[0] == hits_of[dup][0]  # every lower hit with the first upper hit
+        else:
+            assert len(los) == 1 and los[0] == hits_of[dlo][0] and ups == sorted(hits_of[dup])
+        assert len(mds) == len(set(mds))
+    assert counters["nMDCand"] == counters["nMD"] == len(sets["MD"])
