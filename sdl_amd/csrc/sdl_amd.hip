@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <initializer_list>
 #include <iterator>
 #include <sstream>
 #include <string>
@@ -176,65 +177,85 @@ __device__ __forceinline__ unsigned long long scanPack(unsigned epoch, unsigned 
     return ((unsigned long long)epoch << 44) | ((unsigned long long)flag << 42) | val;
 }
 
-// Sum over the block of one value per thread (waveSum: kWavesPerBlock shared slots).
-__device__ __forceinline__ long long blockSum(long long v, long long* waveSum) {
+typedef long long ll2v __attribute__((ext_vector_type(2)));
+
+// The kScanItems items a thread owns, zeros past n; whole groups of aligned
+// items are read with vector loads.
+template <class TIn>
+__device__ __forceinline__ void loadItems(const TIn* in, long long b, long long n, bool vec, long long (&v)[kScanItems]) {
+    if (vec && b + kScanItems <= n) {
+        if constexpr (sizeof(TIn) == 4) {
+            const int4 a = *reinterpret_cast<const int4*>(in + b);
+            const int4 c = *reinterpret_cast<const int4*>(in + b + 4);
+            const int w[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) v += __shfl_xor(v, off);
-    __syncthreads();
-    if ((threadIdx.x & (kWave - 1)) == 0) waveSum[threadIdx.x / kWave] = v;
-    __syncthreads();
-    long long t = 0;
+            for (int k = 0; k < kScanItems; ++k) v[k] = (long long)TIn(w[k]);
+            return;
+        } else if constexpr (sizeof(TIn) == 1) {
+            const unsigned long long w = *reinterpret_cast<const unsigned long long*>(in + b);
 #pragma unroll
-    for (int w = 0; w < kWavesPerBlock; ++w) t += waveSum[w];
-    return t;
+            for (int k = 0; k < kScanItems; ++k) v[k] = (long long)TIn((unsigned char)(w >> (8 * k)));
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) v[k] = b + k < n ? (long long)in[b + k] : 0;
 }
 
-// Exclusive scan of the kScanTile items at [base, base + kScanTile) (zeros past
-// n) written as run + prefix; returns the items' total.
+// Writes the exclusive scan of a thread's items v (zeros past n), starting at r.
 template <class TIn>
-__device__ __forceinline__ long long scanSubTile(const TIn* in, long long* out, long long base, long long n,
-                                                 long long run, long long* waveSum) {
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const long long b = base + (long long)threadIdx.x * kScanItems;
-    long long v[kScanItems];
-    long long s = 0;
+__device__ __forceinline__ void writeTile(long long* out, long long b, long long n, bool vec,
+                                          const long long (&v)[kScanItems], long long r) {
+    if (vec && b + kScanItems <= n) {
+        ll2v o[kScanItems / 2];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const long long i = b + k;
-        v[k] = i < n ? (long long)in[i] : 0;
-        s += v[k];
+        for (int k = 0; k < kScanItems; k += 2) {
+            o[k / 2].x = r;
+            r += v[k];
+            o[k / 2].y = r;
+            r += v[k + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kScanItems / 2; ++k) reinterpret_cast<ll2v*>(out + b)[k] = o[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (b + k < n) out[b + k] = r;
+            r += v[k];
+        }
     }
+}
+
+// Block-wide inclusive scan of one value per thread: returns (exclusive prefix
+// of the thread, block total).
+__device__ __forceinline__ void blockScan(long long s, long long* waveSum, long long& excl, long long& total) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     long long x = s;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
         const long long y = __shfl_up(x, off);
         if (lane >= off) x += y;
     }
-    __syncthreads();
     if (lane == kWave - 1) waveSum[wave] = x;
     __syncthreads();
-    long long wPrefix = 0, total = 0;
+    long long wPrefix = 0;
+    total = 0;
 #pragma unroll
     for (int w = 0; w < kWavesPerBlock; ++w) {
         if (w < wave) wPrefix += waveSum[w];
         total += waveSum[w];
     }
-    long long r = run + wPrefix + x - s;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const long long i = b + k;
-        if (i < n) out[i] = r;
-        r += v[k];
-    }
-    return total;
+    excl = wPrefix + x - s;
 }
 
-// Tiles own `chunk` consecutive items (a multiple of kScanTile) and there are
-// at most kWave of them, so one look-back window covers every predecessor: a
-// tile sums its chunk, publishes the sum, reads its predecessors' words in one
-// round and scans its chunk a second time writing the offsets.
+// Each tile scans kScanTile consecutive items held in registers: it sums them,
+// publishes the sum, looks back over its predecessors' words for its prefix
+// and writes the offsets.  Every tile of the launch takes part (the ticket
+// protocol needs all of them); tiles past the end of the input scan zeros.
+// Used for at most kWave tiles: one look-back window then covers every
+// predecessor (each further window costs a device-coherent round trip).
 template <class TIn>
-__global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, Count cn, int nTiles, long long chunk,
+__global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, Count cn, int nTiles,
                                                  unsigned long long* state, unsigned long long stateCap,
                                                  ScanCtl* ctl, long long* tot, int totIdx) {
     __shared__ long long waveSum[kWavesPerBlock];
@@ -249,26 +270,21 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
     const int tile = tileSh;
     const unsigned epoch = epochSh;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    // every tile of the launch takes part (the ticket protocol needs all of
-    // them); tiles past the end of the input scan zeros
     long long n = cn.get();
-    if (n > (long long)nTiles * chunk) {  // launched for fewer items than the total: abort the run
+    if (n > (long long)nTiles * kScanTile) {  // launched for fewer items than the total: abort the run
         if (threadIdx.x == 0) tot[TOT_ABORT] = 1 + 100;
         n = 0;
     }
-    const long long begin = (long long)tile * chunk;
-    const long long end = min(begin + chunk, n);
-    // pass 1: the chunk's sum
-    long long part = 0;
-    for (long long sub = begin; sub < end; sub += kScanTile) {
-        const long long b = sub + (long long)threadIdx.x * kScanItems;
+    const bool vec = ((unsigned long long)in & (sizeof(TIn) * kScanItems - 1)) == 0 &&
+                     ((unsigned long long)out & 15) == 0;
+    const long long b = (long long)tile * kScanTile + (long long)threadIdx.x * kScanItems;
+    long long v[kScanItems];
+    loadItems(in, b, n, vec, v);
+    long long s = 0;
 #pragma unroll
-        for (int k = 0; k < kScanItems; ++k) {
-            const long long i = b + k;
-            if (i < end) part += (long long)in[i];
-        }
-    }
-    const long long agg = blockSum(part, waveSum);
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
+    long long excl0, agg;
+    blockScan(s, waveSum, excl0, agg);
     if (wave == 0) {
         long long excl = 0;
         if (tile == 0) {
@@ -307,9 +323,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
     }
     __syncthreads();
     const long long prefix = prefixSh;
-    // pass 2: offsets
-    long long run = prefix;
-    for (long long sub = begin; sub < end; sub += kScanTile) run += scanSubTile(in, out, sub, end, run, waveSum);
+    writeTile<TIn>(out, b, n, vec, v, prefix + excl0);
     if (tile == nTiles - 1 && threadIdx.x == 0) {
         const long long total = prefix + agg;
         const long long t = (unsigned long long)prefix >= kScanValMask || (unsigned long long)total >= kScanValMask ? -1 : total;
@@ -317,7 +331,6 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
         if (totIdx >= 0) tot[totIdx] = t;
     }
     // the last tile to finish re-arms the ticket and advances the epoch
-    __syncthreads();
     __shared__ bool lastSh;
     if (threadIdx.x == 0)
         lastSh = __hip_atomic_fetch_add(&ctl->done, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
@@ -334,64 +347,82 @@ __global__ void __launch_bounds__(kBlock) k_scan(const TIn* in, long long* out, 
     }
 }
 
-// Large scans: reduce-then-scan over tiles of kScanTile items, fully parallel.
+// Large scans, pass 1: the sum of every tile of kScanTile items.
 template <class TIn>
 __global__ void __launch_bounds__(kBlock) k_scan_reduce(const TIn* in, Count cn, long long* tileSum) {
     __shared__ long long waveSum[kWavesPerBlock];
     const long long n = cn.get();
+    const bool vec = ((unsigned long long)in & (sizeof(TIn) * kScanItems - 1)) == 0;
     const long long b = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
-    long long part = 0;
+    long long v[kScanItems];
+    loadItems(in, b, n, vec, v);
+    long long s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (b + k < n) part += (long long)in[b + k];
-    const long long t = blockSum(part, waveSum);
-    if (threadIdx.x == 0) tileSum[blockIdx.x] = t;
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) waveSum[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) t += waveSum[w];
+        tileSum[blockIdx.x] = t;
+    }
 }
 
-// One block: exclusive scan of the tile sums; the grand total goes to out[n] and tot[totIdx].
-__global__ void __launch_bounds__(1024) k_scan_tiles(const long long* tileSum, int nTiles, long long* tileOff,
-                                                     long long* out, Count cn, long long* tot, int totIdx) {
-    __shared__ long long sh[1024];
+// Large scans, pass 2: each tile takes its prefix from tileOff (the scanned
+// tile sums) or, without it, sums its predecessors' tile sums itself (used up
+// to kScanSelfTiles tiles: a few thousand values, read from cache), then scans
+// its items; the last tile writes the total to out[n] and tot[totIdx].
+constexpr int kScanSelfTiles = 4096;
+template <class TIn>
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const TIn* in, long long* out, Count cn, int nTiles,
+                                                       const long long* tileSum, const long long* tileOff,
+                                                       long long* tot, int totIdx) {
+    __shared__ long long waveSum[kWavesPerBlock];
+    __shared__ long long prefixSh;
+    const int tile = blockIdx.x;
     long long n = cn.get();
-    if (n > (long long)nTiles * kScanTile) {  // launched for fewer items than the total: abort the run
-        if (threadIdx.x == 0) tot[TOT_ABORT] = 1 + 100;
-        n = 0;
-    }
-    const int per = (nTiles + 1023) / 1024;
-    const int t0 = threadIdx.x * per;
-    long long s = 0;
-    for (int k = 0; k < per; ++k)
-        if (t0 + k < nTiles) s += tileSum[t0 + k];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const long long v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    const bool over = n > (long long)nTiles * kScanTile;  // launched for fewer items than the total: abort the run
+    if (over) n = 0;
+    long long prefix;
+    if (tileOff) {
+        prefix = tileOff[tile];
+    } else {
+        long long p = 0;
+        for (int t = threadIdx.x; t < tile; t += blockDim.x) p += tileSum[t];
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) p += __shfl_xor(p, off);
+        if ((threadIdx.x & (kWave - 1)) == 0) waveSum[threadIdx.x / kWave] = p;
         __syncthreads();
-        sh[threadIdx.x] += v;
-        __syncthreads();
-    }
-    long long run = sh[threadIdx.x] - s;
-    for (int k = 0; k < per; ++k)
-        if (t0 + k < nTiles) {
-            tileOff[t0 + k] = run;
-            run += tileSum[t0 + k];
+        if (threadIdx.x == 0) {
+            long long t = 0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w) t += waveSum[w];
+            prefixSh = t;
         }
-    if (threadIdx.x == 1023) {
-        const long long total = sh[1023];
+        __syncthreads();
+        prefix = prefixSh;
+    }
+    const bool vec = ((unsigned long long)in & (sizeof(TIn) * kScanItems - 1)) == 0 &&
+                     ((unsigned long long)out & 15) == 0;
+    const long long b = (long long)tile * kScanTile + (long long)threadIdx.x * kScanItems;
+    long long v[kScanItems];
+    loadItems(in, b, n, vec, v);
+    long long s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
+    long long excl, agg;
+    blockScan(s, waveSum, excl, agg);
+    writeTile<TIn>(out, b, n, vec, v, prefix + excl);
+    if (tile == nTiles - 1 && threadIdx.x == 0) {
+        if (over) tot[TOT_ABORT] = 1 + 100;
+        const long long total = prefix + agg;
         const long long t = (unsigned long long)total >= kScanValMask ? -1 : total;
         out[n] = t;
         if (totIdx >= 0) tot[totIdx] = t;
     }
-}
-
-template <class TIn>
-__global__ void __launch_bounds__(kBlock) k_scan_apply(const TIn* in, long long* out, Count cn,
-                                                       const long long* tileOff) {
-    __shared__ long long waveSum[kWavesPerBlock];
-    const long long n = cn.get();
-    const long long base = (long long)blockIdx.x * kScanTile;
-    if (base >= n) return;  // uniform over the block
-    scanSubTile(in, out, base, n, tileOff[blockIdx.x], waveSum);
 }
 
 // Zero-fill (a kernel rather than a memset, so that captured graphs replay it reliably).
@@ -399,6 +430,26 @@ template <class T>
 __global__ void k_zero(T* p, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = T(0);
+}
+
+// Zero-fill of up to kZeroArrays arrays in one launch (sizes in 32-bit words).
+constexpr int kZeroArrays = 4;
+struct ZeroSet {
+    unsigned* p[kZeroArrays];
+    long long end[kZeroArrays];  // running word totals
+    int n;
+};
+
+__global__ void k_zero_many(ZeroSet z) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long start = 0;
+    for (int a = 0; a < z.n; ++a) {
+        if (i < z.end[a]) {
+            z.p[a][i - start] = 0u;
+            return;
+        }
+        start = z.end[a];
+    }
 }
 
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long n, unsigned long long v) {
@@ -410,7 +461,8 @@ struct ScanScratch {
     unsigned long long* state = nullptr;
     unsigned long long cap = 0;
     ScanCtl* ctl = nullptr;
-    DBuf<long long> tileSum, tileOff;
+    static constexpr int kLevels = 3;  // 2048^3 items
+    DBuf<long long> tileSum[kLevels], tileOff[kLevels];
     ~ScanScratch() {
         if (state) (void)hipFree(state);
         if (ctl) (void)hipFree(ctl);
@@ -447,24 +499,33 @@ struct ScanScratch {
 // out must hold n+1 entries; out[n] (and tot[totIdx], if totIdx >= 0) receives
 // the total (-1 if it does not fit 42 bits).  nMax bounds the item count on
 // the host (it sizes the launch); the count itself may live on the device.
+// Up to kWave tiles: one single-pass launch.  More: tile sums, then the tiles'
+// offsets (from the sums directly, or by scanning the sums at the next level),
+// then the items.
 template <class TIn>
 int scanExclusive(const TIn* in, long long* out, Count cn, long long nMax, ScanScratch& sc, hipStream_t st,
-                  long long* tot, int totIdx) {
+                  long long* tot, int totIdx, int level = 0) {
     const long long nTiles = std::max(1LL, (nMax + kScanTile - 1) / kScanTile);
     if (nTiles <= kWave) {
-        // one launch; every predecessor of a tile lies in its first look-back window
         if (int r = sc.ensure((unsigned long long)nTiles, st)) return r;
-        hipLaunchKernelGGL(k_scan<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, cn, int(nTiles),
-                           (long long)kScanTile, sc.state, sc.cap, sc.ctl, tot, totIdx);
+        hipLaunchKernelGGL(k_scan<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, cn, int(nTiles), sc.state,
+                           sc.cap, sc.ctl, tot, totIdx);
         return 0;
     }
-    if (nTiles > (1LL << 30)) return fail(SDL_ERR_CAPACITY, "scan too large");
-    if (int r = sc.tileSum.ensure(nTiles)) return r;
-    if (int r = sc.tileOff.ensure(nTiles)) return r;
-    hipLaunchKernelGGL(k_scan_reduce<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, cn, sc.tileSum.p);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, sc.tileSum.p, int(nTiles), sc.tileOff.p, out, cn, tot,
-                       totIdx);
-    hipLaunchKernelGGL(k_scan_apply<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, cn, sc.tileOff.p);
+    if (nTiles > (1LL << 30) || level >= ScanScratch::kLevels) return fail(SDL_ERR_CAPACITY, "scan too large");
+    DBuf<long long>& sum = sc.tileSum[level];
+    DBuf<long long>& off = sc.tileOff[level];
+    if (int r = sum.ensure(nTiles)) return r;
+    hipLaunchKernelGGL(k_scan_reduce<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, cn, sum.p);
+    const long long* tileOff = nullptr;
+    if (nTiles > kScanSelfTiles) {
+        if (int r = off.ensure(nTiles + 1)) return r;
+        if (int r = scanExclusive<long long>(sum.p, off.p, Count{nTiles, tot, -1}, nTiles, sc, st, tot, -1, level + 1))
+            return r;
+        tileOff = off.p;
+    }
+    hipLaunchKernelGGL(k_scan_apply<TIn>, dim3(unsigned(nTiles)), dim3(kBlock), 0, st, in, out, cn, int(nTiles),
+                       sum.p, tileOff, tot, totIdx);
     return 0;
 }
 
@@ -1147,27 +1208,71 @@ __global__ void k_slot_md_sg_list(Count cNS, const long long* mdStart, const lon
         if (sgStart[m + 1] > sgStart[m]) list[d++] = int(m);
 }
 
-// One thread per inner SG: its pairs, in connected-module and MD order, into
-// one list ordered by variant (offsets pairOff[v * stride + s]).
-__global__ void k_tl_enum(Count cSG, int NM, const ModDev* mods, const int* sgOut, const int* sgFlags,
-                          const int* mdSlot, const int* connStart, const int* connList, const long long* listStart,
-                          const int* list, const long long* pairOff, long long stride, int2* pair, Guard g) {
+// A group of kEnumLanes lanes per inner SG writes its pairs, in connected-module
+// and MD order, into one list ordered by variant (offsets pairOff[v * stride + s]).
+// Lane k of the group takes connected module k (in rounds of kEnumLanes); one
+// scan of the per-variant counts packed into 21-bit fields places every
+// module's run, and the lanes then write the round's pairs contiguously.
+constexpr int kEnumLanes = 16;
+__global__ void __launch_bounds__(kBlock) k_tl_enum(Count cSG, int NM, const ModDev* mods, const int* sgOut,
+                                                    const int* sgFlags, const int* mdSlot, const int* connStart,
+                                                    const int* connList, const long long* listStart, const int* list,
+                                                    const long long* pairOff, long long stride, int2* pair, Guard g) {
     if (g.over()) return;
     const int nSG = int(cSG.get());
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nSG) return;
+    const int gl = threadIdx.x & (kEnumLanes - 1);
+    const long long s = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kEnumLanes;
+    if (s >= nSG) return;  // uniform over the group
     const int b = mdSlot[sgOut[s]];
     const int m = b % NM, eb = b - m;
     const int fl = sgFlags[s];
     long long pos[kNVariantsTL];
     for (int v = 0; v < kNVariantsTL; ++v) pos[v] = pairOff[v * stride + s];
-    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
-        const int v = tlModuleVariant(fl, mods[connList[k]]);
-        if (v >= kNVariantsTL) continue;
-        const int t = eb + connList[k];
-        long long p = pos[v];
-        for (long long q = listStart[t]; q < listStart[t + 1]; ++q) pair[p++] = make_int2(s, list[q]);
-        pos[v] = p;
+    const int k0 = connStart[m], k1 = connStart[m + 1];
+    for (int kc = k0; kc < k1; kc += kEnumLanes) {
+        const int k = kc + gl;
+        int v = kNVariantsTL, cnt = 0;
+        long long ls = 0;
+        if (k < k1) {
+            v = tlModuleVariant(fl, mods[connList[k]]);
+            if (v < kNVariantsTL) {
+                const int t = eb + connList[k];
+                ls = listStart[t];
+                cnt = int(listStart[t + 1] - ls);
+            }
+        }
+        // inclusive scan of the packed per-variant counts over the group
+        const unsigned long long mine = v < kNVariantsTL ? (unsigned long long)cnt << (21 * v) : 0ULL;
+        unsigned long long inc = mine;
+#pragma unroll
+        for (int off = 1; off < kEnumLanes; off <<= 1) {
+            const unsigned long long y = __shfl_up(inc, off, kEnumLanes);
+            if (gl >= off) inc += y;
+        }
+        const unsigned long long exc = inc - mine;
+        const unsigned long long tot = __shfl(inc, kEnumLanes - 1, kEnumLanes);
+        auto field = [](unsigned long long w, int f) { return (long long)((w >> (21 * f)) & 0x1FFFFFULL); };
+        // this lane's run: destination and flat position among the round's pairs
+        const long long dst = v < kNVariantsTL ? pos[v] + field(exc, v) : 0;
+        const int flat = int(field(exc, 0) + field(exc, 1) + field(exc, 2));
+        const int total = int(field(tot, 0) + field(tot, 1) + field(tot, 2));
+        // every lane of the group takes part in each step (the shuffles read
+        // all of the group's lanes)
+        for (int j0 = 0; j0 < total; j0 += kEnumLanes) {
+            const int j = j0 + gl;
+            // the last lane whose run starts at or before j owns pair j
+            int lo = 0;
+#pragma unroll
+            for (int step = kEnumLanes / 2; step >= 1; step >>= 1) {
+                const int f = __shfl(flat, lo + step, kEnumLanes);
+                if (f <= j) lo += step;
+            }
+            const int q = j - __shfl(flat, lo, kEnumLanes);
+            const long long d = __shfl(dst, lo, kEnumLanes);
+            const long long l0 = __shfl(ls, lo, kEnumLanes);
+            if (j < total) pair[d + q] = make_int2(int(s), list[l0 + q]);
+        }
+        for (int vv = 0; vv < kNVariantsTL; ++vv) pos[vv] += field(tot, vv);
     }
 }
 
@@ -1722,6 +1827,7 @@ struct CounterIn {
     const int *tpIn, *tpOut;                                 // per TP
     const int *tlIn, *tclCount;                              // per TL
     const long long* tcCand;  // per TL: TC candidates of the layer-pair form (null: the module-map count)
+    long long* connSum;       // per slot, [3][NS]: SGs, TPs, TLs on the modules connected to the slot's module
 };
 
 __device__ __forceinline__ long long connected(const CounterIn& C, int slot, const long long* slotStart) {
@@ -1734,13 +1840,61 @@ __device__ __forceinline__ long long connected(const CounterIn& C, int slot, con
     return c;
 }
 
+// Objects on the modules connected to each slot's module, for the candidate
+// counters that count them per object (SGs for TP candidates, TPs and TLs for TC
+// candidates): one walk of the connection list per slot instead of one per
+// object.  kConnLanes lanes per slot take one connection each and sum by shuffles.
+constexpr int kConnLanes = 16;
+__global__ void __launch_bounds__(kBlock) k_conn_sums(CounterIn C, const long long* tot) {
+    const int gl = threadIdx.x & (kConnLanes - 1);
+    const int slot = (blockIdx.x * blockDim.x + threadIdx.x) / kConnLanes;
+    if (slot >= C.NS || tot[TOT_ABORT] != 0) return;  // uniform over the group
+    const int m = slot % C.NM, eb = slot - m;
+    const bool sg = C.stages & 4, tp = C.stages & 16, tl = (C.stages & 32) && !C.tcCand;
+    long long a = 0, b = 0, c = 0;
+    for (int k = C.connStart[m] + gl; k < C.connStart[m + 1]; k += kConnLanes) {
+        const int t = eb + C.connList[k];
+        if (sg) a += C.slotSgStart[t + 1] - C.slotSgStart[t];
+        if (tp) b += C.slotTpStart[t + 1] - C.slotTpStart[t];
+        if (tl) c += C.slotTlStart[t + 1] - C.slotTlStart[t];
+    }
+#pragma unroll
+    for (int off = 1; off < kConnLanes; off <<= 1) {
+        a += __shfl_xor(a, off, kConnLanes);
+        b += __shfl_xor(b, off, kConnLanes);
+        c += __shfl_xor(c, off, kConnLanes);
+    }
+    if (gl != 0) return;
+    if (sg) C.connSum[slot] = a;
+    if (tp) C.connSum[C.NS + slot] = b;
+    if (tl) C.connSum[2 * C.NS + slot] = c;
+}
+
+// Adds v (0: nothing) to bins[key] for every lane of the wave, one atomic per
+// distinct key (lanes of a wave mostly share one).  Every lane must be active.
+__device__ __forceinline__ void waveBinAdd(unsigned long long* bins, int key, long long v) {
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long rem = __ballot(v != 0);
+    while (rem) {
+        const int leader = __builtin_ctzll(rem);
+        const int k = __shfl(key, leader);
+        const bool mine = ((rem >> lane) & 1ULL) && key == k;
+        long long x = mine ? v : 0;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) x += __shfl_xor(x, off);
+        if (lane == leader) atomicAdd(&bins[k], (unsigned long long)x);
+        rem &= ~__ballot(mine);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long long* tot, unsigned long long* counters) {
-    __shared__ unsigned long long bins[kMaxLdsBins];
+    __shared__ unsigned long long ldsBins[kMaxLdsBins];
     const int nb = C.E * kNCounters * kNBins;
     const bool lds = nb <= kMaxLdsBins;
     if (lds)
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) ldsBins[b] = 0;
     __syncthreads();
+    unsigned long long* bins = lds ? ldsBins : counters;
     const bool aborted = tot[TOT_ABORT] != 0;
     const long long nSlot = aborted || !(C.stages & 1) ? 0 : C.NS;
     const long long nMd = aborted || !(C.stages & 2) ? 0 : tot[TOT_MD];
@@ -1748,61 +1902,73 @@ __global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long lon
     const long long nTp = aborted || !(C.stages & 16) ? 0 : tot[TOT_TP];
     const long long nTl = aborted || !(C.stages & 32) ? 0 : tot[TOT_TL];
     const long long n = nSlot + nMd + nSg + nTp + nTl;
-    auto add = [&](int slot, int which, long long v) {
-        if (v == 0) return;
-        const int e = slot / C.NM, bin = layerBin(C.mods[slot % C.NM]);
-        const int k = (e * kNCounters + which) * kNBins + bin;
-        if (lds) atomicAdd(&bins[k], (unsigned long long)v);
-        else atomicAdd(&counters[k], (unsigned long long)v);
-    };
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        long long j = i;
-        if (j < nSlot) {  // MD candidates / objects per lower slot (Event.cc:257-305)
-            add(int(j), 0, C.mdCap[j]);
-            add(int(j), 5, C.mdCount[j]);
-            continue;
-        }
-        j -= nSlot;
-        if (j < nMd) {  // SG candidates / objects per inner MD (Event.cc:457-522)
-            const int slot = C.mdSlot[j];
-            add(slot, 1, C.sgCap[j]);
-            add(slot, 6, C.sgCount[j]);
-            continue;
-        }
-        j -= nMd;
-        if (j < nSg) {  // TP (Event.cc:539-597) and TL (Event.cc:627-693) per inner SG
-            const int slot = C.mdSlot[C.sgIn[j]];
-            if (C.stages & 4) {
-                add(slot, 2, connected(C, slot, C.slotSgStart));
-                add(slot, 7, C.tpCount[j]);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    // block-uniform loop: every lane reaches the wave-wide adds
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
+        long long j = base + threadIdx.x;
+        int slot = -1;
+        int w[4] = {0, 0, 0, 0};
+        long long v[4] = {0, 0, 0, 0};
+        if (j < n) {
+            if (j < nSlot) {  // MD candidates / objects per lower slot (Event.cc:257-305)
+                slot = int(j);
+                w[0] = 0; v[0] = C.mdCap[j];
+                w[1] = 5; v[1] = C.mdCount[j];
+            } else if ((j -= nSlot) < nMd) {  // SG candidates / objects per inner MD (Event.cc:457-522)
+                slot = C.mdSlot[j];
+                w[0] = 1; v[0] = C.sgCap[j];
+                w[1] = 6; v[1] = C.sgCount[j];
+            } else if ((j -= nMd) < nSg) {  // TP (Event.cc:539-597) and TL (Event.cc:627-693) per inner SG
+                slot = C.mdSlot[C.sgIn[j]];
+                if (C.stages & 4) {
+                    w[0] = 2; v[0] = C.connSum[slot];
+                    w[1] = 7; v[1] = C.tpCount[j];
+                }
+                if (C.stages & 8) {
+                    w[2] = 3; v[2] = C.tlCap[j];
+                    w[3] = 8; v[3] = C.tlCount[j];
+                }
+            } else if ((j -= nSg) < nTp) {  // TC candidates per inner TP: TPs at modules connected to its outermost MD (Event.cc:889-982)
+                slot = C.mdSlot[C.sgIn[C.tpIn[j]]];
+                w[0] = 4; v[0] = C.connSum[C.NS + C.mdSlot[C.sgOut[C.tpOut[j]]]];
+            } else {  // TC candidates and objects per inner TL (Event.cc:999-1062; layer pairs: Event.cc:840-872)
+                j -= nTp;
+                slot = C.mdSlot[C.sgIn[C.tlIn[j]]];
+                w[0] = 4; v[0] = C.tcCand ? C.tcCand[j] : C.connSum[2 * C.NS + C.mdSlot[C.sgOut[C.tlIn[j]]]];
+                w[1] = 9; v[1] = C.tclCount[j];
             }
-            if (C.stages & 8) {
-                add(slot, 3, C.tlCap[j]);
-                add(slot, 8, C.tlCount[j]);
-            }
-            continue;
         }
-        j -= nSg;
-        if (j < nTp) {  // TC candidates per inner TP: TPs at modules connected to its outermost MD (Event.cc:889-982)
-            add(C.mdSlot[C.sgIn[C.tpIn[j]]], 4, connected(C, C.mdSlot[C.sgOut[C.tpOut[j]]], C.slotTpStart));
-            continue;
-        }
-        j -= nTp;
-        {  // TC candidates and objects per inner TL (Event.cc:999-1062; layer pairs: Event.cc:840-872)
-            const int slot = C.mdSlot[C.sgIn[C.tlIn[j]]];
-            add(slot, 4, C.tcCand ? C.tcCand[j] : connected(C, C.mdSlot[C.sgOut[C.tlIn[j]]], C.slotTlStart));
-            add(slot, 9, C.tclCount[j]);
-        }
+        const int key0 = slot >= 0 ? (slot / C.NM) * kNCounters * kNBins + layerBin(C.mods[slot % C.NM]) : 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) waveBinAdd(bins, key0 + w[c] * kNBins, v[c]);
     }
     __syncthreads();
     if (lds)
         for (int b = threadIdx.x; b < nb; b += blockDim.x)
-            if (bins[b]) atomicAdd(&counters[b], bins[b]);
+            if (ldsBins[b]) atomicAdd(&counters[b], ldsBins[b]);
 }
 
-// TCs from triplets, one wave per tracklet (grid-stride over tracklets): the
-// lanes copy the tracklet's inner TPs.  Each TC counts at the module of its
+// Adds 1 to bins[key] for every active lane: one LDS atomic per distinct key of
+// the wave instead of one per lane (lanes of a wave mostly share a key).
+__device__ __forceinline__ void binCount(unsigned long long* bins, int key) {
+    const int lane = threadIdx.x & (kWave - 1);
+    bool done = false;
+    for (;;) {
+        const unsigned long long rem = __ballot(!done);
+        if (!rem) break;
+        const int leader = __builtin_ctzll(rem);
+        const int k = __shfl(key, leader);
+        const bool mine = !done && key == k;
+        const unsigned long long m = __ballot(mine);
+        if (lane == leader) atomicAdd(&bins[k], (unsigned long long)__popcll(m));
+        done = done || mine;
+    }
+}
+
+// TCs from triplets, kTcpLanes lanes per tracklet (grid-stride over tracklets):
+// the lanes copy the tracklet's inner TPs.  Each TC counts at the module of its
 // inner TP (Event.cc:893-897,965) in a per-block LDS histogram.
+constexpr int kTcpLanes = 8;
 __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long* start, const int2* info,
                                                       const int* tpEndList, const int* tpIn, const int* sgIn,
                                                       const int* mdSlot, int NM, int E, const ModDev* mods, int2* tc,
@@ -1815,18 +1981,18 @@ __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long
     if (lds)
         for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0;
     __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1);
-    const int w0 = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6), nw = gridDim.x * kWavesPerBlock;
-    for (int l = w0; l < nTL; l += nw) {
+    const int gl = threadIdx.x & (kTcpLanes - 1);
+    const int g0 = (blockIdx.x * blockDim.x + threadIdx.x) / kTcpLanes, ng = gridDim.x * blockDim.x / kTcpLanes;
+    for (int l = g0; l < nTL; l += ng) {
         const long long a = start[l], n = start[l + 1] - a;
         if (n == 0) continue;
         const int2 f = info[l];
-        for (long long q = lane; q < n; q += kWave) {
+        for (long long q = gl; q < n; q += kTcpLanes) {
             const int t = tpEndList[f.x + q];
             tc[a + q] = make_int2(t, f.y);
             const int sl = mdSlot[sgIn[tpIn[t]]];
             const int e = sl / NM, bin = layerBin(mods[sl % NM]);
-            if (lds) atomicAdd(&bins[e * kNBins + bin], 1ULL);
+            if (lds) binCount(bins, e * kNBins + bin);
             else atomicAdd(&counters[(e * kNCounters + 9) * kNBins + bin], 1ULL);
         }
     }
@@ -2075,6 +2241,7 @@ struct sdl_event {
     DBuf<unsigned char> tlFlag;
     // layer-pair stages: per-event layer lists of MDs and SGs, TL candidate slabs
     DBuf<int> layerCnt, layerMd, layerSg, tlCapInt, tllSlab, tlPerLayer;
+    DBuf<long long> connSum;  // per slot [3][NS], for the counters
     DBuf<long long> layerMdOff, layerSgOff, tcCand;
     DBuf<int> tlScored;  // learned proposal: pairs scored per wave, per variant (diagnostic)
     long long tlScoredStride = 0;
@@ -2263,6 +2430,35 @@ void zeroAsync(T* p, long long n, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_zero<T>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0, st, p, n);
 }
 
+// One launch zeroing several arrays: zeroMany(st, {span(p, n), ...}).
+struct ZeroSpan {
+    void* p;
+    long long bytes;
+};
+
+template <class T>
+ZeroSpan span(T* p, long long n) {
+    static_assert(sizeof(T) % 4 == 0, "zeroMany works in 32-bit words");
+    return ZeroSpan{p, n * (long long)sizeof(T)};
+}
+
+void zeroMany(hipStream_t st, std::initializer_list<ZeroSpan> spans) {
+    ZeroSet z{};
+    long long total = 0;
+    for (const ZeroSpan& sp : spans) {
+        if (sp.bytes <= 0) continue;
+        if (z.n == kZeroArrays) {
+            hipLaunchKernelGGL(k_zero_many, dim3(gridFor(total, kBlock)), dim3(kBlock), 0, st, z);
+            z = ZeroSet{};
+            total = 0;
+        }
+        total += sp.bytes / 4;
+        z.p[z.n] = static_cast<unsigned*>(sp.p);
+        z.end[z.n++] = total;
+    }
+    if (z.n) hipLaunchKernelGGL(k_zero_many, dim3(gridFor(total, kBlock)), dim3(kBlock), 0, st, z);
+}
+
 // Uploads the batch's event offsets (outside any captured graph: the host array
 // is read when the copy runs).
 int uploadOffsets(sdl_event* ev) {
@@ -2277,10 +2473,8 @@ int runPrep(sdl_event* ev) {
     const int NS = ev->E * G->NM;
     ev->ranMask = 0;
     if (int r = ensureHits(ev)) return r;
-    zeroAsync(ev->tot.p, TOT_N, st);
-    zeroAsync(ev->slotHitCount.p, NS, st);
-    zeroAsync(ev->slotFill.p, NS, st);
-    zeroAsync(ev->counters.p, (long long)ev->E * kNCounters * kNBins, st);
+    zeroMany(st, {span(ev->tot.p, TOT_N), span(ev->slotHitCount.p, NS), span(ev->slotFill.p, NS),
+                  span(ev->counters.p, (long long)ev->E * kNCounters * kNBins)});
     // launches cover the hit buffers' capacity; the kernels read the hit count
     const long long nhGrid = (long long)ev->rt.cap;
     timerBegin(ev, T_PREP);
@@ -2427,8 +2621,7 @@ int runSG(sdl_event* ev, bool layers) {
     hipLaunchKernelGGL(k_slot_range, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->mdStart.p, ev->sgStart.p, ev->slotSgStart.p);
     // reverse index: SGs by outer MD
-    zeroAsync(ev->sgEndCount.p, nm, st);
-    zeroAsync(ev->sgEndFill.p, nm, st);
+    zeroMany(st, {span(ev->sgEndCount.p, nm), span(ev->sgEndFill.p, nm)});
     if (nSG > 0)
         hipLaunchKernelGGL(k_count_by, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
                            ev->sgEndCount.p);
@@ -2523,7 +2716,7 @@ int runTL(sdl_event* ev) {
     if (int r = ev->tlNOut.ensure(npc)) return r;
     if (int r = ev->tlCOff.ensure(npc + 1)) return r;
     if (nSG > 0)
-        hipLaunchKernelGGL(k_tl_enum, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
+        hipLaunchKernelGGL(k_tl_enum, dim3(gridFor(nSG * kEnumLanes, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
                            ev->sgOut.p, ev->sgFlags.p, ev->mdSlot.p, G->dConnStart, G->dConnList,
                            ev->slotMdSgStart.p, ev->slotMdSgList.p, ev->tlPOff.p, ns, ev->tlPair.p,
                            guardOf(ev, TOT_TLPAIRS, nPairs, SDL_STAGE_TL));
@@ -2591,8 +2784,7 @@ int runTL(sdl_event* ev) {
         if (nc[v] < 0) return fail(SDL_ERR_HIP, "readback failed");
     }
     // second stage per candidate
-    zeroAsync(ev->tlCount.p, ns, st);
-    zeroAsync(ev->tlFill.p, ns, st);
+    zeroMany(st, {span(ev->tlCount.p, ns), span(ev->tlFill.p, ns)});
     const HitsDev H = hitsView(ev);
     for (int v = 0; v < kNVariantsTL; ++v) {
         if (nc[v] == 0) continue;
@@ -2659,7 +2851,7 @@ int runTCP(sdl_event* ev) {
     ev->nTCP = nTCP;
     if (int r = ev->tcp.ensure(std::max(nTCP, 1LL))) return r;
     if (nTCP > 0)
-        hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(nTL, kWavesPerBlock), 4096u)), dim3(kBlock), 0, st, cTL,
+        hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(nTL, kBlock / kTcpLanes), 65536u)), dim3(kBlock), 0, st, cTL,
                            ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p, ev->sgIn.p, ev->mdSlot.p, G->NM,
                            ev->E, G->dMods, ev->tcp.p, ev->counters.p, guardOf(ev, TOT_TCP, nTCP, SDL_STAGE_TCP));
     return 0;
@@ -2796,10 +2988,9 @@ int runTCLayers(sdl_event* ev) {
     if (int r = ev->tcpStart.ensure(nl + 1)) return r;
     if (int r = ev->tlPerLayer.ensure((long long)ev->E * kNBins)) return r;
     // no TCs from triplets in this form
-    zeroAsync(ev->tcpStart.p, nl + 1, st);
-    zeroAsync(ev->tot.p + TOT_TCP, 1, st);
+    zeroMany(st, {span(ev->tcpStart.p, nl + 1), span(ev->tot.p + TOT_TCP, 1),
+                  span(ev->tlPerLayer.p, (long long)ev->E * kNBins)});
     ev->nTCP = 0;
-    zeroAsync(ev->tlPerLayer.p, (long long)ev->E * kNBins, st);
     const Count cTL = devCount(ev, TOT_TL);
     if (nTL > 0) {
         hipLaunchKernelGGL(k_tl_per_layer, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, G->NM, G->dMods,
@@ -2838,7 +3029,7 @@ int runStage(sdl_event* ev, int stage) {
 }
 
 // Per-layer counters of stages s0..s1 (one pass, after the stages ran).
-void launchCounters(sdl_event* ev, int s0, int s1) {
+int launchCounters(sdl_event* ev, int s0, int s1) {
     sdl_geom* G = ev->g;
     CounterIn C;
     C.NS = ev->E * G->NM;
@@ -2871,7 +3062,22 @@ void launchCounters(sdl_event* ev, int s0, int s1) {
     C.slotSgStart = ev->slotSgStart.p; C.slotTpStart = ev->slotTpStart.p; C.slotTlStart = ev->slotTlStart.p;
     C.tpIn = ev->tpIn.p; C.tpOut = ev->tpOut.p;
     C.tlIn = ev->tlIn.p; C.tclCount = ev->tcCount.p;
-    hipLaunchKernelGGL(k_counters, dim3(2048), dim3(kBlock), 0, ev->st, C, ev->tot.p, ev->counters.p);
+    if (int r = ev->connSum.ensure(3LL * C.NS)) return r;
+    C.connSum = ev->connSum.p;
+    if (C.stages & (4 | 16 | 32))
+        hipLaunchKernelGGL(k_conn_sums, dim3(gridFor((long long)C.NS * kConnLanes, kBlock)), dim3(kBlock), 0, ev->st,
+                           C, ev->tot.p);
+    // one thread per slot and object (host counts bound the device totals; the
+    // kernel's grid-stride loop covers any excess)
+    long long n = 0;
+    if (C.stages & 1) n += C.NS;
+    if (C.stages & 2) n += ev->nMD;
+    if (C.stages & (4 | 8)) n += ev->nSG;
+    if (C.stages & 16) n += ev->nTP;
+    if (C.stages & 32) n += ev->nTL;
+    const unsigned grid = unsigned(std::min<long long>(std::max<long long>(gridFor(n, kBlock), 1), 65536));
+    hipLaunchKernelGGL(k_counters, dim3(grid), dim3(kBlock), 0, ev->st, C, ev->tot.p, ev->counters.p);
+    return 0;
 }
 
 // Enqueues hit preparation (if pending), stages s0..s1 and their counters.
@@ -2880,8 +3086,7 @@ int enqueueStages(sdl_event* ev, int s0, int s1) {
         if (int r = runPrep(ev)) return r;
     for (int s = s0; s <= s1; ++s)
         if (int r = runStage(ev, s)) return r;
-    launchCounters(ev, s0, s1);
-    return 0;
+    return launchCounters(ev, s0, s1);
 }
 
 bool sameGraphKey(const sdl_event* ev) {
@@ -2973,7 +3178,7 @@ int runStages(sdl_event* ev, int s0, int s1) {
         zeroAsync(ev->tot.p + TOT_ABORT, 1, st);
         for (int s = std::max(from, s0); s <= s1; ++s)
             if (int r = runStage(ev, s)) return r;
-        launchCounters(ev, s0, s1);
+        if (int r = launchCounters(ev, s0, s1)) return r;
         HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         if (t[TOT_ABORT] != 0) return fail(SDL_ERR_CAPACITY, "device capacity check failed on an exactly sized run");

@@ -9,7 +9,7 @@ counters).
 
 A chain is the run of kernels that ends with k_counters; its stages start at
 the first kernel each stage launches (the order of sdl_amd/csrc/sdl_amd.hip's
-runStage).  Chains that launch k_tl_score_rows are the learned form.
+runStage).  Chains that launch k_tl_stage1_scored are the learned form.
 """
 import argparse
 import collections
@@ -39,7 +39,7 @@ def chains_of(rows, key):
                 stage = s
         cur += [(stage, r) for r in rs]
         if "k_counters" in name:
-            out.append((any("k_tl_score_rows" in x["Kernel_Name"] for _, x in cur), cur))
+            out.append((any("k_tl_stage1_scored" in x["Kernel_Name"] for _, x in cur), cur))
             cur, stage = [], "prep"
     return out
 

@@ -77,6 +77,22 @@ def test_batch_equals_single_events():
         assert np.array_equal(batch.layer_counts(i), single.layer_counts(0))
 
 
+def test_pu200_batch_equals_single_events():
+    """Four PU200-density events in one batch: the scans over the batch's pair
+    and candidate lists (tens of millions of items) take the multi-level path,
+    the single events the one-level path; every object list and counter agrees."""
+    print("case batch of 4 workload-2 events", flush=True)
+    evs = [gen.event(2, e) for e in range(4)]
+    batch = _run_gpu(2, evs)
+    for i, e in enumerate(evs):
+        single = _run_gpu(2, [e])
+        got, ref = batch.export_all(i), single.export_all(0)
+        for s in oracle.STAGES:
+            assert np.array_equal(got[s], ref[s]), f"batch event {i} {s} differs"
+        assert batch.counts(i) == single.counts(0)
+        assert np.array_equal(batch.layer_counts(i), single.layer_counts(0))
+
+
 def test_device_resident_input_and_rerun():
     from sdl_amd import hip
     print("case device-resident input, workload 15 event 1", flush=True)
