@@ -22,6 +22,10 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no hoisting of constant materialisations out of loops: the double-precision
+    # libm polynomials of the cut kernels otherwise pin ~80 VGPRs of constants
+    # across their loops and halve their occupancy
+    "-mllvm", "-disable-machine-licm",
     "-Wno-unused-result",
 ]
 
