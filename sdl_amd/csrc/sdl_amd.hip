@@ -588,27 +588,41 @@ __global__ void k_hit_scatter(const long long* evOff, int E, const int* slot, co
 
 // Sorts each segment ascending: restores insertion order inside each slot
 // (Module::addHit appends, Module.cc:128-137) and makes atomically scattered
-// lists deterministic.  One wave per segment; up to 64 elements are ranked in
-// registers, longer segments are sorted by the wave's first lane.
-__global__ void k_sort_segments(Count cseg, const long long* start, int* vals) {
+// lists deterministic.  kSortLanes lanes per segment (segments hold a few
+// elements on average); up to 4 * kSortLanes elements are ranked in registers,
+// longer segments are sorted by the group's first lane.
+constexpr int kSortLanes = 16;
+__global__ void __launch_bounds__(kBlock) k_sort_segments(Count cseg, const long long* start, int* vals) {
     const int nseg = int(cseg.get());
-    const int s = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (s >= nseg) return;
-    const long long a = start[s], b = start[s + 1];
+    const int gl = threadIdx.x & (kSortLanes - 1);
+    const long long sg = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kSortLanes;
+    if (sg >= nseg) return;  // uniform over the group
+    const long long a = start[sg], b = start[sg + 1];
     const int n = int(b - a);
     if (n <= 1) return;
-    if (n <= kWave) {
-        const int v = lane < n ? vals[a + lane] : INT_MAX;
-        int rank = 0;
-        for (int j = 0; j < n; ++j) {
-            const int o = __shfl(v, j);
-            rank += (o < v) || (o == v && j < lane);
+    if (n <= 4 * kSortLanes) {  // every lane of the group takes part in the shuffles below
+        int v[4], rank[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int idx = gl + k * kSortLanes;
+            v[k] = idx < n ? vals[a + idx] : INT_MAX;
         }
-        if (lane < n) vals[a + rank] = v;
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            const int m = min(n - kb * kSortLanes, kSortLanes);
+            for (int src = 0; src < m; ++src) {
+                const int o = __shfl(v[kb], src, kSortLanes);
+                const int j = kb * kSortLanes + src;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) rank[k] += (o < v[k]) || (o == v[k] && j < gl + k * kSortLanes);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (gl + k * kSortLanes < n) vals[a + rank[k]] = v[k];
         return;
     }
-    if (lane != 0) return;
+    if (gl != 0) return;
     for (long long i = a + 1; i < b; ++i) {
         int v = vals[i];
         long long j = i - 1;
@@ -1027,6 +1041,7 @@ struct SGDev {
     float4* x;    // rt(out - in), out.rt - in.rt, deltaPhi(out, out - in), deltaPhi(in, out)   (anchors)
     float2* e;    // deltaPhi(edge, edge - in) for the high / low strip edges of an endcap-2S outer anchor
     int* flags;   // 1 inner barrel, 2 outer barrel, 4 inner PS, 8 outer PS, 16 outer endcap 2S
+    int* key;     // counter bin of the inner MD's lower module: event * kNBins + layer bin
 };
 
 // Per-segment quantities that the tracklet algebra reads for either role.
@@ -1042,6 +1057,7 @@ __global__ void k_sg_extra(Count cSG, int NM, const ModDev* mods, HitsDev H, MDD
     const bool ec2s = Mo.subdet == SUB_ENDCAP && Mo.type == TYPE_2S;
     S.flags[s] = (Mi.subdet == SUB_BARREL ? 1 : 0) | (Mo.subdet == SUB_BARREL ? 2 : 0) | (Mi.type == TYPE_PS ? 4 : 0) |
                  (Mo.type == TYPE_PS ? 8 : 0) | (ec2s ? 16 : 0);
+    S.key[s] = (D.slot[mi] / NM) * kNBins + layerBin(Mi);
     float2 e = make_float2(0.f, 0.f);
     if (ec2s) {
         const int an = D.anchor[mo];
@@ -1828,6 +1844,7 @@ struct CounterIn {
     const int *tlIn, *tclCount;                              // per TL
     const long long* tcCand;  // per TL: TC candidates of the layer-pair form (null: the module-map count)
     long long* connSum;       // per slot, [3][NS]: SGs, TPs, TLs on the modules connected to the slot's module
+    const int* sgKey;         // per SG: event * kNBins + layer bin of its inner MD's lower module
 };
 
 __device__ __forceinline__ long long connected(const CounterIn& C, int slot, const long long* slotStart) {
@@ -1888,13 +1905,12 @@ __device__ __forceinline__ void waveBinAdd(unsigned long long* bins, int key, lo
 }
 
 __global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long long* tot, unsigned long long* counters) {
-    __shared__ unsigned long long ldsBins[kMaxLdsBins];
+    extern __shared__ unsigned long long ldsBins[];  // nb entries when they fit kMaxLdsBins (else none)
     const int nb = C.E * kNCounters * kNBins;
     const bool lds = nb <= kMaxLdsBins;
     if (lds)
         for (int b = threadIdx.x; b < nb; b += blockDim.x) ldsBins[b] = 0;
     __syncthreads();
-    unsigned long long* bins = lds ? ldsBins : counters;
     const bool aborted = tot[TOT_ABORT] != 0;
     const long long nSlot = aborted || !(C.stages & 1) ? 0 : C.NS;
     const long long nMd = aborted || !(C.stages & 2) ? 0 : tot[TOT_MD];
@@ -1906,22 +1922,23 @@ __global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long lon
     // block-uniform loop: every lane reaches the wave-wide adds
     for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
         long long j = base + threadIdx.x;
-        int slot = -1;
+        int key = 0;  // event * kNBins + layer bin
         int w[4] = {0, 0, 0, 0};
         long long v[4] = {0, 0, 0, 0};
         if (j < n) {
             if (j < nSlot) {  // MD candidates / objects per lower slot (Event.cc:257-305)
-                slot = int(j);
+                key = int(j / C.NM) * kNBins + layerBin(C.mods[j % C.NM]);
                 w[0] = 0; v[0] = C.mdCap[j];
                 w[1] = 5; v[1] = C.mdCount[j];
             } else if ((j -= nSlot) < nMd) {  // SG candidates / objects per inner MD (Event.cc:457-522)
-                slot = C.mdSlot[j];
+                const int slot = C.mdSlot[j];
+                key = (slot / C.NM) * kNBins + layerBin(C.mods[slot % C.NM]);
                 w[0] = 1; v[0] = C.sgCap[j];
                 w[1] = 6; v[1] = C.sgCount[j];
             } else if ((j -= nMd) < nSg) {  // TP (Event.cc:539-597) and TL (Event.cc:627-693) per inner SG
-                slot = C.mdSlot[C.sgIn[j]];
+                key = C.sgKey[j];
                 if (C.stages & 4) {
-                    w[0] = 2; v[0] = C.connSum[slot];
+                    w[0] = 2; v[0] = C.connSum[C.mdSlot[C.sgIn[j]]];
                     w[1] = 7; v[1] = C.tpCount[j];
                 }
                 if (C.stages & 8) {
@@ -1929,18 +1946,25 @@ __global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long lon
                     w[3] = 8; v[3] = C.tlCount[j];
                 }
             } else if ((j -= nSg) < nTp) {  // TC candidates per inner TP: TPs at modules connected to its outermost MD (Event.cc:889-982)
-                slot = C.mdSlot[C.sgIn[C.tpIn[j]]];
+                key = C.sgKey[C.tpIn[j]];
                 w[0] = 4; v[0] = C.connSum[C.NS + C.mdSlot[C.sgOut[C.tpOut[j]]]];
             } else {  // TC candidates and objects per inner TL (Event.cc:999-1062; layer pairs: Event.cc:840-872)
                 j -= nTp;
-                slot = C.mdSlot[C.sgIn[C.tlIn[j]]];
-                w[0] = 4; v[0] = C.tcCand ? C.tcCand[j] : C.connSum[2 * C.NS + C.mdSlot[C.sgOut[C.tlIn[j]]]];
+                const int s = C.tlIn[j];
+                key = C.sgKey[s];
+                w[0] = 4; v[0] = C.tcCand ? C.tcCand[j] : C.connSum[2 * C.NS + C.mdSlot[C.sgOut[s]]];
                 w[1] = 9; v[1] = C.tclCount[j];
             }
         }
-        const int key0 = slot >= 0 ? (slot / C.NM) * kNCounters * kNBins + layerBin(C.mods[slot % C.NM]) : 0;
+        const int key0 = (key / kNBins) * kNCounters * kNBins + key % kNBins;
+        if (lds) {  // LDS atomics per lane
 #pragma unroll
-        for (int c = 0; c < 4; ++c) waveBinAdd(bins, key0 + w[c] * kNBins, v[c]);
+            for (int c = 0; c < 4; ++c)
+                if (v[c]) atomicAdd(&ldsBins[key0 + w[c] * kNBins], (unsigned long long)v[c]);
+        } else {  // global atomics: one per distinct bin and wave
+#pragma unroll
+            for (int c = 0; c < 4; ++c) waveBinAdd(counters, key0 + w[c] * kNBins, v[c]);
+        }
     }
     __syncthreads();
     if (lds)
@@ -1970,12 +1994,11 @@ __device__ __forceinline__ void binCount(unsigned long long* bins, int key) {
 // inner TP (Event.cc:893-897,965) in a per-block LDS histogram.
 constexpr int kTcpLanes = 8;
 __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long* start, const int2* info,
-                                                      const int* tpEndList, const int* tpIn, const int* sgIn,
-                                                      const int* mdSlot, int NM, int E, const ModDev* mods, int2* tc,
-                                                      unsigned long long* counters, Guard g) {
+                                                      const int* tpEndList, const int* tpIn, const int* sgKey, int E,
+                                                      int2* tc, unsigned long long* counters, Guard g) {
     if (g.over()) return;
     const int nTL = int(cTL.get());
-    __shared__ unsigned long long bins[kMaxLdsBins];
+    extern __shared__ unsigned long long bins[];  // nb entries when they fit kMaxLdsBins (else none)
     const int nb = E * kNBins;
     const bool lds = nb <= kMaxLdsBins;
     if (lds)
@@ -1990,10 +2013,9 @@ __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long
         for (long long q = gl; q < n; q += kTcpLanes) {
             const int t = tpEndList[f.x + q];
             tc[a + q] = make_int2(t, f.y);
-            const int sl = mdSlot[sgIn[tpIn[t]]];
-            const int e = sl / NM, bin = layerBin(mods[sl % NM]);
-            if (lds) binCount(bins, e * kNBins + bin);
-            else atomicAdd(&counters[(e * kNCounters + 9) * kNBins + bin], 1ULL);
+            const int key = sgKey[tpIn[t]];  // the TC's inner TP's inner SG
+            if (lds) binCount(bins, key);
+            else atomicAdd(&counters[((key / kNBins) * kNCounters + 9) * kNBins + key % kNBins], 1ULL);
         }
     }
     __syncthreads();
@@ -2118,6 +2140,7 @@ struct sdl_geom {
     int *dConnStart = nullptr, *dConnList = nullptr;
     // modules ordered by layer bin (then index), and where each bin starts
     int *dLayerMods = nullptr, *dLayerModStart = nullptr;
+    int nCU = 256;  // the device's compute units (bounded grids)
 };
 
 struct sdl_mlp {
@@ -2223,7 +2246,7 @@ struct sdl_event {
     DBuf<SGF> sgSlabF, sgF;
     DBuf<float4> sgX;
     DBuf<float2> sgE;
-    DBuf<int> sgFlags;
+    DBuf<int> sgFlags, sgKey;
     DBuf<int> sgEndCount, sgEndFill, sgEndList;
     DBuf<long long> sgEndStart;
     DBuf<int> tpCount, tpIn, tpOut, tpeCount, tpEndList;
@@ -2394,7 +2417,7 @@ MDDev mdView(sdl_event* ev) {
                  ev->mdQ.p};
 }
 SGDev sgView(sdl_event* ev) {
-    return SGDev{ev->sgIn.p, ev->sgOut.p, ev->sgF.p, ev->sgX.p, ev->sgE.p, ev->sgFlags.p};
+    return SGDev{ev->sgIn.p, ev->sgOut.p, ev->sgF.p, ev->sgX.p, ev->sgE.p, ev->sgFlags.p, ev->sgKey.p};
 }
 
 Count hostCount(sdl_event* ev, long long n) { return Count{n, ev->tot.p, -1}; }
@@ -2487,7 +2510,7 @@ int runPrep(sdl_event* ev) {
         return r;
     hipLaunchKernelGGL(k_hit_scatter, dim3(gridFor(nhGrid, kBlock)), dim3(kBlock), 0, st, ev->dEvOff.p, ev->E,
                        ev->hslot.p, ev->slotHitStart.p, ev->slotFill.p, ev->sortedHit.p);
-    hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(NS, kWavesPerBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+    hipLaunchKernelGGL(k_sort_segments, dim3(gridFor((long long)NS * kSortLanes, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->slotHitStart.p, ev->sortedHit.p);
     ev->hitsReady = true;
     return 0;
@@ -2605,7 +2628,7 @@ int runSG(sdl_event* ev, bool layers) {
     if (nSG < 0) return fail(SDL_ERR_HIP, "readback failed");
     ev->nSG = nSG;
     const long long ns = std::max(nSG, 1LL);
-    for (DBuf<int>* b : {&ev->sgIn, &ev->sgOut, &ev->sgFlags, &ev->sgEndList})
+    for (DBuf<int>* b : {&ev->sgIn, &ev->sgOut, &ev->sgFlags, &ev->sgKey, &ev->sgEndList})
         if (int r = b->ensure(ns)) return r;
     if (int r = ev->sgF.ensure(ns)) return r;
     if (int r = ev->sgX.ensure(ns)) return r;
@@ -2629,7 +2652,7 @@ int runSG(sdl_event* ev, bool layers) {
     if (nSG > 0) {
         hipLaunchKernelGGL(k_scatter_by, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
                            ev->sgEndStart.p, ev->sgEndFill.p, ev->sgEndList.p);
-        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, cMD,
+        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nMD * kSortLanes, kBlock)), dim3(kBlock), 0, st, cMD,
                            ev->sgEndStart.p, ev->sgEndList.p);
     }
     return 0;
@@ -2812,7 +2835,7 @@ int runTL(sdl_event* ev) {
                            devCount(ev, TOT_TLCANDS), ev->tlCand.p, ev->tlPair.p, ev->tlFlag.p, ev->tlStart.p,
                            ev->tlFill.p, ev->tlIn.p, ev->tlOut.p, guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
     if (nSG > 0)
-        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG,
+        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nSG * kSortLanes, kBlock)), dim3(kBlock), 0, st, cSG,
                            ev->tlStart.p, ev->tlOut.p);
     hipLaunchKernelGGL(k_tl_bounds, dim3(1), dim3(64), 0, st, ev->tlPOff.p, ev->tlCOff.p, ns, ev->tot.p);
     timerEnd(ev, T_TL);
@@ -2851,9 +2874,9 @@ int runTCP(sdl_event* ev) {
     ev->nTCP = nTCP;
     if (int r = ev->tcp.ensure(std::max(nTCP, 1LL))) return r;
     if (nTCP > 0)
-        hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(nTL, kBlock / kTcpLanes), 65536u)), dim3(kBlock), 0, st, cTL,
-                           ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p, ev->sgIn.p, ev->mdSlot.p, G->NM,
-                           ev->E, G->dMods, ev->tcp.p, ev->counters.p, guardOf(ev, TOT_TCP, nTCP, SDL_STAGE_TCP));
+        hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(nTL, kBlock / kTcpLanes), 65536u)), dim3(kBlock), ev->E * kNBins <= kMaxLdsBins ? size_t(ev->E) * kNBins * 8 : 0, st, cTL,
+                           ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p, ev->sgKey.p, ev->E,
+                           ev->tcp.p, ev->counters.p, guardOf(ev, TOT_TCP, nTCP, SDL_STAGE_TCP));
     return 0;
 }
 
@@ -3064,6 +3087,7 @@ int launchCounters(sdl_event* ev, int s0, int s1) {
     C.tlIn = ev->tlIn.p; C.tclCount = ev->tcCount.p;
     if (int r = ev->connSum.ensure(3LL * C.NS)) return r;
     C.connSum = ev->connSum.p;
+    C.sgKey = ev->sgKey.p;
     if (C.stages & (4 | 16 | 32))
         hipLaunchKernelGGL(k_conn_sums, dim3(gridFor((long long)C.NS * kConnLanes, kBlock)), dim3(kBlock), 0, ev->st,
                            C, ev->tot.p);
@@ -3075,8 +3099,13 @@ int launchCounters(sdl_event* ev, int s0, int s1) {
     if (C.stages & (4 | 8)) n += ev->nSG;
     if (C.stages & 16) n += ev->nTP;
     if (C.stages & 32) n += ev->nTL;
-    const unsigned grid = unsigned(std::min<long long>(std::max<long long>(gridFor(n, kBlock), 1), 65536));
-    hipLaunchKernelGGL(k_counters, dim3(grid), dim3(kBlock), 0, ev->st, C, ev->tot.p, ev->counters.p);
+    // with LDS bins, a bounded grid: every block flushes its bins with global atomics
+    const int nb = C.E * kNCounters * kNBins;
+    const bool lds = nb <= kMaxLdsBins;
+    const long long cap = lds ? 4LL * G->nCU : 65536;
+    const unsigned grid = unsigned(std::min<long long>(std::max<long long>(gridFor(n, kBlock), 1), cap));
+    hipLaunchKernelGGL(k_counters, dim3(grid), dim3(kBlock), lds ? size_t(nb) * 8 : 0, ev->st, C, ev->tot.p,
+                       ev->counters.p);
     return 0;
 }
 
@@ -3299,6 +3328,8 @@ int repackMlp(sdl_mlp* m) {
 // ---- geometry: device upload and the binary cache
 int uploadGeom(sdl_geom* G) {
     (void)hipGetDevice(&G->device);
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, G->device) == hipSuccess && cu > 0) G->nCU = cu;
     auto up = [&](void** d, const void* h, size_t bytes) -> int {
         if (hipMalloc(d, std::max<size_t>(bytes, 4)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
         if (bytes && hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(SDL_ERR_HIP, "hipMemcpy failed");
