@@ -1044,8 +1044,11 @@ struct SGDev {
     int* key;     // counter bin of the inner MD's lower module: event * kNBins + layer bin
 };
 
-// Per-segment quantities that the tracklet algebra reads for either role.
-__global__ void k_sg_extra(Count cSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S) {
+// Per-segment quantities that the tracklet algebra reads for either role; also
+// the SGs ending at each MD (sgEndCount, zeroed before) and the triplets of each
+// inner SG (tpCount: the SGs that start at its outer MD, Triplet.cc:50-122).
+__global__ void k_sg_extra(Count cSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
+                           const long long* sgStart, int* sgEndCount, int* tpCount) {
     const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSG) return;
@@ -1058,6 +1061,8 @@ __global__ void k_sg_extra(Count cSG, int NM, const ModDev* mods, HitsDev H, MDD
     S.flags[s] = (Mi.subdet == SUB_BARREL ? 1 : 0) | (Mo.subdet == SUB_BARREL ? 2 : 0) | (Mi.type == TYPE_PS ? 4 : 0) |
                  (Mo.type == TYPE_PS ? 8 : 0) | (ec2s ? 16 : 0);
     S.key[s] = (D.slot[mi] / NM) * kNBins + layerBin(Mi);
+    atomicAdd(&sgEndCount[mo], 1);
+    tpCount[s] = int(sgStart[mo + 1] - sgStart[mo]);
     float2 e = make_float2(0.f, 0.f);
     if (ec2s) {
         const int an = D.anchor[mo];
@@ -1085,20 +1090,21 @@ __global__ void k_sg_repack(Count cMD, const long long* sgCapOff, const int* sgC
 }
 
 // Per-slot ranges of objects ordered by an inner MD: first = start[mdStart[slot]].
-__global__ void k_slot_range(Count cNS, const long long* mdStart, const long long* objStartByMd, long long* slotObjStart) {
+// Per slot: where its SGs start, and how many of its MDs start at least one SG
+// (the tracklet pairs' outer MDs).
+__global__ void k_slot_range(Count cNS, const long long* mdStart, const long long* sgStart, long long* slotSgStart,
+                             int* slotMdSg) {
     const int NS = int(cNS.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s > NS) return;
-    slotObjStart[s] = objStartByMd[mdStart[s]];
+    slotSgStart[s] = sgStart[mdStart[s]];
+    if (s == NS) return;
+    int c = 0;
+    for (long long m = mdStart[s]; m < mdStart[s + 1]; ++m) c += sgStart[m + 1] > sgStart[m];
+    slotMdSg[s] = c;
 }
 
 // SGs ending at each MD (reverse index, used by the joins)
-__global__ void k_count_by(Count cn, const int* key, int* count) {
-    const int n = int(cn.get());
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    atomicAdd(&count[key[i]], 1);
-}
 __global__ void k_scatter_by(Count cn, const int* key, const long long* start, int* fill, int* list) {
     const int n = int(cn.get());
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1112,13 +1118,6 @@ __global__ void k_scatter_by(Count cn, const int* key, const long long* start, i
 // Triplet = two SGs sharing the middle MD (Triplet.cc:50-122, Segment.cc:861-866).
 // The inner SG s ends at MD m; the outer SGs stored at m's module with inner MD m
 // are exactly the dense range of SGs whose inner MD is m.
-__global__ void k_tp_count(Count cSG, const int* sgOut, const long long* sgStart, int* tpCount) {
-    const int nSG = int(cSG.get());
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nSG) return;
-    const int m = sgOut[s];
-    tpCount[s] = int(sgStart[m + 1] - sgStart[m]);
-}
 __global__ void k_tp_write(Count cSG, const int* sgOut, const long long* sgStart, const long long* tpStart, int* tpIn,
                            int* tpOutArr, Guard g) {
     if (g.over()) return;
@@ -1147,16 +1146,6 @@ __global__ void k_tp_write(Count cSG, const int* sgOut, const long long* sgStart
 //  5. survivors grouped by inner SG, sorted by outer SG inside each group.
 __device__ __forceinline__ int tlModuleVariant(int innerFlags, const ModDev& outLoModule) {
     return tlVariant(innerFlags & 1, innerFlags & 2, outLoModule.subdet == SUB_BARREL);
-}
-
-// MDs of a slot that start at least one SG
-__global__ void k_slot_md_sg(Count cNS, const long long* mdStart, const long long* sgStart, int* out) {
-    const int NS = int(cNS.get());
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= NS) return;
-    int c = 0;
-    for (long long m = mdStart[t]; m < mdStart[t + 1]; ++m) c += sgStart[m + 1] > sgStart[m];
-    out[t] = c;
 }
 
 // Per inner SG: TL candidates (for the counters) and pairs per variant.
@@ -2598,6 +2587,9 @@ int runSG(sdl_event* ev, bool layers) {
     for (DBuf<long long>* b : {&ev->sgCapOff, &ev->sgStart, &ev->sgEndStart})
         if (int r = b->ensure(nm + 1)) return r;
     if (int r = ev->slotSgStart.ensure(NS + 1)) return r;
+    if (int r = ev->slotMdSg.ensure(NS)) return r;
+    // reverse index (SGs by outer MD): counted by k_sg_extra, filled after its scan
+    zeroMany(st, {span(ev->sgEndCount.p, nm), span(ev->sgEndFill.p, nm)});
     const Count cMD = devCount(ev, TOT_MD);
     if (nMD > 0 && !layers)
         hipLaunchKernelGGL(k_sg_cap, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, cMD, G->NM, ev->mdSlot.p,
@@ -2628,7 +2620,7 @@ int runSG(sdl_event* ev, bool layers) {
     if (nSG < 0) return fail(SDL_ERR_HIP, "readback failed");
     ev->nSG = nSG;
     const long long ns = std::max(nSG, 1LL);
-    for (DBuf<int>* b : {&ev->sgIn, &ev->sgOut, &ev->sgFlags, &ev->sgKey, &ev->sgEndList})
+    for (DBuf<int>* b : {&ev->sgIn, &ev->sgOut, &ev->sgFlags, &ev->sgKey, &ev->sgEndList, &ev->tpCount})
         if (int r = b->ensure(ns)) return r;
     if (int r = ev->sgF.ensure(ns)) return r;
     if (int r = ev->sgX.ensure(ns)) return r;
@@ -2640,14 +2632,9 @@ int runSG(sdl_event* ev, bool layers) {
                            guardOf(ev, TOT_SG, nSG, SDL_STAGE_SG));
     if (nSG > 0)
         hipLaunchKernelGGL(k_sg_extra, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
-                           hitsView(ev), mdView(ev), sgView(ev));
+                           hitsView(ev), mdView(ev), sgView(ev), ev->sgStart.p, ev->sgEndCount.p, ev->tpCount.p);
     hipLaunchKernelGGL(k_slot_range, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
-                       ev->mdStart.p, ev->sgStart.p, ev->slotSgStart.p);
-    // reverse index: SGs by outer MD
-    zeroMany(st, {span(ev->sgEndCount.p, nm), span(ev->sgEndFill.p, nm)});
-    if (nSG > 0)
-        hipLaunchKernelGGL(k_count_by, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
-                           ev->sgEndCount.p);
+                       ev->mdStart.p, ev->sgStart.p, ev->slotSgStart.p, ev->slotMdSg.p);
     if (int r = scanExclusive<int>(ev->sgEndCount.p, ev->sgEndStart.p, cMD, nMD, ev->scan, st, ev->tot.p, -1)) return r;
     if (nSG > 0) {
         hipLaunchKernelGGL(k_scatter_by, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
@@ -2669,9 +2656,7 @@ int runTP(sdl_event* ev) {
     if (int r = ev->tpStart.ensure(ns + 1)) return r;
     if (int r = ev->slotTpStart.ensure(NS + 1)) return r;
     const Count cSG = devCount(ev, TOT_SG);
-    if (nSG > 0)
-        hipLaunchKernelGGL(k_tp_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
-                           ev->sgStart.p, ev->tpCount.p);
+    // tpCount was written by the SG stage (k_sg_extra)
     if (int r = scanExclusive<int>(ev->tpCount.p, ev->tpStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TP)) return r;
     const long long nTP = sizeOf(ev, TOT_TP);
     if (nTP < 0) return fail(SDL_ERR_HIP, "readback failed");
@@ -2710,13 +2695,11 @@ int runTL(sdl_event* ev) {
     if (int r = ev->tlFill.ensure(ns)) return r;
     if (int r = ev->tlStart.ensure(ns + 1)) return r;
     if (int r = ev->slotTlStart.ensure(NS + 1)) return r;
-    if (int r = ev->slotMdSg.ensure(NS)) return r;
     if (int r = ev->tlNP.ensure(kNVariantsTL * ns)) return r;
     if (int r = ev->tlPOff.ensure(kNVariantsTL * ns + 1)) return r;
     const Count cSG = devCount(ev, TOT_SG);
     timerBegin(ev, T_TL);
-    hipLaunchKernelGGL(k_slot_md_sg, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS), ev->mdStart.p,
-                       ev->sgStart.p, ev->slotMdSg.p);
+    // slotMdSg was written by the SG stage (k_slot_range)
     if (int r = ev->slotMdSgStart.ensure(NS + 1)) return r;
     if (int r = ev->slotMdSgList.ensure(std::max(ev->nMD, 1LL))) return r;
     if (int r = scanExclusive<int>(ev->slotMdSg.p, ev->slotMdSgStart.p, hostCount(ev, NS), NS, ev->scan, st, ev->tot.p,

@@ -8,14 +8,15 @@ counters).
         [--counters C.csv ...]
 
 A chain is the run of kernels that ends with k_counters; its stages start at
-the first kernel each stage launches (the order of sdl_amd/csrc/sdl_amd.hip's
-runStage).  Chains that launch k_tl_stage1_scored are the learned form.
+a marker kernel of each stage (the order of sdl_amd/csrc/sdl_amd.hip's
+runStage; a scan that opens a stage, such as the triplet counts' scan, is
+counted with the stage before it).  Chains that launch k_tl_stage1_scored are the learned form.
 """
 import argparse
 import collections
 import csv
 
-STAGE_START = (("k_md_cap", "md"), ("k_sg_cap", "sg"), ("k_tp_count", "tp"), ("k_slot_md_sg(", "tl"),
+STAGE_START = (("k_md_cap", "md"), ("k_sg_cap", "sg"), ("k_tp_write", "tp"), ("k_slot_md_sg_list", "tl"),
                ("k_tpe_count", "tcp"), ("k_tcl_count", "tcl"), ("k_counters", "counters"))
 STAGES = ("prep", "md", "sg", "tp", "tl", "tcp", "tcl", "counters")
 
