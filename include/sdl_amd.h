@@ -138,6 +138,22 @@ int sdl_event_layer_counts(sdl_event* ev, int e, unsigned long long out[110]);
  * *n receives the number of objects; returns SDL_ERR_CAPACITY if cap is short. */
 int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n);
 
+/* Per-object values of event e's stored MDs, SGs or TLs (stage SDL_STAGE_MD,
+ * _SG or _TL; also the objects of the layer-pair, navigation and pseudo-MD
+ * stages): what the reference keeps on each stored object, its typed getters
+ * and setRecoVars maps (MiniDoublet.h:85-97 / MiniDoublet.cc:113-197,
+ * Segment.h:128-149 / Segment.cc:104-284, Tracklet.cc:43-98,
+ * TrackletBase.cc:50-60), which the reference's users read through the
+ * objects' getters.  sdl_object_var_count(stage) floats per object (MD 7,
+ * SG 22, TL 6; 0 for the other stages), named by sdl_object_var_names(stage)
+ * (comma-separated), in the object order of sdl_event_export.  Fields a path
+ * never sets keep the reference's constructor value 0; reco variables a path
+ * never sets are -999, or NaN where the map has no entry (pseudo MDs' miniCut).
+ * Returns SDL_ERR_CAPACITY if cap (floats) is short; out = NULL returns *n. */
+int sdl_object_var_count(int stage);
+const char* sdl_object_var_names(int stage);
+int sdl_event_export_vars(sdl_event* ev, int e, int stage, float* out, long long cap, long long* n);
+
 /* Number of events in the batch. */
 int sdl_event_num_events(const sdl_event* ev);
 

@@ -38,6 +38,9 @@ def lib():
         L.oracle_result_copy.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         L.oracle_result_counters.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_result_free.argtypes = [ctypes.c_void_p]
+        L.oracle_var_count.restype = ctypes.c_int
+        L.oracle_var_count.argtypes = [ctypes.c_int]
+        L.oracle_result_vars.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         L.oracle_explain.restype = ctypes.c_int
         L.oracle_explain.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 2 + [ctypes.c_int, ctypes.c_void_p]
         L.oracle_time.restype = ctypes.c_double
@@ -63,8 +66,20 @@ def _ptrs(ev):
     return [ev.x.ctypes.data, ev.y.ctypes.data, ev.z.ctypes.data, ev.detId.ctypes.data, ev.idx.ctypes.data]
 
 
-def run(paths, ev, libm=GLIBC, stages=6):
-    """Returns ({stage: int32 array [n, width]}, counters dict).  stages = 1..6
+# Per-object variables (oracle_result_vars; the order of oracle/sdl_oracle.h)
+VAR_NAMES = {
+    "MD": ("dz", "shiftedDz", "deltaPhi", "deltaPhiChange", "deltaPhiNoShift", "deltaPhiChangeNoShift", "miniCut"),
+    "SG": ("rtIn", "rtOut", "zIn", "zOut", "zLo", "zHi", "rtLo", "rtHi", "deltaPhi", "deltaPhiMin", "deltaPhiMax",
+           "deltaPhiChange", "deltaPhiMinChange", "deltaPhiMaxChange", "sdCut", "sdSlope", "sdMuls", "sdPVoff",
+           "reco_deltaPhi", "dAlpha_innerMD_segment", "dAlpha_outerMD_segment", "dAlpha_innerMD_outerMD"),
+    "TL": ("betaIn", "betaInCut", "betaOut", "betaOutCut", "deltaBeta", "deltaBetaCut"),
+}
+
+
+def run(paths, ev, libm=GLIBC, stages=6, vars=False):
+    """Returns ({stage: int32 array [n, width]}, counters dict), and with
+    vars=True also {"MD"|"SG"|"TL": float32 array [n, nvars]} (VAR_NAMES), the
+    per-object variables in the same object order.  stages = 1..6
     runs that many module-map stages; LAYER_CHAIN runs the layer-pair chain; a
     list runs those stages in that order (0-5 module-map, 6-8 createSegments,
     createTracklets, createTrackCandidates, 9 createTrackletsViaNavigation,
@@ -88,7 +103,16 @@ def run(paths, ev, libm=GLIBC, stages=6):
         L.oracle_result_counters(r, c.ctypes.data)
         counters = dict(zip(("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC"),
                             [int(v) for v in c]))
-        return out, counters
+        if not vars:
+            return out, counters
+        v = {}
+        for name, st in (("MD", 0), ("SG", 1), ("TL", 3)):
+            nv = L.oracle_var_count(st)
+            a = np.zeros((len(out[name]), nv), np.float32)
+            if len(a):
+                L.oracle_result_vars(r, st, a.ctypes.data)
+            v[name] = a
+        return out, counters, v
     finally:
         L.oracle_result_free(r)
 

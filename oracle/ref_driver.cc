@@ -22,6 +22,9 @@
 //   sdl_ref_driver dumppseudo  <workload> <geomdir> <event0> <nevents> <outfile>
 //     (createPseudoMiniDoubletsFromAnchorModule, then the module-map chain)
 //     (createMiniDoublets, createSegmentsWithModuleMap, createTrackletsViaNavigation)
+//   sdl_ref_driver dumpvars    <workload> <geomdir> <event0> <nevents> <outfile>
+//     (the module-map chain up to the tracklets; every MD, SG and TL with its
+//     typed per-object values and reco variables, as float bit patterns)
 //   sdl_ref_driver time        <workload> <geomdir> <event0> <nevents> <threads> [stages]
 // `stages` limits the chain to its first k stages (1..6, default 6).
 #include <chrono>
@@ -33,6 +36,8 @@
 #include <thread>
 #include <vector>
 #include <atomic>
+#include <cmath>
+#include <map>
 
 #include "Event.h"
 #include "EndcapGeometry.h"
@@ -197,6 +202,57 @@ void dumpEvent(FILE* f, SDL::Event& sev, long ievent) {
                  sev.getNumberOfTriplets(), sev.getNumberOfTracklets(), sev.getNumberOfTrackCandidates());
 }
 
+// A reco variable of an object's map, NaN when the algorithm never set it.
+float recoVar(const std::map<std::string, float>& m, const char* key) {
+    auto it = m.find(key);
+    return it == m.end() ? std::nanf("") : it->second;
+}
+void putBits(FILE* f, float v) {
+    unsigned u;
+    std::memcpy(&u, &v, 4);
+    std::fprintf(f, " %08x", u);
+}
+
+// Per-object values of every MD, SG and TL (MiniDoublet.h, Segment.h,
+// Tracklet.h getters and the setRecoVars maps), reached as in dumpEvent.
+void dumpVars(FILE* f, SDL::Event& sev, long ievent) {
+    std::fprintf(f, "E %ld\n", ievent);
+    for (SDL::Module* m : sev.getModulePtrs()) {
+        for (SDL::MiniDoublet* md : m->getMiniDoubletPtrs()) {
+            std::fprintf(f, "MV");
+            putMD(f, md);
+            const float v[7] = {md->getDz(), md->getShiftedDz(), md->getDeltaPhi(), md->getDeltaPhiChange(),
+                                md->getDeltaPhiNoShift(), md->getDeltaPhiChangeNoShift(),
+                                recoVar(md->getRecoVars(), "miniCut")};
+            for (float x : v) putBits(f, x);
+            std::fprintf(f, "\n");
+        }
+        for (SDL::Segment* sg : m->getSegmentPtrs()) {
+            std::fprintf(f, "SV");
+            putSG(f, sg);
+            const std::map<std::string, float>& r = sg->getRecoVars();
+            const float v[22] = {sg->getRtIn(), sg->getRtOut(), sg->getZIn(), sg->getZOut(), sg->getZLo(), sg->getZHi(),
+                                 sg->getRtLo(), sg->getRtHi(), sg->getDeltaPhi(), sg->getDeltaPhiMin(),
+                                 sg->getDeltaPhiMax(), sg->getDeltaPhiChange(), sg->getDeltaPhiMinChange(),
+                                 sg->getDeltaPhiMaxChange(), recoVar(r, "sdCut"), recoVar(r, "sdSlope"),
+                                 recoVar(r, "sdMuls"), recoVar(r, "sdPVoff"), recoVar(r, "deltaPhi"),
+                                 recoVar(r, "dAlpha_innerMD_segment"), recoVar(r, "dAlpha_outerMD_segment"),
+                                 recoVar(r, "dAlpha_innerMD_outerMD")};
+            for (float x : v) putBits(f, x);
+            std::fprintf(f, "\n");
+        }
+        for (SDL::Tracklet* tl : m->getTrackletPtrs()) {
+            std::fprintf(f, "TV");
+            putSG(f, tl->innerSegmentPtr());
+            putSG(f, tl->outerSegmentPtr());
+            const float v[6] = {tl->getBetaIn(), tl->getBetaInCut(), tl->getBetaOut(), tl->getBetaOutCut(),
+                                tl->getDeltaBeta(), tl->getDeltaBetaCut()};
+            for (float x : v) putBits(f, x);
+            std::fprintf(f, "\n");
+        }
+    }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -259,6 +315,20 @@ int main(int argc, char** argv) {
             fill(sev, ev);
             runLayerChain(sev);
             dumpLayerEvent(f, sev, ie);
+        }
+        std::fclose(f);
+        return 0;
+    }
+    if (mode == "dumpvars") {
+        FILE* f = std::fopen(argv[6], "w");
+        if (!f) return 3;
+        for (long ie = e0; ie < e0 + ne; ++ie) {
+            sdlgen::Event ev;
+            sdlgen::generateEvent(g, cfg, ie, ev);
+            SDL::Event sev;
+            fill(sev, ev);
+            runChain(sev, 4);
+            dumpVars(f, sev, ie);
         }
         std::fclose(f);
         return 0;

@@ -241,6 +241,7 @@ struct MDRec {
     int lo, up, anchor;  // hit indices in the event
     int slot;            // lower module slot
     float dz, shiftedDz, dphi, dphichange;
+    bool pseudo = false;  // made by createPseudoMiniDoubletsFromAnchorModule (no cuts run)
 };
 struct SGRec {
     int mdIn, mdOut;
@@ -266,6 +267,10 @@ struct Event {
     std::vector<PairRec> tps, tls;
     std::vector<std::pair<int, PairRec>> tcs;  // (0 = from triplets, 1 = from tracklets)
     unsigned long long nCand[5] = {0, 0, 0, 0, 0};
+    // While a stored object is re-evaluated for its per-object variables
+    // (objectVars), the algorithms write them here (layouts: kMDVars, kSGVars,
+    // kTLVars); null otherwise.
+    mutable float* rec = nullptr;
 
     // Event::getModule (Event.cc:54-78)
     int getSlot(uint32_t d) {
@@ -421,6 +426,12 @@ struct Event {
             md.dphichange = dPhiChange(lo.p, lo.phi, up.p);
         }
         CUT(2, std::abs(md.dphichange) < miniCut, std::abs(md.dphichange), kNone, miniCut);
+        if (rec) {  // MiniDoublet.cc:259, 327-343, 346, 391-410
+            rec[0] = md.dz; rec[1] = md.shiftedDz; rec[2] = md.dphi; rec[3] = md.dphichange;
+            rec[4] = deltaPhi(lo.phi, up.p);
+            rec[5] = (!tilted || lo.rt < up.rt) ? dPhiChange(lo.p, lo.phi, up.p) : dPhiChange(up.p, up.phi, lo.p);
+            rec[6] = miniCut;
+        }
         return true;
     }
 
@@ -452,6 +463,12 @@ struct Event {
         float dzFrac = float(double(std::abs(dz)) / std::fabs(double(lo.p.z)));
         md.dphichange = md.dphi / dzFrac * (1.f + dzFrac);
         CUT(13, std::abs(md.dphichange) < miniCut, std::abs(md.dphichange), kNone, miniCut);
+        if (rec) {  // MiniDoublet.cc:467, 563-597, 608, 644-645
+            rec[0] = md.dz; rec[1] = md.shiftedDz; rec[2] = md.dphi; rec[3] = md.dphichange;
+            rec[4] = deltaPhi(lo.phi, up.p);
+            rec[5] = rec[4] / dzFrac * (1.f + dzFrac);
+            rec[6] = miniCut;
+        }
         return true;
     }
 
@@ -506,6 +523,8 @@ struct Event {
     bool sgAlphaCuts(const MDRec& inMD, const MDRec& outMD, float sgAlpha, Trace* tr) const {
         float th[3];
         dAlphaThreshold(inMD, outMD, th);
+        if (rec)  // recovars dAlpha_* (Segment.cc:532-534, 801-803)
+            for (int k = 0; k < 3; ++k) rec[19 + k] = th[k];
         float a = inMD.dphichange - sgAlpha;
         CUT(24, std::abs(a) < th[0], std::abs(a), kNone, th[0]);
         float b = outMD.dphichange - sgAlpha;
@@ -534,6 +553,11 @@ struct Event {
         const float dPhi = deltaPhi(ai.phi, ao.p);
         CUT(21, std::abs(dPhi) <= sdCut, std::abs(dPhi), kNone, sdCut);
         sg.dphichange = dPhiChange(ai.p, ai.phi, ao.p);
+        if (rec) {  // Segment.cc:443-460, 480-484, 501 (rt windows, dphi, min/max stay 0)
+            const float v[19] = {rtIn, rtOut, zIn, zOut, zLo, zHi, 0.f, 0.f, 0.f, 0.f, 0.f, sg.dphichange, 0.f, 0.f,
+                                 sdCut, sdSlope, sdMuls, sdPVoff, dPhi};
+            for (int k = 0; k < 19; ++k) rec[k] = v[k];
+        }
         CUT(22, std::abs(sg.dphichange) <= sdCut, std::abs(sg.dphichange), kNone, sdCut);
         return sgAlphaCuts(inMD, outMD, sg.dphichange, tr);
     }
@@ -578,6 +602,12 @@ struct Event {
         sg.dphichange = dPhiPos / dzFrac * (1.f + dzFrac);
         sg.dphichangemin = sg.dphimin / dzFrac * (1.f + dzFrac);
         sg.dphichangemax = sg.dphimax / dzFrac * (1.f + dzFrac);
+        if (rec) {  // Segment.cc:681-682, 721-722, 746-752, 769-771 (z windows stay 0; sdMuls, sdPVoff -999)
+            const float v[19] = {rtIn, rtOut, 0.f, 0.f, 0.f, 0.f, rtLo, rtHi, sg.dphi, sg.dphimin, sg.dphimax,
+                                 sg.dphichange, sg.dphichangemin, sg.dphichangemax, sdCut, sdSlope, -999.f, -999.f,
+                                 dPhiPos};
+            for (int k = 0; k < 19; ++k) rec[k] = v[k];
+        }
         CUT(34, std::abs(sg.dphichange) <= sdCut, std::abs(sg.dphichange), kNone, sdCut);
         return sgAlphaCuts(inMD, outMD, sg.dphichange, tr);
     }
@@ -699,6 +729,7 @@ struct Event {
         const float rt_InSeg = sub(hInUp.p, hInLo.p).rt();
         const float betaIn_cut = lm.asinF(fmin2((-rt_InSeg * 1.f + drt_tl_axis) * k2Rinv1GeVf / kPtCut, kSinAlphaMax)) + (0.02f / drt_InSeg);
         CUT(44, std::abs(betaInRHmin) < betaIn_cut, std::abs(betaInRHmin), kNone, betaIn_cut);
+        if (rec) { rec[0] = betaInRHmin; rec[1] = betaIn_cut; }  // setBetaIn, setBetaInCut (Tracklet.cc:1218-1219)
         float betaAv = 0.5f * (betaIn + betaOut);
         float pt_beta = float(double(drt_tl_axis * k2Rinv1GeVf) / std::sin(double(betaAv)));
         const float pt_betaMax = 7.0f;
@@ -720,11 +751,13 @@ struct Event {
         const float betaOut_cut = float(double(lm.asinF(fmin2(drt_tl_axis * k2Rinv1GeVf / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
                                         std::sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
         CUT(45, std::abs(betaOut) < betaOut_cut, std::abs(betaOut), kNone, betaOut_cut);
+        if (rec) { rec[2] = betaOut; rec[3] = betaOut_cut; }  // Tracklet.cc:1375-1376
         const float dBetaRes = 0.02f / fmin2(sdOut_d, drt_InSeg);
         const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
                                       0.25 * sq(double(std::abs(betaInRHmin - betaInRHmax) + std::abs(betaOutRHmin - betaOutRHmax))));
         const float dBeta = betaIn - betaOut;
         CUT(46, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
+        if (rec) { rec[4] = dBeta; rec[5] = std::sqrt(dBetaCut2); }  // Tracklet.cc:1498-1499
         return true;
     }
 
@@ -819,6 +852,7 @@ struct Event {
                                       0.25 * sq(double(std::abs(betaInRHmin - betaInRHmax) + std::abs(betaOutRHmin - betaOutRHmax))));
         const float dBeta = betaIn - betaOut;
         CUT(58, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
+        if (rec) rec[4] = dBeta;  // the only typed value this variant sets (Tracklet.cc:1977); the rest stay 0
         return true;
     }
 
@@ -889,7 +923,8 @@ struct Event {
         const float sdIn_dr = sub(hInUp.p, hInLo.p).rt();
         const float sdIn_d = hInUp.rt - hInLo.rt;
         const float dr = dr3.rt();
-        // the betaIn cut is computed but never rejects (A-11)
+        // the betaIn cut is computed but never rejects (A-11); its value has no asin (A-12)
+        if (rec) { rec[0] = betaInRHmin; rec[1] = (-sdIn_dr * 1.f + dr) * k2Rinv1GeVf / kPtCut + (0.02f / sdIn_d); }  // 2319-2324
         float betaAv = 0.5f * (betaIn + betaOut);
         float pt_beta = float(double(dr * k2Rinv1GeVf) / std::sin(double(betaAv)));
         const float pt_betaMax = 7.0f;
@@ -908,11 +943,13 @@ struct Event {
         const float betaOut_cut = float(double(lm.asinF(fmin2(dr * k2Rinv1GeVf / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
                                         std::sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
         CUT(67, std::abs(betaOut) < betaOut_cut, std::abs(betaOut), kNone, betaOut_cut);
+        if (rec) { rec[2] = betaOut; rec[3] = betaOut_cut; }  // Tracklet.cc:2499-2500
         const float dBetaRes = 0.02f / fmin2(sdOut_d, sdIn_d);
         const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + 0.f) +
                                       0.25 * sq(double(std::abs(betaInRHmin - betaInRHmax) + std::abs(betaOutRHmin - betaOutRHmax))));
         const float dBeta = betaIn - betaOut;
         CUT(68, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
+        if (rec) { rec[4] = dBeta; rec[5] = std::sqrt(dBetaCut2); }  // Tracklet.cc:2621-2622
         return true;
     }
 
@@ -983,6 +1020,7 @@ struct Event {
                 md.slot = ls;
                 md.dz = md.shiftedDz = md.dphi = md.dphichange = 0;
                 md.anchor = loopLower ? md.lo : md.up;  // MiniDoublet::setAnchorHit (MiniDoublet.cc:60-81)
+                md.pseudo = true;
                 slots[ls].mds.push_back(int(mds.size()));
                 mds.push_back(md);
             }
@@ -1222,6 +1260,47 @@ struct Event {
     // ------------------------------------------------------------- exporting
     void putMD(std::vector<int>& o, int m) const { o.push_back(hits[mds[m].lo].idx); o.push_back(hits[mds[m].up].idx); }
     void putSG(std::vector<int>& o, int s) const { putMD(o, sgs[s].mdIn); putMD(o, sgs[s].mdOut); }
+    // Per-object variables of the stored MDs, SGs or TLs (the typed getters and
+    // reco variables of MiniDoublet.h, Segment.h, Tracklet.h as the reference's
+    // default algorithms leave them on a stored object), kMDVars / kSGVars /
+    // kTLVars floats per object in object order: each object is evaluated once
+    // more with the recording hooks on.  Pseudo MDs keep their zero typed
+    // values and have no miniCut (NaN).
+    std::vector<float> objectVars(int stage) {
+        std::vector<float> out;
+        const int nv = stage == ORACLE_MD ? kMDVars : stage == ORACLE_SG ? kSGVars : stage == ORACLE_TL ? kTLVars : 0;
+        if (nv == 0) return out;
+        float buf[32];
+        rec = buf;
+        auto put = [&] { out.insert(out.end(), buf, buf + nv); };
+        if (stage == ORACLE_MD)
+            for (const MDRec& md : mds) {
+                std::fill(buf, buf + 32, 0.f);
+                if (md.pseudo) {
+                    buf[6] = std::nanf("");
+                } else {
+                    MDRec t;
+                    mdAlgo(md.lo, md.up, t, nullptr);
+                }
+                put();
+            }
+        if (stage == ORACLE_SG)
+            for (const SGRec& sg : sgs) {
+                std::fill(buf, buf + 32, 0.f);
+                SGRec t;
+                sgAlgo(sg.mdIn, sg.mdOut, t, nullptr);
+                put();
+            }
+        if (stage == ORACLE_TL)
+            for (const PairRec& tl : tls) {
+                std::fill(buf, buf + 32, 0.f);
+                tlAlgo(sgs[tl.in], sgs[tl.out], nullptr);
+                put();
+            }
+        rec = nullptr;
+        return out;
+    }
+
     std::vector<int> tuples(int stage) const {
         std::vector<int> o;
         if (stage == ORACLE_MD) for (size_t i = 0; i < mds.size(); ++i) putMD(o, int(i));
@@ -1244,6 +1323,7 @@ struct Event {
 
 struct Result {
     std::vector<int> t[6];
+    std::vector<float> vars[4];  // per-object variables of MDs, SGs and TLs (index = stage)
     unsigned long long counters[10];
 };
 
@@ -1280,6 +1360,7 @@ void* oracle_run(void* geom, int nhits, const float* x, const float* y, const fl
     else ev.run(stages);
     auto* r = new Result();
     for (int s = 0; s < 6; ++s) r->t[s] = ev.tuples(s);
+    for (int s : {ORACLE_MD, ORACLE_SG, ORACLE_TL}) r->vars[s] = ev.objectVars(s);
     for (int s = 0; s < 5; ++s) r->counters[s] = ev.nCand[s];
     r->counters[5] = ev.mds.size();
     r->counters[6] = ev.sgs.size();
@@ -1296,6 +1377,7 @@ void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y,
     ev.runSequence(seq, n);
     auto* r = new Result();
     for (int s = 0; s < 6; ++s) r->t[s] = ev.tuples(s);
+    for (int s : {ORACLE_MD, ORACLE_SG, ORACLE_TL}) r->vars[s] = ev.objectVars(s);
     for (int s = 0; s < 5; ++s) r->counters[s] = ev.nCand[s];
     r->counters[5] = ev.mds.size();
     r->counters[6] = ev.sgs.size();
@@ -1306,6 +1388,15 @@ void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y,
 }
 
 static const int kWidth[6] = {2, 4, 8, 8, 16, 16};
+
+int oracle_var_count(int stage) {
+    return stage == ORACLE_MD ? kMDVars : stage == ORACLE_SG ? kSGVars : stage == ORACLE_TL ? kTLVars : 0;
+}
+void oracle_result_vars(void* res, int stage, float* out) {
+    if (stage < 0 || stage > ORACLE_TL) return;
+    const std::vector<float>& v = static_cast<Result*>(res)->vars[stage];
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(float));
+}
 
 long oracle_result_count(void* res, int stage) {
     return long(static_cast<Result*>(res)->t[stage].size() / kWidth[stage]);

@@ -33,6 +33,18 @@ enum { ORACLE_LIBM_GLIBC = 0, ORACLE_LIBM_CR = 1,
 // Stage indices used by the result accessors.
 enum { ORACLE_MD = 0, ORACLE_SG = 1, ORACLE_TP = 2, ORACLE_TL = 3, ORACLE_TCP = 4, ORACLE_TCL = 5 };
 
+// Per-object variables (oracle_result_vars), in this order:
+// MD (7): getDz, getShiftedDz, getDeltaPhi, getDeltaPhiChange, getDeltaPhiNoShift,
+//         getDeltaPhiChangeNoShift, recovar miniCut (MiniDoublet.h);
+// SG (22): getRtIn, getRtOut, getZIn, getZOut, getZLo, getZHi, getRtLo, getRtHi,
+//         getDeltaPhi, getDeltaPhiMin, getDeltaPhiMax, getDeltaPhiChange,
+//         getDeltaPhiMinChange, getDeltaPhiMaxChange, recovars sdCut, sdSlope,
+//         sdMuls, sdPVoff, deltaPhi, dAlpha_innerMD_segment,
+//         dAlpha_outerMD_segment, dAlpha_innerMD_outerMD (Segment.h);
+// TL (6): getBetaIn, getBetaInCut, getBetaOut, getBetaOutCut, getDeltaBeta,
+//         getDeltaBetaCut (Tracklet.h).
+enum { kMDVars = 7, kSGVars = 22, kTLVars = 6 };
+
 // Geometry in the reference's three text formats (TiltedGeometry.cc:36,
 // EndcapGeometry.cc:48, ModuleConnectionMap.cc:36-44).  Returns null on error.
 void* oracle_geom_load(const char* tiltedPath, const char* endcapPath, const char* mapPath);
@@ -56,6 +68,10 @@ void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y,
 // Tuple width: MD 2, SG 4, TP 8, TL 8, TCP 16, TCL 16 hit indices.
 long oracle_result_count(void* res, int stage);
 void oracle_result_copy(void* res, int stage, int* out);
+// Per-object variables of MDs, SGs and TLs (stage ORACLE_MD, ORACLE_SG,
+// ORACLE_TL): oracle_var_count(stage) floats per object, in object order.
+int oracle_var_count(int stage);
+void oracle_result_vars(void* res, int stage, float* out);
 // out[0..4] candidates (MD, SG, TP, TL, TC), out[5..9] objects (MD, SG, TP, TL, TC)
 void oracle_result_counters(void* res, unsigned long long* out10);
 void oracle_result_free(void* res);

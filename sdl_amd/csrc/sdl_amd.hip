@@ -847,7 +847,8 @@ __device__ __forceinline__ int findEntry(int k, int pre, int nEntries, int& inne
 // One SG candidate (inner MD `in` on module mi, outer MD mdOut on module mO):
 // the segment algorithm with the endcap-2S strip edges of the outer anchor.
 __device__ __forceinline__ bool sgCandidate(const MDView& in, const ModDev& mi, int mdOut, const ModDev& mO,
-                                            bool connFound, const HitsDev& H, const MDDev& D, SGOut& o) {
+                                            bool connFound, const HitsDev& H, const MDDev& D, SGOut& o,
+                                            float* rec = nullptr) {
     const MDView out = loadMD(mdOut, D);
     HitView hiE = out.a, loE = out.a;
     if (mO.subdet == SUB_ENDCAP && mO.type == TYPE_2S) {
@@ -856,7 +857,7 @@ __device__ __forceinline__ bool sgCandidate(const MDView& in, const ModDev& mi, 
         hiE.phi = a.w;
         loE.phi = b.w;
     }
-    return sgAlgo(in, out, mi, mO, connFound, hiE, loE, o);
+    return sgAlgo(in, out, mi, mO, connFound, hiE, loE, o, rec);
 }
 
 // Appends the wave's passing SG candidates, in lane order, to the inner MD's slab.
@@ -2036,6 +2037,59 @@ __device__ __forceinline__ void putSG(int s, const int* sgIn, const int* sgOut, 
     o[0] = idx[lo[a]]; o[1] = idx[up[a]]; o[2] = idx[lo[b]]; o[3] = idx[up[b]];
 }
 // kind: 1 SG, 2 TP/TL pair of SGs, 3 TC of TP pairs, 4 TC of TL pairs
+// ------------------------------------------------------------------ per-object values
+// The reference's typed getters and reco variables of stored MDs, SGs and TLs
+// (MiniDoublet.h, Segment.h, Tracklet.h; layouts kMDVars, kSGVars, kTLVars in
+// sdl_cuts.h): every object of [a, a + n) is evaluated once more with the
+// algorithms' recording on; one thread per object.  Pseudo MDs (no cuts run)
+// keep their zero values and have no miniCut (NaN).
+__global__ void k_vars_md(long long a, long long n, int NM, const ModDev* mods, HitsDev H, MDDev D, bool pseudo,
+                          float* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long m = a + i;
+    float rec[kMDVars] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (pseudo) {
+        rec[6] = __builtin_nanf("");
+    } else {
+        const HitView hl = loadHit(D.lo[m], H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
+        const HitView hu = loadHit(D.up[m], H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
+        MDOut o;
+        mdAlgo(hl, hu, mods[D.slot[m] % NM], o, rec);
+    }
+    for (int k = 0; k < kMDVars; ++k) out[i * kMDVars + k] = rec[k];
+}
+
+__global__ void k_vars_sg(long long a, long long n, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
+                          float* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long s = a + i;
+    const int mi = S.in[s], mo = S.out[s];
+    float rec[kSGVars];
+    for (int k = 0; k < kSGVars; ++k) rec[k] = 0.f;
+    SGOut o;
+    // the connection test only decides pass/fail before any value is computed
+    sgCandidate(loadMD(mi, D), mods[D.slot[mi] % NM], mo, mods[D.slot[mo] % NM], true, H, D, o, rec);
+    for (int k = 0; k < kSGVars; ++k) out[i * kSGVars + k] = rec[k];
+}
+
+__global__ void k_vars_tl(long long a, long long n, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
+                          const int* tlIn, const int* tlOutArr, float* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = tlIn[a + i], so = tlOutArr[a + i];
+    const int mdOutLo = S.in[so];
+    const int flo = S.flags[so];
+    const int V = tlModuleVariant(S.flags[s], mods[D.slot[mdOutLo] % NM]);
+    float rec[kTLVars] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const TLInner I = loadInner(s, D, S);
+    const TLOutLo L = loadOutLo(mdOutLo, flo, D);  // the flags' inner-MD bits are those of every SG at mdOutLo
+    TLCarry c{0.f, 0.f, 0.f};
+    if (tlStage1(V, I, L, c)) tlStage2(V, (flo & 2) != 0, I, L, loadOutUp(so, flo, H, D, S), c, rec);
+    for (int k = 0; k < kTLVars; ++k) out[i * kTLVars + k] = rec[k];
+}
+
 __global__ void k_export(long long a, long long n, int kind, const int* pA, const int* pB, const int2* pC,
                          const int* tpIn, const int* tpOutArr, const int* tlIn, const int* tlOutArr, const int* sgIn,
                          const int* sgOut, const int* lo, const int* up, const int* idx, int* out) {
@@ -2267,6 +2321,7 @@ struct sdl_event {
     DBuf<unsigned long long> counters;
     DBuf<long long> tmp64;
     DBuf<int> exportBuf;
+    DBuf<float> exportVars;
     ScanScratch scan;
     std::vector<unsigned long long> hostCounters;
     // optional per-kernel timing of the combinatorial kernels (prep, MD, SG, TL)
@@ -3699,6 +3754,56 @@ int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, l
                            ev->pidx, ev->exportBuf.p);
     }
     HIP_OK(hipMemcpyAsync(out, ev->exportBuf.p, sizeof(int) * cnt * width[stage], hipMemcpyDeviceToHost, ev->st));
+    HIP_OK(hipStreamSynchronize(ev->st));
+    return 0;
+}
+
+int sdl_object_var_count(int stage) {
+    return stage == SDL_STAGE_MD ? kMDVars : stage == SDL_STAGE_SG ? kSGVars : stage == SDL_STAGE_TL ? kTLVars : 0;
+}
+
+const char* sdl_object_var_names(int stage) {
+    switch (stage) {
+        case SDL_STAGE_MD:
+            return "dz,shiftedDz,deltaPhi,deltaPhiChange,deltaPhiNoShift,deltaPhiChangeNoShift,miniCut";
+        case SDL_STAGE_SG:
+            return "rtIn,rtOut,zIn,zOut,zLo,zHi,rtLo,rtHi,deltaPhi,deltaPhiMin,deltaPhiMax,deltaPhiChange,"
+                   "deltaPhiMinChange,deltaPhiMaxChange,sdCut,sdSlope,sdMuls,sdPVoff,reco_deltaPhi,"
+                   "dAlpha_innerMD_segment,dAlpha_outerMD_segment,dAlpha_innerMD_outerMD";
+        case SDL_STAGE_TL: return "betaIn,betaInCut,betaOut,betaOutCut,deltaBeta,deltaBetaCut";
+        default: return "";
+    }
+}
+
+int sdl_event_export_vars(sdl_event* ev, int e, int stage, float* out, long long cap, long long* n) {
+    if (!ev || !n || e < 0 || e >= ev->E || sdl_object_var_count(stage) == 0) return fail(SDL_ERR_ARG, "bad arguments");
+    unsigned need = 1u << stage;
+    if (stage == SDL_STAGE_MD) need |= 1u << SDL_STAGE_MD_PSEUDO;
+    if (stage == SDL_STAGE_SG) need |= 1u << SDL_STAGE_SG_LAYERS;
+    if (stage == SDL_STAGE_TL) need |= (1u << SDL_STAGE_TL_LAYERS) | (1u << SDL_STAGE_TL_NAV);
+    if (!(ev->ranMask & need)) return fail(SDL_ERR_STATE, "stage has not run");
+    long long a = 0, cnt = 0;
+    if (int r = stageRange(ev, e, stage, a, cnt)) return r;
+    *n = cnt;
+    const int nv = sdl_object_var_count(stage);
+    if (!out) return 0;
+    if (cap < cnt * nv) return fail(SDL_ERR_CAPACITY, "export buffer too small");
+    if (cnt == 0) return 0;
+    if (int r = ev->exportVars.ensure(cnt * nv)) return r;
+    sdl_geom* G = ev->g;
+    const unsigned grid = gridFor(cnt, kBlock);
+    if (stage == SDL_STAGE_MD) {
+        const bool pseudo = !(ev->ranMask & (1u << SDL_STAGE_MD));
+        hipLaunchKernelGGL(k_vars_md, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, G->NM, G->dMods, hitsView(ev),
+                           mdView(ev), pseudo, ev->exportVars.p);
+    } else if (stage == SDL_STAGE_SG) {
+        hipLaunchKernelGGL(k_vars_sg, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, G->NM, G->dMods, hitsView(ev),
+                           mdView(ev), sgView(ev), ev->exportVars.p);
+    } else {
+        hipLaunchKernelGGL(k_vars_tl, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, G->NM, G->dMods, hitsView(ev),
+                           mdView(ev), sgView(ev), ev->tlIn.p, ev->tlOut.p, ev->exportVars.p);
+    }
+    HIP_OK(hipMemcpyAsync(out, ev->exportVars.p, sizeof(float) * cnt * nv, hipMemcpyDeviceToHost, ev->st));
     HIP_OK(hipStreamSynchronize(ev->st));
     return 0;
 }

@@ -164,8 +164,12 @@ struct MDOut {
     float dz, shiftedDz, dphi, dphichange;
 };
 
-// runMiniDoubletDefaultAlgo{Barrel,Endcap} (MiniDoublet.cc:243-684)
-__device__ __forceinline__ bool mdAlgo(const HitView& lo, const HitView& up, const ModDev& m, MDOut& o) {
+// runMiniDoubletDefaultAlgo{Barrel,Endcap} (MiniDoublet.cc:243-684).  rec (a
+// passing MD's per-object values, kMDVars floats): getDz, getShiftedDz,
+// getDeltaPhi, getDeltaPhiChange, the NoShift pair and the miniCut reco variable.
+constexpr int kMDVars = 7, kSGVars = 22, kTLVars = 6;
+__device__ __forceinline__ bool mdAlgo(const HitView& lo, const HitView& up, const ModDev& m, MDOut& o,
+                                       float* rec = nullptr) {
     o.dz = lo.z - up.z;
     o.shiftedDz = 0.f;
     o.dphi = 0.f;
@@ -200,6 +204,13 @@ __device__ __forceinline__ bool mdAlgo(const HitView& lo, const HitView& up, con
             if (!(fabsF(o.dphi) < miniCut)) return false;
             o.dphichange = dPhiChange(lo.x, lo.y, lo.phi, up.x, up.y);
         }
+        if (rec) {  // MiniDoublet.cc:259, 327-343, 346, 391-410
+            rec[0] = o.dz; rec[1] = o.shiftedDz; rec[2] = o.dphi; rec[3] = o.dphichange;
+            rec[4] = dPhi(lo.phi, up.phi);
+            rec[5] = (m.side == SIDE_CENTER || lo.rt < up.rt) ? dPhiChange(lo.x, lo.y, lo.phi, up.x, up.y)
+                                                              : dPhiChange(up.x, up.y, up.phi, lo.x, lo.y);
+            rec[6] = miniCut;
+        }
         return fabsF(o.dphichange) < miniCut;
     }
     // endcap
@@ -221,6 +232,12 @@ __device__ __forceinline__ bool mdAlgo(const HitView& lo, const HitView& up, con
     if (!(fabsF(o.dphi) < miniCut)) return false;
     const float dzFrac = float(double(fabsF(dz)) / double(fabsF(lo.z)));
     o.dphichange = o.dphi / dzFrac * (1.f + dzFrac);
+    if (rec) {  // MiniDoublet.cc:467, 563-597, 608, 644-645
+        rec[0] = o.dz; rec[1] = o.shiftedDz; rec[2] = o.dphi; rec[3] = o.dphichange;
+        rec[4] = dPhi(lo.phi, up.phi);
+        rec[5] = rec[4] / dzFrac * (1.f + dzFrac);
+        rec[6] = miniCut;
+    }
     return fabsF(o.dphichange) < miniCut;
 }
 
@@ -241,7 +258,7 @@ struct SGOut {
 // Segment::dAlphaThreshold (Segment.cc:886-994) and the three alpha cuts
 // (Segment.cc:538-592 / 789-853).
 __device__ __forceinline__ bool sgAlphaCuts(const MDView& in, const MDView& out, const ModDev& mi, const ModDev& mo,
-                                            float sgAlpha) {
+                                            float sgAlpha, float* rec = nullptr) {
     const bool inB = mi.subdet == SUB_BARREL, outB = mo.subdet == SUB_BARREL;
     const float sdMuls = inB ? mulsPtScaleBarrel(mi.layer - 1) * 3.f / kPtCut : 0.006f * 3.f / kPtCut;
     const float segmentDr = float(sqrt(sq(out.a.y - in.a.y) + sq(out.a.x - in.a.x)));
@@ -275,6 +292,11 @@ __device__ __forceinline__ bool sgAlphaCuts(const MDView& in, const MDView& out,
     const float thIO = float(double(dAlphaB) + sqrt(base));
     const float thIn = (inB && mi.side == SIDE_CENTER) ? thIO : float(double(dAlphaB) + sqrt(base + sq(lumIn)));
     const float thOut = (outB && mo.side == SIDE_CENTER) ? thIO : float(double(dAlphaB) + sqrt(base + sq(lumOut)));
+    if (rec) {  // recovars dAlpha_* (Segment.cc:532-534, 801-803)
+        rec[19] = thIn;
+        rec[20] = thOut;
+        rec[21] = thIO;
+    }
     if (!(fabsF(in.dphichange - sgAlpha) < thIn)) return false;
     if (!(fabsF(out.dphichange - sgAlpha) < thOut)) return false;
     return fabsF(out.dphichange - in.dphichange) < thIO;
@@ -282,8 +304,14 @@ __device__ __forceinline__ bool sgAlphaCuts(const MDView& in, const MDView& out,
 
 // runSegmentDefaultAlgo{Barrel,Endcap} (Segment.cc:337-859).  connFound: the
 // outer module is in the connection list of the inner one (Segment.cc:627-634).
+// rec (a passing SG's per-object values, kSGVars floats): rtIn, rtOut, zIn,
+// zOut, zLo, zHi, rtLo, rtHi, the six deltaPhi values, then the reco variables
+// sdCut, sdSlope, sdMuls, sdPVoff, deltaPhi and the three dAlpha thresholds;
+// the typed fields a path never sets keep the constructor's 0, its unset reco
+// variables -999 (Segment.cc:38-57, 382-390, 615-623).
 __device__ __forceinline__ bool sgAlgo(const MDView& in, const MDView& out, const ModDev& mi, const ModDev& mo,
-                                       bool connFound, const HitView& outHi, const HitView& outLo, SGOut& o) {
+                                       bool connFound, const HitView& outHi, const HitView& outLo, SGOut& o,
+                                       float* rec = nullptr) {
     o.dphi = o.dphimin = o.dphimax = o.dphichange = o.dphichangemin = o.dphichangemax = 0.f;
     const float rtIn = in.a.rt, rtOut = out.a.rt, zIn = in.a.z, zOut = out.a.z;
     const float sdSlope = out.alpha1;  // asin(min(rtOut k2 / ptCut, 0.95))
@@ -298,8 +326,13 @@ __device__ __forceinline__ bool sgAlgo(const MDView& in, const MDView& out, cons
         const float sdCut = float(double(sdSlope) + sqrt(double(sdMuls * sdMuls + sdPVoff * sdPVoff)));
         if (!(fabsF(dPhi(in.a.phi, out.a.phi)) <= sdCut)) return false;
         o.dphichange = dPhiChange(in.a.x, in.a.y, in.a.phi, out.a.x, out.a.y);
+        if (rec) {  // Segment.cc:443-460, 480-484, 501
+            const float v[19] = {rtIn, rtOut, zIn, zOut, zLo, zHi, 0.f, 0.f, 0.f, 0.f, 0.f, o.dphichange, 0.f, 0.f,
+                                 sdCut, sdSlope, sdMuls, sdPVoff, dPhi(in.a.phi, out.a.phi)};
+            for (int k = 0; k < 19; ++k) rec[k] = v[k];
+        }
         if (!(fabsF(o.dphichange) <= sdCut)) return false;
-        return sgAlphaCuts(in, out, mi, mo, o.dphichange);
+        return sgAlphaCuts(in, out, mi, mo, o.dphichange, rec);
     }
     const bool l2r123 = (mi.ring == 1 || mi.ring == 2 || mi.ring == 3) && mi.layer == 2;
     if (!(connFound || l2r123)) return false;
@@ -324,8 +357,13 @@ __device__ __forceinline__ bool sgAlgo(const MDView& in, const MDView& out, cons
     o.dphichange = dPhiPos / dzFrac * (1.f + dzFrac);
     o.dphichangemin = o.dphimin / dzFrac * (1.f + dzFrac);
     o.dphichangemax = o.dphimax / dzFrac * (1.f + dzFrac);
+    if (rec) {  // Segment.cc:681-682, 721-722, 740-752, 769-771
+        const float v[19] = {rtIn, rtOut, 0.f, 0.f, 0.f, 0.f, rtLo, rtHi, o.dphi, o.dphimin, o.dphimax, o.dphichange,
+                             o.dphichangemin, o.dphichangemax, sdSlope, sdSlope, -999.f, -999.f, dPhiPos};
+        for (int k = 0; k < 19; ++k) rec[k] = v[k];
+    }
     if (!(fabsF(o.dphichange) <= sdSlope)) return false;
-    return sgAlphaCuts(in, out, mi, mo, o.dphichange);
+    return sgAlphaCuts(in, out, mi, mo, o.dphichange, rec);
 }
 
 // ------------------------------------------------------------------ TL
@@ -586,7 +624,10 @@ __device__ __forceinline__ bool tlStage1(int variant, const TLInner& I, const TL
 
 // ---- second stage: cuts that involve the outer segment's outer MD
 // _v2: Tracklet.cc:1097-1100 and 1156-1501
-__device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L, const TLOutUp& U, const TLCarry& c) {
+// rec (a passing TL's per-object values, kTLVars floats): getBetaIn, getBetaInCut,
+// getBetaOut, getBetaOutCut, getDeltaBeta, getDeltaBetaCut.
+__device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L, const TLOutUp& U, const TLCarry& c,
+                                             float* rec = nullptr) {
     const HitView& hInLo = I.lo;
     const HitView& hInUp = I.up;
     const HitView& hOutLo = L.a;
@@ -613,6 +654,7 @@ __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L,
     const float drt_InSeg = I.d;
     const float betaIn_cut = asinF(fmin2((-rt_InSeg * 1.f + drt_tl_axis) * k2 / kPtCut, kSinAlphaMax)) + (0.02f / drt_InSeg);
     if (!(fabsF(betaInRHmin) < betaIn_cut)) return false;
+    if (rec) { rec[0] = betaInRHmin; rec[1] = betaIn_cut; }  // Tracklet.cc:1218-1219
     float betaAv = 0.5f * (betaIn + betaOut);
     float pt_beta = float(double(drt_tl_axis * k2) / sin(double(betaAv)));
     const int lOut = isEC_lastLayer ? 11 : 5;
@@ -630,16 +672,19 @@ __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L,
     const float betaOut_cut = float(double(asinF(fmin2(drt_tl_axis * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
                                     sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
     if (!(fabsF(betaOut) < betaOut_cut)) return false;
+    if (rec) { rec[2] = betaOut; rec[3] = betaOut_cut; }  // Tracklet.cc:1375-1376
     const float dBetaRes = 0.02f / fmin2(sdOut_d, drt_InSeg);
     const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
                                   0.25 * sq(double(fabsF(betaInRHmin - betaInRHmax) + fabsF(betaOutRHmin - betaOutRHmax))));
     const float dBeta = betaIn - betaOut;
+    if (rec) { rec[4] = dBeta; rec[5] = sqrtF(dBetaCut2); }  // Tracklet.cc:1498-1499
     return dBeta * dBeta <= dBetaCut2;
 }
 
 // BBEE: Tracklet.cc:1774-2016; EEEE: Tracklet.cc:2273-2665
 template <bool EEEE>
-__device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, const TLOutUp& U, const TLCarry& c) {
+__device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, const TLOutUp& U, const TLCarry& c,
+                                           float* rec = nullptr) {
     const HitView& hInLo = I.lo;
     const HitView& hInUp = I.up;
     const HitView& hOutLo = L.a;
@@ -682,6 +727,10 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
         const float betaIn_cut = asinF(fmin2((-sdIn_dr * 1.f + dr) * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdIn_d);
         if (!(fabsF(betaInRHmin) < betaIn_cut)) return false;
     }  // in EEEE the betaIn cut never rejects (A-11)
+    if (EEEE && rec) {  // its value has no asin (A-12; Tracklet.cc:2319-2324)
+        rec[0] = betaInRHmin;
+        rec[1] = (-sdIn_dr * 1.f + dr) * k2 / kPtCut + (0.02f / sdIn_d);
+    }
     float betaAv = 0.5f * (betaIn + betaOut);
     float pt_beta = float(double(dr * k2) / sin(double(betaAv)));
     const int lIn = EEEE ? 11 : 5;
@@ -709,20 +758,25 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
     const float betaOut_cut = float(double(asinF(fmin2(dr * k2 / kPtCut, kSinAlphaMax)) + (0.02f / sdOut_d)) +
                                     sqrt(double(dBetaLum2 + dBetaMuls * dBetaMuls)));
     if (!(fabsF(betaOut) < betaOut_cut)) return false;
+    if (EEEE && rec) { rec[2] = betaOut; rec[3] = betaOut_cut; }  // Tracklet.cc:2499-2500
     const float dBetaRes = 0.02f / fmin2(sdOut_d, sdIn_d);
     const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
                                   0.25 * sq(double(fabsF(betaInRHmin - betaInRHmax) + fabsF(betaOutRHmin - betaOutRHmax))));
     const float dBeta = betaIn - betaOut;
+    if (rec) {  // BBEE sets only deltaBeta (Tracklet.cc:1977); EEEE both (2621-2622)
+        rec[4] = dBeta;
+        if (EEEE) rec[5] = sqrtF(dBetaCut2);
+    }
     return dBeta * dBeta <= dBetaCut2;
 }
 
 // bOutUp: the outer segment's outer MD is in the barrel (the fourth subdet of the dispatch)
 __device__ __forceinline__ bool tlStage2(int variant, bool bOutUp, const TLInner& I, const TLOutLo& L,
-                                         const TLOutUp& U, const TLCarry& c) {
-    if (variant == TL_BBBB) return tlStage2BBBB(I, L, U, c);
+                                         const TLOutUp& U, const TLCarry& c, float* rec = nullptr) {
+    if (variant == TL_BBBB) return tlStage2BBBB(I, L, U, c, rec);
     if (bOutUp) return false;
-    if (variant == TL_BBEE) return tlStage2EC<false>(I, L, U, c);
-    return tlStage2EC<true>(I, L, U, c);
+    if (variant == TL_BBEE) return tlStage2EC<false>(I, L, U, c, rec);
+    return tlStage2EC<true>(I, L, U, c, rec);
 }
 
 }  // namespace sdl

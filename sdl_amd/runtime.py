@@ -27,7 +27,8 @@ EXPORTS = (
     "sdl_last_error", "sdl_geom_load", "sdl_geom_free", "sdl_geom_num_modules", "sdl_event_create",
     "sdl_event_destroy", "sdl_event_reset", "sdl_event_add_hits", "sdl_event_next", "sdl_event_set_hits_device",
     "sdl_event_run", "sdl_event_run_chain", "sdl_event_sync", "sdl_event_counts", "sdl_event_layer_counts",
-    "sdl_event_export", "sdl_event_num_events", "sdl_event_stream", "sdl_event_set_timing", "sdl_event_kernel_ms",
+    "sdl_event_export", "sdl_object_var_count", "sdl_object_var_names", "sdl_event_export_vars",
+    "sdl_event_num_events", "sdl_event_stream", "sdl_event_set_timing", "sdl_event_kernel_ms",
     "sdl_event_set_headroom", "sdl_event_set_graphs",
     "sdl_mlp_create", "sdl_mlp_free", "sdl_mlp_num_params", "sdl_mlp_set_params", "sdl_mlp_get_params",
     "sdl_mlp_gradient", "sdl_mlp_train", "sdl_mlp_forward", "sdl_event_set_proposal", "sdl_event_tl_pair_rows", "sdl_event_tl_work",
@@ -67,6 +68,10 @@ def lib():
         L.sdl_event_counts.argtypes = [vp, i, vp]
         L.sdl_event_layer_counts.argtypes = [vp, i, vp]
         L.sdl_event_export.argtypes = [vp, i, i, vp, ll, ctypes.POINTER(ll)]
+        L.sdl_object_var_count.argtypes = [i]
+        L.sdl_object_var_names.restype = ctypes.c_char_p
+        L.sdl_object_var_names.argtypes = [i]
+        L.sdl_event_export_vars.argtypes = [vp, i, i, vp, ll, ctypes.POINTER(ll)]
         L.sdl_event_num_events.argtypes = [vp]
         L.sdl_event_stream.argtypes = [vp]
         L.sdl_event_stream.restype = vp
@@ -96,6 +101,11 @@ def lib():
         L.sdl_event_set_tuning.argtypes = [vp, vp, i]
         _lib = L
     return _lib
+
+
+def var_names(stage):
+    """Names of the per-object values of a stage (sdl_object_var_names)."""
+    return tuple(lib().sdl_object_var_names(stage).decode().split(",")) if lib().sdl_object_var_count(stage) else ()
 
 
 def _check(rc):
@@ -211,6 +221,18 @@ class Event:
         out = np.empty((n.value, WIDTH[name]), np.int32)
         if n.value:
             _check(lib().sdl_event_export(self.h, e, stage, out.ctypes.data, out.size, ctypes.byref(n)))
+        return out
+
+    def export_vars(self, stage, e=0):
+        """Per-object values of the MDs, SGs or TLs of event e (float32 [n, nvars],
+        object order of export; names: var_names(stage))."""
+        L = lib()
+        nv = L.sdl_object_var_count(stage)
+        n = ctypes.c_longlong(0)
+        _check(L.sdl_event_export_vars(self.h, e, stage, None, 0, ctypes.byref(n)))
+        out = np.empty((n.value, nv), np.float32)
+        if n.value:
+            _check(L.sdl_event_export_vars(self.h, e, stage, out.ctypes.data, out.size, ctypes.byref(n)))
         return out
 
     def set_timing(self, on=True):

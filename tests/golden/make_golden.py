@@ -10,9 +10,12 @@ createTrackCandidates) with the per-layer counters in
 ``tests/golden/ref_layer_sets.npz``; and createMiniDoublets,
 createSegmentsWithModuleMap, createTrackletsViaNavigation in
 ``tests/golden/ref_nav_sets.npz``; and createPseudoMiniDoubletsFromAnchorModule
-followed by the module-map chain in ``tests/golden/ref_pseudo_sets.npz``.
+followed by the module-map chain in ``tests/golden/ref_pseudo_sets.npz``; and
+the per-object values of every MD, SG and TL of the module-map chain (typed
+getters and reco variables, as float bit patterns) in
+``tests/golden/ref_vars.npz``.
 
-Usage:  python tests/golden/make_golden.py [layers|nav|pseudo]   (only those fixtures)
+Usage:  python tests/golden/make_golden.py [layers|nav|pseudo|vars]   (only those fixtures)
 """
 import os
 import subprocess
@@ -31,6 +34,8 @@ CASES = [(1, 0, 2), (12, 0, 2), (15, 0, 1)]
 LAYER_CASES = [(1, 0, 2), (12, 0, 1)]
 NAV_CASES = [(1, 0, 2), (12, 0, 2), (15, 0, 1)]
 PSEUDO_CASES = [(1, 0, 2), (12, 0, 1), (15, 0, 1)]
+VAR_CASES = [(1, 0, 2), (12, 0, 1), (15, 0, 1)]
+VAR_WIDTH = {"MV": (2, 7), "SV": (4, 22), "TV": (8, 6)}  # tuple width, values per object
 COUNTERS = ("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")
 
 
@@ -45,6 +50,51 @@ def main():
         nav_fixtures()
     if which in ("all", "pseudo"):
         pseudo_fixtures()
+    if which in ("all", "vars"):
+        var_fixtures()
+
+
+def sorted_rows(keys, bits):
+    """Rows (tuple, value bits) in lexicographic order: a canonical multiset."""
+    rows = np.concatenate([keys.astype(np.int64), bits.astype(np.int64)], axis=1)
+    if len(rows) == 0:
+        return rows
+    return rows[np.lexsort(rows.T[::-1])]
+
+
+def parse_var_dump(path):
+    """{event: {"MV"|"SV"|"TV": (keys int32 [n, w], bits uint32 [n, v])}} of a dumpvars file."""
+    out, cur = {}, None
+    for line in open(path):
+        t = line.split()
+        if not t:
+            continue
+        if t[0] == "E":
+            cur = out.setdefault(int(t[1]), {k: [] for k in VAR_WIDTH})
+            continue
+        w, nv = VAR_WIDTH[t[0]]
+        cur[t[0]].append(([int(x) for x in t[1:1 + w]], [int(x, 16) for x in t[1 + w:1 + w + nv]]))
+    res = {}
+    for e, d in out.items():
+        res[e] = {}
+        for k, rows in d.items():
+            w, nv = VAR_WIDTH[k]
+            keys = np.array([r[0] for r in rows], np.int32).reshape(-1, w)
+            bits = np.array([r[1] for r in rows], np.uint32).reshape(-1, nv)
+            res[e][k] = (keys, bits)
+    return res
+
+
+def var_fixtures():
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for wl, e0, ne in VAR_CASES:
+            dump = os.path.join(tmp, f"v{wl}.txt")
+            subprocess.run([oracle.REF_DRIVER, "dumpvars", str(wl), tmp, str(e0), str(ne), dump], check=True)
+            for e, d in parse_var_dump(dump).items():
+                for k, (keys, bits) in d.items():
+                    out[f"w{wl}_e{e}_{k}"] = sorted_rows(keys, bits)
+    np.savez_compressed(os.path.join(HERE, "ref_vars.npz"), **out)
 
 
 def module_map_fixtures():

@@ -1,0 +1,82 @@
+"""Per-object values on the device (sdl_event_export_vars) against the CPU
+oracle: for every stored MD, SG and TL, the reference's typed getters and reco
+variables (the oracle's values are pinned to the compiled reference in
+test_oracle.py::test_object_vars_match_reference).  The device evaluates the
+float transcendentals correctly rounded, so it is compared bit for bit with the
+oracle's CR policy, object by object (rows keyed by the hit-index tuple)."""
+import numpy as np
+import pytest
+
+import oracle
+from sdl_amd import gen, runtime
+
+pytestmark = pytest.mark.gpu
+
+STAGES = (("MD", runtime.STAGE_MD), ("SG", runtime.STAGE_SG), ("TL", runtime.STAGE_TL))
+
+
+def _rows(keys, values):
+    rows = np.concatenate([keys.astype(np.int64), np.ascontiguousarray(values, np.float32).view(np.uint32).astype(np.int64)],
+                          axis=1)
+    return rows[np.lexsort(rows.T[::-1])] if len(rows) else rows
+
+
+def _device(wl, e, seq=None):
+    g = runtime.Geometry(*gen.geometry(gen.geom_kind(wl))[1])
+    ev = runtime.Event(g)
+    x = gen.event(wl, e)
+    ev.add_hits(x.x, x.y, x.z, x.idx, x.detId)
+    if seq is None:
+        ev.run_chain()
+    else:
+        for s in seq:
+            ev.run(s)
+    ev.sync()
+    return ev, x
+
+
+def _compare(ev, sets, values, what, stages=STAGES):
+    for name, st in stages:
+        got = _rows(ev.export(st), ev.export_vars(st))
+        ref = _rows(sets[name], values[name])
+        assert len(ref) > 0, f"{what} {name}: no objects"
+        assert got.shape == ref.shape, (what, name, got.shape, ref.shape)
+        bad = np.flatnonzero((got != ref).any(axis=1))
+        assert len(bad) == 0, f"{what} {name}: {len(bad)} objects differ; first device {got[bad[0]]} oracle {ref[bad[0]]}"
+
+
+def test_var_names_and_counts():
+    assert runtime.var_names(runtime.STAGE_MD) == oracle.VAR_NAMES["MD"]
+    assert runtime.var_names(runtime.STAGE_SG) == oracle.VAR_NAMES["SG"]
+    assert runtime.var_names(runtime.STAGE_TL) == oracle.VAR_NAMES["TL"]
+    assert runtime.var_names(runtime.STAGE_TP) == ()
+
+
+@pytest.mark.parametrize("wl,e", [(1, 0), (12, 0), (15, 0), (15, 1)])
+def test_object_vars_match_oracle(wl, e):
+    print(f"case vars workload={wl} event={e}", flush=True)
+    ev, x = _device(wl, e)
+    sets, _, values = oracle.run(gen.geometry(gen.geom_kind(wl))[1], x, oracle.CR, vars=True)
+    _compare(ev, sets, values, f"w{wl}e{e}")
+
+
+def test_object_vars_pu200():
+    print("case vars PU200 workload 2 event 0", flush=True)
+    ev, x = _device(2, 0)
+    sets, _, values = oracle.run(gen.geometry(2)[1], x, oracle.CR, vars=True)
+    _compare(ev, sets, values, "pu200")
+
+
+def test_object_vars_layer_chain_and_pseudo():
+    """The layer-pair chain's SGs and TLs, and pseudo MDs (zero values, no
+    miniCut) with the module-map chain on them."""
+    print("case vars layer chain and pseudo MDs, workload 12 event 0", flush=True)
+    wl, e = 12, 0
+    ev, x = _device(wl, e, seq=[runtime.STAGE_MD, runtime.SG_LAYERS, runtime.TL_LAYERS])
+    sets, _, values = oracle.run(gen.geometry(gen.geom_kind(wl))[1], x, oracle.CR, stages=[0, 6, 7], vars=True)
+    _compare(ev, sets, values, "layers")
+    ev, x = _device(wl, e, seq=[runtime.MD_PSEUDO, runtime.STAGE_SG, runtime.STAGE_TP, runtime.STAGE_TL])
+    sets, _, values = oracle.run(gen.geometry(gen.geom_kind(wl))[1], x, oracle.CR, stages=[10, 1, 2, 3], vars=True)
+    _compare(ev, sets, values, "pseudo")
+    md = ev.export_vars(runtime.STAGE_MD)
+    assert (md[:, :6] == 0).all() and np.isnan(md[:, 6]).all()

@@ -1,5 +1,7 @@
 """The CPU oracle against the compiled reference's golden object sets, and
 the libm policy comparison (CPU only)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -254,3 +256,69 @@ def test_oracle_pseudo_md_closed_form(geom_paths):
             assert len(los) == 1 and los[0] == hits_of[dlo][0] and ups == sorted(hits_of[dup])
         assert len(mds) == len(set(mds))
     assert counters["nMDCand"] == counters["nMD"] == len(sets["MD"])
+
+
+VAR_CASES = [(1, 0), (1, 1), (12, 0), (15, 0)]
+VAR_KEYS = (("MV", "MD"), ("SV", "SG"), ("TV", "TL"))
+
+
+def canonical_rows(keys, values):
+    """(hit-index tuple, value bit patterns) rows in lexicographic order."""
+    rows = np.concatenate([keys.astype(np.int64), np.ascontiguousarray(values, np.float32).view(np.uint32).astype(np.int64)],
+                          axis=1)
+    return rows[np.lexsort(rows.T[::-1])] if len(rows) else rows
+
+
+@pytest.mark.parametrize("wl,e", VAR_CASES)
+def test_object_vars_match_reference(wl, e, geom_paths):
+    """Per-object values of every MD, SG and TL (the typed getters and reco
+    variables of MiniDoublet.h, Segment.h and Tracklet.h, as the reference's
+    default algorithms leave them on stored objects): the oracle under the
+    GLIBC policy reproduces the compiled reference's bit patterns, object by
+    object (barrel-only, full-geometry and endcap-heavy hard-mix events: every
+    SG and TL variant)."""
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vars.npz")) as f:
+        golden = {k: f[k] for k in f.files}
+    ev = gen.event(wl, e)
+    sets, _, values = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, stages=4, vars=True)
+    for k, s in VAR_KEYS:
+        ref = golden[f"w{wl}_e{e}_{k}"]
+        assert len(ref) > 0, f"{k}: empty golden set makes the check vacuous"
+        mine = canonical_rows(sets[s], values[s])
+        assert mine.shape == ref.shape, (k, mine.shape, ref.shape)
+        bad = np.flatnonzero((mine != ref).any(axis=1))
+        assert len(bad) == 0, f"{k}: {len(bad)} objects differ, first oracle {mine[bad[0]]} reference {ref[bad[0]]}"
+
+
+def test_object_vars_layout_and_defaults(geom_paths):
+    """What the values must be by construction: the typed fields a variant
+    never sets stay at the constructor's 0 (barrel SGs: rt window and the
+    deltaPhi min/max; endcap SGs: z window), reco variables that are never set
+    on a path are -999 (endcap sdMuls, sdPVoff), BBEE tracklets set only
+    deltaBeta, and the values that are plain hit coordinates equal them."""
+    wl, e = 15, 0
+    ev = gen.event(wl, e)
+    sets, _, v = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, stages=4, vars=True)
+    names = oracle.VAR_NAMES
+    assert v["MD"].shape[1] == len(names["MD"]) and v["SG"].shape[1] == len(names["SG"])
+    assert v["TL"].shape[1] == len(names["TL"])
+    sg = v["SG"]
+    col = {n: i for i, n in enumerate(names["SG"])}
+    endcap = sg[:, col["sdMuls"]] == -999.0
+    barrel = ~endcap
+    assert endcap.any() and barrel.any()
+    assert (sg[barrel][:, [col["rtLo"], col["rtHi"], col["deltaPhi"], col["deltaPhiMin"], col["deltaPhiMax"]]] == 0).all()
+    assert (sg[endcap][:, [col["zIn"], col["zOut"], col["zLo"], col["zHi"]]] == 0).all()
+    assert (sg[endcap][:, col["sdPVoff"]] == -999.0).all()
+    assert (sg[endcap][:, col["sdCut"]] == sg[endcap][:, col["sdSlope"]]).all()  # Segment.cc:740
+    # barrel: zOut lies in [zLo, zHi] and |deltaPhi| <= sdCut (the cuts the SG passed)
+    b = sg[barrel]
+    assert ((b[:, col["zOut"]] >= b[:, col["zLo"]]) & (b[:, col["zOut"]] <= b[:, col["zHi"]])).all()
+    assert (np.abs(b[:, col["reco_deltaPhi"]]) <= b[:, col["sdCut"]]).all()
+    md = v["MD"]
+    assert np.isfinite(md[:, 6]).all() and (np.abs(md[:, 2]) < md[:, 6]).all()  # |dphi| < miniCut
+    tl = v["TL"]
+    bbee = (tl[:, 0] == 0) & (tl[:, 1] == 0) & (tl[:, 5] == 0)
+    assert bbee.any() and (tl[bbee][:, 4] != 0).all()
+    assert (np.abs(tl[~bbee][:, 4]) <= tl[~bbee][:, 5]).all()  # |dBeta| <= sqrt(dBetaCut2)
+    assert (np.abs(tl[~bbee][:, 2]) < tl[~bbee][:, 3]).all()   # |betaOut| < betaOut cut
