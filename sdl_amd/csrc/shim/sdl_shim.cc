@@ -159,20 +159,55 @@ unsigned int Event::getNumberOfTripletsByLayerEndcap(unsigned int l) { return co
 unsigned int Event::getNumberOfTrackletsByLayerEndcap(unsigned int l) { return counter(8, 6 + int(l)); }
 unsigned int Event::getNumberOfTrackCandidatesByLayerEndcap(unsigned int l) { return counter(9, 6 + int(l)); }
 
-template <int W>
-std::vector<std::array<int, W>> Event::objects(int stage) {
-    std::vector<std::array<int, W>> out;
-    // the layer-pair stages store their objects as SG, TL and TCL objects
+// Whether a stage that stores `stage`'s objects ran (the layer-pair stages
+// store theirs as SG, TL and TCL objects, pseudo MDs as MD objects).
+bool Event::exported(int stage) const {
     unsigned need = 1u << stage;
     if (stage == SDL_STAGE_MD) need |= 1u << SDL_STAGE_MD_PSEUDO;
     if (stage == SDL_STAGE_SG) need |= 1u << SDL_STAGE_SG_LAYERS;
     if (stage == SDL_STAGE_TL) need |= (1u << SDL_STAGE_TL_LAYERS) | (1u << SDL_STAGE_TL_NAV);
     if (stage == SDL_STAGE_TCL) need |= 1u << SDL_STAGE_TC_LAYERS;
-    if (!(ran_ & need)) return out;
+    return (ran_ & need) != 0;
+}
+
+template <int W>
+std::vector<std::array<int, W>> Event::objects(int stage) {
+    std::vector<std::array<int, W>> out;
+    if (!exported(stage)) return out;
     long long n = 0;
     check(sdl_event_export(ev_, 0, stage, nullptr, 0, &n), "export");
     out.resize(size_t(n));
     if (n) check(sdl_event_export(ev_, 0, stage, out.front().data(), n * W, &n), "export");
+    return out;
+}
+
+template <int W>
+std::vector<std::array<float, W>> Event::variables(int stage) {
+    std::vector<std::array<float, W>> out;
+    if (sdl_object_var_count(stage) != W)
+        throw std::logic_error("SDL: per-object value width differs from the library's");
+    if (!exported(stage)) return out;
+    long long n = 0;
+    check(sdl_event_export_vars(ev_, 0, stage, nullptr, 0, &n), "export values");
+    out.resize(size_t(n));
+    if (n) check(sdl_event_export_vars(ev_, 0, stage, out.front().data(), n * W, &n), "export values");
+    return out;
+}
+
+std::vector<std::array<float, 7>> Event::getMiniDoubletVariables() { return variables<7>(SDL_STAGE_MD); }
+std::vector<std::array<float, 22>> Event::getSegmentVariables() { return variables<22>(SDL_STAGE_SG); }
+std::vector<std::array<float, 6>> Event::getTrackletVariables() { return variables<6>(SDL_STAGE_TL); }
+
+std::vector<std::string> Event::variableNames(const std::string& objectType) {
+    std::vector<std::string> out;
+    const int stage = objectType == "MD" ? SDL_STAGE_MD
+                      : objectType == "SG" ? SDL_STAGE_SG
+                      : objectType == "TL" ? SDL_STAGE_TL : -1;
+    if (stage < 0) return out;
+    const std::string all = sdl_object_var_names(stage);
+    size_t a = 0;
+    for (size_t b; (b = all.find(',', a)) != std::string::npos; a = b + 1) out.push_back(all.substr(a, b - a));
+    if (!all.empty()) out.push_back(all.substr(a));
     return out;
 }
 

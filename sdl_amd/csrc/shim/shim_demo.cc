@@ -4,9 +4,12 @@
 // Runs the module-map chain, with "layers" the layer-pair chain
 // (createMiniDoublets, createSegments, createTracklets, createTrackCandidates),
 // with "pseudo" the module-map chain on pseudo mini-doublets;
-// prints the counters and a checksum of every object set as one line each.
+// prints the counters, a checksum of every object set, a checksum of the MD, SG
+// and TL objects joined with their per-object values, and the value column
+// names, one line each.
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 
 #include "SDL/Event.h"
@@ -19,6 +22,24 @@ unsigned long long checksum(const V& v) {
         unsigned long long h = 1469598103934665603ULL;
         for (int x : a) h = (h ^ (unsigned)x) * 1099511628211ULL;
         s += h;  // order-independent
+    }
+    return s;
+}
+
+// Order-independent checksum of (object tuple, value bits) rows.
+template <class O, class V>
+unsigned long long checksumWithValues(const O& objs, const V& vals) {
+    if (objs.size() != vals.size()) return 0;
+    unsigned long long s = 0;
+    for (size_t i = 0; i < objs.size(); ++i) {
+        unsigned long long h = 1469598103934665603ULL;
+        for (int x : objs[i]) h = (h ^ (unsigned)x) * 1099511628211ULL;
+        for (float f : vals[i]) {
+            unsigned bits;
+            std::memcpy(&bits, &f, 4);
+            h = (h ^ bits) * 1099511628211ULL;
+        }
+        s += h;
     }
     return s;
 }
@@ -64,5 +85,13 @@ int main(int argc, char** argv) {
     std::printf("checksums %llu %llu %llu %llu %llu %llu\n", checksum(event.getMiniDoublets()),
                 checksum(event.getSegments()), checksum(event.getTriplets()), checksum(event.getTracklets()),
                 checksum(event.getTrackCandidatesFromTriplets()), checksum(event.getTrackCandidatesFromTracklets()));
+    std::printf("valuesums %llu %llu %llu\n",
+                checksumWithValues(event.getMiniDoublets(), event.getMiniDoubletVariables()),
+                checksumWithValues(event.getSegments(), event.getSegmentVariables()),
+                checksumWithValues(event.getTracklets(), event.getTrackletVariables()));
+    std::string names;
+    for (const char* t : {"MD", "SG", "TL"})
+        for (const auto& n : SDL::Event::variableNames(t)) names += std::string(names.empty() ? "" : ",") + t + "." + n;
+    std::printf("valuenames %s\n", names.c_str());
     return 0;
 }

@@ -10,7 +10,8 @@
 //  * only the Default algorithms run on the device; another algorithm value
 //    prints a warning and leaves the stage empty, as the reference does for an
 //    unrecognised value (MiniDoublet.cc:216-220);
-//  * the layer-pair and navigation stage variants are not provided.
+//  * per-object values (the objects' getters and reco variables) are read back
+//    as float rows parallel to the object tuples (getMiniDoubletVariables() etc.).
 #ifndef SDL_AMD_EVENT_H
 #define SDL_AMD_EVENT_H
 
@@ -151,11 +152,28 @@ class Event {
     std::vector<std::array<int, 16>> getTrackCandidatesFromTriplets();
     std::vector<std::array<int, 16>> getTrackCandidatesFromTracklets();
 
+    // Per-object values, one row per object in the order of the matching object
+    // getter above: the typed getters and reco variables of the reference's
+    // MiniDoublet (MiniDoublet.h:85-97, setRecoVars in MiniDoublet.cc:113-197),
+    // Segment (Segment.h:128-149, Segment.cc:104-284) and Tracklet
+    // (TrackletBase.cc:50-60, Tracklet.cc:43-98).  Column names come from
+    // variableNames(); a value a path never sets keeps the reference's default
+    // (0 for typed fields, -999 for reco variables, NaN where the reco map has
+    // no entry).
+    std::vector<std::array<float, 7>> getMiniDoubletVariables();
+    std::vector<std::array<float, 22>> getSegmentVariables();
+    std::vector<std::array<float, 6>> getTrackletVariables();
+    // Column names of the rows above for "MD", "SG" or "TL" (empty otherwise).
+    static std::vector<std::string> variableNames(const std::string& objectType);
+
   private:
     bool run(int stage, int algo, const char* what);
     unsigned int counter(int which, int bin);
     template <int W>
     std::vector<std::array<int, W>> objects(int stage);
+    template <int W>
+    std::vector<std::array<float, W>> variables(int stage);
+    bool exported(int stage) const;
 
     sdl_event* ev_ = nullptr;
     LogLevel logLevel_ = Log_Nothing;
