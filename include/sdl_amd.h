@@ -145,10 +145,13 @@ int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, l
  * Segment.h:128-149 / Segment.cc:104-284, Tracklet.cc:43-98,
  * TrackletBase.cc:50-60), which the reference's users read through the
  * objects' getters.  sdl_object_var_count(stage) floats per object (MD 7,
- * SG 22, TL 6; 0 for the other stages), named by sdl_object_var_names(stage)
+ * SG 22, TL 114: six typed values, then the 108 keys of the tracklet's
+ * reco-variable map, Tracklet.cc:144-296,902-2665,3181-3324, in std::map order,
+ * named reco_<key>; 0 for the other stages), named by sdl_object_var_names(stage)
  * (comma-separated), in the object order of sdl_event_export.  Fields a path
  * never sets keep the reference's constructor value 0; reco variables a path
- * never sets are -999, or NaN where the map has no entry (pseudo MDs' miniCut).
+ * never sets are -999, or NaN where the map has no entry (pseudo MDs' miniCut,
+ * the ten keys only the barrel tracklet algorithm writes).
  * Returns SDL_ERR_CAPACITY if cap (floats) is short; out = NULL returns *n. */
 int sdl_object_var_count(int stage);
 const char* sdl_object_var_names(int stage);

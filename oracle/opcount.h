@@ -6,7 +6,7 @@
 // double -> double; compound assignment rounds back to the left type), so the
 // oracle computes bit-identical results; every arithmetic operation, comparison
 // and math-library call also adds one to opc::ops.  The convention: add, sub,
-// mul, div, compare, abs, copysign = 1; sqrt, sin, cos, tan, asin, atan, atan2
+// mul, div, compare, abs, copysign = 1; sqrt, sin, cos, tan, asin, acos, atan, atan2
 // = 1 each (float or double); casts, negation and isnan/isinf = 0.  min/max in
 // the oracle are a compare and so count 1.  Ternary selects are not visible to
 // a library type and are not counted.
@@ -122,6 +122,7 @@ OPC_F1(sin)
 OPC_F1(cos)
 OPC_F1(tan)
 OPC_F1(asin)
+OPC_F1(acos)
 OPC_F1(atan)
 OPC_F1(abs)
 OPC_F1(fabs)

@@ -67,12 +67,30 @@ def _ptrs(ev):
 
 
 # Per-object variables (oracle_result_vars; the order of oracle/sdl_oracle.h)
+# The keys of a stored tracklet's reco-variable map (Tracklet.cc:152-293 and the
+# _v2 algorithm's own ten), in std::map order.
+TL_KEYS = tuple("""
+    algo alpha1GeV_OutLo betaAv betaAv_0th betaAv_1st betaAv_2nd betaAv_3rd betaAv_4th betaIn
+    betaInRHmax betaInRHmin betaIn_0th betaIn_1st betaIn_1stCorr betaIn_2nd betaIn_2ndCorr
+    betaIn_3rd betaIn_3rdCorr betaIn_4th betaIn_cut betaOut betaOutRHmax betaOutRHmin betaOut_0th
+    betaOut_1st betaOut_1stCorr betaOut_2nd betaOut_2ndCorr betaOut_3rd betaOut_3rdCorr betaOut_4th
+    betaPt_0th betaPt_1st betaPt_2nd betaPt_3rd betaPt_4th betacormode dBeta dBetaCut2 dBetaLum2
+    dBetaMuls dBetaRIn2 dBetaROut2 dBetaRes dBeta_0th dBeta_1st dBeta_2nd dBeta_3rd dBeta_4th
+    dBeta_midPoint dPhi deltaPhiPos deltaZLum dr drt_InSeg drt_tl_axis dzDrtScale hit1_x hit1_y
+    hit2_x hit2_y hit3_x hit3_y hit4_x hit4_y innerSgInnerMdDetId innerSgOuterMdDetId k2Rinv1GeVf
+    kRinv1GeVf kZ outerSgInnerMdDetId outerSgOuterMdDetId pixelPSZpitch ptCut pt_beta pt_betaIn
+    pt_betaOut rawBetaIn rawBetaInCorrection rawBetaOut rawBetaOutCorrection rtHi_point rtIn
+    rtLo_point rtOut rtOut_o_rtIn rtRatio_OutLoInLo rt_InLo rt_OutLo sdIn_alpha sdIn_d
+    sdOut_alphaOut sdOut_d sdlCut sdlSlope sinAlphaMax strip2SZpitch zGeom zHi zHiPointed zIn zLo
+    zLoPointed zOut z_InLo z_OutLo zpitch_InLo zpitch_OutLo
+""".split())
 VAR_NAMES = {
     "MD": ("dz", "shiftedDz", "deltaPhi", "deltaPhiChange", "deltaPhiNoShift", "deltaPhiChangeNoShift", "miniCut"),
     "SG": ("rtIn", "rtOut", "zIn", "zOut", "zLo", "zHi", "rtLo", "rtHi", "deltaPhi", "deltaPhiMin", "deltaPhiMax",
            "deltaPhiChange", "deltaPhiMinChange", "deltaPhiMaxChange", "sdCut", "sdSlope", "sdMuls", "sdPVoff",
            "reco_deltaPhi", "dAlpha_innerMD_segment", "dAlpha_outerMD_segment", "dAlpha_innerMD_outerMD"),
-    "TL": ("betaIn", "betaInCut", "betaOut", "betaOutCut", "deltaBeta", "deltaBetaCut"),
+    "TL": ("betaIn", "betaInCut", "betaOut", "betaOutCut", "deltaBeta", "deltaBetaCut") + tuple(
+        "reco_" + k for k in TL_KEYS),
 }
 
 

@@ -24,7 +24,8 @@
 //     (createMiniDoublets, createSegmentsWithModuleMap, createTrackletsViaNavigation)
 //   sdl_ref_driver dumpvars    <workload> <geomdir> <event0> <nevents> <outfile>
 //     (the module-map chain up to the tracklets; every MD, SG and TL with its
-//     typed per-object values and reco variables, as float bit patterns)
+//     typed per-object values and reco variables, TLs with their whole
+//     reco-variable map, as float bit patterns)
 //   sdl_ref_driver time        <workload> <geomdir> <event0> <nevents> <threads> [stages]
 // `stages` limits the chain to its first k stages (1..6, default 6).
 #include <chrono>
@@ -215,6 +216,26 @@ void putBits(FILE* f, float v) {
 
 // Per-object values of every MD, SG and TL (MiniDoublet.h, Segment.h,
 // Tracklet.h getters and the setRecoVars maps), reached as in dumpEvent.
+// The keys of a stored tracklet's reco-variable map (the union over the three
+// default algorithms, in std::map order); a key a variant never sets is dumped
+// as NaN, like an absent MD key.
+const char* const kTrackletKeys[] = {
+    "algo", "alpha1GeV_OutLo", "betaAv", "betaAv_0th", "betaAv_1st", "betaAv_2nd", "betaAv_3rd", "betaAv_4th",
+    "betaIn", "betaInRHmax", "betaInRHmin", "betaIn_0th", "betaIn_1st", "betaIn_1stCorr", "betaIn_2nd",
+    "betaIn_2ndCorr", "betaIn_3rd", "betaIn_3rdCorr", "betaIn_4th", "betaIn_cut", "betaOut", "betaOutRHmax",
+    "betaOutRHmin", "betaOut_0th", "betaOut_1st", "betaOut_1stCorr", "betaOut_2nd", "betaOut_2ndCorr",
+    "betaOut_3rd", "betaOut_3rdCorr", "betaOut_4th", "betaPt_0th", "betaPt_1st", "betaPt_2nd", "betaPt_3rd",
+    "betaPt_4th", "betacormode", "dBeta", "dBetaCut2", "dBetaLum2", "dBetaMuls", "dBetaRIn2", "dBetaROut2",
+    "dBetaRes", "dBeta_0th", "dBeta_1st", "dBeta_2nd", "dBeta_3rd", "dBeta_4th", "dBeta_midPoint", "dPhi",
+    "deltaPhiPos", "deltaZLum", "dr", "drt_InSeg", "drt_tl_axis", "dzDrtScale", "hit1_x", "hit1_y", "hit2_x",
+    "hit2_y", "hit3_x", "hit3_y", "hit4_x", "hit4_y", "innerSgInnerMdDetId", "innerSgOuterMdDetId",
+    "k2Rinv1GeVf", "kRinv1GeVf", "kZ", "outerSgInnerMdDetId", "outerSgOuterMdDetId", "pixelPSZpitch", "ptCut",
+    "pt_beta", "pt_betaIn", "pt_betaOut", "rawBetaIn", "rawBetaInCorrection", "rawBetaOut",
+    "rawBetaOutCorrection", "rtHi_point", "rtIn", "rtLo_point", "rtOut", "rtOut_o_rtIn", "rtRatio_OutLoInLo",
+    "rt_InLo", "rt_OutLo", "sdIn_alpha", "sdIn_d", "sdOut_alphaOut", "sdOut_d", "sdlCut", "sdlSlope",
+    "sinAlphaMax", "strip2SZpitch", "zGeom", "zHi", "zHiPointed", "zIn", "zLo", "zLoPointed", "zOut", "z_InLo",
+    "z_OutLo", "zpitch_InLo", "zpitch_OutLo"};
+
 void dumpVars(FILE* f, SDL::Event& sev, long ievent) {
     std::fprintf(f, "E %ld\n", ievent);
     for (SDL::Module* m : sev.getModulePtrs()) {
@@ -248,6 +269,7 @@ void dumpVars(FILE* f, SDL::Event& sev, long ievent) {
             const float v[6] = {tl->getBetaIn(), tl->getBetaInCut(), tl->getBetaOut(), tl->getBetaOutCut(),
                                 tl->getDeltaBeta(), tl->getDeltaBetaCut()};
             for (float x : v) putBits(f, x);
+            for (const char* k : kTrackletKeys) putBits(f, recoVar(tl->getRecoVars(), k));
             std::fprintf(f, "\n");
         }
     }

@@ -45,7 +45,33 @@ struct Libm {
     float sinF(float v) const { return mode == ORACLE_LIBM_GLIBC ? std::sin(v) : pick(0.f, std::sin(double(v))); }
     float cosF(float v) const { return mode == ORACLE_LIBM_GLIBC ? std::cos(v) : pick(0.f, std::cos(double(v))); }
     float tanF(float v) const { return mode == ORACLE_LIBM_GLIBC ? std::tan(v) : pick(0.f, std::tan(double(v))); }
+    float acosF(float v) const { return mode == ORACLE_LIBM_GLIBC ? std::acos(v) : pick(0.f, std::acos(double(v))); }
 };
+
+// Keys of a tracklet's reco-variable map (Tracklet.cc:152-248 resets 97 of them
+// to -999, :256-293 sets "algo", _v2 adds ten of its own), in std::map order:
+// column kTLTyped + key of the per-object values.
+enum TLKey {
+    TK_algo, TK_alpha1GeV_OutLo, TK_betaAv, TK_betaAv_0th, TK_betaAv_1st, TK_betaAv_2nd, TK_betaAv_3rd,
+    TK_betaAv_4th, TK_betaIn, TK_betaInRHmax, TK_betaInRHmin, TK_betaIn_0th, TK_betaIn_1st, TK_betaIn_1stCorr,
+    TK_betaIn_2nd, TK_betaIn_2ndCorr, TK_betaIn_3rd, TK_betaIn_3rdCorr, TK_betaIn_4th, TK_betaIn_cut, TK_betaOut,
+    TK_betaOutRHmax, TK_betaOutRHmin, TK_betaOut_0th, TK_betaOut_1st, TK_betaOut_1stCorr, TK_betaOut_2nd,
+    TK_betaOut_2ndCorr, TK_betaOut_3rd, TK_betaOut_3rdCorr, TK_betaOut_4th, TK_betaPt_0th, TK_betaPt_1st,
+    TK_betaPt_2nd, TK_betaPt_3rd, TK_betaPt_4th, TK_betacormode, TK_dBeta, TK_dBetaCut2, TK_dBetaLum2, TK_dBetaMuls,
+    TK_dBetaRIn2, TK_dBetaROut2, TK_dBetaRes, TK_dBeta_0th, TK_dBeta_1st, TK_dBeta_2nd, TK_dBeta_3rd, TK_dBeta_4th,
+    TK_dBeta_midPoint, TK_dPhi, TK_deltaPhiPos, TK_deltaZLum, TK_dr, TK_drt_InSeg, TK_drt_tl_axis, TK_dzDrtScale,
+    TK_hit1_x, TK_hit1_y, TK_hit2_x, TK_hit2_y, TK_hit3_x, TK_hit3_y, TK_hit4_x, TK_hit4_y, TK_innerSgInnerMdDetId,
+    TK_innerSgOuterMdDetId, TK_k2Rinv1GeVf, TK_kRinv1GeVf, TK_kZ, TK_outerSgInnerMdDetId, TK_outerSgOuterMdDetId,
+    TK_pixelPSZpitch, TK_ptCut, TK_pt_beta, TK_pt_betaIn, TK_pt_betaOut, TK_rawBetaIn, TK_rawBetaInCorrection,
+    TK_rawBetaOut, TK_rawBetaOutCorrection, TK_rtHi_point, TK_rtIn, TK_rtLo_point, TK_rtOut, TK_rtOut_o_rtIn,
+    TK_rtRatio_OutLoInLo, TK_rt_InLo, TK_rt_OutLo, TK_sdIn_alpha, TK_sdIn_d, TK_sdOut_alphaOut, TK_sdOut_d,
+    TK_sdlCut, TK_sdlSlope, TK_sinAlphaMax, TK_strip2SZpitch, TK_zGeom, TK_zHi, TK_zHiPointed, TK_zIn, TK_zLo,
+    TK_zLoPointed, TK_zOut, TK_z_InLo, TK_z_OutLo, TK_zpitch_InLo, TK_zpitch_OutLo, kTLKeyCount
+};
+static_assert(int(kTLKeyCount) == int(kTLKeys), "tracklet key count");
+// the keys only runTrackletDefaultAlgoBarrelBarrelBarrelBarrel_v2 writes (absent elsewhere)
+const int kTLv2Keys[] = {TK_alpha1GeV_OutLo, TK_drt_InSeg, TK_drt_tl_axis, TK_rtRatio_OutLoInLo, TK_rt_InLo,
+                         TK_rt_OutLo, TK_z_InLo, TK_z_OutLo, TK_zpitch_InLo, TK_zpitch_OutLo};
 
 // std::min / std::max semantics (NaN handling follows the argument order).
 inline float fmin2(float a, float b) { return (b < a) ? b : a; }
@@ -625,25 +651,63 @@ struct Event {
     }
 
     // ---------------------------------------------------------------- TL
-    // Tracklet::runDeltaBetaIterations (Tracklet.cc:3181-3324): the state the cuts use
+    // Tracklet::runDeltaBetaIterations (Tracklet.cc:3181-3324): the state the
+    // cuts use and, with tk (a tracklet's map keys), the values it records.
     void deltaBetaIterations(float& betaIn, float& betaOut, float& betaAv, float& ptBeta, float sdInDr, float sdOutDr,
-                             float dr, int lIn, float ptBetaMax) const {
+                             float dr, int lIn, float ptBetaMax, float* tk = nullptr) const {
         auto corr = [&](float len, float pt, float signOf) {
             return double(std::copysign(double(lm.asinF(fmin2(len * k2Rinv1GeVf / std::abs(pt), kSinAlphaMax))), double(signOf)));
         };
         auto ptOf = [&](float beta) { return float(double(dr * k2Rinv1GeVf) / std::sin(double(beta))); };
+        // one iteration's record: betaIn_<n>, betaOut_<n>, betaAv_<n>, betaPt_<n>, dBeta_<n>
+        auto put = [&](int in, int out, int av, int pt, int db, float bi, float bo, float ba, float bp) {
+            if (!tk) return;
+            tk[in] = bi; tk[out] = bo; tk[av] = ba; tk[pt] = bp; tk[db] = bi - bo;
+        };
+        auto put0 = [&](float bi, float bo, float ba, float bp) { put(TK_betaIn_0th, TK_betaOut_0th, TK_betaAv_0th, TK_betaPt_0th, TK_dBeta_0th, bi, bo, ba, bp); };
+        auto put1 = [&](float bi, float bo, float ba, float bp) { put(TK_betaIn_1st, TK_betaOut_1st, TK_betaAv_1st, TK_betaPt_1st, TK_dBeta_1st, bi, bo, ba, bp); };
+        auto put2 = [&](float bi, float bo, float ba, float bp) { put(TK_betaIn_2nd, TK_betaOut_2nd, TK_betaAv_2nd, TK_betaPt_2nd, TK_dBeta_2nd, bi, bo, ba, bp); };
+        auto put3 = [&](float bi, float bo, float ba, float bp) { put(TK_betaIn_3rd, TK_betaOut_3rd, TK_betaAv_3rd, TK_betaPt_3rd, TK_dBeta_3rd, bi, bo, ba, bp); };
+        auto put4 = [&](float bi, float bo, float ba, float bp) { put(TK_betaIn_4th, TK_betaOut_4th, TK_betaAv_4th, TK_betaPt_4th, TK_dBeta_4th, bi, bo, ba, bp); };
         if (betaIn * betaOut > 0.f &&
             (std::abs(ptBeta) < 4.f * ptBetaMax || (lIn >= 11 && std::abs(ptBeta) < 8.f * ptBetaMax))) {
+            const float in0 = betaIn, out0 = betaOut;
+            if (tk) {  // Tracklet.cc:3192-3200
+                tk[TK_betacormode] = 1;
+                put0(betaIn, betaOut, betaAv, ptBeta);
+                tk[TK_betaIn_1stCorr] = float(corr(sdInDr, ptBeta, betaIn));
+                tk[TK_betaOut_1stCorr] = float(corr(sdOutDr, ptBeta, betaOut));
+            }
             const float inUpd = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
             const float outUpd = float(double(betaOut) + corr(sdOutDr, ptBeta, betaOut));
             betaAv = 0.5f * (inUpd + outUpd);
             ptBeta = ptOf(betaAv);
+            if (tk) {  // :3207-3213
+                put1(inUpd, outUpd, betaAv, ptBeta);
+                tk[TK_betaIn_2ndCorr] = float(corr(sdInDr, ptBeta, betaIn));
+                tk[TK_betaOut_2ndCorr] = float(corr(sdOutDr, ptBeta, betaOut));
+            }
             betaIn = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
             betaOut = float(double(betaOut) + corr(sdOutDr, ptBeta, betaOut));
             betaAv = 0.5f * (betaIn + betaOut);
             ptBeta = ptOf(betaAv);
+            if (tk) {  // :3221-3239: the 3rd restarts from the 0th values, the 4th from the 3rd
+                put2(betaIn, betaOut, betaAv, ptBeta);
+                tk[TK_betaIn_3rdCorr] = float(corr(sdInDr, ptBeta, betaIn));
+                tk[TK_betaOut_3rdCorr] = float(corr(sdOutDr, ptBeta, betaOut));
+                const float in3 = float(double(in0) + corr(sdInDr, ptBeta, betaIn));
+                const float out3 = float(double(out0) + corr(sdOutDr, ptBeta, betaOut));
+                const float av3 = 0.5f * (in3 + out3);
+                const float pt3 = ptOf(av3);
+                put3(in3, out3, av3, pt3);
+                const float in4 = float(double(in0) + corr(sdInDr, pt3, in3));
+                const float out4 = float(double(out0) + corr(sdOutDr, pt3, out3));
+                const float av4 = 0.5f * (in4 + out4);
+                put4(in4, out4, av4, ptOf(av4));
+            }
         } else if (lIn < 11 && double(std::abs(betaOut)) < 0.2 * double(std::abs(betaIn)) &&
                    std::abs(ptBeta) < 12.f * ptBetaMax) {
+            if (tk) { tk[TK_betacormode] = 2; put0(betaIn, betaOut, betaAv, ptBeta); }  // :3245-3250
             const float ptBetaIn = ptOf(betaIn);
             const float inUpd = float(double(betaIn) + corr(sdInDr, ptBetaIn, betaIn));
             const float outUpd = float(double(betaOut) + corr(sdOutDr, ptBetaIn, betaIn));
@@ -653,7 +717,51 @@ struct Event {
             betaOut = float(double(betaOut) + corr(sdOutDr, ptBeta, betaIn));
             betaAv = 0.5f * (betaIn + betaOut);
             ptBeta = ptOf(betaAv);
+            // :3263-3285: the 1st ... 4th all hold the final values
+            put1(betaIn, betaOut, betaAv, ptBeta); put2(betaIn, betaOut, betaAv, ptBeta);
+            put3(betaIn, betaOut, betaAv, ptBeta); put4(betaIn, betaOut, betaAv, ptBeta);
+        } else if (tk) {  // :3290-3320: nothing changes
+            tk[TK_betacormode] = 3;
+            put0(betaIn, betaOut, betaAv, ptBeta); put1(betaIn, betaOut, betaAv, ptBeta);
+            put2(betaIn, betaOut, betaAv, ptBeta); put3(betaIn, betaOut, betaAv, ptBeta);
+            put4(betaIn, betaOut, betaAv, ptBeta);
         }
+    }
+
+    // The values the BBBB (_v2) and EEEE algorithms record alike at their end
+    // (Tracklet.cc:1399-1493, 2522-2617): the four anchors' x, y and module
+    // detIds (as float), the constants, and the midpoint dBeta.
+    void tlCommonKeys(const SGRec& sIn, const SGRec& sOut, float* tk) const {
+        const HitD* h[4] = {&hits[mds[sIn.mdIn].anchor], &hits[mds[sIn.mdOut].anchor], &hits[mds[sOut.mdIn].anchor],
+                            &hits[mds[sOut.mdOut].anchor]};
+        const int xk[4] = {TK_hit1_x, TK_hit2_x, TK_hit3_x, TK_hit4_x};
+        const int yk[4] = {TK_hit1_y, TK_hit2_y, TK_hit3_y, TK_hit4_y};
+        for (int k = 0; k < 4; ++k) { tk[xk[k]] = h[k]->p.x; tk[yk[k]] = h[k]->p.y; }
+        tk[TK_innerSgInnerMdDetId] = float(slots[h[0]->slot].info.detId);
+        tk[TK_innerSgOuterMdDetId] = float(slots[h[1]->slot].info.detId);
+        tk[TK_outerSgInnerMdDetId] = float(slots[h[2]->slot].info.detId);
+        tk[TK_outerSgOuterMdDetId] = float(slots[h[3]->slot].info.detId);
+        tk[TK_sinAlphaMax] = kSinAlphaMax;
+        tk[TK_deltaZLum] = kDeltaZLum;
+        tk[TK_k2Rinv1GeVf] = k2Rinv1GeVf;
+        tk[TK_kRinv1GeVf] = kRinv1GeVf;
+        tk[TK_pixelPSZpitch] = kPixelPSZpitch;
+        tk[TK_strip2SZpitch] = kStrip2SZpitch;
+        tk[TK_ptCut] = kPtCut;
+        tk[TK_betacormode] = 0;  // the algorithm's own betacormode, never changed from 0 (:1245, :2351)
+        // dBeta_midPoint (Tracklet.cc:1409-1437): turning angles at the two segment midpoints
+        const float x1 = h[0]->p.x, y1 = h[0]->p.y, x2 = h[1]->p.x, y2 = h[1]->p.y;
+        const float x3 = h[2]->p.x, y3 = h[2]->p.y, x4 = h[3]->p.x, y4 = h[3]->p.y;
+        const float inMidX = float(double(x1 + x2) / 2.), inMidY = float(double(y1 + y2) / 2.);
+        const float outMidX = float(double(x3 + x4) / 2.), outMidY = float(double(y3 + y4) / 2.);
+        const float ax = x2 - inMidX, ay = y2 - inMidY;
+        const float bx = outMidX - inMidX, by = outMidY - inMidY;
+        const float cx = x4 - outMidX, cy = y4 - outMidY;
+        const float aMag = float(std::sqrt(double(ax * ax + ay * ay)));
+        const float bMag = float(std::sqrt(double(bx * bx + by * by)));
+        const float cMag = float(std::sqrt(double(cx * cx + cy * cy)));
+        const float ab = ax * bx + ay * by, bc = bx * cx + by * cy;
+        tk[TK_dBeta_midPoint] = lm.acosF(ab / aMag / bMag) - lm.acosF(bc / bMag / cMag);
     }
 
     // Shared tail of the three variants: rescale of the RH ranges and the dBeta cut
@@ -683,6 +791,8 @@ struct Event {
         const float zpitch_OutLo = isPS_OutLo ? kPixelPSZpitch : kStrip2SZpitch;
         const float zHi = z_InLo + (z_InLo + kDeltaZLum) * (rtRatio - 1.f) * (z_InLo < 0.f ? 1.f : dzDrtScale) + (zpitch_InLo + zpitch_OutLo);
         const float zLo = z_InLo + (z_InLo - kDeltaZLum) * (rtRatio - 1.f) * (z_InLo > 0.f ? 1.f : dzDrtScale) - (zpitch_InLo + zpitch_OutLo);
+        float* tk = rec ? rec + kTLTyped : nullptr;
+        if (tk) { tk[TK_z_OutLo] = z_OutLo; tk[TK_zLo] = zLo; tk[TK_zHi] = zHi; }  // Tracklet.cc:992-994
         CUT(40, z_OutLo >= zLo && z_OutLo <= zHi, z_OutLo, zLo, zHi);
         const float drt_OutLo_InLo = rt_OutLo - rt_InLo;
         const float r3_InLo = float(std::sqrt(double(z_InLo * z_InLo + rt_InLo * rt_InLo)));
@@ -700,13 +810,16 @@ struct Event {
         const float zWindow = dzErr / drt_InSeg * drt_OutLo_InLo + (zpitch_InLo + zpitch_OutLo);
         const float zLoP = z_InLo + dzMean * (z_InLo > 0.f ? 1.f : dzDrtScale) - zWindow;
         const float zHiP = z_InLo + dzMean * (z_InLo < 0.f ? 1.f : dzDrtScale) + zWindow;
+        if (tk) { tk[TK_zLoPointed] = zLoP; tk[TK_zHiPointed] = zHiP; }  // :1066-1067
         CUT(41, z_OutLo >= zLoP && z_OutLo <= zHiP, z_OutLo, zLoP, zHiP);
         const float sdlPVoff = 0.1f / rt_OutLo;
         const float sdlCut = float(double(alpha1GeV_OutLo) + std::sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
         const float deltaPhiPos = deltaPhi(hInUp.phi, hOutUp.p);
+        if (tk) { tk[TK_deltaPhiPos] = deltaPhiPos; tk[TK_sdlCut] = sdlCut; }  // :1098-1099
         CUT(42, std::abs(deltaPhiPos) <= sdlCut, std::abs(deltaPhiPos), kNone, sdlCut);
         const P3 mid = midpoint(hInLo.p, hOutLo.p);
         const float dPhi = deltaPhi(mid.phi(), sub(hOutLo.p, hInLo.p));
+        if (tk) tk[TK_dPhi] = dPhi;  // :1152
         CUT(43, std::abs(dPhi) <= sdlCut, std::abs(dPhi), kNone, sdlCut);
         const float alpha_InLo = sIn.dphichange;
         const ModInfo& mOU = slots[mOutUp.slot].info;
@@ -730,13 +843,18 @@ struct Event {
         const float betaIn_cut = lm.asinF(fmin2((-rt_InSeg * 1.f + drt_tl_axis) * k2Rinv1GeVf / kPtCut, kSinAlphaMax)) + (0.02f / drt_InSeg);
         CUT(44, std::abs(betaInRHmin) < betaIn_cut, std::abs(betaInRHmin), kNone, betaIn_cut);
         if (rec) { rec[0] = betaInRHmin; rec[1] = betaIn_cut; }  // setBetaIn, setBetaInCut (Tracklet.cc:1218-1219)
+        if (tk) {  // :1197-1204, 1217 (the RH ranges are recorded again, rescaled, at the end)
+            tk[TK_betaIn_0th] = betaIn; tk[TK_betaOut_0th] = betaOut;
+            tk[TK_rawBetaIn] = betaIn; tk[TK_rawBetaOut] = betaOut;
+            tk[TK_betaIn_cut] = betaIn_cut;
+        }
         float betaAv = 0.5f * (betaIn + betaOut);
         float pt_beta = float(double(drt_tl_axis * k2Rinv1GeVf) / std::sin(double(betaAv)));
         const float pt_betaMax = 7.0f;
         const int lIn = 5, lOut = isEC_lastLayer ? 11 : 5;
         const float sdOut_dr = sub(hOutUp.p, hOutLo.p).rt();
         const float sdOut_d = hOutUp.rt - hOutLo.rt;
-        deltaBetaIterations(betaIn, betaOut, betaAv, pt_beta, rt_InSeg, sdOut_dr, drt_tl_axis, lIn, pt_betaMax);
+        deltaBetaIterations(betaIn, betaOut, betaAv, pt_beta, rt_InSeg, sdOut_dr, drt_tl_axis, lIn, pt_betaMax, tk);
         rescaleRH(betaIn, betaInRHmin, betaInRHmax);
         rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
         const float dBetaMuls = sdlThetaMulsF * 4.f / fmin2(std::abs(pt_beta), pt_betaMax);
@@ -758,6 +876,20 @@ struct Event {
         const float dBeta = betaIn - betaOut;
         CUT(46, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
         if (rec) { rec[4] = dBeta; rec[5] = std::sqrt(dBetaCut2); }  // Tracklet.cc:1498-1499
+        if (tk) {  // :1438-1494
+            tlCommonKeys(sIn, sOut, tk);
+            tk[TK_betaIn] = betaIn; tk[TK_betaInRHmin] = betaInRHmin; tk[TK_betaInRHmax] = betaInRHmax;
+            tk[TK_betaOut] = betaOut; tk[TK_betaOutRHmin] = betaOutRHmin; tk[TK_betaOutRHmax] = betaOutRHmax;
+            tk[TK_dBeta] = dBeta; tk[TK_dBetaCut2] = dBetaCut2; tk[TK_dBetaLum2] = dBetaLum2;
+            tk[TK_dBetaMuls] = dBetaMuls; tk[TK_dBetaRIn2] = 0.f; tk[TK_dBetaROut2] = dBetaROut2;
+            tk[TK_dBetaRes] = dBetaRes; tk[TK_drt_tl_axis] = drt_tl_axis; tk[TK_dzDrtScale] = dzDrtScale;
+            tk[TK_pt_betaIn] = float(double(drt_tl_axis * k2Rinv1GeVf) / std::sin(double(betaIn)));  // :1393-1394
+            tk[TK_pt_betaOut] = float(double(drt_tl_axis * k2Rinv1GeVf) / std::sin(double(betaOut)));
+            tk[TK_rt_InLo] = rt_InLo; tk[TK_rt_OutLo] = rt_OutLo; tk[TK_rtRatio_OutLoInLo] = rtRatio;
+            tk[TK_drt_InSeg] = drt_InSeg; tk[TK_sdOut_d] = sdOut_d; tk[TK_alpha1GeV_OutLo] = alpha1GeV_OutLo;
+            tk[TK_zpitch_InLo] = zpitch_InLo; tk[TK_zpitch_OutLo] = zpitch_OutLo; tk[TK_z_InLo] = z_InLo;
+            tk[TK_pt_beta] = pt_beta; tk[TK_betaAv] = betaAv;
+        }
         return true;
     }
 
@@ -830,7 +962,9 @@ struct Event {
         const int lIn = 5, lOut = 11;
         const float sdOut_dr = sub(hOutUp.p, hOutLo.p).rt();
         const float sdOut_d = hOutUp.rt - hOutLo.rt;
-        deltaBetaIterations(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn, pt_betaMax);
+        // the only map keys this variant writes (Tracklet.cc:1885)
+        deltaBetaIterations(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn, pt_betaMax,
+                            rec ? rec + kTLTyped : nullptr);
         rescaleRH(betaIn, betaInRHmin, betaInRHmax);
         rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
         const float dBetaMuls = sdlThetaMulsF * 4.f / fmin2(std::abs(pt_beta), pt_betaMax);
@@ -925,13 +1059,20 @@ struct Event {
         const float dr = dr3.rt();
         // the betaIn cut is computed but never rejects (A-11); its value has no asin (A-12)
         if (rec) { rec[0] = betaInRHmin; rec[1] = (-sdIn_dr * 1.f + dr) * k2Rinv1GeVf / kPtCut + (0.02f / sdIn_d); }  // 2319-2324
+        float* tk = rec ? rec + kTLTyped : nullptr;
+        if (tk) {  // Tracklet.cc:2303-2308
+            tk[TK_sdIn_alpha] = sdIn_alpha; tk[TK_sdOut_alphaOut] = sdOut_alphaOut;
+            tk[TK_rawBetaInCorrection] = deltaPhi(hInLo.phi, dr3);
+            tk[TK_rawBetaOutCorrection] = deltaPhi(hOutUp.phi, dr3);
+            tk[TK_rawBetaIn] = betaIn; tk[TK_rawBetaOut] = betaOut;
+        }
         float betaAv = 0.5f * (betaIn + betaOut);
         float pt_beta = float(double(dr * k2Rinv1GeVf) / std::sin(double(betaAv)));
         const float pt_betaMax = 7.0f;
         const int lIn = 11, lOut = 13;
         const float sdOut_dr = sub(hOutUp.p, hOutLo.p).rt();
         const float sdOut_d = hOutUp.rt - hOutLo.rt;
-        deltaBetaIterations(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn, pt_betaMax);
+        deltaBetaIterations(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn, pt_betaMax, tk);
         rescaleRH(betaIn, betaInRHmin, betaInRHmax);
         rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
         const float dBetaMuls = sdlThetaMulsF * 4.f / fmin2(std::abs(pt_beta), pt_betaMax);
@@ -950,6 +1091,24 @@ struct Event {
         const float dBeta = betaIn - betaOut;
         CUT(68, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
         if (rec) { rec[4] = dBeta; rec[5] = std::sqrt(dBetaCut2); }  // Tracklet.cc:2621-2622
+        if (tk) {  // :2576-2617
+            tlCommonKeys(sIn, sOut, tk);
+            tk[TK_betaIn] = betaIn; tk[TK_betaInRHmin] = betaInRHmin; tk[TK_betaInRHmax] = betaInRHmax;
+            tk[TK_betaOut] = betaOut; tk[TK_betaOutRHmin] = betaOutRHmin; tk[TK_betaOutRHmax] = betaOutRHmax;
+            tk[TK_dBeta] = dBeta; tk[TK_dBetaCut2] = dBetaCut2; tk[TK_dBetaLum2] = dBetaLum2;
+            tk[TK_dBetaMuls] = dBetaMuls; tk[TK_dBetaRIn2] = 0.f; tk[TK_dBetaROut2] = 0.f;
+            tk[TK_dBetaRes] = dBetaRes; tk[TK_dr] = dr; tk[TK_dzDrtScale] = dzDrtScale;
+            tk[TK_pt_betaIn] = float(double(dr * k2Rinv1GeVf) / std::sin(double(betaIn)));  // :2516-2517
+            tk[TK_pt_betaOut] = float(double(dr * k2Rinv1GeVf) / std::sin(double(betaOut)));
+            const float rtOut_o_rtIn = rtOut / rtIn;
+            const float zGeom = (psInLo ? kPixelPSZpitch : kStrip2SZpitch) + (psOutLo ? kPixelPSZpitch : kStrip2SZpitch);
+            tk[TK_rtIn] = rtIn; tk[TK_rtOut] = rtOut; tk[TK_rtOut_o_rtIn] = rtOut_o_rtIn;
+            tk[TK_sdIn_d] = sdIn_d; tk[TK_sdOut_d] = sdOut_d; tk[TK_sdlSlope] = sdlSlope; tk[TK_zGeom] = zGeom;
+            tk[TK_zIn] = zIn; tk[TK_zOut] = zOut;
+            // :2127, the z window of the barrel algorithm, computed and recorded but not cut on
+            tk[TK_zLo] = zIn + (zIn - kDeltaZLum) * (rtOut_o_rtIn - 1.f) * (zIn > 0.f ? 1.f : dzDrtScale) - zGeom;
+            tk[TK_pt_beta] = pt_beta; tk[TK_betaAv] = betaAv;
+        }
         return true;
     }
 
@@ -959,6 +1118,12 @@ struct Event {
         bool b1 = slots[mds[sIn.mdOut].slot].info.subdet == SUB_BARREL;
         bool b2 = slots[mds[sOut.mdIn].slot].info.subdet == SUB_BARREL;
         bool b3 = slots[mds[sOut.mdOut].slot].info.subdet == SUB_BARREL;
+        if (rec) {  // the map's reset (Tracklet.cc:152-248) and "algo" (:256-293)
+            float* tk = rec + kTLTyped;
+            for (int k = 0; k < kTLKeys; ++k) tk[k] = -999.f;
+            for (int k : kTLv2Keys) tk[k] = std::nanf("");  // absent unless _v2 runs
+            tk[TK_algo] = (b0 && b1 && b2) ? 1.f : (b0 ? 2.f : 3.f);
+        }
         if (b0 && b1 && b2) return tlBBBB(sIn, sOut, tr);          // BBBB, BBBE
         if (b0 && b1 && !b2 && !b3) return tlBBEE(sIn, sOut, tr);  // BBEE
         if (b0 && !b1 && !b2 && !b3) return tlBBEE(sIn, sOut, tr); // BEEE
@@ -1270,12 +1435,12 @@ struct Event {
         std::vector<float> out;
         const int nv = stage == ORACLE_MD ? kMDVars : stage == ORACLE_SG ? kSGVars : stage == ORACLE_TL ? kTLVars : 0;
         if (nv == 0) return out;
-        float buf[32];
+        float buf[kTLVars];
         rec = buf;
         auto put = [&] { out.insert(out.end(), buf, buf + nv); };
         if (stage == ORACLE_MD)
             for (const MDRec& md : mds) {
-                std::fill(buf, buf + 32, 0.f);
+                std::fill(buf, buf + kTLVars, 0.f);
                 if (md.pseudo) {
                     buf[6] = std::nanf("");
                 } else {
@@ -1286,14 +1451,14 @@ struct Event {
             }
         if (stage == ORACLE_SG)
             for (const SGRec& sg : sgs) {
-                std::fill(buf, buf + 32, 0.f);
+                std::fill(buf, buf + kTLVars, 0.f);
                 SGRec t;
                 sgAlgo(sg.mdIn, sg.mdOut, t, nullptr);
                 put();
             }
         if (stage == ORACLE_TL)
             for (const PairRec& tl : tls) {
-                std::fill(buf, buf + 32, 0.f);
+                std::fill(buf, buf + kTLVars, 0.f);
                 tlAlgo(sgs[tl.in], sgs[tl.out], nullptr);
                 put();
             }

@@ -41,9 +41,12 @@ enum { ORACLE_MD = 0, ORACLE_SG = 1, ORACLE_TP = 2, ORACLE_TL = 3, ORACLE_TCP = 
 //         getDeltaPhiMinChange, getDeltaPhiMaxChange, recovars sdCut, sdSlope,
 //         sdMuls, sdPVoff, deltaPhi, dAlpha_innerMD_segment,
 //         dAlpha_outerMD_segment, dAlpha_innerMD_outerMD (Segment.h);
-// TL (6): getBetaIn, getBetaInCut, getBetaOut, getBetaOutCut, getDeltaBeta,
-//         getDeltaBetaCut (Tracklet.h).
-enum { kMDVars = 7, kSGVars = 22, kTLVars = 6 };
+// TL (114): getBetaIn, getBetaInCut, getBetaOut, getBetaOutCut, getDeltaBeta,
+//         getDeltaBetaCut (Tracklet.h), then the 108 keys of the reco-variable
+//         map (TrackletBase.cc:48-60) in std::map order, from "algo" to
+//         "zpitch_OutLo" (oracle.py: VAR_NAMES); a key the TL's variant never
+//         writes is NaN, one it resets and never sets is -999.
+enum { kMDVars = 7, kSGVars = 22, kTLTyped = 6, kTLKeys = 108, kTLVars = kTLTyped + kTLKeys };
 
 // Geometry in the reference's three text formats (TiltedGeometry.cc:36,
 // EndcapGeometry.cc:48, ModuleConnectionMap.cc:36-44).  Returns null on error.

@@ -2082,12 +2082,30 @@ __global__ void k_vars_tl(long long a, long long n, int NM, const ModDev* mods, 
     const int mdOutLo = S.in[so];
     const int flo = S.flags[so];
     const int V = tlModuleVariant(S.flags[s], mods[D.slot[mdOutLo] % NM]);
-    float rec[kTLVars] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    // written in place: the typed values start at the constructor's 0, the map
+    // at its reset (Tracklet.cc:152-248), the _v2 algorithm's own keys absent
+    float* rec = out + i * kTLVars;
+    float* tk = rec + kTLTyped;
+    for (int k = 0; k < kTLTyped; ++k) rec[k] = 0.f;
+    for (int k = 0; k < kTLKeys; ++k) tk[k] = -999.f;
+    const int v2Keys[10] = {TK_alpha1GeV_OutLo, TK_drt_InSeg, TK_drt_tl_axis, TK_rtRatio_OutLoInLo, TK_rt_InLo,
+                            TK_rt_OutLo, TK_z_InLo, TK_z_OutLo, TK_zpitch_InLo, TK_zpitch_OutLo};
+    for (int k = 0; k < 10; ++k) tk[v2Keys[k]] = __builtin_nanf("");
+    tk[TK_algo] = V == TL_BBBB ? 1.f : (V == TL_BBEE ? 2.f : 3.f);  // Tracklet.cc:256-293
     const TLInner I = loadInner(s, D, S);
     const TLOutLo L = loadOutLo(mdOutLo, flo, D);  // the flags' inner-MD bits are those of every SG at mdOutLo
+    const TLOutUp U = loadOutUp(so, flo, H, D, S);
     TLCarry c{0.f, 0.f, 0.f};
-    if (tlStage1(V, I, L, c)) tlStage2(V, (flo & 2) != 0, I, L, loadOutUp(so, flo, H, D, S), c, rec);
-    for (int k = 0; k < kTLVars; ++k) out[i * kTLVars + k] = rec[k];
+    if (tlStage1(V, I, L, c, nullptr, tk) && tlStage2(V, (flo & 2) != 0, I, L, U, c, rec) && V != TL_BBEE) {
+        // the anchors' module detIds: the lower sensor's, or its partner's when the anchor is the upper hit
+        const int md[4] = {S.in[s], S.out[s], mdOutLo, S.out[so]};
+        uint32_t detId[4];
+        for (int k = 0; k < 4; ++k) {
+            const ModDev& lm = mods[D.slot[md[k]] % NM];
+            detId[k] = D.anchor[md[k]] == D.lo[md[k]] ? lm.detId : (lm.partner >= 0 ? mods[lm.partner].detId : 0u);
+        }
+        tlCommonKeys(I.lo, I.up, L.a, U.a, detId, tk);
+    }
 }
 
 __global__ void k_export(long long a, long long n, int kind, const int* pA, const int* pB, const int2* pC,
@@ -3770,7 +3788,27 @@ const char* sdl_object_var_names(int stage) {
             return "rtIn,rtOut,zIn,zOut,zLo,zHi,rtLo,rtHi,deltaPhi,deltaPhiMin,deltaPhiMax,deltaPhiChange,"
                    "deltaPhiMinChange,deltaPhiMaxChange,sdCut,sdSlope,sdMuls,sdPVoff,reco_deltaPhi,"
                    "dAlpha_innerMD_segment,dAlpha_outerMD_segment,dAlpha_innerMD_outerMD";
-        case SDL_STAGE_TL: return "betaIn,betaInCut,betaOut,betaOutCut,deltaBeta,deltaBetaCut";
+        case SDL_STAGE_TL:  // typed values, then the reco-variable map's keys (prefixed reco_)
+            return "betaIn,betaInCut,betaOut,betaOutCut,deltaBeta,deltaBetaCut,"
+                   "reco_algo,reco_alpha1GeV_OutLo,reco_betaAv,reco_betaAv_0th,reco_betaAv_1st,reco_betaAv_2nd,"
+                   "reco_betaAv_3rd,reco_betaAv_4th,reco_betaIn,reco_betaInRHmax,reco_betaInRHmin,reco_betaIn_0th,"
+                   "reco_betaIn_1st,reco_betaIn_1stCorr,reco_betaIn_2nd,reco_betaIn_2ndCorr,reco_betaIn_3rd,"
+                   "reco_betaIn_3rdCorr,reco_betaIn_4th,reco_betaIn_cut,reco_betaOut,reco_betaOutRHmax,"
+                   "reco_betaOutRHmin,reco_betaOut_0th,reco_betaOut_1st,reco_betaOut_1stCorr,reco_betaOut_2nd,"
+                   "reco_betaOut_2ndCorr,reco_betaOut_3rd,reco_betaOut_3rdCorr,reco_betaOut_4th,reco_betaPt_0th,"
+                   "reco_betaPt_1st,reco_betaPt_2nd,reco_betaPt_3rd,reco_betaPt_4th,reco_betacormode,reco_dBeta,"
+                   "reco_dBetaCut2,reco_dBetaLum2,reco_dBetaMuls,reco_dBetaRIn2,reco_dBetaROut2,reco_dBetaRes,"
+                   "reco_dBeta_0th,reco_dBeta_1st,reco_dBeta_2nd,reco_dBeta_3rd,reco_dBeta_4th,reco_dBeta_midPoint,"
+                   "reco_dPhi,reco_deltaPhiPos,reco_deltaZLum,reco_dr,reco_drt_InSeg,reco_drt_tl_axis,"
+                   "reco_dzDrtScale,reco_hit1_x,reco_hit1_y,reco_hit2_x,reco_hit2_y,reco_hit3_x,reco_hit3_y,"
+                   "reco_hit4_x,reco_hit4_y,reco_innerSgInnerMdDetId,reco_innerSgOuterMdDetId,reco_k2Rinv1GeVf,"
+                   "reco_kRinv1GeVf,reco_kZ,reco_outerSgInnerMdDetId,reco_outerSgOuterMdDetId,reco_pixelPSZpitch,"
+                   "reco_ptCut,reco_pt_beta,reco_pt_betaIn,reco_pt_betaOut,reco_rawBetaIn,reco_rawBetaInCorrection,"
+                   "reco_rawBetaOut,reco_rawBetaOutCorrection,reco_rtHi_point,reco_rtIn,reco_rtLo_point,reco_rtOut,"
+                   "reco_rtOut_o_rtIn,reco_rtRatio_OutLoInLo,reco_rt_InLo,reco_rt_OutLo,reco_sdIn_alpha,reco_sdIn_d,"
+                   "reco_sdOut_alphaOut,reco_sdOut_d,reco_sdlCut,reco_sdlSlope,reco_sinAlphaMax,reco_strip2SZpitch,"
+                   "reco_zGeom,reco_zHi,reco_zHiPointed,reco_zIn,reco_zLo,reco_zLoPointed,reco_zOut,reco_z_InLo,"
+                   "reco_z_OutLo,reco_zpitch_InLo,reco_zpitch_OutLo";
         default: return "";
     }
 }

@@ -55,6 +55,7 @@ __device__ __forceinline__ float atanF(float v) { return float(atan(double(v)));
 __device__ __forceinline__ float sinF(float v) { return float(sin(double(v))); }
 __device__ __forceinline__ float cosF(float v) { return float(cos(double(v))); }
 __device__ __forceinline__ float tanF(float v) { return float(tan(double(v))); }
+__device__ __forceinline__ float acosF(float v) { return float(acos(double(v))); }
 __device__ __forceinline__ float sqrtF(float v) { return float(sqrt(double(v))); }
 __device__ __forceinline__ float fabsF(float v) { return v < 0.f ? -v : (v == 0.f ? 0.f : v); }
 // std::min / std::max argument-order semantics (NaN follows the first argument)
@@ -167,7 +168,30 @@ struct MDOut {
 // runMiniDoubletDefaultAlgo{Barrel,Endcap} (MiniDoublet.cc:243-684).  rec (a
 // passing MD's per-object values, kMDVars floats): getDz, getShiftedDz,
 // getDeltaPhi, getDeltaPhiChange, the NoShift pair and the miniCut reco variable.
-constexpr int kMDVars = 7, kSGVars = 22, kTLVars = 6;
+constexpr int kMDVars = 7, kSGVars = 22;
+// TL: the six typed values (getBetaIn, getBetaInCut, getBetaOut, getBetaOutCut,
+// getDeltaBeta, getDeltaBetaCut), then the 108 keys of the reco-variable map
+// (TrackletBase.cc:48-60) in std::map order; a key the TL's variant never writes
+// is NaN, one it resets (Tracklet.cc:152-248) and never sets is -999.
+constexpr int kTLTyped = 6, kTLKeys = 108, kTLVars = kTLTyped + kTLKeys;
+enum TLKey {
+    TK_algo, TK_alpha1GeV_OutLo, TK_betaAv, TK_betaAv_0th, TK_betaAv_1st, TK_betaAv_2nd, TK_betaAv_3rd,
+    TK_betaAv_4th, TK_betaIn, TK_betaInRHmax, TK_betaInRHmin, TK_betaIn_0th, TK_betaIn_1st, TK_betaIn_1stCorr,
+    TK_betaIn_2nd, TK_betaIn_2ndCorr, TK_betaIn_3rd, TK_betaIn_3rdCorr, TK_betaIn_4th, TK_betaIn_cut, TK_betaOut,
+    TK_betaOutRHmax, TK_betaOutRHmin, TK_betaOut_0th, TK_betaOut_1st, TK_betaOut_1stCorr, TK_betaOut_2nd,
+    TK_betaOut_2ndCorr, TK_betaOut_3rd, TK_betaOut_3rdCorr, TK_betaOut_4th, TK_betaPt_0th, TK_betaPt_1st,
+    TK_betaPt_2nd, TK_betaPt_3rd, TK_betaPt_4th, TK_betacormode, TK_dBeta, TK_dBetaCut2, TK_dBetaLum2, TK_dBetaMuls,
+    TK_dBetaRIn2, TK_dBetaROut2, TK_dBetaRes, TK_dBeta_0th, TK_dBeta_1st, TK_dBeta_2nd, TK_dBeta_3rd, TK_dBeta_4th,
+    TK_dBeta_midPoint, TK_dPhi, TK_deltaPhiPos, TK_deltaZLum, TK_dr, TK_drt_InSeg, TK_drt_tl_axis, TK_dzDrtScale,
+    TK_hit1_x, TK_hit1_y, TK_hit2_x, TK_hit2_y, TK_hit3_x, TK_hit3_y, TK_hit4_x, TK_hit4_y, TK_innerSgInnerMdDetId,
+    TK_innerSgOuterMdDetId, TK_k2Rinv1GeVf, TK_kRinv1GeVf, TK_kZ, TK_outerSgInnerMdDetId, TK_outerSgOuterMdDetId,
+    TK_pixelPSZpitch, TK_ptCut, TK_pt_beta, TK_pt_betaIn, TK_pt_betaOut, TK_rawBetaIn, TK_rawBetaInCorrection,
+    TK_rawBetaOut, TK_rawBetaOutCorrection, TK_rtHi_point, TK_rtIn, TK_rtLo_point, TK_rtOut, TK_rtOut_o_rtIn,
+    TK_rtRatio_OutLoInLo, TK_rt_InLo, TK_rt_OutLo, TK_sdIn_alpha, TK_sdIn_d, TK_sdOut_alphaOut, TK_sdOut_d,
+    TK_sdlCut, TK_sdlSlope, TK_sinAlphaMax, TK_strip2SZpitch, TK_zGeom, TK_zHi, TK_zHiPointed, TK_zIn, TK_zLo,
+    TK_zLoPointed, TK_zOut, TK_z_InLo, TK_z_OutLo, TK_zpitch_InLo, TK_zpitch_OutLo, kTLKeyCount
+};
+static_assert(int(kTLKeyCount) == kTLKeys, "tracklet key count");
 __device__ __forceinline__ bool mdAlgo(const HitView& lo, const HitView& up, const ModDev& m, MDOut& o,
                                        float* rec = nullptr) {
     o.dz = lo.z - up.z;
@@ -367,9 +391,20 @@ __device__ __forceinline__ bool sgAlgo(const MDView& in, const MDView& out, cons
 }
 
 // ------------------------------------------------------------------ TL
-// Tracklet::runDeltaBetaIterations (Tracklet.cc:3181-3324): the values the cuts read.
+// Tracklet::runDeltaBetaIterations (Tracklet.cc:3181-3324): the values the cuts
+// read and, with tk (a tracklet's map keys), the values it records.
+__device__ __forceinline__ void tkIter(float* tk, int n, float bi, float bo, float ba, float bp) {
+    // betaIn_<n>, betaOut_<n>, betaAv_<n>, betaPt_<n>, dBeta_<n> of iteration n
+    const int in[5] = {TK_betaIn_0th, TK_betaIn_1st, TK_betaIn_2nd, TK_betaIn_3rd, TK_betaIn_4th};
+    const int out[5] = {TK_betaOut_0th, TK_betaOut_1st, TK_betaOut_2nd, TK_betaOut_3rd, TK_betaOut_4th};
+    const int av[5] = {TK_betaAv_0th, TK_betaAv_1st, TK_betaAv_2nd, TK_betaAv_3rd, TK_betaAv_4th};
+    const int pt[5] = {TK_betaPt_0th, TK_betaPt_1st, TK_betaPt_2nd, TK_betaPt_3rd, TK_betaPt_4th};
+    const int db[5] = {TK_dBeta_0th, TK_dBeta_1st, TK_dBeta_2nd, TK_dBeta_3rd, TK_dBeta_4th};
+    tk[in[n]] = bi; tk[out[n]] = bo; tk[av[n]] = ba; tk[pt[n]] = bp; tk[db[n]] = bi - bo;
+}
+
 __device__ __forceinline__ void deltaBetaIter(float& betaIn, float& betaOut, float& betaAv, float& ptBeta, float sdInDr,
-                                              float sdOutDr, float dr, int lIn) {
+                                              float sdOutDr, float dr, int lIn, float* tk = nullptr) {
     const float ptBetaMax = 7.0f;
     const float k2 = k2Rinv();
     auto corr = [&](float len, float pt, float signOf) {
@@ -378,15 +413,42 @@ __device__ __forceinline__ void deltaBetaIter(float& betaIn, float& betaOut, flo
     auto ptOf = [&](float beta) { return float(double(dr * k2) / sin(double(beta))); };
     if (betaIn * betaOut > 0.f &&
         (fabsF(ptBeta) < 4.f * ptBetaMax || (lIn >= 11 && fabsF(ptBeta) < 8.f * ptBetaMax))) {
+        const float in0 = betaIn, out0 = betaOut;
+        if (tk) {  // Tracklet.cc:3192-3200
+            tk[TK_betacormode] = 1.f;
+            tkIter(tk, 0, betaIn, betaOut, betaAv, ptBeta);
+            tk[TK_betaIn_1stCorr] = float(corr(sdInDr, ptBeta, betaIn));
+            tk[TK_betaOut_1stCorr] = float(corr(sdOutDr, ptBeta, betaOut));
+        }
         const float inUpd = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
         const float outUpd = float(double(betaOut) + corr(sdOutDr, ptBeta, betaOut));
         betaAv = 0.5f * (inUpd + outUpd);
         ptBeta = ptOf(betaAv);
+        if (tk) {  // :3207-3213
+            tkIter(tk, 1, inUpd, outUpd, betaAv, ptBeta);
+            tk[TK_betaIn_2ndCorr] = float(corr(sdInDr, ptBeta, betaIn));
+            tk[TK_betaOut_2ndCorr] = float(corr(sdOutDr, ptBeta, betaOut));
+        }
         betaIn = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
         betaOut = float(double(betaOut) + corr(sdOutDr, ptBeta, betaOut));
         betaAv = 0.5f * (betaIn + betaOut);
         ptBeta = ptOf(betaAv);
+        if (tk) {  // :3221-3239: the 3rd restarts from the 0th values, the 4th from the 3rd
+            tkIter(tk, 2, betaIn, betaOut, betaAv, ptBeta);
+            tk[TK_betaIn_3rdCorr] = float(corr(sdInDr, ptBeta, betaIn));
+            tk[TK_betaOut_3rdCorr] = float(corr(sdOutDr, ptBeta, betaOut));
+            const float in3 = float(double(in0) + corr(sdInDr, ptBeta, betaIn));
+            const float out3 = float(double(out0) + corr(sdOutDr, ptBeta, betaOut));
+            const float av3 = 0.5f * (in3 + out3);
+            const float pt3 = ptOf(av3);
+            tkIter(tk, 3, in3, out3, av3, pt3);
+            const float in4 = float(double(in0) + corr(sdInDr, pt3, in3));
+            const float out4 = float(double(out0) + corr(sdOutDr, pt3, out3));
+            const float av4 = 0.5f * (in4 + out4);
+            tkIter(tk, 4, in4, out4, av4, ptOf(av4));
+        }
     } else if (lIn < 11 && double(fabsF(betaOut)) < 0.2 * double(fabsF(betaIn)) && fabsF(ptBeta) < 12.f * ptBetaMax) {
+        if (tk) { tk[TK_betacormode] = 2.f; tkIter(tk, 0, betaIn, betaOut, betaAv, ptBeta); }  // :3245-3250
         const float ptBetaIn = ptOf(betaIn);
         const float inUpd = float(double(betaIn) + corr(sdInDr, ptBetaIn, betaIn));
         const float outUpd = float(double(betaOut) + corr(sdOutDr, ptBetaIn, betaIn));
@@ -396,7 +458,43 @@ __device__ __forceinline__ void deltaBetaIter(float& betaIn, float& betaOut, flo
         betaOut = float(double(betaOut) + corr(sdOutDr, ptBeta, betaIn));
         betaAv = 0.5f * (betaIn + betaOut);
         ptBeta = ptOf(betaAv);
+        if (tk)  // :3263-3285: the 1st ... 4th all hold the final values
+            for (int n = 1; n < 5; ++n) tkIter(tk, n, betaIn, betaOut, betaAv, ptBeta);
+    } else if (tk) {  // :3290-3320: nothing changes
+        tk[TK_betacormode] = 3.f;
+        for (int n = 0; n < 5; ++n) tkIter(tk, n, betaIn, betaOut, betaAv, ptBeta);
     }
+}
+
+// The map keys the BBBB (_v2) and EEEE algorithms record alike at their end
+// (Tracklet.cc:1399-1493, 2522-2617): the four anchors' x, y and module detIds
+// (as float), the constants, the algorithm's own betacormode (0: :1245, :2351
+// never change it) and the midpoint dBeta (:1409-1437, turning angles at the
+// two segment midpoints).
+__device__ __forceinline__ void tlCommonKeys(const HitView& h1, const HitView& h2, const HitView& h3, const HitView& h4,
+                                             const uint32_t detId[4], float* tk) {
+    tk[TK_hit1_x] = h1.x; tk[TK_hit1_y] = h1.y; tk[TK_hit2_x] = h2.x; tk[TK_hit2_y] = h2.y;
+    tk[TK_hit3_x] = h3.x; tk[TK_hit3_y] = h3.y; tk[TK_hit4_x] = h4.x; tk[TK_hit4_y] = h4.y;
+    tk[TK_innerSgInnerMdDetId] = float(detId[0]);
+    tk[TK_innerSgOuterMdDetId] = float(detId[1]);
+    tk[TK_outerSgInnerMdDetId] = float(detId[2]);
+    tk[TK_outerSgOuterMdDetId] = float(detId[3]);
+    tk[TK_sinAlphaMax] = kSinAlphaMax;
+    tk[TK_deltaZLum] = kDeltaZLum;
+    tk[TK_k2Rinv1GeVf] = k2Rinv();
+    tk[TK_kRinv1GeVf] = float(2.99792458e-3 * 3.8);
+    tk[TK_pixelPSZpitch] = kPixelPSZpitch;
+    tk[TK_strip2SZpitch] = kStrip2SZpitch;
+    tk[TK_ptCut] = kPtCut;
+    tk[TK_betacormode] = 0.f;
+    const float inMidX = float(double(h1.x + h2.x) / 2.), inMidY = float(double(h1.y + h2.y) / 2.);
+    const float outMidX = float(double(h3.x + h4.x) / 2.), outMidY = float(double(h3.y + h4.y) / 2.);
+    const float ax = h2.x - inMidX, ay = h2.y - inMidY;
+    const float bx = outMidX - inMidX, by = outMidY - inMidY;
+    const float cx = h4.x - outMidX, cy = h4.y - outMidY;
+    const float aMag = sqrtF(ax * ax + ay * ay), bMag = sqrtF(bx * bx + by * by), cMag = sqrtF(cx * cx + cy * cy);
+    const float ab = ax * bx + ay * by, bc = bx * cx + by * cy;
+    tk[TK_dBeta_midPoint] = acosF(ab / aMag / bMag) - acosF(bc / bMag / cMag);
 }
 
 __device__ __forceinline__ void rescaleRH(float beta, float& mn, float& mx) {
@@ -496,7 +594,9 @@ __device__ __forceinline__ double mulsFactor(float z, float rt) {
 
 // ---- first stage: cuts on the inner segment and the outer segment's inner MD
 // _v2: Tracklet.cc:902-1153 (z window, pointed z window, deltaPhi of the midpoint)
-__device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L, TLCarry& c, TLMargins* m) {
+// tk (a stored TL's map keys, diagnostics only): the window values it records.
+__device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L, TLCarry& c, TLMargins* m,
+                                             float* tk = nullptr) {
     const HitView& hInLo = I.lo;
     const HitView& hInUp = I.up;
     const HitView& hOutLo = L.a;
@@ -507,6 +607,12 @@ __device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L,
     const float zpOut = L.ps ? kPixelPSZpitch : kStrip2SZpitch;
     const float zHi = z_InLo + (z_InLo + kDeltaZLum) * (rtRatio - 1.f) * (z_InLo < 0.f ? 1.f : dzDrtScale) + (zpIn + zpOut);
     const float zLo = z_InLo + (z_InLo - kDeltaZLum) * (rtRatio - 1.f) * (z_InLo > 0.f ? 1.f : dzDrtScale) - (zpIn + zpOut);
+    if (tk) {  // Tracklet.cc:992-994 and the end's 1479-1488
+        tk[TK_z_OutLo] = z_OutLo; tk[TK_zLo] = zLo; tk[TK_zHi] = zHi; tk[TK_z_InLo] = z_InLo;
+        tk[TK_rt_InLo] = rt_InLo; tk[TK_rt_OutLo] = rt_OutLo; tk[TK_rtRatio_OutLoInLo] = rtRatio;
+        tk[TK_alpha1GeV_OutLo] = L.alpha1; tk[TK_dzDrtScale] = dzDrtScale;
+        tk[TK_zpitch_InLo] = zpIn; tk[TK_zpitch_OutLo] = zpOut; tk[TK_drt_InSeg] = I.d;
+    }
     if (!(z_OutLo >= zLo && z_OutLo <= zHi)) return false;
     const float drt_OutLo_InLo = rt_OutLo - rt_InLo;
     const float drt_InSeg = I.d;
@@ -523,10 +629,12 @@ __device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L,
     const float zWindow = dzErr / drt_InSeg * drt_OutLo_InLo + (zpIn + zpOut);
     const float zLoP = z_InLo + dzMean * (z_InLo > 0.f ? 1.f : dzDrtScale) - zWindow;
     const float zHiP = z_InLo + dzMean * (z_InLo < 0.f ? 1.f : dzDrtScale) + zWindow;
+    if (tk) { tk[TK_zLoPointed] = zLoP; tk[TK_zHiPointed] = zHiP; }  // :1066-1067
     if (!(z_OutLo >= zLoP && z_OutLo <= zHiP)) return false;
     const float sdlPVoff = 0.1f / rt_OutLo;
     const float sdlCut = float(double(L.alpha1) + sqrt(double(sdlMuls * sdlMuls + sdlPVoff * sdlPVoff)));
     const float dPhiMid = phiMpiPi(phiOf(hOutLo.x - hInLo.x, hOutLo.y - hInLo.y) - midPhi(hInLo, hOutLo));
+    if (tk) { tk[TK_sdlCut] = sdlCut; tk[TK_dPhi] = dPhiMid; }  // :1099, 1152
     if (!(fabsF(dPhiMid) <= sdlCut)) return false;
     c.sdlCut = sdlCut;
     c.thetaMuls = thetaMuls;
@@ -541,7 +649,8 @@ __device__ __forceinline__ bool tlStage1BBBB(const TLInner& I, const TLOutLo& L,
 
 // BBEE / EEEE: Tracklet.cc:1540-1771 and 2018-2270 (rt windows, pointing, midpoint deltaPhi)
 template <bool EEEE>
-__device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, TLCarry& c, TLMargins* m) {
+__device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, TLCarry& c, TLMargins* m,
+                                           float* tk = nullptr) {
     const HitView& hInLo = I.lo;
     const HitView& hInUp = I.up;
     const HitView& hOutLo = L.a;
@@ -561,6 +670,14 @@ __device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, T
         if (zInForHi * zIn < 0) zInForHi = copysignf(0.1f, zIn);
         rtHi = rtIn * (1.f + (zOut - zIn + zGeom1) / zInForHi) + rtGeom1;
     } else {
+        if (tk) {  // the end's records (Tracklet.cc:2602-2612), with the barrel z window it never cuts on (:2127)
+            const float rtOut_o_rtIn = rtOut / rtIn;
+            const float zGeom = (psInLo ? kPixelPSZpitch : kStrip2SZpitch) + (psOutLo ? kPixelPSZpitch : kStrip2SZpitch);
+            tk[TK_rtIn] = rtIn; tk[TK_rtOut] = rtOut; tk[TK_rtOut_o_rtIn] = rtOut_o_rtIn; tk[TK_zIn] = zIn;
+            tk[TK_zOut] = zOut; tk[TK_zGeom] = zGeom; tk[TK_sdlSlope] = L.alpha1; tk[TK_dzDrtScale] = dzDrtScale;
+            tk[TK_sdIn_d] = I.d;
+            tk[TK_zLo] = zIn + (zIn - kDeltaZLum) * (rtOut_o_rtIn - 1.f) * (zIn > 0.f ? 1.f : dzDrtScale) - zGeom;
+        }
         const float rtGeom = (psInLo && psOutLo) ? 2.f * kPixelPSZpitch
                              : (psInLo || psOutLo) ? (kPixelPSZpitch + kStrip2SZpitch) : 2.f * kStrip2SZpitch;
         const float dz = zOut - zIn;
@@ -615,17 +732,17 @@ __device__ __forceinline__ bool tlStage1EC(const TLInner& I, const TLOutLo& L, T
 }
 
 __device__ __forceinline__ bool tlStage1(int variant, const TLInner& I, const TLOutLo& L, TLCarry& c,
-                                         TLMargins* m = nullptr) {
-    if (variant == TL_BBBB) return tlStage1BBBB(I, L, c, m);
-    if (variant == TL_BBEE) return tlStage1EC<false>(I, L, c, m);
-    if (variant == TL_EEEE) return tlStage1EC<true>(I, L, c, m);
+                                         TLMargins* m = nullptr, float* tk = nullptr) {
+    if (variant == TL_BBBB) return tlStage1BBBB(I, L, c, m, tk);
+    if (variant == TL_BBEE) return tlStage1EC<false>(I, L, c, m, nullptr);  // BBEE records the iterations only
+    if (variant == TL_EEEE) return tlStage1EC<true>(I, L, c, m, tk);
     return false;
 }
 
 // ---- second stage: cuts that involve the outer segment's outer MD
 // _v2: Tracklet.cc:1097-1100 and 1156-1501
 // rec (a passing TL's per-object values, kTLVars floats): getBetaIn, getBetaInCut,
-// getBetaOut, getBetaOutCut, getDeltaBeta, getDeltaBetaCut.
+// getBetaOut, getBetaOutCut, getDeltaBeta, getDeltaBetaCut, then the map keys.
 __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L, const TLOutUp& U, const TLCarry& c,
                                              float* rec = nullptr) {
     const HitView& hInLo = I.lo;
@@ -633,6 +750,8 @@ __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L,
     const HitView& hOutLo = L.a;
     const HitView& hOutUp = U.a;
     const float k2 = k2Rinv();
+    float* tk = rec ? rec + kTLTyped : nullptr;
+    if (tk) tk[TK_deltaPhiPos] = dPhi(hInUp.phi, hOutUp.phi);  // Tracklet.cc:1098
     if (!(fabsF(dPhi(hInUp.phi, hOutUp.phi)) <= c.sdlCut)) return false;
     const float alpha_InLo = I.dphichange;
     const bool isEC_lastLayer = U.ec2s;
@@ -655,12 +774,16 @@ __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L,
     const float betaIn_cut = asinF(fmin2((-rt_InSeg * 1.f + drt_tl_axis) * k2 / kPtCut, kSinAlphaMax)) + (0.02f / drt_InSeg);
     if (!(fabsF(betaInRHmin) < betaIn_cut)) return false;
     if (rec) { rec[0] = betaInRHmin; rec[1] = betaIn_cut; }  // Tracklet.cc:1218-1219
+    if (tk) {  // :1197-1204, 1217 (the RH ranges are recorded again, rescaled, at the end)
+        tk[TK_betaIn_0th] = betaIn; tk[TK_betaOut_0th] = betaOut;
+        tk[TK_rawBetaIn] = betaIn; tk[TK_rawBetaOut] = betaOut; tk[TK_betaIn_cut] = betaIn_cut;
+    }
     float betaAv = 0.5f * (betaIn + betaOut);
     float pt_beta = float(double(drt_tl_axis * k2) / sin(double(betaAv)));
     const int lOut = isEC_lastLayer ? 11 : 5;
     const float sdOut_dr = U.dr;
     const float sdOut_d = U.d;
-    deltaBetaIter(betaIn, betaOut, betaAv, pt_beta, rt_InSeg, sdOut_dr, drt_tl_axis, 5);
+    deltaBetaIter(betaIn, betaOut, betaAv, pt_beta, rt_InSeg, sdOut_dr, drt_tl_axis, 5, tk);
     rescaleRH(betaIn, betaInRHmin, betaInRHmax);
     rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
     const float dBetaMuls = c.thetaMuls * 4.f / fmin2(fabsF(pt_beta), 7.0f);
@@ -678,6 +801,16 @@ __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L,
                                   0.25 * sq(double(fabsF(betaInRHmin - betaInRHmax) + fabsF(betaOutRHmin - betaOutRHmax))));
     const float dBeta = betaIn - betaOut;
     if (rec) { rec[4] = dBeta; rec[5] = sqrtF(dBetaCut2); }  // Tracklet.cc:1498-1499
+    if (tk) {  // :1438-1493
+        tk[TK_betaIn] = betaIn; tk[TK_betaInRHmin] = betaInRHmin; tk[TK_betaInRHmax] = betaInRHmax;
+        tk[TK_betaOut] = betaOut; tk[TK_betaOutRHmin] = betaOutRHmin; tk[TK_betaOutRHmax] = betaOutRHmax;
+        tk[TK_dBeta] = dBeta; tk[TK_dBetaCut2] = dBetaCut2; tk[TK_dBetaLum2] = dBetaLum2;
+        tk[TK_dBetaMuls] = dBetaMuls; tk[TK_dBetaRIn2] = 0.f; tk[TK_dBetaROut2] = dBetaROut2;
+        tk[TK_dBetaRes] = dBetaRes; tk[TK_drt_tl_axis] = drt_tl_axis; tk[TK_sdOut_d] = sdOut_d;
+        tk[TK_pt_betaIn] = float(double(drt_tl_axis * k2) / sin(double(betaIn)));  // :1393-1394
+        tk[TK_pt_betaOut] = float(double(drt_tl_axis * k2) / sin(double(betaOut)));
+        tk[TK_pt_beta] = pt_beta; tk[TK_betaAv] = betaAv;
+    }
     return dBeta * dBeta <= dBetaCut2;
 }
 
@@ -690,6 +823,7 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
     const HitView& hOutLo = L.a;
     const HitView& hOutUp = U.a;
     const float k2 = k2Rinv();
+    float* tk = rec ? rec + kTLTyped : nullptr;
     if (!(fabsF(dPhi(hInUp.phi, hOutUp.phi)) <= c.sdlCut)) return false;
     const float sdIn_alpha = I.dphichange;
     const float sdOut_alpha = I.dphichange;  // A-8
@@ -713,6 +847,12 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
         const float aMax = phiMpiPi(U.dphichangemax - sdOut_dPhiPos);
         const float sdOut_alphaOut = phiMpiPi(U.dphichange - sdOut_dPhiPos);
         betaOut = -sdOut_alphaOut + phiMpiPi(dr3Phi - hOutUp.phi);
+        if (tk) {  // Tracklet.cc:2303-2308
+            tk[TK_sdIn_alpha] = sdIn_alpha; tk[TK_sdOut_alphaOut] = sdOut_alphaOut;
+            tk[TK_rawBetaInCorrection] = phiMpiPi(dr3Phi - hInLo.phi);
+            tk[TK_rawBetaOutCorrection] = phiMpiPi(dr3Phi - hOutUp.phi);
+            tk[TK_rawBetaIn] = betaIn; tk[TK_rawBetaOut] = betaOut;
+        }
         betaInRHmin = betaIn + I.dphichangemin - sdIn_alpha;
         betaInRHmax = betaIn + I.dphichangemax - sdIn_alpha;
         if (fabsF(betaInRHmin) > fabsF(betaInRHmax)) { float t = betaInRHmin; betaInRHmin = betaInRHmax; betaInRHmax = t; }
@@ -736,7 +876,7 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
     const int lIn = EEEE ? 11 : 5;
     const float sdOut_dr = U.dr;
     const float sdOut_d = U.d;
-    deltaBetaIter(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn);
+    deltaBetaIter(betaIn, betaOut, betaAv, pt_beta, sdIn_dr, sdOut_dr, dr, lIn, tk);  // BBEE's only map keys
     rescaleRH(betaIn, betaInRHmin, betaInRHmax);
     rescaleRH(betaOut, betaOutRHmin, betaOutRHmax);
     const float dBetaMuls = c.thetaMuls * 4.f / fmin2(fabsF(pt_beta), 7.0f);
@@ -766,6 +906,16 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
     if (rec) {  // BBEE sets only deltaBeta (Tracklet.cc:1977); EEEE both (2621-2622)
         rec[4] = dBeta;
         if (EEEE) rec[5] = sqrtF(dBetaCut2);
+    }
+    if (EEEE && tk) {  // :2576-2617
+        tk[TK_betaIn] = betaIn; tk[TK_betaInRHmin] = betaInRHmin; tk[TK_betaInRHmax] = betaInRHmax;
+        tk[TK_betaOut] = betaOut; tk[TK_betaOutRHmin] = betaOutRHmin; tk[TK_betaOutRHmax] = betaOutRHmax;
+        tk[TK_dBeta] = dBeta; tk[TK_dBetaCut2] = dBetaCut2; tk[TK_dBetaLum2] = dBetaLum2;
+        tk[TK_dBetaMuls] = dBetaMuls; tk[TK_dBetaRIn2] = 0.f; tk[TK_dBetaROut2] = 0.f;
+        tk[TK_dBetaRes] = dBetaRes; tk[TK_dr] = dr; tk[TK_sdOut_d] = sdOut_d;
+        tk[TK_pt_betaIn] = float(double(dr * k2) / sin(double(betaIn)));  // :2516-2517
+        tk[TK_pt_betaOut] = float(double(dr * k2) / sin(double(betaOut)));
+        tk[TK_pt_beta] = pt_beta; tk[TK_betaAv] = betaAv;
     }
     return dBeta * dBeta <= dBetaCut2;
 }

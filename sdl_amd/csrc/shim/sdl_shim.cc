@@ -196,7 +196,7 @@ std::vector<std::array<float, W>> Event::variables(int stage) {
 
 std::vector<std::array<float, 7>> Event::getMiniDoubletVariables() { return variables<7>(SDL_STAGE_MD); }
 std::vector<std::array<float, 22>> Event::getSegmentVariables() { return variables<22>(SDL_STAGE_SG); }
-std::vector<std::array<float, 6>> Event::getTrackletVariables() { return variables<6>(SDL_STAGE_TL); }
+std::vector<std::array<float, 114>> Event::getTrackletVariables() { return variables<114>(SDL_STAGE_TL); }
 
 std::vector<std::string> Event::variableNames(const std::string& objectType) {
     std::vector<std::string> out;

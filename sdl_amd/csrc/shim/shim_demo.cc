@@ -35,8 +35,8 @@ unsigned long long checksumWithValues(const O& objs, const V& vals) {
         unsigned long long h = 1469598103934665603ULL;
         for (int x : objs[i]) h = (h ^ (unsigned)x) * 1099511628211ULL;
         for (float f : vals[i]) {
-            unsigned bits;
-            std::memcpy(&bits, &f, 4);
+            unsigned bits = 0x7fc00000u;  // every NaN as the canonical quiet NaN
+            if (f == f) std::memcpy(&bits, &f, 4);
             h = (h ^ bits) * 1099511628211ULL;
         }
         s += h;

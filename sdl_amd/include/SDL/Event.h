@@ -162,7 +162,7 @@ class Event {
     // no entry).
     std::vector<std::array<float, 7>> getMiniDoubletVariables();
     std::vector<std::array<float, 22>> getSegmentVariables();
-    std::vector<std::array<float, 6>> getTrackletVariables();
+    std::vector<std::array<float, 114>> getTrackletVariables();
     // Column names of the rows above for "MD", "SG" or "TL" (empty otherwise).
     static std::vector<std::string> variableNames(const std::string& objectType);
 

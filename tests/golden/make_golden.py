@@ -35,7 +35,7 @@ LAYER_CASES = [(1, 0, 2), (12, 0, 1)]
 NAV_CASES = [(1, 0, 2), (12, 0, 2), (15, 0, 1)]
 PSEUDO_CASES = [(1, 0, 2), (12, 0, 1), (15, 0, 1)]
 VAR_CASES = [(1, 0, 2), (12, 0, 1), (15, 0, 1)]
-VAR_WIDTH = {"MV": (2, 7), "SV": (4, 22), "TV": (8, 6)}  # tuple width, values per object
+VAR_WIDTH = {"MV": (2, 7), "SV": (4, 22), "TV": (8, 114)}  # tuple width, values per object
 COUNTERS = ("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")
 
 
@@ -93,7 +93,7 @@ def var_fixtures():
             subprocess.run([oracle.REF_DRIVER, "dumpvars", str(wl), tmp, str(e0), str(ne), dump], check=True)
             for e, d in parse_var_dump(dump).items():
                 for k, (keys, bits) in d.items():
-                    out[f"w{wl}_e{e}_{k}"] = sorted_rows(keys, bits)
+                    out[f"w{wl}_e{e}_{k}"] = sorted_rows(keys, bits).astype(np.uint32)  # indices >= 0, bits
     np.savez_compressed(os.path.join(HERE, "ref_vars.npz"), **out)
 
 

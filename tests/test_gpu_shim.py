@@ -29,7 +29,9 @@ def _checksum(rows):
 
 def _checksum_with_values(rows, values):
     s = 0
-    bits = np.ascontiguousarray(values, np.float32).view(np.uint32)
+    v = np.array(values, np.float32, copy=True)
+    v[np.isnan(v)] = np.float32("nan")  # the demo hashes every NaN as the canonical one
+    bits = v.view(np.uint32)
     for r, b in zip(np.asarray(rows, dtype=np.int64).reshape(len(rows), -1), bits):
         h = 1469598103934665603
         for x in list(r & 0xFFFFFFFF) + list(b):

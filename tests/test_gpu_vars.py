@@ -15,10 +15,19 @@ pytestmark = pytest.mark.gpu
 STAGES = (("MD", runtime.STAGE_MD), ("SG", runtime.STAGE_SG), ("TL", runtime.STAGE_TL))
 
 
+def _bits(values):
+    """Bit patterns, every NaN as the canonical quiet NaN: the device's and the
+    host's libm may return NaNs of another sign (acos of a rounded cosine just
+    above 1 in the tracklets' dBeta_midPoint)."""
+    v = np.array(values, np.float32, copy=True)
+    v[np.isnan(v)] = np.float32("nan")
+    return v.view(np.uint32).astype(np.int64)
+
+
 def _rows(keys, values):
-    rows = np.concatenate([keys.astype(np.int64), np.ascontiguousarray(values, np.float32).view(np.uint32).astype(np.int64)],
-                          axis=1)
-    return rows[np.lexsort(rows.T[::-1])] if len(rows) else rows
+    """Rows (tuple, value bits) ordered by the tuple (unique within a set)."""
+    rows = np.concatenate([keys.astype(np.int64), _bits(values)], axis=1)
+    return rows[np.lexsort(keys.T[::-1])] if len(rows) else rows
 
 
 def _device(wl, e, seq=None):

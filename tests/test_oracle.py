@@ -321,4 +321,40 @@ def test_object_vars_layout_and_defaults(geom_paths):
     bbee = (tl[:, 0] == 0) & (tl[:, 1] == 0) & (tl[:, 5] == 0)
     assert bbee.any() and (tl[bbee][:, 4] != 0).all()
     assert (np.abs(tl[~bbee][:, 4]) <= tl[~bbee][:, 5]).all()  # |dBeta| <= sqrt(dBetaCut2)
+    # the tracklet map (Tracklet.cc:152-296): "algo" names the variant; BBEE
+    # writes only the delta-beta iteration keys; the ten _v2 keys are absent
+    # elsewhere; mode 3 of the iterations changes nothing
+    t = {n: tl[:, i] for i, n in enumerate(names["TL"])}
+    algo = t["reco_algo"]
+    assert set(np.unique(algo)) == {1.0, 2.0, 3.0}
+    assert (algo[bbee] == 2).all() and (algo[~bbee] != 2).all()
+    iter_keys = {n for n in names["TL"] if n.startswith("reco_") and
+                 (n[5:].startswith(("betaIn_", "betaOut_", "betaAv_", "betaPt_", "dBeta_")) or n == "reco_betacormode")}
+    iter_keys.discard("reco_dBeta_midPoint")
+    iter_keys.discard("reco_betaIn_cut")
+    for n in names["TL"][6:]:
+        if n not in iter_keys and n != "reco_algo":
+            col = t[n][algo == 2]
+            assert ((col == -999) | np.isnan(col)).all(), n
+    for n in ("alpha1GeV_OutLo", "rt_InLo", "z_OutLo", "zpitch_InLo"):
+        assert np.isnan(t["reco_" + n][algo != 1]).all() and np.isfinite(t["reco_" + n][algo == 1]).all()
+    ends = algo != 2
+    assert (t["reco_dBeta"][ends] == t["deltaBeta"][ends]).all()
+    assert (t["reco_dBeta"][ends] == (t["reco_betaIn"][ends] - t["reco_betaOut"][ends])).all()
+    assert (t["reco_betacormode"][ends] == 0).all()  # the algorithms' own, never updated
+    assert set(np.unique(t["reco_betacormode"][algo == 2])) <= {1.0, 2.0, 3.0}
+    m3 = (algo == 2) & (t["reco_betacormode"] == 3)
+    for k in ("betaIn", "betaOut", "betaAv", "betaPt", "dBeta"):
+        assert (t[f"reco_{k}_4th"][m3] == t[f"reco_{k}_0th"][m3]).all()
+    # the anchors' coordinates and modules are the tuple's hits'
+    x = dict(zip(ev.idx.tolist(), ev.x.tolist()))
+    det = dict(zip(ev.idx.tolist(), ev.detId.tolist()))
+    tup = sets["TL"][ends]
+    for j, (hx, dk) in enumerate((("hit1_x", "innerSgInnerMdDetId"), ("hit2_x", "innerSgOuterMdDetId"),
+                                  ("hit3_x", "outerSgInnerMdDetId"), ("hit4_x", "outerSgOuterMdDetId"))):
+        lo, up = tup[:, 2 * j], tup[:, 2 * j + 1]
+        got = t["reco_" + hx][ends]
+        assert all(g in (np.float32(x[a]), np.float32(x[b])) for g, a, b in zip(got, lo, up))
+        got = t["reco_" + dk][ends]
+        assert all(g in (np.float32(det[a]), np.float32(det[b])) for g, a, b in zip(got, lo, up))
     assert (np.abs(tl[~bbee][:, 2]) < tl[~bbee][:, 3]).all()   # |betaOut| < betaOut cut
