@@ -138,8 +138,8 @@ int sdl_event_layer_counts(sdl_event* ev, int e, unsigned long long out[110]);
  * *n receives the number of objects; returns SDL_ERR_CAPACITY if cap is short. */
 int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n);
 
-/* Per-object values of event e's stored MDs, SGs or TLs (stage SDL_STAGE_MD,
- * _SG or _TL; also the objects of the layer-pair, navigation and pseudo-MD
+/* Per-object values of event e's stored MDs, SGs, TLs or TCs from tracklets
+ * (stage SDL_STAGE_MD, _SG, _TL or _TCL; also the objects of the layer-pair, navigation and pseudo-MD
  * stages): what the reference keeps on each stored object, its typed getters
  * and setRecoVars maps (MiniDoublet.h:85-97 / MiniDoublet.cc:113-197,
  * Segment.h:128-149 / Segment.cc:104-284, Tracklet.cc:43-98,
@@ -147,7 +147,9 @@ int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, l
  * objects' getters.  sdl_object_var_count(stage) floats per object (MD 7,
  * SG 22, TL 114: six typed values, then the 108 keys of the tracklet's
  * reco-variable map, Tracklet.cc:144-296,902-2665,3181-3324, in std::map order,
- * named reco_<key>; 0 for the other stages), named by sdl_object_var_names(stage)
+ * named reco_<key>; TCL 2: dR, innerR of a TC made from tracklets,
+ * TrackCandidate.cc:206-207; 0 for the other stages, TCs from triplets run no
+ * algorithm and record nothing), named by sdl_object_var_names(stage)
  * (comma-separated), in the object order of sdl_event_export.  Fields a path
  * never sets keep the reference's constructor value 0; reco variables a path
  * never sets are -999, or NaN where the map has no entry (pseudo MDs' miniCut,

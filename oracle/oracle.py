@@ -91,12 +91,13 @@ VAR_NAMES = {
            "reco_deltaPhi", "dAlpha_innerMD_segment", "dAlpha_outerMD_segment", "dAlpha_innerMD_outerMD"),
     "TL": ("betaIn", "betaInCut", "betaOut", "betaOutCut", "deltaBeta", "deltaBetaCut") + tuple(
         "reco_" + k for k in TL_KEYS),
+    "TCL": ("dR", "innerR"),
 }
 
 
 def run(paths, ev, libm=GLIBC, stages=6, vars=False):
     """Returns ({stage: int32 array [n, width]}, counters dict), and with
-    vars=True also {"MD"|"SG"|"TL": float32 array [n, nvars]} (VAR_NAMES), the
+    vars=True also {"MD"|"SG"|"TL"|"TCL": float32 array [n, nvars]} (VAR_NAMES), the
     per-object variables in the same object order.  stages = 1..6
     runs that many module-map stages; LAYER_CHAIN runs the layer-pair chain; a
     list runs those stages in that order (0-5 module-map, 6-8 createSegments,
@@ -124,7 +125,7 @@ def run(paths, ev, libm=GLIBC, stages=6, vars=False):
         if not vars:
             return out, counters
         v = {}
-        for name, st in (("MD", 0), ("SG", 1), ("TL", 3)):
+        for name, st in (("MD", 0), ("SG", 1), ("TL", 3), ("TCL", 5)):
             nv = L.oracle_var_count(st)
             a = np.zeros((len(out[name]), nv), np.float32)
             if len(a):

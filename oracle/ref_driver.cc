@@ -23,9 +23,9 @@
 //     (createPseudoMiniDoubletsFromAnchorModule, then the module-map chain)
 //     (createMiniDoublets, createSegmentsWithModuleMap, createTrackletsViaNavigation)
 //   sdl_ref_driver dumpvars    <workload> <geomdir> <event0> <nevents> <outfile>
-//     (the module-map chain up to the tracklets; every MD, SG and TL with its
-//     typed per-object values and reco variables, TLs with their whole
-//     reco-variable map, as float bit patterns)
+//     (the module-map chain; every MD, SG and TL with its typed per-object
+//     values and reco variables, TLs with their whole reco-variable map, TCs
+//     from tracklets with theirs, as float bit patterns)
 //   sdl_ref_driver time        <workload> <geomdir> <event0> <nevents> <threads> [stages]
 // `stages` limits the chain to its first k stages (1..6, default 6).
 #include <chrono>
@@ -273,6 +273,21 @@ void dumpVars(FILE* f, SDL::Event& sev, long ievent) {
             std::fprintf(f, "\n");
         }
     }
+    // TCs made from tracklets carry dR and innerR (TrackCandidate.cc:206-207);
+    // those made from triplets run no algorithm and have an empty map
+    for (SDL::Layer* L : sev.getLayerPtrs()) {
+        for (SDL::TrackCandidate* tc : L->getTrackCandidatePtrs()) {
+            SDL::TrackletBase* a = tc->innerTrackletBasePtr();
+            SDL::TrackletBase* b = tc->outerTrackletBasePtr();
+            if (dynamic_cast<SDL::Triplet*>(a) != nullptr) continue;
+            std::fprintf(f, "CV");
+            putSG(f, a->innerSegmentPtr()); putSG(f, a->outerSegmentPtr());
+            putSG(f, b->innerSegmentPtr()); putSG(f, b->outerSegmentPtr());
+            putBits(f, recoVar(tc->getRecoVars(), "dR"));
+            putBits(f, recoVar(tc->getRecoVars(), "innerR"));
+            std::fprintf(f, "\n");
+        }
+    }
 }
 
 }  // namespace
@@ -349,7 +364,7 @@ int main(int argc, char** argv) {
             sdlgen::generateEvent(g, cfg, ie, ev);
             SDL::Event sev;
             fill(sev, ev);
-            runChain(sev, 4);
+            runChain(sev, 6);
             dumpVars(f, sev, ie);
         }
         std::fclose(f);

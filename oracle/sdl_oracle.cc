@@ -728,6 +728,38 @@ struct Event {
         }
     }
 
+    // MathUtil::getCenterFromThreePoints (MathUtil.cc:44-106): the centre of the
+    // circle through A, B, C from the perpendicular bisectors of AB and BC; the
+    // origin (a default Hit) when the two slopes are equal.  Returns the distance
+    // from A to it, as TrackCandidate.cc:196-197 computes it.
+    static float radiusFromA(const P3& A, const P3& B, const P3& C) {
+        const float xA = A.x, yA = A.y, xB = B.x, yB = B.y, xC = C.x, yC = C.y;
+        const float xMidAB = float(double(xA + xB) / 2.), yMidAB = float(double(yA + yB) / 2.);
+        const bool infAB = (xB - xA) == 0, infBC = (xC - xB) == 0;
+        const bool zeroAB = (yB - yA) == 0;
+        const float sAB = infAB ? 0.f : (yB - yA) / (xB - xA);
+        const float sBC = infBC ? 0.f : (yC - yB) / (xC - xB);
+        const float perpAB = (infAB || zeroAB) ? 0.f : float(-1. / double(sAB));
+        float cx = 0.f, cy = 0.f;
+        if (!((sAB - sBC) == 0)) {
+            const float num = sAB * sBC * (yA - yC) + sBC * (xA + xB) - sAB * (xB + xC);
+            cx = float(double(num) / (2. * double(sBC - sAB)));
+            cy = perpAB * (cx - xMidAB) + yMidAB;
+        }
+        return float(std::sqrt(sq(double(xA - cx)) + sq(double(yA - cy))));
+    }
+
+    // TrackCandidate::runTrackCandidateDefaultAlgo's recorded values (TrackCandidate.cc:183-207):
+    // innerR through the inner TL's first three anchors, outerR through the outer
+    // TL's last three, dR = (innerR - outerR) / innerR.
+    void tcRecoVars(const PairRec& inner, const PairRec& outer, float* out) const {
+        auto anchor = [&](int md) { return hits[mds[md].anchor].p; };
+        const float innerR = radiusFromA(anchor(sgs[inner.in].mdIn), anchor(sgs[inner.in].mdOut), anchor(sgs[inner.out].mdIn));
+        const float outerR = radiusFromA(anchor(sgs[outer.in].mdOut), anchor(sgs[outer.out].mdIn), anchor(sgs[outer.out].mdOut));
+        out[0] = (innerR - outerR) / innerR;
+        out[1] = innerR;
+    }
+
     // The values the BBBB (_v2) and EEEE algorithms record alike at their end
     // (Tracklet.cc:1399-1493, 2522-2617): the four anchors' x, y and module
     // detIds (as float), the constants, and the midpoint dBeta.
@@ -1433,7 +1465,8 @@ struct Event {
     // values and have no miniCut (NaN).
     std::vector<float> objectVars(int stage) {
         std::vector<float> out;
-        const int nv = stage == ORACLE_MD ? kMDVars : stage == ORACLE_SG ? kSGVars : stage == ORACLE_TL ? kTLVars : 0;
+        const int nv = stage == ORACLE_MD ? kMDVars : stage == ORACLE_SG ? kSGVars : stage == ORACLE_TL ? kTLVars
+                       : stage == ORACLE_TCL ? kTCLVars : 0;
         if (nv == 0) return out;
         float buf[kTLVars];
         rec = buf;
@@ -1462,6 +1495,12 @@ struct Event {
                 tlAlgo(sgs[tl.in], sgs[tl.out], nullptr);
                 put();
             }
+        if (stage == ORACLE_TCL)
+            for (const auto& tc : tcs)
+                if (tc.first == 1) {
+                    tcRecoVars(tls[tc.second.in], tls[tc.second.out], buf);
+                    put();
+                }
         rec = nullptr;
         return out;
     }
@@ -1488,7 +1527,7 @@ struct Event {
 
 struct Result {
     std::vector<int> t[6];
-    std::vector<float> vars[4];  // per-object variables of MDs, SGs and TLs (index = stage)
+    std::vector<float> vars[6];  // per-object variables of MDs, SGs, TLs and TCLs (index = stage)
     unsigned long long counters[10];
 };
 
@@ -1525,7 +1564,7 @@ void* oracle_run(void* geom, int nhits, const float* x, const float* y, const fl
     else ev.run(stages);
     auto* r = new Result();
     for (int s = 0; s < 6; ++s) r->t[s] = ev.tuples(s);
-    for (int s : {ORACLE_MD, ORACLE_SG, ORACLE_TL}) r->vars[s] = ev.objectVars(s);
+    for (int s : {ORACLE_MD, ORACLE_SG, ORACLE_TL, ORACLE_TCL}) r->vars[s] = ev.objectVars(s);
     for (int s = 0; s < 5; ++s) r->counters[s] = ev.nCand[s];
     r->counters[5] = ev.mds.size();
     r->counters[6] = ev.sgs.size();
@@ -1542,7 +1581,7 @@ void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y,
     ev.runSequence(seq, n);
     auto* r = new Result();
     for (int s = 0; s < 6; ++s) r->t[s] = ev.tuples(s);
-    for (int s : {ORACLE_MD, ORACLE_SG, ORACLE_TL}) r->vars[s] = ev.objectVars(s);
+    for (int s : {ORACLE_MD, ORACLE_SG, ORACLE_TL, ORACLE_TCL}) r->vars[s] = ev.objectVars(s);
     for (int s = 0; s < 5; ++s) r->counters[s] = ev.nCand[s];
     r->counters[5] = ev.mds.size();
     r->counters[6] = ev.sgs.size();
@@ -1555,10 +1594,11 @@ void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y,
 static const int kWidth[6] = {2, 4, 8, 8, 16, 16};
 
 int oracle_var_count(int stage) {
-    return stage == ORACLE_MD ? kMDVars : stage == ORACLE_SG ? kSGVars : stage == ORACLE_TL ? kTLVars : 0;
+    return stage == ORACLE_MD ? kMDVars : stage == ORACLE_SG ? kSGVars : stage == ORACLE_TL ? kTLVars
+           : stage == ORACLE_TCL ? kTCLVars : 0;
 }
 void oracle_result_vars(void* res, int stage, float* out) {
-    if (stage < 0 || stage > ORACLE_TL) return;
+    if (stage < 0 || stage > ORACLE_TCL) return;
     const std::vector<float>& v = static_cast<Result*>(res)->vars[stage];
     if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(float));
 }

@@ -46,7 +46,8 @@ enum { ORACLE_MD = 0, ORACLE_SG = 1, ORACLE_TP = 2, ORACLE_TL = 3, ORACLE_TCP = 
 //         map (TrackletBase.cc:48-60) in std::map order, from "algo" to
 //         "zpitch_OutLo" (oracle.py: VAR_NAMES); a key the TL's variant never
 //         writes is NaN, one it resets and never sets is -999.
-enum { kMDVars = 7, kSGVars = 22, kTLTyped = 6, kTLKeys = 108, kTLVars = kTLTyped + kTLKeys };
+// TCL (2): recovars dR, innerR of a TC made from tracklets (TrackCandidate.cc:206-207).
+enum { kMDVars = 7, kSGVars = 22, kTLTyped = 6, kTLKeys = 108, kTLVars = kTLTyped + kTLKeys, kTCLVars = 2 };
 
 // Geometry in the reference's three text formats (TiltedGeometry.cc:36,
 // EndcapGeometry.cc:48, ModuleConnectionMap.cc:36-44).  Returns null on error.
@@ -72,7 +73,7 @@ void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y,
 long oracle_result_count(void* res, int stage);
 void oracle_result_copy(void* res, int stage, int* out);
 // Per-object variables of MDs, SGs and TLs (stage ORACLE_MD, ORACLE_SG,
-// ORACLE_TL): oracle_var_count(stage) floats per object, in object order.
+// ORACLE_TL, ORACLE_TCL): oracle_var_count(stage) floats per object, in object order.
 int oracle_var_count(int stage);
 void oracle_result_vars(void* res, int stage, float* out);
 // out[0..4] candidates (MD, SG, TP, TL, TC), out[5..9] objects (MD, SG, TP, TL, TC)

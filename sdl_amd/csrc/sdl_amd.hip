@@ -2108,6 +2108,23 @@ __global__ void k_vars_tl(long long a, long long n, int NM, const ModDev* mods, 
     }
 }
 
+// TrackCandidate::runTrackCandidateDefaultAlgo's recorded values (TrackCandidate.cc:183-207)
+// of TCs [a, a + n) made from tracklets: innerR through the inner TL's first
+// three anchors, outerR through the outer TL's last three, dR = (innerR - outerR) / innerR.
+__global__ void k_vars_tcl(long long a, long long n, MDDev D, SGDev S, const int* tlIn, const int* tlOutArr,
+                           const int2* tcl, float* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 tc = tcl[a + i];
+    const int sIn1 = tlIn[tc.x], sOut1 = tlOutArr[tc.x], sIn2 = tlIn[tc.y], sOut2 = tlOutArr[tc.y];
+    const float4 iA = D.a[S.in[sIn1]], iB = D.a[S.out[sIn1]], iC = D.a[S.in[sOut1]];
+    const float4 oA = D.a[S.out[sIn2]], oB = D.a[S.in[sOut2]], oC = D.a[S.out[sOut2]];
+    const float innerR = radiusFromA(iA.x, iA.y, iB.x, iB.y, iC.x, iC.y);
+    const float outerR = radiusFromA(oA.x, oA.y, oB.x, oB.y, oC.x, oC.y);
+    out[i * kTCLVars] = (innerR - outerR) / innerR;
+    out[i * kTCLVars + 1] = innerR;
+}
+
 __global__ void k_export(long long a, long long n, int kind, const int* pA, const int* pB, const int2* pC,
                          const int* tpIn, const int* tpOutArr, const int* tlIn, const int* tlOutArr, const int* sgIn,
                          const int* sgOut, const int* lo, const int* up, const int* idx, int* out) {
@@ -3777,7 +3794,8 @@ int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, l
 }
 
 int sdl_object_var_count(int stage) {
-    return stage == SDL_STAGE_MD ? kMDVars : stage == SDL_STAGE_SG ? kSGVars : stage == SDL_STAGE_TL ? kTLVars : 0;
+    return stage == SDL_STAGE_MD ? kMDVars : stage == SDL_STAGE_SG ? kSGVars : stage == SDL_STAGE_TL ? kTLVars
+           : stage == SDL_STAGE_TCL ? kTCLVars : 0;
 }
 
 const char* sdl_object_var_names(int stage) {
@@ -3809,6 +3827,7 @@ const char* sdl_object_var_names(int stage) {
                    "reco_sdOut_alphaOut,reco_sdOut_d,reco_sdlCut,reco_sdlSlope,reco_sinAlphaMax,reco_strip2SZpitch,"
                    "reco_zGeom,reco_zHi,reco_zHiPointed,reco_zIn,reco_zLo,reco_zLoPointed,reco_zOut,reco_z_InLo,"
                    "reco_z_OutLo,reco_zpitch_InLo,reco_zpitch_OutLo";
+        case SDL_STAGE_TCL: return "dR,innerR";
         default: return "";
     }
 }
@@ -3819,6 +3838,7 @@ int sdl_event_export_vars(sdl_event* ev, int e, int stage, float* out, long long
     if (stage == SDL_STAGE_MD) need |= 1u << SDL_STAGE_MD_PSEUDO;
     if (stage == SDL_STAGE_SG) need |= 1u << SDL_STAGE_SG_LAYERS;
     if (stage == SDL_STAGE_TL) need |= (1u << SDL_STAGE_TL_LAYERS) | (1u << SDL_STAGE_TL_NAV);
+    if (stage == SDL_STAGE_TCL) need |= 1u << SDL_STAGE_TC_LAYERS;
     if (!(ev->ranMask & need)) return fail(SDL_ERR_STATE, "stage has not run");
     long long a = 0, cnt = 0;
     if (int r = stageRange(ev, e, stage, a, cnt)) return r;
@@ -3837,9 +3857,12 @@ int sdl_event_export_vars(sdl_event* ev, int e, int stage, float* out, long long
     } else if (stage == SDL_STAGE_SG) {
         hipLaunchKernelGGL(k_vars_sg, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, G->NM, G->dMods, hitsView(ev),
                            mdView(ev), sgView(ev), ev->exportVars.p);
-    } else {
+    } else if (stage == SDL_STAGE_TL) {
         hipLaunchKernelGGL(k_vars_tl, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, G->NM, G->dMods, hitsView(ev),
                            mdView(ev), sgView(ev), ev->tlIn.p, ev->tlOut.p, ev->exportVars.p);
+    } else {
+        hipLaunchKernelGGL(k_vars_tcl, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, mdView(ev), sgView(ev),
+                           ev->tlIn.p, ev->tlOut.p, ev->tcl.p, ev->exportVars.p);
     }
     HIP_OK(hipMemcpyAsync(out, ev->exportVars.p, sizeof(float) * cnt * nv, hipMemcpyDeviceToHost, ev->st));
     HIP_OK(hipStreamSynchronize(ev->st));

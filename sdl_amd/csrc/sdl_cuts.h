@@ -174,6 +174,8 @@ constexpr int kMDVars = 7, kSGVars = 22;
 // (TrackletBase.cc:48-60) in std::map order; a key the TL's variant never writes
 // is NaN, one it resets (Tracklet.cc:152-248) and never sets is -999.
 constexpr int kTLTyped = 6, kTLKeys = 108, kTLVars = kTLTyped + kTLKeys;
+// TC from tracklets: recovars dR, innerR (TrackCandidate.cc:206-207).
+constexpr int kTCLVars = 2;
 enum TLKey {
     TK_algo, TK_alpha1GeV_OutLo, TK_betaAv, TK_betaAv_0th, TK_betaAv_1st, TK_betaAv_2nd, TK_betaAv_3rd,
     TK_betaAv_4th, TK_betaIn, TK_betaInRHmax, TK_betaInRHmin, TK_betaIn_0th, TK_betaIn_1st, TK_betaIn_1stCorr,
@@ -464,6 +466,27 @@ __device__ __forceinline__ void deltaBetaIter(float& betaIn, float& betaOut, flo
         tk[TK_betacormode] = 3.f;
         for (int n = 0; n < 5; ++n) tkIter(tk, n, betaIn, betaOut, betaAv, ptBeta);
     }
+}
+
+// MathUtil::getCenterFromThreePoints (MathUtil.cc:44-106): the centre of the
+// circle through A, B, C from the perpendicular bisectors of AB and BC, or the
+// origin (a default Hit) when the two slopes are equal; returns the distance
+// from A to it, as TrackCandidate.cc:196-197 computes it.
+__device__ __forceinline__ float radiusFromA(float xA, float yA, float xB, float yB, float xC, float yC) {
+    const float xMidAB = float(double(xA + xB) / 2.), yMidAB = float(double(yA + yB) / 2.);
+    const bool infAB = (xB - xA) == 0, infBC = (xC - xB) == 0;
+    const bool zeroAB = (yB - yA) == 0;
+    const float sAB = infAB ? 0.f : (yB - yA) / (xB - xA);
+    const float sBC = infBC ? 0.f : (yC - yB) / (xC - xB);
+    const float perpAB = (infAB || zeroAB) ? 0.f : float(-1. / double(sAB));
+    float cx = 0.f, cy = 0.f;
+    if (!((sAB - sBC) == 0)) {
+        const float num = sAB * sBC * (yA - yC) + sBC * (xA + xB) - sAB * (xB + xC);
+        cx = float(double(num) / (2. * double(sBC - sAB)));
+        cy = perpAB * (cx - xMidAB) + yMidAB;
+    }
+    const double dx = double(xA - cx), dy = double(yA - cy);
+    return float(sqrt(dx * dx + dy * dy));
 }
 
 // The map keys the BBBB (_v2) and EEEE algorithms record alike at their end

@@ -197,12 +197,16 @@ std::vector<std::array<float, W>> Event::variables(int stage) {
 std::vector<std::array<float, 7>> Event::getMiniDoubletVariables() { return variables<7>(SDL_STAGE_MD); }
 std::vector<std::array<float, 22>> Event::getSegmentVariables() { return variables<22>(SDL_STAGE_SG); }
 std::vector<std::array<float, 114>> Event::getTrackletVariables() { return variables<114>(SDL_STAGE_TL); }
+std::vector<std::array<float, 2>> Event::getTrackCandidatesFromTrackletsVariables() {
+    return variables<2>(SDL_STAGE_TCL);
+}
 
 std::vector<std::string> Event::variableNames(const std::string& objectType) {
     std::vector<std::string> out;
     const int stage = objectType == "MD" ? SDL_STAGE_MD
                       : objectType == "SG" ? SDL_STAGE_SG
-                      : objectType == "TL" ? SDL_STAGE_TL : -1;
+                      : objectType == "TL" ? SDL_STAGE_TL
+                      : objectType == "TC" ? SDL_STAGE_TCL : -1;
     if (stage < 0) return out;
     const std::string all = sdl_object_var_names(stage);
     size_t a = 0;

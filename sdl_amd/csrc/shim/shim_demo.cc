@@ -4,9 +4,9 @@
 // Runs the module-map chain, with "layers" the layer-pair chain
 // (createMiniDoublets, createSegments, createTracklets, createTrackCandidates),
 // with "pseudo" the module-map chain on pseudo mini-doublets;
-// prints the counters, a checksum of every object set, a checksum of the MD, SG
-// and TL objects joined with their per-object values, and the value column
-// names, one line each.
+// prints the counters, a checksum of every object set, a checksum of the MD, SG,
+// TL and TC-from-tracklets objects joined with their per-object values, and the
+// value column names, one line each.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -85,12 +85,14 @@ int main(int argc, char** argv) {
     std::printf("checksums %llu %llu %llu %llu %llu %llu\n", checksum(event.getMiniDoublets()),
                 checksum(event.getSegments()), checksum(event.getTriplets()), checksum(event.getTracklets()),
                 checksum(event.getTrackCandidatesFromTriplets()), checksum(event.getTrackCandidatesFromTracklets()));
-    std::printf("valuesums %llu %llu %llu\n",
+    std::printf("valuesums %llu %llu %llu %llu\n",
                 checksumWithValues(event.getMiniDoublets(), event.getMiniDoubletVariables()),
                 checksumWithValues(event.getSegments(), event.getSegmentVariables()),
-                checksumWithValues(event.getTracklets(), event.getTrackletVariables()));
+                checksumWithValues(event.getTracklets(), event.getTrackletVariables()),
+                checksumWithValues(event.getTrackCandidatesFromTracklets(),
+                                   event.getTrackCandidatesFromTrackletsVariables()));
     std::string names;
-    for (const char* t : {"MD", "SG", "TL"})
+    for (const char* t : {"MD", "SG", "TL", "TC"})
         for (const auto& n : SDL::Event::variableNames(t)) names += std::string(names.empty() ? "" : ",") + t + "." + n;
     std::printf("valuenames %s\n", names.c_str());
     return 0;

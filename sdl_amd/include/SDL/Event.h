@@ -163,7 +163,9 @@ class Event {
     std::vector<std::array<float, 7>> getMiniDoubletVariables();
     std::vector<std::array<float, 22>> getSegmentVariables();
     std::vector<std::array<float, 114>> getTrackletVariables();
-    // Column names of the rows above for "MD", "SG" or "TL" (empty otherwise).
+    // dR and innerR of each TC of getTrackCandidatesFromTracklets() (TrackCandidate.cc:206-207)
+    std::vector<std::array<float, 2>> getTrackCandidatesFromTrackletsVariables();
+    // Column names of the rows above for "MD", "SG", "TL" or "TC" (empty otherwise).
     static std::vector<std::string> variableNames(const std::string& objectType);
 
   private:

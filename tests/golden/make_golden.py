@@ -35,7 +35,7 @@ LAYER_CASES = [(1, 0, 2), (12, 0, 1)]
 NAV_CASES = [(1, 0, 2), (12, 0, 2), (15, 0, 1)]
 PSEUDO_CASES = [(1, 0, 2), (12, 0, 1), (15, 0, 1)]
 VAR_CASES = [(1, 0, 2), (12, 0, 1), (15, 0, 1)]
-VAR_WIDTH = {"MV": (2, 7), "SV": (4, 22), "TV": (8, 114)}  # tuple width, values per object
+VAR_WIDTH = {"MV": (2, 7), "SV": (4, 22), "TV": (8, 114), "CV": (16, 2)}  # tuple width, values per object
 COUNTERS = ("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC")
 
 

@@ -49,8 +49,9 @@ def _check(out, wl, e, stages):
     assert counts == list(cnt.values())
     assert sums == [_checksum(ref[s]) for s in oracle.STAGES]
     assert [int(v) for v in lines["valuesums"].split()] == [_checksum_with_values(ref[k], vals[k])
-                                                             for k in ("MD", "SG", "TL")]
-    assert lines["valuenames"].split(",") == [f"{k}.{n}" for k in ("MD", "SG", "TL") for n in oracle.VAR_NAMES[k]]
+                                                             for k in ("MD", "SG", "TL", "TCL")]
+    assert lines["valuenames"].split(",") == [f"{t}.{n}" for t, k in (("MD", "MD"), ("SG", "SG"), ("TL", "TL"),
+                                                                      ("TC", "TCL")) for n in oracle.VAR_NAMES[k]]
     return cnt
 
 

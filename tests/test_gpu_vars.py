@@ -12,7 +12,7 @@ from sdl_amd import gen, runtime
 
 pytestmark = pytest.mark.gpu
 
-STAGES = (("MD", runtime.STAGE_MD), ("SG", runtime.STAGE_SG), ("TL", runtime.STAGE_TL))
+STAGES = (("MD", runtime.STAGE_MD), ("SG", runtime.STAGE_SG), ("TL", runtime.STAGE_TL), ("TCL", runtime.STAGE_TCL))
 
 
 def _bits(values):
@@ -58,6 +58,7 @@ def test_var_names_and_counts():
     assert runtime.var_names(runtime.STAGE_MD) == oracle.VAR_NAMES["MD"]
     assert runtime.var_names(runtime.STAGE_SG) == oracle.VAR_NAMES["SG"]
     assert runtime.var_names(runtime.STAGE_TL) == oracle.VAR_NAMES["TL"]
+    assert runtime.var_names(runtime.STAGE_TCL) == oracle.VAR_NAMES["TCL"]
     assert runtime.var_names(runtime.STAGE_TP) == ()
 
 
@@ -77,15 +78,15 @@ def test_object_vars_pu200():
 
 
 def test_object_vars_layer_chain_and_pseudo():
-    """The layer-pair chain's SGs and TLs, and pseudo MDs (zero values, no
+    """The layer-pair chain's SGs, TLs and TCs, and pseudo MDs (zero values, no
     miniCut) with the module-map chain on them."""
     print("case vars layer chain and pseudo MDs, workload 12 event 0", flush=True)
     wl, e = 12, 0
-    ev, x = _device(wl, e, seq=[runtime.STAGE_MD, runtime.SG_LAYERS, runtime.TL_LAYERS])
-    sets, _, values = oracle.run(gen.geometry(gen.geom_kind(wl))[1], x, oracle.CR, stages=[0, 6, 7], vars=True)
-    _compare(ev, sets, values, "layers")
+    ev, x = _device(wl, e, seq=[runtime.STAGE_MD, runtime.SG_LAYERS, runtime.TL_LAYERS, runtime.TC_LAYERS])
+    sets, _, values = oracle.run(gen.geometry(gen.geom_kind(wl))[1], x, oracle.CR, stages=[0, 6, 7, 8], vars=True)
+    _compare(ev, sets, values, "layers", STAGES if len(sets["TCL"]) else STAGES[:3])
     ev, x = _device(wl, e, seq=[runtime.MD_PSEUDO, runtime.STAGE_SG, runtime.STAGE_TP, runtime.STAGE_TL])
     sets, _, values = oracle.run(gen.geometry(gen.geom_kind(wl))[1], x, oracle.CR, stages=[10, 1, 2, 3], vars=True)
-    _compare(ev, sets, values, "pseudo")
+    _compare(ev, sets, values, "pseudo", STAGES[:3])
     md = ev.export_vars(runtime.STAGE_MD)
     assert (md[:, :6] == 0).all() and np.isnan(md[:, 6]).all()

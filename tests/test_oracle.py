@@ -259,7 +259,7 @@ def test_oracle_pseudo_md_closed_form(geom_paths):
 
 
 VAR_CASES = [(1, 0), (1, 1), (12, 0), (15, 0)]
-VAR_KEYS = (("MV", "MD"), ("SV", "SG"), ("TV", "TL"))
+VAR_KEYS = (("MV", "MD"), ("SV", "SG"), ("TV", "TL"), ("CV", "TCL"))
 
 
 def canonical_rows(keys, values):
@@ -273,14 +273,15 @@ def canonical_rows(keys, values):
 def test_object_vars_match_reference(wl, e, geom_paths):
     """Per-object values of every MD, SG and TL (the typed getters and reco
     variables of MiniDoublet.h, Segment.h and Tracklet.h, as the reference's
-    default algorithms leave them on stored objects): the oracle under the
+    default algorithms leave them on stored objects, and the two reco variables
+    of a TC from tracklets): the oracle under the
     GLIBC policy reproduces the compiled reference's bit patterns, object by
     object (barrel-only, full-geometry and endcap-heavy hard-mix events: every
     SG and TL variant)."""
     with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vars.npz")) as f:
         golden = {k: f[k] for k in f.files}
     ev = gen.event(wl, e)
-    sets, _, values = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, stages=4, vars=True)
+    sets, _, values = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, stages=6, vars=True)
     for k, s in VAR_KEYS:
         ref = golden[f"w{wl}_e{e}_{k}"]
         assert len(ref) > 0, f"{k}: empty golden set makes the check vacuous"
