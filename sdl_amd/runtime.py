@@ -224,8 +224,8 @@ class Event:
         return out
 
     def export_vars(self, stage, e=0):
-        """Per-object values of the MDs, SGs or TLs of event e (float32 [n, nvars],
-        object order of export; names: var_names(stage))."""
+        """Per-object values of the MDs, SGs, TLs or TCs from tracklets of event e
+        (float32 [n, nvars], object order of export; names: var_names(stage))."""
         L = lib()
         nv = L.sdl_object_var_count(stage)
         n = ctypes.c_longlong(0)
