@@ -90,3 +90,26 @@ def test_object_vars_layer_chain_and_pseudo():
     _compare(ev, sets, values, "pseudo", STAGES[:3])
     md = ev.export_vars(runtime.STAGE_MD)
     assert (md[:, :6] == 0).all() and np.isnan(md[:, 6]).all()
+
+
+def test_object_vars_of_batched_events():
+    """Events 1 and 2 of a three-event batch (workload 12 events 0-2): the values
+    of each event's objects equal those of the same event run alone, bit for bit
+    and in the same object order (the per-event ranges of the export)."""
+    print("case vars batch of 3 workload-12 events", flush=True)
+    g = runtime.Geometry(*gen.geometry(gen.geom_kind(12))[1])
+    xs = [gen.event(12, e) for e in range(3)]
+    batch = runtime.Event(g)
+    for k, x in enumerate(xs):
+        if k:
+            batch.next_event()
+        batch.add_hits(x.x, x.y, x.z, x.idx, x.detId)
+    batch.run_chain()
+    batch.sync()
+    for k in (1, 2):
+        single, _ = _device(12, k)
+        for name, st in STAGES:
+            got, ref = batch.export_vars(st, k), single.export_vars(st)
+            assert got.shape == ref.shape and len(ref) > 0, (k, name, got.shape, ref.shape)
+            assert np.array_equal(_bits(got), _bits(ref)), f"batch event {k} {name} values differ"
+
