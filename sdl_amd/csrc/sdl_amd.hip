@@ -1119,17 +1119,20 @@ __global__ void k_scatter_by(Count cn, const int* key, const long long* start, i
 // Triplet = two SGs sharing the middle MD (Triplet.cc:50-122, Segment.cc:861-866).
 // The inner SG s ends at MD m; the outer SGs stored at m's module with inner MD m
 // are exactly the dense range of SGs whose inner MD is m.
-__global__ void k_tp_write(Count cSG, const int* sgOut, const long long* sgStart, const long long* tpStart, int* tpIn,
-                           int* tpOutArr, Guard g) {
+// tpKey: the counter key of the TP's inner SG (its layer bin), which the TC and
+// counter kernels would otherwise reach through tpIn.
+__global__ void k_tp_write(Count cSG, const int* sgOut, const long long* sgStart, const long long* tpStart,
+                           const int* sgKey, int* tpIn, int* tpOutArr, int* tpKey, Guard g) {
     if (g.over()) return;
     const int nSG = int(cSG.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSG) return;
-    const int m = sgOut[s];
+    const int m = sgOut[s], key = sgKey[s];
     const long long a = sgStart[m], n = sgStart[m + 1] - a, d = tpStart[s];
     for (long long k = 0; k < n; ++k) {
         tpIn[d + k] = s;
         tpOutArr[d + k] = int(a + k);
+        tpKey[d + k] = key;
     }
 }
 
@@ -1835,6 +1838,7 @@ struct CounterIn {
     const long long* tcCand;  // per TL: TC candidates of the layer-pair form (null: the module-map count)
     long long* connSum;       // per slot, [3][NS]: SGs, TPs, TLs on the modules connected to the slot's module
     const int* sgKey;         // per SG: event * kNBins + layer bin of its inner MD's lower module
+    const int* tpKey;         // per TP: sgKey of its inner SG
 };
 
 __device__ __forceinline__ long long connected(const CounterIn& C, int slot, const long long* slotStart) {
@@ -1936,7 +1940,7 @@ __global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long lon
                     w[3] = 8; v[3] = C.tlCount[j];
                 }
             } else if ((j -= nSg) < nTp) {  // TC candidates per inner TP: TPs at modules connected to its outermost MD (Event.cc:889-982)
-                key = C.sgKey[C.tpIn[j]];
+                key = C.tpKey[j];
                 w[0] = 4; v[0] = C.connSum[C.NS + C.mdSlot[C.sgOut[C.tpOut[j]]]];
             } else {  // TC candidates and objects per inner TL (Event.cc:999-1062; layer pairs: Event.cc:840-872)
                 j -= nTp;
@@ -1984,7 +1988,7 @@ __device__ __forceinline__ void binCount(unsigned long long* bins, int key) {
 // inner TP (Event.cc:893-897,965) in a per-block LDS histogram.
 constexpr int kTcpLanes = 8;
 __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long* start, const int2* info,
-                                                      const int* tpEndList, const int* tpIn, const int* sgKey, int E,
+                                                      const int* tpEndList, const int* tpKey, int E,
                                                       int2* tc, unsigned long long* counters, Guard g) {
     if (g.over()) return;
     const int nTL = int(cTL.get());
@@ -2003,7 +2007,7 @@ __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long
         for (long long q = gl; q < n; q += kTcpLanes) {
             const int t = tpEndList[f.x + q];
             tc[a + q] = make_int2(t, f.y);
-            const int key = sgKey[tpIn[t]];  // the TC's inner TP's inner SG
+            const int key = tpKey[t];  // the TC's inner TP's inner SG
             if (lds) binCount(bins, key);
             else atomicAdd(&counters[((key / kNBins) * kNCounters + 9) * kNBins + key % kNBins], 1ULL);
         }
@@ -2327,7 +2331,7 @@ struct sdl_event {
     DBuf<int> sgFlags, sgKey;
     DBuf<int> sgEndCount, sgEndFill, sgEndList;
     DBuf<long long> sgEndStart;
-    DBuf<int> tpCount, tpIn, tpOut, tpeCount, tpEndList;
+    DBuf<int> tpCount, tpIn, tpOut, tpKey, tpeCount, tpEndList;
     DBuf<long long> tpeOff;
     DBuf<long long> tpStart, slotTpStart;
     DBuf<long long> tlCap, tlCapOff, tlStart, slotTlStart;
@@ -2753,9 +2757,11 @@ int runTP(sdl_event* ev) {
     ev->nTP = nTP;
     if (int r = ev->tpIn.ensure(std::max(nTP, 1LL))) return r;
     if (int r = ev->tpOut.ensure(std::max(nTP, 1LL))) return r;
+    if (int r = ev->tpKey.ensure(std::max(nTP, 1LL))) return r;
     if (nSG > 0)
         hipLaunchKernelGGL(k_tp_write, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
-                           ev->sgStart.p, ev->tpStart.p, ev->tpIn.p, ev->tpOut.p, guardOf(ev, TOT_TP, nTP, SDL_STAGE_TP));
+                           ev->sgStart.p, ev->tpStart.p, ev->sgKey.p, ev->tpIn.p, ev->tpOut.p, ev->tpKey.p,
+                           guardOf(ev, TOT_TP, nTP, SDL_STAGE_TP));
     hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->slotSgStart.p, ev->tpStart.p, ev->slotTpStart.p);
     return 0;
@@ -2948,7 +2954,7 @@ int runTCP(sdl_event* ev) {
     if (int r = ev->tcp.ensure(std::max(nTCP, 1LL))) return r;
     if (nTCP > 0)
         hipLaunchKernelGGL(k_tcp_write, dim3(std::min(gridFor(nTL, kBlock / kTcpLanes), 65536u)), dim3(kBlock), ev->E * kNBins <= kMaxLdsBins ? size_t(ev->E) * kNBins * 8 : 0, st, cTL,
-                           ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpIn.p, ev->sgKey.p, ev->E,
+                           ev->tcpStart.p, ev->tcpInfo.p, ev->tpEndList.p, ev->tpKey.p, ev->E,
                            ev->tcp.p, ev->counters.p, guardOf(ev, TOT_TCP, nTCP, SDL_STAGE_TCP));
     return 0;
 }
@@ -3161,6 +3167,7 @@ int launchCounters(sdl_event* ev, int s0, int s1) {
     if (int r = ev->connSum.ensure(3LL * C.NS)) return r;
     C.connSum = ev->connSum.p;
     C.sgKey = ev->sgKey.p;
+    C.tpKey = ev->tpKey.p;
     if (C.stages & (4 | 16 | 32))
         hipLaunchKernelGGL(k_conn_sums, dim3(gridFor((long long)C.NS * kConnLanes, kBlock)), dim3(kBlock), 0, ev->st,
                            C, ev->tot.p);
