@@ -114,8 +114,13 @@ int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offse
  * module-map joins TP, TCP and TCL assume module-map inputs and do not follow a
  * layer-pair stage).
  * The layer-pair stages store their objects as the SG, TL and TCL objects of
- * the batch.  algo: 0 = Default, the only algorithm with device
- * kernels. */
+ * the batch.  algo: 0 = Default; for the three tracklet stages also 1 =
+ * DefaultNm1_TLAlgo (Algo.h:21-26; Tracklet.cc:127-131): the default cuts
+ * with the last one, dBeta, waived (Tracklet.cc:1513-1536, 1990-2014,
+ * 2636-2653).  After DefaultNm1 module-map tracklets TCP is refused (the
+ * reference re-runs the default algorithm on the bridges, Event.cc:940-946,
+ * which the device join over stored tracklets cannot reproduce) and TCL may
+ * follow TL directly.  sdl_event_run_chain always runs the defaults. */
 int sdl_event_run(sdl_event* ev, int stage, int algo);
 /* Runs MD, SG, TP, TL, TCP and TCL back to back.  After a first call has
  * measured the batch's object counts, later calls size their buffers from

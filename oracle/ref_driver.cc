@@ -22,6 +22,8 @@
 //   sdl_ref_driver dumppseudo  <workload> <geomdir> <event0> <nevents> <outfile>
 //     (createPseudoMiniDoubletsFromAnchorModule, then the module-map chain)
 //     (createMiniDoublets, createSegmentsWithModuleMap, createTrackletsViaNavigation)
+//   sdl_ref_driver dumpnm1     <workload> <geomdir> <event0> <nevents> <outfile>
+//     (the module-map chain with createTrackletsWithModuleMap(DefaultNm1_TLAlgo))
 //   sdl_ref_driver dumpvars    <workload> <geomdir> <event0> <nevents> <outfile>
 //     (the module-map chain; every MD, SG and TL with its typed per-object
 //     values and reco variables, TLs with their whole reco-variable map, TCs
@@ -322,6 +324,25 @@ int main(int argc, char** argv) {
             sev.createTrackCandidatesFromTracklets();
             dumpEvent(f, sev, ie);
             dumpLayerCounters(f, sev);
+        }
+        std::fclose(f);
+        return 0;
+    }
+    if (mode == "dumpnm1") {  // the module-map chain with createTrackletsWithModuleMap(DefaultNm1_TLAlgo)
+        FILE* f = std::fopen(argv[6], "w");
+        if (!f) return 3;
+        for (long ie = e0; ie < e0 + ne; ++ie) {
+            sdlgen::Event ev;
+            sdlgen::generateEvent(g, cfg, ie, ev);
+            SDL::Event sev;
+            fill(sev, ev);
+            sev.createMiniDoublets();
+            sev.createSegmentsWithModuleMap();
+            sev.createTriplets();
+            sev.createTrackletsWithModuleMap(SDL::DefaultNm1_TLAlgo);
+            sev.createTrackCandidatesFromTriplets();
+            sev.createTrackCandidatesFromTracklets();
+            dumpEvent(f, sev, ie);
         }
         std::fclose(f);
         return 0;

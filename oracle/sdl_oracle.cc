@@ -297,6 +297,9 @@ struct Event {
     // (objectVars), the algorithms write them here (layouts: kMDVars, kSGVars,
     // kTLVars); null otherwise.
     mutable float* rec = nullptr;
+    // DefaultNm1_TLAlgo (Tracklet.cc:127-131): the default tracklet algorithm
+    // with its last cut, dBeta, waived (Tracklet.cc:1513-1536, 1990-2014, 2636-2653)
+    bool tlNm1 = false;
 
     // Event::getModule (Event.cc:54-78)
     int getSlot(uint32_t d) {
@@ -906,7 +909,7 @@ struct Event {
         const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
                                       0.25 * sq(double(std::abs(betaInRHmin - betaInRHmax) + std::abs(betaOutRHmin - betaOutRHmax))));
         const float dBeta = betaIn - betaOut;
-        CUT(46, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
+        if (!tlNm1) CUT(46, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
         if (rec) { rec[4] = dBeta; rec[5] = std::sqrt(dBetaCut2); }  // Tracklet.cc:1498-1499
         if (tk) {  // :1438-1494
             tlCommonKeys(sIn, sOut, tk);
@@ -1017,7 +1020,7 @@ struct Event {
         const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + dBetaROut2) +
                                       0.25 * sq(double(std::abs(betaInRHmin - betaInRHmax) + std::abs(betaOutRHmin - betaOutRHmax))));
         const float dBeta = betaIn - betaOut;
-        CUT(58, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
+        if (!tlNm1) CUT(58, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
         if (rec) rec[4] = dBeta;  // the only typed value this variant sets (Tracklet.cc:1977); the rest stay 0
         return true;
     }
@@ -1121,7 +1124,7 @@ struct Event {
         const float dBetaCut2 = float(double(dBetaRes * dBetaRes * 2.0f + dBetaMuls * dBetaMuls + dBetaLum2 + 0.f + 0.f) +
                                       0.25 * sq(double(std::abs(betaInRHmin - betaInRHmax) + std::abs(betaOutRHmin - betaOutRHmax))));
         const float dBeta = betaIn - betaOut;
-        CUT(68, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
+        if (!tlNm1) CUT(68, dBeta * dBeta <= dBetaCut2, dBeta * dBeta, kNone, dBetaCut2);
         if (rec) { rec[4] = dBeta; rec[5] = std::sqrt(dBetaCut2); }  // Tracklet.cc:2621-2622
         if (tk) {  // :2576-2617
             tlCommonKeys(sIn, sOut, tk);
@@ -1440,6 +1443,10 @@ struct Event {
                 case 8: createTrackCandidatesLayers(); break;
                 case 9: createTrackletsNavigation(); break;
                 case 10: createPseudoMiniDoublets(); break;
+                // the three tracklet stages with DefaultNm1_TLAlgo
+                case 11: tlNm1 = true; createTracklets(); tlNm1 = false; break;
+                case 12: tlNm1 = true; createTrackletsLayers(); tlNm1 = false; break;
+                case 13: tlNm1 = true; createTrackletsNavigation(); tlNm1 = false; break;
                 default: break;
             }
         }
@@ -1489,12 +1496,16 @@ struct Event {
                 sgAlgo(sg.mdIn, sg.mdOut, t, nullptr);
                 put();
             }
-        if (stage == ORACLE_TL)
+        if (stage == ORACLE_TL) {
+            const bool nm1 = tlNm1;
+            tlNm1 = true;  // every stored TL passed; under DefaultNm1 it may fail the waived cut
             for (const PairRec& tl : tls) {
                 std::fill(buf, buf + kTLVars, 0.f);
                 tlAlgo(sgs[tl.in], sgs[tl.out], nullptr);
                 put();
             }
+            tlNm1 = nm1;
+        }
         if (stage == ORACLE_TCL)
             for (const auto& tc : tcs)
                 if (tc.first == 1) {

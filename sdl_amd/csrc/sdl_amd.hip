@@ -1508,7 +1508,7 @@ __global__ void k_tl_expand(Count cP, const long long* sgStart, const int2* pair
 template <int V>
 __global__ void __launch_bounds__(kBlock) k_tl_stage2(VarRange R, HitsDev H, MDDev D, SGDev S, const int2* cand,
                                                       const int2* pair, const float4* carry, unsigned char* flag,
-                                                      int* tlCount) {
+                                                      int* tlCount, int nm1) {
     long long b0, e0;
     R.get(b0, e0);
     const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1522,7 +1522,7 @@ __global__ void __launch_bounds__(kBlock) k_tl_stage2(VarRange R, HitsDev H, MDD
     const TLOutLo L = loadOutLo(pr.y, flo, D);
     const TLOutUp U = loadOutUp(so, flo, H, D, S);
     const TLCarry cc{cf.x, cf.y, cf.z};
-    const bool pass = tlStage2(V, (flo & 2) != 0, I, L, U, cc);
+    const bool pass = tlStage2(V, (flo & 2) != 0, I, L, U, cc, nullptr, nm1 != 0);
     flag[i] = pass ? 1 : 0;
     if (pass) atomicAdd(&tlCount[pr.x], 1);
 }
@@ -1608,7 +1608,7 @@ __global__ void k_tll_cap(Count cSG, int NM, const ModDev* mods, const int* sgIn
 __global__ void __launch_bounds__(kBlock) k_tll(Count cSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
                                                 const long long* layerOff, const int* layerModStart, const int* layerSg,
                                                 const int* capInt, const long long* capOff, int* slab, int* tlCount,
-                                                Guard g) {
+                                                int nm1, Guard g) {
     if (g.over()) return;
     const int nSG = int(cSG.get());
     const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1638,7 +1638,7 @@ __global__ void __launch_bounds__(kBlock) k_tll(Count cSG, int NM, const ModDev*
             TLCarry c{0.f, 0.f, 0.f};
             if (tlStage1(V, I, Lo, c)) {
                 const TLOutUp U = loadOutUp(so, flo, H, D, S);
-                pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c);
+                pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c, nullptr, nm1 != 0);
             }
         }
         const unsigned long long mask = __ballot(pass);
@@ -1683,7 +1683,7 @@ __global__ void k_tln_cap(Count cSG, const int* sgOut, const long long* sgStart,
 // tracklet stages with the variant of each candidate's anchors.
 __global__ void __launch_bounds__(kBlock) k_tln(Count cSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
                                                 const long long* sgStart, const int* capInt, const long long* capOff,
-                                                int* slab, int* tlCount, Guard g) {
+                                                int* slab, int* tlCount, int nm1, Guard g) {
     if (g.over()) return;
     const int nSG = int(cSG.get());
     const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1713,7 +1713,7 @@ __global__ void __launch_bounds__(kBlock) k_tln(Count cSG, int NM, const ModDev*
                     TLCarry c{0.f, 0.f, 0.f};
                     if (tlStage1(V, I, Lo, c)) {
                         const TLOutUp U = loadOutUp(int(s3), flo, H, D, S);
-                        pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c);
+                        pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c, nullptr, nm1 != 0);
                     }
                 }
             }
@@ -2100,7 +2100,8 @@ __global__ void k_vars_tl(long long a, long long n, int NM, const ModDev* mods, 
     const TLOutLo L = loadOutLo(mdOutLo, flo, D);  // the flags' inner-MD bits are those of every SG at mdOutLo
     const TLOutUp U = loadOutUp(so, flo, H, D, S);
     TLCarry c{0.f, 0.f, 0.f};
-    if (tlStage1(V, I, L, c, nullptr, tk) && tlStage2(V, (flo & 2) != 0, I, L, U, c, rec) && V != TL_BBEE) {
+    // a stored TL passed; under DefaultNm1 it may fail the waived cut, so the values are read with it waived
+    if (tlStage1(V, I, L, c, nullptr, tk) && tlStage2(V, (flo & 2) != 0, I, L, U, c, rec, true) && V != TL_BBEE) {
         // the anchors' module detIds: the lower sensor's, or its partner's when the anchor is the upper hit
         const int md[4] = {S.in[s], S.out[s], mdOutLo, S.out[so]};
         uint32_t detId[4];
@@ -2332,6 +2333,7 @@ struct sdl_event {
     DBuf<int> sgEndCount, sgEndFill, sgEndList;
     DBuf<long long> sgEndStart;
     DBuf<int> tpCount, tpIn, tpOut, tpKey, tpeCount, tpEndList;
+    int tlNm1 = 0;  // the tracklet stage runs DefaultNm1_TLAlgo (dBeta cut waived)
     DBuf<long long> tpeOff;
     DBuf<long long> tpStart, slotTpStart;
     DBuf<long long> tlCap, tlCapOff, tlStart, slotTlStart;
@@ -2895,13 +2897,13 @@ int runTL(sdl_event* ev) {
         const dim3 grid(gridFor(nc[v], b2));
         if (v == 0)
             hipLaunchKernelGGL(k_tl_stage2<TL_BBBB>, grid, dim3(b2), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
-                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p);
+                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p, ev->tlNm1);
         else if (v == 1)
             hipLaunchKernelGGL(k_tl_stage2<TL_BBEE>, grid, dim3(b2), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
-                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p);
+                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p, ev->tlNm1);
         else
             hipLaunchKernelGGL(k_tl_stage2<TL_EEEE>, grid, dim3(b2), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
-                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p);
+                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p, ev->tlNm1);
     }
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
     const long long nTL = sizeOf(ev, TOT_TL);
@@ -3024,7 +3026,7 @@ int runTLLayers(sdl_event* ev) {
     if (nSG > 0)
         hipLaunchKernelGGL(k_tll, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), sgView(ev), ev->layerSgOff.p, G->dLayerModStart, ev->layerSg.p,
-                           ev->tlCapInt.p, ev->tlCapOff.p, ev->tllSlab.p, ev->tlCount.p,
+                           ev->tlCapInt.p, ev->tlCapOff.p, ev->tllSlab.p, ev->tlCount.p, ev->tlNm1,
                            guardOf(ev, TOT_TLCANDS, slab, SDL_STAGE_TL));
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
     const long long nTL = sizeOf(ev, TOT_TL);
@@ -3064,7 +3066,7 @@ int runTLNavigation(sdl_event* ev) {
     if (nSG > 0)
         hipLaunchKernelGGL(k_tln, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), sgView(ev), ev->sgStart.p, ev->tlCapInt.p, ev->tlCapOff.p,
-                           ev->tllSlab.p, ev->tlCount.p, guardOf(ev, TOT_TLCANDS, slab, SDL_STAGE_TL));
+                           ev->tllSlab.p, ev->tlCount.p, ev->tlNm1, guardOf(ev, TOT_TLCANDS, slab, SDL_STAGE_TL));
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
     const long long nTL = sizeOf(ev, TOT_TL);
     if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
@@ -3655,8 +3657,10 @@ int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offse
 
 int sdl_event_run(sdl_event* ev, int stage, int algo) {
     if (!ev) return fail(SDL_ERR_ARG, "null event");
-    if (algo != 0) return fail(SDL_ERR_ARG, "only the default algorithm has device kernels");
     if (stage < SDL_STAGE_MD || stage > SDL_STAGE_MD_PSEUDO) return fail(SDL_ERR_ARG, "bad stage");
+    const bool tlStage = stage == SDL_STAGE_TL || stage == SDL_STAGE_TL_LAYERS || stage == SDL_STAGE_TL_NAV;
+    if (algo != 0 && !(tlStage && algo == 1))
+        return fail(SDL_ERR_ARG, "only the default algorithms (and DefaultNm1 for tracklets) have device kernels");
     // each stage reads the objects of the stage before it: module-map chain MD,
     // SG, TP, TL, TCP, TCL; layer-pair stages after MD (segments), after either
     // segment stage (tracklets) and after either tracklet stage (track
@@ -3673,7 +3677,15 @@ int sdl_event_run(sdl_event* ev, int stage, int algo) {
         ok = ev->stage == SDL_STAGE_TL || ev->stage == SDL_STAGE_TL_LAYERS || ev->stage == SDL_STAGE_TL_NAV;
     // navigation tracklets over module-map segments keep the TCL join exact
     if (stage == SDL_STAGE_TCL && ev->stage == SDL_STAGE_TL_NAV) ok = !(ev->ranMask & (1u << SDL_STAGE_SG_LAYERS));
+    // TCs from triplets re-run the default tracklet algorithm on their bridges
+    // (Event.cc:940-946); the device join reads the stored tracklets instead, which
+    // is exact only for default ones: after DefaultNm1 tracklets it is refused, and
+    // TCs from tracklets (a join over the stored tracklets) may follow directly
+    if (stage == SDL_STAGE_TCP && ev->stage == SDL_STAGE_TL && ev->tlNm1)
+        return fail(SDL_ERR_STATE, "TCs from triplets need default tracklets (the stored set is DefaultNm1)");
+    if (stage == SDL_STAGE_TCL && ev->stage == SDL_STAGE_TL && ev->tlNm1) ok = true;
     if (!ok) return fail(SDL_ERR_STATE, "stages must run in order");
+    if (tlStage) ev->tlNm1 = algo == 1 ? 1 : 0;
     if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
     if (!ev->hitsReady) {
         if (ev->hostStaged && ev->hoff.size() == 1) ev->hoff.push_back(0);
@@ -3689,6 +3701,7 @@ int sdl_event_run(sdl_event* ev, int stage, int algo) {
 
 int sdl_event_run_chain(sdl_event* ev) {
     if (!ev) return fail(SDL_ERR_ARG, "null event");
+    ev->tlNm1 = 0;  // the chain runs the default algorithms
     if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
     ev->stage = -1;
     ev->hitsReady = false;

@@ -766,8 +766,9 @@ __device__ __forceinline__ bool tlStage1(int variant, const TLInner& I, const TL
 // _v2: Tracklet.cc:1097-1100 and 1156-1501
 // rec (a passing TL's per-object values, kTLVars floats): getBetaIn, getBetaInCut,
 // getBetaOut, getBetaOutCut, getDeltaBeta, getDeltaBetaCut, then the map keys.
+// nm1: DefaultNm1_TLAlgo (Tracklet.cc:127-131, 1513-1536), the last cut (dBeta) waived.
 __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L, const TLOutUp& U, const TLCarry& c,
-                                             float* rec = nullptr) {
+                                             float* rec = nullptr, bool nm1 = false) {
     const HitView& hInLo = I.lo;
     const HitView& hInUp = I.up;
     const HitView& hOutLo = L.a;
@@ -834,13 +835,13 @@ __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L,
         tk[TK_pt_betaOut] = float(double(drt_tl_axis * k2) / sin(double(betaOut)));
         tk[TK_pt_beta] = pt_beta; tk[TK_betaAv] = betaAv;
     }
-    return dBeta * dBeta <= dBetaCut2;
+    return nm1 || dBeta * dBeta <= dBetaCut2;
 }
 
 // BBEE: Tracklet.cc:1774-2016; EEEE: Tracklet.cc:2273-2665
 template <bool EEEE>
 __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, const TLOutUp& U, const TLCarry& c,
-                                           float* rec = nullptr) {
+                                           float* rec = nullptr, bool nm1 = false) {
     const HitView& hInLo = I.lo;
     const HitView& hInUp = I.up;
     const HitView& hOutLo = L.a;
@@ -940,16 +941,17 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
         tk[TK_pt_betaOut] = float(double(dr * k2) / sin(double(betaOut)));
         tk[TK_pt_beta] = pt_beta; tk[TK_betaAv] = betaAv;
     }
-    return dBeta * dBeta <= dBetaCut2;
+    return nm1 || dBeta * dBeta <= dBetaCut2;
 }
 
 // bOutUp: the outer segment's outer MD is in the barrel (the fourth subdet of the dispatch)
 __device__ __forceinline__ bool tlStage2(int variant, bool bOutUp, const TLInner& I, const TLOutLo& L,
-                                         const TLOutUp& U, const TLCarry& c, float* rec = nullptr) {
-    if (variant == TL_BBBB) return tlStage2BBBB(I, L, U, c, rec);
+                                         const TLOutUp& U, const TLCarry& c, float* rec = nullptr,
+                                         bool nm1 = false) {
+    if (variant == TL_BBBB) return tlStage2BBBB(I, L, U, c, rec, nm1);
     if (bOutUp) return false;
-    if (variant == TL_BBEE) return tlStage2EC<false>(I, L, U, c, rec);
-    return tlStage2EC<true>(I, L, U, c, rec);
+    if (variant == TL_BBEE) return tlStage2EC<false>(I, L, U, c, rec, nm1);
+    return tlStage2EC<true>(I, L, U, c, rec, nm1);
 }
 
 }  // namespace sdl

@@ -83,7 +83,8 @@ void Event::addHitToModule(Hit hit, unsigned int detId) {
 }
 
 bool Event::run(int stage, int algo, const char* what) {
-    if (algo != 0) {
+    const bool tl = stage == SDL_STAGE_TL || stage == SDL_STAGE_TL_LAYERS || stage == SDL_STAGE_TL_NAV;
+    if (algo != 0 && !(tl && algo == DefaultNm1_TLAlgo)) {
         std::printf("SDL::  Warning: %s algorithm %d has no device kernels; stage left empty\n", what, algo);
         return false;
     }

@@ -7,9 +7,10 @@
 //    the geometry is built when the first Event is created after a load();
 //  * object graphs (Module / Layer / MiniDoublet pointers) are not materialised:
 //    objects are read back as hit-index tuples (getMiniDoublets() etc.);
-//  * only the Default algorithms run on the device; another algorithm value
-//    prints a warning and leaves the stage empty, as the reference does for an
-//    unrecognised value (MiniDoublet.cc:216-220);
+//  * only the Default algorithms, and DefaultNm1_TLAlgo for the three tracklet
+//    stages, run on the device; another algorithm value prints a warning and
+//    leaves the stage empty, as the reference does for an unrecognised value
+//    (MiniDoublet.cc:216-220);
 //  * per-object values (the objects' getters and reco variables) are read back
 //    as float rows parallel to the object tuples (getMiniDoubletVariables() etc.).
 #ifndef SDL_AMD_EVENT_H

@@ -52,6 +52,8 @@ def main():
         pseudo_fixtures()
     if which in ("all", "vars"):
         var_fixtures()
+    if which in ("all", "nm1"):
+        nm1_fixtures()
 
 
 def sorted_rows(keys, bits):
@@ -136,6 +138,22 @@ def nav_fixtures():
                 out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
                 out[f"w{wl}_e{e}_layers"] = counters["layers"]
     np.savez_compressed(os.path.join(HERE, "ref_nav_sets.npz"), **out)
+
+
+NM1_CASES = [(1, 0, 1), (15, 0, 1)]
+
+
+def nm1_fixtures():
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for wl, e0, ne in NM1_CASES:
+            dump = os.path.join(tmp, f"w{wl}.txt")
+            subprocess.run([oracle.REF_DRIVER, "dumpnm1", str(wl), tmp, str(e0), str(ne), dump], check=True)
+            for e, (sets, counters) in oracle.parse_ref_dump(dump).items():
+                for s in oracle.STAGES:
+                    out[f"w{wl}_e{e}_{s}"] = oracle.as_multiset(sets[s])
+                out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
+    np.savez_compressed(os.path.join(HERE, "ref_nm1_sets.npz"), **out)
 
 
 def pseudo_fixtures():

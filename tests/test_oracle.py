@@ -169,6 +169,28 @@ def test_oracle_layer_chain_closed_forms(geom_paths):
 NAV_CASES = [(1, 0), (1, 1), (12, 0), (12, 1), (15, 0)]
 
 
+@pytest.mark.parametrize("wl,e", [(1, 0), (15, 0)])
+def test_oracle_nm1_tracklets_match_reference(wl, e, geom_paths):
+    """createTrackletsWithModuleMap(DefaultNm1_TLAlgo) in the module-map chain,
+    against the compiled reference (ref_nm1_sets.npz): all six object sets and
+    the counters; the waived dBeta cut makes the TL set a strict superset of
+    the default one."""
+    import os
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_nm1_sets.npz")) as f:
+        golden = {k: f[k] for k in f.files}
+    ev = gen.event(wl, e)
+    paths = geom_paths[gen.geom_kind(wl)]
+    sets, counters = oracle.run(paths, ev, oracle.GLIBC, [0, 1, 2, 11, 4, 5])
+    for s in oracle.STAGES:
+        ref = golden[f"w{wl}_e{e}_{s}"]
+        got = oracle.as_multiset(sets[s])
+        assert got.shape == ref.shape and np.array_equal(got, ref), s
+    assert [counters[k] for k in COUNTERS] == list(golden[f"w{wl}_e{e}_counters"])
+    plain, _ = oracle.run(paths, ev, oracle.GLIBC, 6)
+    nm1 = {tuple(r) for r in sets["TL"]}
+    assert {tuple(r) for r in plain["TL"]} < nm1
+
+
 @pytest.mark.parametrize("wl,e", NAV_CASES)
 def test_oracle_navigation_matches_reference(wl, e, geom_paths):
     """createTrackletsViaNavigation after the module-map segments, against the
