@@ -2,6 +2,8 @@
 
 * ``sdl_amd/_lib/libsdl_amd.so`` -- the HIP kernels and the C-ABI (gfx950).
 * ``sdl_amd/_lib/libsdl_gen.so`` -- the synthetic detector / event generator (host C++).
+* ``sdl_amd/_lib/libsdl_crmath_check.so`` -- checks of the correctly rounded
+  float functions against the host and device libm (test support).
 * ``sdl_amd/_lib/libsdl_shim.so`` and ``sdl_amd/_lib/sdl_shim_demo`` -- the C++
   ``SDL::Event`` shim over the C-ABI and a small driver that uses it.
 """
@@ -73,10 +75,19 @@ def build_calib():
     return out
 
 
+def build_check():
+    """Test support: the host and device scans of sdl_crmath.h."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libsdl_crmath_check.so")
+    _run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(CSRC, "check", "crmath_check.hip"), "-o", out])
+    return out
+
+
 def build_all(with_shim=True):
     build_gen()
     build_hip()
     build_calib()
+    build_check()
     if with_shim and os.path.exists(os.path.join(CSRC, "shim", "sdl_shim.cc")):
         build_shim()
 

@@ -6,7 +6,9 @@
 // arithmetic: sqrt/sin/tan/pow/atan2 and 0.2*, 1./, M_PI terms).  The float
 // transcendentals that the reference takes from libm in float (asinf, atanf,
 // sinf, cosf, tanf) are evaluated in FP64 and rounded once, i.e. correctly
-// rounded in practice.  The unqualified abs(float) of the reference binds to
+// rounded in practice; asin, atan, sin, cos, atan2 and a / sin(b) go through
+// sdl_crmath.h, which returns the same float as that FP64 evaluation at a
+// fraction of its cost.  The unqualified abs(float) of the reference binds to
 // int abs(int) (MiniDoublet.cc:273,787, Segment.cc:743-744) and is emulated
 // with the x86 truncation rule.  Compiled with -ffp-contract=off: no FMAs.
 #pragma once
@@ -14,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <climits>
 #include <cstdint>
+
+#include "sdl_crmath.h"
 
 namespace sdl {
 
@@ -50,10 +54,10 @@ __device__ __forceinline__ float k2Rinv() {
     return float(double(float(2.99792458e-3 * 3.8)) / 2.);
 }
 
-__device__ __forceinline__ float asinF(float v) { return float(asin(double(v))); }
-__device__ __forceinline__ float atanF(float v) { return float(atan(double(v))); }
-__device__ __forceinline__ float sinF(float v) { return float(sin(double(v))); }
-__device__ __forceinline__ float cosF(float v) { return float(cos(double(v))); }
+__device__ __forceinline__ float asinF(float v) { return crm::asinCR(v); }    // float(asin(double(v)))
+__device__ __forceinline__ float atanF(float v) { return crm::atanCR(v); }    // float(atan(double(v)))
+__device__ __forceinline__ float sinF(float v) { return crm::sinCR(v); }      // float(sin(double(v)))
+__device__ __forceinline__ float cosF(float v) { return crm::cosCR(v); }      // float(cos(double(v)))
 __device__ __forceinline__ float tanF(float v) { return float(tan(double(v))); }
 __device__ __forceinline__ float acosF(float v) { return float(acos(double(v))); }
 __device__ __forceinline__ float sqrtF(float v) { return float(sqrt(double(v))); }
@@ -77,7 +81,7 @@ __device__ __forceinline__ float phiMpiPi(float x) {
 }
 // MathUtil.cc:21-27
 __device__ __forceinline__ float atan2Sdl(float y, float x) {
-    if (x != 0) return float(atan2(double(y), double(x)));
+    if (x != 0) return crm::atan2CR(y, x);  // float(atan2(double(y), double(x)))
     if (y == 0) return 0.f;
     if (y > 0) return float(kPI / 2);
     return float(-kPI / 2);
@@ -412,7 +416,7 @@ __device__ __forceinline__ void deltaBetaIter(float& betaIn, float& betaOut, flo
     auto corr = [&](float len, float pt, float signOf) {
         return copysign(double(asinF(fmin2(len * k2 / fabsF(pt), kSinAlphaMax))), double(signOf));
     };
-    auto ptOf = [&](float beta) { return float(double(dr * k2) / sin(double(beta))); };
+    auto ptOf = [&](float beta) { return crm::divSinCR(dr * k2, beta); };
     if (betaIn * betaOut > 0.f &&
         (fabsF(ptBeta) < 4.f * ptBetaMax || (lIn >= 11 && fabsF(ptBeta) < 8.f * ptBetaMax))) {
         const float in0 = betaIn, out0 = betaOut;
@@ -803,7 +807,7 @@ __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L,
         tk[TK_rawBetaIn] = betaIn; tk[TK_rawBetaOut] = betaOut; tk[TK_betaIn_cut] = betaIn_cut;
     }
     float betaAv = 0.5f * (betaIn + betaOut);
-    float pt_beta = float(double(drt_tl_axis * k2) / sin(double(betaAv)));
+    float pt_beta = crm::divSinCR(drt_tl_axis * k2, betaAv);
     const int lOut = isEC_lastLayer ? 11 : 5;
     const float sdOut_dr = U.dr;
     const float sdOut_d = U.d;
@@ -831,8 +835,8 @@ __device__ __forceinline__ bool tlStage2BBBB(const TLInner& I, const TLOutLo& L,
         tk[TK_dBeta] = dBeta; tk[TK_dBetaCut2] = dBetaCut2; tk[TK_dBetaLum2] = dBetaLum2;
         tk[TK_dBetaMuls] = dBetaMuls; tk[TK_dBetaRIn2] = 0.f; tk[TK_dBetaROut2] = dBetaROut2;
         tk[TK_dBetaRes] = dBetaRes; tk[TK_drt_tl_axis] = drt_tl_axis; tk[TK_sdOut_d] = sdOut_d;
-        tk[TK_pt_betaIn] = float(double(drt_tl_axis * k2) / sin(double(betaIn)));  // :1393-1394
-        tk[TK_pt_betaOut] = float(double(drt_tl_axis * k2) / sin(double(betaOut)));
+        tk[TK_pt_betaIn] = crm::divSinCR(drt_tl_axis * k2, betaIn);  // :1393-1394
+        tk[TK_pt_betaOut] = crm::divSinCR(drt_tl_axis * k2, betaOut);
         tk[TK_pt_beta] = pt_beta; tk[TK_betaAv] = betaAv;
     }
     return nm1 || dBeta * dBeta <= dBetaCut2;
@@ -896,7 +900,7 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
         rec[1] = (-sdIn_dr * 1.f + dr) * k2 / kPtCut + (0.02f / sdIn_d);
     }
     float betaAv = 0.5f * (betaIn + betaOut);
-    float pt_beta = float(double(dr * k2) / sin(double(betaAv)));
+    float pt_beta = crm::divSinCR(dr * k2, betaAv);
     const int lIn = EEEE ? 11 : 5;
     const float sdOut_dr = U.dr;
     const float sdOut_d = U.d;
@@ -937,8 +941,8 @@ __device__ __forceinline__ bool tlStage2EC(const TLInner& I, const TLOutLo& L, c
         tk[TK_dBeta] = dBeta; tk[TK_dBetaCut2] = dBetaCut2; tk[TK_dBetaLum2] = dBetaLum2;
         tk[TK_dBetaMuls] = dBetaMuls; tk[TK_dBetaRIn2] = 0.f; tk[TK_dBetaROut2] = 0.f;
         tk[TK_dBetaRes] = dBetaRes; tk[TK_dr] = dr; tk[TK_sdOut_d] = sdOut_d;
-        tk[TK_pt_betaIn] = float(double(dr * k2) / sin(double(betaIn)));  // :2516-2517
-        tk[TK_pt_betaOut] = float(double(dr * k2) / sin(double(betaOut)));
+        tk[TK_pt_betaIn] = crm::divSinCR(dr * k2, betaIn);  // :2516-2517
+        tk[TK_pt_betaOut] = crm::divSinCR(dr * k2, betaOut);
         tk[TK_pt_beta] = pt_beta; tk[TK_betaAv] = betaAv;
     }
     return nm1 || dBeta * dBeta <= dBetaCut2;
