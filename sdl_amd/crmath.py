@@ -1,9 +1,10 @@
 """ctypes binding of the checks of sdl_crmath.h (libsdl_crmath_check.so).
 
 The cut algebra's float transcendentals (asin, sin, cos, atan, atan2 and
-a / sin(b)) are evaluated by short double-precision polynomials with a rounding
-test that falls back to the full libm call; the checks compare every result
-with float(f(double(v))) of the libm, on the host or on the device.  Test
+a / sin(b)) are evaluated by short double-precision polynomials with a
+rounding test that falls back to the full libm call, and float(sqrt(double(v)))
+by the float sqrt; the checks compare every result with float(f(double(v)))
+of the libm, on the host or on the device.  Test
 support only: the product library compiles the same header into its kernels."""
 import ctypes
 import os
@@ -11,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libsdl_crmath_check.so")
 
-ONE_ARG = {"asin": 0, "sin": 1, "cos": 2, "atan": 3}
+ONE_ARG = {"asin": 0, "sin": 1, "cos": 2, "atan": 3, "sqrt": 6}
 TWO_ARG = {"atan2": 4, "divsin": 5}
 _lib = None
 

@@ -19,7 +19,7 @@
 
 namespace {
 
-enum { F_ASIN = 0, F_SIN = 1, F_COS = 2, F_ATAN = 3, F_ATAN2 = 4, F_DIVSIN = 5 };
+enum { F_ASIN = 0, F_SIN = 1, F_COS = 2, F_ATAN = 3, F_ATAN2 = 4, F_DIVSIN = 5, F_SQRT = 6 };
 
 __host__ __device__ inline float bitsToFloat(uint32_t u) {
     float f;
@@ -47,7 +47,8 @@ __host__ __device__ inline Eval eval1(int fn, float v) {
         case F_ASIN: r.ref = float(asin(double(v))); r.fast = asinFast(v, f); r.got = asinCR(v); break;
         case F_SIN: r.ref = float(sin(double(v))); r.fast = sinFast(v, f); r.got = sinCR(v); break;
         case F_COS: r.ref = float(cos(double(v))); r.fast = cosFast(v, f); r.got = cosCR(v); break;
-        default: r.ref = float(atan(double(v))); r.fast = atanFast(v, f); r.got = atanCR(v); break;
+        case F_ATAN: r.ref = float(atan(double(v))); r.fast = atanFast(v, f); r.got = atanCR(v); break;
+        default: r.ref = float(sqrt(double(v))); r.fast = true; r.got = sqrtCR(v); break;
     }
     return r;
 }
@@ -149,12 +150,12 @@ void hostParallel(uint64_t n, int threads, F body) {
 
 extern "C" {
 
-// One-argument functions (0 asin, 1 sin, 2 cos, 3 atan) over the bit patterns
+// One-argument functions (0 asin, 1 sin, 2 cos, 3 atan, 6 sqrt) over the bit patterns
 // lo, lo + stride, ... < hi on the host: out[0] mismatches, out[1] fallbacks,
 // out[2] the first mismatching pattern (or 2^32), out[3] patterns checked.
 int crm_host_scan1(int fn, unsigned long long lo, unsigned long long hi, unsigned long long stride, int threads,
                    unsigned long long out[4]) {
-    if (fn < 0 || fn > 3 || stride == 0 || hi < lo || hi > (1ull << 32)) return -1;
+    if (fn < 0 || fn > 6 || fn == F_ATAN2 || fn == F_DIVSIN || stride == 0 || hi < lo || hi > (1ull << 32)) return -1;
     const uint64_t n = (hi - lo + stride - 1) / stride;
     std::atomic<unsigned long long> bad{0}, fb{0}, first{1ull << 32};
     hostParallel(n, threads, [&](int, uint64_t a, uint64_t b) {
@@ -223,7 +224,8 @@ int crm_pair(int fn, int mode, unsigned long long seed, unsigned long long i, fl
 
 // Fast and reference values of one call on the host: out = {libm path, function, fast decided}.
 int crm_host_eval(int fn, float a, float b, float out[3]) {
-    const Eval r = fn <= 3 ? eval1(fn, a) : eval2(fn, a, b);
+    const Eval r = (fn <= 3 || fn == F_SQRT) ? eval1(fn, a)
+                                              : eval2(fn, a, b);
     out[0] = r.ref;
     out[1] = r.got;
     out[2] = r.fast ? 1.f : 0.f;
@@ -233,7 +235,7 @@ int crm_host_eval(int fn, float a, float b, float out[3]) {
 // The same scans on the current HIP device, against the device libm
 // (out[2] of scan2: first mismatching pair index, or 2^32 - 1 when none).
 int crm_device_scan1(int fn, unsigned long long lo, unsigned long long hi, unsigned long long out[4]) {
-    if (fn < 0 || fn > 3 || hi < lo || hi > (1ull << 32)) return -1;
+    if (fn < 0 || fn > 6 || fn == F_ATAN2 || fn == F_DIVSIN || hi < lo || hi > (1ull << 32)) return -1;
     unsigned long long* dc = nullptr;
     uint32_t* df = nullptr;
     if (hipMalloc(&dc, 2 * sizeof(unsigned long long)) != hipSuccess) return -3;

@@ -46,10 +46,24 @@ constexpr double kPio4Lo = 0x1.1a62633145c07p-55;
 constexpr double kTanPi8 = 0x1.a827999fcef32p-2;   // tan(pi/8), the atan reduction point
 constexpr double k3Pio4 = 0x1.2d97c7f3321d2p1;     // 3 pi / 4: end of the fast sin/cos domain
 
+// One Horner step p z + c.  On the device the coefficient is passed through
+// an empty asm statement as a scalar-register value and added as -(-c): the
+// step is then one v_fma_f64 reading the coefficient from a scalar register
+// pair set by two s_mov_b32 on the scalar unit, where a literal coefficient
+// costs two v_mov_b32 to build the accumulator of a v_fmac_f64.  The value and
+// the rounding are those of fma(p, z, c) on host and device alike.
+__host__ __device__ __forceinline__ double hstep(double p, double z, double c) {
+    double n = -c;
+#ifdef __HIP_DEVICE_COMPILE__
+    asm("" : "+s"(n));
+#endif
+    return fma(p, z, -n);
+}
+
 // Rounding test: true (and f set) when all reals within |y| 2^-42 of y round
 // to the same float.  NaN or infinite y never passes.
 __host__ __device__ __forceinline__ bool roundTest(double y, float& f) {
-    const double e = fabs(y) * 0x1p-42;
+    const double e = ldexp(fabs(y), -42);
     const float lo = float(y - e), hi = float(y + e);
     f = lo;
     return lo == hi;
@@ -59,15 +73,15 @@ __host__ __device__ __forceinline__ bool roundTest(double y, float& f) {
 __host__ __device__ __forceinline__ double asinPoly(double x) {
     const double z = x * x;
     double p = 0x1.c93a92d53b4f1p-6;
-    p = fma(p, z, -0x1.815314c864b09p-9);
-    p = fma(p, z, 0x1.00d47e7966d94p-6);
-    p = fma(p, z, 0x1.b02442413f6bap-7);
-    p = fma(p, z, 0x1.1dc2ef640046fp-6);
-    p = fma(p, z, 0x1.6e72146fda29ep-6);
-    p = fma(p, z, 0x1.f1c81c59ea536p-6);
-    p = fma(p, z, 0x1.6db6d8e71341bp-5);
-    p = fma(p, z, 0x1.33333335a9cd6p-4);
-    p = fma(p, z, 0x1.5555555554f05p-3);
+    p = hstep(p, z, -0x1.815314c864b09p-9);
+    p = hstep(p, z, 0x1.00d47e7966d94p-6);
+    p = hstep(p, z, 0x1.b02442413f6bap-7);
+    p = hstep(p, z, 0x1.1dc2ef640046fp-6);
+    p = hstep(p, z, 0x1.6e72146fda29ep-6);
+    p = hstep(p, z, 0x1.f1c81c59ea536p-6);
+    p = hstep(p, z, 0x1.6db6d8e71341bp-5);
+    p = hstep(p, z, 0x1.33333335a9cd6p-4);
+    p = hstep(p, z, 0x1.5555555554f05p-3);
     return fma(x * z, p, x);
 }
 
@@ -75,11 +89,11 @@ __host__ __device__ __forceinline__ double asinPoly(double x) {
 __host__ __device__ __forceinline__ double sinPoly(double t) {
     const double z = t * t;
     double p = 0x1.5e0b19f8b1451p-33;
-    p = fma(p, z, -0x1.ae600b02b6262p-26);
-    p = fma(p, z, 0x1.71de37968a100p-19);
-    p = fma(p, z, -0x1.a01a019e83aaep-13);
-    p = fma(p, z, 0x1.1111111110bb2p-7);
-    p = fma(p, z, -0x1.5555555555555p-3);
+    p = hstep(p, z, -0x1.ae600b02b6262p-26);
+    p = hstep(p, z, 0x1.71de37968a100p-19);
+    p = hstep(p, z, -0x1.a01a019e83aaep-13);
+    p = hstep(p, z, 0x1.1111111110bb2p-7);
+    p = hstep(p, z, -0x1.5555555555555p-3);
     return fma(t * z, p, t);
 }
 
@@ -87,10 +101,10 @@ __host__ __device__ __forceinline__ double sinPoly(double t) {
 __host__ __device__ __forceinline__ double cosPoly(double t) {
     const double z = t * t;
     double p = 0x1.1c81c3531fff2p-29;
-    p = fma(p, z, -0x1.27e25f4bb4e6fp-22);
-    p = fma(p, z, 0x1.a019ff53a6a1cp-16);
-    p = fma(p, z, -0x1.6c16c16b614fcp-10);
-    p = fma(p, z, 0x1.5555555555437p-5);
+    p = hstep(p, z, -0x1.27e25f4bb4e6fp-22);
+    p = hstep(p, z, 0x1.a019ff53a6a1cp-16);
+    p = hstep(p, z, -0x1.6c16c16b614fcp-10);
+    p = hstep(p, z, 0x1.5555555555437p-5);
     return (1.0 - z * 0.5) + (z * z) * p;
 }
 
@@ -98,24 +112,28 @@ __host__ __device__ __forceinline__ double cosPoly(double t) {
 __host__ __device__ __forceinline__ double atanPoly(double w) {
     const double z = w * w;
     double p = -0x1.be2efe9a9e93bp-6;
-    p = fma(p, z, 0x1.a76e53429f68cp-5);
-    p = fma(p, z, -0x1.0c533de737bdfp-4);
-    p = fma(p, z, 0x1.3a9d98b72e3acp-4);
-    p = fma(p, z, -0x1.74563e04a6f0cp-4);
-    p = fma(p, z, 0x1.c71c382a9b0edp-4);
-    p = fma(p, z, -0x1.249248aa7aa01p-3);
-    p = fma(p, z, 0x1.99999998d17aap-3);
-    p = fma(p, z, -0x1.55555555553a4p-2);
+    p = hstep(p, z, 0x1.a76e53429f68cp-5);
+    p = hstep(p, z, -0x1.0c533de737bdfp-4);
+    p = hstep(p, z, 0x1.3a9d98b72e3acp-4);
+    p = hstep(p, z, -0x1.74563e04a6f0cp-4);
+    p = hstep(p, z, 0x1.c71c382a9b0edp-4);
+    p = hstep(p, z, -0x1.249248aa7aa01p-3);
+    p = hstep(p, z, 0x1.99999998d17aap-3);
+    p = hstep(p, z, -0x1.55555555553a4p-2);
     return fma(w * z, p, w);
 }
 
-// atan(t) for 0 <= t <= 1: above tan(pi/8), atan(t) = pi/4 + atan((t-1)/(t+1)),
-// whose argument has |.| < tan(pi/8); the result is then >= pi/8 and the
-// absolute error of the reduction (a few 2^-53) stays below 2^-50 relative.
-__host__ __device__ __forceinline__ double atanUnit(double t) {
-    const bool red = t > kTanPi8;
-    const double w = red ? (t - 1.0) / (t + 1.0) : t;
-    const double a = atanPoly(w);
+// atan(u / v) for 0 <= u <= v (doubles holding floats, v > 0).  Above
+// tan(pi/8), atan(u/v) = pi/4 + atan(w) with w = (u - v) / (u + v): u - v and
+// u + v are exact (u and v are floats within a factor 2.5 of each other), so
+// w is within 2^-53 relative, |w| < tan(pi/8), and the result is >= pi/8.
+// The operands are selected first, so a wave runs one division and one
+// polynomial whichever side of tan(pi/8) its lanes are on.
+__host__ __device__ __forceinline__ double atanRatio(double u, double v) {
+    const bool red = u > kTanPi8 * v;
+    const double num = red ? u - v : u;
+    const double den = red ? u + v : v;
+    const double a = atanPoly(num / den);
     return red ? kPio4Hi + (a + kPio4Lo) : a;
 }
 
@@ -157,21 +175,25 @@ __host__ __device__ __forceinline__ bool asinFast(float v, float& f) {
     }
     return roundTest(y, f);
 }
-__host__ __device__ __forceinline__ bool sinFast(float v, float& f) { return roundTest(sinSmall(double(v)), f); }
-__host__ __device__ __forceinline__ bool cosFast(float v, float& f) { return roundTest(cosSmall(double(v)), f); }
+__host__ __device__ __forceinline__ bool sinFast(float v, float& f) {
+    return roundTest(sinSmall(double(v)), f);
+}
+__host__ __device__ __forceinline__ bool cosFast(float v, float& f) {
+    return roundTest(cosSmall(double(v)), f);
+}
 // atan(|v|) = pi/2 - atan(1/|v|) above 1
 __host__ __device__ __forceinline__ bool atanFast(float v, float& f) {
     const double x = double(v), ax = fabs(x);
     const bool inv = ax > 1.0;
-    const double a = atanUnit(inv ? 1.0 / ax : ax);
+    const double a = atanRatio(inv ? 1.0 : ax, inv ? ax : 1.0);
     return roundTest(copysign(inv ? kPio2Hi - (a - kPio2Lo) : a, x), f);
 }
-// fx != 0.  The quotient of the smaller by the larger magnitude is within 2^-53
-// relative; the octant corrections add pi/2 and pi in two parts.
+// fx != 0.  atan of the smaller over the larger magnitude; the octant
+// corrections add pi/2 and pi in two parts.
 __host__ __device__ __forceinline__ bool atan2Fast(float fy, float fx, float& f) {
     const double y = double(fy), x = double(fx), ay = fabs(y), ax = fabs(x);
     const bool swap = ay > ax;
-    double r = atanUnit(swap ? ax / ay : ay / ax);
+    double r = atanRatio(swap ? ax : ay, swap ? ay : ax);
     if (swap) r = kPio2Hi - (r - kPio2Lo);
     if (x < 0) r = kPiHi - (r - kPiLo);
     return roundTest(copysign(r, y), f);
@@ -213,6 +235,10 @@ __host__ __device__ __forceinline__ float divSinCR(float a, float b) {
     float f;
     return divSinFast(a, b, f) ? f : float(double(a) / sin(double(b)));
 }
+// float(sqrt(double(v))): with 53 >= 2 * 24 + 2 bits, rounding the double root
+// to float gives the correctly rounded float root (the double rounding is
+// innocuous for sqrt), which sqrtf returns directly.
+__host__ __device__ __forceinline__ float sqrtCR(float v) { return sqrtf(v); }
 
 }  // namespace crm
 }  // namespace sdl

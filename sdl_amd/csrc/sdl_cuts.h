@@ -60,7 +60,7 @@ __device__ __forceinline__ float sinF(float v) { return crm::sinCR(v); }      //
 __device__ __forceinline__ float cosF(float v) { return crm::cosCR(v); }      // float(cos(double(v)))
 __device__ __forceinline__ float tanF(float v) { return float(tan(double(v))); }
 __device__ __forceinline__ float acosF(float v) { return float(acos(double(v))); }
-__device__ __forceinline__ float sqrtF(float v) { return float(sqrt(double(v))); }
+__device__ __forceinline__ float sqrtF(float v) { return crm::sqrtCR(v); }      // float(sqrt(double(v)))
 __device__ __forceinline__ float fabsF(float v) { return v < 0.f ? -v : (v == 0.f ? 0.f : v); }
 // std::min / std::max argument-order semantics (NaN follows the first argument)
 __device__ __forceinline__ float fmin2(float a, float b) { return (b < a) ? b : a; }
