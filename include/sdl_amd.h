@@ -114,13 +114,21 @@ int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offse
  * module-map joins TP, TCP and TCL assume module-map inputs and do not follow a
  * layer-pair stage).
  * The layer-pair stages store their objects as the SG, TL and TCL objects of
- * the batch.  algo: 0 = Default; for the three tracklet stages also 1 =
- * DefaultNm1_TLAlgo (Algo.h:21-26; Tracklet.cc:127-131): the default cuts
- * with the last one, dBeta, waived (Tracklet.cc:1513-1536, 1990-2014,
- * 2636-2653).  After DefaultNm1 module-map tracklets TCP is refused (the
- * reference re-runs the default algorithm on the bridges, Event.cc:940-946,
- * which the device join over stored tracklets cannot reproduce) and TCL may
- * follow TL directly.  sdl_event_run_chain always runs the defaults. */
+ * the batch.  algo takes the reference's Algo.h values (Algo.h:6-40): 0 =
+ * Default for every stage, 1 = AllComb (MiniDoublet.cc:223-226,
+ * Segment.cc:313-335, Triplet.cc:45-48, TrackCandidate.cc:138-141: every
+ * candidate accepted, the objects keeping their constructors' values); for the
+ * three tracklet stages 1 = DefaultNm1_TLAlgo (Tracklet.cc:127-131: the
+ * default cuts with the last one, dBeta, waived, Tracklet.cc:1513-1536,
+ * 1990-2014, 2636-2653) and 2 = AllComb (Tracklet.cc:139-142).  TCs from
+ * triplets ignore algo, as the reference does (Event.cc:874-982: the bridges
+ * always run the default segment and tracklet algorithms), and pseudo
+ * mini-doublets always run AllComb (Event.cc:353).  Any other value is
+ * rejected with SDL_ERR_ARG (the reference warns and stores nothing).  TCs from
+ * tracklets may follow the module-map tracklets directly.  TCs from triplets
+ * after AllComb segments or triplets are refused (SDL_ERR_STATE): their
+ * bridges are default segments and tracklets built on objects the device does
+ * not store.  sdl_event_run_chain always runs the defaults. */
 int sdl_event_run(sdl_event* ev, int stage, int algo);
 /* Runs MD, SG, TP, TL, TCP and TCL back to back.  After a first call has
  * measured the batch's object counts, later calls size their buffers from

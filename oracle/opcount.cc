@@ -46,7 +46,7 @@ int main(int argc, char** argv) {
     const std::string dir = argv[2];
     const int e0 = std::atoi(argv[3]), ne = std::atoi(argv[4]);
     const int libm = argc > 5 ? std::atoi(argv[5]) : ORACLE_LIBM_CR;
-    const int kind = (workload == 1 || workload == 4) ? 1 : 2;
+    const int kind = sdlgen::geometryKindOf(workload);
     sdlgen::Geometry gg;
     sdlgen::buildGeometry(kind, gg, kind == 1 ? 50000 : 200000);
     const std::string t = dir + "/opc_tilted_" + std::to_string(kind) + ".txt";

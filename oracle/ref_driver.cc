@@ -28,6 +28,14 @@
 //     (the module-map chain; every MD, SG and TL with its typed per-object
 //     values and reco variables, TLs with their whole reco-variable map, TCs
 //     from tracklets with theirs, as float bit patterns)
+//   sdl_ref_driver dumpseq     <workload> <geomdir> <event0> <nevents> <outfile> <sequence>
+//     (the create* calls of <sequence>, comma-separated stage:algo tokens with
+//     the Algo.h enum values; stages md, pmd, sg, sgl, tp, tl, tll, tln, tcp,
+//     tcl, tcll for createMiniDoublets, createPseudoMiniDoubletsFromAnchorModule,
+//     createSegmentsWithModuleMap, createSegments, createTriplets,
+//     createTrackletsWithModuleMap, createTracklets, createTrackletsViaNavigation,
+//     createTrackCandidatesFromTriplets, createTrackCandidatesFromTracklets,
+//     createTrackCandidates; objects, counters and per-layer counters)
 //   sdl_ref_driver time        <workload> <geomdir> <event0> <nevents> <threads> [stages]
 // `stages` limits the chain to its first k stages (1..6, default 6).
 #include <chrono>
@@ -57,7 +65,7 @@ bool fileExists(const std::string& p) {
     return stat(p.c_str(), &st) == 0;
 }
 
-int geomKindOf(int workload) { return (workload == 1 || workload == 4) ? 1 : 2; }
+int geomKindOf(int workload) { return sdlgen::geometryKindOf(workload); }
 
 void loadGeometry(int workload, const std::string& dir, sdlgen::Geometry& g) {
     int kind = geomKindOf(workload);
@@ -169,6 +177,55 @@ void dumpLayerEvent(FILE* f, SDL::Event& sev, long ievent) {
     std::fprintf(f, "C %u %u %u %u %u\n", sev.getNumberOfMiniDoublets(), sev.getNumberOfSegments(),
                  sev.getNumberOfTriplets(), sev.getNumberOfTracklets(), sev.getNumberOfTrackCandidates());
     dumpLayerCounters(f, sev);
+}
+
+// Any sequence of stages: MDs, SGs and TPs through the modules, TLs and TCs
+// through the layers (every TL and TC is added to one, Event.cc:179-243).
+void dumpSeqEvent(FILE* f, SDL::Event& sev, long ievent) {
+    std::fprintf(f, "E %ld\n", ievent);
+    for (SDL::Module* m : sev.getModulePtrs()) {
+        for (SDL::MiniDoublet* md : m->getMiniDoubletPtrs()) { std::fprintf(f, "MD"); putMD(f, md); std::fprintf(f, "\n"); }
+        for (SDL::Segment* sg : m->getSegmentPtrs()) { std::fprintf(f, "SG"); putSG(f, sg); std::fprintf(f, "\n"); }
+        for (SDL::Triplet* tp : m->getTripletPtrs()) {
+            std::fprintf(f, "TP"); putSG(f, tp->innerSegmentPtr()); putSG(f, tp->outerSegmentPtr()); std::fprintf(f, "\n");
+        }
+    }
+    for (SDL::Layer* L : sev.getLayerPtrs()) {
+        for (SDL::Tracklet* tl : L->getTrackletPtrs()) {
+            std::fprintf(f, "TL"); putSG(f, tl->innerSegmentPtr()); putSG(f, tl->outerSegmentPtr()); std::fprintf(f, "\n");
+        }
+        for (SDL::TrackCandidate* tc : L->getTrackCandidatePtrs()) {
+            SDL::TrackletBase* a = tc->innerTrackletBasePtr();
+            SDL::TrackletBase* b = tc->outerTrackletBasePtr();
+            std::fprintf(f, dynamic_cast<SDL::Triplet*>(a) != nullptr ? "TCP" : "TCL");
+            putSG(f, a->innerSegmentPtr()); putSG(f, a->outerSegmentPtr());
+            putSG(f, b->innerSegmentPtr()); putSG(f, b->outerSegmentPtr());
+            std::fprintf(f, "\n");
+        }
+    }
+    std::fprintf(f, "N %u %u %u %u %u\n", sev.getNumberOfMiniDoubletCandidates(), sev.getNumberOfSegmentCandidates(),
+                 sev.getNumberOfTripletCandidates(), sev.getNumberOfTrackletCandidates(),
+                 sev.getNumberOfTrackCandidateCandidates());
+    std::fprintf(f, "C %u %u %u %u %u\n", sev.getNumberOfMiniDoublets(), sev.getNumberOfSegments(),
+                 sev.getNumberOfTriplets(), sev.getNumberOfTracklets(), sev.getNumberOfTrackCandidates());
+    dumpLayerCounters(f, sev);
+}
+
+// Runs one stage:algo token of a dumpseq sequence; false for an unknown stage.
+bool runToken(SDL::Event& sev, const std::string& stage, int algo) {
+    if (stage == "md") sev.createMiniDoublets(SDL::MDAlgo(algo));
+    else if (stage == "pmd") sev.createPseudoMiniDoubletsFromAnchorModule(SDL::MDAlgo(algo));
+    else if (stage == "sg") sev.createSegmentsWithModuleMap(SDL::SGAlgo(algo));
+    else if (stage == "sgl") sev.createSegments(SDL::SGAlgo(algo));
+    else if (stage == "tp") sev.createTriplets(SDL::TPAlgo(algo));
+    else if (stage == "tl") sev.createTrackletsWithModuleMap(SDL::TLAlgo(algo));
+    else if (stage == "tll") sev.createTracklets(SDL::TLAlgo(algo));
+    else if (stage == "tln") sev.createTrackletsViaNavigation(SDL::TLAlgo(algo));
+    else if (stage == "tcp") sev.createTrackCandidatesFromTriplets(SDL::TCAlgo(algo));
+    else if (stage == "tcl") sev.createTrackCandidatesFromTracklets(SDL::TCAlgo(algo));
+    else if (stage == "tcll") sev.createTrackCandidates(SDL::TCAlgo(algo));
+    else return false;
+    return true;
 }
 
 void dumpEvent(FILE* f, SDL::Event& sev, long ievent) {
@@ -387,6 +444,34 @@ int main(int argc, char** argv) {
             fill(sev, ev);
             runChain(sev, 6);
             dumpVars(f, sev, ie);
+        }
+        std::fclose(f);
+        return 0;
+    }
+    if (mode == "dumpseq") {  // any sequence of create* calls with their algorithms
+        if (argc < 8) return 2;
+        std::vector<std::pair<std::string, int>> seq;
+        std::string spec = argv[7];
+        size_t pos = 0;
+        while (pos < spec.size()) {
+            size_t comma = spec.find(',', pos);
+            if (comma == std::string::npos) comma = spec.size();
+            const std::string tok = spec.substr(pos, comma - pos);
+            const size_t colon = tok.find(':');
+            if (colon == std::string::npos) return 2;
+            seq.emplace_back(tok.substr(0, colon), std::atoi(tok.c_str() + colon + 1));
+            pos = comma + 1;
+        }
+        FILE* f = std::fopen(argv[6], "w");
+        if (!f) return 3;
+        for (long ie = e0; ie < e0 + ne; ++ie) {
+            sdlgen::Event ev;
+            sdlgen::generateEvent(g, cfg, ie, ev);
+            SDL::Event sev;
+            fill(sev, ev);
+            for (auto& t : seq)
+                if (!runToken(sev, t.first, t.second)) return 2;
+            dumpSeqEvent(f, sev, ie);
         }
         std::fclose(f);
         return 0;

@@ -267,13 +267,19 @@ struct MDRec {
     int lo, up, anchor;  // hit indices in the event
     int slot;            // lower module slot
     float dz, shiftedDz, dphi, dphichange;
-    bool pseudo = false;  // made by createPseudoMiniDoubletsFromAnchorModule (no cuts run)
+    // made without running cuts (createPseudoMiniDoubletsFromAnchorModule, or
+    // the AllComb algorithm): every cached value 0, no reco variable
+    bool pseudo = false;
 };
 struct SGRec {
     int mdIn, mdOut;
     float dphi, dphimin, dphimax, dphichange, dphichangemin, dphichangemax;
+    bool allComb = false;  // accepted by AllComb_SGAlgo: constructor values, no reco variable
 };
-struct PairRec { int in, out; };
+struct PairRec {
+    int in, out;
+    bool allComb = false;  // a TL or TC accepted by the AllComb algorithm (no value set)
+};
 
 struct Slot {
     ModInfo info;
@@ -300,6 +306,17 @@ struct Event {
     // DefaultNm1_TLAlgo (Tracklet.cc:127-131): the default tracklet algorithm
     // with its last cut, dBeta, waived (Tracklet.cc:1513-1536, 1990-2014, 2636-2653)
     bool tlNm1 = false;
+    // the stage being run uses the AllComb algorithm (accepts every candidate)
+    bool acMD = false, acSG = false, acTP = false, acTL = false, acTC = false;
+    // Segment::runSegmentAllCombAlgo (Segment.cc:313-335): the pass bit only; the
+    // segment keeps its constructor values (Segment.cc:35-56)
+    bool allCombSG(int mi, int mo, SGRec& sg) const {
+        sg.mdIn = mi;
+        sg.mdOut = mo;
+        sg.dphi = sg.dphimin = sg.dphimax = sg.dphichange = sg.dphichangemin = sg.dphichangemax = 0;
+        sg.allComb = true;
+        return true;
+    }
 
     // Event::getModule (Event.cc:54-78)
     int getSlot(uint32_t d) {
@@ -1184,14 +1201,24 @@ struct Event {
             int us = getSlot(slots[ls].info.partner);
             const std::vector<int> loHits = slots[ls].hits;
             const std::vector<int> upHits = slots[us].hits;
+            const ModInfo& m = slots[ls].info;
             for (int lo : loHits)
                 for (int up : upHits) {
                     nCand[0]++;
                     MDRec md;
-                    if (mdAlgo(lo, up, md, nullptr)) {
-                        slots[ls].mds.push_back(int(mds.size()));
-                        mds.push_back(md);
+                    if (acMD) {  // runMiniDoubletAllCombAlgo (MiniDoublet.cc:223-226): every candidate, values 0
+                        md.lo = lo;
+                        md.up = up;
+                        md.slot = ls;
+                        md.dz = md.shiftedDz = md.dphi = md.dphichange = 0;
+                        // MiniDoublet::setAnchorHit (MiniDoublet.cc:60-81)
+                        md.anchor = (m.type == TYPE_PS && m.layerType != LT_PIXEL) ? up : lo;
+                        md.pseudo = true;
+                    } else if (!mdAlgo(lo, up, md, nullptr)) {
+                        continue;
                     }
+                    slots[ls].mds.push_back(int(mds.size()));
+                    mds.push_back(md);
                 }
         }
     }
@@ -1237,7 +1264,7 @@ struct Event {
                     for (int mo : slots[os].mds) {
                         nCand[1]++;
                         SGRec sg;
-                        if (sgAlgo(mi, mo, sg, nullptr)) {
+                        if (acSG ? allCombSG(mi, mo, sg) : sgAlgo(mi, mo, sg, nullptr)) {
                             slots[ls].sgs.push_back(int(sgs.size()));
                             sgs.push_back(sg);
                         }
@@ -1255,7 +1282,8 @@ struct Event {
                     if (os < 0) continue;
                     for (int so : slots[os].sgs) {
                         nCand[2]++;
-                        if (mdMatch(sgs[si].mdOut, sgs[so].mdIn)) {
+                        // runTripletAllCombAlgo (Triplet.cc:45-48) accepts every candidate
+                        if (acTP || mdMatch(sgs[si].mdOut, sgs[so].mdIn)) {
                             slots[ls].tps.push_back(int(tps.size()));
                             tps.push_back({si, so});
                         }
@@ -1273,9 +1301,9 @@ struct Event {
                     if (os < 0) continue;
                     for (int so : slots[os].sgs) {
                         nCand[3]++;
-                        if (tlAlgo(sgs[si], sgs[so], nullptr)) {
+                        if (acTL || tlAlgo(sgs[si], sgs[so], nullptr)) {
                             slots[ls].tls.push_back(int(tls.size()));
-                            tls.push_back({si, so});
+                            tls.push_back({si, so, acTL});
                             tlLayer.push_back(layerOfMD(sgs[si].mdIn));  // addTrackletToEvent (Event.cc:179-193)
                         }
                     }
@@ -1314,7 +1342,8 @@ struct Event {
                     if (os < 0) continue;
                     for (int lo : slots[os].tls) {
                         nCand[4]++;
-                        if (sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo}});
+                        // runTrackCandidateAllCombAlgo (TrackCandidate.cc:138-141) accepts every candidate
+                        if (acTC || sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo, acTC}});
                     }
                 }
             }
@@ -1359,7 +1388,7 @@ struct Event {
             for (int mi : in)
                 for (int mo : out) {
                     SGRec sg;
-                    if (sgAlgo(mi, mo, sg, nullptr)) {
+                    if (acSG ? allCombSG(mi, mo, sg) : sgAlgo(mi, mo, sg, nullptr)) {
                         slots[mds[mi].slot].sgs.push_back(int(sgs.size()));
                         sgs.push_back(sg);
                     }
@@ -1376,8 +1405,8 @@ struct Event {
             for (int si : in)
                 for (int so : out) {
                     nCand[3]++;
-                    if (tlAlgo(sgs[si], sgs[so], nullptr)) {
-                        tls.push_back({si, so});
+                    if (acTL || tlAlgo(sgs[si], sgs[so], nullptr)) {
+                        tls.push_back({si, so, acTL});
                         tlLayer.push_back(pr.first);
                     }
                 }
@@ -1396,7 +1425,7 @@ struct Event {
         for (int li : in)
             for (int lo : out) {
                 nCand[4]++;
-                if (sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo}});
+                if (acTC || sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo, acTC}});
             }
     }
     // Event::createTrackletsViaNavigation (Event.cc:696-751): inner SG s1, every
@@ -1412,9 +1441,9 @@ struct Event {
                 for (int s2 : outward[sgs[si].mdOut])
                     for (int s3 : outward[sgs[s2].mdOut]) {
                         nCand[3]++;
-                        if (tlAlgo(sgs[si], sgs[s3], nullptr)) {
+                        if (acTL || tlAlgo(sgs[si], sgs[s3], nullptr)) {
                             slots[ls].tls.push_back(int(tls.size()));
-                            tls.push_back({si, s3});
+                            tls.push_back({si, s3, acTL});
                             tlLayer.push_back(layerOfMD(sgs[si].mdIn));
                         }
                     }
@@ -1428,9 +1457,25 @@ struct Event {
     }
     // Stages in the given order: 0-5 the module-map stages, 6-8 createSegments,
     // createTracklets, createTrackCandidates, 9 createTrackletsViaNavigation,
-    // 10 createPseudoMiniDoubletsFromAnchorModule.
+    // 10 createPseudoMiniDoubletsFromAnchorModule; stage + 100 * algo runs the
+    // stage with that Algo.h value (1: AllComb, for the tracklet stages
+    // 1: DefaultNm1 and 2: AllComb; createTrackCandidatesFromTriplets and the
+    // pseudo mini-doublets take no algorithm, Event.cc:874-982, 307-430).
     void runSequence(const int* seq, int n) {
         for (int k = 0; k < n; ++k) {
+            if (seq[k] >= 100) {
+                const int stage = seq[k] % 100, algo = seq[k] / 100;
+                const bool tl = stage == 3 || stage == 7 || stage == 9;
+                acMD = (stage == 0) && algo == 1;
+                acSG = (stage == 1 || stage == 6) && algo == 1;
+                acTP = stage == 2 && algo == 1;
+                acTL = tl && algo == 2;
+                acTC = (stage == 5 || stage == 8) && algo == 1;
+                tlNm1 = tl && algo == 1;
+                runSequence(&stage, 1);
+                acMD = acSG = acTP = acTL = acTC = tlNm1 = false;
+                continue;
+            }
             switch (seq[k]) {
                 case 0: createMiniDoublets(); break;
                 case 1: createSegments(); break;
@@ -1492,8 +1537,12 @@ struct Event {
         if (stage == ORACLE_SG)
             for (const SGRec& sg : sgs) {
                 std::fill(buf, buf + kTLVars, 0.f);
-                SGRec t;
-                sgAlgo(sg.mdIn, sg.mdOut, t, nullptr);
+                if (sg.allComb) {  // constructor values, no reco variable
+                    for (int k = 14; k < kSGVars; ++k) buf[k] = std::nanf("");
+                } else {
+                    SGRec t;
+                    sgAlgo(sg.mdIn, sg.mdOut, t, nullptr);
+                }
                 put();
             }
         if (stage == ORACLE_TL) {
@@ -1501,7 +1550,10 @@ struct Event {
             tlNm1 = true;  // every stored TL passed; under DefaultNm1 it may fail the waived cut
             for (const PairRec& tl : tls) {
                 std::fill(buf, buf + kTLVars, 0.f);
-                tlAlgo(sgs[tl.in], sgs[tl.out], nullptr);
+                if (tl.allComb)  // constructor values, an empty reco-variable map
+                    for (int k = kTLTyped; k < kTLVars; ++k) buf[k] = std::nanf("");
+                else
+                    tlAlgo(sgs[tl.in], sgs[tl.out], nullptr);
                 put();
             }
             tlNm1 = nm1;
@@ -1509,7 +1561,10 @@ struct Event {
         if (stage == ORACLE_TCL)
             for (const auto& tc : tcs)
                 if (tc.first == 1) {
-                    tcRecoVars(tls[tc.second.in], tls[tc.second.out], buf);
+                    if (tc.second.allComb)
+                        buf[0] = buf[1] = std::nanf("");
+                    else
+                        tcRecoVars(tls[tc.second.in], tls[tc.second.out], buf);
                     put();
                 }
         rec = nullptr;

@@ -468,6 +468,10 @@ void writeGeometryFiles(const Geometry& g, const std::string& tiltedPath, const 
     std::fclose(fm);
 }
 
+int geometryKindOf(int workload) {
+    return (workload == 1 || workload == 4 || workload == 11) ? GEOM_BARREL_FLAT : GEOM_FULL;
+}
+
 EventConfig presetConfig(int workload) {
     EventConfig c;
     c.workload = workload;
@@ -487,6 +491,9 @@ EventConfig presetConfig(int workload) {
             break;
         // small slices with the same geometry and distributions, for parity tests
         // at sizes the CPU reference finishes in seconds
+        case 11:  // flat barrel, 40 tracks: chains with accept-all (AllComb) stages
+            c.ntracks_min = c.ntracks_max = 40;
+            break;
         case 12:
             c.ntracks_min = c.ntracks_max = 300; c.n_hard = 2; c.pt_mode = 1; c.eta_max = 2.4; c.ms = true;
             break;
@@ -611,6 +618,7 @@ void* sdlgen_event_new(void* g, int workload, unsigned long long ievent, unsigne
 int sdlgen_event_nhits(void* e) { return int(static_cast<sdlgen::Event*>(e)->x.size()); }
 
 int sdlgen_event_ntracks(void* e) { return static_cast<sdlgen::Event*>(e)->ntracks; }
+int sdlgen_geometry_kind(int workload) { return sdlgen::geometryKindOf(workload); }
 
 void sdlgen_event_copy(void* e, float* x, float* y, float* z, unsigned int* detId, int* idx, int* sim) {
     auto* ev = static_cast<sdlgen::Event*>(e);

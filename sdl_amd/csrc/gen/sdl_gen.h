@@ -99,6 +99,8 @@ struct Event {
 
 // Builds the detector (deterministic, no randomness) including the connection map.
 void buildGeometry(int kind, Geometry& g, int mapTracks = 200000);
+// Geometry kind of a workload: the flat six-layer barrel (1, 4, 11) or the full detector.
+int geometryKindOf(int workload);
 
 // Writes the three text files the reference loads.
 void writeGeometryFiles(const Geometry& g, const std::string& tiltedPath,

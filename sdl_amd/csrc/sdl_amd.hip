@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -53,14 +54,27 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kNVariantsTL = 3;  // tracklet algorithm variants with device kernels: BBBB, BBEE, EEEE
+// Tracklet algorithms (the TLAlgo values of Algo.h:21-26)
+enum { TLA_DEFAULT = 0, TLA_NM1 = 1, TLA_ALLCOMB = 2 };
 
 // Set while a chain is being captured into a HIP graph: an allocation is not
 // allowed then, so a buffer that must grow ends the capture (the chain then
 // runs uncaptured once, which grows it).
 thread_local bool g_capturing = false;
 thread_local bool g_captureBlocked = false;
-// Bumped by every (re)allocation, so a captured graph knows its buffers moved.
-thread_local unsigned long long g_allocEpoch = 0;
+// Bumped by every (re)allocation on any thread, so a captured graph knows its
+// buffers moved (a handle may be driven from several host threads in turn).
+std::atomic<unsigned long long> g_allocEpoch{0};
+// The stream of the handle whose call runs on this thread (set by the C-ABI
+// entry points, OnStream): buffers grow in that stream's order (hipFreeAsync,
+// hipMallocAsync), so one handle's growth never waits for another handle's
+// work.  Without one (geometry loading) allocation is synchronous.
+thread_local hipStream_t g_allocStream = nullptr;
+struct OnStream {
+    hipStream_t prev;
+    explicit OnStream(hipStream_t s) : prev(g_allocStream) { g_allocStream = s; }
+    ~OnStream() { g_allocStream = prev; }
+};
 
 template <class T>
 struct DBuf {
@@ -79,12 +93,23 @@ struct DBuf {
         ++g_allocEpoch;
         size_t want = std::max<size_t>(n, 1);
         want = std::max(want, cap + cap / 2);
-        // queued work may still read the old buffer
-        if (p) (void)hipDeviceSynchronize();
-        if (p) (void)hipFree(p);
+        // queued work may still read the old buffer: free it in stream order
+        if (p) {
+            if (g_allocStream) {
+                (void)hipFreeAsync(p, g_allocStream);
+            } else {
+                (void)hipDeviceSynchronize();
+                (void)hipFree(p);
+            }
+        }
         p = nullptr;
         cap = 0;
-        if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
+        const hipError_t e = g_allocStream ? hipMallocAsync((void**)&p, want * sizeof(T), g_allocStream)
+                                           : hipMalloc(&p, want * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(SDL_ERR_HIP, "device allocation failed");
+        }
         cap = want;
         return 0;
     }
@@ -483,13 +508,13 @@ struct ScanScratch {
         }
         if (nTiles <= cap) return 0;
         const unsigned long long want = std::max<unsigned long long>(nTiles, cap * 2);
-        if (state) {
-            (void)hipStreamSynchronize(st);
-            (void)hipFree(state);
-        }
+        if (state) (void)hipFreeAsync(state, st);
         state = nullptr;
         cap = 0;
-        if (hipMalloc(&state, want * sizeof(unsigned long long)) != hipSuccess) return fail(SDL_ERR_HIP, "hipMalloc failed");
+        if (hipMallocAsync((void**)&state, want * sizeof(unsigned long long), st) != hipSuccess) {
+            state = nullptr;
+            return fail(SDL_ERR_HIP, "device allocation failed");
+        }
         hipLaunchKernelGGL(k_fill_u64, dim3(unsigned((want + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, state, want, 0ULL);
         cap = want;
         return 0;
@@ -689,7 +714,7 @@ struct HitsDev {
 __global__ void __launch_bounds__(kBlock)
 k_md(Count cCand, int NS, int NM, const ModDev* mods, HitsDev H, const long long* slotHitStart, const int* sortedHit,
      const long long* mdCapOff, int2* slabHits, float4* slabF, int* slabSlot, unsigned char* flag, int* mdCount,
-     Guard g) {
+     int allComb, Guard g) {
     if (g.over()) return;
     const long long nCand = cCand.get();
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -706,8 +731,10 @@ k_md(Count cCand, int NS, int NM, const ModDev* mods, HitsDev H, const long long
     const int up = sortedHit[upStart + jj];
     const HitView hl = loadHit(lo, H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
     const HitView hu = loadHit(up, H.x, H.y, H.z, H.rt, H.r3, H.phi, H.alpha);
-    MDOut o;
-    const bool pass = mdAlgo(hl, hu, md, o);
+    // AllComb_MDAlgo (MiniDoublet.cc:223-226): every candidate, the cached values
+    // left at the constructor's 0 (MiniDoublet.cc:41-51)
+    MDOut o{0.f, 0.f, 0.f, 0.f};
+    const bool pass = allComb || mdAlgo(hl, hu, md, o);
     flag[j] = pass ? 1 : 0;
     if (pass) {
         slabHits[j] = make_int2(lo, up);
@@ -878,7 +905,7 @@ __device__ __forceinline__ void appendSG(bool pass, int lane, int& base, long lo
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_sg(Count cMD, int NM, const ModDev* mods, HitsDev H, MDDev D,
                                                const int* connStart, const int* connList, const long long* mdStart,
                                                const int* mdCount, const int* sgCap, const long long* sgCapOff,
-                                               int* slabOut, SGF* slabF, int* sgCount, Guard g) {
+                                               int* slabOut, SGF* slabF, int* sgCount, int allComb, Guard g) {
     if (g.over()) return;
     const int nMD = int(cMD.get());
     const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -911,14 +938,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
         for (int k0 = 0; k0 < total; k0 += kWave) {
             const int k = k0 + lane;
             bool pass = false;
-            SGOut o;
+            SGOut o{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // AllComb leaves the constructor's values (Segment.cc:35-56)
             int mdOut = 0;
             int inner;
             const int e = findEntry(k, pre, ne, inner);
             if (k < total) {
                 const int modOut = connList[c0 + cb + e];
                 mdOut = int(mdStart[eb + modOut]) + inner;
-                pass = sgCandidate(in, mi, mdOut, mods[modOut], true, H, D, o);
+                pass = allComb || sgCandidate(in, mi, mdOut, mods[modOut], true, H, D, o);
             }
             appendSG(pass, lane, base, off, mdOut, o, slabOut, slabF);
         }
@@ -994,7 +1021,7 @@ __global__ void k_sgl_cap(Count cMD, int NM, const ModDev* mods, const int* mdSl
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8)))
 k_sgl(Count cMD, int NM, const ModDev* mods, HitsDev H, MDDev D, const int* connStart, const int* connList,
       const long long* layerOff, const int* layerModStart, const int* layerMd, const int* sgCap,
-      const long long* sgCapOff, int* slabOut, SGF* slabF, int* sgCount, Guard g) {
+      const long long* sgCapOff, int* slabOut, SGF* slabF, int* sgCount, int allComb, Guard g) {
     if (g.over()) return;
     const int nMD = int(cMD.get());
     const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1019,16 +1046,16 @@ k_sgl(Count cMD, int NM, const ModDev* mods, HitsDev H, MDDev D, const int* conn
         for (long long k0 = a; k0 < b; k0 += kWave) {
             const long long k = k0 + lane;
             bool pass = false;
-            SGOut o;
+            SGOut o{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // AllComb leaves the constructor's values
             int mdOut = 0;
             if (k < b) {
                 mdOut = layerMd[k];
                 const int modOut = D.slot[mdOut] % NM;
                 const ModDev mO = mods[modOut];
                 bool conn = false;
-                if (mO.subdet != SUB_BARREL)
+                if (mO.subdet != SUB_BARREL && !allComb)
                     for (int q = c0; q < c1; ++q) conn = conn || connList[q] == modOut;
-                pass = sgCandidate(in, mi, mdOut, mO, conn, H, D, o);
+                pass = allComb || sgCandidate(in, mi, mdOut, mO, conn, H, D, o);
             }
             appendSG(pass, lane, base, off, mdOut, o, slabOut, slabF);
         }
@@ -1134,6 +1161,89 @@ __global__ void k_tp_write(Count cSG, const int* sgOut, const long long* sgStart
         tpOutArr[d + k] = int(a + k);
         tpKey[d + k] = key;
     }
+}
+
+// ------------------------------------------------------------------ AllComb joins
+// The AllComb algorithms accept every candidate (Triplet.cc:45-48,
+// Tracklet.cc:139-142, TrackCandidate.cc:138-141).  For these three stages the
+// candidates of inner object i are the objects stored at the modules connected
+// to the module of a key MD of i, in connection-list order and, inside a
+// module, in storage order (the reference's loops):
+//   triplets (AC_TP):  inner SG s, the SGs of the modules connected to the
+//                      module of s's inner MD                  Event.cc:539-597
+//   tracklets (AC_TL): inner SG s, the SGs of the modules connected to the
+//                      module of s's outer MD                  Event.cc:627-693
+//   TCs from tracklets (AC_TCL): inner TL l, the TLs of the modules connected to
+//                      the module of its inner SG's outer MD   Event.cc:999-1062
+// objStart holds the per-slot object ranges (SGs or TLs).
+enum { AC_TP = 0, AC_TL = 1, AC_TCL = 2 };
+
+struct AcIn {
+    int kind, NM;
+    const int *sgIn, *sgOut, *tlIn, *mdSlot, *connStart, *connList;
+    const long long* objStart;
+    __device__ __forceinline__ int keySlot(int i) const {
+        const int md = kind == AC_TP ? sgIn[i] : (kind == AC_TL ? sgOut[i] : sgOut[tlIn[i]]);
+        return mdSlot[md];
+    }
+};
+
+// Candidates (= objects) per inner object; cap (optional) the same as 64-bit.
+__global__ void k_ac_count(Count cIn, AcIn A, int* cnt, long long* cap) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cIn.get()) return;
+    const int b = A.keySlot(int(i));
+    const int m = b % A.NM, eb = b - m;
+    long long c = 0;
+    for (int k = A.connStart[m]; k < A.connStart[m + 1]; ++k) {
+        const int t = eb + A.connList[k];
+        c += A.objStart[t + 1] - A.objStart[t];
+    }
+    cnt[i] = int(c);
+    if (cap) cap[i] = c;
+}
+
+// The objects (i, o) of inner object i at start[i]: into a and b (triplets,
+// tracklets; key[i] also into bKey when given) or as pairs (TCs).
+__global__ void k_ac_write(Count cIn, AcIn A, const long long* start, int* outA, int* outB, const int* key,
+                           int* outKey, int2* outPair, Guard g) {
+    if (g.over()) return;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cIn.get()) return;
+    const int b = A.keySlot(int(i));
+    const int m = b % A.NM, eb = b - m;
+    long long d = start[i];
+    for (int k = A.connStart[m]; k < A.connStart[m + 1]; ++k) {
+        const int t = eb + A.connList[k];
+        for (long long o = A.objStart[t]; o < A.objStart[t + 1]; ++o, ++d) {
+            if (outPair) {
+                outPair[d] = make_int2(int(i), int(o));
+            } else {
+                outA[d] = int(i);
+                outB[d] = int(o);
+                if (outKey) outKey[d] = key[i];
+            }
+        }
+    }
+}
+
+// Layer-pair TCs with AllComb (Event.cc:840-872): every TL of barrel layer 1
+// against every TL of barrel layer 3 of its event; layerTl / layerTlOff are the
+// per-event layer lists of the tracklets.
+__global__ void k_tcll_ac_write(Count cTL, int NM, const ModDev* mods, const int* tlIn, const int* sgIn,
+                                const int* mdSlot, const long long* layerTlOff, const int* layerModStart,
+                                const int* layerTl, const long long* start, int2* tc, Guard g) {
+    if (g.over()) return;
+    const int nTL = int(cTL.get());
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nTL) return;
+    const int slot = mdSlot[sgIn[tlIn[l]]];
+    if (layerBin(mods[slot % NM]) != 0) return;
+    const int e = slot / NM;
+    const long long a = layerTlOff[(long long)e * NM + layerModStart[2]];
+    const long long b = layerTlOff[(long long)e * NM + layerModStart[3]];
+    long long d = start[l];
+    for (long long k = a; k < b; ++k) tc[d++] = make_int2(l, layerTl[k]);
 }
 
 // ------------------------------------------------------------------ TL
@@ -1608,7 +1718,7 @@ __global__ void k_tll_cap(Count cSG, int NM, const ModDev* mods, const int* sgIn
 __global__ void __launch_bounds__(kBlock) k_tll(Count cSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
                                                 const long long* layerOff, const int* layerModStart, const int* layerSg,
                                                 const int* capInt, const long long* capOff, int* slab, int* tlCount,
-                                                int nm1, Guard g) {
+                                                int algo, Guard g) {
     if (g.over()) return;
     const int nSG = int(cSG.get());
     const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1631,14 +1741,16 @@ __global__ void __launch_bounds__(kBlock) k_tll(Count cSG, int NM, const ModDev*
         const long long k = k0 + lane;
         bool pass = false;
         int so = 0;
-        if (k < b && V < kNVariantsTL) {
-            so = layerSg[k];
+        if (k < b) so = layerSg[k];
+        if (k < b && algo == TLA_ALLCOMB) {
+            pass = true;
+        } else if (k < b && V < kNVariantsTL) {
             const int flo = S.flags[so];
             const TLOutLo Lo = loadOutLo(S.in[so], flo, D);
             TLCarry c{0.f, 0.f, 0.f};
             if (tlStage1(V, I, Lo, c)) {
                 const TLOutUp U = loadOutUp(so, flo, H, D, S);
-                pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c, nullptr, nm1 != 0);
+                pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c, nullptr, algo == TLA_NM1);
             }
         }
         const unsigned long long mask = __ballot(pass);
@@ -1683,7 +1795,7 @@ __global__ void k_tln_cap(Count cSG, const int* sgOut, const long long* sgStart,
 // tracklet stages with the variant of each candidate's anchors.
 __global__ void __launch_bounds__(kBlock) k_tln(Count cSG, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S,
                                                 const long long* sgStart, const int* capInt, const long long* capOff,
-                                                int* slab, int* tlCount, int nm1, Guard g) {
+                                                int* slab, int* tlCount, int algo, Guard g) {
     if (g.over()) return;
     const int nSG = int(cSG.get());
     const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1704,7 +1816,9 @@ __global__ void __launch_bounds__(kBlock) k_tln(Count cSG, int NM, const ModDev*
         for (long long k0 = a; k0 < b; k0 += kWave) {
             const long long s3 = k0 + lane;
             bool pass = false;
-            if (s3 < b) {
+            if (s3 < b && algo == TLA_ALLCOMB) {
+                pass = true;
+            } else if (s3 < b) {
                 const int flo = S.flags[s3];
                 const int mLo = S.in[s3];
                 const int V = tlModuleVariant(fl, mods[D.slot[mLo] % NM]);
@@ -1713,7 +1827,7 @@ __global__ void __launch_bounds__(kBlock) k_tln(Count cSG, int NM, const ModDev*
                     TLCarry c{0.f, 0.f, 0.f};
                     if (tlStage1(V, I, Lo, c)) {
                         const TLOutUp U = loadOutUp(int(s3), flo, H, D, S);
-                        pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c, nullptr, nm1 != 0);
+                        pass = tlStage2(V, (flo & 2) != 0, I, Lo, U, c, nullptr, algo == TLA_NM1);
                     }
                 }
             }
@@ -1742,7 +1856,7 @@ __global__ void k_tl_per_layer(Count cTL, int NM, const ModDev* mods, const int*
 // when that SG lies in bin 2.  cand[l] = the candidates counted for l.
 __global__ void k_tcll_count(Count cTL, int NM, const ModDev* mods, const int* tlIn, const int* tlOut,
                              const int* sgIn, const int* mdSlot, const long long* tlStart, const int* perLayer,
-                             int* cnt, long long* cand) {
+                             int allComb, int* cnt, long long* cand) {
     const int nTL = int(cTL.get());
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nTL) return;
@@ -1754,7 +1868,8 @@ __global__ void k_tcll_count(Count cTL, int NM, const ModDev* mods, const int* t
         k = perLayer[e * kNBins + 2];
         const int so = tlOut[l];
         const int slotO = mdSlot[sgIn[so]];
-        if (layerBin(mods[slotO % NM]) == 2) c = int(tlStart[so + 1] - tlStart[so]);
+        if (allComb) c = int(k);  // AllComb_TCAlgo: every candidate
+        else if (layerBin(mods[slotO % NM]) == 2) c = int(tlStart[so + 1] - tlStart[so]);
     }
     cnt[l] = c;
     cand[l] = k;
@@ -1795,13 +1910,19 @@ __global__ void k_tpe_write(Count cTP, Count cSG, const long long* tpeOff, const
     tpEndList[j] = int(tpStart[s1] + (s2 - sgStart[mx]));
 }
 // Per tracklet: the bridge test (s3 exists), the number of TCs and where their
-// inner TPs are listed; info = (first listed inner TP, outer TP).
+// inner TPs are listed; info = (first listed inner TP, outer TP).  tlDefault
+// (after DefaultNm1 tracklets, else null): the tracklets the default algorithm passes.
 __global__ void k_tcp_count(Count cTL, const int* tlIn, const int* tlOutArr, const int* sgIn, const int* sgOut,
                             const long long* sgStart, const long long* sgEndStart, const long long* tpeOff,
-                            const long long* tpStart, int2* info, int* cnt) {
+                            const long long* tpStart, const unsigned char* tlDefault, int2* info, int* cnt) {
     const int nTL = int(cTL.get());
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nTL) return;
+    if (tlDefault && !tlDefault[l]) {  // a DefaultNm1 tracklet the default algorithm rejects bridges nothing
+        cnt[l] = 0;
+        info[l] = make_int2(0, -1);
+        return;
+    }
     const int b = tlIn[l], s4 = tlOutArr[l];
     const int mdC = sgIn[b], mdD = sgOut[b], mdE = sgIn[s4];
     int s3 = -1;
@@ -2113,6 +2234,26 @@ __global__ void k_vars_tl(long long a, long long n, int NM, const ModDev* mods, 
     }
 }
 
+// Under DefaultNm1_TLAlgo the stored tracklets include those that fail the
+// waived dBeta cut, but TCs from triplets re-run the default tracklet
+// algorithm on their bridges (Event.cc:940-956).  The TC join therefore reads,
+// per stored tracklet, whether the default algorithm passes it: both cut
+// stages again, nothing waived.
+__global__ void k_tl_default(Count cTL, int NM, const ModDev* mods, HitsDev H, MDDev D, SGDev S, const int* tlIn,
+                             const int* tlOutArr, unsigned char* def) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cTL.get()) return;
+    const int s = tlIn[i], so = tlOutArr[i];
+    const int mdOutLo = S.in[so];
+    const int flo = S.flags[so];
+    const int V = tlModuleVariant(S.flags[s], mods[D.slot[mdOutLo] % NM]);
+    const TLInner I = loadInner(s, D, S);
+    const TLOutLo L = loadOutLo(mdOutLo, flo, D);
+    const TLOutUp U = loadOutUp(so, flo, H, D, S);
+    TLCarry c{0.f, 0.f, 0.f};
+    def[i] = (tlStage1(V, I, L, c) && tlStage2(V, (flo & 2) != 0, I, L, U, c, nullptr, false)) ? 1 : 0;
+}
+
 // TrackCandidate::runTrackCandidateDefaultAlgo's recorded values (TrackCandidate.cc:183-207)
 // of TCs [a, a + n) made from tracklets: innerR through the inner TL's first
 // three anchors, outerR through the outer TL's last three, dR = (innerR - outerR) / innerR.
@@ -2334,6 +2475,13 @@ struct sdl_event {
     DBuf<long long> sgEndStart;
     DBuf<int> tpCount, tpIn, tpOut, tpKey, tpeCount, tpEndList;
     int tlNm1 = 0;  // the tracklet stage runs DefaultNm1_TLAlgo (dBeta cut waived)
+    int tlAlgo = TLA_DEFAULT;  // algorithm of the last tracklet stage (TLAlgo value)
+    int curAlgo = 0;           // algorithm of the stage being run by sdl_event_run (Algo.h value)
+    unsigned acMask = 0;       // stages of the current batch run with an AllComb algorithm (bit = stage)
+    bool forceExact = false;   // size every buffer exactly (calls with a non-default algorithm)
+    DBuf<long long> layerTlOff;  // per-event layer lists of the tracklets (AllComb layer-pair TCs)
+    DBuf<int> layerTl;
+    DBuf<unsigned char> tlDefault;  // after DefaultNm1 tracklets: those the default algorithm passes
     DBuf<long long> tpeOff;
     DBuf<long long> tpStart, slotTpStart;
     DBuf<long long> tlCap, tlCapOff, tlStart, slotTlStart;
@@ -2640,7 +2788,7 @@ int runMD(sdl_event* ev, bool pseudo) {
         hipLaunchKernelGGL(k_md, dim3(gridFor(slab, ev->tune.v[TUNE_MD_BLOCK])), dim3(ev->tune.v[TUNE_MD_BLOCK]), 0, st,
                            devCount(ev, TOT_MDCAP), NS, G->NM,
                            G->dMods, H, ev->slotHitStart.p, ev->sortedHit.p, ev->mdCapOff.p, ev->mdSlabHits.p,
-                           ev->mdSlabF.p, ev->mdSlabSlot.p, ev->mdFlag.p, ev->mdCount.p,
+                           ev->mdSlabF.p, ev->mdSlabSlot.p, ev->mdFlag.p, ev->mdCount.p, ev->curAlgo == 1 ? 1 : 0,
                            guardOf(ev, TOT_MDCAP, slab, SDL_STAGE_MD));
     if (int r = scanExclusive<unsigned char>(ev->mdFlag.p, ev->mdPos.p, devCount(ev, TOT_MDCAP), slab, ev->scan, st,
                                              ev->tot.p, -1))
@@ -2704,12 +2852,12 @@ int runSG(sdl_event* ev, bool layers) {
                            st, cMD, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), G->dConnStart, G->dConnList, ev->mdStart.p, ev->mdCount.p,
                            ev->sgCap.p, ev->sgCapOff.p, ev->sgSlabOut.p, ev->sgSlabF.p, ev->sgCount.p,
-                           guardOf(ev, TOT_SGCAP, slab, SDL_STAGE_SG));
+                           ev->curAlgo == 1 ? 1 : 0, guardOf(ev, TOT_SGCAP, slab, SDL_STAGE_SG));
     if (nMD > 0 && layers)
         hipLaunchKernelGGL(k_sgl, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, cMD, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), G->dConnStart, G->dConnList, ev->layerMdOff.p, G->dLayerModStart,
                            ev->layerMd.p, ev->sgCap.p, ev->sgCapOff.p, ev->sgSlabOut.p, ev->sgSlabF.p, ev->sgCount.p,
-                           guardOf(ev, TOT_SGCAP, slab, SDL_STAGE_SG));
+                           ev->curAlgo == 1 ? 1 : 0, guardOf(ev, TOT_SGCAP, slab, SDL_STAGE_SG));
     timerEnd(ev, T_SG);
     if (int r = scanExclusive<int>(ev->sgCount.p, ev->sgStart.p, cMD, nMD, ev->scan, st, ev->tot.p, TOT_SG)) return r;
     const long long nSG = sizeOf(ev, TOT_SG);
@@ -2752,7 +2900,13 @@ int runTP(sdl_event* ev) {
     if (int r = ev->tpStart.ensure(ns + 1)) return r;
     if (int r = ev->slotTpStart.ensure(NS + 1)) return r;
     const Count cSG = devCount(ev, TOT_SG);
-    // tpCount was written by the SG stage (k_sg_extra)
+    const bool allComb = ev->curAlgo == 1;  // AllComb_TPAlgo: every candidate (Triplet.cc:45-48)
+    const AcIn A{AC_TP, G->NM, ev->sgIn.p, ev->sgOut.p, nullptr, ev->mdSlot.p, G->dConnStart, G->dConnList,
+                 ev->slotSgStart.p};
+    // tpCount was written by the SG stage (k_sg_extra) for the default join
+    if (allComb && nSG > 0)
+        hipLaunchKernelGGL(k_ac_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, A, ev->tpCount.p,
+                           (long long*)nullptr);
     if (int r = scanExclusive<int>(ev->tpCount.p, ev->tpStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TP)) return r;
     const long long nTP = sizeOf(ev, TOT_TP);
     if (nTP < 0) return fail(SDL_ERR_HIP, "readback failed");
@@ -2760,9 +2914,13 @@ int runTP(sdl_event* ev) {
     if (int r = ev->tpIn.ensure(std::max(nTP, 1LL))) return r;
     if (int r = ev->tpOut.ensure(std::max(nTP, 1LL))) return r;
     if (int r = ev->tpKey.ensure(std::max(nTP, 1LL))) return r;
-    if (nSG > 0)
+    if (nSG > 0 && !allComb)
         hipLaunchKernelGGL(k_tp_write, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
                            ev->sgStart.p, ev->tpStart.p, ev->sgKey.p, ev->tpIn.p, ev->tpOut.p, ev->tpKey.p,
+                           guardOf(ev, TOT_TP, nTP, SDL_STAGE_TP));
+    if (nSG > 0 && allComb)
+        hipLaunchKernelGGL(k_ac_write, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, A, ev->tpStart.p,
+                           ev->tpIn.p, ev->tpOut.p, ev->sgKey.p, ev->tpKey.p, (int2*)nullptr,
                            guardOf(ev, TOT_TP, nTP, SDL_STAGE_TP));
     hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->slotSgStart.p, ev->tpStart.p, ev->slotTpStart.p);
@@ -2782,7 +2940,44 @@ long long variantSize(sdl_event* ev, int totK, const long long* off1, const long
     return p1 - p0;
 }
 
+// Module-map tracklets with AllComb_TLAlgo (Tracklet.cc:139-142): every
+// candidate (Event.cc:627-693), also those whose subdet combination the default
+// algorithm rejects outright; stored by inner SG like the default ones.
+int runTLAllComb(sdl_event* ev) {
+    sdl_geom* G = ev->g;
+    hipStream_t st = ev->st;
+    const long long nSG = ev->nSG;
+    const long long ns = std::max(nSG, 1LL);
+    const int NS = ev->E * G->NM;
+    if (int r = ev->tlCap.ensure(ns)) return r;
+    if (int r = ev->tlCount.ensure(ns)) return r;
+    if (int r = ev->tlStart.ensure(ns + 1)) return r;
+    if (int r = ev->slotTlStart.ensure(NS + 1)) return r;
+    const Count cSG = devCount(ev, TOT_SG);
+    const AcIn A{AC_TL, G->NM, ev->sgIn.p, ev->sgOut.p, nullptr, ev->mdSlot.p, G->dConnStart, G->dConnList,
+                 ev->slotSgStart.p};
+    timerBegin(ev, T_TL);
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_ac_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, A, ev->tlCount.p,
+                           ev->tlCap.p);
+    if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
+    const long long nTL = sizeOf(ev, TOT_TL);
+    if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
+    ev->nTL = nTL;
+    if (int r = ev->tlIn.ensure(std::max(nTL, 1LL))) return r;
+    if (int r = ev->tlOut.ensure(std::max(nTL, 1LL))) return r;
+    if (nSG > 0)
+        hipLaunchKernelGGL(k_ac_write, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, A, ev->tlStart.p,
+                           ev->tlIn.p, ev->tlOut.p, (const int*)nullptr, (int*)nullptr, (int2*)nullptr,
+                           guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
+    timerEnd(ev, T_TL);
+    hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+                       ev->slotSgStart.p, ev->tlStart.p, ev->slotTlStart.p);
+    return 0;
+}
+
 int runTL(sdl_event* ev) {
+    if (ev->tlAlgo == TLA_ALLCOMB) return runTLAllComb(ev);
     sdl_geom* G = ev->g;
     hipStream_t st = ev->st;
     const long long nSG = ev->nSG;
@@ -2832,7 +3027,8 @@ int runTL(sdl_event* ev) {
     // first stage per pair; with the learned proposal, the pairs that pass are listed and scored
     const MDDev D = mdView(ev);
     const SGDev S = sgView(ev);
-    const bool learned = ev->proposal != nullptr;
+    // the proposal was trained on the default chain: DefaultNm1 tracklets run without it
+    const bool learned = ev->proposal != nullptr && ev->tlAlgo == TLA_DEFAULT;
     const long long waveStride = (nPairs + 63) / 64 + 1;  // per-variant region of the per-wave scored counts
     ev->tlScoredStride = waveStride;
     if (learned)
@@ -2945,10 +3141,18 @@ int runTCP(sdl_event* ev) {
         hipLaunchKernelGGL(k_tpe_write, dim3(gridFor(nTP, kBlock)), dim3(kBlock), 0, st, cTP, cSG, ev->tpeOff.p,
                            ev->sgEndList.p, ev->sgEndStart.p, ev->sgIn.p, ev->sgStart.p, ev->tpStart.p, ev->tpEndList.p,
                            guardOf(ev, TOT_TPE, nTP, SDL_STAGE_TCP));
+    const unsigned char* tlDefault = nullptr;
+    if (ev->tlAlgo != TLA_DEFAULT) {
+        if (int r = ev->tlDefault.ensure(nl)) return r;
+        tlDefault = ev->tlDefault.p;
+        if (nTL > 0)
+            hipLaunchKernelGGL(k_tl_default, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, G->NM, G->dMods,
+                               hitsView(ev), mdView(ev), sgView(ev), ev->tlIn.p, ev->tlOut.p, ev->tlDefault.p);
+    }
     if (nTL > 0)
         hipLaunchKernelGGL(k_tcp_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlIn.p,
                            ev->tlOut.p, ev->sgIn.p, ev->sgOut.p, ev->sgStart.p, ev->sgEndStart.p, ev->tpeOff.p,
-                           ev->tpStart.p, ev->tcpInfo.p, ev->tcCount.p);
+                           ev->tpStart.p, tlDefault, ev->tcpInfo.p, ev->tcCount.p);
     if (int r = scanExclusive<int>(ev->tcCount.p, ev->tcpStart.p, cTL, nTL, ev->scan, st, ev->tot.p, TOT_TCP)) return r;
     const long long nTCP = sizeOf(ev, TOT_TCP);
     if (nTCP < 0) return fail(SDL_ERR_HIP, "readback failed");
@@ -2969,17 +3173,28 @@ int runTCL(sdl_event* ev) {
     if (int r = ev->tcCount.ensure(nl)) return r;
     if (int r = ev->tclStart.ensure(nl + 1)) return r;
     const Count cTL = devCount(ev, TOT_TL);
-    if (nTL > 0)
+    // AllComb_TCAlgo (TrackCandidate.cc:138-141): every candidate
+    const bool allComb = ev->curAlgo == 1;
+    const AcIn A{AC_TCL, G->NM, ev->sgIn.p, ev->sgOut.p, ev->tlIn.p, ev->mdSlot.p, G->dConnStart, G->dConnList,
+                 ev->slotTlStart.p};
+    if (nTL > 0 && !allComb)
         hipLaunchKernelGGL(k_tcl_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlOut.p,
                            ev->tlStart.p, ev->tcCount.p);
+    if (nTL > 0 && allComb)
+        hipLaunchKernelGGL(k_ac_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, A, ev->tcCount.p,
+                           (long long*)nullptr);
     if (int r = scanExclusive<int>(ev->tcCount.p, ev->tclStart.p, cTL, nTL, ev->scan, st, ev->tot.p, TOT_TCL)) return r;
     const long long nTCL = sizeOf(ev, TOT_TCL);
     if (nTCL < 0) return fail(SDL_ERR_HIP, "readback failed");
     ev->nTCL = nTCL;
     if (int r = ev->tcl.ensure(std::max(nTCL, 1LL))) return r;
-    if (nTL > 0)
+    if (nTL > 0 && !allComb)
         hipLaunchKernelGGL(k_tcl_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlOut.p,
                            ev->tlStart.p, ev->tcCount.p, ev->tclStart.p, ev->tcl.p,
+                           guardOf(ev, TOT_TCL, nTCL, SDL_STAGE_TCL));
+    if (nTL > 0 && allComb)
+        hipLaunchKernelGGL(k_ac_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, A, ev->tclStart.p,
+                           (int*)nullptr, (int*)nullptr, (const int*)nullptr, (int*)nullptr, ev->tcl.p,
                            guardOf(ev, TOT_TCL, nTCL, SDL_STAGE_TCL));
     return 0;
 }
@@ -3026,7 +3241,7 @@ int runTLLayers(sdl_event* ev) {
     if (nSG > 0)
         hipLaunchKernelGGL(k_tll, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), sgView(ev), ev->layerSgOff.p, G->dLayerModStart, ev->layerSg.p,
-                           ev->tlCapInt.p, ev->tlCapOff.p, ev->tllSlab.p, ev->tlCount.p, ev->tlNm1,
+                           ev->tlCapInt.p, ev->tlCapOff.p, ev->tllSlab.p, ev->tlCount.p, ev->tlAlgo,
                            guardOf(ev, TOT_TLCANDS, slab, SDL_STAGE_TL));
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
     const long long nTL = sizeOf(ev, TOT_TL);
@@ -3066,7 +3281,7 @@ int runTLNavigation(sdl_event* ev) {
     if (nSG > 0)
         hipLaunchKernelGGL(k_tln, dim3(gridFor(nSG, kWavesPerBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), sgView(ev), ev->sgStart.p, ev->tlCapInt.p, ev->tlCapOff.p,
-                           ev->tllSlab.p, ev->tlCount.p, ev->tlNm1, guardOf(ev, TOT_TLCANDS, slab, SDL_STAGE_TL));
+                           ev->tllSlab.p, ev->tlCount.p, ev->tlAlgo, guardOf(ev, TOT_TLCANDS, slab, SDL_STAGE_TL));
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
     const long long nTL = sizeOf(ev, TOT_TL);
     if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
@@ -3091,6 +3306,7 @@ int runTCLayers(sdl_event* ev) {
     if (int r = ev->tclStart.ensure(nl + 1)) return r;
     if (int r = ev->tcpStart.ensure(nl + 1)) return r;
     if (int r = ev->tlPerLayer.ensure((long long)ev->E * kNBins)) return r;
+    const bool allComb = ev->curAlgo == 1;  // AllComb_TCAlgo (TrackCandidate.cc:138-141)
     // no TCs from triplets in this form
     zeroMany(st, {span(ev->tcpStart.p, nl + 1), span(ev->tot.p + TOT_TCP, 1),
                   span(ev->tlPerLayer.p, (long long)ev->E * kNBins)});
@@ -3101,17 +3317,23 @@ int runTCLayers(sdl_event* ev) {
                            ev->tlIn.p, ev->sgIn.p, ev->mdSlot.p, ev->tlPerLayer.p);
         hipLaunchKernelGGL(k_tcll_count, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, G->NM, G->dMods,
                            ev->tlIn.p, ev->tlOut.p, ev->sgIn.p, ev->mdSlot.p, ev->tlStart.p, ev->tlPerLayer.p,
-                           ev->tcCount.p, ev->tcCand.p);
+                           allComb ? 1 : 0, ev->tcCount.p, ev->tcCand.p);
     }
     if (int r = scanExclusive<int>(ev->tcCount.p, ev->tclStart.p, cTL, nTL, ev->scan, st, ev->tot.p, TOT_TCL)) return r;
     const long long nTCL = sizeOf(ev, TOT_TCL);
     if (nTCL < 0) return fail(SDL_ERR_HIP, "readback failed");
     ev->nTCL = nTCL;
     if (int r = ev->tcl.ensure(std::max(nTCL, 1LL))) return r;
-    if (nTL > 0)
+    if (nTL > 0 && !allComb)
         hipLaunchKernelGGL(k_tcl_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, ev->tlOut.p,
                            ev->tlStart.p, ev->tcCount.p, ev->tclStart.p, ev->tcl.p,
                            guardOf(ev, TOT_TCL, nTCL, SDL_STAGE_TCL));
+    if (nTL > 0 && allComb) {
+        if (int r = fillLayerList(ev, ev->slotTlStart.p, ev->layerTlOff, ev->layerTl, nTL)) return r;
+        hipLaunchKernelGGL(k_tcll_ac_write, dim3(gridFor(nTL, kBlock)), dim3(kBlock), 0, st, cTL, G->NM, G->dMods,
+                           ev->tlIn.p, ev->sgIn.p, ev->mdSlot.p, ev->layerTlOff.p, G->dLayerModStart, ev->layerTl.p,
+                           ev->tclStart.p, ev->tcl.p, guardOf(ev, TOT_TCL, nTCL, SDL_STAGE_TCL));
+    }
     return 0;
 }
 
@@ -3267,7 +3489,8 @@ int runStages(sdl_event* ev, int s0, int s1) {
     } else {
         zeroAsync(ev->tot.p + TOT_ABORT, 1, st);
     }
-    ev->spec = ev->haveHistory && s0 < SDL_STAGE_SG_LAYERS;  // the layer-pair stages are sized exactly
+    // the layer-pair stages and calls with a non-default algorithm are sized exactly
+    ev->spec = ev->haveHistory && s0 < SDL_STAGE_SG_LAYERS && !ev->forceExact;
     bool launched = false;
     if (ev->spec && ev->useGraphs && !ev->timing && !ev->hitsReady && s0 == SDL_STAGE_MD && s1 == SDL_STAGE_TCL) {
         const int r = launchChainGraph(ev);
@@ -3608,6 +3831,7 @@ void sdl_event_destroy(sdl_event* ev) {
 }
 
 int sdl_event_reset(sdl_event* ev) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev) return fail(SDL_ERR_ARG, "null event");
     ev->hx.clear(); ev->hy.clear(); ev->hz.clear(); ev->hidx.clear(); ev->hdet.clear();
     ev->hoff.assign(1, 0);
@@ -3644,6 +3868,7 @@ int sdl_event_next(sdl_event* ev) {
 
 int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offsets, const float* dx, const float* dy,
                               const float* dz, const int* didx, const unsigned int* ddetId) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || nevents <= 0 || !offsets) return fail(SDL_ERR_ARG, "bad arguments");
     ev->hostStaged = false;
     ev->E = nevents;
@@ -3656,11 +3881,15 @@ int sdl_event_set_hits_device(sdl_event* ev, int nevents, const long long* offse
 }
 
 int sdl_event_run(sdl_event* ev, int stage, int algo) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev) return fail(SDL_ERR_ARG, "null event");
     if (stage < SDL_STAGE_MD || stage > SDL_STAGE_MD_PSEUDO) return fail(SDL_ERR_ARG, "bad stage");
     const bool tlStage = stage == SDL_STAGE_TL || stage == SDL_STAGE_TL_LAYERS || stage == SDL_STAGE_TL_NAV;
-    if (algo != 0 && !(tlStage && algo == 1))
-        return fail(SDL_ERR_ARG, "only the default algorithms (and DefaultNm1 for tracklets) have device kernels");
+    // algorithms, the Algo.h values: 0 Default and 1 AllComb, for the tracklet
+    // stages 1 DefaultNm1 and 2 AllComb.  TCs from triplets ignore theirs (the
+    // bridges always run the default algorithms, Event.cc:874-982) and pseudo
+    // mini-doublets always run AllComb (Event.cc:353).
+    if (algo < 0 || algo > (tlStage ? 2 : 1)) return fail(SDL_ERR_ARG, "unknown algorithm value for this stage");
     // each stage reads the objects of the stage before it: module-map chain MD,
     // SG, TP, TL, TCP, TCL; layer-pair stages after MD (segments), after either
     // segment stage (tracklets) and after either tracklet stage (track
@@ -3677,31 +3906,47 @@ int sdl_event_run(sdl_event* ev, int stage, int algo) {
         ok = ev->stage == SDL_STAGE_TL || ev->stage == SDL_STAGE_TL_LAYERS || ev->stage == SDL_STAGE_TL_NAV;
     // navigation tracklets over module-map segments keep the TCL join exact
     if (stage == SDL_STAGE_TCL && ev->stage == SDL_STAGE_TL_NAV) ok = !(ev->ranMask & (1u << SDL_STAGE_SG_LAYERS));
-    // TCs from triplets re-run the default tracklet algorithm on their bridges
-    // (Event.cc:940-946); the device join reads the stored tracklets instead, which
-    // is exact only for default ones: after DefaultNm1 tracklets it is refused, and
-    // TCs from tracklets (a join over the stored tracklets) may follow directly
-    if (stage == SDL_STAGE_TCP && ev->stage == SDL_STAGE_TL && ev->tlNm1)
-        return fail(SDL_ERR_STATE, "TCs from triplets need default tracklets (the stored set is DefaultNm1)");
-    if (stage == SDL_STAGE_TCL && ev->stage == SDL_STAGE_TL && ev->tlNm1) ok = true;
+    // TCs from tracklets (a join over the stored tracklets) may also follow the
+    // module-map tracklets directly
+    if (stage == SDL_STAGE_TCL && ev->stage == SDL_STAGE_TL) ok = true;
     if (!ok) return fail(SDL_ERR_STATE, "stages must run in order");
-    if (tlStage) ev->tlNm1 = algo == 1 ? 1 : 0;
+    // TCs from triplets run the default segment and tracklet algorithms on their
+    // bridges (Event.cc:929-957).  The device reads those decisions from the
+    // stored segments and tracklets: after default or DefaultNm1 / AllComb
+    // tracklets it re-evaluates the stored tracklets, but AllComb segments or
+    // triplets leave nothing it could read them from.
+    if (stage == SDL_STAGE_TCP && (ev->acMask & ((1u << SDL_STAGE_SG) | (1u << SDL_STAGE_TP))))
+        return fail(SDL_ERR_STATE, "TCs from triplets after AllComb segments or triplets are not supported");
+    ev->curAlgo = (stage == SDL_STAGE_TCP || stage == SDL_STAGE_MD_PSEUDO) ? 0 : algo;
+    if (tlStage) {
+        ev->tlAlgo = algo;
+        ev->tlNm1 = algo == TLA_NM1 ? 1 : 0;
+    }
     if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
     if (!ev->hitsReady) {
         if (ev->hostStaged && ev->hoff.size() == 1) ev->hoff.push_back(0);
         if (int r = beginBatch(ev)) return r;
         if (ev->E <= 0) return fail(SDL_ERR_STATE, "no event");
         ev->ranMask = 0;
+        ev->acMask = 0;
     }
-    if (int r = runStages(ev, stage, stage)) return r;
+    ev->forceExact = ev->curAlgo != 0 || ev->acMask != 0 || ev->tlAlgo != TLA_DEFAULT;
+    const int r = runStages(ev, stage, stage);
+    ev->forceExact = false;
+    if (r) return r;
     if (hipGetLastError() != hipSuccess) return fail(SDL_ERR_HIP, "kernel launch failed");
     ev->stage = stage;
+    if (ev->curAlgo == (tlStage ? TLA_ALLCOMB : 1)) ev->acMask |= 1u << stage;
     return 0;
 }
 
 int sdl_event_run_chain(sdl_event* ev) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev) return fail(SDL_ERR_ARG, "null event");
     ev->tlNm1 = 0;  // the chain runs the default algorithms
+    ev->tlAlgo = TLA_DEFAULT;
+    ev->curAlgo = 0;
+    ev->acMask = 0;
     if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
     ev->stage = -1;
     ev->hitsReady = false;
@@ -3722,6 +3967,7 @@ int sdl_event_sync(sdl_event* ev) {
 }
 
 int sdl_event_layer_counts(sdl_event* ev, int e, unsigned long long out[110]) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || !out || e < 0 || e >= ev->E) return fail(SDL_ERR_ARG, "bad arguments");
     HIP_OK(hipMemcpyAsync(out, ev->counters.p + size_t(e) * kNCounters * kNBins,
                           sizeof(unsigned long long) * kNCounters * kNBins, hipMemcpyDeviceToHost, ev->st));
@@ -3730,6 +3976,7 @@ int sdl_event_layer_counts(sdl_event* ev, int e, unsigned long long out[110]) {
 }
 
 int sdl_event_counts(sdl_event* ev, int e, unsigned long long out[10]) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || !out || e < -1 || e >= ev->E) return fail(SDL_ERR_ARG, "bad arguments");
     std::vector<unsigned long long> all(size_t(ev->E) * kNCounters * kNBins);
     HIP_OK(hipMemcpyAsync(all.data(), ev->counters.p, sizeof(unsigned long long) * all.size(), hipMemcpyDeviceToHost, ev->st));
@@ -3778,6 +4025,7 @@ int sdl_event_counts(sdl_event* ev, int e, unsigned long long out[10]) {
 }
 
 int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || !n || e < 0 || e >= ev->E || stage < 0 || stage > SDL_STAGE_TCL) return fail(SDL_ERR_ARG, "bad arguments");
     // the layer-pair stages store SG, TL and TCL objects
     unsigned need = 1u << stage;
@@ -3853,6 +4101,7 @@ const char* sdl_object_var_names(int stage) {
 }
 
 int sdl_event_export_vars(sdl_event* ev, int e, int stage, float* out, long long cap, long long* n) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || !n || e < 0 || e >= ev->E || sdl_object_var_count(stage) == 0) return fail(SDL_ERR_ARG, "bad arguments");
     unsigned need = 1u << stage;
     if (stage == SDL_STAGE_MD) need |= 1u << SDL_STAGE_MD_PSEUDO;
@@ -3867,11 +4116,23 @@ int sdl_event_export_vars(sdl_event* ev, int e, int stage, float* out, long long
     if (!out) return 0;
     if (cap < cnt * nv) return fail(SDL_ERR_CAPACITY, "export buffer too small");
     if (cnt == 0) return 0;
+    // objects an AllComb algorithm accepted keep their constructors' values (0)
+    // and set no reco variable (NaN): MiniDoublet.cc:41-51, Segment.cc:35-56,
+    // Tracklet.cc:23-33, TrackCandidate.cc:121-141
+    const unsigned acBits = stage == SDL_STAGE_SG ? (1u << SDL_STAGE_SG) | (1u << SDL_STAGE_SG_LAYERS)
+                            : stage == SDL_STAGE_TL ? (1u << SDL_STAGE_TL) | (1u << SDL_STAGE_TL_LAYERS) | (1u << SDL_STAGE_TL_NAV)
+                            : stage == SDL_STAGE_TCL ? (1u << SDL_STAGE_TCL) | (1u << SDL_STAGE_TC_LAYERS) : 0u;
+    if (ev->acMask & acBits) {
+        const int typed = stage == SDL_STAGE_SG ? 14 : stage == SDL_STAGE_TL ? kTLTyped : 0;
+        for (long long i = 0; i < cnt; ++i)
+            for (int k = 0; k < nv; ++k) out[i * nv + k] = k < typed ? 0.f : __builtin_nanf("");
+        return 0;
+    }
     if (int r = ev->exportVars.ensure(cnt * nv)) return r;
     sdl_geom* G = ev->g;
     const unsigned grid = gridFor(cnt, kBlock);
     if (stage == SDL_STAGE_MD) {
-        const bool pseudo = !(ev->ranMask & (1u << SDL_STAGE_MD));
+        const bool pseudo = !(ev->ranMask & (1u << SDL_STAGE_MD)) || (ev->acMask & (1u << SDL_STAGE_MD));
         hipLaunchKernelGGL(k_vars_md, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, G->NM, G->dMods, hitsView(ev),
                            mdView(ev), pseudo, ev->exportVars.p);
     } else if (stage == SDL_STAGE_SG) {
@@ -3931,6 +4192,7 @@ int sdl_event_set_tuning(sdl_event* ev, const int* values, int n) {
 }
 
 int sdl_event_autotune(sdl_event* ev, int reps, char* report, int cap) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || reps < 1) return fail(SDL_ERR_ARG, "bad arguments");
     using clk = std::chrono::steady_clock;
     // one timed chain: wall time of run_chain, which ends with its host round trip
@@ -4099,6 +4361,7 @@ int sdl_mlp_forward(sdl_mlp* m, const float* dX, long long n, float* dLogit) {
 }
 
 int sdl_event_set_proposal(sdl_event* ev, sdl_mlp* m, float threshold) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || (m && !(threshold > 0.f && threshold < 1.f))) return fail(SDL_ERR_ARG, "bad arguments");
     ev->proposal = m;
     ev->proposalLogit = m ? std::log(threshold / (1.f - threshold)) : 0.f;
@@ -4106,6 +4369,7 @@ int sdl_event_set_proposal(sdl_event* ev, sdl_mlp* m, float threshold) {
 }
 
 int sdl_event_tl_work(sdl_event* ev, long long out[4]) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || !out) return fail(SDL_ERR_ARG, "bad arguments");
     long long t[TOT_N];
     HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, ev->st));
@@ -4127,6 +4391,7 @@ int sdl_event_tl_work(sdl_event* ev, long long out[4]) {
 }
 
 int sdl_event_tl_pair_rows(sdl_event* ev, float* dX, float* dY, long long cap, long long* n) {
+    OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || !n) return fail(SDL_ERR_ARG, "bad arguments");
     if (ev->stage < SDL_STAGE_TL) return fail(SDL_ERR_STATE, "the tracklet stage has not run");
     if (ev->proposal) return fail(SDL_ERR_STATE, "training rows come from a plain chain (no proposal set)");

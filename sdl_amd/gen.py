@@ -36,9 +36,9 @@ def _load():
 
 
 def geom_kind(workload):
-    """Workloads 1 and 4 use the flat six-layer barrel; 2, 3 and 5 the full
-    tilted-barrel + endcap detector."""
-    return 1 if workload in (1, 4) else 2
+    """Workloads 1, 4 and 11 use the flat six-layer barrel; 2, 3, 5, 12 and 15
+    the full tilted-barrel + endcap detector."""
+    return 1 if workload in (1, 4, 11) else 2
 
 
 MAP_TRACKS = {1: 50000, 2: 200000}

@@ -15,7 +15,10 @@ the per-object values of every MD, SG and TL of the module-map chain (typed
 getters and reco variables, as float bit patterns) in
 ``tests/golden/ref_vars.npz``.
 
-Usage:  python tests/golden/make_golden.py [layers|nav|pseudo|vars]   (only those fixtures)
+and stage sequences with other algorithms (AllComb, DefaultNm1; the
+reference driver's dumpseq mode) in ``tests/golden/ref_seq_sets.npz``.
+
+Usage:  python tests/golden/make_golden.py [layers|nav|pseudo|vars|nm1|seq]   (only those fixtures)
 """
 import os
 import subprocess
@@ -54,6 +57,8 @@ def main():
         var_fixtures()
     if which in ("all", "nm1"):
         nm1_fixtures()
+    if which in ("all", "seq"):
+        seq_fixtures()
 
 
 def sorted_rows(keys, bits):
@@ -154,6 +159,39 @@ def nm1_fixtures():
                     out[f"w{wl}_e{e}_{s}"] = oracle.as_multiset(sets[s])
                 out[f"w{wl}_e{e}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
     np.savez_compressed(os.path.join(HERE, "ref_nm1_sets.npz"), **out)
+
+
+# Stage sequences with the Algo.h algorithms (sdl_ref_driver dumpseq): name ->
+# (workload, event, driver sequence, oracle stage codes: stage + 100 * algo).
+SEQS = {
+    "md_allcomb": (11, 0, "md:1,sg:0,tp:0,tl:0,tcp:0,tcl:0", [100, 1, 2, 3, 4, 5]),
+    "sg_allcomb": (11, 0, "md:0,sg:1,tp:0,tl:0,tcp:0,tcl:0", [0, 101, 2, 3, 4, 5]),
+    "tp_allcomb": (11, 0, "md:0,sg:0,tp:1,tl:0,tcp:0,tcl:0", [0, 1, 102, 3, 4, 5]),
+    "tl_allcomb": (11, 0, "md:0,sg:0,tp:0,tl:2,tcp:0,tcl:0", [0, 1, 2, 203, 4, 5]),
+    "tc_allcomb": (11, 0, "md:0,sg:0,tp:0,tl:0,tcp:1,tcl:1", [0, 1, 2, 3, 104, 105]),
+    "all_allcomb": (11, 0, "md:1,sg:1,tp:1,tl:2,tcp:0,tcl:1", [100, 101, 102, 203, 4, 105]),
+    "layers_sg_allcomb": (11, 0, "md:0,sgl:1,tll:0,tcll:0", [0, 106, 7, 8]),
+    "layers_tl_allcomb": (11, 0, "md:0,sgl:0,tll:2,tcll:0", [0, 6, 207, 8]),
+    "layers_tc_allcomb": (11, 1, "md:0,sgl:0,tll:0,tcll:1", [0, 6, 7, 108]),
+    "nav_allcomb": (11, 0, "md:0,sg:0,tln:2,tcl:1", [0, 1, 209, 105]),
+    "nm1_chain": (1, 0, "md:0,sg:0,tp:0,tl:1,tcp:0,tcl:0", [0, 1, 2, 103, 4, 5]),
+    "nm1_layers": (1, 0, "md:0,sgl:0,tll:1,tcll:0", [0, 6, 107, 8]),
+    "nm1_nav": (1, 0, "md:0,sg:0,tln:1,tcl:0", [0, 1, 109, 5]),
+}
+
+
+def seq_fixtures():
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, (wl, e, spec, _) in SEQS.items():
+            dump = os.path.join(tmp, f"{name}.txt")
+            subprocess.run([oracle.REF_DRIVER, "dumpseq", str(wl), tmp, str(e), "1", dump, spec], check=True)
+            (sets, counters), = oracle.parse_ref_dump(dump).values()
+            for s in oracle.STAGES:
+                out[f"{name}_{s}"] = oracle.as_multiset(sets[s])
+            out[f"{name}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
+            out[f"{name}_layers"] = counters["layers"]
+    np.savez_compressed(os.path.join(HERE, "ref_seq_sets.npz"), **out)
 
 
 def pseudo_fixtures():

@@ -3,6 +3,8 @@ stages; Tracklet.cc:127-131, the default cuts with dBeta waived) against the
 oracle, whose restatement is pinned to the compiled reference in
 test_oracle.py::test_oracle_nm1_tracklets_match_reference: object sets,
 counters and the tracklets' per-object values."""
+import os
+
 import numpy as np
 import pytest
 
@@ -35,18 +37,24 @@ def _bits(v):
 def test_nm1_module_map_chain_matches_oracle(wl, e):
     print(f"case nm1 module-map chain workload={wl} event={e}", flush=True)
     S = runtime
-    ev, x = _run(wl, e, [(S.STAGE_MD, 0), (S.STAGE_SG, 0), (S.STAGE_TP, 0), (S.STAGE_TL, NM1)])
-    # TCs from triplets re-run the default algorithm on their bridges: refused after Nm1 tracklets
-    with pytest.raises(runtime.SDLError):
-        ev.run(S.STAGE_TCP)
-    ev.run(S.STAGE_TCL)
-    ev.sync()
+    # the reference's own sequence (ref_driver dumpnm1): TCs from triplets re-run
+    # the default tracklet algorithm on their bridges, TCs from tracklets join
+    # the stored (DefaultNm1) tracklets
+    ev, x = _run(wl, e, [(S.STAGE_MD, 0), (S.STAGE_SG, 0), (S.STAGE_TP, 0), (S.STAGE_TL, NM1), (S.STAGE_TCP, 0),
+                         (S.STAGE_TCL, 0)])
     paths = gen.geometry(gen.geom_kind(wl))[1]
-    sets, counters, values = oracle.run(paths, x, oracle.CR, [0, 1, 2, 11, 5], vars=True)
-    got = {s: ev.export(i) for i, s in enumerate(runtime.STAGES) if s != "TCP"}
-    for s in ("MD", "SG", "TP", "TL", "TCL"):
+    sets, counters, values = oracle.run(paths, x, oracle.CR, [0, 1, 2, 11, 4, 5], vars=True)
+    got = {s: ev.export(i) for i, s in enumerate(runtime.STAGES)}
+    for s in runtime.STAGES:
         assert np.array_equal(oracle.as_multiset(got[s]), oracle.as_multiset(sets[s])), s
     assert ev.counts(0) == counters
+    if wl in (1, 15):  # the compiled reference's sets for this sequence
+        with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_nm1_sets.npz")) as f:
+            for s in runtime.STAGES:
+                assert np.array_equal(oracle.as_multiset(got[s]), f[f"w{wl}_e{e}_{s}"]), s
+    # TCP may also be skipped: TCs from tracklets directly after the tracklets
+    ev2, _ = _run(wl, e, [(S.STAGE_MD, 0), (S.STAGE_SG, 0), (S.STAGE_TP, 0), (S.STAGE_TL, NM1), (S.STAGE_TCL, 0)])
+    assert np.array_equal(oracle.as_multiset(ev2.export(S.STAGE_TCL)), oracle.as_multiset(sets["TCL"]))
     plain, _ = oracle.run(paths, x, oracle.CR, 4)
     assert len(sets["TL"]) > len(plain["TL"])  # the waived cut admits more tracklets
     if wl != 2:  # the values of every Nm1 tracklet, including those that fail dBeta

@@ -381,3 +381,52 @@ def test_object_vars_layout_and_defaults(geom_paths):
         got = t["reco_" + dk][ends]
         assert all(g in (np.float32(det[a]), np.float32(det[b])) for g, a, b in zip(got, lo, up))
     assert (np.abs(tl[~bbee][:, 2]) < tl[~bbee][:, 3]).all()   # |betaOut| < betaOut cut
+
+
+def _seq_table():
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        "make_golden", os.path.join(os.path.dirname(__file__), "golden", "make_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.SEQS
+
+
+SEQS = _seq_table()
+
+
+@pytest.mark.parametrize("name", sorted(SEQS))
+def test_oracle_sequences_match_reference(name, geom_paths):
+    """Stage sequences with the AllComb algorithms (MiniDoublet.cc:223-226,
+    Segment.cc:313-335, Triplet.cc:45-48, Tracklet.cc:139-142,
+    TrackCandidate.cc:138-141) and with DefaultNm1 tracklets in each tracklet
+    stage, against the compiled reference (ref_seq_sets.npz, the reference
+    driver's dumpseq mode): all six object sets and the counters."""
+    import os
+    wl, e, _, codes = SEQS[name]
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_seq_sets.npz")) as f:
+        golden = {k: f[k] for k in f.files if k.startswith(name + "_")}
+    ev = gen.event(wl, e)
+    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, codes)
+    for s in oracle.STAGES:
+        ref = golden[f"{name}_{s}"]
+        got = oracle.as_multiset(sets[s])
+        assert got.shape == ref.shape and np.array_equal(got, ref), s
+    assert [counters[k] for k in COUNTERS] == list(golden[f"{name}_counters"])
+
+
+def test_allcomb_accepts_every_candidate(geom_paths):
+    """What holds by construction: with every stage AllComb the object counts
+    equal the candidate counts, and an AllComb stage's objects contain the
+    default stage's."""
+    wl, e = 11, 0
+    ev = gen.event(wl, e)
+    paths = geom_paths[gen.geom_kind(wl)]
+    sets, c = oracle.run(paths, ev, oracle.GLIBC, [100, 101, 102, 203, 105])
+    assert (c["nMD"], c["nSG"], c["nTP"], c["nTL"], c["nTC"]) == (
+        c["nMDCand"], c["nSGCand"], c["nTPCand"], c["nTLCand"], c["nTCCand"])
+    plain, _ = oracle.run(paths, ev, oracle.GLIBC, 6)
+    for codes, stage in (([100], "MD"), ([0, 101], "SG"), ([0, 1, 102], "TP"), ([0, 1, 2, 203], "TL")):
+        ac, _ = oracle.run(paths, ev, oracle.GLIBC, codes)
+        assert {tuple(r) for r in plain[stage]} <= {tuple(r) for r in ac[stage]}, stage
