@@ -84,6 +84,37 @@ int sdl_geom_save(const sdl_geom* g, const char* binaryPath);
 int sdl_geom_load_binary(const char* binaryPath, sdl_geom** out);
 int sdl_geom_check_binary(const char* binaryPath, int* numModules, long long* numConnections);
 
+/* The geometry from tables held in memory instead of the three files: the
+ * content of the reference's singletons after TiltedGeometry::load,
+ * EndcapGeometry::load and ModuleConnectionMap::load / add (TiltedGeometry.cc:18-41,
+ * EndcapGeometry.cc:18-61, ModuleConnectionMap.cc:18-85).  nT tilted rows
+ * (detId, drdz, slope); nE endcap rows (detId and its eight float columns
+ * avgr2 yl sl yh sh c7 c8 cz, 8 per row in eCols, column 7 read as the centroid
+ * phi); nM connection rows (detId, mCount[i] connected detIds, concatenated in
+ * mList).  Same checks and result as sdl_geom_load on files with these rows. */
+int sdl_geom_load_tables(int nT, const unsigned* tDet, const float* tDrdz, const float* tSlope, int nE,
+                         const unsigned* eDet, const float* eCols, int nM, const unsigned* mDet, const int* mCount,
+                         const unsigned* mList, sdl_geom** out);
+
+/* The reference's detId decoding (Module.cc:159-358): out = subdet, side,
+ * layer, rod, ring, module, isLower, isInverted, partnerDetId, moduleType
+ * (0 PS, 1 2S), moduleLayerType (0 Pixel, 1 Strip).  Host only. */
+int sdl_module_decode(unsigned int detId, unsigned int out[11]);
+
+/* Connection map from simulated tracks (no counterpart in the reference, whose
+ * map files were produced outside it; the result follows its format,
+ * ModuleConnectionMap.cc:36-44, and the sort + unique of
+ * ModuleConnectionMap::add, ModuleConnectionMap.cc:49-85).  Track t is the run
+ * [trackOffsets[t], trackOffsets[t+1]) of (station, detId) crossings with
+ * non-decreasing stations (one station: one layer or disk crossing); every
+ * lower module of a station connects to every lower module of the next
+ * station of the track (an upper sensor counts as its lower partner).  Writes
+ * "detid n d1 ... dn" lines, sorted, to mapPath; with merge != 0 the lists
+ * already in mapPath are kept and merged (sorted, duplicates removed), as
+ * ModuleConnectionMap::add merges a second file.  Host only. */
+int sdl_map_build(long long nTracks, const long long* trackOffsets, const int* station, const unsigned* detId,
+                  const char* mapPath, int merge);
+
 /* Creates an event handle (batch of events) on the geometry's device. */
 int sdl_event_create(sdl_geom* g, sdl_event** out);
 void sdl_event_destroy(sdl_event* ev);

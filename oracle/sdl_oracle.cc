@@ -1614,6 +1614,16 @@ using namespace oracle;
 
 extern "C" {
 
+// The restated detId decoding, fields in the order subdet, side, layer, rod,
+// ring, module, isLower, isInverted, partnerDetId, moduleType, moduleLayerType.
+void oracle_module_decode(unsigned d, unsigned out[11]) {
+    const ModInfo m = decode(d);
+    const unsigned v[11] = {unsigned(m.subdet), unsigned(m.side), unsigned(m.layer), unsigned(m.rod),
+                            unsigned(m.ring), unsigned(m.module), unsigned(m.isLower), unsigned(m.isInverted),
+                            m.partner, unsigned(m.type), unsigned(m.layerType)};
+    for (int k = 0; k < 11; ++k) out[k] = v[k];
+}
+
 void* oracle_geom_load(const char* tiltedPath, const char* endcapPath, const char* mapPath) {
     auto* g = new Geometry();
     if (!loadGeometry(*g, tiltedPath, endcapPath, mapPath)) { delete g; return nullptr; }

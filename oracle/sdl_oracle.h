@@ -49,6 +49,11 @@ enum { ORACLE_MD = 0, ORACLE_SG = 1, ORACLE_TP = 2, ORACLE_TL = 3, ORACLE_TCP = 
 // TCL (2): recovars dR, innerR of a TC made from tracklets (TrackCandidate.cc:206-207).
 enum { kMDVars = 7, kSGVars = 22, kTLTyped = 6, kTLKeys = 108, kTLVars = kTLTyped + kTLKeys, kTCLVars = 2 };
 
+// The detId decoding of Module.cc:159-358: subdet, side, layer, rod, ring,
+// module, isLower, isInverted, partnerDetId, moduleType (0 PS, 1 2S),
+// moduleLayerType (0 Pixel, 1 Strip).
+void oracle_module_decode(unsigned detId, unsigned out[11]);
+
 // Geometry in the reference's three text formats (TiltedGeometry.cc:36,
 // EndcapGeometry.cc:48, ModuleConnectionMap.cc:36-44).  Returns null on error.
 void* oracle_geom_load(const char* tiltedPath, const char* endcapPath, const char* mapPath);

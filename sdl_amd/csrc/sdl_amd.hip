@@ -23,6 +23,7 @@
 #include <fstream>
 #include <initializer_list>
 #include <iterator>
+#include <map>
 #include <sstream>
 #include <string>
 #include <unordered_map>
@@ -32,6 +33,7 @@
 #include "../../include/sdl_amd.h"
 #include "sdl_cuts.h"
 #include "sdl_mlp.h"
+#include "sdl_module.h"
 
 using namespace sdl;
 
@@ -2294,30 +2296,21 @@ __global__ void k_export(long long a, long long n, int kind, const int* pA, cons
 }
 
 // ------------------------------------------------------------------ host geometry
+// The module table entry of a detId: the decode of sdl_module.h plus the
+// per-module constants of the mini-doublet algebra.
 ModDev decodeModule(uint32_t d) {
-    // Module.cc:159-358
+    const ModuleId id = decodeDetId(d);
     ModDev m{};
     m.detId = d;
     m.partner = -1;
-    m.subdet = (d >> 25) & 7;
-    if (m.subdet == SUB_ENDCAP) { m.side = (d >> 23) & 3; m.layer = (d >> 18) & 7; m.ring = (d >> 12) & 15; m.rod = 0; }
-    else if (m.subdet == SUB_BARREL) { m.side = (d >> 18) & 3; m.layer = (d >> 20) & 7; m.rod = (d >> 10) & 127; m.ring = 0; }
-    const int mod = (d >> 2) & 127;
-    bool inv = false;
-    if (m.subdet == SUB_ENDCAP) {
-        if (m.side == SIDE_NEGZ) inv = mod % 2 == 1;
-        else if (m.side == SIDE_POSZ) inv = mod % 2 == 0;
-    } else if (m.subdet == SUB_BARREL) {
-        if (m.side == SIDE_CENTER) inv = m.layer <= 3 ? mod % 2 == 1 : (m.layer >= 4 ? mod % 2 == 0 : false);
-        else if (m.side == SIDE_NEGZ || m.side == SIDE_POSZ) inv = m.layer <= 2 ? mod % 2 == 1 : (m.layer == 3 ? mod % 2 == 0 : false);
-    }
-    m.isLower = inv ? !(d & 1) : (d & 1);
-    if (m.subdet == SUB_BARREL) m.type = m.layer <= 3 ? TYPE_PS : TYPE_2S;
-    else if (m.layer <= 2) m.type = m.ring <= 10 ? TYPE_PS : TYPE_2S;
-    else m.type = m.ring <= 7 ? TYPE_PS : TYPE_2S;
-    if (m.type == TYPE_2S) m.layerType = LT_STRIP;
-    else if (inv) m.layerType = m.isLower ? LT_STRIP : LT_PIXEL;
-    else m.layerType = m.isLower ? LT_PIXEL : LT_STRIP;
+    m.subdet = uint8_t(id.subdet);
+    m.side = uint8_t(id.side);
+    m.layer = uint8_t(id.layer);
+    m.ring = uint8_t(id.ring);
+    m.rod = uint8_t(id.rod);
+    m.isLower = id.isLower;
+    m.type = id.type;
+    m.layerType = id.layerType;
     // MiniDoublet::isTighterTiltedModules / moduleGapSize (MiniDoublet.cc:1242-1332)
     bool tighter = false;
     if (m.subdet == SUB_BARREL && m.side != SIDE_CENTER) {
@@ -2338,18 +2331,7 @@ ModDev decodeModule(uint32_t d) {
     return m;
 }
 
-uint32_t partnerOf(uint32_t d) {
-    const ModDev m = decodeModule(d);
-    const int mod = (d >> 2) & 127;
-    bool inv = false;
-    if (m.subdet == SUB_ENDCAP) inv = m.side == SIDE_NEGZ ? mod % 2 == 1 : (m.side == SIDE_POSZ ? mod % 2 == 0 : false);
-    else if (m.subdet == SUB_BARREL) {
-        if (m.side == SIDE_CENTER) inv = m.layer <= 3 ? mod % 2 == 1 : mod % 2 == 0;
-        else inv = m.layer <= 2 ? mod % 2 == 1 : (m.layer == 3 ? mod % 2 == 0 : false);
-    }
-    if (m.isLower) return inv ? d - 1 : d + 1;
-    return inv ? d + 1 : d - 1;
-}
+uint32_t partnerOf(uint32_t d) { return decodeDetId(d).partner; }
 
 }  // namespace
 
@@ -2551,9 +2533,16 @@ long long readLL(const long long* d, hipStream_t st) {
 
 unsigned gridFor(long long n, int per) { return unsigned((n + per - 1) / per); }
 
-int loadGeomFiles(const char* tilted, const char* endcap, const char* map, sdl_geom& G) {
+// The three geometry tables as the reference's singletons hold them:
+// TiltedGeometry drdz and slope (TiltedGeometry.cc:36-39), EndcapGeometry's
+// lower slope and column 7, read as the centroid phi (EndcapGeometry.cc:48-58),
+// and the connection lists in file order (ModuleConnectionMap.cc:36-44).
+struct GeomTables {
     std::unordered_map<uint32_t, float> drdz, slope, ecSlope, ecPhi;
     std::vector<std::pair<uint32_t, std::vector<uint32_t>>> conn;
+};
+
+int parseGeomFiles(const char* tilted, const char* endcap, const char* map, GeomTables& T) {
     std::ifstream ft(tilted), fe(endcap), fm(map);
     if (!ft) return fail(SDL_ERR_IO, std::string("cannot open ") + tilted);
     if (!fe) return fail(SDL_ERR_IO, std::string("cannot open ") + endcap);
@@ -2562,33 +2551,43 @@ int loadGeomFiles(const char* tilted, const char* endcap, const char* map, sdl_g
     while (std::getline(ft, line)) {
         std::stringstream ss(line);
         unsigned d; float a, b;
-        if (ss >> d >> a >> b) { drdz[d] = a; slope[d] = b; }
+        if (ss >> d >> a >> b) { T.drdz[d] = a; T.slope[d] = b; }
     }
     while (std::getline(fe, line)) {
         std::stringstream ss(line);
         unsigned d; float avgr2, yl, sl, yh, sh, c7, c8, cz;
-        if (ss >> d >> avgr2 >> yl >> sl >> yh >> sh >> c7 >> c8 >> cz) { ecSlope[d] = sl; ecPhi[d] = c7; }
+        if (ss >> d >> avgr2 >> yl >> sl >> yh >> sh >> c7 >> c8 >> cz) { T.ecSlope[d] = sl; T.ecPhi[d] = c7; }
     }
-    std::unordered_set<uint32_t> ids;
     while (std::getline(fm, line)) {
         std::stringstream ss(line);
         unsigned d; int n;
         if (!(ss >> d >> n)) continue;
         std::vector<uint32_t> v;
-        std::unordered_set<uint32_t> seen;
         for (int i = 0; i < n; ++i) {
             unsigned c;
             if (!(ss >> c)) break;
-            if (!seen.insert(c).second)
-                return fail(SDL_ERR_DUPLICATE_CONNECTION, "connection list of " + std::to_string(d) + " repeats " + std::to_string(c));
             v.push_back(c);
+        }
+        T.conn.push_back({d, v});
+    }
+    return 0;
+}
+
+// The device-ready module table and CSR connection map of a set of tables.
+int buildGeom(const GeomTables& T, sdl_geom& G) {
+    std::unordered_set<uint32_t> ids;
+    for (auto& kv : T.conn) {
+        std::unordered_set<uint32_t> seen;
+        for (uint32_t c : kv.second) {
+            if (!seen.insert(c).second)
+                return fail(SDL_ERR_DUPLICATE_CONNECTION,
+                            "connection list of " + std::to_string(kv.first) + " repeats " + std::to_string(c));
             ids.insert(c);
         }
-        ids.insert(d);
-        conn.push_back({d, v});
+        ids.insert(kv.first);
     }
-    for (auto& kv : drdz) ids.insert(kv.first);
-    for (auto& kv : ecSlope) ids.insert(kv.first);
+    for (auto& kv : T.drdz) ids.insert(kv.first);
+    for (auto& kv : T.ecSlope) ids.insert(kv.first);
     std::vector<uint32_t> base(ids.begin(), ids.end());
     for (uint32_t d : base) ids.insert(partnerOf(d));
     G.detIds.assign(ids.begin(), ids.end());
@@ -2606,13 +2605,13 @@ int loadGeomFiles(const char* tilted, const char* endcap, const char* map, sdl_g
         const uint32_t p = partnerOf(G.detIds[i]);
         m.partner = index.count(p) ? index[p] : -1;
         const uint32_t strip = m.layerType == LT_PIXEL ? p : G.detIds[i];
-        m.drdzStrip = get(drdz, strip);
-        m.slopeStrip = m.subdet == SUB_ENDCAP ? get(ecSlope, strip) : get(slope, strip);
-        m.centroidPhi = get(ecPhi, G.detIds[i]);
+        m.drdzStrip = get(T.drdz, strip);
+        m.slopeStrip = m.subdet == SUB_ENDCAP ? get(T.ecSlope, strip) : get(T.slope, strip);
+        m.centroidPhi = get(T.ecPhi, G.detIds[i]);
         G.mods[i] = m;
     }
     std::vector<std::vector<int>> adj(G.NM);
-    for (auto& kv : conn) {
+    for (auto& kv : T.conn) {
         std::vector<int>& a = adj[index[kv.first]];
         a.clear();  // a later line for the same detId replaces the earlier one (ModuleConnectionMap.cc:45)
         for (uint32_t c : kv.second) a.push_back(index[c]);
@@ -2622,6 +2621,12 @@ int loadGeomFiles(const char* tilted, const char* endcap, const char* map, sdl_g
     G.connList.clear();
     for (int i = 0; i < G.NM; ++i) G.connList.insert(G.connList.end(), adj[i].begin(), adj[i].end());
     return 0;
+}
+
+int loadGeomFiles(const char* tilted, const char* endcap, const char* map, sdl_geom& G) {
+    GeomTables T;
+    if (int r = parseGeomFiles(tilted, endcap, map, T)) return r;
+    return buildGeom(T, G);
 }
 
 int ensureHits(sdl_event* ev) {
@@ -3760,6 +3765,96 @@ int sdl_geom_load(const char* tiltedPath, const char* endcapPath, const char* ma
     if (int r = loadGeomFiles(tiltedPath, endcapPath, mapPath, *G)) { delete G; return r; }
     if (int r = uploadGeom(G)) { sdl_geom_free(G); return r; }
     *out = G;
+    return 0;
+}
+
+int sdl_geom_load_tables(int nT, const unsigned* tDet, const float* tDrdz, const float* tSlope, int nE,
+                         const unsigned* eDet, const float* eCols, int nM, const unsigned* mDet, const int* mCount,
+                         const unsigned* mList, sdl_geom** out) {
+    if (!out || nT < 0 || nE < 0 || nM < 0 || (nT && (!tDet || !tDrdz || !tSlope)) || (nE && (!eDet || !eCols)) ||
+        (nM && (!mDet || !mCount)))
+        return fail(SDL_ERR_ARG, "bad arguments");
+    GeomTables T;
+    for (int i = 0; i < nT; ++i) { T.drdz[tDet[i]] = tDrdz[i]; T.slope[tDet[i]] = tSlope[i]; }
+    // the eight columns after the detId (EndcapGeometry.cc:48): avgr2 yl sl yh sh c7 c8 cz
+    for (int i = 0; i < nE; ++i) { T.ecSlope[eDet[i]] = eCols[8 * i + 2]; T.ecPhi[eDet[i]] = eCols[8 * i + 5]; }
+    long long k = 0;
+    for (int i = 0; i < nM; ++i) {
+        if (mCount[i] < 0 || (mCount[i] && !mList)) return fail(SDL_ERR_ARG, "bad connection count");
+        T.conn.push_back({mDet[i], std::vector<uint32_t>(mList + k, mList + k + mCount[i])});
+        k += mCount[i];
+    }
+    auto* G = new sdl_geom();
+    if (int r = buildGeom(T, *G)) { delete G; return r; }
+    if (int r = uploadGeom(G)) { sdl_geom_free(G); return r; }
+    *out = G;
+    return 0;
+}
+
+int sdl_module_decode(unsigned int detId, unsigned int out[11]) {
+    if (!out) return fail(SDL_ERR_ARG, "null argument");
+    const ModuleId m = decodeDetId(detId);
+    const unsigned v[11] = {m.subdet, m.side, m.layer, m.rod, m.ring, m.module, m.isLower, m.isInverted,
+                            m.partner, m.type, m.layerType};
+    std::copy(v, v + 11, out);
+    return 0;
+}
+
+int sdl_map_build(long long nTracks, const long long* trackOffsets, const int* station, const unsigned* detId,
+                  const char* mapPath, int merge) {
+    if (nTracks < 0 || (nTracks && (!trackOffsets || !station || !detId)) || !mapPath)
+        return fail(SDL_ERR_ARG, "bad arguments");
+    std::map<uint32_t, std::vector<uint32_t>> conn;
+    if (merge) {  // ModuleConnectionMap::add (ModuleConnectionMap.cc:49-85) onto the existing file
+        std::ifstream fm(mapPath);
+        std::string line;
+        while (fm && std::getline(fm, line)) {
+            std::stringstream ss(line);
+            unsigned d; int n;
+            if (!(ss >> d >> n)) continue;
+            std::vector<uint32_t>& v = conn[d];
+            for (int i = 0; i < n; ++i) {
+                unsigned c;
+                if (!(ss >> c)) break;
+                v.push_back(c);
+            }
+        }
+    }
+    for (long long t = 0; t < nTracks; ++t) {
+        const long long a = trackOffsets[t], b = trackOffsets[t + 1];
+        if (b < a) return fail(SDL_ERR_ARG, "track offsets must not decrease");
+        // stations in order; every lower module of one station connects to every
+        // lower module of the next (an upper sensor stands for its lower partner)
+        std::vector<uint32_t> prev, cur;
+        int curStation = 0;
+        for (long long k = a; k <= b; ++k) {
+            if (k == b || (k > a && station[k] != curStation)) {
+                if (!cur.empty()) {
+                    for (uint32_t p : prev)
+                        for (uint32_t c : cur) conn[p].push_back(c);
+                    prev.swap(cur);
+                    cur.clear();
+                }
+                if (k == b) break;
+            }
+            if (k > a && station[k] < curStation) return fail(SDL_ERR_ARG, "stations of a track must not decrease");
+            curStation = station[k];
+            const ModuleId m = decodeDetId(detId[k]);
+            const uint32_t lower = m.isLower ? detId[k] : m.partner;
+            if (std::find(cur.begin(), cur.end(), lower) == cur.end()) cur.push_back(lower);
+        }
+    }
+    std::ofstream fo(mapPath);
+    if (!fo) return fail(SDL_ERR_IO, std::string("cannot write ") + mapPath);
+    for (auto& kv : conn) {
+        std::vector<uint32_t>& v = kv.second;
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        fo << kv.first << ' ' << v.size();
+        for (uint32_t c : v) fo << ' ' << c;
+        fo << '\n';
+    }
+    if (!fo) return fail(SDL_ERR_IO, std::string("cannot write ") + mapPath);
     return 0;
 }
 

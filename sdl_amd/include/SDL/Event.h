@@ -1,75 +1,49 @@
 // SDL::Event over the MI355X C-ABI (include/sdl_amd.h): a drop-in for the
-// reference's host API (Event.h:24-411, Hit.h, TiltedGeometry.h,
-// EndcapGeometry.h, ModuleConnectionMap.h) for the module-map chain.
+// reference's host API (Event.h:24-411 with Hit.h, Module.h, Layer.h,
+// MiniDoublet.h, Segment.h, Triplet.h, Tracklet.h, TrackCandidate.h,
+// TiltedGeometry.h, EndcapGeometry.h, ModuleConnectionMap.h and Algo.h).
 //
 // Differences a caller can see:
-//  * the three geometry singletons only record their file; the device copy of
-//    the geometry is built when the first Event is created after a load();
-//  * object graphs (Module / Layer / MiniDoublet pointers) are not materialised:
-//    objects are read back as hit-index tuples (getMiniDoublets() etc.);
-//  * only the Default algorithms, and DefaultNm1_TLAlgo for the three tracklet
-//    stages, run on the device; another algorithm value prints a warning and
-//    leaves the stage empty, as the reference does for an unrecognised value
-//    (MiniDoublet.cc:216-220);
-//  * per-object values (the objects' getters and reco variables) are read back
-//    as float rows parallel to the object tuples (getMiniDoubletVariables() etc.).
+//  * the device builds the geometry from the three singletons when the first
+//    Event is created after a load() or add();
+//  * the objects (MiniDoublet, Segment, Triplet, Tracklet, TrackCandidate) are
+//    host views of the device results, built when they are first read after a
+//    stage ran (getModulePtrs(), getLayer(), ...): the same objects, with the
+//    same pointers between them and the same values as the reference's, but
+//    inside a module or layer list not necessarily in the reference's order;
+//    the strip-shifted hits of a mini-doublet (getLowerShiftedHit) are not kept;
+//  * an algorithm value that Algo.h does not define is rejected with
+//    std::invalid_argument (the reference warns and stores nothing);
+//    createTrackCandidatesFromTriplets after AllComb segments or triplets
+//    throws std::logic_error;
+//  * the objects can also be read as hit-index tuples (getMiniDoublets() etc.)
+//    and their values as float rows parallel to them (getMiniDoubletVariables() etc.).
 #ifndef SDL_AMD_EVENT_H
 #define SDL_AMD_EVENT_H
 
 #include <array>
-#include <cmath>
+#include <deque>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "Algo.h"
+#include "EndcapGeometry.h"
+#include "Hit.h"
+#include "Layer.h"
+#include "MiniDoublet.h"
+#include "Module.h"
+#include "ModuleConnectionMap.h"
+#include "Segment.h"
+#include "TiltedGeometry.h"
+#include "TrackCandidate.h"
+#include "Tracklet.h"
+#include "Triplet.h"
 
 struct sdl_event;
 
 namespace SDL {
-
-// Hit.h:19-93: a point with an index; derived quantities on demand.
-class Hit {
-  public:
-    Hit() : x_(0), y_(0), z_(0), idx_(-1) {}
-    Hit(float x, float y, float z, int idx = -1) : x_(x), y_(y), z_(z), idx_(idx) {}
-    const float& x() const { return x_; }
-    const float& y() const { return y_; }
-    const float& z() const { return z_; }
-    const int& idx() const { return idx_; }
-    void setIdx(int idx) { idx_ = idx; }
-    float rt() const { return float(std::sqrt(double(x_ * x_ + y_ * y_))); }
-    float r3() const { return float(std::sqrt(double(x_ * x_ + y_ * y_ + z_ * z_))); }
-
-  private:
-    float x_, y_, z_;
-    int idx_;
-};
-
-// The three geometry singletons of the reference, recording their source file.
-class TiltedGeometry {
-  public:
-    void load(std::string filename);
-    const std::string& file() const { return file_; }
-  private:
-    std::string file_;
-};
-class EndcapGeometry {
-  public:
-    void load(std::string filename);
-    const std::string& file() const { return file_; }
-  private:
-    std::string file_;
-};
-class ModuleConnectionMap {
-  public:
-    void load(std::string filename);
-    const std::string& file() const { return file_; }
-  private:
-    std::string file_;
-};
-extern TiltedGeometry tiltedGeometry;
-extern EndcapGeometry endcapGeometry;
-extern ModuleConnectionMap moduleConnectionMap;
 
 class Event {
   public:
@@ -79,6 +53,17 @@ class Event {
     Event& operator=(const Event&) = delete;
 
     void setLogLevel(LogLevel logLevel = Log_Nothing) { logLevel_ = logLevel; }
+
+    // Event.h:197-204: modules are created as hits arrive (and a lower module's
+    // partner by the mini-doublet stages, Event.cc:262); getModule creates a
+    // module that does not exist yet, as the reference's does (Event.cc:54-78)
+    bool hasModule(unsigned int detId);
+    Module& getModule(unsigned int detId);
+    const std::vector<Module*> getModulePtrs();
+    const std::vector<Module*> getLowerModulePtrs();
+    // barrel layers 1-6 and endcap layers 1-5 (Event.cc:90-118)
+    Layer& getLayer(int layer, Layer::SubDet subdet);
+    const std::vector<Layer*> getLayerPtrs();
 
     // Event.cc:120-139 (the hit is buffered on the host until the first stage runs)
     void addHitToModule(Hit hit, unsigned int detId);
@@ -177,13 +162,32 @@ class Event {
     template <int W>
     std::vector<std::array<float, W>> variables(int stage);
     bool exported(int stage) const;
+    void buildViews();
+    void addPartnerModules();
+    Layer& layerOf(const Module& m);
 
     sdl_event* ev_ = nullptr;
     LogLevel logLevel_ = Log_Nothing;
     unsigned ran_ = 0;  // stages run since the last hit was added (bit = SDL_STAGE_*)
+    int algo_[11] = {};  // algorithm each stage last ran with
+    bool tlFromLayers_ = false;  // the stored tracklets came from createTracklets (layer pairs)
     unsigned int nHits_ = 0;
     std::array<unsigned int, 11> hitsByLayer_{};       // lower sensors
     std::array<unsigned int, 11> hitsByLayerUpper_{};  // upper sensors
+    // the reference's storage: hits, modules (by detId, and in creation order), layers
+    std::deque<Hit> hits_, hitEdges_;
+    std::unordered_map<int, Hit*> hitByIdx_;
+    std::map<unsigned int, Module> modules_;
+    std::vector<Module*> modulePtrs_, lowerModulePtrs_;
+    std::map<int, Layer> barrelLayers_, endcapLayers_;
+    std::vector<Layer*> layerPtrs_;
+    // host views of the device's objects (valid while viewsBuilt_)
+    bool viewsBuilt_ = false;
+    std::deque<MiniDoublet> miniDoublets_;
+    std::deque<Segment> segments_;
+    std::deque<Triplet> triplets_;
+    std::deque<Tracklet> tracklets_;
+    std::deque<TrackCandidate> trackCandidates_;
 };
 
 }  // namespace SDL

@@ -34,6 +34,7 @@ EXPORTS = (
     "sdl_mlp_gradient", "sdl_mlp_train", "sdl_mlp_forward", "sdl_event_set_proposal", "sdl_event_tl_pair_rows", "sdl_event_tl_work",
     "sdl_event_autotune", "sdl_event_get_tuning", "sdl_event_set_tuning",
     "sdl_geom_convert", "sdl_geom_save", "sdl_geom_load_binary", "sdl_geom_check_binary",
+    "sdl_geom_load_tables", "sdl_module_decode", "sdl_map_build",
 )
 TUNING = ("md_block", "sg_block", "tl1_block", "tl2_block", "tl_expand_block", "scored_waves")
 TIMERS = ("prep", "md", "sg", "tl")
@@ -97,6 +98,9 @@ def lib():
         L.sdl_geom_save.argtypes = [vp, ctypes.c_char_p]
         L.sdl_geom_load_binary.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sdl_geom_check_binary.argtypes = [ctypes.c_char_p, ctypes.POINTER(i), ctypes.POINTER(ll)]
+        L.sdl_geom_load_tables.argtypes = [i, vp, vp, vp, i, vp, vp, i, vp, vp, vp, ctypes.POINTER(vp)]
+        L.sdl_module_decode.argtypes = [ctypes.c_uint, vp]
+        L.sdl_map_build.argtypes = [ll, vp, vp, vp, ctypes.c_char_p, i]
         L.sdl_event_get_tuning.argtypes = [vp, vp, i]
         L.sdl_event_set_tuning.argtypes = [vp, vp, i]
         _lib = L
@@ -136,6 +140,47 @@ class Geometry:
         if self.h:
             lib().sdl_geom_free(self.h)
             self.h = ctypes.c_void_p()
+
+
+def geometry_from_tables(tilted_rows, endcap_rows, conn):
+    """A Geometry from tables in memory (sdl_geom_load_tables): tilted_rows
+    [(detId, drdz, slope)], endcap_rows [(detId, 8 floats)], conn {detId: [detIds]}
+    (in the order given)."""
+    t = np.array([r[0] for r in tilted_rows], np.uint32), np.array([r[1] for r in tilted_rows], np.float32), \
+        np.array([r[2] for r in tilted_rows], np.float32)
+    ed = np.array([r[0] for r in endcap_rows], np.uint32)
+    ec = np.array([list(r[1:]) for r in endcap_rows], np.float32).reshape(-1)
+    md = np.array(list(conn.keys()), np.uint32)
+    mc = np.array([len(v) for v in conn.values()], np.int32)
+    ml = np.array([c for v in conn.values() for c in v], np.uint32)
+    g = Geometry.__new__(Geometry)
+    g.h = ctypes.c_void_p()
+    _check(lib().sdl_geom_load_tables(len(t[0]), t[0].ctypes.data, t[1].ctypes.data, t[2].ctypes.data, len(ed),
+                                      ed.ctypes.data, ec.ctypes.data, len(md), md.ctypes.data, mc.ctypes.data,
+                                      ml.ctypes.data, ctypes.byref(g.h)))
+    return g
+
+
+MODULE_FIELDS = ("subdet", "side", "layer", "rod", "ring", "module", "isLower", "isInverted", "partnerDetId",
+                 "moduleType", "moduleLayerType")
+
+
+def decode_module(detId):
+    """The reference's detId decoding (Module.cc:159-358) as a dict."""
+    out = np.zeros(11, np.uint32)
+    _check(lib().sdl_module_decode(int(detId), out.ctypes.data))
+    return dict(zip(MODULE_FIELDS, (int(v) for v in out)))
+
+
+def build_connection_map(tracks, path, merge=False):
+    """Connection map from simulated tracks (sdl_map_build): tracks is a list of
+    [(station, detId), ...] in station order; writes the reference's map format."""
+    off = np.zeros(len(tracks) + 1, np.int64)
+    off[1:] = np.cumsum([len(t) for t in tracks])
+    st = np.array([s for t in tracks for s, _ in t], np.int32)
+    det = np.array([d for t in tracks for _, d in t], np.uint32)
+    _check(lib().sdl_map_build(len(tracks), off.ctypes.data, st.ctypes.data, det.ctypes.data, path.encode(),
+                               1 if merge else 0))
 
 
 def convert_geometry(tilted, endcap, cmap, binary):

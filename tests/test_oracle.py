@@ -430,3 +430,38 @@ def test_allcomb_accepts_every_candidate(geom_paths):
     for codes, stage in (([100], "MD"), ([0, 101], "SG"), ([0, 1, 102], "TP"), ([0, 1, 2, 203], "TL")):
         ac, _ = oracle.run(paths, ev, oracle.GLIBC, codes)
         assert {tuple(r) for r in plain[stage]} <= {tuple(r) for r in ac[stage]}, stage
+
+
+DECODE_CHECK = os.path.join(os.path.dirname(oracle.REF_DRIVER), "decode_check")
+
+
+@pytest.mark.skipif(not os.path.exists(DECODE_CHECK), reason="oracle/_ref not built (needs the reference sources)")
+def test_detid_decoding_matches_reference():
+    """Module detId decoding: the compiled reference's Module::parse* (all eleven
+    fields) against the oracle's restatement and the library's
+    sdl_module_decode over every combination of the 25 bits the decoding reads
+    (twice, the unread bits clear and hashed), plus the partner involution and
+    the lower/pixel closed forms on real sensors (oracle/decode_check.cc)."""
+    import subprocess
+    r = subprocess.run([DECODE_CHECK, "8"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stderr.strip() == f"ok {2 << 25}"
+
+
+def test_module_decode_closed_forms():
+    """Hand-decoded detIds through the library's C-ABI (host only): a barrel
+    tilted PS sensor, its partner, and an endcap 2S sensor."""
+    from sdl_amd import runtime
+    # barrel: subdet 5, layer 2, side 1 (NegZ), rod 7, module 12, bit0 = 1
+    d = (5 << 25) | (2 << 20) | (1 << 18) | (7 << 10) | (12 << 2) | 1
+    m = runtime.decode_module(d)
+    assert (m["subdet"], m["layer"], m["side"], m["rod"], m["ring"], m["module"]) == (5, 2, 1, 7, 0, 12)
+    # layer <= 2, NegZ, even module: not inverted, so bit0 = 1 is the lower (pixel) sensor
+    assert (m["isInverted"], m["isLower"], m["partnerDetId"], m["moduleType"], m["moduleLayerType"]) == (0, 1, d + 1, 0, 0)
+    p = runtime.decode_module(d + 1)
+    assert (p["isLower"], p["partnerDetId"], p["moduleLayerType"]) == (0, d, 1)
+    # endcap: subdet 4, side 2 (PosZ), layer 3, ring 9, module 5 (odd: not inverted), bit0 = 0 -> upper
+    e = (4 << 25) | (2 << 23) | (3 << 18) | (9 << 12) | (5 << 2) | 2
+    m = runtime.decode_module(e)
+    assert (m["subdet"], m["side"], m["layer"], m["ring"], m["module"], m["rod"]) == (4, 2, 3, 9, 5, 0)
+    assert (m["isInverted"], m["isLower"], m["partnerDetId"], m["moduleType"], m["moduleLayerType"]) == (0, 0, e - 1, 1, 1)
