@@ -181,6 +181,14 @@ int sdl_event_layer_counts(sdl_event* ev, int e, unsigned long long out[110]);
  * (MD 2, SG 4, TP 8, TL 8, TCP 16, TCL 16 ints per object) into out (cap ints).
  * *n receives the number of objects; returns SDL_ERR_CAPACITY if cap is short. */
 int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n);
+/* sdl_event_export without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  The tuples are gathered in
+ * a static 64 MiB device array per GPU (16777216 ints) instead of the event's
+ * growable export buffer; an export larger than that returns SDL_ERR_CAPACITY.
+ * The first call on a GPU looks the array up (hipGetSymbolAddress).  Calls
+ * on one GPU share the array: calls that overlap in time on one GPU race. */
+int sdl_event_export_noalloc(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n);
 
 /* Per-object values of event e's stored MDs, SGs, TLs or TCs from tracklets
  * (stage SDL_STAGE_MD, _SG, _TL or _TCL; also the objects of the layer-pair, navigation and pseudo-MD

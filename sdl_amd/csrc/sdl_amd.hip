@@ -25,6 +25,7 @@
 #include <iterator>
 #include <map>
 #include <sstream>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
@@ -4119,8 +4120,13 @@ int sdl_event_counts(sdl_event* ev, int e, unsigned long long out[10]) {
     return 0;
 }
 
-int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n) {
-    OnStream onStream(ev ? ev->st : nullptr);
+namespace {
+
+// The body of sdl_event_export and sdl_event_export_noalloc: gathers event e's
+// objects of `stage` as hit-index tuples into `staging` (device; null: the
+// event's own export buffer, grown as needed) and copies them to `out`.
+int exportObjects(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n, int* staging,
+                  long long stagingCap) {
     if (!ev || !n || e < 0 || e >= ev->E || stage < 0 || stage > SDL_STAGE_TCL) return fail(SDL_ERR_ARG, "bad arguments");
     // the layer-pair stages store SG, TL and TCL objects
     unsigned need = 1u << stage;
@@ -4136,11 +4142,19 @@ int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, l
     if (!out) return 0;
     if (cap < cnt * width[stage]) return fail(SDL_ERR_CAPACITY, "export buffer too small");
     if (cnt == 0) return 0;
-    if (int r = ev->exportBuf.ensure(cnt * width[stage])) return r;
+    if (staging) {
+        if (cnt * width[stage] > stagingCap)
+            return fail(SDL_ERR_CAPACITY, "sdl_event_export_noalloc: the objects (" +
+                                              std::to_string(cnt * width[stage]) + " ints) exceed its staging array (" +
+                                              std::to_string(stagingCap) + " ints); use sdl_event_export");
+    } else {
+        if (int r = ev->exportBuf.ensure(cnt * width[stage])) return r;
+        staging = ev->exportBuf.p;
+    }
     const unsigned grid = gridFor(cnt, kBlock);
     if (stage == SDL_STAGE_MD)
         hipLaunchKernelGGL(k_export_md, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, ev->mdLo.p, ev->mdUp.p, ev->pidx,
-                           ev->exportBuf.p);
+                           staging);
     else {
         const int kind = stage == SDL_STAGE_SG ? 1 : (stage == SDL_STAGE_TP || stage == SDL_STAGE_TL) ? 2
                                                                                                      : (stage == SDL_STAGE_TCP ? 3 : 4);
@@ -4149,11 +4163,50 @@ int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, l
         const int2* pC = stage == SDL_STAGE_TCP ? ev->tcp.p : ev->tcl.p;
         hipLaunchKernelGGL(k_export, dim3(grid), dim3(kBlock), 0, ev->st, a, cnt, kind, pA, pB, pC, ev->tpIn.p,
                            ev->tpOut.p, ev->tlIn.p, ev->tlOut.p, ev->sgIn.p, ev->sgOut.p, ev->mdLo.p, ev->mdUp.p,
-                           ev->pidx, ev->exportBuf.p);
+                           ev->pidx, staging);
     }
-    HIP_OK(hipMemcpyAsync(out, ev->exportBuf.p, sizeof(int) * cnt * width[stage], hipMemcpyDeviceToHost, ev->st));
+    HIP_OK(hipMemcpyAsync(out, staging, sizeof(int) * cnt * width[stage], hipMemcpyDeviceToHost, ev->st));
     HIP_OK(hipStreamSynchronize(ev->st));
     return 0;
+}
+
+// Staging of sdl_event_export_noalloc: a file-scope device array (each GPU the
+// library runs on holds its own copy until the process ends).  Host code uses
+// it only through the per-device address from hipGetSymbolAddress.
+constexpr long long kExportScratchInts = 16LL << 20;  // 64 MiB
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) int scratch[kExportScratchInts];
+std::mutex g_scratchMutex;
+int* g_scratchPtr[kMaxDevices] = {};
+
+// The address of `scratch` on `device` (current), looked up once per device.
+int scratchOn(int device, int** ptr) {
+    if (device < 0 || device >= kMaxDevices) return fail(SDL_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (r != hipSuccess || !p)
+            return fail(SDL_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(r));
+        g_scratchPtr[device] = static_cast<int*>(p);
+    }
+    *ptr = g_scratchPtr[device];
+    return 0;
+}
+
+}  // namespace
+
+int sdl_event_export(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n) {
+    OnStream onStream(ev ? ev->st : nullptr);
+    return exportObjects(ev, e, stage, out, cap, n, nullptr, 0);
+}
+
+int sdl_event_export_noalloc(sdl_event* ev, int e, int stage, int* out, long long cap, long long* n) {
+    if (!ev) return fail(SDL_ERR_ARG, "bad arguments");
+    if (hipSetDevice(ev->g->device) != hipSuccess) return fail(SDL_ERR_HIP, "hipSetDevice failed");
+    int* staging = nullptr;
+    if (int r = scratchOn(ev->g->device, &staging)) return r;
+    return exportObjects(ev, e, stage, out, cap, n, staging, kExportScratchInts);
 }
 
 int sdl_object_var_count(int stage) {

@@ -27,7 +27,7 @@ EXPORTS = (
     "sdl_last_error", "sdl_geom_load", "sdl_geom_free", "sdl_geom_num_modules", "sdl_event_create",
     "sdl_event_destroy", "sdl_event_reset", "sdl_event_add_hits", "sdl_event_next", "sdl_event_set_hits_device",
     "sdl_event_run", "sdl_event_run_chain", "sdl_event_sync", "sdl_event_counts", "sdl_event_layer_counts",
-    "sdl_event_export", "sdl_object_var_count", "sdl_object_var_names", "sdl_event_export_vars",
+    "sdl_event_export", "sdl_event_export_noalloc", "sdl_object_var_count", "sdl_object_var_names", "sdl_event_export_vars",
     "sdl_event_num_events", "sdl_event_stream", "sdl_event_set_timing", "sdl_event_kernel_ms",
     "sdl_event_set_headroom", "sdl_event_set_graphs",
     "sdl_mlp_create", "sdl_mlp_free", "sdl_mlp_num_params", "sdl_mlp_set_params", "sdl_mlp_get_params",
@@ -69,6 +69,7 @@ def lib():
         L.sdl_event_counts.argtypes = [vp, i, vp]
         L.sdl_event_layer_counts.argtypes = [vp, i, vp]
         L.sdl_event_export.argtypes = [vp, i, i, vp, ll, ctypes.POINTER(ll)]
+        L.sdl_event_export_noalloc.argtypes = [vp, i, i, vp, ll, ctypes.POINTER(ll)]
         L.sdl_object_var_count.argtypes = [i]
         L.sdl_object_var_names.restype = ctypes.c_char_p
         L.sdl_object_var_names.argtypes = [i]
