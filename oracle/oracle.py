@@ -37,6 +37,7 @@ def lib():
         L.oracle_result_count.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_result_copy.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         L.oracle_result_counters.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_result_layers.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_result_free.argtypes = [ctypes.c_void_p]
         L.oracle_var_count.restype = ctypes.c_int
         L.oracle_var_count.argtypes = [ctypes.c_int]
@@ -95,10 +96,12 @@ VAR_NAMES = {
 }
 
 
-def run(paths, ev, libm=GLIBC, stages=6, vars=False):
+def run(paths, ev, libm=GLIBC, stages=6, vars=False, layers=False):
     """Returns ({stage: int32 array [n, width]}, counters dict), and with
     vars=True also {"MD"|"SG"|"TL"|"TCL": float32 array [n, nvars]} (VAR_NAMES), the
-    per-object variables in the same object order.  stages = 1..6
+    per-object variables in the same object order; with layers=True, last, the
+    per-layer counters as an int64 array [10, 11] (counters in COUNTERS order x
+    barrel layers 1-6, endcap layers 1-5).  stages = 1..6
     runs that many module-map stages; LAYER_CHAIN runs the layer-pair chain; a
     list runs those stages in that order (0-5 module-map, 6-8 createSegments,
     createTracklets, createTrackCandidates, 9 createTrackletsViaNavigation,
@@ -122,8 +125,13 @@ def run(paths, ev, libm=GLIBC, stages=6, vars=False):
         L.oracle_result_counters(r, c.ctypes.data)
         counters = dict(zip(("nMDCand", "nSGCand", "nTPCand", "nTLCand", "nTCCand", "nMD", "nSG", "nTP", "nTL", "nTC"),
                             [int(v) for v in c]))
+        extra = ()
+        if layers:
+            lc = np.zeros(110, np.uint64)
+            L.oracle_result_layers(r, lc.ctypes.data)
+            extra = (lc.astype(np.int64).reshape(10, 11),)
         if not vars:
-            return out, counters
+            return (out, counters) + extra
         v = {}
         for name, st in (("MD", 0), ("SG", 1), ("TL", 3), ("TCL", 5)):
             nv = L.oracle_var_count(st)
@@ -131,7 +139,7 @@ def run(paths, ev, libm=GLIBC, stages=6, vars=False):
             if len(a):
                 L.oracle_result_vars(r, st, a.ctypes.data)
             v[name] = a
-        return out, counters, v
+        return (out, counters, v) + extra
     finally:
         L.oracle_result_free(r)
 

@@ -299,6 +299,20 @@ struct Event {
     std::vector<PairRec> tps, tls;
     std::vector<std::pair<int, PairRec>> tcs;  // (0 = from triplets, 1 = from tracklets)
     unsigned long long nCand[5] = {0, 0, 0, 0, 0};
+    // Per-layer counters (Event.cc:1193-1344): kinds 0-4 candidates, 5-9 stored
+    // objects (MD, SG, TP, TL, TC); bins 0-5 barrel layers 1-6, 6-10 endcap
+    // layers 1-5, of the module (or, in the layer-pair stages, the inner layer)
+    // each create* loop passes to the reference's increment functions.
+    unsigned long long layerCnt[10][11] = {};
+    void countAt(int kind, int bin) {
+        if (bin >= 0 && bin < 11) layerCnt[kind][bin]++;
+    }
+    int binOfSlot(int s) const {
+        const ModInfo& m = slots[s].info;
+        if (m.subdet == SUB_BARREL && m.layer >= 1 && m.layer <= 6) return m.layer - 1;
+        if (m.subdet == SUB_ENDCAP && m.layer >= 1 && m.layer <= 5) return 6 + m.layer - 1;
+        return -1;
+    }
     // While a stored object is re-evaluated for its per-object variables
     // (objectVars), the algorithms write them here (layouts: kMDVars, kSGVars,
     // kTLVars); null otherwise.
@@ -1205,6 +1219,7 @@ struct Event {
             for (int lo : loHits)
                 for (int up : upHits) {
                     nCand[0]++;
+                    countAt(0, binOfSlot(ls));
                     MDRec md;
                     if (acMD) {  // runMiniDoubletAllCombAlgo (MiniDoublet.cc:223-226): every candidate, values 0
                         md.lo = lo;
@@ -1217,6 +1232,7 @@ struct Event {
                     } else if (!mdAlgo(lo, up, md, nullptr)) {
                         continue;
                     }
+                    countAt(5, binOfSlot(ls));
                     slots[ls].mds.push_back(int(mds.size()));
                     mds.push_back(md);
                 }
@@ -1241,6 +1257,8 @@ struct Event {
             for (int h : outer) {
                 if (inner.empty()) break;
                 nCand[0]++;
+                countAt(0, binOfSlot(ls));
+                countAt(5, binOfSlot(ls));
                 MDRec md;
                 md.lo = loopLower ? h : inner[0];
                 md.up = loopLower ? inner[0] : h;
@@ -1263,8 +1281,10 @@ struct Event {
                     if (os < 0) continue;
                     for (int mo : slots[os].mds) {
                         nCand[1]++;
+                        countAt(1, binOfSlot(ls));
                         SGRec sg;
                         if (acSG ? allCombSG(mi, mo, sg) : sgAlgo(mi, mo, sg, nullptr)) {
+                            countAt(6, binOfSlot(ls));
                             slots[ls].sgs.push_back(int(sgs.size()));
                             sgs.push_back(sg);
                         }
@@ -1282,8 +1302,10 @@ struct Event {
                     if (os < 0) continue;
                     for (int so : slots[os].sgs) {
                         nCand[2]++;
+                        countAt(2, binOfSlot(ls));
                         // runTripletAllCombAlgo (Triplet.cc:45-48) accepts every candidate
                         if (acTP || mdMatch(sgs[si].mdOut, sgs[so].mdIn)) {
+                            countAt(7, binOfSlot(ls));
                             slots[ls].tps.push_back(int(tps.size()));
                             tps.push_back({si, so});
                         }
@@ -1301,7 +1323,9 @@ struct Event {
                     if (os < 0) continue;
                     for (int so : slots[os].sgs) {
                         nCand[3]++;
+                        countAt(3, binOfSlot(ls));
                         if (acTL || tlAlgo(sgs[si], sgs[so], nullptr)) {
+                            countAt(8, binOfSlot(ls));
                             slots[ls].tls.push_back(int(tls.size()));
                             tls.push_back({si, so, acTL});
                             tlLayer.push_back(layerOfMD(sgs[si].mdIn));  // addTrackletToEvent (Event.cc:179-193)
@@ -1322,9 +1346,11 @@ struct Event {
                     if (os < 0) continue;
                     for (int to : slots[os].tps) {
                         nCand[4]++;
+                        countAt(4, binOfSlot(ls));
                         SGRec bridge;
                         if (!sgAlgo(mdC, sgs[tps[to].in].mdIn, bridge, nullptr)) continue;
                         if (!tlAlgo(bridge, sgs[tps[to].out], nullptr)) continue;
+                        countAt(9, binOfSlot(ls));
                         tcs.push_back({0, {ti, to}});
                     }
                 }
@@ -1342,8 +1368,12 @@ struct Event {
                     if (os < 0) continue;
                     for (int lo : slots[os].tls) {
                         nCand[4]++;
+                        countAt(4, binOfSlot(ls));
                         // runTrackCandidateAllCombAlgo (TrackCandidate.cc:138-141) accepts every candidate
-                        if (acTC || sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo, acTC}});
+                        if (acTC || sgMatch(tls[li].out, tls[lo].in)) {
+                            countAt(9, binOfSlot(ls));
+                            tcs.push_back({1, {li, lo, acTC}});
+                        }
                     }
                 }
             }
@@ -1405,7 +1435,9 @@ struct Event {
             for (int si : in)
                 for (int so : out) {
                     nCand[3]++;
+                    countAt(3, pr.first);
                     if (acTL || tlAlgo(sgs[si], sgs[so], nullptr)) {
+                        countAt(8, pr.first);
                         tls.push_back({si, so, acTL});
                         tlLayer.push_back(pr.first);
                     }
@@ -1425,7 +1457,11 @@ struct Event {
         for (int li : in)
             for (int lo : out) {
                 nCand[4]++;
-                if (acTC || sgMatch(tls[li].out, tls[lo].in)) tcs.push_back({1, {li, lo, acTC}});
+                countAt(4, 0);  // the inner layer, barrel 1
+                if (acTC || sgMatch(tls[li].out, tls[lo].in)) {
+                    countAt(9, 0);
+                    tcs.push_back({1, {li, lo, acTC}});
+                }
             }
     }
     // Event::createTrackletsViaNavigation (Event.cc:696-751): inner SG s1, every
@@ -1441,7 +1477,9 @@ struct Event {
                 for (int s2 : outward[sgs[si].mdOut])
                     for (int s3 : outward[sgs[s2].mdOut]) {
                         nCand[3]++;
+                        countAt(3, binOfSlot(ls));
                         if (acTL || tlAlgo(sgs[si], sgs[s3], nullptr)) {
+                            countAt(8, binOfSlot(ls));
                             slots[ls].tls.push_back(int(tls.size()));
                             tls.push_back({si, s3, acTL});
                             tlLayer.push_back(layerOfMD(sgs[si].mdIn));
@@ -1595,6 +1633,7 @@ struct Result {
     std::vector<int> t[6];
     std::vector<float> vars[6];  // per-object variables of MDs, SGs, TLs and TCLs (index = stage)
     unsigned long long counters[10];
+    unsigned long long layers[110];  // Event::layerCnt
 };
 
 void fillEvent(Event& ev, const Geometry* g, int libm, int n, const float* x, const float* y, const float* z,
@@ -1647,6 +1686,7 @@ void* oracle_run(void* geom, int nhits, const float* x, const float* y, const fl
     r->counters[7] = ev.tps.size();
     r->counters[8] = ev.tls.size();
     r->counters[9] = ev.tcs.size();
+    std::memcpy(r->layers, ev.layerCnt, sizeof(r->layers));
     return r;
 }
 
@@ -1664,6 +1704,7 @@ void* oracle_run_sequence(void* geom, int nhits, const float* x, const float* y,
     r->counters[7] = ev.tps.size();
     r->counters[8] = ev.tls.size();
     r->counters[9] = ev.tcs.size();
+    std::memcpy(r->layers, ev.layerCnt, sizeof(r->layers));
     return r;
 }
 
@@ -1688,6 +1729,9 @@ void oracle_result_copy(void* res, int stage, int* out) {
 }
 void oracle_result_counters(void* res, unsigned long long* out10) {
     std::memcpy(out10, static_cast<Result*>(res)->counters, sizeof(unsigned long long) * 10);
+}
+void oracle_result_layers(void* res, unsigned long long* out110) {
+    std::memcpy(out110, static_cast<Result*>(res)->layers, sizeof(unsigned long long) * 110);
 }
 void oracle_result_free(void* res) { delete static_cast<Result*>(res); }
 

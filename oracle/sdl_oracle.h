@@ -83,6 +83,9 @@ int oracle_var_count(int stage);
 void oracle_result_vars(void* res, int stage, float* out);
 // out[0..4] candidates (MD, SG, TP, TL, TC), out[5..9] objects (MD, SG, TP, TL, TC)
 void oracle_result_counters(void* res, unsigned long long* out10);
+// Per-layer counters: out110[11 * k + j], k = the ten counters above, j = 0-5
+// barrel layers 1-6, 6-10 endcap layers 1-5 (Event.cc:1193-1344).
+void oracle_result_layers(void* res, unsigned long long* out110);
 void oracle_result_free(void* res);
 
 // Re-evaluates one candidate under the GLIBC policy and under policy libmB and

@@ -16,9 +16,12 @@ getters and reco variables, as float bit patterns) in
 ``tests/golden/ref_vars.npz``.
 
 and stage sequences with other algorithms (AllComb, DefaultNm1; the
-reference driver's dumpseq mode) in ``tests/golden/ref_seq_sets.npz``.
+reference driver's dumpseq mode) in ``tests/golden/ref_seq_sets.npz``; and a
+digest of a PU200-density event's module-map chain (object counts, counters,
+per-layer counters, SHA-256 of each sorted object set) in
+``tests/golden/ref_pu200_digest.json``.
 
-Usage:  python tests/golden/make_golden.py [layers|nav|pseudo|vars|nm1|seq]   (only those fixtures)
+Usage:  python tests/golden/make_golden.py [layers|nav|pseudo|vars|nm1|seq|digest]   (only those fixtures)
 """
 import os
 import subprocess
@@ -59,6 +62,8 @@ def main():
         nm1_fixtures()
     if which in ("all", "seq"):
         seq_fixtures()
+    if which in ("all", "digest"):
+        digest_fixture()
 
 
 def sorted_rows(keys, bits):
@@ -192,6 +197,39 @@ def seq_fixtures():
             out[f"{name}_counters"] = np.array([counters[k] for k in COUNTERS], np.int64)
             out[f"{name}_layers"] = counters["layers"]
     np.savez_compressed(os.path.join(HERE, "ref_seq_sets.npz"), **out)
+
+
+# PU200-density events (workload 2: 8000 tracks + 20 hard, full geometry),
+# too large to store as tuples: the module-map chain's object counts, counters,
+# per-layer counters and a SHA-256 of each sorted object set.
+DIGEST_CASES = [(2, 0)]
+DIGEST_SEQ = "md:0,sg:0,tp:0,tl:0,tcp:0,tcl:0"
+
+
+def set_digest(rows):
+    """SHA-256 of a sorted object set: its rows in lexicographic order as
+    little-endian int32, row after row."""
+    import hashlib
+    a = np.ascontiguousarray(oracle.as_multiset(np.asarray(rows, np.int32)), dtype="<i4")
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def digest_fixture():
+    import json
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for wl, e in DIGEST_CASES:
+            dump = os.path.join(tmp, f"d{wl}.txt")
+            subprocess.run([oracle.REF_DRIVER, "dumpseq", str(wl), tmp, str(e), "1", dump, DIGEST_SEQ], check=True)
+            (sets, counters), = oracle.parse_ref_dump(dump).values()
+            out[f"w{wl}_e{e}"] = {
+                "counts": {s: int(len(sets[s])) for s in oracle.STAGES},
+                "sha256": {s: set_digest(sets[s]) for s in oracle.STAGES},
+                "counters": [int(counters[k]) for k in COUNTERS],
+                "layers": counters["layers"].tolist(),
+            }
+    with open(os.path.join(HERE, "ref_pu200_digest.json"), "w") as f:
+        json.dump(out, f, indent=1)
 
 
 def pseudo_fixtures():

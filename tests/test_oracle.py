@@ -123,7 +123,8 @@ def test_oracle_layer_chain_matches_reference(wl, e, geom_paths, layer_golden):
     ten totals (the object totals are list sizes, the candidate totals sums of
     the per-layer counters, Event.cc:1080-1112)."""
     ev = gen.event(wl, e)
-    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, oracle.LAYER_CHAIN)
+    sets, counters, lay = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, oracle.LAYER_CHAIN, layers=True)
+    assert np.array_equal(lay, layer_golden[f"w{wl}_e{e}_layers"])
     for s in ("MD", "SG", "TL", "TCL"):
         ref = layer_golden[f"w{wl}_e{e}_{s}"]
         got = oracle.as_multiset(sets[s])
@@ -199,7 +200,8 @@ def test_oracle_navigation_matches_reference(wl, e, geom_paths):
     with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_nav_sets.npz")) as f:
         golden = {k: f[k] for k in f.files}
     ev = gen.event(wl, e)
-    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, [0, 1, 9])
+    sets, counters, lay = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, [0, 1, 9], layers=True)
+    assert np.array_equal(lay, golden[f"w{wl}_e{e}_layers"])
     for s in ("MD", "SG", "TL"):
         ref = golden[f"w{wl}_e{e}_{s}"]
         got = oracle.as_multiset(sets[s])
@@ -246,7 +248,9 @@ def test_oracle_pseudo_chain_matches_reference(wl, e, geom_paths):
     with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_pseudo_sets.npz")) as f:
         golden = {k: f[k] for k in f.files}
     ev = gen.event(wl, e)
-    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, [10, 1, 2, 3, 4, 5])
+    sets, counters, lay = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, [10, 1, 2, 3, 4, 5],
+                                     layers=True)
+    assert np.array_equal(lay, golden[f"w{wl}_e{e}_layers"])
     for s in oracle.STAGES:
         ref = golden[f"w{wl}_e{e}_{s}"]
         got = oracle.as_multiset(sets[s])
@@ -408,7 +412,8 @@ def test_oracle_sequences_match_reference(name, geom_paths):
     with np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_seq_sets.npz")) as f:
         golden = {k: f[k] for k in f.files if k.startswith(name + "_")}
     ev = gen.event(wl, e)
-    sets, counters = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, codes)
+    sets, counters, lay = oracle.run(geom_paths[gen.geom_kind(wl)], ev, oracle.GLIBC, codes, layers=True)
+    assert np.array_equal(lay, golden[f"{name}_layers"]), "per-layer counters"
     for s in oracle.STAGES:
         ref = golden[f"{name}_{s}"]
         got = oracle.as_multiset(sets[s])
@@ -465,3 +470,26 @@ def test_module_decode_closed_forms():
     m = runtime.decode_module(e)
     assert (m["subdet"], m["side"], m["layer"], m["ring"], m["module"], m["rod"]) == (4, 2, 3, 9, 5, 0)
     assert (m["isInverted"], m["isLower"], m["partnerDetId"], m["moduleType"], m["moduleLayerType"]) == (0, 0, e - 1, 1, 1)
+
+
+def test_oracle_matches_reference_digest_at_pu200(geom_paths):
+    """A PU200-density event (workload 2 event 0: 8000 tracks + 20 hard, 2.6 M
+    track candidates), too large for stored tuples: the GLIBC-policy oracle's
+    module-map chain against the compiled reference's digest
+    (ref_pu200_digest.json: object counts, the ten counters, the per-layer
+    counters and a SHA-256 of each sorted object set)."""
+    import importlib.util
+    import json
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(here, "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    with open(os.path.join(here, "golden", "ref_pu200_digest.json")) as f:
+        digest = json.load(f)
+    for wl, e in mg.DIGEST_CASES:
+        ref = digest[f"w{wl}_e{e}"]
+        sets, counters, lay = oracle.run(geom_paths[gen.geom_kind(wl)], gen.event(wl, e), oracle.GLIBC, 6, layers=True)
+        assert {s: len(sets[s]) for s in oracle.STAGES} == ref["counts"]
+        assert [counters[k] for k in COUNTERS] == ref["counters"]
+        assert lay.tolist() == ref["layers"]
+        assert {s: mg.set_digest(sets[s]) for s in oracle.STAGES} == ref["sha256"]
