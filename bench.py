@@ -21,7 +21,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3  # vector FP32 (packed); 78.6 without packed FP32
+PEAK_BF16_TFLOPS = 2500.0  # dense MFMA
+# frozen record sizes of the chain-level roofline (bytes): hit 16 + 16 derived,
+# MD 8 + 32, SG 8 + 48, TP 8, TL 8 + 24, TC 8
+REC = {"hit": 32, "md": 40, "sg": 56, "tp": 8, "tl": 32, "tc": 8}
+F_MLP = 2 * (24 * 64 + 64 * 64 + 64 * 64 + 64 * 1)  # FLOP per scored pair
 
 
 def parse():
@@ -46,12 +51,18 @@ def parse():
     p.add_argument("--hard-mix-batch", type=int, default=64)
     p.add_argument("--learned-batch", type=int, default=32,
                    help="events per batch of the batched plain-vs-learned leg on the main workload, 0 skips it")
+    p.add_argument("--config1-batch", type=int, default=32,
+                   help="events per batch of the plain-vs-learned leg on 500-track barrel events (workload 1, "
+                        "proposal_w1.json), 0 skips it")
+    p.add_argument("--families", type=int, default=1,
+                   help="0 skips the timing of the other stage families (Nm1, navigation, pseudo MDs, layer pairs)")
     return p.parse_args()
 
 
 def cpu_baseline(workload, threads):
     """The CPU oracle (the C++ restatement of the reference, port) timed on a
-    bounded sample: one event per thread, all threads busy at once."""
+    bounded sample: one event per thread, all threads busy at once; and one
+    event on one thread (ms/event)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from sdl_amd import gen
@@ -59,9 +70,17 @@ def cpu_baseline(workload, threads):
     evs = [gen.event(workload, 1000 + i) for i in range(n)]
     paths = gen.geometry(gen.geom_kind(workload))[1]
     wall = oracle.time_events(paths, evs, oracle.GLIBC, threads)
+    single = oracle.time_events(paths, evs[:1], oracle.GLIBC, 1)
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": n / wall, "unit": "events/s", "cores": threads, "kind": "port",
+            "single_thread_ms_per_event": round(1000.0 * single, 1), "cpu_model": cpu,
             "sample": f"{n} workload-{workload} events (seeds 1000..{1000 + n - 1}), one per thread, "
-                      f"full chain, glibc libm, {wall:.1f} s wall"}
+                      f"full chain, glibc libm, {wall:.1f} s wall; single thread: seed 1000, {single:.1f} s"}
 
 
 def tl_roofline(workload, nevents, tl_ms):
@@ -85,14 +104,59 @@ def tl_roofline(workload, nevents, tl_ms):
     tl_s = tl_ms / 1000.0
     achieved = flops / tl_s / 1e12 if flops and tl_s > 0 else None
     traffic = table.get("traffic", {}).get(f"k_tl:{workload}:{nevents}")
-    return {"bound": "mfma", "achieved": None if achieved is None else round(achieved, 4),
+    return {"bound": "valu", "achieved": None if achieved is None else round(achieved, 4),
             "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": None if achieved is None else round(achieved / PEAK_FP32_TFLOPS, 6),
             "traffic": None if traffic is None else traffic["bytes"],
             "kernel": "tracklet stage (k_tl_* launches of one chain)", "kernel_ms": round(tl_ms, 4),
-            "peak_note": "FP32 peak 157.3 TFLOP/s; the stage issues FP32/FP64 VALU and no MFMA, and is "
-                         "VALU-issue bound (profiles/chain_pu200_counters.md)",
+            "peak_note": "FP32 vector peak 157.3 TFLOP/s (78.6 without packed FP32, which this build does "
+                         "not use); the stage issues FP32/FP64 VALU and no MFMA, and is VALU-issue bound "
+                         "(profiles/chain_pu200_counters.md)",
+            "frac_nonpacked": None if achieved is None else round(achieved / 78.6, 6),
             "flops_per_launch": flops}
+
+
+def chain_roofline(workload, nevents, ms_per_step, scored=0):
+    """Bound of the whole chain: T_roof = sum over stages of max(F_s / P_FP32,
+    B_s / P_HBM) (+ F_mlp x scored pairs / P_BF16 for the learned form), with
+    F_s the algorithmic FP operations of bench_work.json and B_s each stage's
+    input records read once plus its output records written once at the frozen
+    record sizes (REC); frac = T_roof / measured time per step."""
+    path = os.path.join(ROOT, "bench_work.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    rows = [table.get("events", {}).get(f"{workload}:{e}") for e in range(nevents)]
+    if not all(rows):
+        return None
+    stages = {}
+    for r in rows:
+        F = r["flops"]
+        nh = r["hits"]
+        nmd, nsg, ntp, ntl, ntcp, ntcl = r["objects"]
+        B = {"hits": 32 * nh,
+             "md": REC["hit"] * nh + REC["md"] * nmd,
+             "sg": REC["md"] * nmd + REC["sg"] * nsg,
+             "tp": REC["sg"] * nsg + REC["tp"] * ntp,
+             "tl": REC["sg"] * nsg + REC["tl"] * ntl,
+             "tcp": REC["tp"] * ntp + REC["tc"] * ntcp,
+             "tcl": REC["tl"] * ntl + REC["tc"] * ntcl}
+        for k in B:
+            t_f = F[k] / (PEAK_FP32_TFLOPS * 1e12)
+            t_b = B[k] / (PEAK_HBM_GBS * 1e9)
+            st = stages.setdefault(k, {"flops": 0, "bytes": 0, "t_roof_us": 0.0})
+            st["flops"] += F[k]
+            st["bytes"] += B[k]
+            st["t_roof_us"] += 1e6 * max(t_f, t_b)
+    t_roof = sum(v["t_roof_us"] for v in stages.values()) + 1e6 * F_MLP * scored / (PEAK_BF16_TFLOPS * 1e12)
+    for v in stages.values():
+        v["t_roof_us"] = round(v["t_roof_us"], 3)
+        v["bound"] = "valu" if v["flops"] / (PEAK_FP32_TFLOPS * 1e12) >= v["bytes"] / (PEAK_HBM_GBS * 1e9) else "hbm"
+    return {"t_roof_us": round(t_roof, 3), "t_measured_us": round(1000.0 * ms_per_step, 3),
+            "frac": round(t_roof / (1000.0 * ms_per_step), 5), "scored_pairs": scored, "stages": stages,
+            "formula": "sum_s max(F_s/P_FP32, B_s/P_HBM) + F_mlp*scored/P_BF16; P_FP32 157.3 TFLOP/s, "
+                       "P_HBM 8 TB/s, P_BF16 2.5 PFLOP/s"}
 
 
 def learned_leg(a, ev, step, plain_counts, plain_ms, plain_work):
@@ -197,17 +261,19 @@ def batch_leg(a, geom):
     return out
 
 
-def plain_vs_learned(a, geom, workload, nevents, batch, steps):
+def plain_vs_learned(a, geom, workload, nevents, batch, steps, weights=None, recall=None):
     """Events 0 .. nevents-1 of a workload in batches of `batch` events: events/s
     of the plain chain and of the learned proposal (at the threshold chosen for
     the target TC recall) on the same batches, and the per-stage recall of the
     learned chain against the plain one (its objects are a subset of the plain
     chain's)."""
     from sdl_amd import proposal, runtime
-    m, meta = proposal.MLP.load(a.learned)
-    t = (meta.get("thresholds") or {}).get(a.recall)
+    weights = weights or a.learned
+    recall = recall or a.recall
+    m, meta = proposal.MLP.load(weights)
+    t = (meta.get("thresholds") or {}).get(recall)
     if t is None:
-        return {"skipped": f"no threshold for recall {a.recall} in {os.path.basename(a.learned)}"}
+        return {"skipped": f"no threshold for recall {recall} in {os.path.basename(weights)}"}
     ev = runtime.Event(geom)
     keys = ("nMD", "nSG", "nTP", "nTL", "nTC")
     tot = {"plain": dict.fromkeys(keys, 0), "learned": dict.fromkeys(keys, 0)}
@@ -232,9 +298,64 @@ def plain_vs_learned(a, geom, workload, nevents, batch, steps):
             "plain_events_per_s": round(nevents / sec["plain"], 3),
             "learned_events_per_s": round(nevents / sec["learned"], 3),
             "speedup_vs_plain": round(sec["plain"] / sec["learned"], 4), "threshold": t,
-            "target_tc_recall": float(a.recall), "recall_per_stage": rec,
+            "target_tc_recall": float(recall), "recall_per_stage": rec,
+            "weights": os.path.relpath(weights, ROOT),
             "objects_plain": tot["plain"], "objects_learned": tot["learned"],
             "tl_work_last_batch": work}
+
+
+def config1_leg(a, geom):
+    """500-track barrel events (workload 1), --config1-batch events per batch, 4
+    batches: plain vs the learned proposal trained on 2,000 such events
+    (proposal_w1.json; these events are neither training nor validation seeds)
+    at each threshold of its 99 / 99.5 / 99.9 % TC-recall sweep."""
+    w = os.path.join(ROOT, "sdl_amd", "data", "proposal_w1.json")
+    if not os.path.exists(w):
+        return {"skipped": "sdl_amd/data/proposal_w1.json absent"}
+    out = {"workload": f"workload 1 (500-track barrel events): {4 * a.config1_batch} events in batches of "
+                       f"{a.config1_batch}", "by_target_recall": {}}
+    for r in ("0.99", "0.995", "0.999"):
+        res = plain_vs_learned(a, geom, 1, 4 * a.config1_batch, a.config1_batch, 5, weights=w, recall=r)
+        out["by_target_recall"][r] = res
+    return out
+
+
+FAMILIES = {  # name -> [(stage, algo), ...] through sdl_event_run
+    "module_map_nm1": [(0, 0), (1, 0), (2, 0), (3, 1), (4, 0), (5, 0)],
+    "navigation": [(0, 0), (1, 0), (9, 0), (5, 0)],
+    "navigation_nm1": [(0, 0), (1, 0), (9, 1)],
+    "pseudo_md_module_map": [(10, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0)],
+    "layer_pairs": [(0, 0), (6, 0), (7, 0), (8, 0)],
+}
+
+
+def families_leg(a, geom):
+    """The other stage families of the SDL::Event API on one PU200-density event
+    (workload 2 event 0), stage by stage through sdl_event_run: ms per event
+    (median of 3 after a warmup) and the objects it stores."""
+    from sdl_amd import gen, hip, runtime
+    b = Batch([gen.event(2, 0)])
+    out = {}
+    for name, seq in FAMILIES.items():
+        ev = runtime.Event(geom)
+
+        def once():
+            ev.set_hits_device(b.off, *[d.address for d in b.dev])
+            for st, al in seq:
+                ev.run(st, al)
+            ev.sync()
+
+        once()
+        ts = []
+        for _ in range(3):
+            hip.synchronize()
+            t0 = time.perf_counter()
+            once()
+            ts.append(time.perf_counter() - t0)
+        out[name] = {"stages": seq, "ms_per_event": round(1000.0 * sorted(ts)[1], 3), "objects": ev.counts(0)}
+        del ev
+    b.free()
+    return out
 
 
 def hard_mix_leg(a, geom):
@@ -308,6 +429,7 @@ def main():
         kms[k] /= a.steps
 
     roofline = tl_roofline(a.workload, B, kms["tl"])
+    chain = chain_roofline(a.workload, B, gpu_ms / a.steps)
     plain_work = ev.tl_work()
     learned = learned_leg(a, ev, step, counts, ms_per_step, plain_work) if os.path.exists(a.learned) else None
     out = {
@@ -330,7 +452,11 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
         "objects_per_step": counts,
         "roofline": roofline,
+        "chain_roofline": chain,
     }
+    if learned is not None and "ms_per_step" in learned:
+        learned["chain_roofline"] = chain_roofline(a.workload, B, learned["ms_per_step"],
+                                                   scored=learned["tl_work_learned"]["scored"])
     if learned is not None:
         out["learned"] = learned
     if a.learned_batch > 0 and os.path.exists(a.learned):
@@ -339,6 +465,10 @@ def main():
         out["learned_batched"] = lb
     if a.batch > 0:
         out["batched"] = batch_leg(a, runtime.Geometry(*gen.geometry(gen.geom_kind(a.batch_workload))[1]))
+    if a.config1_batch > 0:
+        out["config1_learned"] = config1_leg(a, runtime.Geometry(*gen.geometry(gen.geom_kind(1))[1]))
+    if a.families:
+        out["families"] = families_leg(a, geom)
     if a.hard_mix > 0 and os.path.exists(a.learned):
         out["hard_mix"] = hard_mix_leg(a, runtime.Geometry(*gen.geometry(gen.geom_kind(5))[1]))
     if a.cpu_baseline:

@@ -109,9 +109,12 @@ int sdl_module_decode(unsigned int detId, unsigned int out[11]);
  * non-decreasing stations (one station: one layer or disk crossing); every
  * lower module of a station connects to every lower module of the next
  * station of the track (an upper sensor counts as its lower partner).  Writes
- * "detid n d1 ... dn" lines, sorted, to mapPath; with merge != 0 the lists
- * already in mapPath are kept and merged (sorted, duplicates removed), as
- * ModuleConnectionMap::add merges a second file.  Host only. */
+ * "detid n d1 ... dn" lines, sorted by detId, each list sorted without
+ * duplicates, to mapPath.  With merge != 0 mapPath is first read as
+ * ModuleConnectionMap::load reads it (a repeated detId keeps its last line) and
+ * the new lists are merged as ModuleConnectionMap::add merges a second file: a
+ * detId with new connections gets the sorted union without duplicates, every
+ * other line is written back as read.  Host only. */
 int sdl_map_build(long long nTracks, const long long* trackOffsets, const int* station, const unsigned* detId,
                   const char* mapPath, int merge);
 

@@ -3805,22 +3805,26 @@ int sdl_map_build(long long nTracks, const long long* trackOffsets, const int* s
                   const char* mapPath, int merge) {
     if (nTracks < 0 || (nTracks && (!trackOffsets || !station || !detId)) || !mapPath)
         return fail(SDL_ERR_ARG, "bad arguments");
+    // the existing file as ModuleConnectionMap::load reads it (ModuleConnectionMap.cc:18-47):
+    // one list per detId, a repeated detId keeps its last line
     std::map<uint32_t, std::vector<uint32_t>> conn;
-    if (merge) {  // ModuleConnectionMap::add (ModuleConnectionMap.cc:49-85) onto the existing file
+    if (merge) {
         std::ifstream fm(mapPath);
         std::string line;
         while (fm && std::getline(fm, line)) {
             std::stringstream ss(line);
             unsigned d; int n;
             if (!(ss >> d >> n)) continue;
-            std::vector<uint32_t>& v = conn[d];
+            std::vector<uint32_t> v;
             for (int i = 0; i < n; ++i) {
                 unsigned c;
                 if (!(ss >> c)) break;
                 v.push_back(c);
             }
+            conn[d] = std::move(v);
         }
     }
+    std::map<uint32_t, std::vector<uint32_t>> built;  // this call's connections
     for (long long t = 0; t < nTracks; ++t) {
         const long long a = trackOffsets[t], b = trackOffsets[t + 1];
         if (b < a) return fail(SDL_ERR_ARG, "track offsets must not decrease");
@@ -3832,7 +3836,7 @@ int sdl_map_build(long long nTracks, const long long* trackOffsets, const int* s
             if (k == b || (k > a && station[k] != curStation)) {
                 if (!cur.empty()) {
                     for (uint32_t p : prev)
-                        for (uint32_t c : cur) conn[p].push_back(c);
+                        for (uint32_t c : cur) built[p].push_back(c);
                     prev.swap(cur);
                     cur.clear();
                 }
@@ -3845,12 +3849,19 @@ int sdl_map_build(long long nTracks, const long long* trackOffsets, const int* s
             if (std::find(cur.begin(), cur.end(), lower) == cur.end()) cur.push_back(lower);
         }
     }
+    // merged as ModuleConnectionMap::add merges a file (ModuleConnectionMap.cc:49-85):
+    // a detId with new connections gets the sorted union without duplicates,
+    // every other list stays as read
+    for (auto& kv : built) {
+        std::vector<uint32_t>& v = conn[kv.first];
+        v.insert(v.end(), kv.second.begin(), kv.second.end());
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
     std::ofstream fo(mapPath);
     if (!fo) return fail(SDL_ERR_IO, std::string("cannot write ") + mapPath);
     for (auto& kv : conn) {
-        std::vector<uint32_t>& v = kv.second;
-        std::sort(v.begin(), v.end());
-        v.erase(std::unique(v.begin(), v.end()), v.end());
+        const std::vector<uint32_t>& v = kv.second;
         fo << kv.first << ' ' << v.size();
         for (uint32_t c : v) fo << ' ' << c;
         fo << '\n';

@@ -185,3 +185,54 @@ def test_pseudo_then_layer_pairs():
     ref, cnt = oracle.run(gen.geometry(1)[1], ev_in, oracle.CR, seq)
     _assert_same(_exported(ev, seq), ref, "pseudo + layer pairs")
     assert ev.counts(0) == cnt
+
+
+@pytest.mark.parametrize("seq", [[0, 1, runtime.TL_NAV, 5], [0, 1, 13], [runtime.MD_PSEUDO, 1, 9]])
+def test_families_at_pu200_density(seq):
+    """The navigation tracklets (default and DefaultNm1, the oracle's stage 13 =
+    createTrackletsViaNavigation(DefaultNm1_TLAlgo)) and the pseudo mini-doublets
+    on a PU200-density event (workload 2 event 0: 8000 tracks + 20 hard):
+    object sets, counters and per-layer counters against the oracle."""
+    print(f"case families at PU200 density, stages {seq}", flush=True)
+    wl, e = 2, 0
+    ev_in = gen.event(wl, e)
+    ev = runtime.Event(_geom(2))
+    ev.add_hits(ev_in.x, ev_in.y, ev_in.z, ev_in.idx, ev_in.detId)
+    for s in seq:
+        if s == 13:
+            ev.run(runtime.TL_NAV, 1)
+        else:
+            ev.run(s)
+    ev.sync()
+    ref, cnt, lay = oracle.run(gen.geometry(2)[1], ev_in, oracle.CR, seq, layers=True)
+    have = {"MD", "SG", "TL"} | ({"TCL"} if 5 in seq else set())
+    got = {s: ev.export(i, 0) if s in have else np.zeros((0, oracle.WIDTH[s]), np.int32)
+           for i, s in enumerate(oracle.STAGES)}
+    _assert_same(got, ref, f"stages {seq} at PU200 density")
+    assert ev.counts(0) == cnt
+    assert np.array_equal(ev.layer_counts(0), lay)
+    assert cnt["nTL"] > 100000
+
+
+def test_layer_chain_at_pu200_matches_oracle_digest():
+    """The layer-pair chain on a PU200-density event (workload 2 event 0; 2.5e9
+    tracklet candidates, every segment of an inner layer against every segment of
+    an outer layer) against the oracle's result precomputed on the CPU
+    (oracle_pu200_digest.json: object counts, counters, per-layer counters,
+    SHA-256 of each sorted object set; tests/golden/make_oracle_digests.py)."""
+    import importlib.util
+    import json
+    print("case layer chain at PU200 density (workload 2 event 0)", flush=True)
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(here, "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    with open(os.path.join(here, "golden", "oracle_pu200_digest.json")) as f:
+        ref = json.load(f)["layers_w2_e0"]
+    ev = _run(2, [gen.event(2, 0)], LAYER_SEQ)
+    got = _exported(ev, LAYER_SEQ)
+    assert {s: len(got[s]) for s in oracle.STAGES} == ref["counts"]
+    c = ev.counts(0)
+    assert [c[k] for k in runtime.COUNTER_NAMES] == ref["counters"]
+    assert ev.layer_counts(0).tolist() == ref["layers"]
+    assert {s: mg.set_digest(got[s]) for s in oracle.STAGES} == ref["sha256"]

@@ -266,7 +266,8 @@ def test_autotune_keeps_objects():
 
 def test_binary_geometry_cache(tmp_path):
     """A geometry loaded from the binary cache gives the same objects as the
-    text files; a loaded geometry saves the same cache bytes as the converter."""
+    text files and as the oracle (which reads the text files); a loaded
+    geometry saves the same cache bytes as the converter."""
     print("case binary geometry cache, workload 15 event 0 and workload 2 event 1", flush=True)
     paths = gen.geometry(2)[1]
     conv = str(tmp_path / "conv.bin")
@@ -286,4 +287,8 @@ def test_binary_geometry_cache(tmp_path):
             ev.run_chain()
             ev.sync()
             out.append(ev.export_all(0))
-        _assert_same(out[1], out[0], f"binary geometry, workload {wl} event {e}")
+            counts = ev.counts(0)
+        ref, cnt = oracle.run(paths, ev_in, oracle.CR)
+        _assert_same(out[1], ref, f"binary geometry vs oracle, workload {wl} event {e}")
+        _assert_same(out[0], ref, f"text geometry vs oracle, workload {wl} event {e}")
+        assert counts == cnt
