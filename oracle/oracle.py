@@ -44,6 +44,15 @@ def lib():
         L.oracle_result_vars.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         L.oracle_explain.restype = ctypes.c_int
         L.oracle_explain.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] + [ctypes.c_void_p] * 2 + [ctypes.c_int, ctypes.c_void_p]
+        L.oracle_module_decode.argtypes = [ctypes.c_uint, ctypes.c_void_p]
+        L.oracle_kat_phi_mpi_pi.restype = ctypes.c_float
+        L.oracle_kat_phi_mpi_pi.argtypes = [ctypes.c_float]
+        L.oracle_kat_atan2.restype = ctypes.c_float
+        L.oracle_kat_atan2.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.oracle_kat_center.argtypes = [ctypes.c_void_p] * 3
+        L.oracle_kat_module_gap.restype = ctypes.c_float
+        L.oracle_kat_module_gap.argtypes = [ctypes.c_uint, ctypes.c_void_p]
+        L.oracle_kat_shift.argtypes = [ctypes.c_void_p, ctypes.c_uint] + [ctypes.c_void_p] * 3
         L.oracle_time.restype = ctypes.c_double
         L.oracle_time.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_int, ctypes.c_int]
         _lib = L
@@ -157,6 +166,24 @@ def explain(paths, ev, kind, a, b=None, policy=None):
                      out.ctypes.data)
     keys = ("cut_glibc", "cut_cr", "var", "lo", "hi", "var_cr", "lo_cr", "hi_cr", "ulps")
     return dict(zip(keys, out.tolist()))
+
+
+def kat_center(xs, ys):
+    """Centre of MathUtil::getCenterFromThreePoints through (xs[k], ys[k])."""
+    x = np.asarray(xs, np.float32)
+    y = np.asarray(ys, np.float32)
+    out = np.zeros(2, np.float32)
+    lib().oracle_kat_center(x.ctypes.data, y.ctypes.data, out.ctypes.data)
+    return out
+
+
+def kat_shift(paths, lower_det, lo, up):
+    """MiniDoublet::shiftStripHits of hit lo on module lower_det and hit up on its partner."""
+    a = np.asarray(lo, np.float32)
+    b = np.asarray(up, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().oracle_kat_shift(geometry(paths), lower_det, a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out
 
 
 def time_events(paths, events, libm=GLIBC, threads=1):

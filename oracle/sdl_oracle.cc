@@ -766,7 +766,7 @@ struct Event {
     // circle through A, B, C from the perpendicular bisectors of AB and BC; the
     // origin (a default Hit) when the two slopes are equal.  Returns the distance
     // from A to it, as TrackCandidate.cc:196-197 computes it.
-    static float radiusFromA(const P3& A, const P3& B, const P3& C) {
+    static void centerFromThree(const P3& A, const P3& B, const P3& C, float& cx, float& cy) {
         const float xA = A.x, yA = A.y, xB = B.x, yB = B.y, xC = C.x, yC = C.y;
         const float xMidAB = float(double(xA + xB) / 2.), yMidAB = float(double(yA + yB) / 2.);
         const bool infAB = (xB - xA) == 0, infBC = (xC - xB) == 0;
@@ -774,13 +774,18 @@ struct Event {
         const float sAB = infAB ? 0.f : (yB - yA) / (xB - xA);
         const float sBC = infBC ? 0.f : (yC - yB) / (xC - xB);
         const float perpAB = (infAB || zeroAB) ? 0.f : float(-1. / double(sAB));
-        float cx = 0.f, cy = 0.f;
+        cx = 0.f;
+        cy = 0.f;
         if (!((sAB - sBC) == 0)) {
             const float num = sAB * sBC * (yA - yC) + sBC * (xA + xB) - sAB * (xB + xC);
             cx = float(double(num) / (2. * double(sBC - sAB)));
             cy = perpAB * (cx - xMidAB) + yMidAB;
         }
-        return float(std::sqrt(sq(double(xA - cx)) + sq(double(yA - cy))));
+    }
+    static float radiusFromA(const P3& A, const P3& B, const P3& C) {
+        float cx, cy;
+        centerFromThree(A, B, C, cx, cy);
+        return float(std::sqrt(sq(double(A.x - cx)) + sq(double(A.y - cy))));
     }
 
     // TrackCandidate::runTrackCandidateDefaultAlgo's recorded values (TrackCandidate.cc:183-207):
@@ -1652,6 +1657,33 @@ void fillEvent(Event& ev, const Geometry* g, int libm, int n, const float* x, co
 using namespace oracle;
 
 extern "C" {
+
+// Known-answer hooks of single functions of the restatement (closed-form and
+// compiled-reference checks of the tests; oracle/kat_check.cc).
+// MathUtil::Phi_mpi_pi, MathUtil::ATan2 (MathUtil.cc:3-27)
+float oracle_kat_phi_mpi_pi(float x) { return phiMpiPi(x); }
+float oracle_kat_atan2(float y, float x) { return atan2Sdl(y, x); }
+// MathUtil::getCenterFromThreePoints (MathUtil.cc:44-106) on the points (x[k], y[k]), k = 0..2
+void oracle_kat_center(const float* x, const float* y, float* out2) {
+    Event::centerFromThree(P3{x[0], y[0], 0.f}, P3{x[1], y[1], 0.f}, P3{x[2], y[2], 0.f}, out2[0], out2[1]);
+}
+// MiniDoublet::moduleGapSize and isTighterTiltedModules (MiniDoublet.cc:1242-1332) of a module
+float oracle_kat_module_gap(unsigned detId, int* tighter) {
+    const ModInfo m = decode(detId);
+    if (tighter) *tighter = isTighterTilted(m) ? 1 : 0;
+    return moduleGapSize(m);
+}
+// MiniDoublet::shiftStripHits (MiniDoublet.cc:989-1209) of a lower-module hit
+// and an upper-module hit with the given geometry (glibc libm): out = x, y, z
+void oracle_kat_shift(void* geom, unsigned lowerDetId, const float* lo, const float* up, float* out3) {
+    Event ev;
+    ev.g = static_cast<Geometry*>(geom);
+    ev.lm.mode = ORACLE_LIBM_GLIBC;
+    const ModInfo m = decode(lowerDetId);
+    ev.addHit(lo[0], lo[1], lo[2], 0, lowerDetId);
+    ev.addHit(up[0], up[1], up[2], 1, m.partner);
+    ev.shiftStripHits(ev.hits[0], ev.hits[1], ev.slots[ev.hits[0].slot], out3[0], out3[1], out3[2]);
+}
 
 // The restated detId decoding, fields in the order subdet, side, layer, rod,
 // ring, module, isLower, isInverted, partnerDetId, moduleType, moduleLayerType.

@@ -54,6 +54,20 @@ enum { kMDVars = 7, kSGVars = 22, kTLTyped = 6, kTLKeys = 108, kTLVars = kTLType
 // moduleLayerType (0 Pixel, 1 Strip).
 void oracle_module_decode(unsigned detId, unsigned out[11]);
 
+// Single functions of the restatement, for known-answer checks:
+// MathUtil::Phi_mpi_pi and ATan2 (MathUtil.cc:3-27); the centre (out2 = x, y)
+// of MathUtil::getCenterFromThreePoints (MathUtil.cc:44-106) through
+// (x[k], y[k]); MiniDoublet::moduleGapSize (returned) and
+// isTighterTiltedModules (*tighter) of a module (MiniDoublet.cc:1242-1332);
+// MiniDoublet::shiftStripHits (MiniDoublet.cc:989-1209) of a hit lo (x, y, z)
+// on lower module lowerDetId and a hit up on its partner, with the geometry
+// of oracle_geom_load and glibc's libm (out3 = shifted x, y, z).
+float oracle_kat_phi_mpi_pi(float x);
+float oracle_kat_atan2(float y, float x);
+void oracle_kat_center(const float* x, const float* y, float* out2);
+float oracle_kat_module_gap(unsigned detId, int* tighter);
+void oracle_kat_shift(void* geom, unsigned lowerDetId, const float* lo, const float* up, float* out3);
+
 // Geometry in the reference's three text formats (TiltedGeometry.cc:36,
 // EndcapGeometry.cc:48, ModuleConnectionMap.cc:36-44).  Returns null on error.
 void* oracle_geom_load(const char* tiltedPath, const char* endcapPath, const char* mapPath);

@@ -6,6 +6,8 @@
   float functions against the host and device libm (test support).
 * ``sdl_amd/_lib/libsdl_shim.so`` and ``sdl_amd/_lib/sdl_shim_demo`` -- the C++
   ``SDL::Event`` shim over the C-ABI and a small driver that uses it.
+* ``sdl_amd/_lib/sdl_host_asan`` -- the host-only entry points under
+  AddressSanitizer and UndefinedBehaviorSanitizer (test support).
 """
 import os
 import subprocess
@@ -83,11 +85,42 @@ def build_check():
     return out
 
 
+# host-code sanitizers for the test-support driver (the device code is built as usual)
+HOST_SAN = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer"]
+CLANG = os.environ.get("SDL_CLANG", "/opt/rocm/llvm/bin/clang++")
+
+
+def build_host_asan():
+    """Test support: sdl_host_asan, the host-only C-ABI entry points (geometry
+    parsing and cache, detId decoding, map builder) in a process whose host
+    code carries AddressSanitizer and UndefinedBehaviorSanitizer."""
+    os.makedirs(LIB, exist_ok=True)
+    tmp = os.path.join(LIB, "_asan_obj")
+    os.makedirs(tmp, exist_ok=True)
+    lib_o = os.path.join(tmp, "sdl_amd.o")
+    _run([HIPCC, *[f for f in HIP_FLAGS if f not in ("-O3", "-fPIC")], "-O1", "-g", *HOST_SAN, "-c",
+          os.path.join(CSRC, "sdl_amd.hip"), "-o", lib_o])
+    host = ["-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+            "-fno-sanitize-recover=all"]
+    objs = [lib_o]
+    for src, name in ((os.path.join(CSRC, "check", "host_asan.cc"), "host_asan.o"),
+                      (os.path.join(CSRC, "gen", "sdl_gen.cc"), "sdl_gen.o")):
+        o = os.path.join(tmp, name)
+        _run([CLANG, *host, "-c", src, "-o", o])
+        objs.append(o)
+    out = os.path.join(LIB, "sdl_host_asan")
+    _run([CLANG, "-fsanitize=address,undefined", *objs, "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib",
+          "-o", out])
+    return out
+
+
 def build_all(with_shim=True):
     build_gen()
     build_hip()
     build_calib()
     build_check()
+    build_host_asan()
     if with_shim and os.path.exists(os.path.join(CSRC, "shim", "sdl_shim.cc")):
         build_shim()
 

@@ -493,3 +493,126 @@ def test_oracle_matches_reference_digest_at_pu200(geom_paths):
         assert [counters[k] for k in COUNTERS] == ref["counters"]
         assert lay.tolist() == ref["layers"]
         assert {s: mg.set_digest(sets[s]) for s in oracle.STAGES} == ref["sha256"]
+
+
+KAT_CHECK = os.path.join(os.path.dirname(oracle.REF_DRIVER), "kat_check")
+
+
+def _tilted_with_flat_strips(geom_paths, tmp_path):
+    """The workload-2 geometry with the strip slopes of 40 tilted modules set to
+    0 (no such module in the generated detector) and of 40 others to SDL_INF:
+    the two special branches of MiniDoublet::shiftStripHits (MiniDoublet.cc:1145-1159)."""
+    tilted, endcap, cmap = geom_paths[gen.geom_kind(2)]
+    rows = [l.split() for l in open(tilted) if l.strip()]
+    tilt = [r for r in rows if float(r[1]) != 0.0]
+    zero = {r[0] for r in tilt[:80]}
+    inf = {r[0] for r in tilt[80:160]}
+    out = tmp_path / "tilted_special.txt"
+    with open(out, "w") as f:
+        for r in rows:
+            s = "0" if r[0] in zero else ("123456789" if r[0] in inf else r[2])
+            f.write(f"{r[0]} {r[1]} {s}\n")
+    return str(out), endcap, cmap
+
+
+@pytest.mark.skipif(not os.path.exists(KAT_CHECK), reason="oracle/_ref not built (needs the reference sources)")
+def test_single_functions_match_reference(geom_paths, tmp_path):
+    """Single functions of the compiled reference against the oracle's, value
+    for value (oracle/kat_check.cc): Phi_mpi_pi on about 9 M floats of
+    [-1000, 1000], ATan2 on a signed grid with zeros, getCenterFromThreePoints
+    on random and degenerate triples, moduleGapSize and isTighterTiltedModules
+    on every barrel and endcap sensor, and shiftStripHits on random hit pairs
+    of every tilted and endcap lower module -- once with the generated
+    detector, once with strip slopes set to 0 and SDL_INF."""
+    import subprocess
+    for files in (geom_paths[gen.geom_kind(2)], _tilted_with_flat_strips(geom_paths, tmp_path)):
+        r = subprocess.run([KAT_CHECK, *files], capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        last = r.stderr.strip().splitlines()[-1]
+        assert last.startswith("ok ") and int(last.split()[1]) > 10_000_000, last
+
+
+def test_phi_and_atan2_closed_forms():
+    """Phi_mpi_pi (MathUtil.cc:3-19) and ATan2 (MathUtil.cc:21-27) against what
+    the mathematics fixes: the result lies in [-pi, pi) and differs from the
+    argument by a whole number of turns; the float nearest pi (above pi) maps
+    below -pi's neighbour and the float nearest -pi (below -pi) maps to the top
+    of the range; ATan2's axis cases and octant values."""
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    for x in np.concatenate([rng.uniform(-1000, 1000, 20000), rng.uniform(-7, 7, 20000)]).astype(np.float32):
+        r = L.oracle_kat_phi_mpi_pi(float(x))
+        assert -np.pi <= r < np.pi, (x, r)
+        turns = (float(x) - r) / (2 * np.pi)
+        # each of the whole turns is subtracted in double and rounded to float
+        bound = (abs(round(turns)) + 1) * float(np.spacing(np.float32(abs(x) + 7)))
+        assert abs(turns - round(turns)) * 2 * np.pi <= bound, (x, r)
+    pi32 = float(np.float32(np.pi))  # 3.14159274 > pi
+    assert L.oracle_kat_phi_mpi_pi(pi32) == pytest.approx(pi32 - 2 * np.pi, abs=1e-6)
+    assert L.oracle_kat_phi_mpi_pi(-pi32) == pytest.approx(-pi32 + 2 * np.pi, abs=1e-6)
+    assert L.oracle_kat_phi_mpi_pi(0.0) == 0.0 and np.isnan(L.oracle_kat_phi_mpi_pi(float("nan")))
+    half = float(np.float32(np.pi / 2))
+    assert L.oracle_kat_atan2(0.0, 0.0) == 0.0
+    assert L.oracle_kat_atan2(2.5, 0.0) == half and L.oracle_kat_atan2(-2.5, 0.0) == -half
+    assert L.oracle_kat_atan2(0.0, -1.0) == pi32
+    assert L.oracle_kat_atan2(1.0, 1.0) == float(np.float32(np.pi / 4))
+    assert L.oracle_kat_atan2(-1.0, -1.0) == float(np.float32(-3 * np.pi / 4))
+
+
+def test_circle_centre_closed_forms():
+    """getCenterFromThreePoints (MathUtil.cc:44-106): three points of a circle
+    give its centre; points on one line (equal slopes) give the origin, the
+    reference's default Hit."""
+    rng = np.random.default_rng(5)
+    for _ in range(2000):
+        cx, cy = rng.uniform(-50, 50, 2)
+        R = rng.uniform(20, 200)
+        a = np.sort(rng.uniform(0, 2 * np.pi, 3))
+        if np.min(np.diff(np.concatenate([a, [a[0] + 2 * np.pi]]))) < 0.3:
+            continue
+        xs, ys = cx + R * np.cos(a), cy + R * np.sin(a)
+        if np.min(np.abs(np.diff(np.concatenate([xs, xs[:1]])))) < 1e-2:
+            continue  # a vertical chord: the reference sets its slope to 0
+        c = oracle.kat_center(xs, ys)
+        assert np.hypot(c[0] - cx, c[1] - cy) < 2e-3 * R, (cx, cy, R, c)
+    assert oracle.kat_center([0, 1, 2], [0, 2, 4]).tolist() == [0.0, 0.0]
+    assert oracle.kat_center([5, 6, 8], [1, 3, 7]).tolist() == [0.0, 0.0]
+
+
+def test_strip_shift_closed_forms(geom_paths, tmp_path):
+    """shiftStripHits (MiniDoublet.cc:989-1209): the shifted strip hit is the
+    foot of the perpendicular from the strip hit onto the strip line through
+    the anchored pixel position -- with slope SDL_INF it keeps the strip hit's y,
+    with slope 0 its x -- and keeps the sign of the pixel hit's z."""
+    files = _tilted_with_flat_strips(geom_paths, tmp_path)
+    slope = {int(r.split()[0]): float(r.split()[2]) for r in open(files[0]) if r.strip()}
+    L = oracle.lib()
+    rng = np.random.default_rng(9)
+    seen = {"inf": 0, "zero": 0, "other": 0}
+    for d in list(slope)[:4000]:
+        m = np.zeros(11, np.uint32)
+        L.oracle_module_decode(d, m.ctypes.data)
+        subdet, side, is_lower, partner, mtype, ltype = int(m[0]), int(m[1]), int(m[6]), int(m[8]), int(m[9]), int(m[10])
+        if subdet != 5 or side == 3 or not is_lower:
+            continue
+        strip_is_lower = mtype == 1 or ltype == 1
+        strip_id = d if strip_is_lower else partner
+        s = slope.get(strip_id, 0.0)
+        phi, r, z = rng.uniform(-np.pi, np.pi), rng.uniform(20, 60), rng.uniform(-100, 100)
+        lo = np.array([r * np.cos(phi), r * np.sin(phi), z], np.float32)
+        up = (lo + rng.uniform(-0.3, 0.3, 3)).astype(np.float32)
+        out = oracle.kat_shift(files, d, lo, up)
+        strip, pix = (lo, up) if strip_is_lower else (up, lo)
+        if s == np.float32(123456789):
+            assert out[1] == strip[1], (d, out, strip)
+            seen["inf"] += 1
+        elif s == 0:
+            assert out[0] == strip[0], (d, out, strip)
+            seen["zero"] += 1
+        else:
+            # (out - strip) is perpendicular to the strip direction (1, s)
+            dx, dy = float(out[0] - strip[0]), float(out[1] - strip[1])
+            assert abs(dx + s * dy) <= 1e-4 * (1 + abs(s)) * (abs(dx) + abs(dy) + 1), (d, s, out, strip)
+            seen["other"] += 1
+        assert np.sign(out[2]) == np.sign(pix[2])
+    assert min(seen.values()) > 10, seen
