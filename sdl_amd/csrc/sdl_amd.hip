@@ -1947,6 +1947,11 @@ __global__ void k_tcp_count(Count cTL, const int* tlIn, const int* tlOutArr, con
 // from triplets are counted where they are written).  Bins are pre-aggregated
 // in LDS, one global atomic per touched bin and block.
 constexpr int kMaxLdsBins = 4096;
+// The per-layer counters are kept in kCounterCopies copies, each block adding
+// into copy (block index mod kCounterCopies) and the readers summing them:
+// device-scope atomics on one address serialise (about 10 ns each), so a
+// single copy makes the per-block flushes of a kernel queue behind each other.
+constexpr int kCounterCopies = 16;
 
 struct CounterIn {
     int NS, NM, E, stages;
@@ -2025,6 +2030,7 @@ __device__ __forceinline__ void waveBinAdd(unsigned long long* bins, int key, lo
 __global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long long* tot, unsigned long long* counters) {
     extern __shared__ unsigned long long ldsBins[];  // nb entries when they fit kMaxLdsBins (else none)
     const int nb = C.E * kNCounters * kNBins;
+    unsigned long long* copy = counters + size_t(blockIdx.x % kCounterCopies) * nb;
     const bool lds = nb <= kMaxLdsBins;
     if (lds)
         for (int b = threadIdx.x; b < nb; b += blockDim.x) ldsBins[b] = 0;
@@ -2075,19 +2081,15 @@ __global__ void __launch_bounds__(kBlock) k_counters(CounterIn C, const long lon
             }
         }
         const int key0 = (key / kNBins) * kNCounters * kNBins + key % kNBins;
-        if (lds) {  // LDS atomics per lane
+        // one atomic per distinct bin and wave (lanes of a wave mostly share a bin)
+        unsigned long long* dst = lds ? ldsBins : copy;
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if (v[c]) atomicAdd(&ldsBins[key0 + w[c] * kNBins], (unsigned long long)v[c]);
-        } else {  // global atomics: one per distinct bin and wave
-#pragma unroll
-            for (int c = 0; c < 4; ++c) waveBinAdd(counters, key0 + w[c] * kNBins, v[c]);
-        }
+        for (int c = 0; c < 4; ++c) waveBinAdd(dst, key0 + w[c] * kNBins, v[c]);
     }
     __syncthreads();
     if (lds)
         for (int b = threadIdx.x; b < nb; b += blockDim.x)
-            if (ldsBins[b]) atomicAdd(&counters[b], ldsBins[b]);
+            if (ldsBins[b]) atomicAdd(&copy[b], ldsBins[b]);
 }
 
 // Adds 1 to bins[key] for every active lane: one LDS atomic per distinct key of
@@ -2118,6 +2120,7 @@ __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long
     const int nTL = int(cTL.get());
     extern __shared__ unsigned long long bins[];  // nb entries when they fit kMaxLdsBins (else none)
     const int nb = E * kNBins;
+    counters += size_t(blockIdx.x % kCounterCopies) * E * kNCounters * kNBins;
     const bool lds = nb <= kMaxLdsBins;
     if (lds)
         for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0;
@@ -2644,7 +2647,7 @@ int ensureHits(sdl_event* ev) {
     if (int r = ev->slotHitCount.ensure(NS)) return r;
     if (int r = ev->slotFill.ensure(NS)) return r;
     if (int r = ev->slotHitStart.ensure(NS + 1)) return r;
-    if (int r = ev->counters.ensure(size_t(ev->E) * kNCounters * kNBins)) return r;
+    if (int r = ev->counters.ensure(size_t(kCounterCopies) * ev->E * kNCounters * kNBins)) return r;
     return 0;
 }
 
@@ -2736,7 +2739,7 @@ int runPrep(sdl_event* ev) {
     ev->ranMask = 0;
     if (int r = ensureHits(ev)) return r;
     zeroMany(st, {span(ev->tot.p, TOT_N), span(ev->slotHitCount.p, NS), span(ev->slotFill.p, NS),
-                  span(ev->counters.p, (long long)ev->E * kNCounters * kNBins)});
+                  span(ev->counters.p, (long long)kCounterCopies * ev->E * kNCounters * kNBins)});
     // launches cover the hit buffers' capacity; the kernels read the hit count
     const long long nhGrid = (long long)ev->rt.cap;
     timerBegin(ev, T_PREP);
@@ -3753,6 +3756,19 @@ int readGeomBinary(const char* path, sdl_geom& G) {
     return 0;
 }
 
+// The per-layer counters of all events, summed over the counter copies.
+int readCounters(sdl_event* ev, std::vector<unsigned long long>& all) {
+    const size_t per = size_t(ev->E) * kNCounters * kNBins;
+    std::vector<unsigned long long> copies(per * kCounterCopies);
+    HIP_OK(hipMemcpyAsync(copies.data(), ev->counters.p, sizeof(unsigned long long) * copies.size(),
+                          hipMemcpyDeviceToHost, ev->st));
+    HIP_OK(hipStreamSynchronize(ev->st));
+    all.assign(per, 0ULL);
+    for (int c = 0; c < kCounterCopies; ++c)
+        for (size_t i = 0; i < per; ++i) all[i] += copies[size_t(c) * per + i];
+    return 0;
+}
+
 }  // namespace
 
 // ====================================================================== C-ABI
@@ -4076,18 +4092,17 @@ int sdl_event_sync(sdl_event* ev) {
 int sdl_event_layer_counts(sdl_event* ev, int e, unsigned long long out[110]) {
     OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || !out || e < 0 || e >= ev->E) return fail(SDL_ERR_ARG, "bad arguments");
-    HIP_OK(hipMemcpyAsync(out, ev->counters.p + size_t(e) * kNCounters * kNBins,
-                          sizeof(unsigned long long) * kNCounters * kNBins, hipMemcpyDeviceToHost, ev->st));
-    HIP_OK(hipStreamSynchronize(ev->st));
+    std::vector<unsigned long long> all;
+    if (int r = readCounters(ev, all)) return r;
+    std::memcpy(out, all.data() + size_t(e) * kNCounters * kNBins, sizeof(unsigned long long) * kNCounters * kNBins);
     return 0;
 }
 
 int sdl_event_counts(sdl_event* ev, int e, unsigned long long out[10]) {
     OnStream onStream(ev ? ev->st : nullptr);
     if (!ev || !out || e < -1 || e >= ev->E) return fail(SDL_ERR_ARG, "bad arguments");
-    std::vector<unsigned long long> all(size_t(ev->E) * kNCounters * kNBins);
-    HIP_OK(hipMemcpyAsync(all.data(), ev->counters.p, sizeof(unsigned long long) * all.size(), hipMemcpyDeviceToHost, ev->st));
-    HIP_OK(hipStreamSynchronize(ev->st));
+    std::vector<unsigned long long> all;
+    if (int r = readCounters(ev, all)) return r;
     for (int k = 0; k < kNCounters; ++k) out[k] = 0;
     for (int ee = 0; ee < ev->E; ++ee) {
         if (e >= 0 && ee != e) continue;

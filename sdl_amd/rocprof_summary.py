@@ -5,7 +5,7 @@ counters).
 
     python -m sdl_amd.rocprof_summary --trace DIR/kt_kernel_trace.csv \
         [--fetch F.csv --write W.csv --calib-fetch CF.csv --calib-write CW.csv --calib-bytes N] \
-        [--counters C.csv ...]
+        [--counters C.csv ...] [--kernel-counters K.csv ...]
 
 A chain is the run of kernels that ends with k_counters; its stages start at
 a marker kernel of each stage (the order of sdl_amd/csrc/sdl_amd.hip's
@@ -111,6 +111,54 @@ def calib(path, counter, nbytes):
     return nbytes / (kib * 1024.0), kib
 
 
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies its SIMD (16
+# lanes) for 4 cycles at up to 2.4 GHz.
+N_SIMD = 1024
+CLOCK_HZ = 2.4e9
+VALU_CYCLES = 4
+
+
+def kernel_table(trace_path, counter_paths, top=12):
+    """Per-kernel counters of the plain chains (mean per launch) with the
+    kernel's duration from the trace: VALU instructions per wave and the VALU
+    issue utilisation, SQ_INSTS_VALU x 4 cycles / (duration x 2.4 GHz x 1024
+    SIMDs), the fraction of the chip's VALU issue slots the kernel filled."""
+    dur = collections.defaultdict(list)
+    for learned, c in chains_of(list(csv.DictReader(open(trace_path))), lambda r: int(r["Start_Timestamp"])):
+        if learned:
+            continue
+        for s, r in c:
+            dur[short(r["Kernel_Name"])].append(dur_us(r))
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in counter_paths:
+        rows = counter_rows(path)
+        for learned, c in chains_of(rows, lambda r: int(r["Dispatch_Id"])):
+            if learned:
+                continue
+            per = collections.defaultdict(lambda: collections.defaultdict(float))
+            for s, r in c:
+                per[(short(r["Kernel_Name"]), r["Dispatch_Id"])][r["Counter_Name"]] += float(r["Counter_Value"])
+            for (k, _), vals in per.items():
+                for n, v in vals.items():
+                    acc[k][n].append(v)
+    mean = lambda v: sum(v) / len(v) if v else 0.0
+    names = sorted({n for k in acc for n in acc[k]} - {"SQ_WAVES", "SQ_INSTS_VALU"})
+    kern = sorted((k for k in acc if k in dur), key=lambda k: -mean(dur[k]))[:top]
+    lines = ["### per kernel, plain chain (mean per launch; duration from the kernel trace)\n",
+             "| kernel | us | waves | VALU / wave | VALU issue utilisation | " + " | ".join(
+                 n.replace("SQ_INSTS_VALU_", "").replace("SQ_INSTS_", "") + " / wave" for n in names) + " |",
+             "|---|---|---|---|---|" + "---|" * len(names)]
+    for k in kern:
+        d = sorted(dur[k])[len(dur[k]) // 2]
+        w = mean(acc[k]["SQ_WAVES"])
+        valu = mean(acc[k]["SQ_INSTS_VALU"])
+        util = valu * VALU_CYCLES / (d * 1e-6 * CLOCK_HZ * N_SIMD) if d > 0 else 0.0
+        extra = " | ".join(f"{mean(acc[k][n]) / w:.1f}" if w else "-" for n in names)
+        lines.append(f"| `{k}` | {d:.1f} | {w:.0f} | {valu / w if w else 0:.1f} | {util:.3f} | {extra} |")
+    lines.append("")
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace")
@@ -120,6 +168,8 @@ def main():
     ap.add_argument("--calib-write")
     ap.add_argument("--calib-bytes", type=float, default=268435456.0)
     ap.add_argument("--counters", nargs="*", default=[])
+    ap.add_argument("--kernel-counters", nargs="*", default=[],
+                    help="counter CSVs (each with SQ_WAVES and SQ_INSTS_VALU) for the per-kernel table (needs --trace)")
     a = ap.parse_args()
     lines = []
     if a.trace:
@@ -158,6 +208,8 @@ def main():
                     vals = " | ".join(f"{acc[s][n]:.4g}" for n in names)
                     lines.append(f"| {'learned' if learned else 'plain'} | {s} | {vals} |")
         lines.append("")
+    if a.kernel_counters and a.trace:
+        lines += kernel_table(a.trace, a.kernel_counters)
     print("\n".join(lines))
 
 
