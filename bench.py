@@ -261,12 +261,13 @@ def batch_leg(a, geom):
     return out
 
 
-def plain_vs_learned(a, geom, workload, nevents, batch, steps, weights=None, recall=None):
-    """Events 0 .. nevents-1 of a workload in batches of `batch` events: events/s
-    of the plain chain and of the learned proposal (at the threshold chosen for
-    the target TC recall) on the same batches, and the per-stage recall of the
-    learned chain against the plain one (its objects are a subset of the plain
-    chain's)."""
+def plain_vs_learned(a, geom, workload, nevents, batch, steps, weights=None, recall=None, first_seed=0):
+    """Events first_seed .. first_seed + nevents - 1 of a workload (none of them a
+    training or validation seed of the weights) in batches of `batch` events:
+    events/s of the plain chain and of the learned proposal (at the threshold
+    chosen for the target TC recall) on the same batches, and the per-stage
+    recall of the learned chain against the plain one (its objects are a subset
+    of the plain chain's)."""
     from sdl_amd import proposal, runtime
     weights = weights or a.learned
     recall = recall or a.recall
@@ -274,13 +275,19 @@ def plain_vs_learned(a, geom, workload, nevents, batch, steps, weights=None, rec
     t = (meta.get("thresholds") or {}).get(recall)
     if t is None:
         return {"skipped": f"no threshold for recall {recall} in {os.path.basename(weights)}"}
+    seeds = set(range(first_seed, first_seed + nevents))
+    if meta.get("workload") == workload:
+        used = set(meta.get("train_seeds") or []) | set(meta.get("val_seeds") or [])
+        if seeds & used:
+            raise SystemExit(f"bench: events {first_seed}..{first_seed + nevents - 1} overlap the seeds the "
+                             f"proposal was trained or validated on")
     ev = runtime.Event(geom)
     keys = ("nMD", "nSG", "nTP", "nTL", "nTC")
     tot = {"plain": dict.fromkeys(keys, 0), "learned": dict.fromkeys(keys, 0)}
     sec = {"plain": 0.0, "learned": 0.0}
     work = {"plain": None, "learned": None}
-    for first in range(0, nevents, batch):
-        b = Batch(generate(workload, first, min(batch, nevents - first)))
+    for first in range(first_seed, first_seed + nevents, batch):
+        b = Batch(generate(workload, first, min(batch, first_seed + nevents - first)))
         for form in ("plain", "learned"):
             proposal.set_proposal(ev, m if form == "learned" else None, t if form == "learned" else 0.5)
             # two warmup calls: the first call of a new batch or form may size its
@@ -305,17 +312,18 @@ def plain_vs_learned(a, geom, workload, nevents, batch, steps, weights=None, rec
 
 
 def config1_leg(a, geom):
-    """500-track barrel events (workload 1), --config1-batch events per batch, 4
-    batches: plain vs the learned proposal trained on 2,000 such events
-    (proposal_w1.json; these events are neither training nor validation seeds)
-    at each threshold of its 99 / 99.5 / 99.9 % TC-recall sweep."""
+    """500-track barrel events (workload 1, seeds from 10000: neither training
+    nor validation seeds), --config1-batch events per batch, 4 batches: plain
+    vs the learned proposal trained on 2,000 such events (proposal_w1.json) at
+    each threshold of its 99 / 99.5 / 99.9 % TC-recall sweep."""
     w = os.path.join(ROOT, "sdl_amd", "data", "proposal_w1.json")
     if not os.path.exists(w):
         return {"skipped": "sdl_amd/data/proposal_w1.json absent"}
-    out = {"workload": f"workload 1 (500-track barrel events): {4 * a.config1_batch} events in batches of "
-                       f"{a.config1_batch}", "by_target_recall": {}}
+    out = {"workload": f"workload 1 (500-track barrel events): {4 * a.config1_batch} events (seeds 10000..."
+                       f"{10000 + 4 * a.config1_batch - 1}) in batches of {a.config1_batch}", "by_target_recall": {}}
     for r in ("0.99", "0.995", "0.999"):
-        res = plain_vs_learned(a, geom, 1, 4 * a.config1_batch, a.config1_batch, 5, weights=w, recall=r)
+        res = plain_vs_learned(a, geom, 1, 4 * a.config1_batch, a.config1_batch, 5, weights=w, recall=r,
+                               first_seed=10000)
         out["by_target_recall"][r] = res
     return out
 

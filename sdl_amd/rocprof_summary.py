@@ -142,7 +142,8 @@ def kernel_table(trace_path, counter_paths, top=12):
                 for n, v in vals.items():
                     acc[k][n].append(v)
     mean = lambda v: sum(v) / len(v) if v else 0.0
-    names = sorted({n for k in acc for n in acc[k]} - {"SQ_WAVES", "SQ_INSTS_VALU"})
+    # per-wave columns: instruction and wait counters (not the chip-wide cycle counters)
+    names = sorted({n for k in acc for n in acc[k]} - {"SQ_WAVES", "SQ_INSTS_VALU", "GRBM_GUI_ACTIVE", "SQ_BUSY_CYCLES"})
     kern = sorted((k for k in acc if k in dur), key=lambda k: -mean(dur[k]))[:top]
     lines = ["### per kernel, plain chain (mean per launch; duration from the kernel trace)\n",
              "| kernel | us | waves | VALU / wave | VALU issue utilisation | " + " | ".join(
