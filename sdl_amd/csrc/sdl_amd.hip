@@ -705,6 +705,18 @@ __global__ void k_pmd_cap(int NS, int NM, const ModDev* mods, const long long* s
     mdCap[s] = cap;
 }
 
+// Records of hits, mini-doublets and segments (32 bytes at most) are gathered
+// with 32-bit byte offsets: the load then takes the array base from scalar
+// registers and a 32-bit lane offset, where a 64-bit index costs a sign
+// extension, a shift and a 64-bit add per gather.  The sizing steps refuse a
+// call with more than kMaxObjects32 hits, mini-doublets or segments.
+constexpr long long kMaxObjects32 = (1LL << 32) / 32;
+template <class T>
+__device__ __forceinline__ T ld32(const T* base, int i) {
+    static_assert(sizeof(T) <= 32, "record size");
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + unsigned(i) * unsigned(sizeof(T)));
+}
+
 struct HitsDev {
     const float *x, *y, *z, *rt, *r3, *phi, *alpha;
     const int* idx;
@@ -828,7 +840,7 @@ __device__ __forceinline__ MDView loadMD(int i, const MDDev& D) {
     return v;
 }
 __device__ __forceinline__ HitView loadAnchor(int i, const MDDev& D) {
-    const float4 a = D.a[i], b = D.b[i];
+    const float4 a = ld32(D.a, i), b = ld32(D.b, i);
     return HitView{a.x, a.y, a.z, a.w, b.x, b.y, 0.f};
 }
 
@@ -1401,7 +1413,7 @@ __global__ void __launch_bounds__(kBlock) k_tl_enum(Count cSG, int NM, const Mod
 __device__ __forceinline__ TLOutLo loadOutLo(int mLo, int flags, const MDDev& D) {
     TLOutLo L;
     L.a = loadAnchor(mLo, D);
-    const float4 p = D.p[mLo];
+    const float4 p = ld32(D.p, mLo);
     L.alpha1 = p.x;
     L.dzScale = p.z;
     L.alphaReg3 = p.w;
@@ -1412,38 +1424,38 @@ __device__ __forceinline__ TLOutLo loadOutLo(int mLo, int flags, const MDDev& D)
 
 __device__ __forceinline__ TLInner loadInner(int s, const MDDev& D, const SGDev& S) {
     TLInner I;
-    const int mi = S.in[s], mo = S.out[s];
+    const int mi = ld32(S.in, s), mo = ld32(S.out, s);
     I.lo = loadAnchor(mi, D);
     I.up = loadAnchor(mo, D);
-    const SGF fi = S.f[s];
+    const SGF fi = ld32(S.f, s);
     I.dphichange = fi.v[3];
     I.dphichangemin = fi.v[4];
     I.dphichangemax = fi.v[5];
-    const float4 xi = S.x[s];
+    const float4 xi = ld32(S.x, s);
     I.dr = xi.x;
     I.d = xi.y;
-    I.alphaReg3Lo = D.p[mi].w;
-    I.sqrtR3Rt = D.q[mi];
-    const int fl = S.flags[s];
+    I.alphaReg3Lo = ld32(D.p, mi).w;
+    I.sqrtR3Rt = ld32(D.q, mi);
+    const int fl = ld32(S.flags, s);
     I.bLo = fl & 1; I.bUp = fl & 2; I.psLo = fl & 4; I.psUp = fl & 8;
     return I;
 }
 
 __device__ __forceinline__ TLOutUp loadOutUp(int so, int flo, HitsDev H, const MDDev& D, const SGDev& S) {
     TLOutUp U;
-    const int mUp = S.out[so];
+    const int mUp = ld32(S.out, so);
     U.a = loadAnchor(mUp, D);
-    const SGF sf = S.f[so];
+    const SGF sf = ld32(S.f, so);
     U.dphimin = sf.v[1]; U.dphimax = sf.v[2];
     U.dphichange = sf.v[3]; U.dphichangemin = sf.v[4]; U.dphichangemax = sf.v[5];
-    const float4 sx = S.x[so];
+    const float4 sx = ld32(S.x, so);
     U.dr = sx.x; U.d = sx.y; U.aOU = sx.z; U.dPhiPos = sx.w;
     U.ec2s = flo & 16;
     U.hiX = U.hiY = U.hiRt = U.hiPhi = U.loX = U.loY = U.loRt = U.loPhi = U.aOUhi = U.aOUlo = 0.f;
     if (U.ec2s) {
-        const int an = D.anchor[mUp];
-        const float4 hi = H.eHi[an], lo = H.eLo[an];
-        const float2 ee = S.e[so];
+        const int an = ld32(D.anchor, mUp);
+        const float4 hi = ld32(H.eHi, an), lo = ld32(H.eLo, an);
+        const float2 ee = ld32(S.e, so);
         U.hiX = hi.x; U.hiY = hi.y; U.hiRt = hi.z; U.hiPhi = hi.w;
         U.loX = lo.x; U.loY = lo.y; U.loRt = lo.z; U.loPhi = lo.w;
         U.aOUhi = ee.x; U.aOUlo = ee.y;
@@ -1461,13 +1473,13 @@ __global__ void __launch_bounds__(kBlock) k_tl_stage1(VarRange R, MDDev D, SGDev
     const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= e0) return;
     const int2 pr = pair[i];
-    const int sgA = int(sgStart[pr.y]);
+    const int sgA = int(ld32(sgStart, pr.y));
     const TLInner I = loadInner(pr.x, D, S);
-    const TLOutLo L = loadOutLo(pr.y, S.flags[sgA], D);
+    const TLOutLo L = loadOutLo(pr.y, ld32(S.flags, sgA), D);
     TLCarry c{0.f, 0.f, 0.f};
     const bool ok = tlStage1(V, I, L, c);
     if (ok) carry[i] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);  // read for surviving pairs only
-    nOut[i] = ok ? int(sgStart[pr.y + 1]) - sgA : 0;
+    nOut[i] = ok ? int(ld32(sgStart, pr.y + 1)) - sgA : 0;
 }
 
 // ---- learned proposal (pairs that passed the first stage are scored)
@@ -1511,13 +1523,13 @@ template <int V>
 __device__ __forceinline__ bool stage1Features(long long i, const int2* pair, const long long* sgStart, MDDev D,
                                                SGDev S, TLCarry& c, int& nOuter, float* f) {
     const int2 pr = pair[i];
-    const int sgA = int(sgStart[pr.y]);
+    const int sgA = int(ld32(sgStart, pr.y));
     const TLInner I = loadInner(pr.x, D, S);
-    const TLOutLo L = loadOutLo(pr.y, S.flags[sgA], D);
+    const TLOutLo L = loadOutLo(pr.y, ld32(S.flags, sgA), D);
     TLMargins m{0.f, 0.f, 0.f};
     c = TLCarry{0.f, 0.f, 0.f};
     const bool ok = tlStage1(V, I, L, c, &m);
-    nOuter = int(sgStart[pr.y + 1]) - sgA;
+    nOuter = int(ld32(sgStart, pr.y + 1)) - sgA;
     if (ok) tlPairFeatures(I, L, c, m, V, f);
     return ok;
 }
@@ -1630,7 +1642,7 @@ __global__ void __launch_bounds__(kBlock) k_tl_stage2(VarRange R, HitsDev H, MDD
     const int2 pr = pair[c.x];
     const float4 cf = carry[c.x];
     const int so = c.y;
-    const int flo = S.flags[so];
+    const int flo = ld32(S.flags, so);
     const TLInner I = loadInner(pr.x, D, S);
     const TLOutLo L = loadOutLo(pr.y, flo, D);
     const TLOutUp U = loadOutUp(so, flo, H, D, S);
@@ -2636,6 +2648,7 @@ int loadGeomFiles(const char* tilted, const char* endcap, const char* map, sdl_g
 int ensureHits(sdl_event* ev) {
     const long long NH = ev->NH;
     const int NS = ev->E * ev->g->NM;
+    if (NH > kMaxObjects32) return fail(SDL_ERR_ARG, "more than 134,217,728 hits in one call: split the batch");
     if (int r = ev->rt.ensure(NH)) return r;
     if (int r = ev->r3.ensure(NH)) return r;
     if (int r = ev->phi.ensure(NH)) return r;
@@ -2809,6 +2822,7 @@ int runMD(sdl_event* ev, bool pseudo) {
     if (nMD < 0) return fail(SDL_ERR_HIP, "readback failed");
     ev->nMD = nMD;
     const long long n = std::max(nMD, 1LL);
+    if (n > kMaxObjects32) return fail(SDL_ERR_ARG, "more than 134,217,728 mini-doublets in one call: split the batch");
     for (DBuf<int>* b : {&ev->mdLo, &ev->mdUp, &ev->mdAnchor, &ev->mdSlot})
         if (int r = b->ensure(n)) return r;
     for (DBuf<float4>* b : {&ev->mdA, &ev->mdB, &ev->mdP})
@@ -2873,6 +2887,7 @@ int runSG(sdl_event* ev, bool layers) {
     if (nSG < 0) return fail(SDL_ERR_HIP, "readback failed");
     ev->nSG = nSG;
     const long long ns = std::max(nSG, 1LL);
+    if (ns > kMaxObjects32) return fail(SDL_ERR_ARG, "more than 134,217,728 segments in one call: split the batch");
     for (DBuf<int>* b : {&ev->sgIn, &ev->sgOut, &ev->sgFlags, &ev->sgKey, &ev->sgEndList, &ev->tpCount})
         if (int r = b->ensure(ns)) return r;
     if (int r = ev->sgF.ensure(ns)) return r;

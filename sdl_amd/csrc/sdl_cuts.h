@@ -61,7 +61,9 @@ __device__ __forceinline__ float cosF(float v) { return crm::cosCR(v); }      //
 __device__ __forceinline__ float tanF(float v) { return float(tan(double(v))); }
 __device__ __forceinline__ float acosF(float v) { return float(acos(double(v))); }
 __device__ __forceinline__ float sqrtF(float v) { return crm::sqrtCR(v); }      // float(sqrt(double(v)))
-__device__ __forceinline__ float fabsF(float v) { return v < 0.f ? -v : (v == 0.f ? 0.f : v); }
+// std::abs(float) (the sign bit cleared, NaN included): a source modifier of
+// the instruction that reads it
+__device__ __forceinline__ float fabsF(float v) { return fabsf(v); }
 // std::min / std::max argument-order semantics (NaN follows the first argument)
 __device__ __forceinline__ float fmin2(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmax2(float a, float b) { return (a < b) ? b : a; }
@@ -72,9 +74,13 @@ __device__ __forceinline__ int absTrunc(float v) {
     return i == INT_MIN ? INT_MIN : (i < 0 ? -i : i);
 }
 
-// MathUtil.cc:3-19
+// MathUtil.cc:3-19.  For a float x, double(x) >= kPI iff x >= kPIUpF (the
+// smallest float above pi) and double(x) < -kPI iff x <= -kPIUpF, so one float
+// compare of |x| decides whether either loop runs (false for NaN as well).
+constexpr float kPIUpF = 0x1.921fb6p+1f;
+static_assert(double(kPIUpF) > kPI && double(0x1.921fb4p+1f) < kPI, "kPIUpF is the float above pi");
 __device__ __forceinline__ float phiMpiPi(float x) {
-    if (isnan(x) || isinf(x)) return x;
+    if (!(fabsf(x) >= kPIUpF) || isinf(x)) return x;
     while (double(x) >= kPI) x = float(double(x) - 2. * kPI);
     while (double(x) < -kPI) x = float(double(x) + 2. * kPI);
     return x;
@@ -417,8 +423,33 @@ __device__ __forceinline__ void deltaBetaIter(float& betaIn, float& betaOut, flo
         return copysign(double(asinF(fmin2(len * k2 / fabsF(pt), kSinAlphaMax))), double(signOf));
     };
     auto ptOf = [&](float beta) { return crm::divSinCR(dr * k2, beta); };
-    if (betaIn * betaOut > 0.f &&
-        (fabsF(ptBeta) < 4.f * ptBetaMax || (lIn >= 11 && fabsF(ptBeta) < 8.f * ptBetaMax))) {
+    const bool mode1 = betaIn * betaOut > 0.f &&
+                       (fabsF(ptBeta) < 4.f * ptBetaMax || (lIn >= 11 && fabsF(ptBeta) < 8.f * ptBetaMax));
+    const bool mode2 = !mode1 && lIn < 11 && double(fabsF(betaOut)) < 0.2 * double(fabsF(betaIn)) &&
+                       fabsF(ptBeta) < 12.f * ptBetaMax;
+    if (!tk) {
+        // The cuts' path: the two corrected modes below as one sequence (the same
+        // operations on the same values), so that a wave with lanes in both
+        // modes runs the corrections once.  Mode 2 starts from the pt of betaIn
+        // and takes the sign of both corrections from betaIn (the updated one in
+        // the second step); mode 1 from ptBeta, each correction with its own sign.
+        if (!(mode1 || mode2)) return;
+        float pt0 = ptBeta;
+        if (mode2) pt0 = ptOf(betaIn);
+        const float sign0 = mode1 ? betaOut : betaIn;
+        const float inUpd = float(double(betaIn) + corr(sdInDr, pt0, betaIn));
+        const float outUpd = float(double(betaOut) + corr(sdOutDr, pt0, sign0));
+        betaAv = (mode1 || fabsF(betaOut) > 0.2f * fabsF(betaIn)) ? (0.5f * (inUpd + outUpd)) : inUpd;
+        ptBeta = ptOf(betaAv);
+        const float in2 = float(double(betaIn) + corr(sdInDr, ptBeta, betaIn));
+        const float sign1 = mode1 ? betaOut : in2;
+        betaOut = float(double(betaOut) + corr(sdOutDr, ptBeta, sign1));
+        betaIn = in2;
+        betaAv = 0.5f * (betaIn + betaOut);
+        ptBeta = ptOf(betaAv);
+        return;
+    }
+    if (mode1) {
         const float in0 = betaIn, out0 = betaOut;
         if (tk) {  // Tracklet.cc:3192-3200
             tk[TK_betacormode] = 1.f;
@@ -453,7 +484,7 @@ __device__ __forceinline__ void deltaBetaIter(float& betaIn, float& betaOut, flo
             const float av4 = 0.5f * (in4 + out4);
             tkIter(tk, 4, in4, out4, av4, ptOf(av4));
         }
-    } else if (lIn < 11 && double(fabsF(betaOut)) < 0.2 * double(fabsF(betaIn)) && fabsF(ptBeta) < 12.f * ptBetaMax) {
+    } else if (mode2) {
         if (tk) { tk[TK_betacormode] = 2.f; tkIter(tk, 0, betaIn, betaOut, betaAv, ptBeta); }  // :3245-3250
         const float ptBetaIn = ptOf(betaIn);
         const float inUpd = float(double(betaIn) + corr(sdInDr, ptBetaIn, betaIn));
