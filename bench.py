@@ -173,12 +173,14 @@ def learned_leg(a, ev, step, plain_counts, plain_ms, plain_work):
         step()
     ev.sync()
     hip.synchronize()
+    s0 = ev.run_stats()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         step()
     ev.sync()
     hip.synchronize()
     wall = time.perf_counter() - t0
+    calls = stats_delta(s0, ev.run_stats())
     c = ev.counts(-1)
     work = ev.tl_work()
     proposal.set_proposal(ev, None, 0.5)
@@ -189,7 +191,7 @@ def learned_leg(a, ev, step, plain_counts, plain_ms, plain_work):
             "tl_recall": round(c["nTL"] / max(plain_counts["nTL"], 1), 6),
             "tl_work_learned": work, "tl_work_plain": plain_work,
             "weights": os.path.relpath(a.learned, ROOT), "trained_on": meta.get("train_seeds"),
-            "validated_on": meta.get("val_seeds")}
+            "validated_on": meta.get("val_seeds"), "timed_calls": calls}
 
 
 def generate(workload, first, n):
@@ -223,18 +225,26 @@ class Batch:
             d.free()
 
 
+def stats_delta(before, after):
+    """Graph replays, captures and exact reruns between two run_stats() readings."""
+    return {k: after[k] - before[k] for k in ("replays", "captures", "reruns")}
+
+
 def timed(ev, fn, warmup, steps):
+    """Seconds per step over `steps` calls after `warmup` calls, and the
+    handle's graph replays, captures and exact reruns during the timed calls."""
     from sdl_amd import hip
     for _ in range(warmup):
         fn()
     ev.sync()
     hip.synchronize()
+    s0 = ev.run_stats()
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
     ev.sync()
     hip.synchronize()
-    return (time.perf_counter() - t0) / steps
+    return (time.perf_counter() - t0) / steps, stats_delta(s0, ev.run_stats())
 
 
 def batch_leg(a, geom):
@@ -249,14 +259,14 @@ def batch_leg(a, geom):
     tune = None
     if a.autotune:
         tune = ev.autotune(reps=3)
-    sec = timed(ev, lambda: b.run(ev), 2, a.batch_steps)
+    sec, st = timed(ev, lambda: b.run(ev), 2, a.batch_steps)
     counts = ev.counts(-1)
     out = {"workload": f"workload {a.batch_workload}: {a.batch} events, tracks per event uniform in "
                        f"[2000, 10000] + 20 hard, one batch per step",
            "events_per_step": a.batch, "value": round(a.batch / sec, 3), "unit": "events/s",
            "ms_per_step": round(1000 * sec, 3), "hits_per_step": int(b.off[-1]),
            "tracks_min_max": [min(b.tracks), max(b.tracks)], "objects_per_step": counts,
-           "tuning": ev.tuning(), "autotune": tune}
+           "tuning": ev.tuning(), "autotune": tune, "timed_calls": st}
     b.free()
     return out
 
@@ -286,14 +296,19 @@ def plain_vs_learned(a, geom, workload, nevents, batch, steps, weights=None, rec
     tot = {"plain": dict.fromkeys(keys, 0), "learned": dict.fromkeys(keys, 0)}
     sec = {"plain": 0.0, "learned": 0.0}
     work = {"plain": None, "learned": None}
+    calls = {f: {"replays": 0, "captures": 0, "reruns": 0} for f in ("plain", "learned")}
     for first in range(first_seed, first_seed + nevents, batch):
         b = Batch(generate(workload, first, min(batch, first_seed + nevents - first)))
         for form in ("plain", "learned"):
             proposal.set_proposal(ev, m if form == "learned" else None, t if form == "learned" else 0.5)
-            # two warmup calls: the first call of a new batch or form may size its
-            # buffers exactly, the next one grows them for the speculative form
-            # and captures its graph; only calls after that are timed
-            sec[form] += timed(ev, lambda: b.run(ev), 2, steps)
+            # three warmup calls: the first call of a new batch or form may rerun
+            # sized exactly (its predictions come from the last batch or form),
+            # the next one grows the buffers for the speculative form and
+            # captures its graph; only calls after that are timed
+            dt, st = timed(ev, lambda: b.run(ev), 3, steps)
+            sec[form] += dt
+            for k in st:
+                calls[form][k] += st[k]
             c = ev.counts(-1)
             for k in keys:
                 tot[form][k] += c[k]
@@ -308,7 +323,7 @@ def plain_vs_learned(a, geom, workload, nevents, batch, steps, weights=None, rec
             "target_tc_recall": float(recall), "recall_per_stage": rec,
             "weights": os.path.relpath(weights, ROOT),
             "objects_plain": tot["plain"], "objects_learned": tot["learned"],
-            "tl_work_last_batch": work}
+            "tl_work_last_batch": work, "timed_calls": calls}
 
 
 def config1_leg(a, geom):
@@ -412,6 +427,7 @@ def main():
     # Timed region: K steps back to back, no per-kernel timing.
     start, stop = hip.Event(), hip.Event()
     hip.synchronize()
+    stats0 = ev.run_stats()
     t0 = time.perf_counter()
     start.record(stream)
     for _ in range(a.steps):
@@ -422,6 +438,7 @@ def main():
     wall = time.perf_counter() - t0
     gpu_ms = start.elapsed_ms(stop)
     ms_per_step = 1000.0 * wall / a.steps
+    timed_calls = stats_delta(stats0, ev.run_stats())
     counts = ev.counts(-1)
 
     # Per-kernel durations, measured in a second pass (HIP events around the
@@ -457,6 +474,7 @@ def main():
                                f"~{int(off[-1] / B)} hits/event, plain module-map chain",
                    "events_per_step": B, "hits_per_step": int(off[-1])},
         "gpu_ms_per_step": round(gpu_ms / a.steps, 4),
+        "timed_calls": timed_calls,
         "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
         "objects_per_step": counts,
         "roofline": roofline,

@@ -229,6 +229,12 @@ int sdl_event_kernel_ms(sdl_event* ev, double* out, int n);
 /* With no counterpart in the reference: whether run_chain may capture its
  * speculative form into a HIP graph and replay it (default on). */
 int sdl_event_set_graphs(sdl_event* ev, int on);
+/* With no counterpart in the reference: counts since the handle was created,
+ * out[0] stage runs (sdl_event_run or sdl_event_run_chain calls), out[1] runs
+ * replayed from a captured graph, out[2] graph captures, out[3] exact reruns
+ * (a speculative run whose predicted sizes fell short; the rerun sizes each
+ * stage from its read-back totals and a captured graph is captured anew). */
+int sdl_event_run_stats(sdl_event* ev, long long out[4]);
 /* Tuning knob with no counterpart in the reference: the headroom factor of the
  * predicted buffer sizes for the stages >= fromStage (default 1.25 for all).
  * A factor below 1 forces the resume path; the tests use it to check it. */

@@ -2501,6 +2501,8 @@ struct sdl_event {
     // learned proposal (null: the plain chain)
     sdl_mlp* proposal = nullptr;
     float proposalLogit = 0.f;
+    // run statistics: stage runs, graph replays, graph captures, exact reruns
+    long long stats[4] = {0, 0, 0, 0};
     DBuf<int> tcCount;
     DBuf<int2> tcpInfo;
     DBuf<long long> tcpStart, tclStart;
@@ -3469,6 +3471,7 @@ int launchChainGraph(sdl_event* ev) {
     if (ev->graphExec && sameGraphKey(ev)) {
         HIP_OK(hipGraphLaunch(ev->graphExec, st));
         ev->hitsReady = true;
+        ++ev->stats[1];
         return 1;
     }
     dropGraph(ev);
@@ -3498,6 +3501,7 @@ int launchChainGraph(sdl_event* ev) {
     k.proposal = ev->proposal; k.proposalLogit = ev->proposalLogit; k.tune = ev->tune;
     HIP_OK(hipGraphLaunch(ev->graphExec, st));
     ev->hitsReady = true;
+    ++ev->stats[2];
     return 1;
 }
 
@@ -3523,6 +3527,7 @@ int runStages(sdl_event* ev, int s0, int s1) {
     }
     if (!launched)
         if (int r = enqueueStages(ev, s0, s1)) return r;
+    ++ev->stats[0];
     long long t[TOT_N];
     HIP_OK(hipMemcpyAsync(t, ev->tot.p, sizeof(t), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -3530,8 +3535,12 @@ int runStages(sdl_event* ev, int s0, int s1) {
         return fail(SDL_ERR_UNKNOWN_DETID, std::to_string(t[TOT_UNKNOWN]) + " hits have a detId unknown to the geometry");
     if (t[TOT_ABORT] != 0) {
         if (!ev->spec) return fail(SDL_ERR_CAPACITY, "device capacity check failed on an exactly sized run");
-        // rerun from the stage that overflowed, sized exactly
+        // rerun from the stage that overflowed, sized exactly; a captured graph
+        // holds the capacities that fell short, so the next call captures anew
+        // (from the totals of this run) instead of replaying it into another rerun
         const int from = t[TOT_ABORT] > SDL_STAGE_TCL + 1 ? s0 : int(t[TOT_ABORT] - 1);
+        if (launched) dropGraph(ev);
+        ++ev->stats[3];
         ev->spec = false;
         zeroAsync(ev->tot.p + TOT_ABORT, 1, st);
         for (int s = std::max(from, s0); s <= s1; ++s)
@@ -4554,6 +4563,12 @@ int sdl_event_set_proposal(sdl_event* ev, sdl_mlp* m, float threshold) {
     if (!ev || (m && !(threshold > 0.f && threshold < 1.f))) return fail(SDL_ERR_ARG, "bad arguments");
     ev->proposal = m;
     ev->proposalLogit = m ? std::log(threshold / (1.f - threshold)) : 0.f;
+    return 0;
+}
+
+int sdl_event_run_stats(sdl_event* ev, long long out[4]) {
+    if (!ev || !out) return fail(SDL_ERR_ARG, "bad arguments");
+    for (int k = 0; k < 4; ++k) out[k] = ev->stats[k];
     return 0;
 }
 

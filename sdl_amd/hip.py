@@ -69,6 +69,14 @@ class DeviceArray:
     def address(self):
         return self.ptr.value
 
+    def upload(self, host):
+        """Copies a host array of at most nbytes bytes to the start of the array."""
+        host = np.ascontiguousarray(host)
+        if host.nbytes > self.nbytes:
+            raise ValueError("host array larger than the device array")
+        if host.nbytes:
+            _ok(hip().hipMemcpy(self.ptr, host.ctypes.data, host.nbytes, H2D), "hipMemcpy")
+
     def to_host(self):
         out = np.empty(self.shape, self.dtype)
         if self.nbytes:

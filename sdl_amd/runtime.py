@@ -29,9 +29,10 @@ EXPORTS = (
     "sdl_event_run", "sdl_event_run_chain", "sdl_event_sync", "sdl_event_counts", "sdl_event_layer_counts",
     "sdl_event_export", "sdl_event_export_noalloc", "sdl_object_var_count", "sdl_object_var_names", "sdl_event_export_vars",
     "sdl_event_num_events", "sdl_event_stream", "sdl_event_set_timing", "sdl_event_kernel_ms",
-    "sdl_event_set_headroom", "sdl_event_set_graphs",
+    "sdl_event_set_headroom", "sdl_event_set_graphs", "sdl_event_run_stats",
     "sdl_mlp_create", "sdl_mlp_free", "sdl_mlp_num_params", "sdl_mlp_set_params", "sdl_mlp_get_params",
-    "sdl_mlp_gradient", "sdl_mlp_train", "sdl_mlp_forward", "sdl_event_set_proposal", "sdl_event_tl_pair_rows", "sdl_event_tl_work",
+    "sdl_mlp_gradient", "sdl_mlp_train", "sdl_mlp_forward", "sdl_event_set_proposal",
+    "sdl_event_tl_pair_rows", "sdl_event_tl_work",
     "sdl_event_autotune", "sdl_event_get_tuning", "sdl_event_set_tuning",
     "sdl_geom_convert", "sdl_geom_save", "sdl_geom_load_binary", "sdl_geom_check_binary",
     "sdl_geom_load_tables", "sdl_module_decode", "sdl_map_build",
@@ -81,6 +82,7 @@ def lib():
         L.sdl_event_kernel_ms.argtypes = [vp, vp, i]
         L.sdl_event_set_headroom.argtypes = [vp, i, ctypes.c_double]
         L.sdl_event_set_graphs.argtypes = [vp, i]
+        L.sdl_event_run_stats.argtypes = [vp, vp]
         f = ctypes.c_float
         L.sdl_mlp_create.argtypes = [ctypes.POINTER(vp)]
         L.sdl_mlp_free.argtypes = [vp]
@@ -316,6 +318,13 @@ class Event:
     def set_graphs(self, on=True):
         """Allow the speculative chain to be captured into a HIP graph and replayed."""
         _check(lib().sdl_event_set_graphs(self.h, 1 if on else 0))
+
+    def run_stats(self):
+        """Counts since the handle was created: stage runs, graph replays, graph
+        captures and exact reruns (speculative sizes that fell short)."""
+        out = np.zeros(4, np.int64)
+        _check(lib().sdl_event_run_stats(self.h, out.ctypes.data))
+        return dict(zip(("runs", "replays", "captures", "reruns"), (int(v) for v in out)))
 
     def kernel_ms(self):
         out = np.zeros(len(TIMERS), np.float64)

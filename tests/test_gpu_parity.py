@@ -224,6 +224,50 @@ def test_graph_replays_match_eager():
         assert np.array_equal(ev.layer_counts(0), eager.layer_counts(0))
 
 
+def test_graph_recaptured_after_short_prediction():
+    """A captured graph whose sizes are too small for new input in the same
+    device arrays (same pointers, so the same graph key): that call reruns
+    sized exactly, the next call captures a new graph instead of replaying the
+    old one into another rerun, and later calls replay it; every call equals
+    the oracle."""
+    from sdl_amd import hip
+    print("case graph recaptured after a short prediction", flush=True)
+    small, big = gen.event(12, 0), gen.event(5, 2)
+    assert big.nhits > 5 * small.nhits
+    fields = ("x", "y", "z", "idx", "detId")
+    pad = lambda a: np.concatenate([a, np.zeros(big.nhits - len(a), a.dtype)])
+    dev = [hip.DeviceArray(pad(getattr(small, f))) for f in fields]
+    hip.synchronize()
+    refs = {id(e): oracle.run(gen.geometry(2)[1], e, oracle.CR) for e in (small, big)}
+    ev = runtime.Event(_geom(2))
+
+    def run(e, what):
+        for d, f in zip(dev, fields):
+            d.upload(getattr(e, f))
+        ev.set_hits_device([0, e.nhits], *[d.address for d in dev])
+        ev.run_chain()
+        ev.sync()
+        ref, cnt = refs[id(e)]
+        _assert_same(ev.export_all(0), ref, what)
+        assert ev.counts(0) == cnt
+        return ev.run_stats()
+
+    run(big, "big exact")
+    run(big, "big capture")  # every buffer now holds the big event
+    ev.set_graphs(False)
+    run(small, "small eager")
+    ev.set_graphs(True)
+    s0 = run(small, "small capture")  # a graph sized for the small event
+    s0 = run(small, "small replay")
+    assert s0["replays"] >= 1
+    s1 = run(big, "big after small")  # replays the small graph, falls short, reruns
+    assert s1["reruns"] == s0["reruns"] + 1
+    s2 = run(big, "big recapture")
+    assert s2["reruns"] == s1["reruns"] and s2["captures"] == s1["captures"] + 1
+    s3 = run(big, "big replay")
+    assert s3["reruns"] == s2["reruns"] and s3["replays"] == s2["replays"] + 1
+
+
 def test_autotune_keeps_objects():
     """The autotuner only changes launch shapes: after tuning, and with every
     knob at another allowed value, a batch of uneven events gives the same
