@@ -1463,14 +1463,31 @@ __device__ __forceinline__ TLOutUp loadOutUp(int so, int flo, HitsDev H, const M
     return U;
 }
 
+// The three variants of a tracklet stage in one launch: blocks [0, nb[0]) run
+// variant 0, the next nb[1] blocks variant 1, the rest variant 2 (nb[v] blocks
+// cover variant v's capacity), so the variants' tails and heads overlap instead
+// of the chip draining between three launches.
+struct VarRanges {
+    VarRange r[kNVariantsTL];
+    int nb[kNVariantsTL];
+};
+__device__ __forceinline__ int variantOfBlock(const VarRanges& R, int& blk) {
+    blk = blockIdx.x;
+    for (int v = 0; v < kNVariantsTL - 1; ++v) {
+        if (blk < R.nb[v]) return v;
+        blk -= R.nb[v];
+    }
+    return kNVariantsTL - 1;
+}
+
 // First stage, one thread per pair: carry = (sdlCut, thetaMuls, sin of the
 // midpoint deltaPhi); nOut = number of outer SGs to try (0 if the pair failed).
 template <int V>
-__global__ void __launch_bounds__(kBlock) k_tl_stage1(VarRange R, MDDev D, SGDev S, const long long* sgStart,
-                                                      const int2* pair, float4* carry, int* nOut) {
+__device__ __forceinline__ void tlStage1Item(const VarRange& R, int blk, MDDev D, SGDev S, const long long* sgStart,
+                                             const int2* pair, float4* carry, int* nOut) {
     long long b0, e0;
     R.get(b0, e0);
-    const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = b0 + (long long)blk * blockDim.x + threadIdx.x;
     if (i >= e0) return;
     const int2 pr = pair[i];
     const int sgA = int(ld32(sgStart, pr.y));
@@ -1480,6 +1497,14 @@ __global__ void __launch_bounds__(kBlock) k_tl_stage1(VarRange R, MDDev D, SGDev
     const bool ok = tlStage1(V, I, L, c);
     if (ok) carry[i] = make_float4(c.sdlCut, c.thetaMuls, c.sinDPhiMid, 0.f);  // read for surviving pairs only
     nOut[i] = ok ? int(ld32(sgStart, pr.y + 1)) - sgA : 0;
+}
+__global__ void __launch_bounds__(kBlock) k_tl_stage1(VarRanges R, MDDev D, SGDev S, const long long* sgStart,
+                                                      const int2* pair, float4* carry, int* nOut) {
+    int blk;
+    const int v = variantOfBlock(R, blk);
+    if (v == TL_BBBB) tlStage1Item<TL_BBBB>(R.r[v], blk, D, S, sgStart, pair, carry, nOut);
+    else if (v == TL_BBEE) tlStage1Item<TL_BBEE>(R.r[v], blk, D, S, sgStart, pair, carry, nOut);
+    else tlStage1Item<TL_EEEE>(R.r[v], blk, D, S, sgStart, pair, carry, nOut);
 }
 
 // ---- learned proposal (pairs that passed the first stage are scored)
@@ -1631,12 +1656,12 @@ __global__ void k_tl_expand(Count cP, const long long* sgStart, const int2* pair
 
 // Second stage, one thread per candidate; survivors are counted per inner SG.
 template <int V>
-__global__ void __launch_bounds__(kBlock) k_tl_stage2(VarRange R, HitsDev H, MDDev D, SGDev S, const int2* cand,
-                                                      const int2* pair, const float4* carry, unsigned char* flag,
-                                                      int* tlCount, int nm1) {
+__device__ __forceinline__ void tlStage2Item(const VarRange& R, int blk, const HitsDev& H, const MDDev& D,
+                                             const SGDev& S, const int2* cand, const int2* pair, const float4* carry,
+                                             unsigned char* flag, int* tlCount, int nm1) {
     long long b0, e0;
     R.get(b0, e0);
-    const long long i = b0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = b0 + (long long)blk * blockDim.x + threadIdx.x;
     if (i >= e0) return;
     const int2 c = cand[i];
     const int2 pr = pair[c.x];
@@ -1650,6 +1675,15 @@ __global__ void __launch_bounds__(kBlock) k_tl_stage2(VarRange R, HitsDev H, MDD
     const bool pass = tlStage2(V, (flo & 2) != 0, I, L, U, cc, nullptr, nm1 != 0);
     flag[i] = pass ? 1 : 0;
     if (pass) atomicAdd(&tlCount[pr.x], 1);
+}
+__global__ void __launch_bounds__(kBlock) k_tl_stage2(VarRanges R, HitsDev H, MDDev D, SGDev S, const int2* cand,
+                                                      const int2* pair, const float4* carry, unsigned char* flag,
+                                                      int* tlCount, int nm1) {
+    int blk;
+    const int v = variantOfBlock(R, blk);
+    if (v == TL_BBBB) tlStage2Item<TL_BBBB>(R.r[v], blk, H, D, S, cand, pair, carry, flag, tlCount, nm1);
+    else if (v == TL_BBEE) tlStage2Item<TL_BBEE>(R.r[v], blk, H, D, S, cand, pair, carry, flag, tlCount, nm1);
+    else tlStage2Item<TL_EEEE>(R.r[v], blk, H, D, S, cand, pair, carry, flag, tlCount, nm1);
 }
 
 // Pairs and candidates per variant, for the next call's size predictions.
@@ -3059,14 +3093,25 @@ int runTL(sdl_event* ev) {
     ev->tlScoredStride = waveStride;
     if (learned)
         if (int r = ev->tlScored.ensure(kNVariantsTL * waveStride)) return r;
+    const int b1 = ev->tune.v[TUNE_TL1_BLOCK];
+    VarRanges R1{};
+    long long nb1 = 0;
     for (int v = 0; v < kNVariantsTL; ++v) {
-        if (np[v] == 0) continue;
-        const VarRange R{ev->tlPOff.p, nullptr, ns, v, np[v], ev->tot.p};
-        const int b1 = ev->tune.v[TUNE_TL1_BLOCK];
-        const dim3 grid(gridFor(np[v], b1));
-        if (learned) {
+        R1.r[v] = VarRange{ev->tlPOff.p, nullptr, ns, v, np[v], ev->tot.p};
+        // a speculative launch gives every variant a block, so that a variant
+        // predicted empty still checks its true count against the capacity
+        R1.nb[v] = std::max(int(gridFor(np[v], b1)), ev->spec ? 1 : 0);
+        nb1 += R1.nb[v];
+    }
+    if (!learned && nb1 > 0)
+        hipLaunchKernelGGL(k_tl_stage1, dim3(unsigned(nb1)), dim3(b1), 0, st, R1, D, S, ev->sgStart.p, ev->tlPair.p,
+                           ev->tlCarry.p, ev->tlNOut.p);
+    for (int v = 0; learned && v < kNVariantsTL; ++v) {
+        if (R1.nb[v] == 0) continue;
+        const VarRange R = R1.r[v];
+        {
             const int sw = ev->tune.v[TUNE_SCORED_WAVES];
-            const dim3 sgrid(gridFor(np[v], sw * kWave));
+            const dim3 sgrid(std::max(gridFor(np[v], sw * kWave), 1u));
             const size_t lds = size_t(sw) * kScoredLdsPerWave;
             const mlp::Packed* P = ev->proposal->dPacked;
             const float cut = ev->proposalLogit;
@@ -3079,17 +3124,7 @@ int runTL(sdl_event* ev) {
             else
                 hipLaunchKernelGGL(k_tl_stage1_scored<TL_EEEE>, sgrid, dim3(sw * kWave), lds, st, R, D, S, ev->sgStart.p,
                                    ev->tlPair.p, P, cut, ev->tlCarry.p, ev->tlNOut.p, ev->tlScored.p + 2 * waveStride);
-            continue;
         }
-        if (v == 0)
-            hipLaunchKernelGGL(k_tl_stage1<TL_BBBB>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
-                               ev->tlCarry.p, ev->tlNOut.p);
-        else if (v == 1)
-            hipLaunchKernelGGL(k_tl_stage1<TL_BBEE>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
-                               ev->tlCarry.p, ev->tlNOut.p);
-        else
-            hipLaunchKernelGGL(k_tl_stage1<TL_EEEE>, grid, dim3(b1), 0, st, R, D, S, ev->sgStart.p, ev->tlPair.p,
-                               ev->tlCarry.p, ev->tlNOut.p);
     }
     // candidates of the surviving pairs: one list, variant-major
     const Count cPairs = devCount(ev, TOT_TLPAIRS);
@@ -3112,21 +3147,17 @@ int runTL(sdl_event* ev) {
     // second stage per candidate
     zeroMany(st, {span(ev->tlCount.p, ns), span(ev->tlFill.p, ns)});
     const HitsDev H = hitsView(ev);
+    const int b2 = ev->tune.v[TUNE_TL2_BLOCK];
+    VarRanges R2{};
+    long long nb2 = 0;
     for (int v = 0; v < kNVariantsTL; ++v) {
-        if (nc[v] == 0) continue;
-        const VarRange R{ev->tlPOff.p, ev->tlCOff.p, ns, v, nc[v], ev->tot.p};
-        const int b2 = ev->tune.v[TUNE_TL2_BLOCK];
-        const dim3 grid(gridFor(nc[v], b2));
-        if (v == 0)
-            hipLaunchKernelGGL(k_tl_stage2<TL_BBBB>, grid, dim3(b2), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
-                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p, ev->tlNm1);
-        else if (v == 1)
-            hipLaunchKernelGGL(k_tl_stage2<TL_BBEE>, grid, dim3(b2), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
-                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p, ev->tlNm1);
-        else
-            hipLaunchKernelGGL(k_tl_stage2<TL_EEEE>, grid, dim3(b2), 0, st, R, H, D, S, ev->tlCand.p, ev->tlPair.p,
-                               ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p, ev->tlNm1);
+        R2.r[v] = VarRange{ev->tlPOff.p, ev->tlCOff.p, ns, v, nc[v], ev->tot.p};
+        R2.nb[v] = std::max(int(gridFor(nc[v], b2)), ev->spec ? 1 : 0);
+        nb2 += R2.nb[v];
     }
+    if (nb2 > 0)
+        hipLaunchKernelGGL(k_tl_stage2, dim3(unsigned(nb2)), dim3(b2), 0, st, R2, H, D, S, ev->tlCand.p, ev->tlPair.p,
+                           ev->tlCarry.p, ev->tlFlag.p, ev->tlCount.p, ev->tlNm1);
     if (int r = scanExclusive<int>(ev->tlCount.p, ev->tlStart.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TL)) return r;
     const long long nTL = sizeOf(ev, TOT_TL);
     if (nTL < 0) return fail(SDL_ERR_HIP, "readback failed");
