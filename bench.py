@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--config1-batch", type=int, default=32,
                    help="events per batch of the plain-vs-learned leg on 500-track barrel events (workload 1, "
                         "proposal_w1.json), 0 skips it")
+    p.add_argument("--scaling", default="1,4,16,64",
+                   help="events per step of the batch-scaling leg on the main workload (comma list, empty skips it)")
     p.add_argument("--families", type=int, default=1,
                    help="0 skips the timing of the other stage families (Nm1, navigation, pseudo MDs, layer pairs)")
     return p.parse_args()
@@ -326,6 +328,24 @@ def plain_vs_learned(a, geom, workload, nevents, batch, steps, weights=None, rec
             "tl_work_last_batch": work, "timed_calls": calls}
 
 
+def scaling_leg(a, geom):
+    """The plain chain on batches of 1, 4, 16, 64 ... events of the main
+    workload (seeds 0 .. n-1), one batch per step: events/s against the batch
+    size."""
+    from sdl_amd import runtime
+    sizes = [int(x) for x in a.scaling.split(",") if x.strip()]
+    evs = generate(a.workload, 0, max(sizes))
+    ev = runtime.Event(geom)
+    rows = []
+    for n in sizes:
+        b = Batch(evs[:n])
+        sec, st = timed(ev, lambda: b.run(ev), 3, 5)
+        rows.append({"events_per_step": n, "events_per_s": round(n / sec, 3), "ms_per_step": round(1000 * sec, 3),
+                     "timed_calls": st})
+        b.free()
+    return {"workload": f"workload {a.workload}: the first n events as one batch per step", "rows": rows}
+
+
 def config1_leg(a, geom):
     """500-track barrel events (workload 1, seeds from 10000: neither training
     nor validation seeds), --config1-batch events per batch, 4 batches: plain
@@ -489,6 +509,8 @@ def main():
         lb = plain_vs_learned(a, geom, a.workload, a.learned_batch, a.learned_batch, 3)
         lb["workload"] = f"workload {a.workload}: {a.learned_batch} events as one batch per step"
         out["learned_batched"] = lb
+    if a.scaling.strip():
+        out["batch_scaling"] = scaling_leg(a, geom)
     if a.batch > 0:
         out["batched"] = batch_leg(a, runtime.Geometry(*gen.geometry(gen.geom_kind(a.batch_workload))[1]))
     if a.config1_batch > 0:

@@ -558,8 +558,16 @@ int scanExclusive(const TIn* in, long long* out, Count cn, long long nMax, ScanS
 }
 
 // ------------------------------------------------------------------ hits
+// detId -> module index as an open-addressing hash table (linear probing, a
+// power-of-two size of at least 4 NM slots, empty slots holding kNoDet): a hit
+// finds its module in one or two loads instead of a binary search over the
+// sorted detIds (15 dependent loads for 27 K modules).
+constexpr uint32_t kNoDet = 0xFFFFFFFFu;
+__host__ __device__ inline uint32_t detHash(uint32_t d, int bits) { return (d * 2654435761u) >> (32 - bits); }
+
 __global__ void k_hit_prep(const float* x, const float* y, const float* z, const unsigned* detId,
-                           const long long* evOff, int E, const uint32_t* sortedDet, int NM, const ModDev* mods,
+                           const long long* evOff, int E, const uint2* detTable, int hashBits, int NM,
+                           const ModDev* mods,
                            float* rt, float* r3, float* phi, float* alpha, int* slot, float4* eHi, float4* eLo,
                            int* slotCount, long long* tot) {
     const long long nh = evOff[E];  // hits of the batch (uploaded with the event offsets)
@@ -572,14 +580,19 @@ __global__ void k_hit_prep(const float* x, const float* y, const float* z, const
         if (evOff[mid] <= i) lo = mid; else hi = mid;
     }
     const int e = lo;
-    // module of the hit: binary search of the sorted detIds
+    // module of the hit (the table always has empty slots, so every probe ends)
     const uint32_t d = detId[i];
-    int a = 0, b = NM;
-    while (a < b) {
-        int mid = (a + b) >> 1;
-        if (sortedDet[mid] < d) a = mid + 1; else b = mid;
+    const uint32_t mask = (1u << hashBits) - 1u;
+    int a = -1;
+    for (uint32_t h = detHash(d, hashBits); d != kNoDet; h = (h + 1u) & mask) {
+        const uint2 t = detTable[h];
+        if (t.x == d) {
+            a = int(t.y);
+            break;
+        }
+        if (t.x == kNoDet) break;
     }
-    if (a >= NM || sortedDet[a] != d) {
+    if (a < 0 || a >= NM) {
         atomicAdd((unsigned long long*)&tot[TOT_UNKNOWN], 1ULL);
         slot[i] = -1;
         return;
@@ -2391,7 +2404,8 @@ struct sdl_geom {
     std::vector<uint32_t> detIds;
     std::vector<ModDev> mods;
     std::vector<int> connStart, connList;
-    uint32_t* dDet = nullptr;
+    uint2* dDetTable = nullptr;  // detId -> module hash table (detHash)
+    int hashBits = 0;
     ModDev* dMods = nullptr;
     int *dConnStart = nullptr, *dConnList = nullptr;
     // modules ordered by layer bin (then index), and where each bin starts
@@ -2793,7 +2807,7 @@ int runPrep(sdl_event* ev) {
     const long long nhGrid = (long long)ev->rt.cap;
     timerBegin(ev, T_PREP);
     hipLaunchKernelGGL(k_hit_prep, dim3(gridFor(nhGrid, kBlock)), dim3(kBlock), 0, st, ev->px, ev->py, ev->pz, ev->pdet,
-                       ev->dEvOff.p, ev->E, G->dDet, G->NM, G->dMods, ev->rt.p, ev->r3.p, ev->phi.p, ev->halpha.p,
+                       ev->dEvOff.p, ev->E, G->dDetTable, G->hashBits, G->NM, G->dMods, ev->rt.p, ev->r3.p, ev->phi.p, ev->halpha.p,
                        ev->hslot.p, ev->eHi.p, ev->eLo.p, ev->slotHitCount.p, ev->tot.p);
     timerEnd(ev, T_PREP);
     if (int r = scanExclusive<int>(ev->slotHitCount.p, ev->slotHitStart.p, hostCount(ev, NS), NS, ev->scan, st,
@@ -3715,7 +3729,17 @@ int uploadGeom(sdl_geom* G) {
         std::vector<int> fill(layerModStart.begin(), layerModStart.end() - 1);
         for (int m = 0; m < G->NM; ++m) layerMods[fill[layerBin(G->mods[m])]++] = m;
     }
-    int r = up((void**)&G->dDet, G->detIds.data(), sizeof(uint32_t) * G->detIds.size());
+    int bits = 4;
+    while ((1LL << bits) < 4LL * G->NM) ++bits;
+    std::vector<uint2> table(size_t(1) << bits, make_uint2(kNoDet, 0u));
+    for (int m = 0; m < G->NM; ++m) {
+        if (G->detIds[m] == kNoDet) return fail(SDL_ERR_IO, "module detId 0xFFFFFFFF (reserved)");
+        uint32_t h = detHash(G->detIds[m], bits);
+        while (table[h].x != kNoDet) h = (h + 1u) & uint32_t(table.size() - 1);
+        table[h] = make_uint2(G->detIds[m], uint32_t(m));
+    }
+    G->hashBits = bits;
+    int r = up((void**)&G->dDetTable, table.data(), sizeof(uint2) * table.size());
     if (!r) r = up((void**)&G->dMods, G->mods.data(), sizeof(ModDev) * G->mods.size());
     if (!r) r = up((void**)&G->dConnStart, G->connStart.data(), sizeof(int) * G->connStart.size());
     if (!r) r = up((void**)&G->dConnList, G->connList.data(), sizeof(int) * G->connList.size());
@@ -3973,7 +3997,7 @@ int sdl_geom_load_binary(const char* binaryPath, sdl_geom** out) {
 
 void sdl_geom_free(sdl_geom* g) {
     if (!g) return;
-    if (g->dDet) (void)hipFree(g->dDet);
+    if (g->dDetTable) (void)hipFree(g->dDetTable);
     if (g->dMods) (void)hipFree(g->dMods);
     if (g->dConnStart) (void)hipFree(g->dConnStart);
     if (g->dConnList) (void)hipFree(g->dConnList);
