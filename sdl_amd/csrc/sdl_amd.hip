@@ -886,16 +886,20 @@ constexpr int kNBins = 11;
 
 // Flattened walk over the connected modules of one work item: returns the
 // connection entry holding candidate k and the index inside it.  pre holds the
-// wave-wide inclusive prefix of per-entry counts (entry j in lane j).  Must be
-// called by all 64 lanes (the shuffles read every lane).
+// wave-wide inclusive prefix of per-entry counts (entry j in lane j, 1 <=
+// nEntries <= 64), so the entries with pre <= k are a prefix [0, e): a binary
+// search of six shuffles finds e.  Must be called by all 64 lanes (the
+// shuffles read every lane).
 __device__ __forceinline__ int findEntry(int k, int pre, int nEntries, int& inner) {
     int e = 0;
-    int prev = 0;
-    for (int j = 0; j < nEntries; ++j) {
-        const int pj = __shfl(pre, j);
-        if (k >= pj) { e = j + 1; prev = pj; }
+#pragma unroll
+    for (int step = kWave / 2; step >= 1; step >>= 1) {
+        const int j = e + step - 1;
+        const int pj = __shfl(pre, min(j, nEntries - 1));
+        if (j < nEntries && pj <= k) e += step;
     }
-    inner = k - prev;
+    const int prev = __shfl(pre, max(e - 1, 0));
+    inner = k - (e > 0 ? prev : 0);
     return e;
 }
 
