@@ -884,21 +884,22 @@ __host__ __device__ __forceinline__ int layerBin(const ModDev& m) {
 constexpr int kNCounters = 10;
 constexpr int kNBins = 11;
 
-// Flattened walk over the connected modules of one work item: returns the
-// connection entry holding candidate k and the index inside it.  pre holds the
-// wave-wide inclusive prefix of per-entry counts (entry j in lane j, 1 <=
-// nEntries <= 64), so the entries with pre <= k are a prefix [0, e): a binary
-// search of six shuffles finds e.  Must be called by all 64 lanes (the
-// shuffles read every lane).
+// Flattened walk over the connected modules of one work item, W lanes per
+// item: returns the connection entry holding candidate k and the index inside
+// it.  pre holds the group's inclusive prefix of per-entry counts (entry j in
+// lane j of the group, 1 <= nEntries <= W), so the entries with pre <= k are a
+// prefix [0, e): a binary search of log2(W) shuffles finds e.  Must be called
+// by all W lanes of the group (the shuffles read every lane of it).
+template <int W>
 __device__ __forceinline__ int findEntry(int k, int pre, int nEntries, int& inner) {
     int e = 0;
 #pragma unroll
-    for (int step = kWave / 2; step >= 1; step >>= 1) {
+    for (int step = W / 2; step >= 1; step >>= 1) {
         const int j = e + step - 1;
-        const int pj = __shfl(pre, min(j, nEntries - 1));
+        const int pj = __shfl(pre, min(j, nEntries - 1), W);
         if (j < nEntries && pj <= k) e += step;
     }
-    const int prev = __shfl(pre, max(e - 1, 0));
+    const int prev = __shfl(pre, max(e - 1, 0), W);
     inner = k - (e > 0 ? prev : 0);
     return e;
 }
@@ -919,12 +920,16 @@ __device__ __forceinline__ bool sgCandidate(const MDView& in, const ModDev& mi, 
     return sgAlgo(in, out, mi, mO, connFound, hiE, loE, o, rec);
 }
 
-// Appends the wave's passing SG candidates, in lane order, to the inner MD's slab.
+// Appends the passing SG candidates of a group of W lanes (the wave, or one
+// half of it), in lane order, to the inner MD's slab; gl = lane in the group.
+template <int W = kWave>
 __device__ __forceinline__ void appendSG(bool pass, int lane, int& base, long long off, int mdOut, const SGOut& o,
                                          int* slabOut, SGF* slabF) {
-    const unsigned long long mask = __ballot(pass);
+    const int gl = lane & (W - 1);
+    unsigned long long mask = __ballot(pass) >> (lane - gl);
+    if constexpr (W < kWave) mask &= (1ull << W) - 1ull;
     if (pass) {
-        const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+        const int pos = base + __popcll(mask & ((1ull << gl) - 1ull));
         slabOut[off + pos] = mdOut;
         SGF f;
         f.v[0] = o.dphi; f.v[1] = o.dphimin; f.v[2] = o.dphimax;
@@ -934,18 +939,24 @@ __device__ __forceinline__ void appendSG(bool pass, int lane, int& base, long lo
     base += __popcll(mask);
 }
 
+// One group of kSgLanes lanes per inner MD (two MDs per wave: half of the MDs
+// of a PU200 event have fewer than 30 candidates, so a wave per MD left about
+// 40 % of its lanes idle); the lanes walk the flattened (connected module,
+// outer MD) space.
+constexpr int kSgLanes = 32;
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) k_sg(Count cMD, int NM, const ModDev* mods, HitsDev H, MDDev D,
                                                const int* connStart, const int* connList, const long long* mdStart,
                                                const int* mdCount, const int* sgCap, const long long* sgCapOff,
                                                int* slabOut, SGF* slabF, int* sgCount, int allComb, Guard g) {
     if (g.over()) return;
     const int nMD = int(cMD.get());
-    const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (item >= nMD) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int gl = lane & (kSgLanes - 1);
+    const int item = int(((long long)blockIdx.x * blockDim.x + threadIdx.x) / kSgLanes);
+    if (item >= nMD) return;  // uniform over the group
     const int cap = sgCap[item];
     if (cap == 0) {
-        if (lane == 0) sgCount[item] = 0;
+        if (gl == 0) sgCount[item] = 0;
         return;
     }
     const int sIn = D.slot[item];
@@ -955,34 +966,34 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
     const int c0 = connStart[mIn], nc = connStart[mIn + 1] - c0;
     const long long off = sgCapOff[item];
     int base = 0;
-    // connection entries are processed in chunks of 64 (one per lane)
-    for (int cb = 0; cb < nc; cb += kWave) {
-        const int ne = min(kWave, nc - cb);
+    // connection entries are processed in chunks of kSgLanes (one per lane)
+    for (int cb = 0; cb < nc; cb += kSgLanes) {
+        const int ne = min(kSgLanes, nc - cb);
         int cnt = 0;
-        if (lane < ne) cnt = mdCount[eb + connList[c0 + cb + lane]];
-        // inclusive prefix over lanes
+        if (gl < ne) cnt = mdCount[eb + connList[c0 + cb + gl]];
+        // inclusive prefix over the group's lanes
         int pre = cnt;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int t = __shfl_up(pre, o);
-            if (lane >= o) pre += t;
+        for (int o = 1; o < kSgLanes; o <<= 1) {
+            const int t = __shfl_up(pre, o, kSgLanes);
+            if (gl >= o) pre += t;
         }
-        const int total = __shfl(pre, kWave - 1);
-        for (int k0 = 0; k0 < total; k0 += kWave) {
-            const int k = k0 + lane;
+        const int total = __shfl(pre, kSgLanes - 1, kSgLanes);
+        for (int k0 = 0; k0 < total; k0 += kSgLanes) {
+            const int k = k0 + gl;
             bool pass = false;
             SGOut o{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // AllComb leaves the constructor's values (Segment.cc:35-56)
             int mdOut = 0;
             int inner;
-            const int e = findEntry(k, pre, ne, inner);
+            const int e = findEntry<kSgLanes>(k, pre, ne, inner);
             if (k < total) {
                 const int modOut = connList[c0 + cb + e];
                 mdOut = int(mdStart[eb + modOut]) + inner;
                 pass = allComb || sgCandidate(in, mi, mdOut, mods[modOut], true, H, D, o);
             }
-            appendSG(pass, lane, base, off, mdOut, o, slabOut, slabF);
+            appendSG<kSgLanes>(pass, lane, base, off, mdOut, o, slabOut, slabF);
         }
     }
-    if (lane == 0) sgCount[item] = base;
+    if (gl == 0) sgCount[item] = base;
 }
 
 // ------------------------------------------------------- layer-pair form
@@ -2925,7 +2936,7 @@ int runSG(sdl_event* ev, bool layers) {
     if (int r = ev->sgSlabF.ensure(slab)) return r;
     timerBegin(ev, T_SG);
     if (nMD > 0 && !layers)
-        hipLaunchKernelGGL(k_sg, dim3(gridFor(nMD, ev->tune.v[TUNE_SG_BLOCK] / kWave)), dim3(ev->tune.v[TUNE_SG_BLOCK]), 0,
+        hipLaunchKernelGGL(k_sg, dim3(gridFor(nMD, ev->tune.v[TUNE_SG_BLOCK] / kSgLanes)), dim3(ev->tune.v[TUNE_SG_BLOCK]), 0,
                            st, cMD, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), G->dConnStart, G->dConnList, ev->mdStart.p, ev->mdCount.p,
                            ev->sgCap.p, ev->sgCapOff.p, ev->sgSlabOut.p, ev->sgSlabF.p, ev->sgCount.p,
