@@ -1144,16 +1144,18 @@ __global__ void k_sg_extra(Count cSG, int NM, const ModDev* mods, HitsDev H, MDD
     S.e[s] = e;
 }
 
+// kRepackLanes lanes per inner MD (a PU200 MD stores about four SGs).
+constexpr int kRepackLanes = 8;
 __global__ void k_sg_repack(Count cMD, const long long* sgCapOff, const int* sgCount, const long long* sgStart,
                             const int* slabOut, const SGF* slabF, SGDev S, Guard g) {
     if (g.over()) return;
     const int nMD = int(cMD.get());
-    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    const int item = int(((long long)blockIdx.x * blockDim.x + threadIdx.x) / kRepackLanes);
+    const int gl = threadIdx.x & (kRepackLanes - 1);
     if (item >= nMD) return;
     const int n = sgCount[item];
     const long long src = sgCapOff[item], dst = sgStart[item];
-    for (int k = lane; k < n; k += kWave) {
+    for (int k = gl; k < n; k += kRepackLanes) {
         S.in[dst + k] = item;
         S.out[dst + k] = slabOut[src + k];
         S.f[dst + k] = slabF[src + k];
@@ -2960,7 +2962,7 @@ int runSG(sdl_event* ev, bool layers) {
     if (int r = ev->sgE.ensure(ns)) return r;
     const Count cSG = devCount(ev, TOT_SG);
     if (nMD > 0)
-        hipLaunchKernelGGL(k_sg_repack, dim3(gridFor(nMD, kWavesPerBlock)), dim3(kBlock), 0, st, cMD,
+        hipLaunchKernelGGL(k_sg_repack, dim3(gridFor(nMD * kRepackLanes, kBlock)), dim3(kBlock), 0, st, cMD,
                            ev->sgCapOff.p, ev->sgCount.p, ev->sgStart.p, ev->sgSlabOut.p, ev->sgSlabF.p, sgView(ev),
                            guardOf(ev, TOT_SG, nSG, SDL_STAGE_SG));
     if (nSG > 0)
