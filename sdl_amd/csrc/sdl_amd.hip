@@ -1377,7 +1377,7 @@ __global__ void k_slot_md_sg_list(Count cNS, const long long* mdStart, const lon
 // Lane k of the group takes connected module k (in rounds of kEnumLanes); one
 // scan of the per-variant counts packed into 21-bit fields places every
 // module's run, and the lanes then write the round's pairs contiguously.
-constexpr int kEnumLanes = 16;
+constexpr int kEnumLanes = 8;
 __global__ void __launch_bounds__(kBlock) k_tl_enum(Count cSG, int NM, const ModDev* mods, const int* sgOut,
                                                     const int* sgFlags, const int* mdSlot, const int* connStart,
                                                     const int* connList, const long long* listStart, const int* list,
@@ -2188,7 +2188,7 @@ __device__ __forceinline__ void binCount(unsigned long long* bins, int key) {
 // TCs from triplets, kTcpLanes lanes per tracklet (grid-stride over tracklets):
 // the lanes copy the tracklet's inner TPs.  Each TC counts at the module of its
 // inner TP (Event.cc:893-897,965) in a per-block LDS histogram.
-constexpr int kTcpLanes = 8;
+constexpr int kTcpLanes = 16;
 __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long* start, const int2* info,
                                                       const int* tpEndList, const int* tpKey, int E,
                                                       int2* tc, unsigned long long* counters, Guard g) {
