@@ -631,8 +631,12 @@ __global__ void k_hit_scatter(const long long* evOff, int E, const int* slot, co
 // (Module::addHit appends, Module.cc:128-137) and makes atomically scattered
 // lists deterministic.  kSortLanes lanes per segment (segments hold a few
 // elements on average); up to 4 * kSortLanes elements are ranked in registers,
-// longer segments are sorted by the group's first lane.
-constexpr int kSortLanes = 16;
+// longer segments are sorted by the group's first lane.  The hit and SG lists
+// use 16 lanes; the tracklet lists (two or three per inner SG on average at
+// PU200 density) use 8, where fewer idle lanes outweigh the rare list longer
+// than 32 that one lane sorts.
+constexpr int kSortLanesWide = 16, kSortLanesTL = 8;
+template <int kSortLanes>
 __global__ void __launch_bounds__(kBlock) k_sort_segments(Count cseg, const long long* start, int* vals) {
     const int nseg = int(cseg.get());
     const int gl = threadIdx.x & (kSortLanes - 1);
@@ -2832,7 +2836,8 @@ int runPrep(sdl_event* ev) {
         return r;
     hipLaunchKernelGGL(k_hit_scatter, dim3(gridFor(nhGrid, kBlock)), dim3(kBlock), 0, st, ev->dEvOff.p, ev->E,
                        ev->hslot.p, ev->slotHitStart.p, ev->slotFill.p, ev->sortedHit.p);
-    hipLaunchKernelGGL(k_sort_segments, dim3(gridFor((long long)NS * kSortLanes, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+    hipLaunchKernelGGL(k_sort_segments<kSortLanesWide>, dim3(gridFor((long long)NS * kSortLanesWide, kBlock)), dim3(kBlock), 0, st,
+                       hostCount(ev, NS),
                        ev->slotHitStart.p, ev->sortedHit.p);
     ev->hitsReady = true;
     return 0;
@@ -2974,7 +2979,8 @@ int runSG(sdl_event* ev, bool layers) {
     if (nSG > 0) {
         hipLaunchKernelGGL(k_scatter_by, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
                            ev->sgEndStart.p, ev->sgEndFill.p, ev->sgEndList.p);
-        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nMD * kSortLanes, kBlock)), dim3(kBlock), 0, st, cMD,
+        hipLaunchKernelGGL(k_sort_segments<kSortLanesWide>, dim3(gridFor(nMD * kSortLanesWide, kBlock)), dim3(kBlock), 0, st,
+                           cMD,
                            ev->sgEndStart.p, ev->sgEndList.p);
     }
     return 0;
@@ -3200,7 +3206,8 @@ int runTL(sdl_event* ev) {
                            devCount(ev, TOT_TLCANDS), ev->tlCand.p, ev->tlPair.p, ev->tlFlag.p, ev->tlStart.p,
                            ev->tlFill.p, ev->tlIn.p, ev->tlOut.p, guardOf(ev, TOT_TL, nTL, SDL_STAGE_TL));
     if (nSG > 0)
-        hipLaunchKernelGGL(k_sort_segments, dim3(gridFor(nSG * kSortLanes, kBlock)), dim3(kBlock), 0, st, cSG,
+        hipLaunchKernelGGL(k_sort_segments<kSortLanesTL>, dim3(gridFor(nSG * kSortLanesTL, kBlock)), dim3(kBlock), 0, st,
+                           cSG,
                            ev->tlStart.p, ev->tlOut.p);
     hipLaunchKernelGGL(k_tl_bounds, dim3(1), dim3(64), 0, st, ev->tlPOff.p, ev->tlCOff.p, ns, ev->tot.p);
     timerEnd(ev, T_TL);
