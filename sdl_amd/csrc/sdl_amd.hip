@@ -1977,20 +1977,24 @@ __global__ void k_tpe_count(Count cSG, const int* sgEndList, const long long* sg
     const int mx = sgIn[sgEndList[k]];
     cnt[k] = int(sgEndStart[mx + 1] - sgEndStart[mx]);
 }
-// One thread per listed TP (nTP of them in total).
+// kTpeLanes lanes per listed SG write its TPs (nTP in total; the two totals
+// are equal, every TP (s1, s2) is listed once).
+constexpr int kTpeLanes = 8;
 __global__ void k_tpe_write(Count cTP, Count cSG, const long long* tpeOff, const int* sgEndList,
                             const long long* sgEndStart, const int* sgIn, const long long* sgStart,
                             const long long* tpStart, int* tpEndList, Guard g) {
     if (g.over()) return;
     const long long nTP = cTP.get();
     const int nSG = int(cSG.get());
-    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nTP || j >= tpeOff[nSG]) return;  // the two totals are equal (every TP (s1, s2) is listed once)
-    const long long k = rangeOf(tpeOff, nSG, j);
+    const int gl = threadIdx.x & (kTpeLanes - 1);
+    const long long k = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kTpeLanes;
+    if (k >= nSG) return;
+    const long long a = tpeOff[k], e = min(tpeOff[k + 1], nTP);
+    if (a + gl >= e) return;
     const int s2 = sgEndList[k];
     const int mx = sgIn[s2];
-    const int s1 = sgEndList[sgEndStart[mx] + (j - tpeOff[k])];
-    tpEndList[j] = int(tpStart[s1] + (s2 - sgStart[mx]));
+    const long long b1 = sgEndStart[mx], d = s2 - sgStart[mx];
+    for (long long j = a + gl; j < e; j += kTpeLanes) tpEndList[j] = int(tpStart[sgEndList[b1 + (j - a)]] + d);
 }
 // Per tracklet: the bridge test (s3 exists), the number of TCs and where their
 // inner TPs are listed; info = (first listed inner TP, outer TP).  tlDefault
@@ -3232,8 +3236,8 @@ int runTCP(sdl_event* ev) {
         hipLaunchKernelGGL(k_tpe_count, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgEndList.p,
                            ev->sgEndStart.p, ev->sgIn.p, ev->tpeCount.p);
     if (int r = scanExclusive<int>(ev->tpeCount.p, ev->tpeOff.p, cSG, nSG, ev->scan, st, ev->tot.p, TOT_TPE)) return r;
-    if (nTP > 0)
-        hipLaunchKernelGGL(k_tpe_write, dim3(gridFor(nTP, kBlock)), dim3(kBlock), 0, st, cTP, cSG, ev->tpeOff.p,
+    if (nTP > 0 && nSG > 0)
+        hipLaunchKernelGGL(k_tpe_write, dim3(gridFor(nSG * kTpeLanes, kBlock)), dim3(kBlock), 0, st, cTP, cSG, ev->tpeOff.p,
                            ev->sgEndList.p, ev->sgEndStart.p, ev->sgIn.p, ev->sgStart.p, ev->tpStart.p, ev->tpEndList.p,
                            guardOf(ev, TOT_TPE, nTP, SDL_STAGE_TCP));
     const unsigned char* tlDefault = nullptr;
