@@ -1166,19 +1166,32 @@ __global__ void k_sg_repack(Count cMD, const long long* sgCapOff, const int* sgC
     }
 }
 
+// Per-slot kernels over the MDs of a slot: kSlotLanes lanes per slot, one MD
+// each per round; a ballot over the group keeps MD order.
+constexpr int kSlotLanes = 8;
+__device__ __forceinline__ unsigned long long slotGroupMask(bool p) {
+    const int lane = threadIdx.x & (kWave - 1), gl = threadIdx.x & (kSlotLanes - 1);
+    return (__ballot(p) >> (lane - gl)) & ((1ull << kSlotLanes) - 1ull);
+}
+
 // Per-slot ranges of objects ordered by an inner MD: first = start[mdStart[slot]].
 // Per slot: where its SGs start, and how many of its MDs start at least one SG
 // (the tracklet pairs' outer MDs).
 __global__ void k_slot_range(Count cNS, const long long* mdStart, const long long* sgStart, long long* slotSgStart,
                              int* slotMdSg) {
     const int NS = int(cNS.get());
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > NS) return;
-    slotSgStart[s] = sgStart[mdStart[s]];
+    const int gl = threadIdx.x & (kSlotLanes - 1);
+    const long long s = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kSlotLanes;
+    if (s > NS) return;  // uniform over the group
+    if (gl == 0) slotSgStart[s] = sgStart[mdStart[s]];
     if (s == NS) return;
     int c = 0;
-    for (long long m = mdStart[s]; m < mdStart[s + 1]; ++m) c += sgStart[m + 1] > sgStart[m];
-    slotMdSg[s] = c;
+    const long long m0 = mdStart[s], m1 = mdStart[s + 1];
+    for (long long mb = m0; mb < m1; mb += kSlotLanes) {
+        const long long m = mb + gl;
+        c += __popcll(slotGroupMask(m < m1 && sgStart[m + 1] > sgStart[m]));
+    }
+    if (gl == 0) slotMdSg[s] = c;
 }
 
 // SGs ending at each MD (reverse index, used by the joins)
@@ -1369,11 +1382,18 @@ struct VarRange {
 __global__ void k_slot_md_sg_list(Count cNS, const long long* mdStart, const long long* sgStart,
                                   const long long* listStart, int* list) {
     const int NS = int(cNS.get());
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= NS) return;
+    const int gl = threadIdx.x & (kSlotLanes - 1);
+    const long long t = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kSlotLanes;
+    if (t >= NS) return;  // uniform over the group
     long long d = listStart[t];
-    for (long long m = mdStart[t]; m < mdStart[t + 1]; ++m)
-        if (sgStart[m + 1] > sgStart[m]) list[d++] = int(m);
+    const long long m0 = mdStart[t], m1 = mdStart[t + 1];
+    for (long long mb = m0; mb < m1; mb += kSlotLanes) {  // the same trips for the whole group
+        const long long m = mb + gl;
+        const bool has = m < m1 && sgStart[m + 1] > sgStart[m];
+        const unsigned long long g = slotGroupMask(has);
+        if (has) list[d + __popcll(g & ((1ull << gl) - 1ull))] = int(m);
+        d += __popcll(g);
+    }
 }
 
 // A group of kEnumLanes lanes per inner SG writes its pairs, in connected-module
@@ -2977,7 +2997,7 @@ int runSG(sdl_event* ev, bool layers) {
     if (nSG > 0)
         hipLaunchKernelGGL(k_sg_extra, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
                            hitsView(ev), mdView(ev), sgView(ev), ev->sgStart.p, ev->sgEndCount.p, ev->tpCount.p);
-    hipLaunchKernelGGL(k_slot_range, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+    hipLaunchKernelGGL(k_slot_range, dim3(gridFor((NS + 1LL) * kSlotLanes, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
                        ev->mdStart.p, ev->sgStart.p, ev->slotSgStart.p, ev->slotMdSg.p);
     if (int r = scanExclusive<int>(ev->sgEndCount.p, ev->sgEndStart.p, cMD, nMD, ev->scan, st, ev->tot.p, -1)) return r;
     if (nSG > 0) {
@@ -3099,7 +3119,8 @@ int runTL(sdl_event* ev) {
     if (int r = scanExclusive<int>(ev->slotMdSg.p, ev->slotMdSgStart.p, hostCount(ev, NS), NS, ev->scan, st, ev->tot.p,
                                    -1))
         return r;
-    hipLaunchKernelGGL(k_slot_md_sg_list, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
+    hipLaunchKernelGGL(k_slot_md_sg_list, dim3(gridFor((long long)NS * kSlotLanes, kBlock)), dim3(kBlock), 0, st,
+                       hostCount(ev, NS),
                        ev->mdStart.p, ev->sgStart.p, ev->slotMdSgStart.p, ev->slotMdSgList.p);
     hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(ns, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods, ev->sgOut.p,
                        ev->sgFlags.p, ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->slotMdSg.p,
