@@ -863,16 +863,21 @@ __device__ __forceinline__ HitView loadAnchor(int i, const MDDev& D) {
 
 // ------------------------------------------------------------------ SG
 // SG candidates of one inner MD: MDs of every connected lower module (Event.cc:457-522).
+// kSgCapLanes lanes per MD, one connection each per round.
+constexpr int kSgCapLanes = 8;
 __global__ void k_sg_cap(Count cMD, int NM, const int* mdSlot, const int* connStart, const int* connList,
                          const int* mdCount, int* sgCap) {
     const int nMD = int(cMD.get());
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nMD) return;
+    const int gl = threadIdx.x & (kSgCapLanes - 1);
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kSgCapLanes;
+    if (i >= nMD) return;  // uniform over the group
     const int s = mdSlot[i];
     const int m = s % NM, eb = s - m;
     int c = 0;
-    for (int k = connStart[m]; k < connStart[m + 1]; ++k) c += mdCount[eb + connList[k]];
-    sgCap[i] = c;
+    for (int k = connStart[m] + gl; k < connStart[m + 1]; k += kSgCapLanes) c += mdCount[eb + connList[k]];
+#pragma unroll
+    for (int off = 1; off < kSgCapLanes; off <<= 1) c += __shfl_xor(c, off, kSgCapLanes);
+    if (gl == 0) sgCap[i] = c;
 }
 
 struct SGF {
@@ -1325,14 +1330,17 @@ __device__ __forceinline__ int tlModuleVariant(int innerFlags, const ModDev& out
 }
 
 // Per inner SG: TL candidates (for the counters) and pairs per variant.
+// kCapLanes lanes per SG, one connection each per round, summed by shuffles.
+constexpr int kCapLanes = 4;
 __global__ void k_tl_cap(Count cSG, int NM, const ModDev* mods, const int* sgOut, const int* sgFlags,
                          const int* mdSlot, const int* connStart, const int* connList, const long long* slotSgStart,
                          const int* slotMdSg, long long* tlCap, int* nPair, long long stride) {
     const int nSG = int(cSG.get());
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= stride) return;
+    const int gl = threadIdx.x & (kCapLanes - 1);
+    const long long s = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kCapLanes;
+    if (s >= stride) return;  // uniform over the group
     if (s >= nSG) {  // the three per-variant rows are scanned as one array
-        for (int v = 0; v < kNVariantsTL; ++v) nPair[v * stride + s] = 0;
+        if (gl < kNVariantsTL) nPair[gl * stride + s] = 0;
         return;
     }
     const int b = mdSlot[sgOut[s]];
@@ -1340,14 +1348,22 @@ __global__ void k_tl_cap(Count cSG, int NM, const ModDev* mods, const int* sgOut
     const int fl = sgFlags[s];
     long long c = 0;
     int np[kNVariantsTL] = {0, 0, 0};
-    for (int k = connStart[m]; k < connStart[m + 1]; ++k) {
+    for (int k = connStart[m] + gl; k < connStart[m + 1]; k += kCapLanes) {
         const int t = eb + connList[k];
         c += slotSgStart[t + 1] - slotSgStart[t];
         const int v = tlModuleVariant(fl, mods[connList[k]]);
         if (v < kNVariantsTL) np[v] += slotMdSg[t];
     }
-    tlCap[s] = c;
-    for (int v = 0; v < kNVariantsTL; ++v) nPair[v * stride + s] = np[v];
+#pragma unroll
+    for (int off = 1; off < kCapLanes; off <<= 1) {
+        c += __shfl_xor(c, off, kCapLanes);
+#pragma unroll
+        for (int v = 0; v < kNVariantsTL; ++v) np[v] += __shfl_xor(np[v], off, kCapLanes);
+    }
+    if (gl == 0) {
+        tlCap[s] = c;
+        for (int v = 0; v < kNVariantsTL; ++v) nPair[v * stride + s] = np[v];
+    }
 }
 
 // The items of one variant in a list ordered by variant: [b, e) with
@@ -2955,7 +2971,8 @@ int runSG(sdl_event* ev, bool layers) {
     zeroMany(st, {span(ev->sgEndCount.p, nm), span(ev->sgEndFill.p, nm)});
     const Count cMD = devCount(ev, TOT_MD);
     if (nMD > 0 && !layers)
-        hipLaunchKernelGGL(k_sg_cap, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, cMD, G->NM, ev->mdSlot.p,
+        hipLaunchKernelGGL(k_sg_cap, dim3(gridFor(nMD * kSgCapLanes, kBlock)), dim3(kBlock), 0, st, cMD, G->NM,
+                           ev->mdSlot.p,
                            G->dConnStart, G->dConnList, ev->mdCount.p, ev->sgCap.p);
     if (nMD > 0 && layers)
         hipLaunchKernelGGL(k_sgl_cap, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, cMD, G->NM, G->dMods,
@@ -3122,7 +3139,8 @@ int runTL(sdl_event* ev) {
     hipLaunchKernelGGL(k_slot_md_sg_list, dim3(gridFor((long long)NS * kSlotLanes, kBlock)), dim3(kBlock), 0, st,
                        hostCount(ev, NS),
                        ev->mdStart.p, ev->sgStart.p, ev->slotMdSgStart.p, ev->slotMdSgList.p);
-    hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(ns, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods, ev->sgOut.p,
+    hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(ns * kCapLanes, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
+                       ev->sgOut.p,
                        ev->sgFlags.p, ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->slotMdSg.p,
                        ev->tlCap.p, ev->tlNP.p, ns);
     // pairs: one list, variant-major
