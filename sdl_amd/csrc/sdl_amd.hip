@@ -1215,15 +1215,19 @@ __global__ void k_scatter_by(Count cn, const int* key, const long long* start, i
 // are exactly the dense range of SGs whose inner MD is m.
 // tpKey: the counter key of the TP's inner SG (its layer bin), which the TC and
 // counter kernels would otherwise reach through tpIn.
+// kTpLanes lanes per inner SG write its triplets.
+constexpr int kTpLanes = 4;
 __global__ void k_tp_write(Count cSG, const int* sgOut, const long long* sgStart, const long long* tpStart,
                            const int* sgKey, int* tpIn, int* tpOutArr, int* tpKey, Guard g) {
     if (g.over()) return;
     const int nSG = int(cSG.get());
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nSG) return;
+    const int gl = threadIdx.x & (kTpLanes - 1);
+    const long long sl = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kTpLanes;
+    if (sl >= nSG) return;
+    const int s = int(sl);
     const int m = sgOut[s], key = sgKey[s];
     const long long a = sgStart[m], n = sgStart[m + 1] - a, d = tpStart[s];
-    for (long long k = 0; k < n; ++k) {
+    for (long long k = gl; k < n; k += kTpLanes) {
         tpIn[d + k] = s;
         tpOutArr[d + k] = int(a + k);
         tpKey[d + k] = key;
@@ -3053,7 +3057,7 @@ int runTP(sdl_event* ev) {
     if (int r = ev->tpOut.ensure(std::max(nTP, 1LL))) return r;
     if (int r = ev->tpKey.ensure(std::max(nTP, 1LL))) return r;
     if (nSG > 0 && !allComb)
-        hipLaunchKernelGGL(k_tp_write, dim3(gridFor(nSG, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
+        hipLaunchKernelGGL(k_tp_write, dim3(gridFor(nSG * kTpLanes, kBlock)), dim3(kBlock), 0, st, cSG, ev->sgOut.p,
                            ev->sgStart.p, ev->tpStart.p, ev->sgKey.p, ev->tpIn.p, ev->tpOut.p, ev->tpKey.p,
                            guardOf(ev, TOT_TP, nTP, SDL_STAGE_TP));
     if (nSG > 0 && allComb)
