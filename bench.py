@@ -55,7 +55,7 @@ def parse():
                    help="events per batch of the plain-vs-learned leg on 500-track barrel events (workload 1, "
                         "proposal_w1.json), 0 skips it")
     p.add_argument("--scaling", default="1,4,16,64",
-                   help="events per step of the batch-scaling leg on the main workload (comma list, empty skips it)")
+                   help="events per step of the batch-scaling leg on the main workload (comma list; 0 skips it)")
     p.add_argument("--families", type=int, default=1,
                    help="0 skips the timing of the other stage families (Nm1, navigation, pseudo MDs, layer pairs)")
     return p.parse_args()
@@ -509,7 +509,7 @@ def main():
         lb = plain_vs_learned(a, geom, a.workload, a.learned_batch, a.learned_batch, 3)
         lb["workload"] = f"workload {a.workload}: {a.learned_batch} events as one batch per step"
         out["learned_batched"] = lb
-    if a.scaling.strip():
+    if a.scaling.strip() not in ("", "0"):
         out["batch_scaling"] = scaling_leg(a, geom)
     if a.batch > 0:
         out["batched"] = batch_leg(a, runtime.Geometry(*gen.geometry(gen.geom_kind(a.batch_workload))[1]))
