@@ -686,9 +686,11 @@ __device__ __forceinline__ HitView loadHit(int h, const float* x, const float* y
 
 // ------------------------------------------------------------------ MD
 // Candidate count per lower slot: lower hits x upper hits (Event.cc:257-305).
-__global__ void k_md_cap(int NS, int NM, const ModDev* mods, const long long* slotHitStart, int* mdCap) {
+// (also clears the slot's MD count, which the MD kernel accumulates)
+__global__ void k_md_cap(int NS, int NM, const ModDev* mods, const long long* slotHitStart, int* mdCap, int* mdCount) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= NS) return;
+    mdCount[s] = 0;
     const int m = s % NM;
     const ModDev md = mods[m];
     int cap = 0;
@@ -707,9 +709,11 @@ __global__ void k_md_cap(int NS, int NM, const ModDev* mods, const long long* sl
 // all accepted (AllComb, MiniDoublet.cc:223-226).
 __device__ __forceinline__ bool pseudoLoopsLower(const ModDev& md) { return !(md.type == TYPE_PS && md.layerType != LT_PIXEL); }
 
-__global__ void k_pmd_cap(int NS, int NM, const ModDev* mods, const long long* slotHitStart, int* mdCap) {
+__global__ void k_pmd_cap(int NS, int NM, const ModDev* mods, const long long* slotHitStart, int* mdCap,
+                          int* mdCount) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= NS) return;
+    mdCount[s] = 0;
     const int m = s % NM;
     const ModDev md = mods[m];
     int cap = 0;
@@ -865,8 +869,9 @@ __device__ __forceinline__ HitView loadAnchor(int i, const MDDev& D) {
 // SG candidates of one inner MD: MDs of every connected lower module (Event.cc:457-522).
 // kSgCapLanes lanes per MD, one connection each per round.
 constexpr int kSgCapLanes = 8;
+// (also clears the MD's counts of the SGs ending at it, accumulated later)
 __global__ void k_sg_cap(Count cMD, int NM, const int* mdSlot, const int* connStart, const int* connList,
-                         const int* mdCount, int* sgCap) {
+                         const int* mdCount, int* sgCap, int* sgEndCount, int* sgEndFill) {
     const int nMD = int(cMD.get());
     const int gl = threadIdx.x & (kSgCapLanes - 1);
     const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kSgCapLanes;
@@ -877,7 +882,11 @@ __global__ void k_sg_cap(Count cMD, int NM, const int* mdSlot, const int* connSt
     for (int k = connStart[m] + gl; k < connStart[m + 1]; k += kSgCapLanes) c += mdCount[eb + connList[k]];
 #pragma unroll
     for (int off = 1; off < kSgCapLanes; off <<= 1) c += __shfl_xor(c, off, kSgCapLanes);
-    if (gl == 0) sgCap[i] = c;
+    if (gl == 0) {
+        sgCap[i] = c;
+        sgEndCount[i] = 0;
+        sgEndFill[i] = 0;
+    }
 }
 
 struct SGF {
@@ -1050,7 +1059,7 @@ __device__ __forceinline__ void layerRange(const long long* off, const int* laye
 
 // Per inner MD: SG candidates of the layer pairs that start at its layer.
 __global__ void k_sgl_cap(Count cMD, int NM, const ModDev* mods, const int* mdSlot, const long long* layerOff,
-                          const int* layerModStart, int* sgCap) {
+                          const int* layerModStart, int* sgCap, int* sgEndCount, int* sgEndFill) {
     const int nMD = int(cMD.get());
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nMD) return;
@@ -1064,6 +1073,8 @@ __global__ void k_sgl_cap(Count cMD, int NM, const ModDev* mods, const int* mdSl
             c += b - a;
         }
     sgCap[j] = int(c);
+    sgEndCount[j] = 0;  // counts of the SGs ending at the MD, accumulated later
+    sgEndFill[j] = 0;
 }
 
 // One wave per inner MD: the SG algorithm against every MD of each outer layer
@@ -1338,7 +1349,8 @@ __device__ __forceinline__ int tlModuleVariant(int innerFlags, const ModDev& out
 constexpr int kCapLanes = 4;
 __global__ void k_tl_cap(Count cSG, int NM, const ModDev* mods, const int* sgOut, const int* sgFlags,
                          const int* mdSlot, const int* connStart, const int* connList, const long long* slotSgStart,
-                         const int* slotMdSg, long long* tlCap, int* nPair, long long stride) {
+                         const int* slotMdSg, long long* tlCap, int* nPair, long long stride, int* tlCount,
+                         int* tlFill) {
     const int nSG = int(cSG.get());
     const int gl = threadIdx.x & (kCapLanes - 1);
     const long long s = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kCapLanes;
@@ -1367,6 +1379,8 @@ __global__ void k_tl_cap(Count cSG, int NM, const ModDev* mods, const int* sgOut
     if (gl == 0) {
         tlCap[s] = c;
         for (int v = 0; v < kNVariantsTL; ++v) nPair[v * stride + s] = np[v];
+        tlCount[s] = 0;  // accumulated by the second stage
+        tlFill[s] = 0;   // and by the scatter
     }
 }
 
@@ -1760,15 +1774,6 @@ __global__ void __launch_bounds__(kBlock) k_tl_stage2(VarRanges R, HitsDev H, MD
     else tlStage2Item<TL_EEEE>(R.r[v], blk, H, D, S, cand, pair, carry, flag, tlCount, nm1);
 }
 
-// Pairs and candidates per variant, for the next call's size predictions.
-__global__ void k_tl_bounds(const long long* pairOff, const long long* candOff, long long stride, long long* tot) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || tot[TOT_ABORT]) return;
-    for (int v = 0; v < kNVariantsTL; ++v) {
-        const long long p0 = pairOff[v * stride], p1 = pairOff[(v + 1) * stride];
-        tot[TOT_TLP0 + v] = p1 - p0;
-        tot[TOT_TLC0 + v] = candOff[p1] - candOff[p0];
-    }
-}
 
 // Survivors to their inner SG's range (the order inside a range is fixed
 // afterwards by sorting on the outer SG); four candidates per thread, whose
@@ -2271,12 +2276,22 @@ __global__ void __launch_bounds__(kBlock) k_tcp_write(Count cTL, const long long
 
 
 // Per-slot object starts for objects ordered by inner SG: slotStart[s] = start[slotSgStart[s]]
+// With pairOff set (the module-map tracklet stage), the thread of slot NS also
+// records the pairs and candidates per variant for the next call's size
+// predictions.
 __global__ void k_slot_range_sg(Count cNS, const long long* slotSgStart, const long long* objStartBySg,
-                                long long* slotObjStart) {
+                                long long* slotObjStart, const long long* pairOff = nullptr,
+                                const long long* candOff = nullptr, long long stride = 0, long long* tot = nullptr) {
     const int NS = int(cNS.get());
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s > NS) return;
     slotObjStart[s] = objStartBySg[slotSgStart[s]];
+    if (s != NS || !pairOff || tot[TOT_ABORT]) return;
+    for (int v = 0; v < kNVariantsTL; ++v) {
+        const long long p0 = pairOff[v * stride], p1 = pairOff[(v + 1) * stride];
+        tot[TOT_TLP0 + v] = p1 - p0;
+        tot[TOT_TLC0 + v] = candOff[p1] - candOff[p0];
+    }
 }
 
 // ------------------------------------------------------------------ export
@@ -2900,10 +2915,10 @@ int runMD(sdl_event* ev, bool pseudo) {
     if (int r = ev->mdStart.ensure(NS + 1)) return r;
     if (pseudo)
         hipLaunchKernelGGL(k_pmd_cap, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods,
-                           ev->slotHitStart.p, ev->mdCap.p);
+                           ev->slotHitStart.p, ev->mdCap.p, ev->mdCount.p);
     else
         hipLaunchKernelGGL(k_md_cap, dim3(gridFor(NS, kBlock)), dim3(kBlock), 0, st, NS, G->NM, G->dMods,
-                           ev->slotHitStart.p, ev->mdCap.p);
+                           ev->slotHitStart.p, ev->mdCap.p, ev->mdCount.p);
     if (int r = scanExclusive<int>(ev->mdCap.p, ev->mdCapOff.p, hostCount(ev, NS), NS, ev->scan, st, ev->tot.p,
                                    TOT_MDCAP))
         return r;
@@ -2915,7 +2930,6 @@ int runMD(sdl_event* ev, bool pseudo) {
     if (int r = ev->mdSlabSlot.ensure(std::max(slab, 1LL))) return r;
     if (int r = ev->mdPos.ensure(std::max(slab, 1LL) + 1)) return r;
     HitsDev H = hitsView(ev);
-    zeroAsync(ev->mdCount.p, NS, st);
     timerBegin(ev, T_MD);
     if (slab > 0 && pseudo)
         hipLaunchKernelGGL(k_pmd, dim3(gridFor(slab, kBlock)), dim3(kBlock), 0, st, devCount(ev, TOT_MDCAP), NS, G->NM,
@@ -2971,16 +2985,18 @@ int runSG(sdl_event* ev, bool layers) {
         if (int r = b->ensure(nm + 1)) return r;
     if (int r = ev->slotSgStart.ensure(NS + 1)) return r;
     if (int r = ev->slotMdSg.ensure(NS)) return r;
-    // reverse index (SGs by outer MD): counted by k_sg_extra, filled after its scan
-    zeroMany(st, {span(ev->sgEndCount.p, nm), span(ev->sgEndFill.p, nm)});
+    // reverse index (SGs by outer MD): cleared by the capacity kernel, counted by
+    // k_sg_extra, filled after its scan
     const Count cMD = devCount(ev, TOT_MD);
     if (nMD > 0 && !layers)
         hipLaunchKernelGGL(k_sg_cap, dim3(gridFor(nMD * kSgCapLanes, kBlock)), dim3(kBlock), 0, st, cMD, G->NM,
                            ev->mdSlot.p,
-                           G->dConnStart, G->dConnList, ev->mdCount.p, ev->sgCap.p);
+                           G->dConnStart, G->dConnList, ev->mdCount.p, ev->sgCap.p, ev->sgEndCount.p,
+                           ev->sgEndFill.p);
     if (nMD > 0 && layers)
         hipLaunchKernelGGL(k_sgl_cap, dim3(gridFor(nMD, kBlock)), dim3(kBlock), 0, st, cMD, G->NM, G->dMods,
-                           ev->mdSlot.p, ev->layerMdOff.p, G->dLayerModStart, ev->sgCap.p);
+                           ev->mdSlot.p, ev->layerMdOff.p, G->dLayerModStart, ev->sgCap.p, ev->sgEndCount.p,
+                           ev->sgEndFill.p);
     if (int r = scanExclusive<int>(ev->sgCap.p, ev->sgCapOff.p, cMD, nMD, ev->scan, st, ev->tot.p, TOT_SGCAP)) return r;
     const long long slab = sizeOf(ev, TOT_SGCAP);
     if (slab < 0) return fail(SDL_ERR_HIP, "readback failed");
@@ -3146,7 +3162,7 @@ int runTL(sdl_event* ev) {
     hipLaunchKernelGGL(k_tl_cap, dim3(gridFor(ns * kCapLanes, kBlock)), dim3(kBlock), 0, st, cSG, G->NM, G->dMods,
                        ev->sgOut.p,
                        ev->sgFlags.p, ev->mdSlot.p, G->dConnStart, G->dConnList, ev->slotSgStart.p, ev->slotMdSg.p,
-                       ev->tlCap.p, ev->tlNP.p, ns);
+                       ev->tlCap.p, ev->tlNP.p, ns, ev->tlCount.p, ev->tlFill.p);
     // pairs: one list, variant-major
     if (int r = scanExclusive<int>(ev->tlNP.p, ev->tlPOff.p, hostCount(ev, kNVariantsTL * ns), kNVariantsTL * ns,
                                    ev->scan, st, ev->tot.p, TOT_TLPAIRS))
@@ -3228,8 +3244,7 @@ int runTL(sdl_event* ev) {
         nc[v] = variantSize(ev, TOT_TLC0 + v, ev->tlPOff.p, ev->tlCOff.p, ns, v);
         if (nc[v] < 0) return fail(SDL_ERR_HIP, "readback failed");
     }
-    // second stage per candidate
-    zeroMany(st, {span(ev->tlCount.p, ns), span(ev->tlFill.p, ns)});
+    // second stage per candidate (tlCount and tlFill were cleared by k_tl_cap)
     const HitsDev H = hitsView(ev);
     const int b2 = ev->tune.v[TUNE_TL2_BLOCK];
     VarRanges R2{};
@@ -3256,10 +3271,9 @@ int runTL(sdl_event* ev) {
         hipLaunchKernelGGL(k_sort_segments<kSortLanesTL>, dim3(gridFor(nSG * kSortLanesTL, kBlock)), dim3(kBlock), 0, st,
                            cSG,
                            ev->tlStart.p, ev->tlOut.p);
-    hipLaunchKernelGGL(k_tl_bounds, dim3(1), dim3(64), 0, st, ev->tlPOff.p, ev->tlCOff.p, ns, ev->tot.p);
     timerEnd(ev, T_TL);
     hipLaunchKernelGGL(k_slot_range_sg, dim3(gridFor(NS + 1, kBlock)), dim3(kBlock), 0, st, hostCount(ev, NS),
-                       ev->slotSgStart.p, ev->tlStart.p, ev->slotTlStart.p);
+                       ev->slotSgStart.p, ev->tlStart.p, ev->slotTlStart.p, ev->tlPOff.p, ev->tlCOff.p, ns, ev->tot.p);
     return 0;
 }
 
