@@ -2575,6 +2575,9 @@ struct sdl_event {
     DBuf<int> bidx;
     DBuf<unsigned> bdet;
     DBuf<long long> dEvOff;
+    // the offsets dEvOff holds, and the allocation epoch when they were sent
+    std::vector<long long> evOffSent;
+    unsigned long long evOffSentEpoch = ~0ULL;
     DBuf<float> rt, r3, phi, halpha;
     DBuf<int> hslot;
     DBuf<float4> eHi, eLo;
@@ -2868,10 +2871,16 @@ void zeroMany(hipStream_t st, std::initializer_list<ZeroSpan> spans) {
 }
 
 // Uploads the batch's event offsets (outside any captured graph: the host array
-// is read when the copy runs).
+// is read when the copy runs).  Skipped when the device copy already holds
+// them: the same values were sent and no buffer has been allocated since (an
+// allocation anywhere advances the epoch, so a buffer freed and allocated
+// again at the same address is never taken for the old one).
 int uploadOffsets(sdl_event* ev) {
     if (int r = ev->dEvOff.ensure(ev->E + 1)) return r;
+    if (ev->evOffSentEpoch == g_allocEpoch && ev->evOffSent == ev->evOff) return 0;
     HIP_OK(hipMemcpyAsync(ev->dEvOff.p, ev->evOff.data(), sizeof(long long) * (ev->E + 1), hipMemcpyHostToDevice, ev->st));
+    ev->evOffSent = ev->evOff;
+    ev->evOffSentEpoch = g_allocEpoch;
     return 0;
 }
 
